@@ -1,0 +1,1823 @@
+// SPDX-License-Identifier: MIT
+//
+// CPU oracle for the ParallelMater rigid-body step.  TEST INFRASTRUCTURE:
+// only the tests, the smoke check and the benchmark's CPU-baseline leg use
+// it, as the checker.  It restates, serially and with O(n) storage, the
+// reference's Slang path as its CPU driver executes it (one lane per
+// dispatch, src/device/cpu_dispatch.hpp:30-60).  Every function cites the
+// reference lines it follows.  Compile with -ffp-contract=off: the
+// reference's CPU driver evaluates every float expression unfused.
+//
+// Deliberate storage changes (results unchanged):
+//  * manifolds are indexed by active pair, not by the n(n-1)/2 table;
+//  * the contact cache is keyed by pair code in a map, not an n^2 slot
+//    array (geometry.slang:2941-3070 semantics kept: hit iff the pair had a
+//    non-empty manifold in the immediately preceding step);
+//  * free_mean reduces each island's members in ascending index order
+//    instead of every root scanning all n bodies
+//    (geometry_constraints_solve.slang:617-675).
+//
+// Parity status per part (how each is pinned) is listed in the header
+// comment of each section below.
+
+#include "pm_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+static_assert(sizeof(PmoState) == 52, "RigidBodyState layout");
+static_assert(sizeof(PmoParams) == 64, "BodyParameters layout");
+static_assert(sizeof(PmoHingeFrame) == 72, "HingeContactFrame layout");
+static_assert(sizeof(PmoContact) == 284, "Contact layout");
+static_assert(sizeof(PmoManifold) == 2304, "ContactManifold layout");
+static_assert(sizeof(PmoAvbdBody) == 236, "AvbdBody layout");
+static_assert(sizeof(PmoSchedule) == 264, "ContactSchedule layout");
+
+namespace {
+
+using V3 = PmoVec3;
+using Q = PmoQuat;
+
+constexpr float kEpsilon = 1.0e-6f;            // types.slang:19
+constexpr uint32_t kInvalid = 0xffffffffu;     // types.slang:20
+constexpr uint32_t kStatic = 0u, kKinematic = 1u, kDynamic = 2u;
+
+// ---------------------------------------------------------------------
+// math.slang:6-248.  rsqrt is 1/sqrt as on the reference's CPU targets
+// (host reciprocal_sqrt, src/simulation/runtime.cpp:173-175).
+// ---------------------------------------------------------------------
+inline float rsq(float v) { return 1.0f / std::sqrt(v); }
+inline float fmin_(float a, float b) { return std::fmin(a, b); }
+inline float fmax_(float a, float b) { return std::fmax(a, b); }
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 mul(V3 v, float s) { return {v.x * s, v.y * s, v.z * s}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z,
+            a.x * b.y - a.y * b.x};
+}
+inline V3 cmin(V3 a, V3 b) { return {fmin_(a.x, b.x), fmin_(a.y, b.y), fmin_(a.z, b.z)}; }
+inline V3 cmax(V3 a, V3 b) { return {fmax_(a.x, b.x), fmax_(a.y, b.y), fmax_(a.z, b.z)}; }
+inline float len2(V3 v) { return dot(v, v); }
+inline float len(V3 v) { return std::sqrt(len2(v)); }
+inline float clampf(float v, float lo, float hi) { return fmin_(fmax_(v, lo), hi); }
+inline V3 normalized_or(V3 v, V3 fallback) {                    // math.slang:59-64
+    float s = len2(v);
+    return s <= kEpsilon * kEpsilon ? fallback : mul(v, rsq(s));
+}
+inline Q conj(Q q) { return {-q.x, -q.y, -q.z, q.w}; }
+inline Q qmul(Q a, Q b) {                                       // math.slang:71-78
+    return {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+            a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w,
+            a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+inline Q qnormalize(Q v) {                                       // math.slang:80-88
+    float s = v.x * v.x + v.y * v.y + v.z * v.z + v.w * v.w;
+    if (s <= kEpsilon * kEpsilon) return {0, 0, 0, 0};
+    float inv = rsq(s);
+    return {v.x * inv, v.y * inv, v.z * inv, v.w * inv};
+}
+inline Q relative_quaternion(Q from, Q to, bool local) {         // math.slang:90-105
+    float c = from.x * to.x + from.y * to.y + from.z * to.z + from.w * to.w;
+    if (c < 0.0f) to = {-to.x, -to.y, -to.z, -to.w};
+    Q d = {to.x - from.x, to.y - from.y, to.z - from.z, to.w - from.w};
+    Q r = local ? qmul(conj(from), d) : qmul(d, conj(from));
+    r.w = std::fabs(c);
+    return qnormalize(r);
+}
+inline V3 rotate(Q q, V3 v) {                                    // math.slang:107-113
+    V3 u = {q.x, q.y, q.z};
+    V3 t = mul(cross(u, v), 2.0f);
+    return add(v, add(mul(t, q.w), cross(u, t)));
+}
+inline V3 inv_rotate(Q q, V3 v) { return rotate(conj(q), v); }
+inline V3 rotation_displacement(Q from, Q to, V3 v) {            // math.slang:122-135
+    if (from.x * to.x + from.y * to.y + from.z * to.z + from.w * to.w < 0.0f)
+        to = {-to.x, -to.y, -to.z, -to.w};
+    V3 ov = {from.x, from.y, from.z}, nv = {to.x, to.y, to.z};
+    V3 d = sub(nv, ov);
+    V3 oc = cross(ov, v), dc = cross(d, v);
+    return mul(add(add(mul(oc, to.w - from.w), mul(dc, to.w)),
+                   add(cross(d, oc), cross(nv, dc))), 2.0f);
+}
+inline V3 transform_point(const PmoState &s, V3 p) { return add(s.position, rotate(s.orientation, p)); }
+inline V3 inverse_inertia_world(const PmoParams &p, const PmoState &s, V3 w) {  // math.slang:208-217
+    V3 l = inv_rotate(s.orientation, w);
+    V3 t = {l.x * p.inverse_inertia_local.x, l.y * p.inverse_inertia_local.y,
+            l.z * p.inverse_inertia_local.z};
+    return rotate(s.orientation, t);
+}
+inline V3 quaternion_rotation_vector(Q d) {                      // math.slang:221-233
+    float s = d.x * d.x + d.y * d.y + d.z * d.z;
+    float scale = 2.0f + s * (1.0f / 3.0f + s * (3.0f / 20.0f + s * (5.0f / 56.0f +
+                  s * (35.0f / 576.0f))));
+    if (s > 0.01f) {
+        float vs = std::sqrt(s);
+        scale = 2.0f * std::atan2(vs, clampf(d.w, 0.0f, 1.0f)) / vs;
+    }
+    return {d.x * scale, d.y * scale, d.z * scale};
+}
+inline V3 quaternion_delta_velocity(Q from, Q to, float dt) {    // math.slang:235-240
+    if (dt <= 0.0f) return {0, 0, 0};
+    return mul(quaternion_rotation_vector(relative_quaternion(from, to, false)), 1.0f / dt);
+}
+inline V3 clamp_length(V3 v, float m) {                          // math.slang:242-248
+    float s = len2(v);
+    if (s <= m * m || s <= kEpsilon * kEpsilon) return v;
+    return mul(v, m * rsq(s));
+}
+inline bool finite3(V3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+// ---------------------------------------------------------------------
+// Numerical core, constraint_math.slang:8-345.
+// Pinned by the reference's 85 rigid AVBD fixtures
+// (tests/solver/geometry_constraint_cases.hpp, block cases 0-28 and scalar
+// cases 0-55) through tests/test_oracle_fixtures.py.
+// ---------------------------------------------------------------------
+constexpr float kAlpha = 0.99f, kGamma = 0.999f;
+constexpr float kMinPenalty = 1.0f, kMaxPenalty = 1.0e10f;
+
+struct Dual { float lambda, penalty; };
+struct Force3 { float normal, t0, t1; };
+struct Vec6 { float v[6]; };
+struct Block { float h[21]; float g[6]; };
+struct Row {
+    Vec6 a, b; Dual dual; float error, velocity, lower, upper, stiffness,
+    damping, beta, reference_force;
+};
+
+inline float amin(float a, float b) { return a < b ? a : b; }   // :16
+inline float amax(float a, float b) { return a > b ? a : b; }   // :17
+inline float aabs(float v) { return v < 0.0f ? -v : v; }         // :18
+inline float aclamp(float v, float lo, float hi) { return amin(amax(v, lo), hi); }
+inline uint32_t bidx(uint32_t r, uint32_t c) {                   // :134-139
+    uint32_t hi = r > c ? r : c, lo = r > c ? c : r;
+    return hi * (hi + 1u) / 2u + lo;
+}
+inline Dual warm_start(Dual d, float stiffness) {                // :30-37
+    d.lambda *= kAlpha * kGamma;
+    d.penalty = amin(stiffness, aclamp(d.penalty * kGamma, kMinPenalty, kMaxPenalty));
+    return d;
+}
+inline float dual_force(Dual d, float e, float lo, float hi) { return aclamp(d.lambda + d.penalty * e, lo, hi); }
+inline Dual update_dual(Dual d, float e, float beta, float lo, float hi, float stiffness) {  // :44-56
+    float u = d.lambda + d.penalty * e;
+    d.lambda = stiffness < kMaxPenalty ? 0.0f : aclamp(u, lo, hi);
+    if (u > lo && u < hi)
+        d.penalty = amin(stiffness, amin(kMaxPenalty, d.penalty + beta * aabs(e)));
+    return d;
+}
+inline float friction_scale(float t0, float t1, float bound) {   // :65-73
+    if (bound <= 0.0f) return 0.0f;
+    float s = t0 * t0 + t1 * t1;
+    return s <= bound * bound ? 1.0f : amin(1.0f, bound * rsq(s));
+}
+inline Force3 project_contact(Force3 f, float mu) {              // :75-83
+    f.normal = amax(f.normal, 0.0f);
+    float bound = mu * f.normal;
+    float s = friction_scale(f.t0, f.t1, bound);
+    f.t0 *= s; f.t1 *= s;
+    return f;
+}
+inline Force3 stiffness_scales(Force3 trial, float mu) {         // :85-96
+    float n = amax(trial.normal, 0.0f);
+    float bound = mu * n;
+    float s = friction_scale(trial.t0, trial.t1, bound);
+    return {trial.normal > 0.0f ? 1.0f : 0.0f, s, s};
+}
+inline float dot6(const Vec6 &a, const Vec6 &b) {                // :116-123
+    float r = 0.0f;
+    for (int i = 0; i < 6; ++i) r += a.v[i] * b.v[i];
+    return r;
+}
+inline Block zero_block() { Block b; std::memset(&b, 0, sizeof b); return b; }
+inline Vec6 zero6() { Vec6 v; std::memset(&v, 0, sizeof v); return v; }
+inline void add_row(Block &b, const Vec6 &j, float k, float f) { // :151-163
+    for (uint32_t i = 0; i < 6; ++i) {
+        b.g[i] += j.v[i] * f;
+        for (uint32_t c = 0; c <= i; ++c) b.h[bidx(i, c)] += k * j.v[i] * j.v[c];
+    }
+}
+bool block_solve(const Block &input, Vec6 &delta) {              // :165-276
+    const float largest = 3.402823466e38f;
+    const float eps = 1.1920928955078125e-7f;
+    const float pivot_floor = 8.0f * eps;
+    const float roundoff = 64.0f * eps;
+    delta = zero6();
+    for (uint32_t i = 0; i < 6; ++i) {
+        if (!(input.h[bidx(i, i)] > 0.0f) || !(input.h[bidx(i, i)] < largest) ||
+            !(aabs(input.g[i]) < largest)) return false;
+        for (uint32_t j = 0; j < i; ++j)
+            if (!(aabs(input.h[bidx(i, j)]) < largest)) return false;
+    }
+    for (uint32_t attempt = 0; attempt < 2; ++attempt) {
+        Block b = input;
+        float scale[6], diag[6], fwd[6];
+        for (uint32_t i = 0; i < 6; ++i) {
+            scale[i] = attempt != 0 ? rsq(b.h[bidx(i, i)]) : 1.0f;
+            diag[i] = 0.0f; fwd[i] = 0.0f;
+        }
+        if (attempt != 0) {
+            for (uint32_t i = 0; i < 6; ++i) {
+                b.g[i] *= scale[i];
+                if (!(aabs(b.g[i]) < largest)) return false;
+                for (uint32_t j = 0; j < i; ++j) {
+                    b.h[bidx(i, j)] *= scale[i] * scale[j];
+                    if (!(aabs(b.h[bidx(i, j)]) <= 1.0f + roundoff)) return false;
+                }
+                b.h[bidx(i, i)] = 1.0f;
+            }
+        }
+        bool retry = false;
+        for (uint32_t i = 0; i < 6; ++i) {
+            for (uint32_t j = 0; j < i; ++j) {
+                float sum = b.h[bidx(i, j)];
+                for (uint32_t k = 0; k < j; ++k)
+                    sum -= b.h[bidx(i, k)] * diag[k] * b.h[bidx(j, k)];
+                b.h[bidx(i, j)] = sum / diag[j];
+            }
+            float d = b.h[bidx(i, i)];
+            for (uint32_t k = 0; k < i; ++k)
+                d -= b.h[bidx(i, k)] * b.h[bidx(i, k)] * diag[k];
+            if (attempt == 0) {
+                if (!(d > pivot_floor * input.h[bidx(i, i)]) || !(d < largest)) {
+                    retry = true;
+                    break;
+                }
+                diag[i] = d;
+            } else {
+                if (!(d >= -roundoff) || !(d < largest)) return false;
+                diag[i] = amax(d, pivot_floor);
+            }
+            fwd[i] = -b.g[i];
+            for (uint32_t k = 0; k < i; ++k) fwd[i] -= b.h[bidx(i, k)] * fwd[k];
+        }
+        if (retry) continue;
+        for (int i = 5; i >= 0; --i) {
+            delta.v[i] = fwd[i] / diag[i];
+            for (uint32_t k = uint32_t(i) + 1; k < 6; ++k)
+                delta.v[i] -= b.h[bidx(k, uint32_t(i))] * delta.v[k];
+        }
+        for (uint32_t i = 0; i < 6; ++i) {
+            delta.v[i] *= scale[i];
+            if (!(aabs(delta.v[i]) < largest)) return false;
+        }
+        return true;
+    }
+    return false;
+}
+inline float row_error(const Row &r, const Vec6 &a, const Vec6 &b) {   // :312-315
+    return r.error + dot6(r.a, a) + dot6(r.b, b);
+}
+inline float row_trial(const Row &r, const Vec6 &a, const Vec6 &b, float dt) {  // :317-322
+    float d = dot6(r.a, a) + dot6(r.b, b);
+    return r.dual.lambda + r.dual.penalty * (r.error + d) + r.damping * (r.velocity + d / dt);
+}
+inline float row_force(const Row &r, const Vec6 &a, const Vec6 &b, float dt) {
+    return aclamp(row_trial(r, a, b, dt), r.lower, r.upper);
+}
+inline void row_accumulate(Block &blk, const Row &r, bool first, const Vec6 &a,
+                           const Vec6 &b, float dt) {                     // :329-338
+    float trial = row_trial(r, a, b, dt);
+    bool two_sided = r.lower < 0.0f && r.upper > 0.0f;
+    float k = two_sided || (trial > r.lower && trial < r.upper)
+        ? r.dual.penalty + r.damping / dt : 0.0f;
+    add_row(blk, first ? r.a : r.b, k, aclamp(trial, r.lower, r.upper) - r.reference_force);
+}
+inline void row_advance(Row &r, const Vec6 &a, const Vec6 &b) {          // :340-345
+    r.dual = update_dual(r.dual, row_error(r, a, b), r.beta, r.lower, r.upper, r.stiffness);
+}
+Row row_from(const float *f) {
+    Row r;
+    for (int i = 0; i < 6; ++i) { r.a.v[i] = f[i]; r.b.v[i] = f[6 + i]; }
+    r.dual = {f[12], f[13]};
+    r.error = f[14]; r.velocity = f[15]; r.lower = f[16]; r.upper = f[17];
+    r.stiffness = f[18]; r.damping = f[19]; r.beta = f[20]; r.reference_force = f[21];
+    return r;
+}
+void row_to(const Row &r, float *f) {
+    for (int i = 0; i < 6; ++i) { f[i] = r.a.v[i]; f[6 + i] = r.b.v[i]; }
+    f[12] = r.dual.lambda; f[13] = r.dual.penalty; f[14] = r.error; f[15] = r.velocity;
+    f[16] = r.lower; f[17] = r.upper; f[18] = r.stiffness; f[19] = r.damping;
+    f[20] = r.beta; f[21] = r.reference_force;
+}
+
+// ---------------------------------------------------------------------
+// Shapes.  Mesh upload preparation restates runtime_deformables.cu.inc:
+// 104-281 (median-split BVH with <=4 triangles per leaf and the index
+// buffer rewritten in BVH order), closed_convex_planes
+// (runtime_domains.cu.inc:1061-1108) and unique_convex_planes
+// (src/convex_plane_dedup.hpp).  Pinned by closed forms in the tests
+// (unit cube planes and offsets, box bounding sphere, inertia formula).
+// ---------------------------------------------------------------------
+struct Plane { V3 normal; float offset; };
+enum ShapeKind : uint32_t { kMesh = 0, kSphere = 1, kCapsule = 2 };
+
+struct Shape {
+    uint32_t kind = kMesh;
+    std::vector<V3> vertices;
+    std::vector<uint32_t> indices;
+    std::vector<Plane> planes;      // per triangle, then the unique tail
+    uint32_t unique_planes = 0;
+    bool has_planes = false;
+    V3 minimum{}, maximum{}, center{};
+    float radius = 0.0f;            // bounding radius
+    V3 unit_inertia{};
+    float sphere_radius = 0.0f, half_length = 0.0f;
+    uint32_t index_count() const { return uint32_t(indices.size()); }
+    uint32_t vertex_count() const { return uint32_t(vertices.size()); }
+};
+
+std::vector<Plane> closed_convex_planes(const std::vector<V3> &v,
+                                        const std::vector<uint32_t> &idx) {
+    std::map<std::array<float, 3>, uint32_t> welded;
+    std::vector<uint32_t> remap(v.size());
+    V3 center{0, 0, 0};
+    for (uint32_t i = 0; i < v.size(); ++i) {
+        auto e = welded.emplace(std::array<float, 3>{v[i].x, v[i].y, v[i].z},
+                                uint32_t(welded.size()));
+        remap[i] = e.first->second;
+        if (e.second) center = add(center, v[i]);
+    }
+    if (welded.size() < 4u) return {};
+    center = mul(center, 1.0f / float(welded.size()));
+    std::unordered_map<uint64_t, uint32_t> edges;
+    for (size_t i = 0; i < idx.size(); i += 3)
+        for (size_t e = 0; e < 3; ++e) {
+            uint32_t a = remap[idx[i + e]], b = remap[idx[i + (e + 1) % 3]];
+            ++edges[(uint64_t(std::min(a, b)) << 32) | std::max(a, b)];
+        }
+    for (auto &e : edges) if (e.second != 2u) return {};
+    float scale = 0.0f;
+    for (auto &p : v) scale = std::max(scale, len(sub(p, center)));
+    float tol = std::max(1.0e-7f, scale * 1.0e-5f);
+    std::vector<Plane> planes;
+    for (size_t i = 0; i < idx.size(); i += 3) {
+        V3 a = v[idx[i]];
+        V3 n = normalized_or(cross(sub(v[idx[i + 1]], a), sub(v[idx[i + 2]], a)), {0, 0, 0});
+        float side = dot(n, sub(a, center));
+        if (std::fabs(side) <= tol) return {};
+        if (side < 0.0f) n = mul(n, -1.0f);
+        float off = dot(n, a);
+        for (auto &p : v) if (dot(n, p) - off > tol) return {};
+        planes.push_back({n, off});
+    }
+    return planes;
+}
+
+bool prepare_mesh(Shape &s, const float *verts, uint32_t nv, const uint32_t *idx, uint32_t ni) {
+    if (nv < 3 || ni < 3 || ni % 3 != 0) return false;
+    s.kind = kMesh;
+    s.vertices.resize(nv);
+    for (uint32_t i = 0; i < nv; ++i) s.vertices[i] = {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    std::vector<uint32_t> in(idx, idx + ni);
+    for (auto &p : s.vertices) if (!finite3(p)) return false;
+    V3 mn = s.vertices[0], mx = s.vertices[0];
+    for (uint32_t i = 1; i < nv; ++i) { mn = cmin(mn, s.vertices[i]); mx = cmax(mx, s.vertices[i]); }
+    for (uint32_t i = 0; i < ni; i += 3) {
+        if (in[i] >= nv || in[i + 1] >= nv || in[i + 2] >= nv) return false;
+        V3 area = cross(sub(s.vertices[in[i + 1]], s.vertices[in[i]]),
+                        sub(s.vertices[in[i + 2]], s.vertices[in[i]]));
+        if (len2(area) <= kEpsilon * kEpsilon) return false;
+    }
+    // Median-split BVH order (runtime_deformables.cu.inc:140-200).
+    const uint32_t tri = ni / 3;
+    std::vector<uint32_t> order(tri);
+    for (uint32_t i = 0; i < tri; ++i) order[i] = i;
+    auto coord = [](V3 v, int axis) { return axis == 0 ? v.x : (axis == 1 ? v.y : v.z); };
+    auto centroid = [&](uint32_t t) {
+        uint32_t o = t * 3;
+        return mul(add(add(s.vertices[in[o]], s.vertices[in[o + 1]]), s.vertices[in[o + 2]]), 1.0f / 3.0f);
+    };
+    struct Builder {
+        std::vector<uint32_t> &order;
+        decltype(coord) &coordf;
+        decltype(centroid) &centroidf;
+        const std::vector<V3> &verts;
+        const std::vector<uint32_t> &in;
+        void build(uint32_t begin, uint32_t end) {
+            V3 mn = {FLT_MAX, FLT_MAX, FLT_MAX}, mx = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            for (uint32_t it = begin; it < end; ++it) {
+                uint32_t o = order[it] * 3;
+                for (uint32_t c = 0; c < 3; ++c) { mn = cmin(mn, verts[in[o + c]]); mx = cmax(mx, verts[in[o + c]]); }
+            }
+            if (end - begin <= 4u) return;
+            V3 ext = sub(mx, mn);
+            int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
+            std::stable_sort(order.begin() + begin, order.begin() + end, [&](uint32_t a, uint32_t b) {
+                float fa = coordf(centroidf(a), axis), fb = coordf(centroidf(b), axis);
+                return fa < fb || (fa == fb && a < b);
+            });
+            uint32_t mid = begin + (end - begin) / 2u;
+            build(begin, mid);
+            build(mid, end);
+        }
+    } builder{order, coord, centroid, s.vertices, in};
+    builder.build(0, tri);
+    s.indices.resize(ni);
+    for (uint32_t t = 0; t < tri; ++t)
+        for (uint32_t c = 0; c < 3; ++c) s.indices[t * 3 + c] = in[order[t] * 3 + c];
+    auto planes = closed_convex_planes(s.vertices, s.indices);
+    s.has_planes = !planes.empty();
+    if (s.has_planes) {
+        std::map<std::array<float, 4>, bool> seen;
+        std::vector<Plane> unique;
+        for (auto &p : planes)
+            if (seen.emplace(std::array<float, 4>{p.normal.x, p.normal.y, p.normal.z, p.offset}, true).second)
+                unique.push_back(p);
+        s.unique_planes = uint32_t(unique.size());
+        s.planes = planes;
+        s.planes.insert(s.planes.end(), unique.begin(), unique.end());
+    }
+    s.minimum = mn; s.maximum = mx;
+    s.center = mul(add(mn, mx), 0.5f);
+    float r2 = 0.0f;
+    for (auto &p : s.vertices) r2 = std::fmax(r2, len2(sub(p, s.center)));
+    s.radius = std::sqrt(r2);
+    V3 h = mul(sub(mx, mn), 0.5f);
+    s.unit_inertia = {std::fmax((h.y * h.y + h.z * h.z) / 3.0f, kEpsilon),
+                      std::fmax((h.x * h.x + h.z * h.z) / 3.0f, kEpsilon),
+                      std::fmax((h.x * h.x + h.y * h.y) / 3.0f, kEpsilon)};
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// Contact bookkeeping shared by every narrowphase.
+// ---------------------------------------------------------------------
+void clear_manifold(PmoManifold &m) { std::memset(&m, 0, sizeof m); }   // geometry.slang:192-202
+
+// rigid_add_contact, geometry.slang:1129-1220, for unguided contacts (no
+// hinge joints exist on this path) and no swept-witness canonicalisation
+// (the convex face path passes false).
+void add_contact(PmoManifold &m, const PmoContact &c, float spacing) {
+    float sp2 = spacing * spacing;
+    for (uint32_t i = 0; i < m.count; ++i) {
+        if (len2(sub(m.contacts[i].point, c.point)) < sp2) {
+            float dd = c.penetration - m.contacts[i].penetration;
+            if (dd > 0.0f) m.contacts[i] = c;
+            return;
+        }
+    }
+    if (m.count < 8u) { m.contacts[m.count++] = c; return; }
+    uint32_t sh = 0;
+    for (uint32_t i = 1; i < m.count; ++i)
+        if (m.contacts[i].penetration < m.contacts[sh].penetration) sh = i;
+    if (c.penetration > m.contacts[sh].penetration) m.contacts[sh] = c;
+}
+
+// rigid_reduce_face_patch, geometry.slang:1228-1270.
+void reduce_face_patch(PmoManifold &m, float tol) {
+    float t2 = tol * tol;
+    bool removed = true;
+    while (removed) {
+        removed = false;
+        for (uint32_t cand = 0; cand < m.count && !removed; ++cand) {
+            V3 p = m.contacts[cand].point;
+            for (uint32_t f = 0; f < m.count && !removed; ++f) {
+                if (f == cand) continue;
+                for (uint32_t s = f + 1; s < m.count; ++s) {
+                    if (s == cand) continue;
+                    V3 e = sub(m.contacts[s].point, m.contacts[f].point);
+                    float e2 = len2(e);
+                    if (e2 <= t2) continue;
+                    float t = dot(sub(p, m.contacts[f].point), e) / e2;
+                    if (t <= 1.0e-4f || t >= 0.9999f) continue;
+                    V3 cl = add(m.contacts[f].point, mul(e, t));
+                    if (len2(sub(p, cl)) > t2) continue;
+                    for (uint32_t k = cand + 1; k < m.count; ++k) m.contacts[k - 1] = m.contacts[k];
+                    --m.count;
+                    removed = true;
+                    break;
+                }
+            }
+        }
+    }
+}
+
+inline float rotational_motion_bound(const PmoState &prev, const PmoState &cur, const Shape &s) {  // math.slang:192-206
+    Q d = relative_quaternion(prev.orientation, cur.orientation, false);
+    float sh = len({d.x, d.y, d.z});
+    V3 ma = {fmax_(std::fabs(s.minimum.x), std::fabs(s.maximum.x)),
+             fmax_(std::fabs(s.minimum.y), std::fabs(s.maximum.y)),
+             fmax_(std::fabs(s.minimum.z), std::fabs(s.maximum.z))};
+    return 2.0f * len(ma) * sh;
+}
+
+bool convex_face_shape(const Shape &s) { return s.kind == kMesh && s.has_planes && s.index_count() <= 96u; }
+
+// rigid_convex_face_manifold, geometry.slang:1606-1745 (hinge frames are
+// absent on this path: both are zero).
+bool convex_face_manifold(const PmoState &bs, const Shape &bm, const PmoState &cs,
+                          const Shape &cm, float margin, float reach_corr, PmoManifold &out) {
+    clear_manifold(out);
+    float rest = fmin_(margin, 1.0e-3f);
+    float reach = margin + reach_corr;
+    if (!convex_face_shape(bm) || !convex_face_shape(cm)) return false;
+    float best = -3.402823466e+38f;
+    uint32_t ref_face = 0;
+    bool ref_is_body = false;
+    for (uint32_t side = 0; side < 2; ++side) {
+        const Shape &rm = side == 0 ? cm : bm;
+        const PmoState &rs = side == 0 ? cs : bs;
+        const Shape &im = side == 0 ? bm : cm;
+        const PmoState &is = side == 0 ? bs : cs;
+        for (uint32_t f = 0; f < rm.index_count() / 3; ++f) {
+            Plane pl = rm.planes[f];
+            V3 n = rotate(rs.orientation, pl.normal);
+            V3 inn = inv_rotate(is.orientation, n);
+            float support = 3.402823466e+38f;
+            for (uint32_t v = 0; v < im.vertex_count(); ++v) support = fmin_(support, dot(inn, im.vertices[v]));
+            float sep = support + dot(n, sub(is.position, rs.position)) - pl.offset;
+            if (sep > best) { best = sep; ref_face = f; ref_is_body = side != 0; }
+        }
+    }
+    if (best > reach) return true;
+    const Shape &rm = ref_is_body ? bm : cm;
+    const PmoState &rs = ref_is_body ? bs : cs;
+    const Shape &im = ref_is_body ? cm : bm;
+    const PmoState &is = ref_is_body ? cs : bs;
+    Plane reference = rm.planes[ref_face];
+    V3 outward = rotate(rs.orientation, reference.normal);
+    V3 iaxis = inv_rotate(is.orientation, outward);
+    float alignment = 1.0f;
+    uint32_t inc_face = 0;
+    for (uint32_t f = 0; f < im.index_count() / 3; ++f) {
+        float val = dot(iaxis, im.planes[f].normal);
+        if (val < alignment) { alignment = val; inc_face = f; }
+    }
+    V3 normal = mul(outward, ref_is_body ? -1.0f : 1.0f);
+    Plane incident = im.planes[inc_face];
+    for (uint32_t t = 0; t < im.index_count() / 3; ++t) {
+        Plane face = im.planes[t];
+        if (dot(face.normal, incident.normal) < 0.99999f ||
+            std::fabs(face.offset - incident.offset) > 1.0e-5f) continue;
+        V3 poly[36], clipped[36];
+        uint32_t count = 3;
+        for (uint32_t c = 0; c < 3; ++c)
+            poly[c] = inv_rotate(rs.orientation, sub(transform_point(is, im.vertices[im.indices[t * 3 + c]]), rs.position));
+        for (uint32_t pi = 0; pi < rm.index_count() / 3 && count != 0; ++pi) {
+            Plane pl = rm.planes[pi];
+            float off = pl.offset + (dot(pl.normal, reference.normal) > 0.99999f ? reach : 0.0f);
+            uint32_t cc = 0;
+            V3 prev = poly[count - 1];
+            float pd = dot(pl.normal, prev) - off;
+            for (uint32_t v = 0; v < count; ++v) {
+                V3 cur = poly[v];
+                float d = dot(pl.normal, cur) - off;
+                if ((d <= 0.0f) != (pd <= 0.0f))
+                    clipped[cc++] = add(prev, mul(sub(cur, prev), pd / (pd - d)));
+                if (d <= 0.0f) clipped[cc++] = cur;
+                prev = cur; pd = d;
+            }
+            count = cc;
+            for (uint32_t v = 0; v < count; ++v) poly[v] = clipped[v];
+        }
+        for (uint32_t v = 0; v < count; ++v) {
+            float d = dot(reference.normal, poly[v]) - reference.offset;
+            if (d > reach) continue;
+            PmoContact c;
+            std::memset(&c, 0, sizeof c);
+            c.normal = normal;
+            c.point = transform_point(rs, sub(poly[v], mul(reference.normal, d * 0.5f)));
+            c.penetration = rest - d + 1.0e-5f;
+            c.impact_fraction = 1.0f;
+            add_contact(out, c, fmax_(margin * 2.0f, 1.0e-4f));
+        }
+    }
+    if (out.count == 0u) return false;
+    reduce_face_patch(out, fmax_(1.0e-6f, margin * 1.0e-3f));
+    out.face_patch = 1u;
+    return true;
+}
+
+// rigid_swept_convex_face_manifold, geometry.slang:1751-1853.
+bool swept_convex_face_manifold(const PmoState &pb, const PmoState &b, const Shape &bm,
+                                const PmoState &pc, const PmoState &c, const Shape &cm,
+                                float margin, PmoManifold &out) {
+    clear_manifold(out);
+    Q bd = relative_quaternion(pb.orientation, b.orientation, false);
+    Q cd = relative_quaternion(pc.orientation, c.orientation, false);
+    if (bd.x != 0.0f || bd.y != 0.0f || bd.z != 0.0f || cd.x != 0.0f || cd.y != 0.0f || cd.z != 0.0f)
+        return false;
+    V3 bmo = sub(b.position, pb.position), cmo = sub(c.position, pc.position);
+    float rest = fmin_(margin, 1.0e-3f);
+    float entry = 0.0f, exit_time = 1.0f;
+    V3 entry_normal{0, 0, 0};
+    for (uint32_t side = 0; side < 2; ++side) {
+        const Shape &rm = side == 0 ? cm : bm;
+        const Shape &im = side == 0 ? bm : cm;
+        const PmoState &rs = side == 0 ? pc : pb;
+        const PmoState &is = side == 0 ? pb : pc;
+        V3 rel = side == 0 ? sub(bmo, cmo) : sub(cmo, bmo);
+        for (uint32_t f = 0; f < rm.index_count() / 3; ++f) {
+            Plane pl = rm.planes[f];
+            V3 n = rotate(rs.orientation, pl.normal);
+            V3 inn = inv_rotate(is.orientation, n);
+            float support = 3.402823466e+38f;
+            for (uint32_t v = 0; v < im.vertex_count(); ++v) support = fmin_(support, dot(inn, im.vertices[v]));
+            float gap = support + dot(n, sub(is.position, rs.position)) - pl.offset;
+            float speed = dot(n, rel);
+            if (speed == 0.0f) {
+                if (gap > rest) return true;
+            } else {
+                float crossing = (rest - gap) / speed;
+                if (speed < 0.0f && crossing > entry) {
+                    entry = crossing;
+                    entry_normal = mul(n, side == 0 ? 1.0f : -1.0f);
+                }
+                if (speed > 0.0f) exit_time = fmin_(exit_time, crossing);
+            }
+        }
+    }
+    if (entry > exit_time || entry > 1.0f || exit_time < 0.0f) return true;
+    if (entry == 0.0f) {
+        bool clipped = convex_face_manifold(b, bm, c, cm, margin, 0.0f, out);
+        if (clipped && out.count != 0u) return true;
+        if (!convex_face_manifold(pb, bm, pc, cm, margin, 0.0f, out) || out.count == 0u) return false;
+        for (uint32_t p = 0; p < out.count; ++p) {
+            float closing = -dot(sub(bmo, cmo), out.contacts[p].normal);
+            if (closing <= 0.0f) return false;
+            out.contacts[p].penetration += closing;
+            out.contacts[p].impact_fraction = 0.0f;
+        }
+        return true;
+    }
+    PmoState ib = pb, ic = pc;
+    ib.position = add(pb.position, mul(bmo, entry));
+    ic.position = add(pc.position, mul(cmo, entry));
+    if (!convex_face_manifold(ib, bm, ic, cm, rest + 1.0e-5f, 0.0f, out) || out.count == 0u) return false;
+    for (uint32_t p = 0; p < out.count; ++p) {
+        PmoContact &ct = out.contacts[p];
+        if (dot(ct.normal, entry_normal) < 0.99999f) return false;
+        float remaining = -dot(sub(bmo, cmo), ct.normal) * (1.0f - entry);
+        ct.penetration = fmax_(0.0f, remaining + ct.penetration) + rest + 1.0e-5f;
+        ct.impact_fraction = entry;
+    }
+    return true;
+}
+
+// rigid_requires_swept_pair_contact, geometry.slang:1586-1604.
+bool requires_swept(const PmoState &pa, const PmoState &a, const Shape &ma,
+                    const PmoState &pb, const PmoState &b, const Shape &mb, float threshold) {
+    bool planes_a = ma.kind != kMesh || ma.has_planes;
+    bool planes_b = mb.kind != kMesh || mb.has_planes;
+    if (!planes_a || !planes_b) threshold = fmin_(threshold, 1.0e-3f);
+    V3 rel = sub(sub(a.position, pa.position), sub(b.position, pb.position));
+    return len(rel) + rotational_motion_bound(pa, a, ma) + rotational_motion_bound(pb, b, mb) > threshold;
+}
+
+// rigid_solver_correction_reach, geometry.slang:513-522.
+float correction_reach(const PmoParams &p, const Shape &s) {
+    if (p.motion != kDynamic) return 0.0f;
+    V3 e = sub(s.maximum, s.minimum);
+    return fmin_(0.5f, 0.25f * fmax_(0.0f, fmin_(e.x, fmin_(e.y, e.z))));
+}
+
+// ---------------------------------------------------------------------
+// Analytic spheres and capsules: no reference counterpart.  Contacts
+// follow the face path's conventions: the normal points from the collider
+// towards the body, the point is the midpoint of the two surface points,
+// penetration = rest offset - separation + 1e-5.  Pinned by closed forms
+// (sphere on plane / sphere on sphere / box corner) in the tests.
+// ---------------------------------------------------------------------
+bool box_half_extents(const Shape &s, V3 &half) {
+    if (s.kind != kMesh || !s.has_planes || s.vertex_count() != 8u) return false;
+    V3 c = s.center;
+    if (c.x != 0.0f || c.y != 0.0f || c.z != 0.0f) return false;
+    half = mul(sub(s.maximum, s.minimum), 0.5f);
+    for (auto &p : s.vertices)
+        if (std::fabs(p.x) != half.x || std::fabs(p.y) != half.y || std::fabs(p.z) != half.z) return false;
+    return true;
+}
+
+// Closest point of segment [p0,p1] to point q.
+inline V3 closest_on_segment(V3 p0, V3 p1, V3 q) {
+    V3 d = sub(p1, p0);
+    float dd = len2(d);
+    float t = dd > 0.0f ? clampf(dot(sub(q, p0), d) / dd, 0.0f, 1.0f) : 0.0f;
+    return add(p0, mul(d, t));
+}
+
+// A sphere of radius r centred at world point c against a shape; returns
+// the normal (from the shape towards c), the surface point on the shape
+// and the signed distance from the shape surface to c.  Boxes, spheres
+// and capsules are supported.
+bool point_shape_distance(V3 c, const PmoState &ss, const Shape &sh, V3 &normal,
+                          V3 &surface, float &distance) {
+    if (sh.kind == kSphere) {
+        V3 d = sub(c, ss.position);
+        normal = normalized_or(d, {0.0f, 1.0f, 0.0f});
+        distance = len(d) - sh.sphere_radius;
+        surface = add(ss.position, mul(normal, sh.sphere_radius));
+        return true;
+    }
+    if (sh.kind == kCapsule) {
+        V3 axis = rotate(ss.orientation, {0.0f, sh.half_length, 0.0f});
+        V3 p = closest_on_segment(sub(ss.position, axis), add(ss.position, axis), c);
+        V3 d = sub(c, p);
+        normal = normalized_or(d, rotate(ss.orientation, {1.0f, 0.0f, 0.0f}));
+        distance = len(d) - sh.sphere_radius;
+        surface = add(p, mul(normal, sh.sphere_radius));
+        return true;
+    }
+    V3 half;
+    if (!box_half_extents(sh, half)) return false;
+    V3 l = inv_rotate(ss.orientation, sub(c, ss.position));
+    V3 q = {clampf(l.x, -half.x, half.x), clampf(l.y, -half.y, half.y), clampf(l.z, -half.z, half.z)};
+    bool inside = q.x == l.x && q.y == l.y && q.z == l.z;
+    V3 ln;
+    float dist;
+    if (!inside) {
+        V3 d = sub(l, q);
+        float dl = len(d);
+        ln = mul(d, 1.0f / dl);
+        dist = dl;
+    } else {
+        // Deepest-face exit: the axis with the least remaining depth.
+        float dx = half.x - std::fabs(l.x), dy = half.y - std::fabs(l.y), dz = half.z - std::fabs(l.z);
+        if (dx <= dy && dx <= dz) { ln = {l.x < 0.0f ? -1.0f : 1.0f, 0, 0}; q.x = l.x < 0.0f ? -half.x : half.x; dist = -dx; }
+        else if (dy <= dz) { ln = {0, l.y < 0.0f ? -1.0f : 1.0f, 0}; q.y = l.y < 0.0f ? -half.y : half.y; dist = -dy; }
+        else { ln = {0, 0, l.z < 0.0f ? -1.0f : 1.0f}; q.z = l.z < 0.0f ? -half.z : half.z; dist = -dz; }
+    }
+    normal = rotate(ss.orientation, ln);
+    surface = transform_point(ss, q);
+    distance = dist;
+    return true;
+}
+
+void push_sphere_contact(PmoManifold &m, V3 normal, V3 surface_collider, float gap,
+                         float margin, float reach) {
+    if (gap > reach) return;
+    float rest = fmin_(margin, 1.0e-3f);
+    PmoContact c;
+    std::memset(&c, 0, sizeof c);
+    c.normal = normal;
+    c.point = add(surface_collider, mul(normal, gap * 0.5f));
+    c.penetration = rest - gap + 1.0e-5f;
+    c.impact_fraction = 1.0f;
+    add_contact(m, c, fmax_(margin * 2.0f, 1.0e-4f));
+}
+
+// Round shapes (sphere, capsule) against anything supported.  Returns
+// false when the pair has no analytic path.
+bool analytic_manifold(const PmoState &as, const Shape &am, const PmoState &bs, const Shape &bm,
+                       float margin, float reach, PmoManifold &out) {
+    clear_manifold(out);
+    bool a_round = am.kind == kSphere || am.kind == kCapsule;
+    bool b_round = bm.kind == kSphere || bm.kind == kCapsule;
+    if (!a_round && !b_round) return false;
+    // Sample points on the round shape's core: the centre for a sphere,
+    // the segment points closest to the other shape for a capsule.
+    V3 n, s;
+    float d;
+    if (am.kind == kSphere) {
+        if (!point_shape_distance(as.position, bs, bm, n, s, d)) return false;
+        push_sphere_contact(out, n, s, d - am.sphere_radius, margin, reach);
+        return true;
+    }
+    if (bm.kind == kSphere) {
+        if (!point_shape_distance(bs.position, as, am, n, s, d)) return false;
+        // Normal must point from the collider (b) towards the body (a).
+        V3 nb = mul(n, -1.0f);
+        V3 sb = sub(bs.position, mul(n, bm.sphere_radius));
+        (void)s;
+        push_sphere_contact(out, nb, sb, d - bm.sphere_radius, margin, reach);
+        return true;
+    }
+    // Capsule body against a box or capsule collider: both segment end
+    // points plus, for capsule-capsule, the mutual closest point.
+    const Shape &cap = am.kind == kCapsule ? am : bm;
+    const PmoState &cs = am.kind == kCapsule ? as : bs;
+    const Shape &other = am.kind == kCapsule ? bm : am;
+    const PmoState &os = am.kind == kCapsule ? bs : as;
+    float sign = am.kind == kCapsule ? 1.0f : -1.0f;
+    V3 axis = rotate(cs.orientation, {0.0f, cap.half_length, 0.0f});
+    V3 ends[3] = {sub(cs.position, axis), add(cs.position, axis), cs.position};
+    uint32_t samples = 2;
+    if (other.kind == kCapsule) {
+        V3 oaxis = rotate(os.orientation, {0.0f, other.half_length, 0.0f});
+        ends[2] = closest_on_segment(ends[0], ends[1], closest_on_segment(sub(os.position, oaxis), add(os.position, oaxis), cs.position));
+        samples = 3;
+    }
+    for (uint32_t k = 0; k < samples; ++k) {
+        if (!point_shape_distance(ends[k], os, other, n, s, d)) return false;
+        float gap = d - cap.sphere_radius;
+        if (sign > 0.0f) push_sphere_contact(out, n, s, gap, margin, reach);
+        else push_sphere_contact(out, mul(n, -1.0f), sub(ends[k], mul(n, cap.sphere_radius)), gap, margin, reach);
+    }
+    return true;
+}
+
+// rigid_contact_normal_speed, geometry.slang:1272-1280.
+float contact_normal_speed(const PmoState &b, const PmoState &c, V3 p, V3 n) {
+    V3 bv = add(b.linear_velocity, cross(b.angular_velocity, sub(p, b.position)));
+    V3 cv = add(c.linear_velocity, cross(c.angular_velocity, sub(p, c.position)));
+    return dot(sub(bv, cv), n);
+}
+
+// ---------------------------------------------------------------------
+// AVBD prepare and solve.  geometry_constraints.slang:374-445,
+// 1213-1591 and geometry_constraints_solve.slang:211-865.  Pinned by the
+// kernel-level physical gates of tests/solver/rigid_realtime_cuda_tests.cu
+// re-run on this oracle (tests/test_oracle_physics.py) and by the
+// invariants listed there; contact preparation, the crossing safeguard and
+// free_mean have no reference-held vectors (parity unpinned beyond those).
+// ---------------------------------------------------------------------
+constexpr float kSurfaceTolerance = 1.0e-5f;   // geometry_constraints.slang:28
+constexpr float kRestOffset = 1.0e-3f;         // :29
+constexpr uint32_t kRealtimeProfile = 1u;      // :32
+constexpr uint32_t kRealtimeSweeps = 8u;       // :33
+constexpr float kRealtimeDamping = 0.05f;      // :34
+
+inline Vec6 jac(V3 l, V3 a) { return {{l.x, l.y, l.z, a.x, a.y, a.z}}; }
+inline V3 trans(const float *d) { return {d[0], d[1], d[2]}; }
+inline V3 rot6(const float *d) { return {d[3], d[4], d[5]}; }
+inline Vec6 disp(const PmoAvbdBody &b) { Vec6 v; std::memcpy(v.v, b.displacement, sizeof v.v); return v; }
+
+inline void basis(V3 n, V3 &t0, V3 &t1) {                        // :424-430
+    t0 = normalized_or(cross(n, std::fabs(n.x) < 0.57735f ? V3{1, 0, 0} : V3{0, 1, 0}), {0, 0, 1});
+    t1 = cross(n, t0);
+}
+inline float comp(V3 v, uint32_t a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+inline float pair_stiffness(const PmoParams &a, const PmoParams &b, float dt) {  // :432-436
+    return 1.0f / (fmax_(a.inverse_mass + b.inverse_mass, 1.0e-6f) * dt * dt);
+}
+inline float contact_skin(const PmoParams &a, const PmoParams &b) {  // :438-445 (unguided)
+    return fmin_(a.collision_margin + b.collision_margin, kRestOffset) + 2.0f * kSurfaceTolerance;
+}
+
+PmoState avbd_pose(const PmoAvbdBody &b) {                       // :374-384
+    PmoState s = b.target;
+    s.position = add(s.position, trans(b.displacement));
+    Q d = b.orientation_delta;
+    if (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f)
+        s.orientation = qnormalize(qmul(d, s.orientation));
+    return s;
+}
+
+void apply_pose_update(PmoAvbdBody &b, const Vec6 &u) {          // :386-408
+    for (int i = 0; i < 3; ++i) b.displacement[i] += u.v[i];
+    V3 r = {u.v[3], u.v[4], u.v[5]};
+    if (r.x == 0.0f && r.y == 0.0f && r.z == 0.0f) return;
+    Q rel = b.orientation_delta;
+    Q der = qmul({r.x, r.y, r.z, 0.0f}, rel);
+    rel = qnormalize({rel.x + 0.5f * der.x, rel.y + 0.5f * der.y, rel.z + 0.5f * der.z, rel.w + 0.5f * der.w});
+    if (rel.w < 0.0f) rel = {-rel.x, -rel.y, -rel.z, -rel.w};
+    b.orientation_delta = rel;
+    b.displacement[3] = 2.0f * rel.x;
+    b.displacement[4] = 2.0f * rel.y;
+    b.displacement[5] = 2.0f * rel.z;
+}
+
+// rigid_avbd_solve_contact_rows, geometry_constraints.slang:506-529:
+// normal arm from the world contact point, tangent arms from the anchors.
+void solve_contact_rows(const PmoContact &c, const PmoAvbdBody &A, const PmoAvbdBody &B,
+                        Vec6 ja[3], Vec6 jb[3], float err[3]) {
+    V3 t0, t1;
+    basis(c.normal, t0, t1);
+    V3 axes[3] = {c.normal, t0, t1};
+    V3 armA = sub(c.point, A.target.position), armB = sub(c.point, B.target.position);
+    Vec6 da = disp(A), db = disp(B);
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+        V3 ra = ax == 0 ? armA : rotate(A.target.orientation, c.avbd_anchor_a);
+        V3 rb = ax == 0 ? armB : rotate(B.target.orientation, c.avbd_anchor_b);
+        ja[ax] = jac(mul(axes[ax], -1.0f), mul(cross(ra, axes[ax]), -1.0f));
+        jb[ax] = jac(axes[ax], cross(rb, axes[ax]));
+        err[ax] = comp(c.avbd_error, ax) + dot6(ja[ax], da) + dot6(jb[ax], db);
+    }
+}
+float solve_contact_normal_row(const PmoContact &c, const PmoAvbdBody &A, const PmoAvbdBody &B,
+                               Vec6 &ja, Vec6 &jb) {              // :531-544
+    V3 armA = sub(c.point, A.target.position), armB = sub(c.point, B.target.position);
+    ja = jac(mul(c.normal, -1.0f), mul(cross(armA, c.normal), -1.0f));
+    jb = jac(c.normal, cross(armB, c.normal));
+    Vec6 da = disp(A), db = disp(B);
+    return c.avbd_error.x + dot6(ja, da) + dot6(jb, db);
+}
+
+struct Ctx {
+    const PmoParams *P;
+    const PmoState *prev;
+    PmoState *states;
+    uint32_t n;
+    PmoManifold *M;
+    const uint32_t *pairs;
+    uint32_t np;
+    PmoAvbdBody *B;
+    PmoSchedule *S;
+    float dt;
+    uint32_t lanes;
+};
+
+inline bool dyn(const Ctx &x, uint32_t i) { return x.P[i].motion == kDynamic; }
+
+// rigid_avbd_prepare_contact_impact, geometry_constraints.slang:983-1005.
+bool contact_impact(const Ctx &x, uint32_t active) {
+    uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+    const PmoManifold &m = x.M[active];
+    bool impact = false;
+    for (uint32_t p = 0; p < m.count; ++p) {
+        const PmoContact &c = m.contacts[p];
+        float skin = contact_skin(x.P[a], x.P[b]);
+        bool fresh = (c.avbd_cached == 0u || c.avbd_dual[0].lambda <= 1.0e-6f ||
+                      (c.impact_fraction > 0.0f && c.impact_fraction < 1.0f)) &&
+                     c.initial_normal_speed < -0.1f && c.penetration >= -kSurfaceTolerance;
+        float prev_pen = c.penetration + c.initial_normal_speed * x.dt;
+        impact = impact || fresh || prev_pen > skin + 1.0e-4f;
+    }
+    return impact;
+}
+
+void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, uint32_t profile) {
+    const bool realtime = profile == kRealtimeProfile;
+    const float dt = x.dt;
+    // Bodies, geometry_constraints.slang:1235-1295.
+    for (uint32_t i = 0; i < x.n; ++i) {
+        V3 old_v = x.B[i].previous_velocity;
+        bool hist = x.B[i].history_valid != 0u;
+        PmoAvbdBody body;
+        std::memset(&body, 0, sizeof body);
+        body.parent = i;
+        body.color = kInvalid;
+        body.contact_head = kInvalid;
+        body.joint_head = kInvalid;
+        body.target = x.states[i];
+        body.orientation_delta = {0, 0, 0, 1};
+        if (x.P[i].motion != kDynamic) { x.B[i] = body; continue; }
+        float g2 = len2(gravity);
+        V3 acc = mul(sub(x.prev[i].linear_velocity, old_v), 1.0f / dt);
+        float w = hist && g2 > 0.0f ? clampf(dot(acc, gravity) / g2, 0.0f, 1.0f) : 0.0f;
+        if (!std::isfinite(w)) w = 0.0f;
+        V3 corr = mul(gravity, -(1.0f - w) * dt * dt);
+        body.displacement[0] = corr.x;
+        body.displacement[1] = corr.y;
+        body.displacement[2] = corr.z;
+        body.previous_velocity = x.prev[i].linear_velocity;
+        body.history_valid = 1u;
+        float mass = 1.0f / fmax_(x.P[i].inverse_mass, 1.0e-12f);
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            body.inertia_h[bidx(ax, ax)] = mass / (dt * dt);
+            V3 unit = ax == 0 ? V3{1, 0, 0} : (ax == 1 ? V3{0, 1, 0} : V3{0, 0, 1});
+            V3 la = inv_rotate(x.states[i].orientation, unit);
+            V3 col = rotate(x.states[i].orientation,
+                {la.x / fmax_(x.P[i].inverse_inertia_local.x, 1.0e-12f),
+                 la.y / fmax_(x.P[i].inverse_inertia_local.y, 1.0e-12f),
+                 la.z / fmax_(x.P[i].inverse_inertia_local.z, 1.0e-12f)});
+            for (uint32_t r = ax; r < 3; ++r)
+                body.inertia_h[bidx(3 + r, 3 + ax)] = comp(col, r) / (dt * dt);
+        }
+        x.B[i] = body;
+    }
+    // Contacts, geometry_constraints.slang:1297-1375.
+    for (uint32_t active = 0; active < x.np; ++active) {
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        float penalty = pair_stiffness(x.P[a], x.P[b], dt);
+        PmoManifold &m = x.M[active];
+        for (uint32_t p = 0; p < m.count; ++p) {
+            PmoContact &c = m.contacts[p];
+            if (c.avbd_cached == 0u) {
+                c.avbd_anchor_a = inv_rotate(x.prev[a].orientation, sub(c.point, x.prev[a].position));
+                c.avbd_anchor_b = inv_rotate(x.prev[b].orientation, sub(c.point, x.prev[b].position));
+            }
+            V3 ma = sub(x.states[a].position, x.prev[a].position);
+            V3 mb = sub(x.states[b].position, x.prev[b].position);
+            V3 wa = mul(quaternion_delta_velocity(x.prev[a].orientation, x.states[a].orientation, dt), dt);
+            V3 wb = mul(quaternion_delta_velocity(x.prev[b].orientation, x.states[b].orientation, dt), dt);
+            V3 armA = sub(c.point, x.states[a].position), armB = sub(c.point, x.states[b].position);
+            float closing = dot(c.normal, sub(add(mb, cross(wb, armB)), add(ma, cross(wa, armA))));
+            float old_pen = c.penetration - closing;
+            V3 t0, t1;
+            basis(c.normal, t0, t1);
+            V3 anchor_motion = add(sub(mb, ma),
+                sub(rotation_displacement(x.prev[b].orientation, x.states[b].orientation, c.avbd_anchor_b),
+                    rotation_displacement(x.prev[a].orientation, x.states[a].orientation, c.avbd_anchor_a)));
+            V3 old_delta = c.avbd_cached == 0u ? V3{0, 0, 0} : add(
+                sub(x.prev[b].position, x.prev[a].position),
+                sub(rotate(x.prev[b].orientation, c.avbd_anchor_b), rotate(x.prev[a].orientation, c.avbd_anchor_a)));
+            V3 terr = add(anchor_motion, mul(old_delta, 1.0f - kAlpha));
+            c.initial_normal_speed = -closing / dt;
+            float skin = contact_skin(x.P[a], x.P[b]);
+            float impact_disp = realtime && c.avbd_cached == 0u && c.initial_normal_speed < -0.1f
+                ? -(1.0f + fmin_(x.P[a].restitution, x.P[b].restitution)) * c.initial_normal_speed * dt
+                : 0.0f;
+            c.avbd_error = {c.penetration - clampf(old_pen, 0.0f, skin) + impact_disp,
+                            dot(terr, t0), dot(terr, t1)};
+            for (uint32_t ax = 0; ax < 3; ++ax) {
+                Dual d = {c.avbd_dual[ax].lambda, c.avbd_dual[ax].penalty};
+                if (c.avbd_cached == 0u) d = {0.0f, penalty};
+                d = warm_start(d, kMaxPenalty);
+                c.avbd_dual[ax] = {d.lambda, d.penalty};
+            }
+        }
+    }
+    // Schedule and graph (lane 0), geometry_constraints.slang:1432-1560.
+    PmoSchedule s;
+    std::memset(&s, 0, sizeof s);
+    s.budget = realtime ? kRealtimeSweeps : (requested != 0u ? requested : base_iterations);
+    s.offsets[0] = profile;
+    s.offsets[1] = base_iterations;
+    s.counts[2] = 0u;
+    s.candidates = x.np;
+    std::vector<uint32_t> parent(x.n);
+    for (uint32_t i = 0; i < x.n; ++i) parent[i] = i;
+    auto root = [&](uint32_t v) { while (v != parent[v]) v = parent[v]; return v; };
+    auto unite = [&](uint32_t a, uint32_t b) {
+        if (!dyn(x, a) || !dyn(x, b)) return;
+        a = root(a); b = root(b);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+    };
+    for (uint32_t active = 0; active < x.np; ++active) {
+        if (x.M[active].count == 0u) continue;
+        s.contacts++;
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        unite(a, b);
+        x.M[active].avbd_next_a = x.B[a].contact_head;
+        x.B[a].contact_head = active;
+        x.M[active].avbd_next_b = x.B[b].contact_head;
+        x.B[b].contact_head = active;
+    }
+    for (uint32_t i = 0; i < x.n; ++i) parent[i] = root(i);
+    std::vector<uint32_t> impact(x.n, 0u);
+    for (uint32_t active = 0; active < x.np; ++active) {
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        if (contact_impact(x, active)) {
+            if (dyn(x, a)) impact[parent[a]] = 1u;
+            if (dyn(x, b)) impact[parent[b]] = 1u;
+            s.counts[0] = 1u;
+        }
+    }
+    for (uint32_t i = 0; i < x.n; ++i) {
+        if (!dyn(x, i)) continue;
+        impact[i] = impact[parent[i]];
+        if (parent[i] == i && x.B[i].contact_head != kInvalid) s.island_count++;
+    }
+    for (uint32_t i = 0; i < x.n; ++i) {
+        if (!dyn(x, i)) continue;
+        uint32_t color = 0;
+        for (;; ++color) {
+            bool conflict = false;
+            for (uint32_t active = x.B[i].contact_head; active != kInvalid;) {
+                uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+                bool first = i == a;
+                conflict = conflict || x.B[first ? b : a].color == color;
+                active = first ? x.M[active].avbd_next_a : x.M[active].avbd_next_b;
+            }
+            if (!conflict) break;
+        }
+        x.B[i].color = color;
+        s.remaining = std::max(s.remaining, color + 1u);
+    }
+    for (uint32_t i = 0; i < x.n; ++i) {
+        x.B[i].parent = parent[i];
+        x.B[i].impact = dyn(x, i) ? impact[i] : 0u;
+    }
+    s.blocks = 1u;
+    *x.S = s;
+}
+
+// Butterfly reduction of L partial blocks, matching a wavefront's xor
+// shuffle tree: level s combines partial[l] and partial[l ^ s].
+Block tree_reduce(std::vector<Block> &parts) {
+    const size_t L = parts.size();
+    for (size_t s = 1; s < L; s <<= 1) {
+        std::vector<Block> next(L);
+        for (size_t l = 0; l < L; ++l) {
+            const Block &u = parts[l], &v = parts[l ^ s];
+            Block r;
+            for (int i = 0; i < 21; ++i) r.h[i] = u.h[i] + v.h[i];
+            for (int i = 0; i < 6; ++i) r.g[i] = u.g[i] + v.g[i];
+            next[l] = r;
+        }
+        parts.swap(next);
+    }
+    return parts[0];
+}
+
+struct PointRef { uint32_t active, point; };
+
+// Contact points incident to a body in list order (reverse active order).
+void body_points(const Ctx &x, uint32_t i, std::vector<PointRef> &out) {
+    out.clear();
+    for (uint32_t active = x.B[i].contact_head; active != kInvalid;) {
+        uint32_t a = x.pairs[active] / x.n;
+        bool first = i == a;
+        for (uint32_t p = 0; p < x.M[active].count; ++p) out.push_back({active, p});
+        active = first ? x.M[active].avbd_next_a : x.M[active].avbd_next_b;
+    }
+}
+
+// One contact point's three rows for body i (solve_color inner body,
+// geometry_constraints_solve.slang:356-397).
+void contact_point_rows(const Ctx &x, uint32_t i, PointRef r, uint32_t phase, Block &blk) {
+    uint32_t a = x.pairs[r.active] / x.n, b = x.pairs[r.active] % x.n;
+    bool first = i == a;
+    float mu = std::sqrt(x.P[a].friction * x.P[b].friction);
+    const PmoContact &c = x.M[r.active].contacts[r.point];
+    Vec6 ja[3], jb[3];
+    float err[3];
+    solve_contact_rows(c, x.B[a], x.B[b], ja, jb, err);
+    Force3 trial = {c.avbd_dual[0].lambda + c.avbd_dual[0].penalty * err[0],
+                    c.avbd_dual[1].lambda + c.avbd_dual[1].penalty * err[1],
+                    c.avbd_dual[2].lambda + c.avbd_dual[2].penalty * err[2]};
+    Force3 f = project_contact(trial, mu);
+    Force3 sc = stiffness_scales(trial, mu);
+    float stiff[3] = {sc.normal, sc.t0, sc.t1};
+    V3 t0, t1;
+    basis(c.normal, t0, t1);
+    float forces[3] = {
+        f.normal - (phase != 0u ? c.accumulated_normal_impulse / x.dt : 0.0f),
+        f.t0 - (phase != 0u ? dot(c.accumulated_friction_impulse, t0) / x.dt : 0.0f),
+        f.t1 - (phase != 0u ? dot(c.accumulated_friction_impulse, t1) / x.dt : 0.0f)};
+    for (uint32_t ax = 0; ax < 3; ++ax)
+        add_row(blk, first ? ja[ax] : jb[ax], c.avbd_dual[ax].penalty * stiff[ax], forces[ax]);
+}
+
+// Crossing safeguard for one point (geometry_constraints_solve.slang:
+// 433-465); returns true when the row was added.
+bool crossing_row(const Ctx &x, uint32_t i, PointRef r, const Vec6 &update, Block &blk) {
+    uint32_t a = x.pairs[r.active] / x.n, b = x.pairs[r.active] % x.n;
+    bool first = i == a;
+    const PmoContact &c = x.M[r.active].contacts[r.point];
+    Vec6 ja, jb;
+    float e = solve_contact_normal_row(c, x.B[a], x.B[b], ja, jb);
+    const Vec6 &nr = first ? ja : jb;
+    float trial = c.avbd_dual[0].lambda + c.avbd_dual[0].penalty * e;
+    if (trial <= 0.0f && trial + c.avbd_dual[0].penalty * dot6(nr, update) >
+                             c.avbd_dual[0].penalty * 1.0e-6f) {
+        add_row(blk, nr, c.avbd_dual[0].penalty, trial);
+        return true;
+    }
+    return false;
+}
+
+void solve_body(Ctx &x, uint32_t i, uint32_t phase, std::vector<PointRef> &pts) {
+    Block base;
+    std::memcpy(base.h, x.B[i].inertia_h, sizeof base.h);
+    std::memcpy(base.g, x.B[i].inertia_g, sizeof base.g);
+    for (uint32_t r = 0; r < 6; ++r)
+        for (uint32_t c = 0; c < 6; ++c) base.g[r] += base.h[bidx(r, c)] * x.B[i].displacement[c];
+    body_points(x, i, pts);
+    Block blk;
+    if (x.lanes == 0u) {
+        blk = base;
+        for (auto &r : pts) contact_point_rows(x, i, r, phase, blk);
+    } else {
+        std::vector<Block> parts(x.lanes, zero_block());
+        parts[0] = base;
+        for (size_t k = 0; k < pts.size(); ++k) contact_point_rows(x, i, pts[k], phase, parts[k % x.lanes]);
+        blk = tree_reduce(parts);
+    }
+    // Joint rows would follow here (rigid_avbd_visit_joint): no joints.
+    Vec6 update = zero6();
+    if (!block_solve(blk, update)) return;
+    bool crossing = false;
+    if (x.lanes == 0u) {
+        for (auto &r : pts) crossing = crossing_row(x, i, r, update, blk) || crossing;
+    } else {
+        std::vector<Block> parts(x.lanes, zero_block());
+        for (size_t k = 0; k < pts.size(); ++k)
+            crossing = crossing_row(x, i, pts[k], update, parts[k % x.lanes]) || crossing;
+        if (crossing) {
+            Block c = tree_reduce(parts);
+            for (int q = 0; q < 21; ++q) blk.h[q] += c.h[q];
+            for (int q = 0; q < 6; ++q) blk.g[q] += c.g[q];
+        }
+    }
+    if (crossing) {
+        Vec6 safe = zero6();
+        if (block_solve(blk, safe)) update = safe;
+    }
+    if (phase == 0u) apply_pose_update(x.B[i], update);
+    else for (int q = 0; q < 6; ++q) x.B[i].displacement[q] += update.v[q];
+}
+
+// advance_duals, geometry_constraints_solve.slang:492-566.
+void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
+    for (uint32_t active = 0; active < x.np; ++active) {
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        if (phase != 0u && x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
+        float mu = std::sqrt(x.P[a].friction * x.P[b].friction);
+        float beta = pair_stiffness(x.P[a], x.P[b], x.dt) * 100.0f;
+        PmoManifold &m = x.M[active];
+        for (uint32_t p = 0; p < m.count; ++p) {
+            PmoContact &c = m.contacts[p];
+            Vec6 ja[3], jb[3];
+            float err[3];
+            solve_contact_rows(c, x.B[a], x.B[b], ja, jb, err);
+            Force3 trial = {c.avbd_dual[0].lambda + c.avbd_dual[0].penalty * err[0],
+                            c.avbd_dual[1].lambda + c.avbd_dual[1].penalty * err[1],
+                            c.avbd_dual[2].lambda + c.avbd_dual[2].penalty * err[2]};
+            Force3 f = project_contact(trial, mu);
+            float tx = c.avbd_dual[1].lambda + c.avbd_dual[1].penalty * err[1];
+            float ty = c.avbd_dual[2].lambda + c.avbd_dual[2].penalty * err[2];
+            float fb = mu * f.normal;
+            bool stick = tx * tx + ty * ty <= fb * fb;
+            c.avbd_dual[0].lambda = f.normal;
+            c.avbd_dual[1].lambda = f.t0;
+            c.avbd_dual[2].lambda = f.t1;
+            for (uint32_t ax = 0; ax < 3; ++ax)
+                if (ax == 0 ? f.normal > 0.0f : stick)
+                    c.avbd_dual[ax].penalty = fmin_(kMaxPenalty, c.avbd_dual[ax].penalty + beta * std::fabs(err[ax]));
+            if (iteration + 1u == x.S->budget) {
+                V3 t0, t1;
+                basis(c.normal, t0, t1);
+                c.accumulated_normal_impulse = f.normal * x.dt;
+                c.reported_normal_impulse = c.accumulated_normal_impulse;
+                c.accumulated_friction_impulse = mul(add(mul(t0, f.t0), mul(t1, f.t1)), x.dt);
+                c.reported_friction_impulse = c.accumulated_friction_impulse;
+                if (!stick) {
+                    PmoState pa = phase != 0u ? x.B[a].target : avbd_pose(x.B[a]);
+                    PmoState pb = phase != 0u ? x.B[b].target : avbd_pose(x.B[b]);
+                    c.avbd_anchor_a = inv_rotate(pa.orientation, sub(c.point, pa.position));
+                    c.avbd_anchor_b = inv_rotate(pb.orientation, sub(c.point, pb.position));
+                }
+            }
+        }
+    }
+}
+
+// free_mean / free_subtract / free_clear, :617-698, with each island's
+// members reduced in ascending index order.
+void free_motion(Ctx &x, uint32_t phase) {
+    const bool vel = phase != 0u;
+    std::vector<std::vector<uint32_t>> members(x.n);
+    for (uint32_t i = 0; i < x.n; ++i)
+        if (dyn(x, i)) members[x.B[i].parent].push_back(i);
+    std::vector<V3> mean(x.n, V3{0, 0, 0});
+    for (uint32_t r = 0; r < x.n; ++r) {
+        if (!dyn(x, r) || x.B[r].parent != r) continue;
+        x.B[r].inertia_g[0] = x.B[r].inertia_g[1] = x.B[r].inertia_g[2] = 0.0f;
+        if ((vel && x.B[r].impact == 0u) || (x.B[r].contact_head == kInvalid && x.B[r].joint_head == kInvalid)) continue;
+        V3 weighted{0, 0, 0};
+        float total = 0.0f;
+        bool anchored = false;
+        for (uint32_t i : members[r]) {
+            if (anchored) break;
+            float mass = 1.0f / fmax_(x.P[i].inverse_mass, 1.0e-12f);
+            weighted = add(weighted, mul(trans(x.B[i].displacement), mass));
+            total += mass;
+            for (uint32_t active = x.B[i].contact_head; active != kInvalid;) {
+                uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+                bool first = i == a;
+                anchored = anchored || x.P[first ? b : a].motion != kDynamic;
+                active = first ? x.M[active].avbd_next_a : x.M[active].avbd_next_b;
+            }
+        }
+        if (!anchored && total > 0.0f) {
+            weighted = mul(weighted, 1.0f / total);
+            x.B[r].inertia_g[0] = weighted.x;
+            x.B[r].inertia_g[1] = weighted.y;
+            x.B[r].inertia_g[2] = weighted.z;
+        }
+    }
+    for (uint32_t i = 0; i < x.n; ++i) {
+        if (!dyn(x, i) || (vel && x.B[i].impact == 0u)) continue;
+        const PmoAvbdBody &root = x.B[x.B[i].parent];
+        for (int ax = 0; ax < 3; ++ax) x.B[i].displacement[ax] -= root.inertia_g[ax];
+    }
+    for (uint32_t i = 0; i < x.n; ++i) x.B[i].inertia_g[0] = x.B[i].inertia_g[1] = x.B[i].inertia_g[2] = 0.0f;
+}
+
+// apply_states, :699-733.
+void apply_states(Ctx &x, uint32_t phase, bool realtime) {
+    for (uint32_t i = 0; i < x.n; ++i) {
+        if (!dyn(x, i)) continue;
+        if (phase != 0u && x.B[i].impact == 0u) continue;
+        PmoState s = x.B[i].target;
+        const float *d = x.B[i].displacement;
+        if (phase != 0u) {
+            s.linear_velocity = add(s.linear_velocity, mul(trans(d), 1.0f / x.dt));
+            s.angular_velocity = add(s.angular_velocity, mul(rot6(d), 1.0f / x.dt));
+            x.states[i] = s;
+            continue;
+        }
+        s.position = add(s.position, trans(d));
+        s.orientation = avbd_pose(x.B[i]).orientation;
+        s.linear_velocity = add(s.linear_velocity, mul(trans(d), 1.0f / x.dt));
+        s.angular_velocity = add(s.angular_velocity, mul(quaternion_rotation_vector(x.B[i].orientation_delta), 1.0f / x.dt));
+        if (realtime) {
+            s.linear_velocity = mul(s.linear_velocity, 1.0f - kRealtimeDamping);
+            s.angular_velocity = mul(s.angular_velocity, 1.0f - kRealtimeDamping);
+        }
+        x.states[i] = s;
+    }
+}
+
+// Impact (velocity) phase set-up, :232-329.
+void impact_stages(Ctx &x) {
+    for (uint32_t active = 0; active < x.np; ++active) {                // impact_gaps
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        if (x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
+        PmoManifold &m = x.M[active];
+        for (uint32_t p = 0; p < m.count; ++p) {
+            PmoContact &c = m.contacts[p];
+            V3 armA = sub(c.point, x.B[a].target.position), armB = sub(c.point, x.B[b].target.position);
+            Vec6 ja = jac(mul(c.normal, -1.0f), mul(cross(armA, c.normal), -1.0f));
+            Vec6 jb = jac(c.normal, cross(armB, c.normal));
+            float gap = fmax_(0.0f, -(c.penetration + dot6(ja, disp(x.B[a])) + dot6(jb, disp(x.B[b]))));
+            c.avbd_error.x = gap;
+        }
+    }
+    for (uint32_t i = 0; i < x.n; ++i) {                                // impact_bodies
+        if (x.B[i].impact != 0u || x.P[i].motion != kDynamic) {
+            x.B[i].target = x.states[i];
+            std::memset(x.B[i].displacement, 0, sizeof x.B[i].displacement);
+            x.B[i].orientation_delta = {0, 0, 0, 1};
+        }
+    }
+    for (uint32_t active = 0; active < x.np; ++active) {                // impact_constraints
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        if (x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
+        Vec6 va = jac(mul(x.states[a].linear_velocity, x.dt), mul(x.states[a].angular_velocity, x.dt));
+        Vec6 vb = jac(mul(x.states[b].linear_velocity, x.dt), mul(x.states[b].angular_velocity, x.dt));
+        PmoManifold &m = x.M[active];
+        for (uint32_t p = 0; p < m.count; ++p) {
+            PmoContact &c = m.contacts[p];
+            float gap = c.avbd_error.x;
+            // rigid_avbd_contact_rows (:447-471): same Jacobians as the
+            // solve rows, anchors from the contact.
+            Vec6 ja[3], jb[3];
+            float err[3];
+            solve_contact_rows(c, x.B[a], x.B[b], ja, jb, err);
+            float restitution = c.avbd_cached == 0u && c.initial_normal_speed < -1.0f
+                ? fmin_(x.P[a].restitution, x.P[b].restitution) : 0.0f;
+            c.avbd_error = {dot6(ja[0], va) + dot6(jb[0], vb) - gap - restitution * c.initial_normal_speed * x.dt,
+                            dot6(ja[1], va) + dot6(jb[1], vb),
+                            dot6(ja[2], va) + dot6(jb[2], vb)};
+        }
+    }
+}
+
+int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, uint32_t profile) {
+    prepare(x, gravity, base_iterations, requested, profile);
+    const bool realtime = x.S->offsets[0] == kRealtimeProfile;
+    const uint32_t phases = realtime ? 1u : 2u;
+    std::vector<uint32_t> by_color;
+    std::vector<PointRef> pts;
+    for (uint32_t phase = 0; phase < phases; ++phase) {
+        if (phase != 0u && x.S->counts[0] == 0u) break;
+        if (phase != 0u) impact_stages(x);
+        for (uint32_t it = 0; it < x.S->budget; ++it) {
+            for (uint32_t color = 0; color < x.S->remaining; ++color) {
+                for (uint32_t i = 0; i < x.n; ++i) {
+                    if (x.B[i].color != color) continue;
+                    if (phase != 0u && x.B[i].impact == 0u) continue;
+                    solve_body(x, i, phase, pts);
+                }
+            }
+            advance_duals(x, phase, it);
+        }
+        free_motion(x, phase);
+        apply_states(x, phase, realtime);
+        x.S->counts[phase == 0u ? 6u : 7u] = x.S->budget;
+        x.S->maximum_passes += x.S->budget;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------
+// World: the production step order of emit_rigid_solve
+// (production_realtime_backend.cu.inc:7664-7861) preceded by the predictor
+// and proxy phases (:2642-2687).
+// ---------------------------------------------------------------------
+struct Proxy { V3 mn, mx; };
+
+struct CachedPair {
+    uint32_t count;
+    std::array<V3, 8> local_point, normal, anchor_a, anchor_b;
+    std::array<float, 8> normal_impulse;
+    std::array<V3, 8> friction_impulse;
+    std::array<std::array<PmoDual, 3>, 8> dual;
+};
+
+struct World {
+    PmoWorldDesc desc;
+    std::vector<Shape> shapes;
+    std::vector<PmoParams> params;
+    std::vector<PmoState> states;
+    std::vector<PmoAvbdBody> bodies;
+    std::vector<uint32_t> pairs;
+    std::vector<PmoManifold> manifolds;
+    std::vector<Proxy> proxies;
+    PmoSchedule schedule{};
+    PmoWorldStats stats{};
+    std::map<uint32_t, CachedPair> cache;
+    uint32_t cache_bodies = 0;
+    uint64_t epoch = 0;
+};
+
+// pm_integrate_rigid_bodies, geometry.slang:524-612 (no force/impulse
+// accumulators and no kinematic targets are set on this path).
+PmoState integrate(const PmoParams &b, const PmoState &prev, V3 g, float dt) {
+    PmoState next = prev;
+    if (b.motion == kStatic || b.motion == kKinematic) {
+        next.linear_velocity = {0, 0, 0};
+        next.angular_velocity = {0, 0, 0};
+        return next;
+    }
+    V3 force{0, 0, 0}, torque{0, 0, 0};
+    next.linear_velocity = add(next.linear_velocity, mul(add(g, mul(force, b.inverse_mass)), dt));
+    next.angular_velocity = add(next.angular_velocity, mul(inverse_inertia_world(b, next, torque), dt));
+    V3 imp{0, 0, 0};
+    next.linear_velocity = add(next.linear_velocity, mul(imp, b.inverse_mass));
+    next.angular_velocity = add(next.angular_velocity, inverse_inertia_world(b, next, imp));
+    next.linear_velocity = mul(next.linear_velocity, 1.0f / (1.0f + b.linear_damping * dt));
+    next.angular_velocity = mul(next.angular_velocity, 1.0f / (1.0f + b.angular_damping * dt));
+    next.linear_velocity = clamp_length(next.linear_velocity, b.maximum_linear_speed);
+    next.angular_velocity = clamp_length(next.angular_velocity, b.maximum_angular_speed);
+    next.position = add(next.position, mul(next.linear_velocity, dt));
+    Q w = {next.angular_velocity.x, next.angular_velocity.y, next.angular_velocity.z, 0.0f};
+    Q d = qmul(w, next.orientation);
+    next.orientation = qnormalize({next.orientation.x + 0.5f * d.x * dt, next.orientation.y + 0.5f * d.y * dt,
+                                   next.orientation.z + 0.5f * d.z * dt, next.orientation.w + 0.5f * d.w * dt});
+    return next;
+}
+
+// pm_emit_rigid_spatial_proxies, geometry.slang:716-827, with the
+// production arguments path_deviation 0, correction_budget 1, epsilon 1e-5
+// (production_realtime_backend.cu.inc:2663-2684).
+Proxy make_proxy(const PmoParams &p, const Shape &s, const PmoState &prev, const PmoState &pred) {
+    const float eps = 1.0e-5f;
+    float reach = len(s.center) + s.radius + p.collision_margin + eps;
+    float cr = correction_reach(p, s);
+    V3 bc = {fmin_(1.0f, cr), fmin_(1.0f, cr), fmin_(1.0f, cr)};
+    V3 e = add(add(V3{0, 0, 0}, bc), V3{reach, reach, reach});
+    return {sub(cmin(prev.position, pred.position), e), add(cmax(prev.position, pred.position), e)};
+}
+
+inline bool overlap(const Proxy &a, const Proxy &b) {          // spatial_build.slang:631-635
+    return a.mn.x <= b.mx.x && a.mx.x >= b.mn.x && a.mn.y <= b.mx.y && a.mx.y >= b.mn.y &&
+           a.mn.z <= b.mx.z && a.mx.z >= b.mn.z;
+}
+
+// Candidate pairs (H3-H6): every proxy overlap whose first body is dynamic,
+// filtered as pm_filter_rigid_spatial_candidates (geometry.slang:897-918)
+// with identity order = dense order (ids {index = dense, generation 1}),
+// no fixed groups and no constraints; ascending pair code.
+void find_pairs(World &w, const std::vector<PmoState> &pred) {
+    const uint32_t n = uint32_t(w.params.size());
+    w.pairs.clear();
+    auto keep = [&](uint32_t a, uint32_t b) {
+        if (w.params[a].motion != kDynamic || a == b) return false;
+        if (w.params[b].motion == kDynamic && !(a < b)) return false;
+        return true;
+    };
+    if (w.desc.flags & 1u) {
+        for (uint32_t a = 0; a < n; ++a)
+            for (uint32_t b = 0; b < n; ++b)
+                if (a != b && keep(a, b) && overlap(w.proxies[a], w.proxies[b])) w.pairs.push_back(a * n + b);
+        return;
+    }
+    // Sort-and-sweep on x: same inclusive comparisons, so the same set.
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return w.proxies[a].mn.x < w.proxies[b].mn.x; });
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t a = order[k];
+        for (uint32_t j = k + 1; j < n; ++j) {
+            uint32_t b = order[j];
+            if (w.proxies[b].mn.x > w.proxies[a].mx.x) break;
+            if (!overlap(w.proxies[a], w.proxies[b])) continue;
+            if (keep(a, b)) w.pairs.push_back(a * n + b);
+            if (keep(b, a)) w.pairs.push_back(b * n + a);
+        }
+    }
+    std::sort(w.pairs.begin(), w.pairs.end());
+    (void)pred;
+}
+
+// Per pair narrowphase: pm_prepare_rigid_face_manifolds (geometry.slang:
+// 2741-2803) for convex meshes, analytic round shapes otherwise, then
+// pm_finalize_rigid_contact_manifolds (:2805-2873).
+void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<PmoState> &pred) {
+    const uint32_t n = uint32_t(w.params.size());
+    w.manifolds.assign(w.pairs.size(), PmoManifold{});
+    w.stats.unsupported_pairs = 0;
+    for (size_t active = 0; active < w.pairs.size(); ++active) {
+        uint32_t a = w.pairs[active] / n, b = w.pairs[active] % n;
+        const Shape &sa = w.shapes[w.params[a].mesh_index], &sb = w.shapes[w.params[b].mesh_index];
+        float margin = w.params[a].collision_margin + w.params[b].collision_margin;
+        float cr = correction_reach(w.params[a], sa) + correction_reach(w.params[b], sb);
+        PmoManifold m;
+        clear_manifold(m);
+        if (convex_face_shape(sa) && convex_face_shape(sb)) {
+            bool swept = requires_swept(prev[a], pred[a], sa, prev[b], pred[b], sb, margin);
+            bool face = false;
+            if (swept) face = swept_convex_face_manifold(prev[a], pred[a], sa, prev[b], pred[b], sb, margin, m);
+            if (!face) {
+                face = convex_face_manifold(pred[a], sa, pred[b], sb, margin, cr, m);
+                if (swept && m.count == 0u) face = false;
+            }
+            if (!face) { clear_manifold(m); w.stats.unsupported_pairs++; }
+        } else if (!analytic_manifold(pred[a], sa, pred[b], sb, margin, margin + cr, m)) {
+            clear_manifold(m);
+            w.stats.unsupported_pairs++;
+        }
+        bool persistent = (sa.kind != kMesh || (sa.has_planes && sa.index_count() <= 96u)) &&
+            (w.params[b].motion != kDynamic || sb.kind != kMesh || (sb.has_planes && sb.index_count() <= 96u));
+        m.initial_relative_position = sub(pred[a].position, pred[b].position);
+        for (uint32_t p = 0; p < m.count; ++p) {
+            m.contacts[p].persistent = persistent ? 1u : 0u;
+            m.contacts[p].initial_normal_speed = contact_normal_speed(pred[a], pred[b], m.contacts[p].point, m.contacts[p].normal);
+            m.contacts[p].reported_normal_impulse = m.contacts[p].accumulated_normal_impulse;
+            m.contacts[p].reported_friction_impulse = m.contacts[p].accumulated_friction_impulse;
+        }
+        w.manifolds[active] = m;
+    }
+}
+
+uint32_t slot_capacity(const World &w) {                          // resource_plan.hpp:151-156
+    uint64_t cap = w.desc.body_capacity ? w.desc.body_capacity : w.params.size();
+    uint64_t man = cap * (cap - 1) / 2;
+    return uint32_t(std::min<uint64_t>(man, std::max<uint64_t>(4096, cap * 8)));
+}
+
+// Response slots (pm_prepare_parallel_contact_events, geometry.slang:
+// 2902-2939) and pm_load_rigid_contact_cache (:2941-3009).
+void load_cache(World &w, const std::vector<PmoState> &pred) {
+    const uint32_t n = uint32_t(w.params.size());
+    uint32_t slot = 0;
+    for (auto &m : w.manifolds) m.response_slot = m.count != 0u ? slot++ : kInvalid;
+    w.stats.cache_hits = 0;
+    for (size_t active = 0; active < w.pairs.size(); ++active) {
+        auto it = w.cache.find(w.pairs[active]);
+        if (it == w.cache.end()) continue;
+        const CachedPair &saved = it->second;
+        uint32_t body = w.pairs[active] / n;
+        PmoManifold &m = w.manifolds[active];
+        uint32_t used = 0;
+        for (uint32_t p = 0; p < m.count; ++p) {
+            PmoContact &c = m.contacts[p];
+            V3 local = inv_rotate(pred[body].orientation, sub(c.point, pred[body].position));
+            float nearest = 0.02f * 0.02f;
+            uint32_t match = 8;
+            for (uint32_t q = 0; q < saved.count; ++q) {
+                if ((used & (1u << q)) != 0u || dot(c.normal, saved.normal[q]) < 0.99f) continue;
+                float d = len2(sub(local, saved.local_point[q]));
+                if (d < nearest) { nearest = d; match = q; }
+            }
+            if (match == 8u) continue;
+            used |= 1u << match;
+            for (int ax = 0; ax < 3; ++ax) c.avbd_dual[ax] = saved.dual[match][ax];
+            V3 t0 = normalized_or(cross(c.normal, std::fabs(c.normal.x) < 0.57735f ? V3{1, 0, 0} : V3{0, 1, 0}), {0, 0, 1});
+            V3 t1 = cross(c.normal, t0);
+            c.avbd_dual[1].lambda = dot(saved.friction_impulse[match], t0) / w.desc.dt;
+            c.avbd_dual[2].lambda = dot(saved.friction_impulse[match], t1) / w.desc.dt;
+            c.avbd_anchor_a = saved.anchor_a[match];
+            c.avbd_anchor_b = saved.anchor_b[match];
+            c.avbd_cached = 1u;
+            c.accumulated_normal_impulse = saved.normal_impulse[match];
+            c.accumulated_friction_impulse = sub(saved.friction_impulse[match],
+                mul(c.normal, dot(saved.friction_impulse[match], c.normal)));
+            w.stats.cache_hits++;
+        }
+    }
+}
+
+// pm_save_rigid_contact_cache, geometry.slang:3011-3070.
+void save_cache(World &w) {
+    const uint32_t n = uint32_t(w.params.size());
+    const uint32_t cap = slot_capacity(w);
+    w.cache.clear();
+    for (size_t active = 0; active < w.pairs.size(); ++active) {
+        const PmoManifold &m = w.manifolds[active];
+        if (m.response_slot >= cap) continue;
+        uint32_t body = w.pairs[active] / n;
+        CachedPair s{};
+        s.count = m.count;
+        for (uint32_t p = 0; p < m.count; ++p) {
+            const PmoContact &c = m.contacts[p];
+            s.local_point[p] = inv_rotate(w.states[body].orientation, sub(c.point, w.states[body].position));
+            s.normal[p] = c.normal;
+            s.normal_impulse[p] = c.accumulated_normal_impulse;
+            s.friction_impulse[p] = c.accumulated_friction_impulse;
+            for (int ax = 0; ax < 3; ++ax) s.dual[p][ax] = c.avbd_dual[ax];
+            s.anchor_a[p] = c.avbd_anchor_a;
+            s.anchor_b[p] = c.avbd_anchor_b;
+        }
+        w.cache[w.pairs[active]] = s;
+    }
+}
+
+void world_step(World &w) {
+    const uint32_t n = uint32_t(w.params.size());
+    const V3 g = {w.desc.gravity[0], w.desc.gravity[1], w.desc.gravity[2]};
+    const float dt = w.desc.dt;
+    if (w.cache_bodies != n) { w.cache.clear(); w.cache_bodies = n; }
+    std::vector<PmoState> prev = w.states, pred(n);
+    for (uint32_t i = 0; i < n; ++i) pred[i] = integrate(w.params[i], prev[i], g, dt);
+    w.proxies.resize(n);
+    for (uint32_t i = 0; i < n; ++i)
+        w.proxies[i] = make_proxy(w.params[i], w.shapes[w.params[i].mesh_index], prev[i], pred[i]);
+    find_pairs(w, pred);
+    narrowphase(w, prev, pred);
+    load_cache(w, pred);
+    if (w.bodies.size() != n) w.bodies.assign(n, PmoAvbdBody{});
+    Ctx x{w.params.data(), prev.data(), pred.data(), n, w.manifolds.data(), w.pairs.data(),
+          uint32_t(w.pairs.size()), w.bodies.data(), &w.schedule, dt, w.desc.lanes};
+    prepare_solve(x, g, w.desc.base_iterations, w.desc.requested_iterations, w.desc.profile_version);
+    for (uint32_t i = 0; i < n; ++i) {                               // pm_clamp_rigid_speeds, :3086-3100
+        if (w.params[i].motion != kDynamic) continue;
+        pred[i].linear_velocity = clamp_length(pred[i].linear_velocity, w.params[i].maximum_linear_speed);
+        pred[i].angular_velocity = clamp_length(pred[i].angular_velocity, w.params[i].maximum_angular_speed);
+    }
+    w.states = pred;
+    save_cache(w);
+    ++w.epoch;
+    w.stats.pairs = uint32_t(w.pairs.size());
+    w.stats.manifolds = 0;
+    w.stats.points = 0;
+    for (auto &m : w.manifolds) { w.stats.manifolds += m.count != 0u; w.stats.points += m.count; }
+    w.stats.colors = w.schedule.remaining;
+    w.stats.islands = w.schedule.island_count;
+    w.stats.impact = w.schedule.counts[0];
+    w.stats.budget = w.schedule.budget;
+    w.stats.phases = w.schedule.counts[7] != 0u ? 2u : 1u;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// C API
+// ---------------------------------------------------------------------
+extern "C" {
+
+void pmo_warm_start(const float d[2], float s, float out[2]) {
+    Dual r = warm_start({d[0], d[1]}, s);
+    out[0] = r.lambda; out[1] = r.penalty;
+}
+float pmo_force(const float d[2], float e, float lo, float hi) { return dual_force({d[0], d[1]}, e, lo, hi); }
+void pmo_update_dual(const float d[2], float e, float beta, float lo, float hi, float s, float out[2]) {
+    Dual r = update_dual({d[0], d[1]}, e, beta, lo, hi, s);
+    out[0] = r.lambda; out[1] = r.penalty;
+}
+void pmo_project_contact(const float f[3], float mu, float out[3]) {
+    Force3 r = project_contact({f[0], f[1], f[2]}, mu);
+    out[0] = r.normal; out[1] = r.t0; out[2] = r.t1;
+}
+void pmo_stiffness_scales(const float f[3], float mu, float out[3]) {
+    Force3 r = stiffness_scales({f[0], f[1], f[2]}, mu);
+    out[0] = r.normal; out[1] = r.t0; out[2] = r.t1;
+}
+void pmo_block_add_row(float h[21], float g[6], const float j[6], float k, float f) {
+    Block b;
+    std::memcpy(b.h, h, sizeof b.h); std::memcpy(b.g, g, sizeof b.g);
+    Vec6 v; std::memcpy(v.v, j, sizeof v.v);
+    add_row(b, v, k, f);
+    std::memcpy(h, b.h, sizeof b.h); std::memcpy(g, b.g, sizeof b.g);
+}
+int pmo_block_solve(const float h[21], const float g[6], float delta[6]) {
+    Block b;
+    std::memcpy(b.h, h, sizeof b.h); std::memcpy(b.g, g, sizeof b.g);
+    Vec6 d;
+    bool ok = block_solve(b, d);
+    std::memcpy(delta, d.v, sizeof d.v);
+    return ok ? 1 : 0;
+}
+static Vec6 v6(const float *p) { Vec6 v; std::memcpy(v.v, p, sizeof v.v); return v; }
+float pmo_row_error(const float row[22], const float a[6], const float b[6]) { return row_error(row_from(row), v6(a), v6(b)); }
+float pmo_row_trial(const float row[22], const float a[6], const float b[6], float dt) { return row_trial(row_from(row), v6(a), v6(b), dt); }
+float pmo_row_force(const float row[22], const float a[6], const float b[6], float dt) { return row_force(row_from(row), v6(a), v6(b), dt); }
+void pmo_row_accumulate(float h[21], float g[6], const float row[22], int first, const float a[6], const float b[6], float dt) {
+    Block blk;
+    std::memcpy(blk.h, h, sizeof blk.h); std::memcpy(blk.g, g, sizeof blk.g);
+    row_accumulate(blk, row_from(row), first != 0, v6(a), v6(b), dt);
+    std::memcpy(h, blk.h, sizeof blk.h); std::memcpy(g, blk.g, sizeof blk.g);
+}
+void pmo_row_advance(float row[22], const float a[6], const float b[6]) {
+    Row r = row_from(row);
+    row_advance(r, v6(a), v6(b));
+    row_to(r, row);
+}
+
+int pmo_avbd_prepare_solve(const PmoParams *parameters, const PmoState *previous, PmoState *states,
+                           uint32_t count, PmoManifold *manifolds, const uint32_t *pairs,
+                           uint32_t pair_count, PmoAvbdBody *bodies, PmoSchedule *schedule,
+                           float dt, const float gravity[3], uint32_t base_iterations,
+                           uint32_t requested_iterations, uint32_t profile_version, uint32_t lanes) {
+    if (lanes & (lanes - 1u)) return -1;
+    Ctx x{parameters, previous, states, count, manifolds, pairs, pair_count, bodies, schedule, dt, lanes};
+    return prepare_solve(x, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations, profile_version);
+}
+
+void *pmo_world_create(const PmoWorldDesc *desc) {
+    if (!desc || !(desc->dt > 0.0f) || (desc->lanes & (desc->lanes - 1u))) return nullptr;
+    World *w = new World();
+    w->desc = *desc;
+    return w;
+}
+void pmo_world_destroy(void *world) { delete static_cast<World *>(world); }
+int pmo_world_add_mesh(void *world, const float *v, uint32_t nv, const uint32_t *idx, uint32_t ni) {
+    World &w = *static_cast<World *>(world);
+    Shape s;
+    if (!prepare_mesh(s, v, nv, idx, ni)) return -1;
+    w.shapes.push_back(std::move(s));
+    return int(w.shapes.size() - 1);
+}
+int pmo_world_add_sphere(void *world, float r) {
+    World &w = *static_cast<World *>(world);
+    if (!(r > 0.0f)) return -1;
+    Shape s;
+    s.kind = kSphere;
+    s.sphere_radius = r;
+    s.minimum = {-r, -r, -r}; s.maximum = {r, r, r};
+    s.center = {0, 0, 0}; s.radius = r;
+    float i = 0.4f * r * r;
+    s.unit_inertia = {i, i, i};
+    w.shapes.push_back(s);
+    return int(w.shapes.size() - 1);
+}
+int pmo_world_add_capsule(void *world, float r, float h) {
+    World &w = *static_cast<World *>(world);
+    if (!(r > 0.0f) || !(h >= 0.0f)) return -1;
+    Shape s;
+    s.kind = kCapsule;
+    s.sphere_radius = r;
+    s.half_length = h;
+    s.minimum = {-r, -(h + r), -r}; s.maximum = {r, h + r, r};
+    s.center = {0, 0, 0}; s.radius = h + r;
+    V3 half = {r, h + r, r};
+    s.unit_inertia = {(half.y * half.y + half.z * half.z) / 3.0f, (half.x * half.x + half.z * half.z) / 3.0f,
+                      (half.x * half.x + half.y * half.y) / 3.0f};
+    w.shapes.push_back(s);
+    return int(w.shapes.size() - 1);
+}
+int pmo_world_add_bodies(void *world, uint32_t count, const PmoParams *p, const PmoState *s) {
+    World &w = *static_cast<World *>(world);
+    for (uint32_t i = 0; i < count; ++i)
+        if (p[i].mesh_index >= w.shapes.size() || p[i].motion > kDynamic) return -1;
+    w.params.insert(w.params.end(), p, p + count);
+    w.states.insert(w.states.end(), s, s + count);
+    return 0;
+}
+int pmo_world_step(void *world, uint32_t steps) {
+    World &w = *static_cast<World *>(world);
+    uint64_t n = w.params.size();
+    if (n * n > 0xffffffffull) return -2;  // pair codes a*n+b must fit 32 bits
+    for (uint32_t s = 0; s < steps; ++s) world_step(w);
+    return 0;
+}
+uint32_t pmo_world_body_count(void *world) { return uint32_t(static_cast<World *>(world)->params.size()); }
+void pmo_world_get_states(void *world, PmoState *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.states.begin(), w.states.end(), out);
+}
+void pmo_world_set_states(void *world, const PmoState *s) {
+    World &w = *static_cast<World *>(world);
+    std::copy(s, s + w.states.size(), w.states.begin());
+    w.cache.clear();
+    for (auto &b : w.bodies) b = PmoAvbdBody{};
+}
+void pmo_world_get_stats(void *world, PmoWorldStats *out) { *out = static_cast<World *>(world)->stats; }
+uint32_t pmo_world_pair_count(void *world) { return uint32_t(static_cast<World *>(world)->pairs.size()); }
+void pmo_world_get_pairs(void *world, uint32_t *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.pairs.begin(), w.pairs.end(), out);
+}
+void pmo_world_get_manifolds(void *world, PmoManifold *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.manifolds.begin(), w.manifolds.end(), out);
+}
+void pmo_world_get_bodies(void *world, PmoAvbdBody *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.bodies.begin(), w.bodies.end(), out);
+}
+void pmo_world_get_schedule(void *world, PmoSchedule *out) { *out = static_cast<World *>(world)->schedule; }
+void pmo_world_get_proxies(void *world, float *out) {
+    World &w = *static_cast<World *>(world);
+    for (size_t i = 0; i < w.proxies.size(); ++i) {
+        const Proxy &p = w.proxies[i];
+        float v[6] = {p.mn.x, p.mn.y, p.mn.z, p.mx.x, p.mx.y, p.mx.z};
+        std::memcpy(out + 6 * i, v, sizeof v);
+    }
+}
+void pmo_world_get_shape(void *world, uint32_t shape, float out[16]) {
+    World &w = *static_cast<World *>(world);
+    const Shape &s = w.shapes[shape];
+    float v[16] = {s.minimum.x, s.minimum.y, s.minimum.z, s.maximum.x, s.maximum.y, s.maximum.z,
+                   s.center.x, s.center.y, s.center.z, s.radius, s.unit_inertia.x, s.unit_inertia.y,
+                   s.unit_inertia.z, float(s.kind), float(s.index_count()), float(s.unique_planes)};
+    std::memcpy(out, v, sizeof v);
+}
+void pmo_world_get_mesh_indices(void *world, uint32_t shape, uint32_t *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.shapes[shape].indices.begin(), w.shapes[shape].indices.end(), out);
+}
+void pmo_world_get_mesh_planes(void *world, uint32_t shape, float *out) {
+    World &w = *static_cast<World *>(world);
+    const Shape &s = w.shapes[shape];
+    for (size_t i = 0; i < s.planes.size(); ++i) {
+        out[4 * i] = s.planes[i].normal.x; out[4 * i + 1] = s.planes[i].normal.y;
+        out[4 * i + 2] = s.planes[i].normal.z; out[4 * i + 3] = s.planes[i].offset;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * CPU oracle for the ParallelMater rigid-body step (broadphase, convex
+ * narrowphase, AVBD prepare and solve).  TEST INFRASTRUCTURE ONLY: tests,
+ * the smoke check and the CPU-baseline leg of the benchmark call it as the
+ * checker.  The product never links, loads or calls anything here.
+ *
+ * The record layouts below restate the reference's storage layouts
+ * byte-for-byte because the kernel seam exchanges them
+ * (src/simulation/geometry_storage.hpp:18-213, src/slang/types.slang:81-101,
+ * src/slang/geometry.slang:143-238).  Sizes are checked by static_assert in
+ * pm_oracle.cpp.
+ */
+#ifndef PM_ORACLE_H
+#define PM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct PmoVec3 { float x, y, z; } PmoVec3;
+typedef struct PmoQuat { float x, y, z, w; } PmoQuat;
+
+/* RigidBodyState, 52 bytes (types.slang:97-103). */
+typedef struct PmoState {
+    PmoVec3 position;
+    PmoQuat orientation;
+    PmoVec3 linear_velocity;
+    PmoVec3 angular_velocity;
+} PmoState;
+
+/* BodyParameters, 64 bytes (types.slang:81-95). motion: 0 static,
+ * 1 kinematic, 2 dynamic. mesh_index selects the shape. */
+typedef struct PmoParams {
+    uint32_t motion;
+    uint32_t mesh_index;
+    uint32_t mesh_generation;
+    float inverse_mass;
+    PmoVec3 inverse_inertia_local;
+    float friction;
+    float restitution;
+    float linear_damping;
+    float angular_damping;
+    float maximum_linear_speed;
+    float maximum_angular_speed;
+    float collision_margin;
+    uint64_t user_data;
+} PmoParams;
+
+/* HingeContactFrame, 72 bytes (geometry.slang:143-157). */
+typedef struct PmoHingeFrame {
+    PmoVec3 anchor, axis, local_anchor;
+    float inverse_mass, inverse_moment;
+    uint32_t present, fixed, axial, axial_rotation, fixed_member,
+        point_member, static_body;
+} PmoHingeFrame;
+
+typedef struct PmoDual { float lambda, penalty; } PmoDual;
+
+/* Contact, 284 bytes (geometry.slang:159-179). */
+typedef struct PmoContact {
+    PmoVec3 normal;
+    PmoVec3 point;
+    float penetration;
+    PmoHingeFrame body_hinge;
+    PmoHingeFrame collider_hinge;
+    float accumulated_normal_impulse;
+    PmoVec3 accumulated_friction_impulse;
+    float initial_normal_speed;
+    uint32_t warm_started;
+    uint32_t persistent;
+    float impact_fraction;
+    float reported_normal_impulse;
+    PmoVec3 reported_friction_impulse;
+    PmoDual avbd_dual[3];
+    PmoVec3 avbd_error;
+    PmoVec3 avbd_anchor_a;
+    PmoVec3 avbd_anchor_b;
+    uint32_t avbd_cached;
+} PmoContact;
+
+/* ContactManifold, 2304 bytes (geometry.slang:181-190). */
+typedef struct PmoManifold {
+    PmoContact contacts[8];
+    uint32_t count;
+    PmoVec3 initial_relative_position;
+    uint32_t face_patch;
+    uint32_t response_slot;
+    uint32_t avbd_next_a;
+    uint32_t avbd_next_b;
+} PmoManifold;
+
+/* AvbdBody / GeometryConstraintBody, 236 bytes
+ * (geometry_constraints.slang:42-55). */
+typedef struct PmoAvbdBody {
+    PmoState target;
+    PmoQuat orientation_delta;
+    float displacement[6];
+    float inertia_h[21];
+    float inertia_g[6];
+    uint32_t contact_head, joint_head, color, parent, impact;
+    PmoVec3 previous_velocity;
+    uint32_t history_valid;
+} PmoAvbdBody;
+
+/* ContactSchedule, 264 bytes (geometry.slang:226-238). */
+typedef struct PmoSchedule {
+    uint32_t counts[29];
+    uint32_t offsets[29];
+    uint32_t budget, remaining, island_count, early_exit_count,
+        maximum_passes, blocks, candidates, contacts;
+} PmoSchedule;
+
+/* ---- numerical core (constraint_math.slang:16-345) ------------------- */
+/* Row record, 22 floats: a[6] b[6] lambda penalty error velocity lower
+ * upper stiffness damping beta reference_force. */
+void pmo_warm_start(const float dual[2], float stiffness, float out[2]);
+float pmo_force(const float dual[2], float error, float lower, float upper);
+void pmo_update_dual(const float dual[2], float error, float beta,
+                     float lower, float upper, float stiffness,
+                     float out[2]);
+void pmo_project_contact(const float force[3], float friction,
+                         float out[3]);
+void pmo_stiffness_scales(const float trial[3], float friction,
+                          float out[3]);
+void pmo_block_add_row(float h[21], float g[6], const float jacobian[6],
+                       float stiffness, float row_force);
+int pmo_block_solve(const float h[21], const float g[6], float delta[6]);
+float pmo_row_error(const float row[22], const float a[6], const float b[6]);
+float pmo_row_trial(const float row[22], const float a[6], const float b[6],
+                    float dt);
+float pmo_row_force(const float row[22], const float a[6], const float b[6],
+                    float dt);
+void pmo_row_accumulate(float h[21], float g[6], const float row[22],
+                        int first, const float a[6], const float b[6],
+                        float dt);
+void pmo_row_advance(float row[22], const float a[6], const float b[6]);
+
+/* ---- kernel seam: prepare + solve on caller-built manifolds ----------
+ * Restates prepare_avbd_kernel (geometry_constraints.slang:1213-1591)
+ * followed by solve_avbd_kernel (geometry_constraints_solve.slang:771-865)
+ * with the reference's serial CPU-driver semantics. `lanes` = 0 keeps the
+ * reference accumulation order; lanes = L > 0 sums each body's contact rows
+ * as L interleaved partial blocks combined by a butterfly tree. */
+int pmo_avbd_prepare_solve(const PmoParams *parameters,
+                           const PmoState *previous, PmoState *states,
+                           uint32_t count, PmoManifold *manifolds,
+                           const uint32_t *pairs, uint32_t pair_count,
+                           PmoAvbdBody *bodies, PmoSchedule *schedule,
+                           float dt, const float gravity[3],
+                           uint32_t base_iterations,
+                           uint32_t requested_iterations,
+                           uint32_t profile_version, uint32_t lanes);
+
+/* ---- world ------------------------------------------------------------ */
+typedef struct PmoWorldDesc {
+    float gravity[3];
+    float dt;
+    uint32_t profile_version;      /* 1 realtime (8 sweeps), 0 legacy */
+    uint32_t base_iterations;
+    uint32_t requested_iterations;
+    uint32_t body_capacity;        /* 0: the body count at first step */
+    uint32_t lanes;                /* accumulation order, see above */
+    uint32_t flags;                /* bit 0: brute-force broadphase */
+} PmoWorldDesc;
+
+typedef struct PmoWorldStats {
+    uint32_t pairs;                /* active pairs after filtering */
+    uint32_t manifolds;            /* non-empty manifolds */
+    uint32_t points;               /* contact points */
+    uint32_t colors;
+    uint32_t islands;
+    uint32_t impact;               /* schedule counts[0] */
+    uint32_t budget;
+    uint32_t unsupported_pairs;    /* would take the triangle-pair path */
+    uint32_t cache_hits;           /* contact points restored from cache */
+    uint32_t phases;
+} PmoWorldStats;
+
+void *pmo_world_create(const PmoWorldDesc *desc);
+void pmo_world_destroy(void *world);
+/* Returns the shape index or -1. Meshes run the reference's upload
+ * preparation (runtime_deformables.cu.inc:4-304). */
+int pmo_world_add_mesh(void *world, const float *vertices, uint32_t vertex_count,
+                       const uint32_t *indices, uint32_t index_count);
+/* Analytic sphere (radius) and capsule (radius, half segment length along
+ * local y): shapes with no reference counterpart. */
+int pmo_world_add_sphere(void *world, float radius);
+int pmo_world_add_capsule(void *world, float radius, float half_length);
+int pmo_world_add_bodies(void *world, uint32_t count,
+                         const PmoParams *parameters, const PmoState *states);
+int pmo_world_step(void *world, uint32_t steps);
+uint32_t pmo_world_body_count(void *world);
+void pmo_world_get_states(void *world, PmoState *out);
+void pmo_world_set_states(void *world, const PmoState *states);
+void pmo_world_get_stats(void *world, PmoWorldStats *out);
+/* Results of the last step: active pair codes a*n+b (ascending), their
+ * manifolds, the AVBD bodies (colour, parent, impact) and schedule. */
+uint32_t pmo_world_pair_count(void *world);
+void pmo_world_get_pairs(void *world, uint32_t *out);
+void pmo_world_get_manifolds(void *world, PmoManifold *out);
+void pmo_world_get_bodies(void *world, PmoAvbdBody *out);
+void pmo_world_get_schedule(void *world, PmoSchedule *out);
+/* Proxy AABBs of the last step, 6 floats per body (min xyz, max xyz). */
+void pmo_world_get_proxies(void *world, float *out);
+/* Shape metadata after upload preparation: 16 floats
+ * (min xyz, max xyz, bounding centre xyz, radius, unit inertia xyz,
+ * kind, index count, unique plane count). */
+void pmo_world_get_shape(void *world, uint32_t shape, float out[16]);
+/* Mesh indices after BVH reordering and per-triangle planes (4 floats). */
+void pmo_world_get_mesh_indices(void *world, uint32_t shape, uint32_t *out);
+void pmo_world_get_mesh_planes(void *world, uint32_t shape, float *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,343 @@
+# SPDX-License-Identifier: MIT
+"""Seeded scene builders shared by the oracle and GPU tests and the bench.
+
+Meshes follow the reference's builders: the box of
+conformance/v1/src/command_runner.cpp:230-242 and the cube-projected
+sphere of examples/support/scene.cpp:599-632.  Kernel-level fixtures follow
+tests/solver/rigid_realtime_cuda_tests.cu:43-155 and :172-211.
+"""
+import numpy as np
+
+from pmamd import layouts as L
+
+DT = np.float32(1.0) / np.float32(60.0)
+BOX_INDICES = np.array([0, 1, 2, 0, 2, 3, 4, 6, 5, 4, 7, 6, 0, 4, 5, 0, 5, 1,
+                        1, 5, 6, 1, 6, 2, 2, 6, 7, 2, 7, 3, 3, 7, 4, 3, 4, 0],
+                       dtype=np.uint32)
+
+
+def box_mesh(hx, hy, hz):
+    v = np.array([[-hx, -hy, -hz], [hx, -hy, -hz], [hx, hy, -hz], [-hx, hy, -hz],
+                  [-hx, -hy, hz], [hx, -hy, hz], [hx, hy, hz], [-hx, hy, hz]],
+                 dtype=np.float32)
+    return v, BOX_INDICES.copy()
+
+
+def cube_sphere_mesh(radius):
+    cube = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                     [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1],
+                     [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1],
+                     [-1, 0, 0], [1, 0, 0]], dtype=np.float32)
+    norm = np.sqrt((cube * cube).sum(axis=1, dtype=np.float32)).astype(np.float32)
+    v = (cube / norm[:, None] * np.float32(radius)).astype(np.float32)
+    idx = []
+    for center, corners in ((8, (0, 1, 5, 4)), (9, (3, 7, 6, 2)), (10, (0, 3, 2, 1)),
+                            (11, (4, 5, 6, 7)), (12, (0, 4, 7, 3)), (13, (1, 2, 6, 5))):
+        for k in range(4):
+            idx += [center, corners[k], corners[(k + 1) % 4]]
+    return v, np.array(idx, dtype=np.uint32)
+
+
+def params(n, motion=L.MOTION_DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0),
+           friction=0.5, restitution=0.0, margin=0.01, shape=0,
+           max_linear=1000.0, max_angular=1000.0):
+    p = np.zeros(n, dtype=L.PARAMS)
+    p["motion"] = motion
+    p["mesh_index"] = shape
+    p["mesh_generation"] = 1
+    dyn = np.asarray(motion) == L.MOTION_DYNAMIC
+    mass = np.broadcast_to(np.asarray(mass, dtype=np.float32), (n,))
+    inertia = np.broadcast_to(np.asarray(inertia, dtype=np.float32), (n, 3))
+    p["inverse_mass"] = np.where(dyn, np.float32(1.0) / mass, 0.0)
+    p["inverse_inertia_local"] = np.where(np.broadcast_to(dyn, (n,))[:, None],
+                                          np.float32(1.0) / inertia, 0.0)
+    p["friction"] = friction
+    p["restitution"] = restitution
+    p["collision_margin"] = margin
+    p["maximum_linear_speed"] = max_linear
+    p["maximum_angular_speed"] = max_angular
+    return p
+
+
+def states(n, positions=None):
+    s = np.zeros(n, dtype=L.STATE)
+    s["orientation"][:, 3] = 1.0
+    if positions is not None:
+        s["position"] = positions
+    return s
+
+
+class Scene:
+    """Shapes (("mesh", v, i) | ("sphere", r) | ("capsule", r, h)), body
+    parameters and initial states."""
+
+    def __init__(self, shapes, params, states, name=""):
+        self.shapes, self.params, self.states, self.name = shapes, params, states, name
+
+    @property
+    def n(self):
+        return len(self.params)
+
+    def build(self, world):
+        for shape in self.shapes:
+            if shape[0] == "mesh":
+                world.add_mesh(shape[1], shape[2])
+            elif shape[0] == "sphere":
+                world.add_sphere(shape[1])
+            else:
+                world.add_capsule(shape[1], shape[2])
+        world.add_bodies(self.params, self.states)
+        return world
+
+
+def box_inertia(mass, half):
+    hx, hy, hz = (np.float32(h) for h in half)
+    m = np.float32(mass)
+    return (m * (hy * hy + hz * hz) / np.float32(3), m * (hx * hx + hz * hz) / np.float32(3),
+            m * (hx * hx + hy * hy) / np.float32(3))
+
+
+def pyramid(rows=10, seed=1, jitter=1e-3, margin=0.01, friction=0.5):
+    """55 unit boxes (10 rows), density 1000,
+    friction 0.5 on a static ground box (half 50, 0.5, 50) at y = -0.5,
+    +-1 mm seeded position jitter.  Body 0 is the ground."""
+    rng = np.random.default_rng(seed)
+    shapes = [("mesh", *box_mesh(50.0, 0.5, 50.0)), ("mesh", *box_mesh(0.5, 0.5, 0.5))]
+    pos = [(0.0, -0.5, 0.0)]
+    for r in range(rows):
+        count = rows - r
+        for i in range(count):
+            pos.append(((i - (count - 1) / 2.0), 0.5 + r, 0.0))
+    pos = np.array(pos, dtype=np.float32)
+    pos[1:] += rng.uniform(-jitter, jitter, size=(len(pos) - 1, 3)).astype(np.float32)
+    n = len(pos)
+    motion = np.full(n, L.MOTION_DYNAMIC, dtype=np.uint32)
+    motion[0] = L.MOTION_STATIC
+    p = params(n, motion=motion, mass=1000.0, inertia=box_inertia(1000.0, (0.5, 0.5, 0.5)),
+               friction=friction, margin=margin, shape=np.where(motion == 0, 0, 1))
+    return Scene(shapes, p, states(n, pos), f"pyramid{rows}")
+
+
+def pile(side=32, seed=2, spacing=0.8, jitter=0.05, analytic_spheres=True,
+         margin=0.01, height=0.6):
+    """BASELINE configuration 2 at side**3 bodies: half boxes (edges
+    U[0.2, 0.6] m) and half spheres (r U[0.1, 0.3] m) on a jittered grid of
+    `spacing`, density log-uniform [200, 5000] kg/m^3, friction U[0.3, 0.8],
+    above a static floor and inside four static walls.  Bodies 0-4 are the
+    bin (floor first).  Spheres are analytic or the reference's polyhedral
+    mesh."""
+    rng = np.random.default_rng(seed)
+    count = side ** 3
+    extent = side * spacing
+    half_bin = extent / 2.0 + 0.4
+    wall_h = extent + 2.0
+    shapes = [("mesh", *box_mesh(half_bin + 0.5, 0.5, half_bin + 0.5)),
+              ("mesh", *box_mesh(0.5, wall_h / 2, half_bin + 0.5)),
+              ("mesh", *box_mesh(half_bin + 0.5, wall_h / 2, 0.5))]
+    bin_pos = [(0.0, -0.5, 0.0), (-half_bin - 0.5, wall_h / 2, 0.0),
+               (half_bin + 0.5, wall_h / 2, 0.0), (0.0, wall_h / 2, -half_bin - 0.5),
+               (0.0, wall_h / 2, half_bin + 0.5)]
+    bin_shape = [0, 1, 1, 2, 2]
+    g = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side),
+                             indexing="ij"), -1).reshape(-1, 3)[:, ::-1]  # x fastest
+    pos = ((g - (side - 1) / 2.0) * spacing).astype(np.float32)
+    pos[:, 1] += np.float32(extent / 2.0 + height)
+    pos += rng.uniform(-jitter, jitter, size=pos.shape).astype(np.float32)
+    is_box = rng.permutation(np.arange(count) % 2 == 0)
+    density = np.exp(rng.uniform(np.log(200.0), np.log(5000.0), size=count)).astype(np.float32)
+    friction = rng.uniform(0.3, 0.8, size=count).astype(np.float32)
+    edges = rng.uniform(0.2, 0.6, size=(count, 3)).astype(np.float32)
+    radii = rng.uniform(0.1, 0.3, size=count).astype(np.float32)
+    body_shape = np.zeros(count, dtype=np.uint32)
+    mass = np.zeros(count, dtype=np.float32)
+    inertia = np.zeros((count, 3), dtype=np.float32)
+    sphere_mesh_cache = {}
+    for i in range(count):
+        if is_box[i]:
+            h = edges[i] / np.float32(2)
+            shapes.append(("mesh", *box_mesh(*h)))
+            m = density[i] * edges[i].prod(dtype=np.float32)
+            mass[i] = m
+            inertia[i] = box_inertia(m, h)
+        else:
+            r = radii[i]
+            if analytic_spheres:
+                shapes.append(("sphere", float(r)))
+            else:
+                shapes.append(("mesh", *cube_sphere_mesh(r)))
+            m = density[i] * np.float32(4.0 / 3.0 * np.pi) * r * r * r
+            mass[i] = m
+            inertia[i] = np.float32(0.4) * m * r * r
+        body_shape[i] = len(shapes) - 1
+    n = count + 5
+    motion = np.full(n, L.MOTION_DYNAMIC, dtype=np.uint32)
+    motion[:5] = L.MOTION_STATIC
+    all_mass = np.concatenate([np.ones(5, np.float32), mass])
+    all_inertia = np.concatenate([np.ones((5, 3), np.float32), inertia])
+    all_mu = np.concatenate([np.full(5, 0.5, np.float32), friction])
+    p = params(n, motion=motion, mass=all_mass, inertia=all_inertia, friction=all_mu,
+               margin=margin, shape=np.concatenate([np.array(bin_shape, np.uint32), body_shape]))
+    s = states(n, np.concatenate([np.array(bin_pos, np.float32), pos]))
+    return Scene(shapes, p, s, f"pile{side}")
+
+
+# ---------------------------------------------------------------------------
+# Kernel-level fixtures of tests/solver/rigid_realtime_cuda_tests.cu
+# ---------------------------------------------------------------------------
+def realtime_fixture():
+    """:43-155 without the fixed joint (joints are not on this build's
+    path): static floor 0; resting penetration 1; cold impact 2; 100:1 mass
+    ratio 3-4; free body 5; fast rotation 6; teleport 7."""
+    n = 8
+    p = np.zeros(n, dtype=L.PARAMS)
+    prev = states(n)
+    for i in range(n):
+        p[i]["motion"] = L.MOTION_STATIC if i == 0 else L.MOTION_DYNAMIC
+        p[i]["inverse_mass"] = 0.0 if i == 0 else 1.0
+        p[i]["inverse_inertia_local"] = (0, 0, 0) if i == 0 else (1, 1, 1)
+        p[i]["friction"] = 0.5
+        p[i]["restitution"] = 0.5
+        p[i]["collision_margin"] = 0.001
+    cur = prev.copy()
+    m = np.zeros(3, dtype=L.MANIFOLD)
+    pairs = np.zeros(3, dtype=np.uint32)
+    prev[1]["position"][1] = 0.04
+    cur[1] = prev[1]
+    cur[1]["position"][1] = 0.0395
+    cur[1]["linear_velocity"][1] = -0.03
+    pairs[0] = 1 * n
+    m[0]["count"] = 1
+    m[0]["contacts"][0]["normal"] = (0, 1, 0)
+    m[0]["contacts"][0]["penetration"] = 0.01
+    prev[2]["position"][1] = 0.1
+    prev[2]["linear_velocity"][1] = -3.0
+    cur[2] = prev[2]
+    cur[2]["position"][1] = 0.05
+    pairs[1] = 2 * n
+    m[1]["count"] = 1
+    m[1]["contacts"][0]["normal"] = (0, 1, 0)
+    m[1]["contacts"][0]["penetration"] = 0.002
+    p[4]["inverse_mass"] = 0.01
+    p[4]["inverse_inertia_local"] = (0.01, 0.01, 0.01)
+    prev[3]["position"] = (-0.002, 1.0, 0.0)
+    prev[4]["position"] = (0.002, 1.0, 0.0)
+    cur[3], cur[4] = prev[3], prev[4]
+    pairs[2] = 3 * n + 4
+    m[2]["count"] = 1
+    m[2]["contacts"][0]["normal"] = (-1, 0, 0)
+    m[2]["contacts"][0]["point"] = (0, 1, 0)
+    m[2]["contacts"][0]["penetration"] = 0.01
+    cur[5] = prev[5]
+    cur[5]["position"][0] = 0.02
+    half = np.float32(0.05)
+    cur[5]["orientation"] = (0, 0, np.sin(half), np.cos(half))
+    half = np.float32(0.25)
+    prev[6]["angular_velocity"][2] = 30.0
+    cur[6] = prev[6]
+    cur[6]["orientation"] = (0, 0, np.sin(half), np.cos(half))
+    prev[7]["position"][0] = -5.0
+    prev[7]["linear_velocity"] = (3, 2, 1)
+    prev[7]["angular_velocity"] = (1, 2, 3)
+    cur[7] = prev[7]
+    cur[7]["position"][0] = 5.0
+    cur[7]["linear_velocity"] = 0
+    cur[7]["angular_velocity"] = 0
+    return p, prev, cur, m, pairs
+
+
+def check_realtime_fixture(cur, start, schedule):
+    """Gates of rigid_realtime_cuda_tests.cu:424-477 (joint gates omitted).
+    `start` is the state array before the solve."""
+    assert np.all(np.isfinite(cur.view(np.float32)))
+    assert schedule["offsets"][0] == 1 and schedule["budget"] == 8
+    assert schedule["counts"][6] == 8 and schedule["counts"][7] == 0
+    assert schedule["maximum_passes"] == 8
+    correction = cur[1]["position"][1] - start[1]["position"][1]
+    assert max(0.0, 0.01 - correction) <= 0.005
+    assert abs(correction) <= 0.01
+    assert cur[2]["linear_velocity"][1] >= -0.05
+    impact_energy = 0.5 * 3.0 * 3.0
+    assert 0.5 * cur[2]["linear_velocity"][1] ** 2 <= 1.05 * impact_energy
+    rel = ((cur[3]["position"][0] - start[3]["position"][0]) -
+           (cur[4]["position"][0] - start[4]["position"][0]))
+    assert max(0.0, 0.01 + rel) <= 0.005
+    assert abs(cur[6]["angular_velocity"][2]) <= 30.0
+    assert abs(cur[7]["position"][0] - 5.0) < 1e-6
+    assert np.all(cur[7]["linear_velocity"] == 0) and np.all(cur[7]["angular_velocity"] == 0)
+
+
+def stack_window(prepare_solve, ticks_cold=120, ticks_settled=600):
+    """:172-371: a two-body stack on a static floor for 720 ticks with
+    externally integrated gravity and hand-built manifolds.  Returns the
+    measured metrics; `prepare_solve(params, previous, states, manifolds,
+    pairs, bodies)` runs one AVBD step in place and returns the schedule."""
+    n = 3
+    p = np.zeros(n, dtype=L.PARAMS)
+    s = states(n)
+    for i in range(n):
+        p[i]["motion"] = L.MOTION_STATIC if i == 0 else L.MOTION_DYNAMIC
+        p[i]["inverse_mass"] = 0.0 if i == 0 else 1.0
+        p[i]["inverse_inertia_local"] = (0, 0, 0) if i == 0 else (1, 1, 1)
+        p[i]["collision_margin"] = 0.001
+    s[1]["position"][1] = 0.05
+    s[2]["position"][1] = 0.149
+    m = np.zeros(2, dtype=L.MANIFOLD)
+    pairs = np.array([n, 2 * n + 1], dtype=np.uint32)
+    bodies = np.zeros(n, dtype=L.AVBD_BODY)
+    half, g, dt = np.float32(0.05), np.float32(-9.81), DT
+    out = dict(max_pen=0.0, drift=0.0, rot=0.0, energy_growth=0.0, tunnel=0)
+    y1 = [None, None]
+    y2 = [None, None]
+    settled = peak = None
+
+    def set_contact(k, pen, point):
+        m[k]["count"] = 1
+        c = m[k]["contacts"][0]
+        c["normal"] = (0, 1, 0)
+        c["point"] = point
+        c["penetration"] = max(np.float32(0), pen)
+
+    for tick in range(ticks_cold + ticks_settled):
+        prev = s.copy()
+        for b in (1, 2):
+            s[b]["linear_velocity"][1] = np.float32(s[b]["linear_velocity"][1] + g * dt)
+            s[b]["position"][1] = np.float32(s[b]["position"][1] + s[b]["linear_velocity"][1] * dt)
+        set_contact(0, half - s[1]["position"][1], (0, 0, 0))
+        y_mid = np.float32(0.5) * (s[1]["position"][1] + half + s[2]["position"][1] - half)
+        set_contact(1, s[1]["position"][1] + np.float32(2) * half - s[2]["position"][1], (0, y_mid, 0))
+        sched = prepare_solve(p, prev, s, m, pairs, bodies)
+        assert sched["budget"] == 8 and sched["counts"][6] == 8 and sched["counts"][7] == 0
+        m["contacts"][:, 0]["avbd_cached"] = 1
+        pen = max(0.0, half - s[1]["position"][1],
+                  s[1]["position"][1] + 2 * half - s[2]["position"][1])
+        out["max_pen"] = max(out["max_pen"], float(pen))
+        for b in (1, 2):
+            q = s[b]["orientation"]
+            ang = 2.0 * np.arctan2(np.sqrt(q[0] ** 2 + q[1] ** 2 + q[2] ** 2), abs(q[3]))
+            out["rot"] = max(out["rot"], float(ang))
+        if s[1]["position"][1] < -half or s[2]["position"][1] <= s[1]["position"][1]:
+            out["tunnel"] += 1
+        if tick >= ticks_cold:
+            e = 0.5 * (s[1]["linear_velocity"][1] ** 2 + s[2]["linear_velocity"][1] ** 2) - \
+                g * (s[1]["position"][1] + s[2]["position"][1])
+            a, b_ = float(s[1]["position"][1]), float(s[2]["position"][1])
+            if tick == ticks_cold:
+                y1 = [a, a]
+                y2 = [b_, b_]
+                settled = peak = float(e)
+            else:
+                y1 = [min(y1[0], a), max(y1[1], a)]
+                y2 = [min(y2[0], b_), max(y2[1], b_)]
+                peak = max(peak, float(e))
+    out["drift"] = max(y1[1] - y1[0], y2[1] - y2[0])
+    out["energy_growth"] = peak / settled - 1.0 if settled > 0 else 0.0
+    return out
+
+
+def check_stack_window(out):
+    """Gates of rigid_realtime_cuda_tests.cu:365-370."""
+    assert out["max_pen"] <= 0.005
+    assert out["drift"] <= 0.01
+    assert out["rot"] <= 0.035
+    assert out["energy_growth"] <= 0.05
+    assert out["tunnel"] == 0
