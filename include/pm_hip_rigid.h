@@ -1,0 +1,251 @@
+/* SPDX-License-Identifier: MIT
+ *
+ * MI355X (gfx950) HIP driver for the ParallelMater rigid-body step:
+ * broadphase, convex/analytic narrowphase and the Augmented Vertex Block
+ * Descent (AVBD) contact solver.
+ *
+ * C ABI only: plain pointers, sizes and the reference's record layouts.
+ * Two levels are exported:
+ *   - the world API, which is what the reference's rigid pm* command path
+ *     drives (pmCmdBuildGeometry, pmCmdWriteRigidBodies,
+ *     pmCmdStepSimulation, pmCmdCopyRigidStates ...);
+ *   - the kernel seam, which stands where the reference's dispatch
+ *     wrappers stand (src/simulation/geometry_constraints_dispatch.hpp:15-43)
+ *     and takes device pointers to the reference's own record layouts.
+ * Every call returns a PmhResult whose values equal the reference's
+ * PmResult (include/parallel_mater/parallel_mater.h:93-110).
+ */
+#ifndef PM_HIP_RIGID_H
+#define PM_HIP_RIGID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t PmhResult;
+#define PMH_SUCCESS 0
+#define PMH_ERROR_INVALID_ARGUMENT (-1)
+#define PMH_ERROR_INVALID_STATE (-2)
+#define PMH_ERROR_OUT_OF_HOST_MEMORY (-3)
+#define PMH_ERROR_OUT_OF_DEVICE_MEMORY (-4)
+#define PMH_ERROR_FEATURE_NOT_PRESENT (-5)
+#define PMH_ERROR_CAPACITY_EXCEEDED (-7)
+#define PMH_ERROR_DEVICE_LOST (-8)
+
+/* ---- records (reference layouts, byte for byte) ---------------------- */
+typedef struct PmhVec3 { float x, y, z; } PmhVec3;
+typedef struct PmhQuat { float x, y, z, w; } PmhQuat;
+
+/* RigidBodyState (src/slang/types.slang:97-103), 52 bytes. */
+typedef struct PmhRigidBodyState {
+    PmhVec3 position;
+    PmhQuat orientation;
+    PmhVec3 linear_velocity;
+    PmhVec3 angular_velocity;
+} PmhRigidBodyState;
+
+#define PMH_MOTION_STATIC 0u
+#define PMH_MOTION_KINEMATIC 1u
+#define PMH_MOTION_DYNAMIC 2u
+
+/* BodyParameters (types.slang:81-95), 64 bytes. mesh_index selects the
+ * shape returned by pmhWorldAdd{Mesh,Sphere,Capsule}. */
+typedef struct PmhBodyParameters {
+    uint32_t motion;
+    uint32_t mesh_index;
+    uint32_t mesh_generation;
+    float inverse_mass;
+    PmhVec3 inverse_inertia_local;
+    float friction;
+    float restitution;
+    float linear_damping;
+    float angular_damping;
+    float maximum_linear_speed;
+    float maximum_angular_speed;
+    float collision_margin;
+    uint64_t user_data;
+} PmhBodyParameters;
+
+/* HingeContactFrame (geometry.slang:143-157), 72 bytes. */
+typedef struct PmhHingeContactFrame {
+    PmhVec3 anchor, axis, local_anchor;
+    float inverse_mass, inverse_moment;
+    uint32_t present, fixed, axial, axial_rotation, fixed_member,
+        point_member, static_body;
+} PmhHingeContactFrame;
+
+typedef struct PmhAvbdDual { float lambda, penalty; } PmhAvbdDual;
+
+/* Contact (geometry.slang:159-179), 284 bytes. */
+typedef struct PmhContact {
+    PmhVec3 normal;
+    PmhVec3 point;
+    float penetration;
+    PmhHingeContactFrame body_hinge;
+    PmhHingeContactFrame collider_hinge;
+    float accumulated_normal_impulse;
+    PmhVec3 accumulated_friction_impulse;
+    float initial_normal_speed;
+    uint32_t warm_started;
+    uint32_t persistent;
+    float impact_fraction;
+    float reported_normal_impulse;
+    PmhVec3 reported_friction_impulse;
+    PmhAvbdDual avbd_dual[3];
+    PmhVec3 avbd_error;
+    PmhVec3 avbd_anchor_a;
+    PmhVec3 avbd_anchor_b;
+    uint32_t avbd_cached;
+} PmhContact;
+
+/* ContactManifold (geometry.slang:181-190), 2304 bytes. */
+typedef struct PmhContactManifold {
+    PmhContact contacts[8];
+    uint32_t count;
+    PmhVec3 initial_relative_position;
+    uint32_t face_patch;
+    uint32_t response_slot;
+    uint32_t avbd_next_a;
+    uint32_t avbd_next_b;
+} PmhContactManifold;
+
+/* AvbdBody / GeometryConstraintBody (geometry_constraints.slang:42-55),
+ * 236 bytes. */
+typedef struct PmhAvbdBody {
+    PmhRigidBodyState target;
+    PmhQuat orientation_delta;
+    float displacement[6];
+    float inertia_h[21];
+    float inertia_g[6];
+    uint32_t contact_head, joint_head, color, parent, impact;
+    PmhVec3 previous_velocity;
+    uint32_t history_valid;
+} PmhAvbdBody;
+
+/* ContactSchedule (geometry.slang:226-238), 264 bytes. */
+typedef struct PmhContactSchedule {
+    uint32_t counts[29];
+    uint32_t offsets[29];
+    uint32_t budget, remaining, island_count, early_exit_count,
+        maximum_passes, blocks, candidates, contacts;
+} PmhContactSchedule;
+
+/* ---- world API ---------------------------------------------------------
+ * One world = one PmSimulation's rigid domain on one GPU.  Bodies are
+ * dense; identities are {index = dense, generation = 1}. */
+typedef struct PmhWorld_T *PmhWorld;
+
+typedef struct PmhWorldDesc {
+    float gravity[3];
+    float timestep;                  /* dt; realtime v1 requires 1/60 */
+    uint32_t profile_version;        /* 1: realtime v1 (8 sweeps); 0: legacy */
+    uint32_t base_iterations;        /* legacy profile iteration budget */
+    uint32_t requested_iterations;   /* legacy override, 0 = base */
+    uint32_t body_capacity;          /* contact-cache slot rule; 0 = bodies */
+    uint32_t pairs_per_body;         /* broadphase capacity, 0 = 32 */
+    uint32_t points_per_pair;        /* average contact capacity, 0 = 4 */
+    uint32_t device;                 /* HIP device ordinal */
+    uint32_t flags;                  /* PMH_WORLD_* */
+} PmhWorldDesc;
+
+#define PMH_WORLD_STAGED_SOLVE 1u     /* one launch per colour (no grid barrier) */
+
+typedef struct PmhWorldStats {
+    uint32_t pairs, manifolds, points, colors, islands, impact, budget,
+        unsupported_pairs, cache_hits, phases;
+} PmhWorldStats;
+
+/* Phase timings of the last step (milliseconds, HIP events on the
+ * world's stream): integrate+proxies, broadphase, narrowphase, contact
+ * cache, prepare, solve, finish. */
+#define PMH_PHASE_COUNT 7
+typedef struct PmhPhaseTimes { float ms[PMH_PHASE_COUNT]; } PmhPhaseTimes;
+
+/* pmCreateSimulation + pmBindSimulationMemory (src/api/resources.cpp:535,
+ * :700) for the rigid domain. */
+PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *world);
+void pmhDestroyWorld(PmhWorld world);
+
+/* pmCmdBuildGeometry -> SimulationRuntime::add_triangle_mesh
+ * (src/api/resources.cpp:523, src/simulation/runtime_deformables.cu.inc:
+ * 4-304): validation, median-split BVH triangle order, convex face
+ * planes, bounds, bounding sphere and unit inertia. */
+PmhResult pmhWorldAddMesh(PmhWorld world, const float *vertices,
+                          uint32_t vertex_count, const uint32_t *indices,
+                          uint32_t index_count, uint32_t *shape);
+/* Analytic sphere and capsule (radius, half segment length along local y).
+ * No reference counterpart: the reference models every shape as a mesh. */
+PmhResult pmhWorldAddSphere(PmhWorld world, float radius, uint32_t *shape);
+PmhResult pmhWorldAddCapsule(PmhWorld world, float radius, float half_length,
+                             uint32_t *shape);
+/* Shape metadata: min xyz, max xyz, bounding centre xyz, bounding radius,
+ * unit inertia xyz, kind, index count, unique plane count. */
+PmhResult pmhWorldGetShapeInfo(PmhWorld world, uint32_t shape, float out[16]);
+
+/* pmCmdWriteRigidBodies -> add_rigid_body (src/api/resources.cpp:821,
+ * src/simulation/runtime_rigid.cu.inc:4-68), host arrays. */
+PmhResult pmhWorldAddBodies(PmhWorld world, uint32_t count,
+                            const PmhBodyParameters *parameters,
+                            const PmhRigidBodyState *states);
+uint32_t pmhWorldBodyCount(PmhWorld world);
+
+/* pmCmdStepSimulation (src/api/resources.cpp:1230-1288) -> step_async ->
+ * the production rigid order: predict, proxies, broadphase, narrowphase,
+ * contact cache, AVBD prepare and solve, speed clamp, cache save
+ * (production_realtime_backend.cu.inc:2642-2687, 7664-7861).  Enqueues
+ * `steps` steps and waits for them. */
+PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
+
+/* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;
+ * runtime_rigid.cu.inc:147-195: a set-state resets AVBD history and the
+ * contact cache). */
+PmhResult pmhWorldCopyStates(PmhWorld world, PmhRigidBodyState *states);
+PmhResult pmhWorldSetStates(PmhWorld world, const PmhRigidBodyState *states);
+
+PmhResult pmhWorldGetStats(PmhWorld world, PmhWorldStats *stats);
+PmhResult pmhWorldGetPhaseTimes(PmhWorld world, PmhPhaseTimes *times);
+/* Kernel-duration probe for the dominant solve kernel: total milliseconds
+ * and launch count of the solve phase since the last reset. */
+PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds,
+                              uint32_t *launches, int reset);
+
+/* Results of the last step in the reference layouts (for parity checks
+ * and pmCmdCopyContactEvents-style consumers). */
+uint32_t pmhWorldPairCount(PmhWorld world);
+PmhResult pmhWorldGetPairs(PmhWorld world, uint32_t *codes);
+PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *manifolds);
+PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *bodies);
+PmhResult pmhWorldGetSchedule(PmhWorld world, PmhContactSchedule *schedule);
+PmhResult pmhWorldGetProxies(PmhWorld world, float *bounds);
+
+/* ---- kernel seam --------------------------------------------------------
+ * prepare_geometry_constraints + solve_geometry_constraints
+ * (src/simulation/geometry_constraints_dispatch.hpp:15-43): all pointers
+ * are device pointers in the reference layouts; `active_pair_count`
+ * points to one device uint32.  `bodies` carries the per-body history
+ * between calls exactly as the reference's avbd_bodies buffer.  Runs on
+ * `stream` (a hipStream_t, NULL for the default stream). */
+PmhResult pmhPrepareSolveGeometryConstraints(
+    void *stream, const PmhBodyParameters *parameters,
+    const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds,
+    const uint32_t *active_pairs, const uint32_t *active_pair_count,
+    PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations,
+    uint32_t requested_iterations, uint32_t profile_version);
+
+/* Device memory helpers for callers without a HIP runtime of their own. */
+PmhResult pmhDeviceAlloc(uint64_t bytes, void **pointer);
+PmhResult pmhDeviceFree(void *pointer);
+PmhResult pmhCopyToDevice(void *device, const void *host, uint64_t bytes);
+PmhResult pmhCopyToHost(void *host, const void *device, uint64_t bytes);
+PmhResult pmhDeviceSynchronize(void);
+/* gfx target name of the current device ("gfx950") and CU count. */
+PmhResult pmhDeviceInfo(char *name, uint32_t name_bytes, uint32_t *compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1219 @@
+// SPDX-License-Identifier: MIT
+// AVBD prepare and solve on gfx950.
+//
+// Prepare (prepare_avbd_kernel, geometry_constraints.slang:1213-1591) runs
+// as data-parallel kernels: per-body inertia and targets, per-contact
+// anchors/errors/duals, adjacency lists in the reference's list order
+// (reverse active order), islands by lock-free hooking onto the smaller
+// root (same min-index roots as the serial union-find), and first-fit
+// colouring in index order as a dataflow over index-ordered tickets: a
+// body takes the smallest colour its lower-index neighbours do not use,
+// exactly the reference's serial greedy result.
+//
+// Solve (solve_avbd_kernel, geometry_constraints_solve.slang:771-865) is
+// one cooperative launch: budget x (colours + dual pass) separated by grid
+// barriers.  Each body of the active colour is owned by a group of L lanes
+// of a wavefront; the lanes gather the body's contact rows from a
+// colour-ordered CSR list, each accumulating a partial 6x6 block, and the
+// partial blocks meet in an xor-butterfly over the group.  The block is
+// then solved in registers by an unrolled LDL^T.  Per-point Jacobians are
+// computed once per phase (they only depend on the target poses).
+#include "pmh_internal.hpp"
+
+#include <hip/hip_cooperative_groups.h>
+
+#include <algorithm>
+#include <cstring>
+
+namespace cg = cooperative_groups;
+
+namespace pmh {
+
+constexpr uint32_t kRealtimeProfile = 1u;
+constexpr uint32_t kRealtimeSweeps = 8u;
+constexpr float kRealtimeDamping = 0.05f;
+constexpr float kSurfaceTolerance = 1.0e-5f;
+constexpr float kRestOffset = 1.0e-3f;
+constexpr uint32_t kSideFirst = 0x80000000u;
+constexpr int kLanes = 8;   // lanes per body in the colour solve
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ V3 v3(float4 f) { return {f.x, f.y, f.z}; }
+__device__ __forceinline__ Q q4(float4 f) { return {f.x, f.y, f.z, f.w}; }
+__device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+__device__ __forceinline__ float4 f4q(Q q) { return make_float4(q.x, q.y, q.z, q.w); }
+
+__device__ __forceinline__ V3 sv(const PmhVec3 &v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ Q sq(const PmhQuat &q) { return {q.x, q.y, q.z, q.w}; }
+__device__ __forceinline__ PmhVec3 pv(V3 v) { return {v.x, v.y, v.z}; }
+__device__ __forceinline__ PmhQuat pq(Q q) { return {q.x, q.y, q.z, q.w}; }
+
+__device__ __forceinline__ float pair_stiffness(const PmhBodyParameters &a, const PmhBodyParameters &b, float dt) {
+    return 1.0f / (smax(a.inverse_mass + b.inverse_mass, 1.0e-6f) * dt * dt);
+}
+__device__ __forceinline__ float contact_skin(const PmhBodyParameters &a, const PmhBodyParameters &b) {
+    return smin(a.collision_margin + b.collision_margin, kRestOffset) + 2.0f * kSurfaceTolerance;
+}
+
+// ---------------------------------------------------------------------
+// Device-wide exclusive scan (three passes, 1024 elements per block).
+// ---------------------------------------------------------------------
+constexpr uint32_t kScanBlock = 256, kScanItems = 4, kScanTile = kScanBlock * kScanItems;
+
+__device__ uint32_t block_exclusive(uint32_t v, uint32_t *total) {
+    __shared__ uint32_t warp_sums[kScanBlock / 64];
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63) warp_sums[wid] = x;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (uint32_t w = 0; w < kScanBlock / 64; ++w) {
+        if (w < wid) base += warp_sums[w];
+        all += warp_sums[w];
+    }
+    __syncthreads();
+    if (total) *total = all;
+    return base + x - v;
+}
+
+__global__ void k_scan_reduce(const uint32_t *in, uint32_t count, const uint32_t *count_ptr, uint32_t *sums) {
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    const uint32_t base = blockIdx.x * kScanTile;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint32_t i = base + k * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t total;
+    block_exclusive(s, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_sums(uint32_t *sums, uint32_t blocks, uint32_t *total_out) {
+    // One block scans up to blocks <= 1024*? sums sequentially in tiles.
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t t = 0; t < blocks; t += kScanBlock) {
+        uint32_t i = t + threadIdx.x;
+        uint32_t v = i < blocks ? sums[i] : 0;
+        uint32_t total;
+        uint32_t ex = block_exclusive(v, &total);
+        if (i < blocks) sums[i] = carry + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t *in, uint32_t *out, uint32_t count, const uint32_t *count_ptr,
+                             const uint32_t *sums) {
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], s = 0;
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint32_t i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_exclusive(s, &total) + sums[blockIdx.x];
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        uint32_t i = base + k;
+        if (i < n) out[i] = ex;
+        ex += v[k];
+    }
+}
+
+__global__ void k_scan_tail(const uint32_t *in, uint32_t *out, uint32_t count, const uint32_t *count_ptr) {
+    const uint32_t n = count_ptr ? *count_ptr : count;
+    out[n] = n ? out[n - 1] + in[n - 1] : 0;
+}
+
+PmhResult exclusive_scan(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t count,
+                         const uint32_t *count_ptr, uint32_t *total, DBuf<uint32_t> &tmp) {
+    // `count` is the capacity bound when count_ptr is given.
+    const uint32_t blocks = std::max<uint32_t>(1, (count + kScanTile - 1) / kScanTile);
+    if (tmp.reserve(blocks) != PMH_SUCCESS) return PMH_ERROR_OUT_OF_DEVICE_MEMORY;
+    k_scan_reduce<<<blocks, kScanBlock, 0, s>>>(in, count, count_ptr, tmp.p);
+    k_scan_sums<<<1, kScanBlock, 0, s>>>(tmp.p, blocks, total);
+    k_scan_apply<<<blocks, kScanBlock, 0, s>>>(in, out, count, count_ptr, tmp.p);
+    k_scan_tail<<<1, 1, 0, s>>>(in, out, count, count_ptr);
+    PMH_CHECK(hipGetLastError());
+    return PMH_SUCCESS;
+}
+
+// ---------------------------------------------------------------------
+// Prepare: bodies (geometry_constraints.slang:1235-1295).
+// ---------------------------------------------------------------------
+__global__ void k_body_init(uint32_t n, const PmhBodyParameters *P, const PmhRigidBodyState *prev,
+                            const PmhRigidBodyState *st, float dt, V3 g, float4 *tpos, float4 *tq,
+                            float4 *tv, float4 *tw, float4 *disp, float4 *dq, float4 *inertia,
+                            float4 *prev_vel, uint32_t *color, uint32_t *parent, uint32_t *impact,
+                            uint32_t *deg, uint32_t *anchored, uint32_t *adj_cursor) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PmhRigidBodyState s = st[i];
+    tpos[i] = f4(sv(s.position), 0.0f);
+    tq[i] = f4q(sq(s.orientation));
+    tv[i] = f4(sv(s.linear_velocity), 0.0f);
+    tw[i] = f4(sv(s.angular_velocity), 0.0f);
+    dq[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    color[i] = kInvalid;
+    parent[i] = i;
+    impact[i] = 0;
+    deg[i] = 0;
+    anchored[i] = 0;
+    adj_cursor[i] = 0;
+    float4 old = prev_vel[i];
+    if (P[i].motion != kDynamic) {
+        disp[2 * i] = make_float4(0, 0, 0, 0);
+        disp[2 * i + 1] = make_float4(0, 0, 0, 0);
+        inertia[2 * i] = make_float4(0, 0, 0, 0);
+        inertia[2 * i + 1] = make_float4(0, 0, 0, 0);
+        prev_vel[i] = make_float4(0, 0, 0, 0);
+        return;
+    }
+    const bool hist = old.w != 0.0f;
+    const float g2 = len2(g);
+    const V3 pv_ = sv(prev[i].linear_velocity);
+    const V3 acc = mul(sub(pv_, v3(old)), 1.0f / dt);
+    float w = hist && g2 > 0.0f ? clampf(dot(acc, g) / g2, 0.0f, 1.0f) : 0.0f;
+    if (!isfinite(w)) w = 0.0f;
+    const V3 corr = mul(g, -(1.0f - w) * dt * dt);
+    disp[2 * i] = f4(corr, 0.0f);
+    disp[2 * i + 1] = make_float4(0, 0, 0, 0);
+    prev_vel[i] = f4(pv_, 1.0f);
+    const float mass = 1.0f / smax(P[i].inverse_mass, 1.0e-12f);
+    const Q o = sq(s.orientation);
+    float rot[6];   // lower triangle of the rotational block: 33 43 44 53 54 55
+    for (uint32_t ax = 0; ax < 3; ++ax) {
+        V3 la = inv_rotate(o, basis_axis(ax));
+        V3 col = rotate(o, {la.x / smax(P[i].inverse_inertia_local.x, 1.0e-12f),
+                            la.y / smax(P[i].inverse_inertia_local.y, 1.0e-12f),
+                            la.z / smax(P[i].inverse_inertia_local.z, 1.0e-12f)});
+        for (uint32_t r = ax; r < 3; ++r) {
+            // element (3+r, 3+ax) of the packed lower triangle
+            const uint32_t k = r * (r + 1) / 2 + ax;
+            rot[k] = comp(col, r) / (dt * dt);
+        }
+    }
+    inertia[2 * i] = make_float4(mass / (dt * dt), rot[0], rot[1], rot[2]);
+    inertia[2 * i + 1] = make_float4(rot[3], rot[4], rot[5], 0.0f);
+}
+
+// Contacts (geometry_constraints.slang:1297-1375) and the per-manifold
+// impact flag (rigid_avbd_prepare_contact_impact, :983-1005).
+__global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *P, const PmhRigidBodyState *prev,
+                               const PmhRigidBodyState *st, const uint2 *ab, const uint32_t *m_count,
+                               const uint32_t *m_start, float4 *c_np, float4 *c_pt, float4 *c_aa, float4 *c_ab,
+                               float4 *c_err, float4 *c_lam, float4 *c_pen, const uint32_t *c_flags,
+                               uint32_t *m_impact, float dt, uint32_t profile) {
+    const uint32_t M = *npairs;
+    const bool realtime = profile == kRealtimeProfile;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint2 p = ab[m];
+        const PmhBodyParameters pa = P[p.x], pb = P[p.y];
+        const float penalty = pair_stiffness(pa, pb, dt);
+        const PmhRigidBodyState A0 = prev[p.x], B0 = prev[p.y], A = st[p.x], B = st[p.y];
+        const float skin = contact_skin(pa, pb);
+        bool impact = false;
+        const uint32_t start = m_start[m], count = m_count[m];
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t c = start + k;
+            const bool cached = (c_flags[c] & 1u) != 0u;
+            float4 np = c_np[c], pt = c_pt[c];
+            const V3 normal = v3(np), point = v3(pt);
+            V3 aa = v3(c_aa[c]), bb = v3(c_ab[c]);
+            if (!cached) {
+                aa = inv_rotate(sq(A0.orientation), sub(point, sv(A0.position)));
+                bb = inv_rotate(sq(B0.orientation), sub(point, sv(B0.position)));
+            }
+            const V3 ma = sub(sv(A.position), sv(A0.position));
+            const V3 mb = sub(sv(B.position), sv(B0.position));
+            const V3 wa = mul(quaternion_delta_velocity(sq(A0.orientation), sq(A.orientation), dt), dt);
+            const V3 wb = mul(quaternion_delta_velocity(sq(B0.orientation), sq(B.orientation), dt), dt);
+            const V3 armA = sub(point, sv(A.position)), armB = sub(point, sv(B.position));
+            const float closing = dot(normal, sub(add(mb, cross(wb, armB)), add(ma, cross(wa, armA))));
+            const float old_pen = np.w - closing;
+            V3 t0, t1;
+            contact_basis(normal, t0, t1);
+            const V3 motion = add(sub(mb, ma), sub(rotation_displacement(sq(B0.orientation), sq(B.orientation), bb),
+                                                   rotation_displacement(sq(A0.orientation), sq(A.orientation), aa)));
+            const V3 old_delta = !cached ? V3{0.0f, 0.0f, 0.0f}
+                : add(sub(sv(B0.position), sv(A0.position)),
+                      sub(rotate(sq(B0.orientation), bb), rotate(sq(A0.orientation), aa)));
+            const V3 terr = add(motion, mul(old_delta, 1.0f - kAlpha));
+            const float speed = -closing / dt;
+            const float impact_disp = realtime && !cached && speed < -0.1f
+                ? -(1.0f + smin(pa.restitution, pb.restitution)) * speed * dt : 0.0f;
+            const V3 err = {np.w - clampf(old_pen, 0.0f, skin) + impact_disp, dot(terr, t0), dot(terr, t1)};
+            float4 lam = c_lam[c], pen = c_pen[c];
+            float l[3] = {lam.x, lam.y, lam.z}, k3[3] = {pen.x, pen.y, pen.z};
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!cached) { l[ax] = 0.0f; k3[ax] = penalty; }
+                warm_start(l[ax], k3[ax], kMaxPenalty);
+            }
+            c_lam[c] = make_float4(l[0], l[1], l[2], 0.0f);
+            c_pen[c] = make_float4(k3[0], k3[1], k3[2], 0.0f);
+            c_aa[c] = f4(aa, c_aa[c].w);
+            c_ab[c] = f4(bb, 0.0f);
+            c_err[c] = f4(err, 0.0f);
+            c_pt[c] = f4(point, speed);
+            // Impact test on the updated contact.
+            const float impact_fraction = c_aa[c].w;
+            const bool fresh = (!cached || l[0] <= 1.0e-6f || (impact_fraction > 0.0f && impact_fraction < 1.0f)) &&
+                               speed < -0.1f && np.w >= -kSurfaceTolerance;
+            const float prev_pen = np.w + speed * dt;
+            impact = impact || fresh || prev_pen > skin + 1.0e-4f;
+        }
+        m_impact[m] = impact ? 1u : 0u;
+    }
+}
+
+// ---------------------------------------------------------------------
+// Graph: adjacency (dynamic endpoints only), islands, impact, colouring.
+// ---------------------------------------------------------------------
+__global__ void k_degree(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
+                         const uint32_t *m_count, uint32_t *deg, uint32_t *anchored_edge, StepCounters *ctr) {
+    const uint32_t M = *npairs;
+    uint32_t local = 0;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        if (m_count[m] == 0) continue;
+        ++local;
+        const uint2 p = ab[m];
+        if (P[p.x].motion == kDynamic) atomicAdd(&deg[p.x], 1u);
+        if (P[p.y].motion == kDynamic) atomicAdd(&deg[p.y], 1u);
+    }
+    if (local) atomicAdd(&ctr->manifolds, local);
+    (void)anchored_edge;
+}
+
+__global__ void k_adj_fill(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
+                           const uint32_t *m_count, const uint32_t *adj_start, uint32_t *cursor, uint32_t *adj) {
+    const uint32_t M = *npairs;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        if (m_count[m] == 0) continue;
+        const uint2 p = ab[m];
+        if (P[p.x].motion == kDynamic) adj[adj_start[p.x] + atomicAdd(&cursor[p.x], 1u)] = m;
+        if (P[p.y].motion == kDynamic) adj[adj_start[p.y] + atomicAdd(&cursor[p.y], 1u)] = m;
+    }
+}
+
+// Reference list order: contacts are pushed to the front in active order,
+// so each body's list is in descending active index.
+__global__ void k_adj_sort(uint32_t n, const uint32_t *adj_start, uint32_t *adj) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = adj_start[i], e = adj_start[i + 1];
+    for (uint32_t k = b + 1; k < e; ++k) {
+        uint32_t v = adj[k];
+        uint32_t j = k;
+        while (j > b && adj[j - 1] < v) { adj[j] = adj[j - 1]; --j; }
+        adj[j] = v;
+    }
+}
+
+__device__ uint32_t uf_find(uint32_t *parent, uint32_t v) {
+    uint32_t p = ld_relaxed(&parent[v]);
+    while (p != v) { v = p; p = ld_relaxed(&parent[v]); }
+    return v;
+}
+
+// Hook the larger root under the smaller one (parents only decrease), so
+// every component's final root is its minimum index, as the serial
+// union-find of geometry_constraints.slang:1178-1189 produces.
+__global__ void k_uf_hook(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
+                          const uint32_t *m_count, uint32_t *parent) {
+    const uint32_t M = *npairs;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        if (m_count[m] == 0) continue;
+        const uint2 p = ab[m];
+        if (P[p.x].motion != kDynamic || P[p.y].motion != kDynamic) continue;
+        uint32_t u = uf_find(parent, p.x), v = uf_find(parent, p.y);
+        while (u != v) {
+            if (u > v) { uint32_t t = u; u = v; v = t; }
+            uint32_t old = atomicCAS(&parent[v], v, u);
+            if (old == v) break;
+            u = uf_find(parent, u);
+            v = uf_find(parent, old);
+        }
+    }
+}
+
+__global__ void k_uf_compress(uint32_t n, uint32_t *parent) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t r = uf_find(parent, i);
+    st_relaxed(&parent[i], r);
+}
+
+__global__ void k_impact_roots(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
+                               const uint32_t *m_impact, const uint32_t *m_count, const uint32_t *parent,
+                               uint32_t *impact, uint32_t *anchored, StepCounters *ctr) {
+    const uint32_t M = *npairs;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint2 p = ab[m];
+        const bool da = P[p.x].motion == kDynamic, db = P[p.y].motion == kDynamic;
+        if (m_impact[m]) {
+            if (da) atomicOr(&impact[parent[p.x]], 1u);
+            if (db) atomicOr(&impact[parent[p.y]], 1u);
+            ctr->impact = 1u;
+        }
+        // An island touching any non-dynamic body keeps its momentum
+        // correction (free_mean's `anchored`, solve.slang:641-652).
+        if (m_count[m] != 0) {
+            if (da && !db) atomicOr(&anchored[parent[p.x]], 1u);
+            if (db && !da) atomicOr(&anchored[parent[p.y]], 1u);
+        }
+    }
+}
+
+__global__ void k_impact_bodies(uint32_t n, const PmhBodyParameters *P, const uint32_t *parent, uint32_t *impact,
+                                const uint32_t *deg, StepCounters *ctr) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (P[i].motion != kDynamic) { impact[i] = 0; return; }
+    const uint32_t r = parent[i];
+    if (r != i) impact[i] = ld_relaxed(&impact[r]);
+    if (r == i && deg[i] != 0) atomicAdd(&ctr->islands, 1u);
+}
+
+// First-fit colouring in index order.  One wavefront per body ticket;
+// tickets are issued in index order, so every lower-index neighbour was
+// taken earlier by a resident wavefront and eventually publishes its
+// colour: the wait chain always ends.
+__global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
+                                               const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
+                                               StepCounters *ctr) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(&ctr->color_ticket, 1u);
+        t = __shfl(t, 0, 64);
+        if (t >= n) break;
+        const uint32_t i = t;
+        if (P[i].motion != kDynamic) continue;
+        uint64_t used[kMaxColors / 64] = {0, 0, 0, 0};
+        const uint32_t b = adj_start[i], e = adj_start[i + 1];
+        for (uint32_t k = b + lane; k < e; k += 64) {
+            const uint2 p = ab[adj[k]];
+            const uint32_t j = p.x == i ? p.y : p.x;
+            if (j >= i || P[j].motion != kDynamic) continue;
+            uint32_t c;
+            uint32_t spins = 0;
+            // Bounded wait: a lower-index neighbour always finishes, but a
+            // logic error must end the kernel with an error, not hang it.
+            while ((c = ld_relaxed(&color[j])) == kInvalid && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
+            if (c == kInvalid) { atomicOr(&ctr->status, kStatusColorTimeout); continue; }
+            if (c < kMaxColors) used[c >> 6] |= 1ull << (c & 63);
+        }
+        uint32_t chosen = kMaxColors;
+#pragma unroll
+        for (uint32_t w = 0; w < kMaxColors / 64; ++w) {
+            uint64_t m = used[w];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) m |= __shfl_xor(m, o, 64);
+            if (chosen == kMaxColors && m != ~0ull) chosen = w * 64 + __builtin_ctzll(~m);
+        }
+        if (lane == 0) {
+            if (chosen >= kMaxColors) { atomicOr(&ctr->status, kStatusColorOverflow); chosen = kMaxColors - 1; }
+            st_relaxed(&color[i], chosen);
+            atomicMax(&ctr->colors, chosen + 1);
+        }
+    }
+}
+
+__global__ void k_color_count(uint32_t n, const uint32_t *color, uint32_t *count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = color[i];
+    if (c != kInvalid) atomicAdd(&count[c], 1u);
+}
+
+__global__ void k_color_scan(uint32_t *count, uint32_t *start, const StepCounters *ctr) {
+    if (threadIdx.x != 0) return;
+    uint32_t s = 0;
+    const uint32_t C = ctr->colors;
+    for (uint32_t c = 0; c <= C && c <= kMaxColors; ++c) {
+        start[c] = s;
+        if (c < C) { s += count[c]; count[c] = 0; }
+    }
+}
+
+__global__ void k_color_place(uint32_t n, const uint32_t *color, const uint32_t *start, uint32_t *cursor,
+                              uint32_t *bodies) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t c = color[i];
+    if (c == kInvalid) return;
+    bodies[start[c] + atomicAdd(&cursor[c], 1u)] = i;
+}
+
+// Per colour position: the body's contact points in list order.
+__global__ void k_bp_count(uint32_t n, const StepCounters *ctr, const uint32_t *start, const uint32_t *bodies,
+                           const uint32_t *adj_start, const uint32_t *adj, const uint32_t *m_count,
+                           uint32_t *bp_count) {
+    const uint32_t D = start[ctr->colors];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (k >= D) { bp_count[k] = 0; return; }
+    const uint32_t i = bodies[k];
+    uint32_t s = 0;
+    for (uint32_t e = adj_start[i]; e < adj_start[i + 1]; ++e) s += m_count[adj[e]];
+    bp_count[k] = s;
+}
+
+__global__ void k_bp_fill(const StepCounters *ctr, const uint32_t *start, const uint32_t *bodies,
+                          const uint32_t *adj_start, const uint32_t *adj, const uint32_t *m_count,
+                          const uint32_t *m_start, const uint2 *ab, const uint32_t *bp_start, uint2 *entries) {
+    const uint32_t D = start[ctr->colors];
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= D) return;
+    const uint32_t i = bodies[k];
+    uint32_t o = bp_start[k];
+    for (uint32_t e = adj_start[i]; e < adj_start[i + 1]; ++e) {
+        const uint32_t m = adj[e];
+        const uint2 p = ab[m];
+        const bool first = p.x == i;
+        const uint32_t other = first ? p.y : p.x;
+        for (uint32_t q = 0; q < m_count[m]; ++q)
+            entries[o++] = make_uint2((m_start[m] + q) | (first ? kSideFirst : 0u), other);
+    }
+}
+
+// Per-point rows: axes, -(r_a x axis), r_b x axis, initial error, the
+// reference forces of the velocity phase, friction and dual stiffness
+// (rigid_avbd_solve_contact_rows, geometry_constraints.slang:506-529).
+__global__ void k_jacobians(const uint32_t *npts, const PmhBodyParameters *P, const uint32_t *c_pair,
+                            const uint2 *ab, const float4 *tpos, const float4 *tq, const float4 *c_np,
+                            const float4 *c_pt, const float4 *c_aa, const float4 *c_ab, const float4 *c_err,
+                            const float4 *c_acc, float4 *jac, float dt, uint32_t phase, const uint32_t *impact) {
+    const uint32_t Pn = *npts;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < Pn; c += gridDim.x * blockDim.x) {
+        const uint2 p = ab[c_pair[c]];
+        if (phase != 0 && impact[p.x] == 0 && impact[p.y] == 0) continue;
+        const V3 n = v3(c_np[c]), point = v3(c_pt[c]);
+        V3 t0, t1;
+        contact_basis(n, t0, t1);
+        const V3 axes[3] = {n, t0, t1};
+        const Q qa = q4(tq[p.x]), qb = q4(tq[p.y]);
+        const V3 armA = sub(point, v3(tpos[p.x])), armB = sub(point, v3(tpos[p.y]));
+        const V3 ra_t = rotate(qa, v3(c_aa[c])), rb_t = rotate(qb, v3(c_ab[c]));
+        const float4 e = c_err[c];
+        const float e0[3] = {e.x, e.y, e.z};
+        float ref[3] = {0.0f, 0.0f, 0.0f};
+        if (phase != 0) {
+            const float4 acc = c_acc[c];
+            ref[0] = acc.x / dt;
+            ref[1] = dot(v3(make_float4(acc.y, acc.z, acc.w, 0.0f)), t0) / dt;
+            ref[2] = dot(v3(make_float4(acc.y, acc.z, acc.w, 0.0f)), t1) / dt;
+        }
+        const float mu = sqrtf(P[p.x].friction * P[p.y].friction);
+        const float beta = pair_stiffness(P[p.x], P[p.y], dt) * 100.0f;
+        float4 *J = jac + 9 * (size_t)c;
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const V3 ra = ax == 0 ? armA : ra_t;
+            const V3 rb = ax == 0 ? armB : rb_t;
+            J[ax] = f4(axes[ax], e0[ax]);
+            J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
+            J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Solve kernel pieces.
+// ---------------------------------------------------------------------
+struct SolveArgs {
+    uint32_t n;
+    const PmhBodyParameters *P;
+    PmhRigidBodyState *states;
+    float4 *tpos, *tq, *tv, *tw, *disp, *dq;
+    const float4 *inertia;
+    const uint32_t *color_bodies, *color_start, *bp_start;
+    const uint2 *entries;
+    const float4 *jac;
+    float4 *c_lam, *c_pen, *c_aa, *c_ab, *c_acc, *c_rep, *c_err;
+    const float4 *c_np, *c_pt;
+    const uint32_t *c_pair;
+    const uint2 *ab;
+    const uint32_t *parent, *impact, *deg, *anchored;
+    float4 *mean;
+    PmhContactSchedule *schedule;
+    StepCounters *ctr;
+    const uint32_t *npts, *npairs;
+    float dt;
+    uint32_t budget, realtime;
+};
+
+__device__ __forceinline__ void load_disp(const float4 *disp, uint32_t i, float *d) {
+    const float4 a = disp[2 * i], b = disp[2 * i + 1];
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = b.x; d[4] = b.y; d[5] = b.z;
+}
+
+// err = e0 + dot6(JA, dA) + dot6(JB, dB) with JA = (-axis, angA),
+// JB = (axis, angB): the operation order of the reference.
+__device__ __forceinline__ float row_error(float e0, float4 axis, float4 angA, float4 angB, const float *dA,
+                                           const float *dB) {
+    const float ja[6] = {-axis.x, -axis.y, -axis.z, angA.x, angA.y, angA.z};
+    const float jb[6] = {axis.x, axis.y, axis.z, angB.x, angB.y, angB.z};
+    return e0 + dot6(ja, dA) + dot6(jb, dB);
+}
+
+template <int L>
+__device__ __forceinline__ void group_reduce(Block &b) {
+#pragma unroll
+    for (int s = 1; s < L; s <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 21; ++k) b.h[k] = b.h[k] + __shfl_xor(b.h[k], s, 64);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) b.g[k] = b.g[k] + __shfl_xor(b.g[k], s, 64);
+    }
+}
+
+// apply_pose_update, geometry_constraints.slang:386-408.
+__device__ __forceinline__ void apply_pose_update(float4 *disp, float4 *dqa, uint32_t i, const float *u) {
+    float4 t = disp[2 * i], r = disp[2 * i + 1];
+    t.x += u[0]; t.y += u[1]; t.z += u[2];
+    disp[2 * i] = t;
+    const V3 rot = {u[3], u[4], u[5]};
+    if (rot.x == 0.0f && rot.y == 0.0f && rot.z == 0.0f) return;
+    Q rel = q4(dqa[i]);
+    const Q der = qmul({rot.x, rot.y, rot.z, 0.0f}, rel);
+    rel = qnormalize({rel.x + 0.5f * der.x, rel.y + 0.5f * der.y, rel.z + 0.5f * der.z, rel.w + 0.5f * der.w});
+    if (rel.w < 0.0f) rel = {-rel.x, -rel.y, -rel.z, -rel.w};
+    dqa[i] = f4q(rel);
+    r.x = 2.0f * rel.x; r.y = 2.0f * rel.y; r.z = 2.0f * rel.z;
+    disp[2 * i + 1] = r;
+}
+
+// One colour sweep for the bodies of `color` (solve_color,
+// geometry_constraints_solve.slang:331-491).  `gw`/`nw` index the
+// wavefront within the launch.
+template <int L>
+__device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, uint32_t gw, uint32_t nw) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t grp = lane / L, sub_lane = lane % L;
+    constexpr uint32_t groups = 64 / L;
+    const uint32_t begin = a.color_start[color], end = a.color_start[color + 1];
+    for (uint32_t base = begin + gw * groups; base < end; base += nw * groups) {
+        const uint32_t k = base + grp;
+        bool active = k < end;
+        uint32_t i = 0;
+        if (active) {
+            i = a.color_bodies[k];
+            if (phase != 0 && a.impact[i] == 0) active = false;
+        }
+        Block blk;
+        block_zero(blk);
+        float dI[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t eb = 0, ee = 0;
+        if (active) {
+            load_disp(a.disp, i, dI);
+            eb = a.bp_start[k];
+            ee = a.bp_start[k + 1];
+            if (sub_lane == 0) {
+                const float4 i0 = a.inertia[2 * i], i1 = a.inertia[2 * i + 1];
+                // Full packed block of the body: diag(m/dt^2) and the
+                // rotational lower triangle; the rest is zero.
+                float h[21];
+#pragma unroll
+                for (int q = 0; q < 21; ++q) h[q] = 0.0f;
+                h[0] = i0.x; h[2] = i0.x; h[5] = i0.x;
+                h[9] = i0.y; h[13] = i0.z; h[14] = i0.w; h[18] = i1.x; h[19] = i1.y; h[20] = i1.z;
+#pragma unroll
+                for (int q = 0; q < 21; ++q) blk.h[q] = h[q];
+#pragma unroll
+                for (int r = 0; r < 6; ++r)
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        blk.g[r] += h[r > c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r] * dI[c];
+            }
+        }
+        for (uint32_t e = eb + sub_lane; e < ee; e += L) {
+            const uint2 ent = a.entries[e];
+            const uint32_t c = ent.x & 0x7fffffffu;
+            const bool first = (ent.x & kSideFirst) != 0u;
+            float dO[6];
+            load_disp(a.disp, ent.y, dO);
+            const float *dA = first ? dI : dO;
+            const float *dB = first ? dO : dI;
+            const float4 *J = a.jac + 9 * (size_t)c;
+            const float4 lam = a.c_lam[c], pen = a.c_pen[c];
+            float4 Jx[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Jx[q] = J[q];
+            const float err0 = row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB);
+            const float err1 = row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB);
+            const float err2 = row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB);
+            const Force3 trial = {lam.x + pen.x * err0, lam.y + pen.y * err1, lam.z + pen.z * err2};
+            const float mu = Jx[6].w;
+            const Force3 f = project_contact(trial, mu);
+            const Force3 sc = stiffness_scales(trial, mu);
+            const float stiff[3] = {pen.x * sc.n, pen.y * sc.t0, pen.z * sc.t1};
+            const float force[3] = {f.n - Jx[3].w, f.t0 - Jx[4].w, f.t1 - Jx[5].w};
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const float4 axis = Jx[ax];
+                const float4 ang = first ? Jx[3 + ax] : Jx[6 + ax];
+                const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
+                                      first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
+                add_row(blk, row, stiff[ax], force[ax]);
+            }
+        }
+        group_reduce<L>(blk);
+        float update[6];
+        const bool solved = active && block_solve(blk, update);
+        // Crossing safeguard (solve.slang:433-475).
+        Block cross_blk;
+        block_zero(cross_blk);
+        bool crossing = false;
+        if (solved) {
+            for (uint32_t e = eb + sub_lane; e < ee; e += L) {
+                const uint2 ent = a.entries[e];
+                const uint32_t c = ent.x & 0x7fffffffu;
+                const bool first = (ent.x & kSideFirst) != 0u;
+                float dO[6];
+                load_disp(a.disp, ent.y, dO);
+                const float *dA = first ? dI : dO;
+                const float *dB = first ? dO : dI;
+                const float4 *J = a.jac + 9 * (size_t)c;
+                const float4 axis = J[0], angA = J[3], angB = J[6];
+                const float err = row_error(axis.w, axis, angA, angB, dA, dB);
+                const float lam0 = a.c_lam[c].x, k0 = a.c_pen[c].x;
+                const float4 ang = first ? angA : angB;
+                const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
+                                      first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
+                const float trial = lam0 + k0 * err;
+                if (trial <= 0.0f && trial + k0 * dot6(row, update) > k0 * 1.0e-6f) {
+                    add_row(cross_blk, row, k0, trial);
+                    crossing = true;
+                }
+            }
+        }
+        // Group-wide "any crossing" (uniform across the group's lanes).
+        uint64_t bal = __ballot(crossing);
+        const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << L) - 1ull)) << (grp * L);
+        const bool group_crossing = (bal & gmask) != 0ull;
+        if (bal != 0ull) {
+            group_reduce<L>(cross_blk);
+            if (group_crossing && solved) {
+#pragma unroll
+                for (int q = 0; q < 21; ++q) blk.h[q] += cross_blk.h[q];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) blk.g[q] += cross_blk.g[q];
+                float safe[6];
+                if (block_solve(blk, safe))
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) update[q] = safe[q];
+            }
+        }
+        if (solved && sub_lane == 0) {
+            if (phase == 0) {
+                apply_pose_update(a.disp, a.dq, i, update);
+            } else {
+                float4 t = a.disp[2 * i], r = a.disp[2 * i + 1];
+                t.x += update[0]; t.y += update[1]; t.z += update[2];
+                r.x += update[3]; r.y += update[4]; r.z += update[5];
+                a.disp[2 * i] = t;
+                a.disp[2 * i + 1] = r;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ PmhRigidBodyState avbd_pose(const SolveArgs &a, uint32_t i) {
+    PmhRigidBodyState s;
+    const V3 t = v3(a.tpos[i]);
+    const float4 d = a.disp[2 * i];
+    s.position = pv(add(t, {d.x, d.y, d.z}));
+    Q o = q4(a.tq[i]);
+    const Q dq = q4(a.dq[i]);
+    if (dq.x != 0.0f || dq.y != 0.0f || dq.z != 0.0f) o = qnormalize(qmul(dq, o));
+    s.orientation = pq(o);
+    return s;
+}
+
+// advance_duals for contacts (solve.slang:492-566): one thread per point.
+__device__ void advance_duals(const SolveArgs &a, uint32_t phase, uint32_t iteration, uint32_t tid, uint32_t nthreads) {
+    const uint32_t Pn = *a.npts;
+    const bool last = iteration + 1u == a.budget;
+    for (uint32_t c = tid; c < Pn; c += nthreads) {
+        const uint2 p = a.ab[a.c_pair[c]];
+        if (phase != 0 && a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+        float dA[6], dB[6];
+        load_disp(a.disp, p.x, dA);
+        load_disp(a.disp, p.y, dB);
+        const float4 *J = a.jac + 9 * (size_t)c;
+        float4 Jx[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Jx[q] = J[q];
+        float4 lam = a.c_lam[c], pen = a.c_pen[c];
+        const float err[3] = {row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB),
+                              row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB),
+                              row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB)};
+        const float mu = Jx[6].w, beta = Jx[7].w;
+        const Force3 trial = {lam.x + pen.x * err[0], lam.y + pen.y * err[1], lam.z + pen.z * err[2]};
+        const Force3 f = project_contact(trial, mu);
+        const float tx = lam.y + pen.y * err[1];
+        const float ty = lam.z + pen.z * err[2];
+        const float fb = mu * f.n;
+        const bool stick = tx * tx + ty * ty <= fb * fb;
+        lam = make_float4(f.n, f.t0, f.t1, 0.0f);
+        if (f.n > 0.0f) pen.x = smin(kMaxPenalty, pen.x + beta * fabsf(err[0]));
+        if (stick) {
+            pen.y = smin(kMaxPenalty, pen.y + beta * fabsf(err[1]));
+            pen.z = smin(kMaxPenalty, pen.z + beta * fabsf(err[2]));
+        }
+        a.c_lam[c] = lam;
+        a.c_pen[c] = pen;
+        if (last) {
+            const V3 t0 = v3(Jx[1]), t1 = v3(Jx[2]);
+            const float an = f.n * a.dt;
+            const V3 af = mul(add(mul(t0, f.t0), mul(t1, f.t1)), a.dt);
+            a.c_acc[c] = make_float4(an, af.x, af.y, af.z);
+            a.c_rep[c] = make_float4(an, af.x, af.y, af.z);
+            if (!stick) {
+                PmhRigidBodyState pa, pb;
+                if (phase != 0) {
+                    pa.position = pv(v3(a.tpos[p.x])); pa.orientation = pq(q4(a.tq[p.x]));
+                    pb.position = pv(v3(a.tpos[p.y])); pb.orientation = pq(q4(a.tq[p.y]));
+                } else {
+                    pa = avbd_pose(a, p.x);
+                    pb = avbd_pose(a, p.y);
+                }
+                const V3 point = v3(a.c_pt[c]);
+                const V3 na = inv_rotate(sq(pa.orientation), sub(point, sv(pa.position)));
+                const V3 nb = inv_rotate(sq(pb.orientation), sub(point, sv(pb.position)));
+                a.c_aa[c] = f4(na, a.c_aa[c].w);
+                a.c_ab[c] = f4(nb, 0.0f);
+            }
+        }
+    }
+}
+
+// free_mean (per unanchored island root, members in ascending index) and
+// free_subtract (solve.slang:617-698).
+__device__ void free_mean(const SolveArgs &a, uint32_t phase, uint32_t tid, uint32_t nthreads) {
+    const bool vel = phase != 0;
+    for (uint32_t r = tid; r < a.n; r += nthreads) {
+        if (a.P[r].motion != kDynamic || a.parent[r] != r) continue;
+        float4 mean = make_float4(0, 0, 0, 0);
+        if (!((vel && a.impact[r] == 0) || a.deg[r] == 0) && a.anchored[r] == 0) {
+            V3 w = {0.0f, 0.0f, 0.0f};
+            float total = 0.0f;
+            for (uint32_t i = r; i < a.n; ++i) {
+                if (a.parent[i] != r || a.P[i].motion != kDynamic) continue;
+                const float mass = 1.0f / smax(a.P[i].inverse_mass, 1.0e-12f);
+                const float4 d = a.disp[2 * i];
+                w = add(w, mul({d.x, d.y, d.z}, mass));
+                total += mass;
+            }
+            if (total > 0.0f) {
+                w = mul(w, 1.0f / total);
+                mean = make_float4(w.x, w.y, w.z, 0.0f);
+            }
+        }
+        a.mean[r] = mean;
+    }
+}
+
+__device__ void free_subtract_apply(const SolveArgs &a, uint32_t phase, uint32_t tid, uint32_t nthreads) {
+    const bool vel = phase != 0;
+    for (uint32_t i = tid; i < a.n; i += nthreads) {
+        if (a.P[i].motion != kDynamic) continue;
+        if (vel && a.impact[i] == 0) continue;
+        const float4 m = a.mean[a.parent[i]];
+        float4 t = a.disp[2 * i];
+        t.x -= m.x; t.y -= m.y; t.z -= m.z;
+        a.disp[2 * i] = t;
+        // apply_states (solve.slang:699-733)
+        PmhRigidBodyState s;
+        s.position = pv(v3(a.tpos[i]));
+        s.orientation = pq(q4(a.tq[i]));
+        V3 lv = v3(a.tv[i]), av = v3(a.tw[i]);
+        const float4 r = a.disp[2 * i + 1];
+        const V3 dt_ = {t.x, t.y, t.z};
+        if (phase != 0) {
+            lv = add(lv, mul(dt_, 1.0f / a.dt));
+            av = add(av, mul({r.x, r.y, r.z}, 1.0f / a.dt));
+            s.linear_velocity = pv(lv);
+            s.angular_velocity = pv(av);
+            a.states[i] = s;
+            continue;
+        }
+        s.position = pv(add(sv(s.position), dt_));
+        s.orientation = avbd_pose(a, i).orientation;
+        lv = add(lv, mul(dt_, 1.0f / a.dt));
+        av = add(av, mul(quaternion_rotation_vector(q4(a.dq[i])), 1.0f / a.dt));
+        if (a.realtime) {
+            lv = mul(lv, 1.0f - kRealtimeDamping);
+            av = mul(av, 1.0f - kRealtimeDamping);
+        }
+        s.linear_velocity = pv(lv);
+        s.angular_velocity = pv(av);
+        a.states[i] = s;
+    }
+}
+
+// Velocity-phase set-up (impact_gaps / impact_bodies /
+// impact_constraints, solve.slang:232-329).
+__device__ void impact_gaps(const SolveArgs &a, uint32_t tid, uint32_t nthreads) {
+    const uint32_t Pn = *a.npts;
+    for (uint32_t c = tid; c < Pn; c += nthreads) {
+        const uint2 p = a.ab[a.c_pair[c]];
+        if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+        float dA[6], dB[6];
+        load_disp(a.disp, p.x, dA);
+        load_disp(a.disp, p.y, dB);
+        const float4 *J = a.jac + 9 * (size_t)c;
+        const float pen = a.c_np[c].w;
+        // normal row with the contact penetration as the constant term
+        const float e = row_error(pen, J[0], J[3], J[6], dA, dB);
+        float4 err = a.c_err[c];
+        err.x = smax(0.0f, -e);
+        a.c_err[c] = err;
+    }
+}
+
+__device__ void impact_bodies(const SolveArgs &a, uint32_t tid, uint32_t nthreads) {
+    for (uint32_t i = tid; i < a.n; i += nthreads) {
+        if (a.impact[i] != 0 || a.P[i].motion != kDynamic) {
+            const PmhRigidBodyState s = a.states[i];
+            a.tpos[i] = f4(sv(s.position), 0.0f);
+            a.tq[i] = f4q(sq(s.orientation));
+            a.tv[i] = f4(sv(s.linear_velocity), 0.0f);
+            a.tw[i] = f4(sv(s.angular_velocity), 0.0f);
+            a.disp[2 * i] = make_float4(0, 0, 0, 0);
+            a.disp[2 * i + 1] = make_float4(0, 0, 0, 0);
+            a.dq[i] = make_float4(0, 0, 0, 1);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t global_wave() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
+
+// The whole solve in one cooperative launch.
+__global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *c_flags) {
+    cg::grid_group grid = cg::this_grid();
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nthreads = gridDim.x * blockDim.x;
+    const uint32_t gw = tid >> 6, nw = nthreads >> 6;
+    const uint32_t colors = a.ctr->colors;
+    const uint32_t phases = a.realtime ? 1u : 2u;
+    for (uint32_t phase = 0; phase < phases; ++phase) {
+        if (phase != 0) {
+            if (a.ctr->impact == 0) break;
+            impact_gaps(a, tid, nthreads);
+            grid.sync();
+            impact_bodies(a, tid, nthreads);
+            grid.sync();
+            // restitution of cold contacts enters through the velocity error
+            const uint32_t Pn = *a.npts;
+            for (uint32_t c = tid; c < Pn; c += nthreads) {
+                const uint2 p = a.ab[a.c_pair[c]];
+                if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+                const PmhRigidBodyState A = a.states[p.x], B = a.states[p.y];
+                const float dt = a.dt;
+                const float va[6] = {A.linear_velocity.x * dt, A.linear_velocity.y * dt, A.linear_velocity.z * dt,
+                                     A.angular_velocity.x * dt, A.angular_velocity.y * dt, A.angular_velocity.z * dt};
+                const float vb[6] = {B.linear_velocity.x * dt, B.linear_velocity.y * dt, B.linear_velocity.z * dt,
+                                     B.angular_velocity.x * dt, B.angular_velocity.y * dt, B.angular_velocity.z * dt};
+                const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
+                V3 t0, t1;
+                contact_basis(n, t0, t1);
+                const V3 axes[3] = {n, t0, t1};
+                const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
+                const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
+                float out[3];
+                for (uint32_t ax = 0; ax < 3; ++ax) {
+                    const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
+                    const V3 la = mul(axes[ax], -1.0f), aa = mul(cross(ra, axes[ax]), -1.0f), ba = cross(rb, axes[ax]);
+                    const float ja[6] = {la.x, la.y, la.z, aa.x, aa.y, aa.z};
+                    const float jb[6] = {axes[ax].x, axes[ax].y, axes[ax].z, ba.x, ba.y, ba.z};
+                    out[ax] = dot6(ja, va) + dot6(jb, vb);
+                }
+                const float gap = a.c_err[c].x;
+                const bool cached = (c_flags[c] & 1u) != 0u;
+                const float speed = a.c_pt[c].w;
+                const float rest = !cached && speed < -1.0f
+                    ? smin(a.P[p.x].restitution, a.P[p.y].restitution) : 0.0f;
+                a.c_err[c] = make_float4(out[0] - gap - rest * speed * dt, out[1], out[2], 0.0f);
+            }
+            grid.sync();
+            // Jacobians from the reset targets.
+            for (uint32_t c = tid; c < Pn; c += nthreads) {
+                const uint2 p = a.ab[a.c_pair[c]];
+                if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+                const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
+                V3 t0, t1;
+                contact_basis(n, t0, t1);
+                const V3 axes[3] = {n, t0, t1};
+                const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
+                const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
+                const float4 e = a.c_err[c];
+                const float e0[3] = {e.x, e.y, e.z};
+                const float4 acc = a.c_acc[c];
+                const V3 af = {acc.y, acc.z, acc.w};
+                const float ref[3] = {acc.x / a.dt, dot(af, t0) / a.dt, dot(af, t1) / a.dt};
+                float4 *J = const_cast<float4 *>(a.jac) + 9 * (size_t)c;
+                const float mu = J[6].w, beta = J[7].w;
+                for (uint32_t ax = 0; ax < 3; ++ax) {
+                    const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
+                    J[ax] = f4(axes[ax], e0[ax]);
+                    J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
+                    J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
+                }
+            }
+            grid.sync();
+        }
+        for (uint32_t it = 0; it < a.budget; ++it) {
+            for (uint32_t c = 0; c < colors; ++c) {
+                solve_color<kLanes>(a, c, phase, gw, nw);
+                grid.sync();
+            }
+            advance_duals(a, phase, it, tid, nthreads);
+            grid.sync();
+        }
+        free_mean(a, phase, tid, nthreads);
+        grid.sync();
+        free_subtract_apply(a, phase, tid, nthreads);
+        grid.sync();
+        if (tid == 0) {
+            a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
+            a.schedule->maximum_passes += a.budget;
+        }
+    }
+}
+
+// Staged variant: one launch per stage (no grid barrier).
+__global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t color, uint32_t phase) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    solve_color<kLanes>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
+}
+__global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
+    advance_duals(a, phase, it, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_mean(SolveArgs a, uint32_t phase) {
+    free_mean(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
+    free_subtract_apply(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
+        a.schedule->maximum_passes += a.budget;
+    }
+}
+
+__global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32_t budget, uint32_t profile,
+                                uint32_t base_iterations) {
+    if (threadIdx.x != 0) return;
+    PmhContactSchedule v;
+    memset(&v, 0, sizeof v);
+    v.budget = budget;
+    v.offsets[0] = profile;
+    v.offsets[1] = base_iterations;
+    *s = v;
+    ctr->manifolds = 0;
+    ctr->colors = 0;
+    ctr->islands = 0;
+    ctr->impact = 0;
+    ctr->color_ticket = 0;
+}
+
+__global__ void k_schedule_finish(PmhContactSchedule *s, const StepCounters *ctr, uint32_t blocks) {
+    if (threadIdx.x != 0) return;
+    s->remaining = ctr->colors;
+    s->island_count = ctr->islands;
+    s->counts[0] = ctr->impact;
+    s->candidates = ctr->pairs;
+    s->contacts = ctr->manifolds;
+    s->blocks = blocks;
+}
+
+static int g_cus = 0;
+int device_compute_units() {
+    if (!g_cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_cus = prop.multiProcessorCount;
+        if (!g_cus) g_cus = 256;
+    }
+    return g_cus;
+}
+
+static uint32_t grid_for(uint32_t items, uint32_t block = 256, uint32_t cap = 4096) {
+    uint32_t g = (items + block - 1) / block;
+    return std::max<uint32_t>(1, std::min(g, cap));
+}
+
+PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
+    PmhResult r = PMH_SUCCESS;
+#define R(buf, cnt) if ((r = buf.reserve(cnt)) != PMH_SUCCESS) return r
+    R(tpos, bodies); R(tq, bodies); R(tv, bodies); R(tw, bodies); R(disp, 2 * (size_t)bodies);
+    R(dq, bodies); R(inertia, 2 * (size_t)bodies); R(color, bodies); R(parent, bodies);
+    R(impact, bodies); R(deg, bodies + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
+    R(color_bodies, bodies); R(bp_start, bodies + 1); R(anchored, bodies); R(mean, bodies);
+    if (prev_vel.n < bodies) {
+        R(prev_vel, bodies);
+        PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
+    }
+    R(code, pairs); R(ab, pairs); R(m_count, pairs + 1); R(m_start, pairs + 1); R(m_slot, pairs + 1);
+    R(m_impact, pairs); R(m_face, pairs); R(m_init_rel, pairs); R(adj, 2 * (size_t)pairs);
+    R(c_np, points); R(c_pt, points); R(c_aa, points); R(c_ab, points); R(c_err, points);
+    R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
+    R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
+    R(schedule, 1); R(color_count, kMaxColors + 1); R(color_start, kMaxColors + 1);
+    if (!counters.p) {
+        R(counters, 1);
+        PMH_CHECK(hipMemset(counters.p, 0, sizeof(StepCounters)));
+    }
+#undef R
+    n = bodies;
+    pair_cap = pairs;
+    point_cap = points;
+    return PMH_SUCCESS;
+}
+
+void Solver::release() {
+    tpos.release(); tq.release(); tv.release(); tw.release(); disp.release(); dq.release();
+    inertia.release(); prev_vel.release(); color.release(); parent.release(); impact.release();
+    deg.release(); adj_start.release(); adj_cursor.release(); color_bodies.release();
+    bp_start.release(); anchored.release(); mean.release(); code.release(); ab.release();
+    m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
+    m_init_rel.release(); adj.release(); c_np.release(); c_pt.release(); c_aa.release();
+    c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
+    c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
+    schedule.release(); color_count.release(); color_start.release(); counters.release();
+    scan_tmp.release();
+}
+
+static uint32_t g_coop_blocks = 0;
+
+PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *P,
+                             const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+                             const SolveParams &sp, SolveTiming *timing) {
+    const uint32_t n = v.n;
+    const float dt = sp.dt;
+    const bool realtime = sp.profile == kRealtimeProfile;
+    const uint32_t budget = realtime ? kRealtimeSweeps
+                                     : (sp.requested_iterations ? sp.requested_iterations : sp.base_iterations);
+    StepCounters *ctr = v.counters.p;
+    const uint32_t *npairs = &ctr->pairs;
+    const uint32_t *npts = &ctr->points;
+    const uint32_t bg = grid_for(n);
+    const uint32_t pg = grid_for(v.pair_cap, 256, 2048);
+    const uint32_t cg_ = grid_for(v.point_cap, 256, 4096);
+
+    k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
+    k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
+                                   v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
+                                   v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p);
+    k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
+                                      v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
+                                      v.m_impact.p, dt, sp.profile);
+    k_degree<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.deg.p, nullptr, ctr);
+    PmhResult r = exclusive_scan(s, v.deg.p, v.adj_start.p, n, nullptr, nullptr, v.scan_tmp);
+    if (r != PMH_SUCCESS) return r;
+    k_adj_fill<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.adj_start.p, v.adj_cursor.p, v.adj.p);
+    k_adj_sort<<<bg, 256, 0, s>>>(n, v.adj_start.p, v.adj.p);
+    k_uf_hook<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.parent.p);
+    k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
+    k_impact_roots<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_impact.p, v.m_count.p, v.parent.p, v.impact.p,
+                                      v.anchored.p, ctr);
+    k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, ctr);
+    const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
+    k_color<<<color_blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, ctr);
+    PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
+    k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
+    k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr);
+    k_color_place<<<bg, 256, 0, s>>>(n, v.color.p, v.color_start.p, v.color_count.p, v.color_bodies.p);
+    // colour-ordered point lists (deg array reused as the per-position count)
+    // The degree array is free again once the adjacency is built (the
+    // fill cursor keeps each body's degree): it holds the point counts.
+    k_bp_count<<<bg, 256, 0, s>>>(n, ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p,
+                                  v.m_count.p, v.deg.p);
+    r = exclusive_scan(s, v.deg.p, v.bp_start.p, n, nullptr, nullptr, v.scan_tmp);
+    if (r != PMH_SUCCESS) return r;
+    k_bp_fill<<<bg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
+                                 v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p));
+    k_jacobians<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.tpos.p, v.tq.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
+                                    v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p);
+    PMH_CHECK(hipGetLastError());
+
+    SolveArgs a;
+    a.n = n;
+    a.P = P;
+    a.states = states;
+    a.tpos = v.tpos.p; a.tq = v.tq.p; a.tv = v.tv.p; a.tw = v.tw.p; a.disp = v.disp.p; a.dq = v.dq.p;
+    a.inertia = v.inertia.p;
+    a.color_bodies = v.color_bodies.p; a.color_start = v.color_start.p; a.bp_start = v.bp_start.p;
+    a.entries = reinterpret_cast<const uint2 *>(v.bp_entries.p);
+    a.jac = v.jac.p;
+    a.c_lam = v.c_lam.p; a.c_pen = v.c_pen.p; a.c_aa = v.c_aa.p; a.c_ab = v.c_ab.p; a.c_acc = v.c_acc.p;
+    a.c_rep = v.c_rep.p; a.c_err = v.c_err.p; a.c_np = v.c_np.p; a.c_pt = v.c_pt.p;
+    a.c_pair = v.c_pair.p; a.ab = v.ab.p;
+    a.parent = v.parent.p; a.impact = v.impact.p; a.deg = v.adj_cursor.p; a.anchored = v.anchored.p;
+    a.mean = v.mean.p;
+    a.schedule = v.schedule.p;
+    a.ctr = ctr;
+    a.npts = npts; a.npairs = npairs;
+    a.dt = dt;
+    a.budget = budget;
+    a.realtime = realtime ? 1u : 0u;
+
+    if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
+    if (!sp.staged) {
+        if (!g_coop_blocks) {
+            int per_cu = 0;
+            PMH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solve_all, 256, 0));
+            // Two resident blocks per CU at most: more blocks only add
+            // grid-barrier arrivals.
+            g_coop_blocks = std::max(1, std::min(per_cu, 2)) * device_compute_units();
+        }
+        uint32_t blocks = g_coop_blocks;
+        const uint32_t *cf = v.c_flags.p;
+        void *args[] = {&a, &cf};
+        PMH_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_all), dim3(blocks), dim3(256),
+                                             args, 0, s));
+        if (timing) timing->launches += 1;
+        k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, blocks);
+    } else {
+        // Host-known schedule: read the colour count back.
+        StepCounters host;
+        PMH_CHECK(hipMemcpyAsync(&host, ctr, sizeof host, hipMemcpyDeviceToHost, s));
+        PMH_CHECK(hipStreamSynchronize(s));
+        const uint32_t blocks = 2 * device_compute_units();
+        const uint32_t phases = realtime ? 1u : 2u;
+        for (uint32_t phase = 0; phase < phases; ++phase) {
+            if (phase != 0) return PMH_ERROR_FEATURE_NOT_PRESENT;  // staged legacy velocity phase: not built
+            for (uint32_t it = 0; it < budget; ++it) {
+                for (uint32_t c = 0; c < host.colors; ++c) k_stage_color<<<blocks, 256, 0, s>>>(a, c, phase);
+                k_stage_duals<<<cg_, 256, 0, s>>>(a, phase, it);
+                if (timing) timing->launches += host.colors + 1;
+            }
+            k_stage_mean<<<bg, 256, 0, s>>>(a, phase);
+            k_stage_apply<<<bg, 256, 0, s>>>(a, phase);
+        }
+        k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, blocks);
+    }
+    if (timing && timing->stop) PMH_CHECK(hipEventRecord(timing->stop, s));
+    PMH_CHECK(hipGetLastError());
+    return PMH_SUCCESS;
+}
+
+}  // namespace pmh

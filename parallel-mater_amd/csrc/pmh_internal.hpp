@@ -1,0 +1,160 @@
+// SPDX-License-Identifier: MIT
+// Internal layout of the rigid step on one GPU.
+//
+// Storage is O(n + pairs + points): per-body arrays, per-active-pair
+// arrays and per-contact-point arrays, all structure-of-arrays with 16-byte
+// records so a wavefront's gathers touch whole cache lines.  Nothing is
+// indexed by pair code, so 32k-body scenes need tens of MB, not the
+// n(n-1)/2 manifold table of the reference (resource_plan.hpp:146-215).
+#pragma once
+
+#include "pmh_math.hpp"
+#include "../../include/pm_hip_rigid.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace pmh {
+
+#define PMH_CHECK(expr)                                                  \
+    do {                                                                 \
+        hipError_t pmh_e_ = (expr);                                      \
+        if (pmh_e_ != hipSuccess) return pmh::hip_result(pmh_e_, #expr); \
+    } while (0)
+
+PmhResult hip_result(hipError_t e, const char *what);
+const std::string &last_error();
+
+// Shape kinds: triangle mesh (boxes and the reference's polyhedral
+// sphere), analytic sphere and capsule.
+enum : uint32_t { kShapeMesh = 0, kShapeSphere = 1, kShapeCapsule = 2 };
+
+struct ShapeDev {
+    uint32_t kind, vtx_off, vtx_count, idx_off;
+    uint32_t idx_count, plane_off, has_planes, convex_face;  // convex_face: planes && <=96 indices
+    float mn[3], bradius;
+    float mx[3], sphere_radius;
+    float center[3], half_length;
+    float box_half[3];
+    uint32_t is_box;   // centred axis-aligned 8-vertex box
+};
+
+struct HostShape {
+    ShapeDev info{};
+    std::vector<V3> vertices;
+    std::vector<uint32_t> indices;
+    std::vector<float> planes;   // 4 per triangle plane, then the unique tail
+    uint32_t unique_planes = 0;
+    float unit_inertia[3]{};
+};
+
+// Device status bits (accumulated across steps, read after a batch).
+enum : uint32_t {
+    kStatusPairOverflow = 1u << 0,
+    kStatusPointOverflow = 1u << 1,
+    kStatusColorOverflow = 1u << 2,
+    kStatusInvalidShape = 1u << 3,
+    kStatusColorTimeout = 1u << 4,
+};
+
+// Device-side counters of one step (one record, read back with stats).
+struct StepCounters {
+    uint32_t pairs;            // active pairs M
+    uint32_t points;           // contact points P
+    uint32_t manifolds;        // non-empty manifolds
+    uint32_t colors;           // colour count
+    uint32_t islands;
+    uint32_t impact;
+    uint32_t unsupported;      // pairs the triangle-pair path would take
+    uint32_t cache_hits;
+    uint32_t status;
+    uint32_t large_count;      // bodies outside the hash grid
+    uint32_t cache_count;      // saved pairs for the next step
+    uint32_t color_ticket;     // work ticket of the colouring kernel
+    float cell_size_bits;      // broadphase cell (as float)
+    uint32_t barrier_count, barrier_gen;
+    uint32_t pad[17];
+};
+
+constexpr uint32_t kMaxColors = 256;
+constexpr uint32_t kCacheFloat4PerPoint = 6;
+
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    PmhResult reserve(size_t count) {
+        if (count <= n && p) return PMH_SUCCESS;
+        if (p) { (void)hipFree(p); p = nullptr; n = 0; }
+        size_t bytes = (count ? count : 1) * sizeof(T);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) return hip_result(e, "hipMalloc");
+        n = count;
+        return PMH_SUCCESS;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+// Everything AVBD needs after the narrowphase: per-body, per-pair and
+// per-point solver state.  Shared by the world step and the kernel seam.
+struct Solver {
+    uint32_t n = 0, pair_cap = 0, point_cap = 0;
+    // bodies
+    DBuf<float4> tpos, tq, tv, tw;       // AVBD target (RigidBodyState)
+    DBuf<float4> disp;                   // 2 per body: translation, rotation
+    DBuf<float4> dq;                     // orientation_delta
+    DBuf<float4> inertia;                // 2 per body: m/dt^2, I/dt^2 lower tri
+    DBuf<float4> prev_vel;               // xyz + history_valid
+    DBuf<uint32_t> color, parent, impact, deg, adj_start, adj_cursor;
+    DBuf<uint32_t> color_bodies, bp_start, anchored;
+    DBuf<float4> mean;                   // free-motion mean per root
+    // pairs
+    DBuf<uint32_t> code;                 // a*n+b
+    DBuf<uint2> ab;
+    DBuf<uint32_t> m_count, m_start, m_slot, m_impact, m_face;
+    DBuf<float4> m_init_rel;
+    DBuf<uint32_t> adj;                  // 2 per pair
+    // points
+    DBuf<float4> c_np, c_pt, c_aa, c_ab, c_err, c_lam, c_pen, c_acc, c_rep;
+    DBuf<uint32_t> c_flags, c_pair;
+    DBuf<float4> jac;                    // 9 per point
+    DBuf<uint32_t> bp_entries;           // 2 per point (one per endpoint)
+    // schedule
+    DBuf<PmhContactSchedule> schedule;
+    DBuf<uint32_t> color_count, color_start;
+    DBuf<StepCounters> counters;
+    DBuf<uint32_t> scan_tmp;
+    PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
+    void release();
+};
+
+struct SolveParams {
+    float dt;
+    V3 gravity;
+    uint32_t base_iterations, requested_iterations, profile;
+    uint32_t staged;                     // per-colour launches instead of one cooperative kernel
+};
+
+struct SolveTiming {
+    hipEvent_t start = nullptr, stop = nullptr;   // recorded around the solve launch
+    float total_ms = 0.0f;
+    uint32_t launches = 0;
+};
+
+// Device-wide exclusive scan of `count_ptr[0]` (or `count` when the
+// pointer is null) uint32 values; writes the total to `total` when given.
+PmhResult exclusive_scan(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t count,
+                         const uint32_t *count_ptr, uint32_t *total, DBuf<uint32_t> &tmp);
+
+// AVBD prepare + solve on the solver state. Bodies/pairs/points must be
+// filled: pair arrays (code, ab, m_count, m_start, m_init_rel), point
+// arrays (c_*), counters.pairs/points, and the states.
+PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *params,
+                             const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+                             const SolveParams &sp, SolveTiming *timing);
+
+int device_compute_units();
+
+}  // namespace pmh

@@ -1,0 +1,173 @@
+// SPDX-License-Identifier: MIT
+// Kernel seam: prepare_geometry_constraints + solve_geometry_constraints
+// (src/simulation/geometry_constraints_dispatch.hpp:15-43) over the
+// reference's ContactManifold / AvbdBody / ContactSchedule records in
+// device memory.  The records are unpacked into the solver's
+// structure-of-arrays form, solved, and the solver-owned fields are written
+// back (duals, errors, anchors, impulses, list links; the AvbdBody record).
+#include "pmh_internal.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+namespace pmh {
+
+__global__ void k_seam_counts(uint32_t M, const PmhContactManifold *man, uint32_t *m_count) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < M) m_count[m] = std::min(man[m].count, 8u);
+}
+
+__global__ void k_seam_in(uint32_t n, uint32_t M, const uint32_t *pairs, const PmhContactManifold *man,
+                          const uint32_t *m_start, uint32_t *code, uint2 *ab, const uint32_t *m_count,
+                          float4 *m_init_rel, uint32_t *m_face, float4 *c_np, float4 *c_pt, float4 *c_aa,
+                          float4 *c_ab, float4 *c_err, float4 *c_lam, float4 *c_pen, float4 *c_acc, float4 *c_rep,
+                          uint32_t *c_flags, uint32_t *c_pair, StepCounters *ctr) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) {
+        ctr->pairs = M;
+        ctr->points = m_start[M];
+    }
+    if (m >= M) return;
+    const uint32_t p = pairs[m];
+    code[m] = p;
+    ab[m] = make_uint2(p / n, p % n);
+    const PmhContactManifold &mm = man[m];
+    m_init_rel[m] = make_float4(mm.initial_relative_position.x, mm.initial_relative_position.y,
+                                mm.initial_relative_position.z, 0.0f);
+    m_face[m] = mm.face_patch;
+    for (uint32_t k = 0; k < m_count[m]; ++k) {
+        const PmhContact &t = mm.contacts[k];
+        const uint32_t c = m_start[m] + k;
+        c_np[c] = make_float4(t.normal.x, t.normal.y, t.normal.z, t.penetration);
+        c_pt[c] = make_float4(t.point.x, t.point.y, t.point.z, t.initial_normal_speed);
+        c_aa[c] = make_float4(t.avbd_anchor_a.x, t.avbd_anchor_a.y, t.avbd_anchor_a.z, t.impact_fraction);
+        c_ab[c] = make_float4(t.avbd_anchor_b.x, t.avbd_anchor_b.y, t.avbd_anchor_b.z, 0.0f);
+        c_err[c] = make_float4(t.avbd_error.x, t.avbd_error.y, t.avbd_error.z, 0.0f);
+        c_lam[c] = make_float4(t.avbd_dual[0].lambda, t.avbd_dual[1].lambda, t.avbd_dual[2].lambda, 0.0f);
+        c_pen[c] = make_float4(t.avbd_dual[0].penalty, t.avbd_dual[1].penalty, t.avbd_dual[2].penalty, 0.0f);
+        c_acc[c] = make_float4(t.accumulated_normal_impulse, t.accumulated_friction_impulse.x,
+                               t.accumulated_friction_impulse.y, t.accumulated_friction_impulse.z);
+        c_rep[c] = make_float4(t.reported_normal_impulse, t.reported_friction_impulse.x,
+                               t.reported_friction_impulse.y, t.reported_friction_impulse.z);
+        c_flags[c] = (t.avbd_cached ? 1u : 0u) | (t.persistent ? 2u : 0u);
+        c_pair[c] = m;
+    }
+}
+
+__global__ void k_seam_history_in(uint32_t n, const PmhAvbdBody *bodies, float4 *prev_vel) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PmhAvbdBody &b = bodies[i];
+    prev_vel[i] = make_float4(b.previous_velocity.x, b.previous_velocity.y, b.previous_velocity.z,
+                              b.history_valid ? 1.0f : 0.0f);
+}
+
+__global__ void k_seam_out(uint32_t M, PmhContactManifold *man, const uint32_t *m_count, const uint32_t *m_start,
+                           const float4 *c_pt, const float4 *c_aa, const float4 *c_ab, const float4 *c_err,
+                           const float4 *c_lam, const float4 *c_pen, const float4 *c_acc, const float4 *c_rep) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    PmhContactManifold &mm = man[m];
+    for (uint32_t k = 0; k < m_count[m]; ++k) {
+        const uint32_t c = m_start[m] + k;
+        PmhContact &t = mm.contacts[k];
+        t.initial_normal_speed = c_pt[c].w;
+        t.avbd_anchor_a = {c_aa[c].x, c_aa[c].y, c_aa[c].z};
+        t.avbd_anchor_b = {c_ab[c].x, c_ab[c].y, c_ab[c].z};
+        t.avbd_error = {c_err[c].x, c_err[c].y, c_err[c].z};
+        t.avbd_dual[0] = {c_lam[c].x, c_pen[c].x};
+        t.avbd_dual[1] = {c_lam[c].y, c_pen[c].y};
+        t.avbd_dual[2] = {c_lam[c].z, c_pen[c].z};
+        t.accumulated_normal_impulse = c_acc[c].x;
+        t.accumulated_friction_impulse = {c_acc[c].y, c_acc[c].z, c_acc[c].w};
+        t.reported_normal_impulse = c_rep[c].x;
+        t.reported_friction_impulse = {c_rep[c].y, c_rep[c].z, c_rep[c].w};
+    }
+}
+
+// Bodies and the contact list links (heads and next pointers of dynamic
+// endpoints, in the reference's descending active order).
+__global__ void k_seam_bodies_out(uint32_t n, PmhAvbdBody *bodies, PmhContactManifold *man, const uint2 *ab,
+                                  const float4 *tpos, const float4 *tq, const float4 *tv, const float4 *tw,
+                                  const float4 *disp, const float4 *dq, const float4 *inertia, const float4 *prev_vel,
+                                  const uint32_t *color, const uint32_t *parent, const uint32_t *impact,
+                                  const uint32_t *adj_start, const uint32_t *adj) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PmhAvbdBody b;
+    memset(&b, 0, sizeof b);
+    b.target.position = {tpos[i].x, tpos[i].y, tpos[i].z};
+    b.target.orientation = {tq[i].x, tq[i].y, tq[i].z, tq[i].w};
+    b.target.linear_velocity = {tv[i].x, tv[i].y, tv[i].z};
+    b.target.angular_velocity = {tw[i].x, tw[i].y, tw[i].z};
+    b.orientation_delta = {dq[i].x, dq[i].y, dq[i].z, dq[i].w};
+    b.displacement[0] = disp[2 * i].x; b.displacement[1] = disp[2 * i].y; b.displacement[2] = disp[2 * i].z;
+    b.displacement[3] = disp[2 * i + 1].x; b.displacement[4] = disp[2 * i + 1].y; b.displacement[5] = disp[2 * i + 1].z;
+    const float4 i0 = inertia[2 * i], i1 = inertia[2 * i + 1];
+    b.inertia_h[0] = b.inertia_h[2] = b.inertia_h[5] = i0.x;
+    b.inertia_h[9] = i0.y; b.inertia_h[13] = i0.z; b.inertia_h[14] = i0.w;
+    b.inertia_h[18] = i1.x; b.inertia_h[19] = i1.y; b.inertia_h[20] = i1.z;
+    b.joint_head = kInvalid;
+    b.color = color[i];
+    b.parent = parent[i];
+    b.impact = impact[i];
+    b.previous_velocity = {prev_vel[i].x, prev_vel[i].y, prev_vel[i].z};
+    b.history_valid = prev_vel[i].w != 0.0f ? 1u : 0u;
+    const uint32_t s = adj_start[i], e = adj_start[i + 1];
+    b.contact_head = s < e ? adj[s] : kInvalid;
+    for (uint32_t k = s; k < e; ++k) {
+        const uint32_t m = adj[k];
+        const uint32_t next = k + 1 < e ? adj[k + 1] : kInvalid;
+        if (ab[m].x == i) man[m].avbd_next_a = next;
+        else man[m].avbd_next_b = next;
+    }
+    bodies[i] = b;
+}
+
+static std::mutex g_seam_mutex;
+static Solver *g_seam_solver = nullptr;
+
+}  // namespace pmh
+
+extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
+    void *stream, const PmhBodyParameters *parameters, const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds, const uint32_t *active_pairs,
+    const uint32_t *active_pair_count, PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
+    using namespace pmh;
+    if (!parameters || !previous || !states || !bodies || !schedule || !gravity || !(timestep > 0.0f) ||
+        body_count == 0 || profile_version > 1 || (!active_pair_count))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_seam_mutex);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint32_t M = 0;
+    PMH_CHECK(hipMemcpyAsync(&M, active_pair_count, sizeof M, hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    if (M && (!manifolds || !active_pairs)) return PMH_ERROR_INVALID_ARGUMENT;
+    if (!g_seam_solver) g_seam_solver = new Solver();
+    Solver &v = *g_seam_solver;
+    PmhResult r = v.reserve(body_count, std::max<uint32_t>(M, 1), std::max<uint32_t>(8 * M, 8));
+    if (r != PMH_SUCCESS) return r;
+    const uint32_t mg = std::max<uint32_t>(1, (M + 255) / 256), bg = (body_count + 255) / 256;
+    if (M) k_seam_counts<<<mg, 256, 0, s>>>(M, manifolds, v.m_count.p);
+    r = exclusive_scan(s, v.m_count.p, v.m_start.p, M, nullptr, nullptr, v.scan_tmp);
+    if (r != PMH_SUCCESS) return r;
+    k_seam_in<<<mg, 256, 0, s>>>(body_count, M, active_pairs, manifolds, v.m_start.p, v.code.p, v.ab.p, v.m_count.p,
+                                 v.m_init_rel.p, v.m_face.p, v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p,
+                                 v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p, v.c_pair.p, v.counters.p);
+    k_seam_history_in<<<bg, 256, 0, s>>>(body_count, bodies, v.prev_vel.p);
+    SolveParams sp{timestep, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations,
+                   profile_version, 0u};
+    r = avbd_prepare_solve(s, v, parameters, previous, states, sp, nullptr);
+    if (r != PMH_SUCCESS) return r;
+    if (M)
+        k_seam_out<<<mg, 256, 0, s>>>(M, manifolds, v.m_count.p, v.m_start.p, v.c_pt.p, v.c_aa.p, v.c_ab.p,
+                                      v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p);
+    k_seam_bodies_out<<<bg, 256, 0, s>>>(body_count, bodies, manifolds, v.ab.p, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
+                                         v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
+                                         v.impact.p, v.adj_start.p, v.adj.p);
+    PMH_CHECK(hipMemcpyAsync(schedule, v.schedule.p, sizeof(PmhContactSchedule), hipMemcpyDeviceToDevice, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    PMH_CHECK(hipGetLastError());
+    return PMH_SUCCESS;
+}

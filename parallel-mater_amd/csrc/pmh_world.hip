@@ -1,0 +1,645 @@
+// SPDX-License-Identifier: MIT
+// World API: shape upload preparation, device buffers, stepping and
+// result read-back in the reference's record layouts.
+#include "pmh_world.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <unordered_map>
+
+namespace pmh {
+
+static std::string g_last_error;
+static std::mutex g_error_mutex;
+
+PmhResult hip_result(hipError_t e, const char *what) {
+    {
+        std::lock_guard<std::mutex> lock(g_error_mutex);
+        g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    }
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return PMH_ERROR_OUT_OF_DEVICE_MEMORY;
+    if (e == hipErrorInvalidValue) return PMH_ERROR_INVALID_ARGUMENT;
+    return PMH_ERROR_DEVICE_LOST;
+}
+const std::string &last_error() { return g_last_error; }
+
+// ---------------------------------------------------------------------
+// Triangle-mesh upload preparation: validation, median-split BVH triangle
+// order (<= 4 triangles per leaf), convex face planes with an exact-dedup
+// tail, bounds, bounding sphere and box unit inertia, as
+// SimulationRuntime::add_triangle_mesh (runtime_deformables.cu.inc:4-304,
+// runtime_domains.cu.inc:1061-1108, src/convex_plane_dedup.hpp).
+// ---------------------------------------------------------------------
+static std::vector<std::array<float, 4>> convex_planes(const std::vector<V3> &v, const std::vector<uint32_t> &idx) {
+    std::map<std::array<float, 3>, uint32_t> weld;
+    std::vector<uint32_t> remap(v.size());
+    V3 c{0.0f, 0.0f, 0.0f};
+    for (size_t i = 0; i < v.size(); ++i) {
+        auto ins = weld.emplace(std::array<float, 3>{v[i].x, v[i].y, v[i].z}, uint32_t(weld.size()));
+        remap[i] = ins.first->second;
+        if (ins.second) c = add(c, v[i]);
+    }
+    if (weld.size() < 4) return {};
+    c = mul(c, 1.0f / float(weld.size()));
+    std::unordered_map<uint64_t, uint32_t> edge_uses;
+    for (size_t t = 0; t < idx.size(); t += 3)
+        for (size_t e = 0; e < 3; ++e) {
+            uint32_t a = remap[idx[t + e]], b = remap[idx[t + (e + 1) % 3]];
+            ++edge_uses[(uint64_t(std::min(a, b)) << 32) | std::max(a, b)];
+        }
+    for (const auto &e : edge_uses) if (e.second != 2) return {};   // open or non-manifold
+    float scale = 0.0f;
+    for (const V3 &p : v) scale = std::max(scale, len(sub(p, c)));
+    const float tol = std::max(1.0e-7f, scale * 1.0e-5f);
+    std::vector<std::array<float, 4>> out;
+    out.reserve(idx.size() / 3);
+    for (size_t t = 0; t < idx.size(); t += 3) {
+        const V3 a = v[idx[t]];
+        V3 nrm = normalized_or(cross(sub(v[idx[t + 1]], a), sub(v[idx[t + 2]], a)), {0.0f, 0.0f, 0.0f});
+        const float side = dot(nrm, sub(a, c));
+        if (std::fabs(side) <= tol) return {};
+        if (side < 0.0f) nrm = mul(nrm, -1.0f);
+        const float off = dot(nrm, a);
+        for (const V3 &p : v) if (dot(nrm, p) - off > tol) return {};   // not convex
+        out.push_back({nrm.x, nrm.y, nrm.z, off});
+    }
+    return out;
+}
+
+static PmhResult prepare_mesh(const float *vp, uint32_t nv, const uint32_t *ip, uint32_t ni, HostShape &sh) {
+    if (!vp || !ip || nv < 3 || ni < 3 || ni % 3) return PMH_ERROR_INVALID_ARGUMENT;
+    std::vector<V3> v(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        v[i] = {vp[3 * i], vp[3 * i + 1], vp[3 * i + 2]};
+        if (!std::isfinite(v[i].x) || !std::isfinite(v[i].y) || !std::isfinite(v[i].z))
+            return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    std::vector<uint32_t> in(ip, ip + ni);
+    V3 mn = v[0], mx = v[0];
+    for (const V3 &p : v) { mn = cmin(mn, p); mx = cmax(mx, p); }
+    for (uint32_t t = 0; t < ni; t += 3) {
+        if (in[t] >= nv || in[t + 1] >= nv || in[t + 2] >= nv) return PMH_ERROR_INVALID_ARGUMENT;
+        if (len2(cross(sub(v[in[t + 1]], v[in[t]]), sub(v[in[t + 2]], v[in[t]]))) <= kEpsilon * kEpsilon)
+            return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    const uint32_t tris = ni / 3;
+    std::vector<uint32_t> order(tris);
+    std::iota(order.begin(), order.end(), 0u);
+    auto centroid = [&](uint32_t t) {
+        return mul(add(add(v[in[3 * t]], v[in[3 * t + 1]]), v[in[3 * t + 2]]), 1.0f / 3.0f);
+    };
+    std::function<void(uint32_t, uint32_t)> split = [&](uint32_t b, uint32_t e) {
+        V3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        for (uint32_t k = b; k < e; ++k)
+            for (uint32_t c = 0; c < 3; ++c) { lo = cmin(lo, v[in[3 * order[k] + c]]); hi = cmax(hi, v[in[3 * order[k] + c]]); }
+        if (e - b <= 4) return;
+        const V3 ext = sub(hi, lo);
+        const int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
+        std::stable_sort(order.begin() + b, order.begin() + e, [&](uint32_t x, uint32_t y) {
+            const float fx = comp(centroid(x), axis), fy = comp(centroid(y), axis);
+            return fx < fy || (fx == fy && x < y);
+        });
+        const uint32_t mid = b + (e - b) / 2;
+        split(b, mid);
+        split(mid, e);
+    };
+    split(0, tris);
+    sh.indices.resize(ni);
+    for (uint32_t t = 0; t < tris; ++t)
+        for (uint32_t c = 0; c < 3; ++c) sh.indices[3 * t + c] = in[3 * order[t] + c];
+    sh.vertices = v;
+    auto planes = convex_planes(v, sh.indices);
+    sh.planes.clear();
+    sh.unique_planes = 0;
+    if (!planes.empty()) {
+        std::map<std::array<float, 4>, bool> seen;
+        std::vector<std::array<float, 4>> uniq;
+        for (const auto &p : planes) if (seen.emplace(p, true).second) uniq.push_back(p);
+        for (const auto &p : planes) sh.planes.insert(sh.planes.end(), p.begin(), p.end());
+        for (const auto &p : uniq) sh.planes.insert(sh.planes.end(), p.begin(), p.end());
+        sh.unique_planes = uint32_t(uniq.size());
+    }
+    ShapeDev &d = sh.info;
+    std::memset(&d, 0, sizeof d);
+    d.kind = kShapeMesh;
+    d.vtx_count = nv;
+    d.idx_count = ni;
+    d.has_planes = planes.empty() ? 0u : 1u;
+    d.convex_face = d.has_planes && ni <= 96u;
+    d.mn[0] = mn.x; d.mn[1] = mn.y; d.mn[2] = mn.z;
+    d.mx[0] = mx.x; d.mx[1] = mx.y; d.mx[2] = mx.z;
+    const V3 c = mul(add(mn, mx), 0.5f);
+    d.center[0] = c.x; d.center[1] = c.y; d.center[2] = c.z;
+    float r2 = 0.0f;
+    for (const V3 &p : v) r2 = std::fmax(r2, len2(sub(p, c)));
+    d.bradius = std::sqrt(r2);
+    const V3 h = mul(sub(mx, mn), 0.5f);
+    sh.unit_inertia[0] = std::fmax((h.y * h.y + h.z * h.z) / 3.0f, kEpsilon);
+    sh.unit_inertia[1] = std::fmax((h.x * h.x + h.z * h.z) / 3.0f, kEpsilon);
+    sh.unit_inertia[2] = std::fmax((h.x * h.x + h.y * h.y) / 3.0f, kEpsilon);
+    // A centred axis-aligned box also gets an analytic description so
+    // round shapes can collide with it in closed form.
+    bool box = d.has_planes && nv == 8 && c.x == 0.0f && c.y == 0.0f && c.z == 0.0f;
+    for (const V3 &p : v)
+        box = box && std::fabs(p.x) == h.x && std::fabs(p.y) == h.y && std::fabs(p.z) == h.z;
+    d.is_box = box ? 1u : 0u;
+    d.box_half[0] = h.x; d.box_half[1] = h.y; d.box_half[2] = h.z;
+    return PMH_SUCCESS;
+}
+
+static uint32_t next_pow2(uint32_t v) {
+    uint32_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+PmhResult World::ensure_device() {
+    if (hipSetDevice(device) != hipSuccess) return PMH_ERROR_DEVICE_LOST;
+    if (!stream) PMH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    if (!events[0])
+        for (auto &e : events) PMH_CHECK(hipEventCreate(&e));
+    if (!solve_timing.start) {
+        PMH_CHECK(hipEventCreate(&solve_timing.start));
+        PMH_CHECK(hipEventCreate(&solve_timing.stop));
+    }
+    PmhResult r;
+    if (shapes_dirty) {
+        std::vector<ShapeDev> info;
+        std::vector<V3> verts;
+        std::vector<uint32_t> idx;
+        std::vector<float4> planes;
+        for (HostShape &s : shapes) {
+            ShapeDev d = s.info;
+            d.vtx_off = uint32_t(verts.size());
+            d.idx_off = uint32_t(idx.size());
+            d.plane_off = uint32_t(planes.size());
+            verts.insert(verts.end(), s.vertices.begin(), s.vertices.end());
+            idx.insert(idx.end(), s.indices.begin(), s.indices.end());
+            for (size_t k = 0; k < s.planes.size(); k += 4)
+                planes.push_back(make_float4(s.planes[k], s.planes[k + 1], s.planes[k + 2], s.planes[k + 3]));
+            info.push_back(d);
+        }
+        if ((r = shapes_dev.reserve(info.size())) || (r = verts_dev.reserve(verts.size())) ||
+            (r = idx_dev.reserve(idx.size())) || (r = planes_dev.reserve(planes.size())))
+            return r;
+        if (!info.empty()) PMH_CHECK(hipMemcpy(shapes_dev.p, info.data(), info.size() * sizeof(ShapeDev), hipMemcpyHostToDevice));
+        if (!verts.empty()) PMH_CHECK(hipMemcpy(verts_dev.p, verts.data(), verts.size() * sizeof(V3), hipMemcpyHostToDevice));
+        if (!idx.empty()) PMH_CHECK(hipMemcpy(idx_dev.p, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!planes.empty()) PMH_CHECK(hipMemcpy(planes_dev.p, planes.data(), planes.size() * sizeof(float4), hipMemcpyHostToDevice));
+        shapes_dirty = false;
+    }
+    if (bodies_dirty) {
+        const uint32_t n = body_count;
+        if (uint64_t(n) * n > 0xffffffffull) return PMH_ERROR_CAPACITY_EXCEEDED;   // pair codes a*n+b
+        const uint32_t pair_cap = std::max<uint32_t>(64, n * pairs_per_body);
+        const uint32_t point_cap = std::max<uint32_t>(256, pair_cap * points_per_pair);
+        table_size = next_pow2(std::max<uint32_t>(1024, 2 * n));
+        const uint64_t cap_n = body_capacity ? body_capacity : n;
+        cache_capacity = uint32_t(std::min<uint64_t>(cap_n * (cap_n - 1) / 2, std::max<uint64_t>(4096, 8 * cap_n)));
+        cache_capacity = std::max<uint32_t>(1, cache_capacity);
+        if ((r = params.reserve(n)) || (r = state_cur.reserve(n)) || (r = state_prev.reserve(n)) ||
+            (r = state_next.reserve(n)) || (r = proxy_mn.reserve(n)) || (r = proxy_mx.reserve(n)) ||
+            (r = cell_of.reserve(n)) || (r = bucket_count.reserve(table_size + 1)) ||
+            (r = bucket_start.reserve(table_size + 1)) || (r = bucket_entries.reserve(n)) ||
+            (r = large.reserve(n)) || (r = cand.reserve(size_t(n) * pairs_per_body)) ||
+            (r = cand_count.reserve(n + 1)) || (r = cand_offset.reserve(n + 1)) ||
+            (r = slot_np.reserve(size_t(pair_cap) * 8)) || (r = slot_pt.reserve(size_t(pair_cap) * 8)) ||
+            (r = slot_ex.reserve(size_t(pair_cap) * 8)) || (r = cache_n.reserve(2)))
+            return r;
+        for (int k = 0; k < 2; ++k)
+            if ((r = cache_code[k].reserve(cache_capacity)) || (r = cache_count[k].reserve(cache_capacity)) ||
+                (r = cache_data[k].reserve(size_t(cache_capacity) * 8 * kCacheFloat4PerPoint)))
+                return r;
+        const bool grew = solver.n != n;
+        if ((r = solver.reserve(n, pair_cap, point_cap))) return r;
+        if (grew) PMH_CHECK(hipMemset(solver.prev_vel.p, 0, sizeof(float4) * n));
+        PMH_CHECK(hipMemcpy(params.p, host_params.data(), sizeof(PmhBodyParameters) * n, hipMemcpyHostToDevice));
+        PMH_CHECK(hipMemcpy(state_cur.p, host_states.data(), sizeof(PmhRigidBodyState) * n, hipMemcpyHostToDevice));
+        PMH_CHECK(hipMemset(cache_n.p, 0, sizeof(uint32_t) * 2));
+        bodies_dirty = false;
+    }
+    return PMH_SUCCESS;
+}
+
+void World::release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    shapes_dev.release(); verts_dev.release(); idx_dev.release(); planes_dev.release();
+    params.release(); state_cur.release(); state_prev.release(); state_next.release();
+    proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
+    bucket_start.release(); bucket_entries.release(); large.release(); cand.release();
+    cand_count.release(); cand_offset.release(); slot_np.release(); slot_pt.release(); slot_ex.release();
+    for (int k = 0; k < 2; ++k) { cache_code[k].release(); cache_count[k].release(); cache_data[k].release(); }
+    cache_n.release();
+    solver.release();
+    for (auto &e : events) if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    for (auto &e : solve_events) if (e) (void)hipEventDestroy(e);
+    solve_events.clear();
+    if (solve_timing.start) (void)hipEventDestroy(solve_timing.start);
+    if (solve_timing.stop) (void)hipEventDestroy(solve_timing.stop);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+}  // namespace pmh
+
+using pmh::World;
+
+static World *W(PmhWorld w) { return reinterpret_cast<World *>(w); }
+
+template <class T>
+static PmhResult fetch(std::vector<T> &out, const T *src, size_t count) {
+    out.resize(count);
+    if (count) PMH_CHECK(hipMemcpy(out.data(), src, sizeof(T) * count, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
+
+extern "C" {
+
+PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
+    if (!desc || !out || !(desc->timestep > 0.0f) || !std::isfinite(desc->timestep) || desc->profile_version > 1)
+        return PMH_ERROR_INVALID_ARGUMENT;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= int(desc->device)) return PMH_ERROR_FEATURE_NOT_PRESENT;
+    World *w = new (std::nothrow) World();
+    if (!w) return PMH_ERROR_OUT_OF_HOST_MEMORY;
+    w->gravity = {desc->gravity[0], desc->gravity[1], desc->gravity[2]};
+    w->dt = desc->timestep;
+    w->profile = desc->profile_version;
+    w->base_iterations = desc->base_iterations ? desc->base_iterations : 10;
+    w->requested_iterations = desc->requested_iterations;
+    w->body_capacity = desc->body_capacity;
+    w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 32;
+    w->points_per_pair = desc->points_per_pair ? desc->points_per_pair : 4;
+    w->staged = (desc->flags & PMH_WORLD_STAGED_SOLVE) != 0;
+    w->device = int(desc->device);
+    w->timing_enabled = true;
+    *out = reinterpret_cast<PmhWorld>(w);
+    return PMH_SUCCESS;
+}
+
+void pmhDestroyWorld(PmhWorld world) {
+    if (!world) return;
+    W(world)->release();
+    delete W(world);
+}
+
+PmhResult pmhWorldAddMesh(PmhWorld world, const float *vertices, uint32_t vertex_count, const uint32_t *indices,
+                          uint32_t index_count, uint32_t *shape) {
+    if (!world || !shape) return PMH_ERROR_INVALID_ARGUMENT;
+    pmh::HostShape s;
+    PmhResult r = pmh::prepare_mesh(vertices, vertex_count, indices, index_count, s);
+    if (r != PMH_SUCCESS) return r;
+    World *w = W(world);
+    w->shapes.push_back(std::move(s));
+    w->shapes_dirty = true;
+    *shape = uint32_t(w->shapes.size() - 1);
+    return PMH_SUCCESS;
+}
+
+static PmhResult add_round(World *w, uint32_t kind, float r, float h, uint32_t *shape) {
+    pmh::HostShape s;
+    pmh::ShapeDev &d = s.info;
+    std::memset(&d, 0, sizeof d);
+    d.kind = kind;
+    d.sphere_radius = r;
+    d.half_length = h;
+    d.mn[0] = -r; d.mn[1] = -(h + r); d.mn[2] = -r;
+    d.mx[0] = r; d.mx[1] = h + r; d.mx[2] = r;
+    d.bradius = h + r;
+    const float hy = h + r;
+    if (kind == pmh::kShapeSphere) {
+        s.unit_inertia[0] = s.unit_inertia[1] = s.unit_inertia[2] = 0.4f * r * r;
+    } else {
+        s.unit_inertia[0] = (hy * hy + r * r) / 3.0f;
+        s.unit_inertia[1] = (r * r + r * r) / 3.0f;
+        s.unit_inertia[2] = (r * r + hy * hy) / 3.0f;
+    }
+    w->shapes.push_back(s);
+    w->shapes_dirty = true;
+    *shape = uint32_t(w->shapes.size() - 1);
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldAddSphere(PmhWorld world, float radius, uint32_t *shape) {
+    if (!world || !shape || !(radius > 0.0f) || !std::isfinite(radius)) return PMH_ERROR_INVALID_ARGUMENT;
+    return add_round(W(world), pmh::kShapeSphere, radius, 0.0f, shape);
+}
+
+PmhResult pmhWorldAddCapsule(PmhWorld world, float radius, float half_length, uint32_t *shape) {
+    if (!world || !shape || !(radius > 0.0f) || !(half_length >= 0.0f) || !std::isfinite(radius + half_length))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    return add_round(W(world), pmh::kShapeCapsule, radius, half_length, shape);
+}
+
+PmhResult pmhWorldGetShapeInfo(PmhWorld world, uint32_t shape, float out[16]) {
+    if (!world || !out || shape >= W(world)->shapes.size()) return PMH_ERROR_INVALID_ARGUMENT;
+    const pmh::HostShape &s = W(world)->shapes[shape];
+    const pmh::ShapeDev &d = s.info;
+    const float v[16] = {d.mn[0], d.mn[1], d.mn[2], d.mx[0], d.mx[1], d.mx[2], d.center[0], d.center[1],
+                         d.center[2], d.bradius, s.unit_inertia[0], s.unit_inertia[1], s.unit_inertia[2],
+                         float(d.kind), float(d.idx_count), float(s.unique_planes)};
+    std::memcpy(out, v, sizeof v);
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldAddBodies(PmhWorld world, uint32_t count, const PmhBodyParameters *parameters,
+                            const PmhRigidBodyState *states) {
+    if (!world || (count && (!parameters || !states))) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    for (uint32_t i = 0; i < count; ++i)
+        if (parameters[i].mesh_index >= w->shapes.size() || parameters[i].motion > PMH_MOTION_DYNAMIC)
+            return PMH_ERROR_INVALID_ARGUMENT;
+    if (w->has_stepped) {
+        // Bring the device states back before the dense arrays grow.
+        std::vector<PmhRigidBodyState> cur(w->body_count);
+        PmhResult r = pmhWorldCopyStates(world, cur.data());
+        if (r != PMH_SUCCESS) return r;
+        w->host_states = cur;
+    }
+    w->host_params.insert(w->host_params.end(), parameters, parameters + count);
+    w->host_states.insert(w->host_states.end(), states, states + count);
+    w->body_count += count;
+    w->bodies_dirty = true;
+    return PMH_SUCCESS;
+}
+
+uint32_t pmhWorldBodyCount(PmhWorld world) { return world ? W(world)->body_count : 0; }
+
+PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
+    if (!world) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (w->body_count == 0 || steps == 0) return PMH_SUCCESS;
+    PmhResult r = w->ensure_device();
+    if (r != PMH_SUCCESS) return r;
+    // Steps are enqueued back to back; each step's solve launch is
+    // bracketed by its own event pair, read once the batch is done.
+    const uint32_t batch = 256;
+    for (uint32_t done = 0; done < steps; done += batch) {
+        const uint32_t count = std::min(batch, steps - done);
+        if (w->timing_enabled && w->solve_events.size() < 2 * size_t(count)) {
+            const size_t old = w->solve_events.size();
+            w->solve_events.resize(2 * size_t(count), nullptr);
+            for (size_t k = old; k < w->solve_events.size(); ++k) PMH_CHECK(hipEventCreate(&w->solve_events[k]));
+        }
+        for (uint32_t k = 0; k < count; ++k) {
+            r = w->step_once(k);
+            if (r != PMH_SUCCESS) return r;
+        }
+        PMH_CHECK(hipStreamSynchronize(w->stream));
+        if (w->timing_enabled) {
+            for (uint32_t k = 0; k < count; ++k) {
+                float ms = 0.0f;
+                PMH_CHECK(hipEventElapsedTime(&ms, w->solve_events[2 * k], w->solve_events[2 * k + 1]));
+                w->solve_timing.total_ms += ms;
+            }
+            w->last_solve_ms = 0.0f;
+            PMH_CHECK(hipEventElapsedTime(&w->last_solve_ms, w->solve_events[2 * (count - 1)],
+                                          w->solve_events[2 * (count - 1) + 1]));
+        }
+    }
+    w->has_stepped = true;
+    pmh::StepCounters c;
+    PMH_CHECK(hipMemcpy(&c, w->solver.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    if (c.status & (pmh::kStatusPairOverflow | pmh::kStatusPointOverflow | pmh::kStatusColorOverflow))
+        return PMH_ERROR_CAPACITY_EXCEEDED;
+    if (c.status & pmh::kStatusColorTimeout) return PMH_ERROR_INVALID_STATE;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldCopyStates(PmhWorld world, PmhRigidBodyState *states) {
+    if (!world || !states) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (!w->has_stepped || w->bodies_dirty) {
+        std::copy(w->host_states.begin(), w->host_states.end(), states);
+        return PMH_SUCCESS;
+    }
+    PMH_CHECK(hipStreamSynchronize(w->stream));
+    PMH_CHECK(hipMemcpy(states, w->state_cur.p, sizeof(PmhRigidBodyState) * w->body_count, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSetStates(PmhWorld world, const PmhRigidBodyState *states) {
+    if (!world || !states) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    w->host_states.assign(states, states + w->body_count);
+    if (w->has_stepped && !w->bodies_dirty) {
+        PMH_CHECK(hipMemcpy(w->state_cur.p, states, sizeof(PmhRigidBodyState) * w->body_count, hipMemcpyHostToDevice));
+        PMH_CHECK(hipMemset(w->cache_n.p, 0, sizeof(uint32_t) * 2));
+        PMH_CHECK(hipMemset(w->solver.prev_vel.p, 0, sizeof(float4) * w->body_count));
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetStats(PmhWorld world, PmhWorldStats *stats) {
+    if (!world || !stats) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    std::memset(stats, 0, sizeof *stats);
+    if (!w->has_stepped) return PMH_SUCCESS;
+    pmh::StepCounters c;
+    PMH_CHECK(hipMemcpy(&c, w->solver.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    PmhContactSchedule s;
+    PMH_CHECK(hipMemcpy(&s, w->solver.schedule.p, sizeof s, hipMemcpyDeviceToHost));
+    stats->pairs = c.pairs;
+    stats->manifolds = c.manifolds;
+    stats->points = c.points;
+    stats->colors = c.colors;
+    stats->islands = c.islands;
+    stats->impact = c.impact;
+    stats->budget = s.budget;
+    stats->unsupported_pairs = c.unsupported;
+    stats->cache_hits = c.cache_hits;
+    stats->phases = s.counts[7] != 0 ? 2 : 1;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetPhaseTimes(PmhWorld world, PmhPhaseTimes *times) {
+    if (!world || !times) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    std::memset(times, 0, sizeof *times);
+    if (!w->has_stepped) return PMH_SUCCESS;
+    for (int k = 0; k < PMH_PHASE_COUNT - 1; ++k)
+        PMH_CHECK(hipEventElapsedTime(&times->ms[k], w->events[k], w->events[k + 1]));
+    // Split the prepare+solve interval at the solve kernel's own events.
+    const float solve = w->last_solve_ms;
+    const float prep_solve = times->ms[4];
+    times->ms[4] = prep_solve - solve;
+    times->ms[6] = times->ms[5];
+    times->ms[5] = solve;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds, uint32_t *launches, int reset) {
+    if (!world) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (milliseconds) *milliseconds = w->solve_timing.total_ms;
+    if (launches) *launches = w->solve_timing.launches;
+    if (reset) { w->solve_timing.total_ms = 0.0f; w->solve_timing.launches = 0; }
+    return PMH_SUCCESS;
+}
+
+uint32_t pmhWorldPairCount(PmhWorld world) {
+    if (!world || !W(world)->has_stepped) return 0;
+    pmh::StepCounters c;
+    if (hipMemcpy(&c, W(world)->solver.counters.p, sizeof c, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    return c.pairs;
+}
+
+PmhResult pmhWorldGetPairs(PmhWorld world, uint32_t *codes) {
+    if (!world || !codes) return PMH_ERROR_INVALID_ARGUMENT;
+    const uint32_t m = pmhWorldPairCount(world);
+    if (m) PMH_CHECK(hipMemcpy(codes, W(world)->solver.code.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *out) {
+    if (!world || !out) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    pmh::Solver &v = w->solver;
+    pmh::StepCounters c;
+    PMH_CHECK(hipMemcpy(&c, v.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    const uint32_t M = c.pairs, P = c.points;
+    std::vector<uint32_t> count, start, slot, face;
+    std::vector<float4> init, np, pt, aa, ab, err, lam, pen, acc, rep;
+    std::vector<uint32_t> flags;
+    PmhResult r;
+    if ((r = fetch(count, v.m_count.p, M)) || (r = fetch(start, v.m_start.p, M)) || (r = fetch(slot, v.m_slot.p, M)) ||
+        (r = fetch(face, v.m_face.p, M)) || (r = fetch(init, v.m_init_rel.p, M)) || (r = fetch(np, v.c_np.p, P)) ||
+        (r = fetch(pt, v.c_pt.p, P)) || (r = fetch(aa, v.c_aa.p, P)) || (r = fetch(ab, v.c_ab.p, P)) ||
+        (r = fetch(err, v.c_err.p, P)) || (r = fetch(lam, v.c_lam.p, P)) || (r = fetch(pen, v.c_pen.p, P)) ||
+        (r = fetch(acc, v.c_acc.p, P)) || (r = fetch(rep, v.c_rep.p, P)) || (r = fetch(flags, v.c_flags.p, P)))
+        return r;
+    for (uint32_t m = 0; m < M; ++m) {
+        PmhContactManifold &o = out[m];
+        std::memset(&o, 0, sizeof o);
+        o.count = count[m];
+        o.initial_relative_position = {init[m].x, init[m].y, init[m].z};
+        o.face_patch = face[m];
+        o.response_slot = count[m] ? slot[m] : 0xffffffffu;
+        for (uint32_t k = 0; k < count[m] && k < 8; ++k) {
+            const uint32_t i = start[m] + k;
+            PmhContact &t = o.contacts[k];
+            t.normal = {np[i].x, np[i].y, np[i].z};
+            t.penetration = np[i].w;
+            t.point = {pt[i].x, pt[i].y, pt[i].z};
+            t.initial_normal_speed = pt[i].w;
+            t.avbd_anchor_a = {aa[i].x, aa[i].y, aa[i].z};
+            t.impact_fraction = aa[i].w;
+            t.avbd_anchor_b = {ab[i].x, ab[i].y, ab[i].z};
+            t.avbd_error = {err[i].x, err[i].y, err[i].z};
+            t.avbd_dual[0] = {lam[i].x, pen[i].x};
+            t.avbd_dual[1] = {lam[i].y, pen[i].y};
+            t.avbd_dual[2] = {lam[i].z, pen[i].z};
+            t.accumulated_normal_impulse = acc[i].x;
+            t.accumulated_friction_impulse = {acc[i].y, acc[i].z, acc[i].w};
+            t.reported_normal_impulse = rep[i].x;
+            t.reported_friction_impulse = {rep[i].y, rep[i].z, rep[i].w};
+            t.avbd_cached = flags[i] & 1u;
+            t.persistent = (flags[i] >> 1) & 1u;
+        }
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *out) {
+    if (!world || !out) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    pmh::Solver &v = w->solver;
+    const uint32_t n = w->body_count;
+    std::vector<float4> tpos, tq, tv, tw, disp, dq, inertia, pvel;
+    std::vector<uint32_t> color, parent, impact;
+    PmhResult r;
+    if ((r = fetch(tpos, v.tpos.p, n)) || (r = fetch(tq, v.tq.p, n)) || (r = fetch(tv, v.tv.p, n)) ||
+        (r = fetch(tw, v.tw.p, n)) || (r = fetch(disp, v.disp.p, 2 * size_t(n))) || (r = fetch(dq, v.dq.p, n)) ||
+        (r = fetch(inertia, v.inertia.p, 2 * size_t(n))) || (r = fetch(pvel, v.prev_vel.p, n)) ||
+        (r = fetch(color, v.color.p, n)) || (r = fetch(parent, v.parent.p, n)) || (r = fetch(impact, v.impact.p, n)))
+        return r;
+    for (uint32_t i = 0; i < n; ++i) {
+        PmhAvbdBody &b = out[i];
+        std::memset(&b, 0, sizeof b);
+        b.target.position = {tpos[i].x, tpos[i].y, tpos[i].z};
+        b.target.orientation = {tq[i].x, tq[i].y, tq[i].z, tq[i].w};
+        b.target.linear_velocity = {tv[i].x, tv[i].y, tv[i].z};
+        b.target.angular_velocity = {tw[i].x, tw[i].y, tw[i].z};
+        b.orientation_delta = {dq[i].x, dq[i].y, dq[i].z, dq[i].w};
+        const float d[6] = {disp[2 * i].x, disp[2 * i].y, disp[2 * i].z, disp[2 * i + 1].x, disp[2 * i + 1].y,
+                            disp[2 * i + 1].z};
+        std::memcpy(b.displacement, d, sizeof d);
+        const float4 i0 = inertia[2 * i], i1 = inertia[2 * i + 1];
+        b.inertia_h[0] = b.inertia_h[2] = b.inertia_h[5] = i0.x;
+        b.inertia_h[9] = i0.y; b.inertia_h[13] = i0.z; b.inertia_h[14] = i0.w;
+        b.inertia_h[18] = i1.x; b.inertia_h[19] = i1.y; b.inertia_h[20] = i1.z;
+        b.contact_head = b.joint_head = 0xffffffffu;
+        b.color = color[i];
+        b.parent = parent[i];
+        b.impact = impact[i];
+        b.previous_velocity = {pvel[i].x, pvel[i].y, pvel[i].z};
+        b.history_valid = pvel[i].w != 0.0f;
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetSchedule(PmhWorld world, PmhContactSchedule *schedule) {
+    if (!world || !schedule) return PMH_ERROR_INVALID_ARGUMENT;
+    PMH_CHECK(hipMemcpy(schedule, W(world)->solver.schedule.p, sizeof *schedule, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetProxies(PmhWorld world, float *bounds) {
+    if (!world || !bounds) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    const uint32_t n = w->body_count;
+    std::vector<float4> mn, mx;
+    PmhResult r;
+    if ((r = fetch(mn, w->proxy_mn.p, n)) || (r = fetch(mx, w->proxy_mx.p, n))) return r;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float v[6] = {mn[i].x, mn[i].y, mn[i].z, mx[i].x, mx[i].y, mx[i].z};
+        std::memcpy(bounds + 6 * i, v, sizeof v);
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhDeviceAlloc(uint64_t bytes, void **pointer) {
+    if (!pointer) return PMH_ERROR_INVALID_ARGUMENT;
+    PMH_CHECK(hipMalloc(pointer, bytes ? bytes : 1));
+    return PMH_SUCCESS;
+}
+PmhResult pmhDeviceFree(void *pointer) {
+    PMH_CHECK(hipFree(pointer));
+    return PMH_SUCCESS;
+}
+PmhResult pmhCopyToDevice(void *device, const void *host, uint64_t bytes) {
+    PMH_CHECK(hipMemcpy(device, host, bytes, hipMemcpyHostToDevice));
+    return PMH_SUCCESS;
+}
+PmhResult pmhCopyToHost(void *host, const void *device, uint64_t bytes) {
+    PMH_CHECK(hipMemcpy(host, device, bytes, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+PmhResult pmhDeviceSynchronize(void) {
+    PMH_CHECK(hipDeviceSynchronize());
+    return PMH_SUCCESS;
+}
+PmhResult pmhDeviceInfo(char *name, uint32_t name_bytes, uint32_t *compute_units) {
+    int dev = 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PMH_ERROR_FEATURE_NOT_PRESENT;
+    PMH_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    PMH_CHECK(hipGetDeviceProperties(&prop, dev));
+    if (name && name_bytes) {
+        std::strncpy(name, prop.gcnArchName, name_bytes - 1);
+        name[name_bytes - 1] = 0;
+    }
+    if (compute_units) *compute_units = uint32_t(prop.multiProcessorCount);
+    return PMH_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// SPDX-License-Identifier: MIT
+#pragma once
+
+#include "pmh_internal.hpp"
+
+namespace pmh {
+
+struct World {
+    // configuration
+    V3 gravity{0.0f, -9.81f, 0.0f};
+    float dt = 1.0f / 60.0f;
+    uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
+    uint32_t body_capacity = 0, pairs_per_body = 32, points_per_pair = 4;
+    bool staged = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // host-side scene
+    std::vector<HostShape> shapes;
+    std::vector<PmhBodyParameters> host_params;
+    std::vector<PmhRigidBodyState> host_states;
+    bool shapes_dirty = true, bodies_dirty = true;
+    uint32_t body_count = 0;
+
+    // device scene
+    DBuf<ShapeDev> shapes_dev;
+    DBuf<V3> verts_dev;
+    DBuf<uint32_t> idx_dev;
+    DBuf<float4> planes_dev;
+    DBuf<PmhBodyParameters> params;
+    DBuf<PmhRigidBodyState> state_cur, state_prev, state_next;
+
+    // broadphase
+    DBuf<float4> proxy_mn, proxy_mx;
+    DBuf<int4> cell_of;
+    uint32_t table_size = 0;
+    DBuf<uint32_t> bucket_count, bucket_start, bucket_entries, large, cand, cand_count, cand_offset;
+
+    // narrowphase slots (8 per pair)
+    DBuf<float4> slot_np, slot_pt, slot_ex;
+
+    // contact cache, double buffered
+    uint32_t cache_capacity = 0;
+    int cache_parity = 0;
+    DBuf<uint32_t> cache_code[2], cache_count[2];
+    DBuf<float4> cache_data[2];
+    DBuf<uint32_t> cache_n;
+
+    Solver solver;
+
+    // timing
+    bool timing_enabled = false;
+    hipEvent_t events[8] = {};
+    SolveTiming solve_timing;
+    std::vector<hipEvent_t> solve_events;   // start/stop per step of a batch
+    float last_solve_ms = 0.0f;
+    bool has_stepped = false;
+
+    PmhResult ensure_device();
+    PmhResult step_once(uint32_t k);
+    void release();
+};
+
+}  // namespace pmh

@@ -1,3 +1,289 @@
 # SPDX-License-Identifier: MIT
-"""Host-side mirror of the ParallelMater rigid step over the HIP driver."""
+"""Host-side mirror of the ParallelMater rigid step over the MI355X HIP
+driver (libpm_hip_rigid.so, C-ABI in include/pm_hip_rigid.h).
+
+`World` follows the reference's rigid command path: add meshes
+(pmCmdBuildGeometry), write bodies (pmCmdWriteRigidBodies), step
+(pmCmdStepSimulation), copy states (pmCmdCopyRigidStates).
+`prepare_solve` drives the kernel seam that stands where
+prepare_geometry_constraints/solve_geometry_constraints stand.
+There is no CPU fallback: every entry point raises when the HIP library or
+a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
 from . import layouts  # noqa: F401
+from .layouts import (AVBD_BODY, MANIFOLD, PARAMS, SCHEDULE, STATE)
+
+HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(HERE, "lib", "libpm_hip_rigid.so")
+
+PMH_SUCCESS = 0
+RESULT_NAMES = {0: "PM_SUCCESS", -1: "PM_ERROR_INVALID_ARGUMENT", -2: "PM_ERROR_INVALID_STATE",
+                -3: "PM_ERROR_OUT_OF_HOST_MEMORY", -4: "PM_ERROR_OUT_OF_DEVICE_MEMORY",
+                -5: "PM_ERROR_FEATURE_NOT_PRESENT", -7: "PM_ERROR_CAPACITY_EXCEEDED",
+                -8: "PM_ERROR_DEVICE_LOST"}
+STAGED_SOLVE = 1
+PHASES = ("integrate", "broadphase", "narrowphase", "contact_cache", "prepare", "solve", "finish")
+
+_P = ctypes.c_void_p
+_U = ctypes.c_uint32
+_F = ctypes.c_float
+
+
+class PmhError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {RESULT_NAMES.get(code, code)}")
+        self.code = code
+
+
+class WorldDesc(ctypes.Structure):
+    _fields_ = [("gravity", _F * 3), ("timestep", _F), ("profile_version", _U),
+                ("base_iterations", _U), ("requested_iterations", _U),
+                ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_pair", _U),
+                ("device", _U), ("flags", _U)]
+
+
+STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
+                "unsupported_pairs", "cache_hits", "phases"]
+
+
+class WorldStats(ctypes.Structure):
+    _fields_ = [(n, _U) for n in STATS_FIELDS]
+
+
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is missing: build it with `python parallel-mater_amd/build.py`")
+    L = ctypes.CDLL(LIB_PATH)
+    i32 = ctypes.c_int32
+    sig = {
+        "pmhCreateWorld": (i32, [_P, _P]),
+        "pmhDestroyWorld": (None, [_P]),
+        "pmhWorldAddMesh": (i32, [_P, _P, _U, _P, _U, _P]),
+        "pmhWorldAddSphere": (i32, [_P, _F, _P]),
+        "pmhWorldAddCapsule": (i32, [_P, _F, _F, _P]),
+        "pmhWorldGetShapeInfo": (i32, [_P, _U, _P]),
+        "pmhWorldAddBodies": (i32, [_P, _U, _P, _P]),
+        "pmhWorldBodyCount": (_U, [_P]),
+        "pmhWorldStep": (i32, [_P, _U]),
+        "pmhWorldCopyStates": (i32, [_P, _P]),
+        "pmhWorldSetStates": (i32, [_P, _P]),
+        "pmhWorldGetStats": (i32, [_P, _P]),
+        "pmhWorldGetPhaseTimes": (i32, [_P, _P]),
+        "pmhWorldSolveTiming": (i32, [_P, _P, _P, ctypes.c_int]),
+        "pmhWorldPairCount": (_U, [_P]),
+        "pmhWorldGetPairs": (i32, [_P, _P]),
+        "pmhWorldGetManifolds": (i32, [_P, _P]),
+        "pmhWorldGetAvbdBodies": (i32, [_P, _P]),
+        "pmhWorldGetSchedule": (i32, [_P, _P]),
+        "pmhWorldGetProxies": (i32, [_P, _P]),
+        "pmhPrepareSolveGeometryConstraints": (
+            i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
+        "pmhDeviceAlloc": (i32, [ctypes.c_uint64, _P]),
+        "pmhDeviceFree": (i32, [_P]),
+        "pmhCopyToDevice": (i32, [_P, _P, ctypes.c_uint64]),
+        "pmhCopyToHost": (i32, [_P, _P, ctypes.c_uint64]),
+        "pmhDeviceSynchronize": (i32, []),
+        "pmhDeviceInfo": (i32, [_P, _U, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code, what):
+    if code != PMH_SUCCESS:
+        raise PmhError(code, what)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(64)
+    cus = _U()
+    _check(lib().pmhDeviceInfo(name, 64, ctypes.byref(cus)), "pmhDeviceInfo")
+    return name.value.decode(), cus.value
+
+
+class World:
+    """One rigid simulation on one GPU (realtime v1 or legacy profile)."""
+
+    def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
+                 base_iterations=10, requested_iterations=0, body_capacity=0,
+                 pairs_per_body=0, points_per_pair=0, device=0, staged=False):
+        desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
+                         requested_iterations, body_capacity, pairs_per_body,
+                         points_per_pair, device, STAGED_SOLVE if staged else 0)
+        h = _P()
+        _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmhDestroyWorld(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_mesh(self, vertices, indices):
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.uint32)
+        out = _U()
+        _check(lib().pmhWorldAddMesh(self._h, _ptr(v), len(v), _ptr(i), len(i),
+                                     ctypes.byref(out)), "pmhWorldAddMesh")
+        return out.value
+
+    def add_sphere(self, radius):
+        out = _U()
+        _check(lib().pmhWorldAddSphere(self._h, radius, ctypes.byref(out)), "pmhWorldAddSphere")
+        return out.value
+
+    def add_capsule(self, radius, half_length):
+        out = _U()
+        _check(lib().pmhWorldAddCapsule(self._h, radius, half_length, ctypes.byref(out)),
+               "pmhWorldAddCapsule")
+        return out.value
+
+    def shape(self, index):
+        out = np.zeros(16, dtype=np.float32)
+        _check(lib().pmhWorldGetShapeInfo(self._h, index, _ptr(out)), "pmhWorldGetShapeInfo")
+        return out
+
+    def add_bodies(self, params, states):
+        p = np.ascontiguousarray(params, dtype=PARAMS)
+        s = np.ascontiguousarray(states, dtype=STATE)
+        _check(lib().pmhWorldAddBodies(self._h, len(p), _ptr(p), _ptr(s)), "pmhWorldAddBodies")
+
+    @property
+    def body_count(self):
+        return lib().pmhWorldBodyCount(self._h)
+
+    def step(self, steps=1):
+        _check(lib().pmhWorldStep(self._h, steps), "pmhWorldStep")
+
+    def states(self):
+        out = np.zeros(self.body_count, dtype=STATE)
+        _check(lib().pmhWorldCopyStates(self._h, _ptr(out)), "pmhWorldCopyStates")
+        return out
+
+    def set_states(self, states):
+        s = np.ascontiguousarray(states, dtype=STATE)
+        _check(lib().pmhWorldSetStates(self._h, _ptr(s)), "pmhWorldSetStates")
+
+    def stats(self):
+        s = WorldStats()
+        _check(lib().pmhWorldGetStats(self._h, ctypes.byref(s)), "pmhWorldGetStats")
+        return {n: getattr(s, n) for n in STATS_FIELDS}
+
+    def phase_times(self):
+        out = np.zeros(len(PHASES), dtype=np.float32)
+        _check(lib().pmhWorldGetPhaseTimes(self._h, _ptr(out)), "pmhWorldGetPhaseTimes")
+        return dict(zip(PHASES, map(float, out)))
+
+    def solve_timing(self, reset=False):
+        ms = _F()
+        launches = _U()
+        _check(lib().pmhWorldSolveTiming(self._h, ctypes.byref(ms), ctypes.byref(launches),
+                                         1 if reset else 0), "pmhWorldSolveTiming")
+        return ms.value, launches.value
+
+    def pairs(self):
+        out = np.zeros(lib().pmhWorldPairCount(self._h), dtype=np.uint32)
+        _check(lib().pmhWorldGetPairs(self._h, _ptr(out)), "pmhWorldGetPairs")
+        return out
+
+    def manifolds(self):
+        out = np.zeros(lib().pmhWorldPairCount(self._h), dtype=MANIFOLD)
+        _check(lib().pmhWorldGetManifolds(self._h, _ptr(out)), "pmhWorldGetManifolds")
+        return out
+
+    def bodies(self):
+        out = np.zeros(self.body_count, dtype=AVBD_BODY)
+        _check(lib().pmhWorldGetAvbdBodies(self._h, _ptr(out)), "pmhWorldGetAvbdBodies")
+        return out
+
+    def schedule(self):
+        out = np.zeros(1, dtype=SCHEDULE)
+        _check(lib().pmhWorldGetSchedule(self._h, _ptr(out)), "pmhWorldGetSchedule")
+        return out[0]
+
+    def proxies(self):
+        out = np.zeros((self.body_count, 6), dtype=np.float32)
+        _check(lib().pmhWorldGetProxies(self._h, _ptr(out)), "pmhWorldGetProxies")
+        return out
+
+
+class DeviceArray:
+    """A device copy of a numpy array (for the kernel seam)."""
+
+    def __init__(self, host):
+        self.host = np.ascontiguousarray(host)
+        self.nbytes = max(1, self.host.nbytes)
+        p = _P()
+        _check(lib().pmhDeviceAlloc(self.nbytes, ctypes.byref(p)), "pmhDeviceAlloc")
+        self.ptr = p
+        if self.host.nbytes:
+            _check(lib().pmhCopyToDevice(self.ptr, _ptr(self.host), self.host.nbytes), "pmhCopyToDevice")
+
+    def read(self):
+        out = np.empty_like(self.host)
+        if out.nbytes:
+            _check(lib().pmhCopyToHost(_ptr(out), self.ptr, out.nbytes), "pmhCopyToHost")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().pmhDeviceFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
+                  dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), base=10, requested=0,
+                  profile=1):
+    """Kernel seam on the GPU with host arrays: copies in, runs
+    pmhPrepareSolveGeometryConstraints, copies back in place (states,
+    manifolds, bodies) and returns the schedule record."""
+    n = len(params)
+    if bodies is None:
+        bodies = np.zeros(n, dtype=AVBD_BODY)
+    d_p = DeviceArray(np.ascontiguousarray(params, dtype=PARAMS))
+    d_prev = DeviceArray(np.ascontiguousarray(previous, dtype=STATE))
+    d_s = DeviceArray(np.ascontiguousarray(states, dtype=STATE))
+    d_m = DeviceArray(np.ascontiguousarray(manifolds, dtype=MANIFOLD))
+    d_pairs = DeviceArray(np.ascontiguousarray(pairs, dtype=np.uint32))
+    d_count = DeviceArray(np.array([len(pairs)], dtype=np.uint32))
+    d_b = DeviceArray(np.ascontiguousarray(bodies, dtype=AVBD_BODY))
+    d_sched = DeviceArray(np.zeros(1, dtype=SCHEDULE))
+    g = np.asarray(gravity, dtype=np.float32)
+    _check(lib().pmhPrepareSolveGeometryConstraints(
+        None, d_p.ptr, d_prev.ptr, d_s.ptr, n, d_m.ptr, d_pairs.ptr, d_count.ptr, d_b.ptr,
+        d_sched.ptr, dt, _ptr(g), base, requested, profile), "pmhPrepareSolveGeometryConstraints")
+    states[...] = d_s.read()
+    manifolds[...] = d_m.read()
+    bodies[...] = d_b.read()
+    return d_sched.read()[0]
