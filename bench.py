@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+# SPDX-License-Identifier: MIT
+"""Benchmark of the rigid AVBD step on one MI355X.
+
+Workload: a 32768-body pile (half boxes with edges U[0.2, 0.6] m, half
+analytic spheres with radius U[0.1, 0.3] m, densities log-uniform in
+[200, 5000] kg/m^3, friction U[0.3, 0.8]) dropped from a jittered 32^3
+grid into a static five-wall bin; dt = 1/60, legacy AVBD profile with 10
+iterations (pose phase and velocity phase), FP32.  W untimed steps, then
+exactly K timed steps (defaults 60 + 540 = 600 steps).
+
+value   = AVBD body-iterations per second of wall time over the K steps
+          (whole steps: predictor, broadphase, narrowphase, contact cache,
+          prepare, solve);
+roofline = the dominant kernel (the cooperative solve), algorithmic bytes
+          per launch over its HIP-event duration;
+cpu_baseline = the oracle (same algorithm, serial, one thread) on a bounded
+          sample of the same workload, body-iterations per second of its
+          solve phase.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("parallel-mater_amd", "tests"):
+    path = os.path.join(ROOT, sub)
+    if path not in sys.path:
+        sys.path.insert(0, path)
+
+METRIC = "AVBD body-iters/s @32k bodies; sim-steps/s to 1e-4 residual, MLP vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def traffic_from_profiles():
+    """HBM bytes per solve launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy-kernel calibration) when recorded."""
+    path = os.path.join(ROOT, "profiles", "solve_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(scene, states, seconds, profile, iterations):
+    import _oracle as O
+    w = scene.build(O.OracleWorld(profile=profile, base_iterations=iterations, lanes=0))
+    w.set_states(states)
+    w.timing(reset=True)
+    steps = 0
+    start = time.process_time()
+    while steps == 0 or time.process_time() - start < seconds:
+        w.step(1)
+        steps += 1
+    t = w.timing()
+    return {
+        "value": t["body_iterations"] / (t["solve_ms"] / 1e3),
+        "unit": "body-iters/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} steps of the same pile from the GPU's state at the timed window start "
+                  f"(cold contact cache), serial oracle, solve phase only; "
+                  f"{t['step_ms'] / steps:.1f} ms per whole step",
+        "steps_per_s": steps / (t["step_ms"] / 1e3),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=540)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--side", type=int, default=32)
+    ap.add_argument("--iterations", type=int, default=10)
+    ap.add_argument("--profile", type=int, default=0, help="0 legacy (10 iterations), 1 realtime v1")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--staged", action="store_true", help="one launch per colour instead of one cooperative solve")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
+              "multi-GPU sharding of this path is a design note, not a measured configuration.")
+        sys.exit(2)
+
+    import pmamd
+    import scenarios as S
+
+    scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
+    world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
+                                    staged=args.staged))
+    print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+    def advance(steps):
+        try:
+            world.step(steps)
+        except pmamd.PmhError:
+            print(f"# bench: step failed, stats {world.stats()}", file=sys.stderr, flush=True)
+            raise
+
+    if args.warmup:
+        advance(args.warmup)
+    pmamd.lib().pmhDeviceSynchronize()
+    start_states = world.states()
+    world.totals(reset=True)
+    world.solve_timing(reset=True)
+    pmamd.lib().pmhDeviceSynchronize()
+    t0 = time.perf_counter()
+    advance(args.steps)
+    pmamd.lib().pmhDeviceSynchronize()
+    wall = time.perf_counter() - t0
+    totals = world.totals()
+    solve_ms, launches = world.solve_timing()
+    stats = world.stats()
+    phases = world.phase_times()
+    body_iters = totals["body_iterations"]
+    launches = max(1, launches)
+    per_launch_bytes = totals["algorithmic_bytes"] / launches
+    per_launch_s = solve_ms / 1e3 / launches
+    achieved = per_launch_bytes / per_launch_s / 1e9
+    traffic = traffic_from_profiles()
+    name, cus = pmamd.device_info()
+    line = {
+        "metric": METRIC,
+        "value": body_iters / wall,
+        "unit": "body-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pile of {args.side ** 3} bodies (half boxes 0.2-0.6 m, half analytic spheres "
+                        f"r 0.1-0.3 m) dropped into a static 5-wall bin, dt 1/60, "
+                        f"{'legacy AVBD, ' + str(args.iterations) + ' iterations' if args.profile == 0 else 'realtime v1, 8 sweeps'}, "
+                        f"steps {args.warmup}..{args.warmup + args.steps}",
+            "bodies": scene.n,
+            "dynamic_bodies": args.side ** 3,
+            "iterations": args.iterations if args.profile == 0 else 8,
+            "profile": "legacy" if args.profile == 0 else "realtime-v1",
+            "device": f"{name} ({cus} CUs)",
+        },
+        "solve_phase_body_iters_per_s": body_iters / (solve_ms / 1e3),
+        "steps_per_s": args.steps / wall,
+        "solve_ms_per_step": solve_ms / args.steps,
+        "velocity_phases": totals["velocity_phases"],
+        "last_step": {"phase_ms": {k: round(v, 4) for k, v in phases.items()}, **stats},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_solve_all (one cooperative launch per step)",
+            "algorithmic_bytes_per_launch": per_launch_bytes,
+            "avg_launch_ms": per_launch_s * 1e3,
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("# bench: cpu baseline (oracle, one thread)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(scene, start_states, args.cpu_seconds, args.profile,
+                                            args.iterations)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

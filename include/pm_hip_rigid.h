@@ -144,8 +144,8 @@ typedef struct PmhWorldDesc {
     uint32_t base_iterations;        /* legacy profile iteration budget */
     uint32_t requested_iterations;   /* legacy override, 0 = base */
     uint32_t body_capacity;          /* contact-cache slot rule; 0 = bodies */
-    uint32_t pairs_per_body;         /* broadphase capacity, 0 = 32 */
-    uint32_t points_per_pair;        /* average contact capacity, 0 = 4 */
+    uint32_t pairs_per_body;         /* pair pool per body (average), 0 = 96 */
+    uint32_t points_per_body;        /* contact-point pool per body, 0 = 256 */
     uint32_t device;                 /* HIP device ordinal */
     uint32_t flags;                  /* PMH_WORLD_* */
 } PmhWorldDesc;
@@ -155,6 +155,8 @@ typedef struct PmhWorldDesc {
 typedef struct PmhWorldStats {
     uint32_t pairs, manifolds, points, colors, islands, impact, budget,
         unsupported_pairs, cache_hits, phases;
+    uint32_t status;             /* sticky overflow bits: 1 pairs of a body, 2 points, 4 colours */
+    uint32_t pair_capacity, point_capacity, points_allocated;
 } PmhWorldStats;
 
 /* Phase timings of the last step (milliseconds, HIP events on the
@@ -210,6 +212,18 @@ PmhResult pmhWorldGetPhaseTimes(PmhWorld world, PmhPhaseTimes *times);
  * and launch count of the solve phase since the last reset. */
 PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds,
                               uint32_t *launches, int reset);
+
+/* Work totals of the AVBD solve since the last reset: steps, body
+ * iterations (dynamic bodies x iterations of the pose phase plus impact
+ * bodies x iterations of the velocity phase), velocity phases run and
+ * algorithmic bytes (per iteration: 136 B per dynamic body, 192 B per
+ * contact point with one dynamic endpoint and 276 B with two, 168 B per
+ * manifold with one dynamic endpoint and 232 B with two). */
+typedef struct PmhWorldTotals {
+    uint64_t steps, body_iterations, velocity_phases;
+    double algorithmic_bytes;
+} PmhWorldTotals;
+PmhResult pmhWorldGetTotals(PmhWorld world, PmhWorldTotals *totals, int reset);
 
 /* Results of the last step in the reference layouts (for parity checks
  * and pmCmdCopyContactEvents-style consumers). */

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -1356,8 +1357,17 @@ void impact_stages(Ctx &x) {
     }
 }
 
-int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, uint32_t profile) {
+struct SolveClock {
+    double prepare_ms = 0.0, solve_ms = 0.0;
+    uint64_t body_iterations = 0;
+};
+
+int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, uint32_t profile,
+                  SolveClock *clock = nullptr) {
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
     prepare(x, gravity, base_iterations, requested, profile);
+    const auto t1 = clk::now();
     const bool realtime = x.S->offsets[0] == kRealtimeProfile;
     const uint32_t phases = realtime ? 1u : 2u;
     std::vector<uint32_t> by_color;
@@ -1379,6 +1389,17 @@ int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t request
         apply_states(x, phase, realtime);
         x.S->counts[phase == 0u ? 6u : 7u] = x.S->budget;
         x.S->maximum_passes += x.S->budget;
+        if (clock) {
+            uint64_t bodies = 0;
+            for (uint32_t i = 0; i < x.n; ++i)
+                bodies += dyn(x, i) && (phase == 0u || x.B[i].impact != 0u);
+            clock->body_iterations += bodies * x.S->budget;
+        }
+    }
+    if (clock) {
+        const auto t2 = clk::now();
+        clock->prepare_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        clock->solve_ms += std::chrono::duration<double, std::milli>(t2 - t1).count();
     }
     return 0;
 }
@@ -1409,6 +1430,8 @@ struct World {
     std::vector<Proxy> proxies;
     PmoSchedule schedule{};
     PmoWorldStats stats{};
+    SolveClock clock;
+    double step_ms = 0.0;
     std::map<uint32_t, CachedPair> cache;
     uint32_t cache_bodies = 0;
     uint64_t epoch = 0;
@@ -1623,7 +1646,7 @@ void world_step(World &w) {
     if (w.bodies.size() != n) w.bodies.assign(n, PmoAvbdBody{});
     Ctx x{w.params.data(), prev.data(), pred.data(), n, w.manifolds.data(), w.pairs.data(),
           uint32_t(w.pairs.size()), w.bodies.data(), &w.schedule, dt, w.desc.lanes};
-    prepare_solve(x, g, w.desc.base_iterations, w.desc.requested_iterations, w.desc.profile_version);
+    prepare_solve(x, g, w.desc.base_iterations, w.desc.requested_iterations, w.desc.profile_version, &w.clock);
     for (uint32_t i = 0; i < n; ++i) {                               // pm_clamp_rigid_speeds, :3086-3100
         if (w.params[i].motion != kDynamic) continue;
         pred[i].linear_velocity = clamp_length(pred[i].linear_velocity, w.params[i].maximum_linear_speed);
@@ -1762,8 +1785,18 @@ int pmo_world_step(void *world, uint32_t steps) {
     World &w = *static_cast<World *>(world);
     uint64_t n = w.params.size();
     if (n * n > 0xffffffffull) return -2;  // pair codes a*n+b must fit 32 bits
+    const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t s = 0; s < steps; ++s) world_step(w);
+    w.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
+}
+void pmo_world_timing(void *world, double out[4], int reset) {
+    World &w = *static_cast<World *>(world);
+    out[0] = w.step_ms;
+    out[1] = w.clock.prepare_ms;
+    out[2] = w.clock.solve_ms;
+    out[3] = double(w.clock.body_iterations);
+    if (reset) { w.step_ms = 0.0; w.clock = SolveClock{}; }
 }
 uint32_t pmo_world_body_count(void *world) { return uint32_t(static_cast<World *>(world)->params.size()); }
 void pmo_world_get_states(void *world, PmoState *out) {

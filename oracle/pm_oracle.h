@@ -193,6 +193,9 @@ int pmo_world_add_bodies(void *world, uint32_t count,
                          const PmoParams *parameters, const PmoState *states);
 int pmo_world_step(void *world, uint32_t steps);
 uint32_t pmo_world_body_count(void *world);
+/* Cumulative wall time of world steps, of AVBD prepare and of the AVBD
+ * solve (milliseconds), and body iterations solved; optionally reset. */
+void pmo_world_timing(void *world, double out[4], int reset);
 void pmo_world_get_states(void *world, PmoState *out);
 void pmo_world_set_states(void *world, const PmoState *states);
 void pmo_world_get_stats(void *world, PmoWorldStats *out);

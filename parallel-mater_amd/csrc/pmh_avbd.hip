@@ -289,15 +289,21 @@ __global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *
 __global__ void k_degree(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
                          const uint32_t *m_count, uint32_t *deg, uint32_t *anchored_edge, StepCounters *ctr) {
     const uint32_t M = *npairs;
-    uint32_t local = 0;
+    uint32_t local = 0, m1 = 0, m2 = 0, p1 = 0, p2 = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         if (m_count[m] == 0) continue;
         ++local;
         const uint2 p = ab[m];
-        if (P[p.x].motion == kDynamic) atomicAdd(&deg[p.x], 1u);
-        if (P[p.y].motion == kDynamic) atomicAdd(&deg[p.y], 1u);
+        const bool da = P[p.x].motion == kDynamic, db = P[p.y].motion == kDynamic;
+        if (da) atomicAdd(&deg[p.x], 1u);
+        if (db) atomicAdd(&deg[p.y], 1u);
+        if (da && db) { ++m2; p2 += m_count[m]; } else { ++m1; p1 += m_count[m]; }
     }
-    if (local) atomicAdd(&ctr->manifolds, local);
+    if (local) {
+        atomicAdd(&ctr->manifolds, local);
+        atomicAdd(&ctr->m1, m1); atomicAdd(&ctr->m2, m2);
+        atomicAdd(&ctr->p1, p1); atomicAdd(&ctr->p2, p2);
+    }
     (void)anchored_edge;
 }
 
@@ -387,8 +393,40 @@ __global__ void k_impact_bodies(uint32_t n, const PmhBodyParameters *P, const ui
     if (i >= n) return;
     if (P[i].motion != kDynamic) { impact[i] = 0; return; }
     const uint32_t r = parent[i];
-    if (r != i) impact[i] = ld_relaxed(&impact[r]);
+    const uint32_t flag = ld_relaxed(&impact[r]);
+    if (r != i) impact[i] = flag;
     if (r == i && deg[i] != 0) atomicAdd(&ctr->islands, 1u);
+    atomicAdd(&ctr->dyn_bodies, 1u);
+    if (flag) atomicAdd(&ctr->impact_bodies, 1u);
+}
+
+// Velocity-phase work: non-empty manifolds touching an impact body.
+__global__ void k_count_impact_pairs(const uint32_t *npairs, const PmhBodyParameters *P, const uint2 *ab,
+                                     const uint32_t *m_count, const uint32_t *impact, StepCounters *ctr) {
+    const uint32_t M = *npairs;
+    uint32_t m1 = 0, m2 = 0, p1 = 0, p2 = 0;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        if (m_count[m] == 0) continue;
+        const uint2 p = ab[m];
+        if (impact[p.x] == 0 && impact[p.y] == 0) continue;
+        const bool da = P[p.x].motion == kDynamic, db = P[p.y].motion == kDynamic;
+        if (da && db) { ++m2; p2 += m_count[m]; } else { ++m1; p1 += m_count[m]; }
+    }
+    if (m1 + m2) {
+        atomicAdd(&ctr->m1i, m1); atomicAdd(&ctr->m2i, m2);
+        atomicAdd(&ctr->p1i, p1); atomicAdd(&ctr->p2i, p2);
+    }
+}
+
+// Algorithmic bytes of one AVBD iteration: per dynamic body 136 B
+// (target pose, displacement, rotation delta and inertia read; displacement
+// and rotation delta written); per contact point 84 B read by each dynamic
+// endpoint in the primal and once in the dual, plus 24 B of dual written;
+// per manifold 12 B of pair record plus 52 B of the other body per dynamic
+// endpoint and 104 B for both bodies in the dual.
+__device__ __forceinline__ double iteration_bytes(uint32_t d, uint32_t m1, uint32_t m2, uint32_t p1, uint32_t p2) {
+    return 136.0 * d + (84.0 * 2 + 24.0) * p1 + (84.0 * 3 + 24.0) * p2 + (64.0 + 104.0) * m1 +
+           (128.0 + 104.0) * m2;
 }
 
 // First-fit colouring in index order.  One wavefront per body ticket;
@@ -554,6 +592,7 @@ struct SolveArgs {
     float4 *mean;
     PmhContactSchedule *schedule;
     StepCounters *ctr;
+    SolveTotals *totals;
     const uint32_t *npts, *npairs;
     float dt;
     uint32_t budget, realtime;
@@ -904,6 +943,20 @@ __device__ void impact_bodies(const SolveArgs &a, uint32_t tid, uint32_t nthread
     }
 }
 
+__device__ void account_phase(const SolveArgs &a, uint32_t phase) {
+    const StepCounters *c = a.ctr;
+    SolveTotals *t = a.totals;
+    if (phase == 0) {
+        t->steps += 1;
+        t->body_iterations += (unsigned long long)c->dyn_bodies * a.budget;
+        t->algorithmic_bytes += a.budget * iteration_bytes(c->dyn_bodies, c->m1, c->m2, c->p1, c->p2);
+    } else {
+        t->velocity_phases += 1;
+        t->body_iterations += (unsigned long long)c->impact_bodies * a.budget;
+        t->algorithmic_bytes += a.budget * iteration_bytes(c->impact_bodies, c->m1i, c->m2i, c->p1i, c->p2i);
+    }
+}
+
 __device__ __forceinline__ uint32_t global_wave() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 
 // The whole solve in one cooperative launch.
@@ -995,6 +1048,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
         if (tid == 0) {
             a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
             a.schedule->maximum_passes += a.budget;
+            account_phase(a, phase);
         }
     }
 }
@@ -1015,6 +1069,7 @@ __global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
         a.schedule->maximum_passes += a.budget;
+        account_phase(a, phase);
     }
 }
 
@@ -1032,6 +1087,9 @@ __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32
     ctr->islands = 0;
     ctr->impact = 0;
     ctr->color_ticket = 0;
+    ctr->dyn_bodies = ctr->impact_bodies = 0;
+    ctr->m1 = ctr->m2 = ctr->p1 = ctr->p2 = 0;
+    ctr->m1i = ctr->m2i = ctr->p1i = ctr->p2i = 0;
 }
 
 __global__ void k_schedule_finish(PmhContactSchedule *s, const StepCounters *ctr, uint32_t blocks) {
@@ -1082,6 +1140,10 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
         R(counters, 1);
         PMH_CHECK(hipMemset(counters.p, 0, sizeof(StepCounters)));
     }
+    if (!totals.p) {
+        R(totals, 1);
+        PMH_CHECK(hipMemset(totals.p, 0, sizeof(SolveTotals)));
+    }
 #undef R
     n = bodies;
     pair_cap = pairs;
@@ -1098,7 +1160,7 @@ void Solver::release() {
     m_init_rel.release(); adj.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
-    schedule.release(); color_count.release(); color_start.release(); counters.release();
+    schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
 }
 
@@ -1136,6 +1198,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     k_impact_roots<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_impact.p, v.m_count.p, v.parent.p, v.impact.p,
                                       v.anchored.p, ctr);
     k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, ctr);
+    if (!realtime)
+        k_count_impact_pairs<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.impact.p, ctr);
     const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
     k_color<<<color_blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, ctr);
     PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
@@ -1171,6 +1235,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.mean = v.mean.p;
     a.schedule = v.schedule.p;
     a.ctr = ctr;
+    a.totals = v.totals.p;
     a.npts = npts; a.npairs = npairs;
     a.dt = dt;
     a.budget = budget;

@@ -98,10 +98,7 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
     const V3 mx = add(cmax(sv(s0.position), sv(s.position)), e);
     pmn[i] = f4(mn, 0.0f);
     pmx[i] = f4(mx, 0.0f);
-    if (b.motion == kDynamic) {
-        const float ext = smax(mx.x - mn.x, smax(mx.y - mn.y, mx.z - mn.z));
-        if (isfinite(ext)) atomicMax(reinterpret_cast<uint32_t *>(&ctr->cell_size_bits), __float_as_uint(ext));
-    }
+    (void)ctr;
 }
 
 // ---------------------------------------------------------------------
@@ -115,17 +112,15 @@ __device__ __forceinline__ uint32_t cell_hash(int x, int y, int z) {
     return h;
 }
 
-__device__ __forceinline__ float grid_cell(const StepCounters *ctr) {
-    return ctr->cell_size_bits * 1.0625f + 1.0e-6f;
-}
-
 constexpr int kLargeCell = 0x7fffffff;
 
+// Proxies no wider than `cell` go into the hash grid under the cell of
+// their minimum corner; wider ones (floors, walls, fast movers) into the
+// large list.  Any overlapping gridded pair then lies in neighbouring cells.
 __global__ void k_bp_cells(uint32_t n, const float4 *pmn, const float4 *pmx, int4 *cell_of, uint32_t *bucket_count,
-                           uint32_t table_mask, uint32_t *large, StepCounters *ctr) {
+                           uint32_t table_mask, uint32_t *large, StepCounters *ctr, float cell) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float cell = grid_cell(ctr);
     const float4 mn = pmn[i], mx = pmx[i];
     const float ext = smax(mx.x - mn.x, smax(mx.y - mn.y, mx.z - mn.z));
     if (!(ext <= cell) || !isfinite(mn.x) || !isfinite(mn.y) || !isfinite(mn.z)) {
@@ -153,28 +148,35 @@ __device__ __forceinline__ bool overlap(float4 amn, float4 amx, float4 bmn, floa
            amx.z >= bmn.z;
 }
 
-// Kept candidates of dynamic body a, ascending (identity order = index).
-__global__ void k_bp_query(uint32_t n, uint32_t cap, const PmhBodyParameters *P, const float4 *pmn,
-                           const float4 *pmx, const int4 *cell_of, const uint32_t *bucket_start,
-                           const uint32_t *entries, uint32_t table_mask, const uint32_t *large,
-                           uint32_t *cand, uint32_t *cand_count, StepCounters *ctr) {
+// Kept candidates of dynamic body a.  Pass 0 counts them; pass 1 writes
+// them at the body's offset (from an exclusive scan of the counts) and
+// sorts the segment ascending (identity order = index), so a body may own
+// any number of pairs.
+template <int kPass>
+__global__ void k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 *pmn, const float4 *pmx,
+                           const int4 *cell_of, const uint32_t *bucket_start, const uint32_t *entries,
+                           uint32_t table_mask, const uint32_t *large, uint32_t *cand_count,
+                           const uint32_t *offset, uint32_t *code, uint2 *ab, const StepCounters *ctr) {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    if (P[a].motion != kDynamic) { cand_count[a] = 0; return; }
+    if (P[a].motion != kDynamic) {
+        if (kPass == 0) cand_count[a] = 0;
+        return;
+    }
+    if (kPass == 1 && ctr->pairs == 0) return;   // pair pool overflow: nothing is emitted
     const float4 amn = pmn[a], amx = pmx[a];
-    uint32_t *out = cand + (size_t)a * cap;
+    const uint32_t base = kPass == 1 ? offset[a] : 0;
     uint32_t count = 0;
-    bool overflow = false;
     auto consider = [&](uint32_t b) {
         if (b == a) return;
         if (P[b].motion == kDynamic && !(a < b)) return;
         if (!overlap(amn, amx, pmn[b], pmx[b])) return;
-        if (count < cap) out[count] = b;
-        else overflow = true;
+        if (kPass == 1) code[base + count] = b;
         ++count;
     };
     const int4 c = cell_of[a];
-    if (c.x != kLargeCell) {
+    if (c.x == kLargeCell) return;   // k_bp_large owns this body
+    {
         for (int dz = -1; dz <= 1; ++dz)
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
@@ -187,36 +189,79 @@ __global__ void k_bp_query(uint32_t n, uint32_t cap, const PmhBodyParameters *P,
                         consider(b);
                     }
                 }
-    } else {
-        // An oversized dynamic proxy (never the case when the grid cell is
-        // the largest dynamic extent, but non-finite proxies land here):
-        // test every body.
-        for (uint32_t b = 0; b < n; ++b) if (cell_of[b].x != kLargeCell) consider(b);
     }
     const uint32_t nl = ctr->large_count;
     for (uint32_t k = 0; k < nl; ++k) consider(large[k]);
-    if (overflow) {
-        atomicOr(&ctr->status, kStatusPairOverflow);
-        count = cap;
+    if (kPass == 0) {
+        cand_count[a] = count;
+        return;
     }
+    uint32_t *out = code + base;
     for (uint32_t k = 1; k < count; ++k) {
         const uint32_t v = out[k];
         uint32_t j = k;
         while (j > 0 && out[j - 1] > v) { out[j] = out[j - 1]; --j; }
         out[j] = v;
     }
-    cand_count[a] = count;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t b = out[k];
+        ab[base + k] = make_uint2(a, b);
+        out[k] = a * n + b;
+    }
 }
 
-__global__ void k_bp_emit(uint32_t n, uint32_t cap, const uint32_t *cand, const uint32_t *cand_count,
-                          const uint32_t *offset, uint32_t *code, uint2 *ab) {
-    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n) return;
-    const uint32_t o = offset[a];
-    for (uint32_t k = 0; k < cand_count[a]; ++k) {
-        const uint32_t b = cand[(size_t)a * cap + k];
-        code[o + k] = a * n + b;
-        ab[o + k] = make_uint2(a, b);
+__global__ void k_set_pairs(StepCounters *ctr, const uint32_t *offset, uint32_t n, uint32_t pair_cap) {
+    if (threadIdx.x != 0) return;
+    const uint32_t M = offset[n];
+    if (M > pair_cap) {
+        atomicOr(&ctr->status, kStatusPairOverflow);
+        ctr->pairs = 0;
+    } else {
+        ctr->pairs = M;
+    }
+}
+
+// Large bodies: one block scans every body in index order; a block-wide
+// prefix sum places the hits, so the segment comes out ascending.
+template <int kPass>
+__global__ void __launch_bounds__(256) k_bp_large(uint32_t n, const PmhBodyParameters *P, const float4 *pmn,
+                                                  const float4 *pmx, const uint32_t *large, uint32_t *cand_count,
+                                                  const uint32_t *offset, uint32_t *code, uint2 *ab,
+                                                  const StepCounters *ctr) {
+    __shared__ uint32_t warp_tot[4];
+    const uint32_t nl = ctr->large_count;
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint32_t k = blockIdx.x; k < nl; k += gridDim.x) {
+        const uint32_t a = large[k];
+        if (P[a].motion != kDynamic) {
+            if (kPass == 0 && threadIdx.x == 0) cand_count[a] = 0;
+            continue;
+        }
+        if (kPass == 1 && ctr->pairs == 0) continue;
+        const float4 amn = pmn[a], amx = pmx[a];
+        uint32_t running = kPass == 1 ? offset[a] : 0u;
+        for (uint32_t base = 0; base < n; base += 256) {
+            const uint32_t b = base + threadIdx.x;
+            bool hit = false;
+            if (b < n && b != a && !(P[b].motion == kDynamic && !(a < b)))
+                hit = overlap(amn, amx, pmn[b], pmx[b]);
+            const uint64_t bal = __ballot(hit);
+            if (lane == 0) warp_tot[wid] = __popcll(bal);
+            __syncthreads();
+            uint32_t before = 0, total = 0;
+            for (uint32_t w = 0; w < 4; ++w) {
+                if (w < wid) before += warp_tot[w];
+                total += warp_tot[w];
+            }
+            if (kPass == 1 && hit) {
+                const uint32_t pos = running + before + __popcll(bal & ((1ull << lane) - 1ull));
+                code[pos] = a * n + b;
+                ab[pos] = make_uint2(a, b);
+            }
+            running += total;
+            __syncthreads();
+        }
+        if (kPass == 0 && threadIdx.x == 0) cand_count[a] = running;
     }
 }
 
@@ -580,9 +625,11 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
                                                      const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
                                                      const float4 *planes, const PmhRigidBodyState *prev,
                                                      const PmhRigidBodyState *pred, const uint2 *ab,
-                                                     uint32_t *m_count, uint32_t *m_face, float4 *m_init_rel,
-                                                     float4 *slot_np, float4 *slot_pt, float4 *slot_ex,
-                                                     StepCounters *ctr) {
+                                                     uint32_t *m_count, uint32_t *m_start, uint32_t *m_face,
+                                                     float4 *m_init_rel, float4 *c_np, float4 *c_pt, float4 *c_aa,
+                                                     float4 *c_ab, float4 *c_err, float4 *c_lam, float4 *c_pen,
+                                                     float4 *c_acc, float4 *c_rep, uint32_t *c_flags,
+                                                     uint32_t *c_pair, uint32_t point_cap, StepCounters *ctr) {
     const uint32_t M = *npairs;
     uint32_t unsupported = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
@@ -612,7 +659,18 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
         }
         const auto sup = [](const ShapeDev &s) { return s.kind != kShapeMesh || (s.has_planes && s.idx_count <= 96u); };
         const bool persistent = sup(sa.s) && (pb.motion != kDynamic || sup(sb.s));
+        // Each manifold owns a contiguous range of the point arrays; where
+        // that range lies does not affect any result.
+        uint32_t start = 0;
+        if (man.count) {
+            start = atomicAdd(&ctr->point_alloc, man.count);
+            if (start + man.count > point_cap) {
+                atomicOr(&ctr->status, kStatusPointOverflow);
+                man.count = 0;
+            }
+        }
         m_count[m] = man.count;
+        m_start[m] = start;
         m_face[m] = man.face_patch;
         m_init_rel[m] = f4(sub(sv(A.position), sv(B.position)), 0.0f);
         for (uint32_t k = 0; k < man.count; ++k) {
@@ -621,10 +679,19 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
             const V3 av = add(sv(A.linear_velocity), cross(sv(A.angular_velocity), sub(c.point, sv(A.position))));
             const V3 bv = add(sv(B.linear_velocity), cross(sv(B.angular_velocity), sub(c.point, sv(B.position))));
             const float speed = dot(sub(av, bv), c.normal);
-            const size_t s = (size_t)m * 8 + k;
-            slot_np[s] = f4(c.normal, c.pen);
-            slot_pt[s] = f4(c.point, speed);
-            slot_ex[s] = make_float4(c.impact_fraction, persistent ? 1.0f : 0.0f, 0.0f, 0.0f);
+            const uint32_t i = start + k;
+            const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            c_np[i] = f4(c.normal, c.pen);
+            c_pt[i] = f4(c.point, speed);
+            c_aa[i] = make_float4(0.0f, 0.0f, 0.0f, c.impact_fraction);   // w: impact fraction
+            c_ab[i] = z;
+            c_err[i] = z;
+            c_lam[i] = z;
+            c_pen[i] = z;
+            c_acc[i] = z;
+            c_rep[i] = z;
+            c_flags[i] = persistent ? 2u : 0u;
+            c_pair[i] = m;
         }
     }
     if (unsupported) atomicAdd(&ctr->unsupported, unsupported);
@@ -636,42 +703,17 @@ __global__ void k_nonempty(const uint32_t *npairs, const uint32_t *m_count, uint
         flag[m] = m_count[m] != 0u ? 1u : 0u;
 }
 
-__global__ void k_guard_points(StepCounters *ctr, uint32_t point_cap, const uint32_t *m_start) {
+// An overflowing step solves nothing (its manifolds are incomplete) and is
+// reported as PM_ERROR_CAPACITY_EXCEEDED.
+__global__ void k_guard_points(StepCounters *ctr, uint32_t point_cap) {
     if (threadIdx.x != 0) return;
-    const uint32_t P = m_start[ctr->pairs];
+    const uint32_t P = ctr->point_alloc;
     if (P > point_cap) {
         atomicOr(&ctr->status, kStatusPointOverflow);
         ctr->points = 0;
         ctr->pairs = 0;
     } else {
         ctr->points = P;
-    }
-}
-
-__global__ void k_points_compact(const uint32_t *npairs, const uint32_t *m_count, const uint32_t *m_start,
-                                 const float4 *slot_np, const float4 *slot_pt, const float4 *slot_ex,
-                                 float4 *c_np, float4 *c_pt, float4 *c_aa, float4 *c_ab, float4 *c_err,
-                                 float4 *c_lam, float4 *c_pen, float4 *c_acc, float4 *c_rep, uint32_t *c_flags,
-                                 uint32_t *c_pair) {
-    const uint32_t M = *npairs;
-    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        const uint32_t start = m_start[m];
-        for (uint32_t k = 0; k < m_count[m]; ++k) {
-            const size_t s = (size_t)m * 8 + k;
-            const uint32_t c = start + k;
-            const float4 ex = slot_ex[s];
-            c_np[c] = slot_np[s];
-            c_pt[c] = slot_pt[s];
-            c_aa[c] = make_float4(0.0f, 0.0f, 0.0f, ex.x);   // w: impact fraction
-            c_ab[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_err[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_lam[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_pen[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_acc[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_rep[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_flags[c] = ex.y != 0.0f ? 2u : 0u;
-            c_pair[c] = m;
-        }
     }
 }
 
@@ -785,11 +827,8 @@ __global__ void k_step_begin(StepCounters *ctr) {
     ctr->cache_hits = 0;
     ctr->large_count = 0;
     ctr->color_ticket = 0;
+    ctr->point_alloc = 0;
     ctr->cell_size_bits = 0.0f;
-}
-
-__global__ void k_set_pairs(StepCounters *ctr, const uint32_t *offset, uint32_t n) {
-    if (threadIdx.x == 0) ctr->pairs = offset[n];
 }
 
 static uint32_t blocks_for(uint32_t items, uint32_t block = 256, uint32_t cap = 4096) {
@@ -814,29 +853,31 @@ PmhResult World::step_once(uint32_t k_step) {
     // Broadphase.
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * (table_size + 1), s));
     k_bp_cells<<<bg, 256, 0, s>>>(n, proxy_mn.p, proxy_mx.p, cell_of.p, bucket_count.p, table_size - 1,
-                                  large.p, ctr);
+                                  large.p, ctr, grid_cell);
     PmhResult r = exclusive_scan(s, bucket_count.p, bucket_start.p, table_size, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * table_size, s));
     k_bp_scatter<<<bg, 256, 0, s>>>(n, cell_of.p, bucket_start.p, bucket_count.p, bucket_entries.p);
-    k_bp_query<<<blocks_for(n, 64), 64, 0, s>>>(n, pairs_per_body, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
-                                               bucket_start.p, bucket_entries.p, table_size - 1, large.p,
-                                               cand.p, cand_count.p, ctr);
+    k_bp_query<0><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
+                                                  bucket_start.p, bucket_entries.p, table_size - 1, large.p,
+                                                  cand_count.p, nullptr, nullptr, nullptr, ctr);
+    k_bp_large<0><<<64, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, nullptr, nullptr,
+                                     nullptr, ctr);
     r = exclusive_scan(s, cand_count.p, cand_offset.p, n, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
-    k_set_pairs<<<1, 64, 0, s>>>(ctr, cand_offset.p, n);
-    k_bp_emit<<<bg, 256, 0, s>>>(n, pairs_per_body, cand.p, cand_count.p, cand_offset.p, v.code.p, v.ab.p);
+    k_set_pairs<<<1, 64, 0, s>>>(ctr, cand_offset.p, n, v.pair_cap);
+    k_bp_large<1><<<64, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, cand_offset.p,
+                                     v.code.p, v.ab.p, ctr);
+    k_bp_query<1><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
+                                                  bucket_start.p, bucket_entries.p, table_size - 1, large.p,
+                                                  cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr);
     mark(2);
     // Narrowphase + point compaction.
     k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
         &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p,
-        v.m_count.p, v.m_face.p, v.m_init_rel.p, slot_np.p, slot_pt.p, slot_ex.p, ctr);
-    r = exclusive_scan(s, v.m_count.p, v.m_start.p, v.pair_cap, &ctr->pairs, nullptr, v.scan_tmp);
-    if (r != PMH_SUCCESS) return r;
-    k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap, v.m_start.p);
-    k_points_compact<<<pg, 256, 0, s>>>(&ctr->pairs, v.m_count.p, v.m_start.p, slot_np.p, slot_pt.p, slot_ex.p,
-                                        v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p,
-                                        v.c_acc.p, v.c_rep.p, v.c_flags.p, v.c_pair.p);
+        v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p,
+        v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p, v.c_pair.p, v.point_cap, ctr);
+    k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap);
     mark(3);
     // Response slots and the contact cache.
     k_nonempty<<<pg, 256, 0, s>>>(&ctr->pairs, v.m_count.p, v.m_impact.p);

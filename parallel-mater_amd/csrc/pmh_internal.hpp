@@ -75,7 +75,20 @@ struct StepCounters {
     uint32_t color_ticket;     // work ticket of the colouring kernel
     float cell_size_bits;      // broadphase cell (as float)
     uint32_t barrier_count, barrier_gen;
-    uint32_t pad[17];
+    // Work of the step's solve: dynamic bodies, impact bodies, non-empty
+    // manifolds and their points by number of dynamic endpoints (1 or 2),
+    // over all pairs (phase 0) and over pairs touching an impact body
+    // (velocity phase).
+    uint32_t dyn_bodies, impact_bodies;
+    uint32_t m1, m2, p1, p2, m1i, m2i, p1i, p2i;
+    uint32_t point_alloc;      // narrowphase point-range allocator
+    uint32_t pad[6];
+};
+
+// Running totals over every step since the last reset.
+struct SolveTotals {
+    unsigned long long steps, body_iterations, velocity_phases;
+    double algorithmic_bytes;
 };
 
 constexpr uint32_t kMaxColors = 256;
@@ -125,6 +138,7 @@ struct Solver {
     DBuf<PmhContactSchedule> schedule;
     DBuf<uint32_t> color_count, color_start;
     DBuf<StepCounters> counters;
+    DBuf<SolveTotals> totals;
     DBuf<uint32_t> scan_tmp;
     PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
     void release();

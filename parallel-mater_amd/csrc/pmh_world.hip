@@ -198,9 +198,29 @@ PmhResult World::ensure_device() {
     if (bodies_dirty) {
         const uint32_t n = body_count;
         if (uint64_t(n) * n > 0xffffffffull) return PMH_ERROR_CAPACITY_EXCEEDED;   // pair codes a*n+b
-        const uint32_t pair_cap = std::max<uint32_t>(64, n * pairs_per_body);
-        const uint32_t point_cap = std::max<uint32_t>(256, pair_cap * points_per_pair);
+        // Per-body candidate lists hold up to pairs_per_body entries; the
+        // pair and point pools are sized by their realistic totals.
+        const uint32_t pair_cap = std::max<uint32_t>(256, n * pairs_per_body);
+        const uint32_t point_cap = std::max<uint32_t>(1024, n * points_per_body);
         table_size = next_pow2(std::max<uint32_t>(1024, 2 * n));
+        // Grid cell: 1.25 x the 95th percentile of the dynamic bodies' proxy
+        // width at rest (bounding sphere, margin, epsilon and the bounded
+        // solver correction on both sides); wider proxies go to the large list.
+        std::vector<float> widths;
+        for (uint32_t i = 0; i < n; ++i) {
+            const PmhBodyParameters &b = host_params[i];
+            if (b.motion != PMH_MOTION_DYNAMIC) continue;
+            const ShapeDev &d = shapes[b.mesh_index].info;
+            const V3 e = sub(V3{d.mx[0], d.mx[1], d.mx[2]}, V3{d.mn[0], d.mn[1], d.mn[2]});
+            const float corr = std::min(0.5f, 0.25f * std::max(0.0f, std::min(e.x, std::min(e.y, e.z))));
+            const float c = std::sqrt(d.center[0] * d.center[0] + d.center[1] * d.center[1] + d.center[2] * d.center[2]);
+            widths.push_back(2.0f * (c + d.bradius + b.collision_margin + 1.0e-5f + std::min(1.0f, corr)));
+        }
+        if (!widths.empty()) {
+            const size_t k = std::min(widths.size() - 1, size_t(0.95 * double(widths.size())));
+            std::nth_element(widths.begin(), widths.begin() + k, widths.end());
+            grid_cell = 1.25f * widths[k];
+        }
         const uint64_t cap_n = body_capacity ? body_capacity : n;
         cache_capacity = uint32_t(std::min<uint64_t>(cap_n * (cap_n - 1) / 2, std::max<uint64_t>(4096, 8 * cap_n)));
         cache_capacity = std::max<uint32_t>(1, cache_capacity);
@@ -208,10 +228,9 @@ PmhResult World::ensure_device() {
             (r = state_next.reserve(n)) || (r = proxy_mn.reserve(n)) || (r = proxy_mx.reserve(n)) ||
             (r = cell_of.reserve(n)) || (r = bucket_count.reserve(table_size + 1)) ||
             (r = bucket_start.reserve(table_size + 1)) || (r = bucket_entries.reserve(n)) ||
-            (r = large.reserve(n)) || (r = cand.reserve(size_t(n) * pairs_per_body)) ||
+            (r = large.reserve(n)) ||
             (r = cand_count.reserve(n + 1)) || (r = cand_offset.reserve(n + 1)) ||
-            (r = slot_np.reserve(size_t(pair_cap) * 8)) || (r = slot_pt.reserve(size_t(pair_cap) * 8)) ||
-            (r = slot_ex.reserve(size_t(pair_cap) * 8)) || (r = cache_n.reserve(2)))
+            (r = cache_n.reserve(2)))
             return r;
         for (int k = 0; k < 2; ++k)
             if ((r = cache_code[k].reserve(cache_capacity)) || (r = cache_count[k].reserve(cache_capacity)) ||
@@ -233,8 +252,8 @@ void World::release() {
     shapes_dev.release(); verts_dev.release(); idx_dev.release(); planes_dev.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
-    bucket_start.release(); bucket_entries.release(); large.release(); cand.release();
-    cand_count.release(); cand_offset.release(); slot_np.release(); slot_pt.release(); slot_ex.release();
+    bucket_start.release(); bucket_entries.release(); large.release();
+    cand_count.release(); cand_offset.release();
     for (int k = 0; k < 2; ++k) { cache_code[k].release(); cache_count[k].release(); cache_data[k].release(); }
     cache_n.release();
     solver.release();
@@ -276,8 +295,8 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     w->base_iterations = desc->base_iterations ? desc->base_iterations : 10;
     w->requested_iterations = desc->requested_iterations;
     w->body_capacity = desc->body_capacity;
-    w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 32;
-    w->points_per_pair = desc->points_per_pair ? desc->points_per_pair : 4;
+    w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 96;
+    w->points_per_body = desc->points_per_body ? desc->points_per_body : 256;
     w->staged = (desc->flags & PMH_WORLD_STAGED_SOLVE) != 0;
     w->device = int(desc->device);
     w->timing_enabled = true;
@@ -457,6 +476,10 @@ PmhResult pmhWorldGetStats(PmhWorld world, PmhWorldStats *stats) {
     stats->unsupported_pairs = c.unsupported;
     stats->cache_hits = c.cache_hits;
     stats->phases = s.counts[7] != 0 ? 2 : 1;
+    stats->status = c.status;
+    stats->pair_capacity = w->solver.pair_cap;
+    stats->point_capacity = w->solver.point_cap;
+    stats->points_allocated = c.point_alloc;
     return PMH_SUCCESS;
 }
 
@@ -482,6 +505,22 @@ PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds, uint32_t *lau
     if (milliseconds) *milliseconds = w->solve_timing.total_ms;
     if (launches) *launches = w->solve_timing.launches;
     if (reset) { w->solve_timing.total_ms = 0.0f; w->solve_timing.launches = 0; }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetTotals(PmhWorld world, PmhWorldTotals *totals, int reset) {
+    if (!world || !totals) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    std::memset(totals, 0, sizeof *totals);
+    if (!w->solver.totals.p) return PMH_SUCCESS;
+    PMH_CHECK(hipStreamSynchronize(w->stream));
+    pmh::SolveTotals t;
+    PMH_CHECK(hipMemcpy(&t, w->solver.totals.p, sizeof t, hipMemcpyDeviceToHost));
+    totals->steps = t.steps;
+    totals->body_iterations = t.body_iterations;
+    totals->velocity_phases = t.velocity_phases;
+    totals->algorithmic_bytes = t.algorithmic_bytes;
+    if (reset) PMH_CHECK(hipMemset(w->solver.totals.p, 0, sizeof t));
     return PMH_SUCCESS;
 }
 

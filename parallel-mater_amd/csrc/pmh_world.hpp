@@ -10,7 +10,7 @@ struct World {
     V3 gravity{0.0f, -9.81f, 0.0f};
     float dt = 1.0f / 60.0f;
     uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
-    uint32_t body_capacity = 0, pairs_per_body = 32, points_per_pair = 4;
+    uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
     bool staged = false;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -34,10 +34,8 @@ struct World {
     DBuf<float4> proxy_mn, proxy_mx;
     DBuf<int4> cell_of;
     uint32_t table_size = 0;
-    DBuf<uint32_t> bucket_count, bucket_start, bucket_entries, large, cand, cand_count, cand_offset;
-
-    // narrowphase slots (8 per pair)
-    DBuf<float4> slot_np, slot_pt, slot_ex;
+    float grid_cell = 1.0f;
+    DBuf<uint32_t> bucket_count, bucket_start, bucket_entries, large, cand_count, cand_offset;
 
     // contact cache, double buffered
     uint32_t cache_capacity = 0;

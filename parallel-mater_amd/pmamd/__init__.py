@@ -43,16 +43,22 @@ class PmhError(RuntimeError):
 class WorldDesc(ctypes.Structure):
     _fields_ = [("gravity", _F * 3), ("timestep", _F), ("profile_version", _U),
                 ("base_iterations", _U), ("requested_iterations", _U),
-                ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_pair", _U),
+                ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_body", _U),
                 ("device", _U), ("flags", _U)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
-                "unsupported_pairs", "cache_hits", "phases"]
+                "unsupported_pairs", "cache_hits", "phases", "status", "pair_capacity",
+                "point_capacity", "points_allocated"]
 
 
 class WorldStats(ctypes.Structure):
     _fields_ = [(n, _U) for n in STATS_FIELDS]
+
+
+class WorldTotals(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_uint64), ("body_iterations", ctypes.c_uint64),
+                ("velocity_phases", ctypes.c_uint64), ("algorithmic_bytes", ctypes.c_double)]
 
 
 _lib = None
@@ -87,6 +93,7 @@ def lib():
         "pmhWorldGetStats": (i32, [_P, _P]),
         "pmhWorldGetPhaseTimes": (i32, [_P, _P]),
         "pmhWorldSolveTiming": (i32, [_P, _P, _P, ctypes.c_int]),
+        "pmhWorldGetTotals": (i32, [_P, _P, ctypes.c_int]),
         "pmhWorldPairCount": (_U, [_P]),
         "pmhWorldGetPairs": (i32, [_P, _P]),
         "pmhWorldGetManifolds": (i32, [_P, _P]),
@@ -131,10 +138,10 @@ class World:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
-                 pairs_per_body=0, points_per_pair=0, device=0, staged=False):
+                 pairs_per_body=0, points_per_body=0, device=0, staged=False):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
-                         points_per_pair, device, STAGED_SOLVE if staged else 0)
+                         points_per_body, device, STAGED_SOLVE if staged else 0)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h
@@ -208,6 +215,12 @@ class World:
         _check(lib().pmhWorldSolveTiming(self._h, ctypes.byref(ms), ctypes.byref(launches),
                                          1 if reset else 0), "pmhWorldSolveTiming")
         return ms.value, launches.value
+
+    def totals(self, reset=False):
+        t = WorldTotals()
+        _check(lib().pmhWorldGetTotals(self._h, ctypes.byref(t), 1 if reset else 0),
+               "pmhWorldGetTotals")
+        return {f: getattr(t, f) for f, _ in WorldTotals._fields_}
 
     def pairs(self):
         out = np.zeros(lib().pmhWorldPairCount(self._h), dtype=np.uint32)

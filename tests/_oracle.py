@@ -59,6 +59,7 @@ def lib():
         _lib.pmo_world_step.argtypes = [_P, _U]
         _lib.pmo_world_body_count.argtypes = [_P]
         _lib.pmo_world_body_count.restype = _U
+        _lib.pmo_world_timing.argtypes = [_P, _P, ctypes.c_int]
         _lib.pmo_world_pair_count.argtypes = [_P]
         _lib.pmo_world_pair_count.restype = _U
         for name in ("pmo_world_get_states", "pmo_world_set_states",
@@ -146,6 +147,12 @@ class OracleWorld:
         out = np.zeros(self.body_count, dtype=L.STATE)
         lib().pmo_world_get_states(self._h, _ptr(out))
         return out
+
+    def timing(self, reset=False):
+        out = np.zeros(4, dtype=np.float64)
+        lib().pmo_world_timing(self._h, _ptr(out), 1 if reset else 0)
+        return {"step_ms": out[0], "prepare_ms": out[1], "solve_ms": out[2],
+                "body_iterations": int(out[3])}
 
     def set_states(self, states):
         s = np.ascontiguousarray(states, dtype=L.STATE)
