@@ -12,8 +12,10 @@ exactly K timed steps (defaults 60 + 540 = 600 steps).
 value   = AVBD body-iterations per second of wall time over the K steps
           (whole steps: predictor, broadphase, narrowphase, contact cache,
           prepare, solve);
-roofline = the dominant kernel (the cooperative solve), algorithmic bytes
-          per launch over its HIP-event duration;
+roofline = the dominant kernel group, the AVBD solve phase of a step (per
+          iteration one primal kernel per colour and one dual kernel, for the
+          pose and velocity phases), algorithmic bytes per step over the
+          HIP-event time between the phase's first and last launch;
 cpu_baseline = the oracle (same algorithm, serial, one thread) on a bounded
           sample of the same workload, body-iterations per second of its
           solve phase.
@@ -78,7 +80,7 @@ def main():
     ap.add_argument("--profile", type=int, default=0, help="0 legacy (10 iterations), 1 realtime v1")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--staged", action="store_true", help="one launch per colour instead of one cooperative solve")
+    ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -90,7 +92,7 @@ def main():
 
     scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
-                                    staged=args.staged))
+                                    cooperative=args.cooperative))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     def advance(steps):
         try:
@@ -115,9 +117,11 @@ def main():
     stats = world.stats()
     phases = world.phase_times()
     body_iters = totals["body_iterations"]
-    launches = max(1, launches)
-    per_launch_bytes = totals["algorithmic_bytes"] / launches
-    per_launch_s = solve_ms / 1e3 / launches
+    # One unit per step: the whole solve phase (a single launch when
+    # cooperative, a sequence of stream-ordered launches otherwise).
+    units = max(1, totals["steps"])
+    per_launch_bytes = totals["algorithmic_bytes"] / units
+    per_launch_s = solve_ms / 1e3 / units
     achieved = per_launch_bytes / per_launch_s / 1e9
     traffic = traffic_from_profiles()
     name, cus = pmamd.device_info()
@@ -149,6 +153,7 @@ def main():
         "steps_per_s": args.steps / wall,
         "solve_ms_per_step": solve_ms / args.steps,
         "velocity_phases": totals["velocity_phases"],
+        "solve_launches_per_step": launches / max(1, args.steps),
         "last_step": {"phase_ms": {k: round(v, 4) for k, v in phases.items()}, **stats},
         "roofline": {
             "bound": "hbm",
@@ -157,7 +162,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_solve_all (one cooperative launch per step)",
+            "kernel": ("k_solve_all, one cooperative launch per step" if args.cooperative else
+                       "solve phase per step: k_stage_color per colour and k_stage_duals per iteration"),
             "algorithmic_bytes_per_launch": per_launch_bytes,
             "avg_launch_ms": per_launch_s * 1e3,
         },

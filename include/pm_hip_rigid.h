@@ -150,7 +150,7 @@ typedef struct PmhWorldDesc {
     uint32_t flags;                  /* PMH_WORLD_* */
 } PmhWorldDesc;
 
-#define PMH_WORLD_STAGED_SOLVE 1u     /* one launch per colour (no grid barrier) */
+#define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */
 
 typedef struct PmhWorldStats {
     uint32_t pairs, manifolds, points, colors, islands, impact, budget,

@@ -943,6 +943,71 @@ __device__ void impact_bodies(const SolveArgs &a, uint32_t tid, uint32_t nthread
     }
 }
 
+// Velocity-level errors from the reset targets, with restitution for cold
+// contacts closing faster than 1 m/s (impact_constraints, solve.slang:
+// 293-329; rows as rigid_avbd_contact_rows).
+__device__ void velocity_errors(const SolveArgs &a, const uint32_t *c_flags, uint32_t tid, uint32_t nthreads) {
+    const uint32_t Pn = *a.npts;
+    for (uint32_t c = tid; c < Pn; c += nthreads) {
+        const uint2 p = a.ab[a.c_pair[c]];
+        if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+        const PmhRigidBodyState A = a.states[p.x], B = a.states[p.y];
+        const float dt = a.dt;
+        const float va[6] = {A.linear_velocity.x * dt, A.linear_velocity.y * dt, A.linear_velocity.z * dt,
+                             A.angular_velocity.x * dt, A.angular_velocity.y * dt, A.angular_velocity.z * dt};
+        const float vb[6] = {B.linear_velocity.x * dt, B.linear_velocity.y * dt, B.linear_velocity.z * dt,
+                             B.angular_velocity.x * dt, B.angular_velocity.y * dt, B.angular_velocity.z * dt};
+        const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
+        V3 t0, t1;
+        contact_basis(n, t0, t1);
+        const V3 axes[3] = {n, t0, t1};
+        const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
+        const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
+        float out[3];
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
+            const V3 la = mul(axes[ax], -1.0f), aa = mul(cross(ra, axes[ax]), -1.0f), ba = cross(rb, axes[ax]);
+            const float ja[6] = {la.x, la.y, la.z, aa.x, aa.y, aa.z};
+            const float jb[6] = {axes[ax].x, axes[ax].y, axes[ax].z, ba.x, ba.y, ba.z};
+            out[ax] = dot6(ja, va) + dot6(jb, vb);
+        }
+        const float gap = a.c_err[c].x;
+        const bool cached = (c_flags[c] & 1u) != 0u;
+        const float speed = a.c_pt[c].w;
+        const float rest = !cached && speed < -1.0f ? smin(a.P[p.x].restitution, a.P[p.y].restitution) : 0.0f;
+        a.c_err[c] = make_float4(out[0] - gap - rest * speed * dt, out[1], out[2], 0.0f);
+    }
+}
+
+// Jacobians of the velocity phase from the reset targets, with the
+// accumulated impulses of the pose phase as reference forces.
+__device__ void velocity_jacobians(const SolveArgs &a, uint32_t tid, uint32_t nthreads) {
+    const uint32_t Pn = *a.npts;
+    for (uint32_t c = tid; c < Pn; c += nthreads) {
+        const uint2 p = a.ab[a.c_pair[c]];
+        if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+        const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
+        V3 t0, t1;
+        contact_basis(n, t0, t1);
+        const V3 axes[3] = {n, t0, t1};
+        const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
+        const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
+        const float4 e = a.c_err[c];
+        const float e0[3] = {e.x, e.y, e.z};
+        const float4 acc = a.c_acc[c];
+        const V3 af = {acc.y, acc.z, acc.w};
+        const float ref[3] = {acc.x / a.dt, dot(af, t0) / a.dt, dot(af, t1) / a.dt};
+        float4 *J = const_cast<float4 *>(a.jac) + 9 * (size_t)c;
+        const float mu = J[6].w, beta = J[7].w;
+        for (uint32_t ax = 0; ax < 3; ++ax) {
+            const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
+            J[ax] = f4(axes[ax], e0[ax]);
+            J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
+            J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
+        }
+    }
+}
+
 __device__ void account_phase(const SolveArgs &a, uint32_t phase) {
     const StepCounters *c = a.ctr;
     SolveTotals *t = a.totals;
@@ -974,63 +1039,9 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
             grid.sync();
             impact_bodies(a, tid, nthreads);
             grid.sync();
-            // restitution of cold contacts enters through the velocity error
-            const uint32_t Pn = *a.npts;
-            for (uint32_t c = tid; c < Pn; c += nthreads) {
-                const uint2 p = a.ab[a.c_pair[c]];
-                if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
-                const PmhRigidBodyState A = a.states[p.x], B = a.states[p.y];
-                const float dt = a.dt;
-                const float va[6] = {A.linear_velocity.x * dt, A.linear_velocity.y * dt, A.linear_velocity.z * dt,
-                                     A.angular_velocity.x * dt, A.angular_velocity.y * dt, A.angular_velocity.z * dt};
-                const float vb[6] = {B.linear_velocity.x * dt, B.linear_velocity.y * dt, B.linear_velocity.z * dt,
-                                     B.angular_velocity.x * dt, B.angular_velocity.y * dt, B.angular_velocity.z * dt};
-                const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
-                V3 t0, t1;
-                contact_basis(n, t0, t1);
-                const V3 axes[3] = {n, t0, t1};
-                const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
-                const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
-                float out[3];
-                for (uint32_t ax = 0; ax < 3; ++ax) {
-                    const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
-                    const V3 la = mul(axes[ax], -1.0f), aa = mul(cross(ra, axes[ax]), -1.0f), ba = cross(rb, axes[ax]);
-                    const float ja[6] = {la.x, la.y, la.z, aa.x, aa.y, aa.z};
-                    const float jb[6] = {axes[ax].x, axes[ax].y, axes[ax].z, ba.x, ba.y, ba.z};
-                    out[ax] = dot6(ja, va) + dot6(jb, vb);
-                }
-                const float gap = a.c_err[c].x;
-                const bool cached = (c_flags[c] & 1u) != 0u;
-                const float speed = a.c_pt[c].w;
-                const float rest = !cached && speed < -1.0f
-                    ? smin(a.P[p.x].restitution, a.P[p.y].restitution) : 0.0f;
-                a.c_err[c] = make_float4(out[0] - gap - rest * speed * dt, out[1], out[2], 0.0f);
-            }
+            velocity_errors(a, c_flags, tid, nthreads);
             grid.sync();
-            // Jacobians from the reset targets.
-            for (uint32_t c = tid; c < Pn; c += nthreads) {
-                const uint2 p = a.ab[a.c_pair[c]];
-                if (a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
-                const V3 n = v3(a.c_np[c]), point = v3(a.c_pt[c]);
-                V3 t0, t1;
-                contact_basis(n, t0, t1);
-                const V3 axes[3] = {n, t0, t1};
-                const V3 armA = sub(point, v3(a.tpos[p.x])), armB = sub(point, v3(a.tpos[p.y]));
-                const V3 ra_t = rotate(q4(a.tq[p.x]), v3(a.c_aa[c])), rb_t = rotate(q4(a.tq[p.y]), v3(a.c_ab[c]));
-                const float4 e = a.c_err[c];
-                const float e0[3] = {e.x, e.y, e.z};
-                const float4 acc = a.c_acc[c];
-                const V3 af = {acc.y, acc.z, acc.w};
-                const float ref[3] = {acc.x / a.dt, dot(af, t0) / a.dt, dot(af, t1) / a.dt};
-                float4 *J = const_cast<float4 *>(a.jac) + 9 * (size_t)c;
-                const float mu = J[6].w, beta = J[7].w;
-                for (uint32_t ax = 0; ax < 3; ++ax) {
-                    const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
-                    J[ax] = f4(axes[ax], e0[ax]);
-                    J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
-                    J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
-                }
-            }
+            velocity_jacobians(a, tid, nthreads);
             grid.sync();
         }
         for (uint32_t it = 0; it < a.budget; ++it) {
@@ -1060,6 +1071,18 @@ __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t color
 }
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
     advance_duals(a, phase, it, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_impact_gaps(SolveArgs a) {
+    impact_gaps(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_impact_bodies(SolveArgs a) {
+    impact_bodies(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_velocity_errors(SolveArgs a, const uint32_t *c_flags) {
+    velocity_errors(a, c_flags, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+__global__ void k_stage_velocity_jacobians(SolveArgs a) {
+    velocity_jacobians(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 __global__ void k_stage_mean(SolveArgs a, uint32_t phase) {
     free_mean(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1162,6 +1185,10 @@ void Solver::release() {
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
+    if (host_ctr) (void)hipHostFree(host_ctr);
+    if (host_color_start) (void)hipHostFree(host_color_start);
+    host_ctr = nullptr;
+    host_color_start = nullptr;
 }
 
 static uint32_t g_coop_blocks = 0;
@@ -1241,8 +1268,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.budget = budget;
     a.realtime = realtime ? 1u : 0u;
 
-    if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
-    if (!sp.staged) {
+    if (sp.cooperative) {
+        if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         if (!g_coop_blocks) {
             int per_cu = 0;
             PMH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solve_all, 256, 0));
@@ -1258,23 +1285,47 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         if (timing) timing->launches += 1;
         k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, blocks);
     } else {
-        // Host-known schedule: read the colour count back.
-        StepCounters host;
-        PMH_CHECK(hipMemcpyAsync(&host, ctr, sizeof host, hipMemcpyDeviceToHost, s));
+        // One launch per stage: a kernel boundary on the stream is a far
+        // cheaper grid-wide barrier than an in-kernel one.  The colour
+        // schedule comes back to the host once per step.
+        if (!v.host_ctr) {
+            PMH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&v.host_ctr), sizeof(StepCounters)));
+            PMH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&v.host_color_start), sizeof(uint32_t) * (kMaxColors + 1)));
+        }
+        PMH_CHECK(hipMemcpyAsync(v.host_ctr, ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
+        PMH_CHECK(hipMemcpyAsync(v.host_color_start, v.color_start.p, sizeof(uint32_t) * (kMaxColors + 1),
+                                 hipMemcpyDeviceToHost, s));
         PMH_CHECK(hipStreamSynchronize(s));
-        const uint32_t blocks = 2 * device_compute_units();
-        const uint32_t phases = realtime ? 1u : 2u;
+        const StepCounters host = *v.host_ctr;
+        if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
+        const uint32_t colors = std::min(host.colors, kMaxColors);
+        const uint32_t pts_blocks = std::max<uint32_t>(1, (host.points + 255) / 256);
+        const uint32_t phases = realtime ? 1u : (host.impact ? 2u : 1u);
+        uint32_t launches = 0;
         for (uint32_t phase = 0; phase < phases; ++phase) {
-            if (phase != 0) return PMH_ERROR_FEATURE_NOT_PRESENT;  // staged legacy velocity phase: not built
+            if (phase != 0) {
+                k_stage_impact_gaps<<<pts_blocks, 256, 0, s>>>(a);
+                k_stage_impact_bodies<<<bg, 256, 0, s>>>(a);
+                k_stage_velocity_errors<<<pts_blocks, 256, 0, s>>>(a, v.c_flags.p);
+                k_stage_velocity_jacobians<<<pts_blocks, 256, 0, s>>>(a);
+                launches += 4;
+            }
             for (uint32_t it = 0; it < budget; ++it) {
-                for (uint32_t c = 0; c < host.colors; ++c) k_stage_color<<<blocks, 256, 0, s>>>(a, c, phase);
-                k_stage_duals<<<cg_, 256, 0, s>>>(a, phase, it);
-                if (timing) timing->launches += host.colors + 1;
+                for (uint32_t c = 0; c < colors; ++c) {
+                    const uint32_t bodies = v.host_color_start[c + 1] - v.host_color_start[c];
+                    const uint32_t waves = (bodies + (64 / kLanes) - 1) / (64 / kLanes);
+                    const uint32_t blocks = std::max<uint32_t>(1, (waves + 3) / 4);
+                    k_stage_color<<<blocks, 256, 0, s>>>(a, c, phase);
+                }
+                k_stage_duals<<<pts_blocks, 256, 0, s>>>(a, phase, it);
+                launches += colors + 1;
             }
             k_stage_mean<<<bg, 256, 0, s>>>(a, phase);
             k_stage_apply<<<bg, 256, 0, s>>>(a, phase);
+            launches += 2;
         }
-        k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, blocks);
+        if (timing) timing->launches += launches;
+        k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, 0);
     }
     if (timing && timing->stop) PMH_CHECK(hipEventRecord(timing->stop, s));
     PMH_CHECK(hipGetLastError());

@@ -140,6 +140,8 @@ struct Solver {
     DBuf<StepCounters> counters;
     DBuf<SolveTotals> totals;
     DBuf<uint32_t> scan_tmp;
+    StepCounters *host_ctr = nullptr;        // pinned read-back of the schedule
+    uint32_t *host_color_start = nullptr;
     PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
     void release();
 };
@@ -148,7 +150,7 @@ struct SolveParams {
     float dt;
     V3 gravity;
     uint32_t base_iterations, requested_iterations, profile;
-    uint32_t staged;                     // per-colour launches instead of one cooperative kernel
+    uint32_t cooperative;                // one cooperative launch with grid barriers instead of per-stage launches
 };
 
 struct SolveTiming {

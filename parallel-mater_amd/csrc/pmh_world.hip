@@ -297,7 +297,7 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     w->body_capacity = desc->body_capacity;
     w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 96;
     w->points_per_body = desc->points_per_body ? desc->points_per_body : 256;
-    w->staged = (desc->flags & PMH_WORLD_STAGED_SOLVE) != 0;
+    w->cooperative = (desc->flags & PMH_WORLD_COOPERATIVE_SOLVE) != 0;
     w->device = int(desc->device);
     w->timing_enabled = true;
     *out = reinterpret_cast<PmhWorld>(w);

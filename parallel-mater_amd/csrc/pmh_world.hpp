@@ -11,7 +11,7 @@ struct World {
     float dt = 1.0f / 60.0f;
     uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
     uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
-    bool staged = false;
+    bool cooperative = false;
     int device = 0;
     hipStream_t stream = nullptr;
 

@@ -26,7 +26,7 @@ RESULT_NAMES = {0: "PM_SUCCESS", -1: "PM_ERROR_INVALID_ARGUMENT", -2: "PM_ERROR_
                 -3: "PM_ERROR_OUT_OF_HOST_MEMORY", -4: "PM_ERROR_OUT_OF_DEVICE_MEMORY",
                 -5: "PM_ERROR_FEATURE_NOT_PRESENT", -7: "PM_ERROR_CAPACITY_EXCEEDED",
                 -8: "PM_ERROR_DEVICE_LOST"}
-STAGED_SOLVE = 1
+COOPERATIVE_SOLVE = 1
 PHASES = ("integrate", "broadphase", "narrowphase", "contact_cache", "prepare", "solve", "finish")
 
 _P = ctypes.c_void_p
@@ -138,10 +138,10 @@ class World:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
-                 pairs_per_body=0, points_per_body=0, device=0, staged=False):
+                 pairs_per_body=0, points_per_body=0, device=0, cooperative=False):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
-                         points_per_body, device, STAGED_SOLVE if staged else 0)
+                         points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h
