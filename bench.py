@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per body in the colour sweep (8, 16, 32, 64; 0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
@@ -92,7 +93,7 @@ def main():
 
     scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
-                                    cooperative=args.cooperative))
+                                    cooperative=args.cooperative, solve_lanes=args.lanes))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     def advance(steps):
         try:
@@ -147,6 +148,7 @@ def main():
             "dynamic_bodies": args.side ** 3,
             "iterations": args.iterations if args.profile == 0 else 8,
             "profile": "legacy" if args.profile == 0 else "realtime-v1",
+            "solve_lanes": args.lanes or 32,
             "device": f"{name} ({cus} CUs)",
         },
         "solve_phase_body_iters_per_s": body_iters / (solve_ms / 1e3),

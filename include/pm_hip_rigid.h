@@ -148,6 +148,8 @@ typedef struct PmhWorldDesc {
     uint32_t points_per_body;        /* contact-point pool per body, 0 = 256 */
     uint32_t device;                 /* HIP device ordinal */
     uint32_t flags;                  /* PMH_WORLD_* */
+    uint32_t solve_lanes;            /* lanes per body in the colour sweep: 8, 16, 32, 64; 0 = 32.
+                                        Sets the order in which a body's contact rows are summed. */
 } PmhWorldDesc;
 
 #define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */

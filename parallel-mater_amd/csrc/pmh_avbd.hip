@@ -35,7 +35,6 @@ constexpr float kRealtimeDamping = 0.05f;
 constexpr float kSurfaceTolerance = 1.0e-5f;
 constexpr float kRestOffset = 1.0e-3f;
 constexpr uint32_t kSideFirst = 0x80000000u;
-constexpr int kLanes = 8;   // lanes per body in the colour solve
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -474,11 +473,21 @@ __global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParamete
     }
 }
 
+// Wave-aggregated histogram: one atomic per distinct colour per wavefront
+// (a few colours, tens of thousands of bodies).
 __global__ void k_color_count(uint32_t n, const uint32_t *color, uint32_t *count) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t c = color[i];
-    if (c != kInvalid) atomicAdd(&count[c], 1u);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t c = i < n ? color[i] : kInvalid;
+    bool pending = c != kInvalid;
+    while (__ballot(pending) != 0ull) {
+        const uint64_t live = __ballot(pending);
+        const uint32_t leader = __builtin_ctzll(live);
+        const uint32_t c0 = __shfl(c, leader, 64);
+        const uint64_t same = __ballot(pending && c == c0);
+        if (lane == leader) atomicAdd(&count[c0], uint32_t(__popcll(same)));
+        if (pending && c == c0) pending = false;
+    }
 }
 
 __global__ void k_color_scan(uint32_t *count, uint32_t *start, const StepCounters *ctr) {
@@ -491,13 +500,27 @@ __global__ void k_color_scan(uint32_t *count, uint32_t *start, const StepCounter
     }
 }
 
+// Order within a colour does not affect results (its bodies share no
+// contact); one atomic per distinct colour per wavefront places them.
 __global__ void k_color_place(uint32_t n, const uint32_t *color, const uint32_t *start, uint32_t *cursor,
                               uint32_t *bodies) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t c = color[i];
-    if (c == kInvalid) return;
-    bodies[start[c] + atomicAdd(&cursor[c], 1u)] = i;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t c = i < n ? color[i] : kInvalid;
+    bool pending = c != kInvalid;
+    while (__ballot(pending) != 0ull) {
+        const uint64_t live = __ballot(pending);
+        const uint32_t leader = __builtin_ctzll(live);
+        const uint32_t c0 = __shfl(c, leader, 64);
+        const uint64_t same = __ballot(pending && c == c0);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[c0], uint32_t(__popcll(same)));
+        base = __shfl(base, leader, 64);
+        if (pending && c == c0) {
+            bodies[start[c0] + base + __popcll(same & ((1ull << lane) - 1ull))] = i;
+            pending = false;
+        }
+    }
 }
 
 // Per colour position: the body's contact points in list order.
@@ -1025,6 +1048,7 @@ __device__ void account_phase(const SolveArgs &a, uint32_t phase) {
 __device__ __forceinline__ uint32_t global_wave() { return (blockIdx.x * blockDim.x + threadIdx.x) >> 6; }
 
 // The whole solve in one cooperative launch.
+template <int L>
 __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *c_flags) {
     cg::grid_group grid = cg::this_grid();
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1046,7 +1070,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
         }
         for (uint32_t it = 0; it < a.budget; ++it) {
             for (uint32_t c = 0; c < colors; ++c) {
-                solve_color<kLanes>(a, c, phase, gw, nw);
+                solve_color<L>(a, c, phase, gw, nw);
                 grid.sync();
             }
             advance_duals(a, phase, it, tid, nthreads);
@@ -1065,9 +1089,10 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
 }
 
 // Staged variant: one launch per stage (no grid barrier).
+template <int L>
 __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t color, uint32_t phase) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    solve_color<kLanes>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
+    solve_color<L>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
     advance_duals(a, phase, it, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1270,9 +1295,13 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
 
     if (sp.cooperative) {
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
+        const void *kernel = sp.lanes == 64 ? reinterpret_cast<const void *>(k_solve_all<64>)
+                           : sp.lanes == 32 ? reinterpret_cast<const void *>(k_solve_all<32>)
+                           : sp.lanes == 8  ? reinterpret_cast<const void *>(k_solve_all<8>)
+                                            : reinterpret_cast<const void *>(k_solve_all<16>);
         if (!g_coop_blocks) {
             int per_cu = 0;
-            PMH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solve_all, 256, 0));
+            PMH_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solve_all<64>, 256, 0));
             // Two resident blocks per CU at most: more blocks only add
             // grid-barrier arrivals.
             g_coop_blocks = std::max(1, std::min(per_cu, 2)) * device_compute_units();
@@ -1280,8 +1309,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         uint32_t blocks = g_coop_blocks;
         const uint32_t *cf = v.c_flags.p;
         void *args[] = {&a, &cf};
-        PMH_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(k_solve_all), dim3(blocks), dim3(256),
-                                             args, 0, s));
+        PMH_CHECK(hipLaunchCooperativeKernel(kernel, dim3(blocks), dim3(256), args, 0, s));
         if (timing) timing->launches += 1;
         k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, blocks);
     } else {
@@ -1297,6 +1325,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                                  hipMemcpyDeviceToHost, s));
         PMH_CHECK(hipStreamSynchronize(s));
         const StepCounters host = *v.host_ctr;
+        const uint32_t lanes = sp.lanes;
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const uint32_t colors = std::min(host.colors, kMaxColors);
         const uint32_t pts_blocks = std::max<uint32_t>(1, (host.points + 255) / 256);
@@ -1313,9 +1342,13 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             for (uint32_t it = 0; it < budget; ++it) {
                 for (uint32_t c = 0; c < colors; ++c) {
                     const uint32_t bodies = v.host_color_start[c + 1] - v.host_color_start[c];
-                    const uint32_t waves = (bodies + (64 / kLanes) - 1) / (64 / kLanes);
+                    const uint32_t per_wave = 64 / lanes;
+                    const uint32_t waves = (bodies + per_wave - 1) / per_wave;
                     const uint32_t blocks = std::max<uint32_t>(1, (waves + 3) / 4);
-                    k_stage_color<<<blocks, 256, 0, s>>>(a, c, phase);
+                    if (lanes == 64) k_stage_color<64><<<blocks, 256, 0, s>>>(a, c, phase);
+                    else if (lanes == 32) k_stage_color<32><<<blocks, 256, 0, s>>>(a, c, phase);
+                    else if (lanes == 8) k_stage_color<8><<<blocks, 256, 0, s>>>(a, c, phase);
+                    else k_stage_color<16><<<blocks, 256, 0, s>>>(a, c, phase);
                 }
                 k_stage_duals<<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 launches += colors + 1;

@@ -92,6 +92,7 @@ struct SolveTotals {
 };
 
 constexpr uint32_t kMaxColors = 256;
+constexpr uint32_t kDefaultLanes = 32;   // lanes per body in the colour sweep
 constexpr uint32_t kCacheFloat4PerPoint = 6;
 
 template <class T>
@@ -151,6 +152,7 @@ struct SolveParams {
     V3 gravity;
     uint32_t base_iterations, requested_iterations, profile;
     uint32_t cooperative;                // one cooperative launch with grid barriers instead of per-stage launches
+    uint32_t lanes;                      // lanes of a wavefront per body in the colour sweep: 8, 16, 32, 64
 };
 
 struct SolveTiming {

@@ -157,7 +157,7 @@ extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
                                  v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p, v.c_pair.p, v.counters.p);
     k_seam_history_in<<<bg, 256, 0, s>>>(body_count, bodies, v.prev_vel.p);
     SolveParams sp{timestep, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations,
-                   profile_version, 0u};
+                   profile_version, 0u, kDefaultLanes};
     r = avbd_prepare_solve(s, v, parameters, previous, states, sp, nullptr);
     if (r != PMH_SUCCESS) return r;
     if (M)

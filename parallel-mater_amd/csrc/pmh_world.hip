@@ -298,6 +298,12 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 96;
     w->points_per_body = desc->points_per_body ? desc->points_per_body : 256;
     w->cooperative = (desc->flags & PMH_WORLD_COOPERATIVE_SOLVE) != 0;
+    if (desc->solve_lanes != 0 && desc->solve_lanes != 8 && desc->solve_lanes != 16 && desc->solve_lanes != 32 &&
+        desc->solve_lanes != 64) {
+        delete w;
+        return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    w->solve_lanes = desc->solve_lanes ? desc->solve_lanes : pmh::kDefaultLanes;
     w->device = int(desc->device);
     w->timing_enabled = true;
     *out = reinterpret_cast<PmhWorld>(w);

@@ -12,6 +12,7 @@ struct World {
     uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
     uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
     bool cooperative = false;
+    uint32_t solve_lanes = kDefaultLanes;
     int device = 0;
     hipStream_t stream = nullptr;
 

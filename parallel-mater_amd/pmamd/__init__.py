@@ -44,7 +44,7 @@ class WorldDesc(ctypes.Structure):
     _fields_ = [("gravity", _F * 3), ("timestep", _F), ("profile_version", _U),
                 ("base_iterations", _U), ("requested_iterations", _U),
                 ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_body", _U),
-                ("device", _U), ("flags", _U)]
+                ("device", _U), ("flags", _U), ("solve_lanes", _U)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
@@ -138,10 +138,12 @@ class World:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
-                 pairs_per_body=0, points_per_body=0, device=0, cooperative=False):
+                 pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
+                 solve_lanes=0):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
-                         points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0)
+                         points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0,
+                         solve_lanes)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h

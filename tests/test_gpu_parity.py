@@ -1,9 +1,10 @@
 # SPDX-License-Identifier: MIT
 """GPU parity: the HIP path against the oracle on identical inputs.
 
-The GPU sums each body's contact rows as 8 interleaved partial blocks
+The GPU sums each body's contact rows as L interleaved partial blocks
+(L = 32 lanes per body by default; 8, 16 and 64 are world options)
 combined by an xor butterfly; the oracle is run with the same order
-(lanes=8), so integer/index results must be equal and float results equal
+(lanes=L), so integer/index results must be equal and float results equal
 up to the transcendental functions (atan2f on large rotations), for which
 the stated tolerance is |a-b| <= 1e-5 + 1e-5|b| per component after one
 step.  The oracle in reference order (lanes=0) is then within the
@@ -20,7 +21,7 @@ import scenarios as S
 from pmamd import layouts as L
 
 pytestmark = pytest.mark.gpu
-LANES = 8
+LANES = 32
 
 
 def say(msg):
@@ -28,8 +29,8 @@ def say(msg):
     sys.stdout.flush()
 
 
-def gpu_world(scene, **kw):
-    return scene.build(pmamd.World(**kw))
+def gpu_world(scene, lanes=LANES, **kw):
+    return scene.build(pmamd.World(solve_lanes=lanes, **kw))
 
 
 def oracle_world(scene, lanes=LANES, **kw):
@@ -107,6 +108,25 @@ def test_pyramid_single_step(profile):
     r = oracle_world(scene, lanes=0, profile=profile, base_iterations=10)
     r.step(1)
     assert_states_close(gs, r.states(), rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("lanes", [8, 16, 64])
+def test_lane_counts_follow_the_oracle(lanes):
+    say(f"pyramid 5 steps with {lanes} lanes per body")
+    scene = S.pyramid()
+    g = gpu_world(scene, lanes=lanes)
+    o = oracle_world(scene, lanes=lanes)
+    g.step(5)
+    o.step(5)
+    np.testing.assert_array_equal(g.pairs(), o.pairs())
+    gs, os_ = g.states(), o.states()
+    say(f"bit-exact state fraction {exact_fraction(gs.view(np.float32), os_.view(np.float32)):.4f}")
+    assert_states_close(gs, os_)
+
+
+def test_invalid_lane_count_is_rejected():
+    with pytest.raises(pmamd.PmhError):
+        pmamd.World(solve_lanes=12)
 
 
 def test_pyramid_trajectory():
