@@ -322,10 +322,20 @@ void row_to(const Row &r, float *f) {
 struct Plane { V3 normal; float offset; };
 enum ShapeKind : uint32_t { kMesh = 0, kSphere = 1, kCapsule = 2 };
 
+// Median-split BVH node (geometry_storage.hpp:174-181): leaves hold
+// triangles [first, first + count) of the reordered index buffer.
+struct BvhNode {
+    V3 minimum, maximum;
+    uint32_t left = 0, right = 0, first = 0, count = 0;
+};
+
 struct Shape {
     uint32_t kind = kMesh;
     std::vector<V3> vertices;
     std::vector<uint32_t> indices;
+    std::vector<BvhNode> nodes;     // preorder
+    std::vector<uint32_t> leaves;   // indices of leaf nodes, ascending
+    std::vector<V3> shell;          // certified outward normal per triangle (may be empty)
     std::vector<Plane> planes;      // per triangle, then the unique tail
     uint32_t unique_planes = 0;
     bool has_planes = false;
@@ -374,6 +384,112 @@ std::vector<Plane> closed_convex_planes(const std::vector<V3> &v,
     return planes;
 }
 
+// closed_shell_normals, runtime_domains.cu.inc:1111-1245: per connected
+// component (vertices welded by position), convex components take their
+// face planes' normals, closed orientable ones the signed-volume
+// orientation, open volumetric ones a support-plane side or a quarter-length
+// two-sided marker on interior triangles.  Empty when nothing is certified.
+std::vector<V3> closed_shell_normals(const std::vector<V3> &v, const std::vector<uint32_t> &idx) {
+    const uint32_t nv = uint32_t(v.size());
+    std::map<std::array<float, 3>, uint32_t> weld;
+    std::vector<uint32_t> rep(nv), up(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        rep[i] = weld.emplace(std::array<float, 3>{v[i].x, v[i].y, v[i].z}, i).first->second;
+        up[i] = i;
+    }
+    auto find = [&](uint32_t i) {
+        while (up[i] != i) { up[i] = up[up[i]]; i = up[i]; }
+        return i;
+    };
+    for (size_t t = 0; t < idx.size(); t += 3)
+        for (size_t c = 1; c < 3; ++c) {
+            uint32_t r0 = find(rep[idx[t]]), rc = find(rep[idx[t + c]]);
+            up[rc] = r0;
+        }
+    const uint32_t tri = uint32_t(idx.size() / 3);
+    std::map<uint32_t, std::vector<uint32_t>> components;
+    for (uint32_t t = 0; t < tri; ++t) components[find(rep[idx[3 * t]])].push_back(t);
+    std::vector<V3> out(tri, V3{0, 0, 0});
+    bool any = false;
+    for (auto &kv : components) {
+        const std::vector<uint32_t> &ts = kv.second;
+        std::map<uint32_t, uint32_t> local;
+        std::vector<V3> pts;
+        std::vector<uint32_t> faces;
+        for (uint32_t t : ts)
+            for (uint32_t c = 0; c < 3; ++c) {
+                auto e = local.emplace(rep[idx[3 * t + c]], uint32_t(pts.size()));
+                if (e.second) pts.push_back(v[rep[idx[3 * t + c]]]);
+                faces.push_back(e.first->second);
+            }
+        auto planes = closed_convex_planes(pts, faces);
+        if (!planes.empty()) {
+            any = true;
+            for (size_t i = 0; i < ts.size(); ++i) out[ts[i]] = planes[i].normal;
+            continue;
+        }
+        std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, int>> edges;
+        double volume = 0.0;
+        for (size_t f = 0; f < faces.size(); f += 3) {
+            for (size_t c = 0; c < 3; ++c) {
+                uint32_t e0 = faces[f + c], e1 = faces[f + (c + 1) % 3];
+                auto &use = edges[{std::min(e0, e1), std::max(e0, e1)}];
+                use.first += 1;
+                use.second += e0 < e1 ? 1 : -1;
+            }
+            double d[3][3];
+            for (size_t c = 0; c < 3; ++c) {
+                V3 q = pts[faces[f + c]];
+                d[c][0] = double(q.x) - pts[0].x;
+                d[c][1] = double(q.y) - pts[0].y;
+                d[c][2] = double(q.z) - pts[0].z;
+            }
+            volume += d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) +
+                      d[0][1] * (d[1][2] * d[2][0] - d[1][0] * d[2][2]) +
+                      d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+        }
+        bool closed = volume != 0.0 && std::isfinite(volume);
+        for (auto &e : edges) closed = closed && e.second.first == 2u && e.second.second == 0;
+        if (closed) {
+            any = true;
+            for (size_t i = 0; i < ts.size(); ++i) {
+                V3 a = pts[faces[3 * i]], b = pts[faces[3 * i + 1]], c = pts[faces[3 * i + 2]];
+                out[ts[i]] = mul(normalized_or(cross(sub(b, a), sub(c, a)), {0, 0, 0}), volume > 0.0 ? 1.0f : -1.0f);
+            }
+            continue;
+        }
+        float scale = 0.0f;
+        for (V3 q : pts) scale = std::max(scale, len(sub(q, pts[0])));
+        const float tol = std::max(1.0e-7f, scale * 1.0e-5f);
+        for (size_t i = 0; i < ts.size(); ++i) {
+            V3 a = pts[faces[3 * i]];
+            V3 n = normalized_or(cross(sub(pts[faces[3 * i + 1]], a), sub(pts[faces[3 * i + 2]], a)), {0, 0, 0});
+            float lo = 0.0f, hi = 0.0f;
+            for (V3 q : pts) {
+                float d = dot(n, sub(q, a));
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
+            }
+            if (hi <= tol && lo < -tol) {
+                out[ts[i]] = n;
+            } else if (lo >= -tol && hi > tol) {
+                out[ts[i]] = mul(n, -1.0f);
+            } else {
+                bool interior = true;
+                for (size_t c = 0; c < 3; ++c) {
+                    uint32_t e0 = faces[3 * i + c], e1 = faces[3 * i + (c + 1) % 3];
+                    auto use = edges.at({std::min(e0, e1), std::max(e0, e1)});
+                    interior = interior && use.first == 2u && use.second == 0;
+                }
+                if (!interior) continue;
+                out[ts[i]] = mul(n, 0.25f);
+            }
+            any = true;
+        }
+    }
+    return any ? out : std::vector<V3>{};
+}
+
 bool prepare_mesh(Shape &s, const float *verts, uint32_t nv, const uint32_t *idx, uint32_t ni) {
     if (nv < 3 || ni < 3 || ni % 3 != 0) return false;
     s.kind = kMesh;
@@ -400,33 +516,53 @@ bool prepare_mesh(Shape &s, const float *verts, uint32_t nv, const uint32_t *idx
     };
     struct Builder {
         std::vector<uint32_t> &order;
+        std::vector<BvhNode> &nodes;
         decltype(coord) &coordf;
         decltype(centroid) &centroidf;
         const std::vector<V3> &verts;
         const std::vector<uint32_t> &in;
-        void build(uint32_t begin, uint32_t end) {
-            V3 mn = {FLT_MAX, FLT_MAX, FLT_MAX}, mx = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        uint32_t build(uint32_t begin, uint32_t end) {
+            BvhNode node;
+            node.minimum = {FLT_MAX, FLT_MAX, FLT_MAX};
+            node.maximum = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
             for (uint32_t it = begin; it < end; ++it) {
                 uint32_t o = order[it] * 3;
-                for (uint32_t c = 0; c < 3; ++c) { mn = cmin(mn, verts[in[o + c]]); mx = cmax(mx, verts[in[o + c]]); }
+                for (uint32_t c = 0; c < 3; ++c) {
+                    node.minimum = cmin(node.minimum, verts[in[o + c]]);
+                    node.maximum = cmax(node.maximum, verts[in[o + c]]);
+                }
             }
-            if (end - begin <= 4u) return;
-            V3 ext = sub(mx, mn);
+            const uint32_t self = uint32_t(nodes.size());
+            nodes.push_back(node);
+            if (end - begin <= 4u) {
+                nodes[self].first = begin;
+                nodes[self].count = end - begin;
+                return self;
+            }
+            V3 ext = sub(node.maximum, node.minimum);
             int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
             std::stable_sort(order.begin() + begin, order.begin() + end, [&](uint32_t a, uint32_t b) {
                 float fa = coordf(centroidf(a), axis), fb = coordf(centroidf(b), axis);
                 return fa < fb || (fa == fb && a < b);
             });
             uint32_t mid = begin + (end - begin) / 2u;
-            build(begin, mid);
-            build(mid, end);
+            uint32_t l = build(begin, mid);
+            uint32_t r = build(mid, end);
+            nodes[self].left = l;
+            nodes[self].right = r;
+            return self;
         }
-    } builder{order, coord, centroid, s.vertices, in};
+    } builder{order, s.nodes, coord, centroid, s.vertices, in};
+    s.nodes.clear();
     builder.build(0, tri);
+    s.leaves.clear();
+    for (uint32_t k = 0; k < s.nodes.size(); ++k)
+        if (s.nodes[k].count != 0u) s.leaves.push_back(k);
     s.indices.resize(ni);
     for (uint32_t t = 0; t < tri; ++t)
         for (uint32_t c = 0; c < 3; ++c) s.indices[t * 3 + c] = in[order[t] * 3 + c];
     auto planes = closed_convex_planes(s.vertices, s.indices);
+    s.shell = closed_shell_normals(s.vertices, s.indices);
     s.has_planes = !planes.empty();
     if (s.has_planes) {
         std::map<std::array<float, 4>, bool> seen;
@@ -682,6 +818,562 @@ float correction_reach(const PmoParams &p, const Shape &s) {
     return fmin_(0.5f, 0.25f * fmax_(0.0f, fmin_(e.x, fmin_(e.y, e.z))));
 }
 
+// rigid_contact_normal_speed, geometry.slang:1272-1280.
+float contact_normal_speed(const PmoState &b, const PmoState &c, V3 p, V3 n) {
+    V3 bv = add(b.linear_velocity, cross(b.angular_velocity, sub(p, b.position)));
+    V3 cv = add(c.linear_velocity, cross(c.angular_velocity, sub(p, c.position)));
+    return dot(sub(bv, cv), n);
+}
+
+// ---------------------------------------------------------------------
+// Triangle-pair narrowphase (mesh pairs the convex face path does not
+// take): closest features of triangle pairs, certified shell normals,
+// bisection recovery and conservative advancement.  Restates
+// geometry.slang:949-1117, 1272-1584, 1855-2285.  Pinned by brute force
+// on small inputs (closest points against dense sampling) and closed
+// forms (edge/vertex/face contacts of boxes on a floor, crossing edges,
+// shell normals of closed, open and concave meshes) in the tests; the
+// order in which candidates reach add_contact has no reference-held
+// vector (parity unpinned beyond those).
+// ---------------------------------------------------------------------
+void closest_segments(V3 p0, V3 p1, V3 q0, V3 q1, V3 &on_p, V3 &on_q) {   // :949-998
+    V3 d0 = sub(p1, p0), d1 = sub(q1, q0), r = sub(p0, q0);
+    float a = dot(d0, d0), e = dot(d1, d1);
+    float s = 0.0f, t = 0.0f;
+    if (a <= 1.0e-12f && e <= 1.0e-12f) { on_p = p0; on_q = q0; return; }
+    if (a <= 1.0e-12f) {
+        t = clampf(dot(d1, r) / e, 0.0f, 1.0f);
+    } else {
+        float c = dot(d0, r);
+        if (e <= 1.0e-12f) {
+            s = clampf(-c / a, 0.0f, 1.0f);
+        } else {
+            float b = dot(d0, d1), f = dot(d1, r);
+            float den = a * e - b * b;
+            if (std::fabs(den) > kEpsilon * a * e) s = clampf((b * f - c * e) / den, 0.0f, 1.0f);
+            t = (b * s + f) / e;
+            if (t < 0.0f) { t = 0.0f; s = clampf(-c / a, 0.0f, 1.0f); }
+            else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / a, 0.0f, 1.0f); }
+        }
+    }
+    on_p = add(p0, mul(d0, s));
+    on_q = add(q0, mul(d1, t));
+}
+
+V3 closest_on_triangle(V3 p, V3 a, V3 b, V3 c) {                    // :1000-1029
+    V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    V3 bp = sub(p, b);
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return add(a, mul(ab, d1 / (d1 - d3)));
+    V3 cp = sub(p, c);
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return add(a, mul(ac, d2 / (d2 - d6)));
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && d4 - d3 >= 0.0f && d5 - d6 >= 0.0f)
+        return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    V3 n = cross(ab, ac);
+    return sub(p, mul(n, dot(ap, n) / len2(n)));
+}
+
+bool point_in_triangle(V3 q, V3 a, V3 b, V3 c, V3 n) {                 // :1031-1038
+    float tol = -1.0e-5f * len2(n);
+    return dot(cross(sub(b, a), sub(q, a)), n) >= tol &&
+           dot(cross(sub(c, b), sub(q, b)), n) >= tol &&
+           dot(cross(sub(a, c), sub(q, c)), n) >= tol;
+}
+
+bool segment_hits_triangle(V3 p, V3 q, V3 a, V3 b, V3 c, V3 &hit) {   // :1040-1053
+    hit = {0, 0, 0};
+    V3 n = cross(sub(b, a), sub(c, a));
+    V3 dir = sub(q, p);
+    float den = dot(n, dir);
+    if (len2(n) <= kEpsilon * kEpsilon || std::fabs(den) <= kEpsilon) return false;
+    float amount = dot(n, sub(a, p)) / den;
+    if (amount < 0.0f || amount > 1.0f) return false;
+    hit = add(p, mul(dir, amount));
+    return point_in_triangle(hit, a, b, c, n);
+}
+
+inline void keep_closer(V3 a, V3 b, float &best, V3 &best_a, V3 &best_b) {   // :1055-1065
+    float d = len2(sub(a, b));
+    if (d < best) { best = d; best_a = a; best_b = b; }
+}
+
+void closest_triangle_pair(const V3 A[3], const V3 B[3], V3 &pa, V3 &pb) {   // :1067-1105
+    pa = {0, 0, 0};
+    pb = {0, 0, 0};
+    V3 hit;
+    for (uint32_t e = 0; e < 3; ++e)
+        if (segment_hits_triangle(A[e], A[(e + 1) % 3], B[0], B[1], B[2], hit)) { pa = hit; pb = hit; return; }
+    for (uint32_t e = 0; e < 3; ++e)
+        if (segment_hits_triangle(B[e], B[(e + 1) % 3], A[0], A[1], A[2], hit)) { pa = hit; pb = hit; return; }
+    float best = 3.402823466e+38f;
+    for (uint32_t i = 0; i < 3; ++i) keep_closer(A[i], closest_on_triangle(A[i], B[0], B[1], B[2]), best, pa, pb);
+    for (uint32_t i = 0; i < 3; ++i) keep_closer(closest_on_triangle(B[i], A[0], A[1], A[2]), B[i], best, pa, pb);
+    for (uint32_t i = 0; i < 3; ++i)
+        for (uint32_t j = 0; j < 3; ++j) {
+            V3 u, w;
+            closest_segments(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3], u, w);
+            keep_closer(u, w, best, pa, pb);
+        }
+}
+
+inline bool boxes_touch(V3 amin, V3 amax, V3 bmin, V3 bmax) {       // math.slang:43-49
+    return amin.x <= bmax.x && amax.x >= bmin.x && amin.y <= bmax.y && amax.y >= bmin.y &&
+           amin.z <= bmax.z && amax.z >= bmin.z;
+}
+
+bool triangle_bounds_overlap(const V3 A[3], const V3 B[3], float margin) {   // :1107-1116
+    V3 e = {margin, margin, margin};
+    V3 amin = sub(cmin(A[0], cmin(A[1], A[2])), e), amax = add(cmax(A[0], cmax(A[1], A[2])), e);
+    V3 bmin = cmin(B[0], cmin(B[1], B[2])), bmax = cmax(B[0], cmax(B[1], B[2]));
+    return boxes_touch(amin, amax, bmin, bmax);
+}
+
+bool reaches_rest_offset(const PmoState &b, const PmoState &c, V3 point, V3 n, float distance,
+                         float rest, float dt) {                       // :1282-1290
+    if (distance <= rest + 1.0e-5f) return true;
+    float speed = contact_normal_speed(b, c, point, n);
+    if (speed > 1.0e-5f) return false;
+    return -speed * dt + rest + 1.0e-5f >= distance;
+}
+
+V3 advect_triangle_point(V3 q, const V3 from[3], const V3 to[3]) {   // :1292-1304
+    V3 ab = sub(from[1], from[0]), ac = sub(from[2], from[0]);
+    V3 n = cross(ab, ac);
+    float sq = len2(n);
+    if (sq <= 1.0e-20f) return to[0];
+    V3 rel = sub(q, from[0]);
+    float v = dot(cross(rel, ac), n) / sq;
+    float w = dot(cross(ab, rel), n) / sq;
+    return add(to[0], add(mul(sub(to[1], to[0]), v), mul(sub(to[2], to[0]), w)));
+}
+
+// rigid_shell_outward, :1308-1322: the certified normal's side applied to
+// the (possibly swept) world triangle; unit length for certified exterior
+// faces, a quarter for two-sided interior features, zero otherwise.
+V3 shell_outward(const Shape &m, uint32_t t, const V3 W[3]) {
+    if (m.shell.empty()) return {0, 0, 0};
+    V3 o = m.shell[t];
+    float sq = len2(o);
+    if (sq < 0.01f) return {0, 0, 0};
+    V3 la = m.vertices[m.indices[3 * t]], lb = m.vertices[m.indices[3 * t + 1]], lc = m.vertices[m.indices[3 * t + 2]];
+    float sign = dot(cross(sub(lb, la), sub(lc, la)), o) >= 0.0f ? 1.0f : -1.0f;
+    return mul(normalized_or(cross(sub(W[1], W[0]), sub(W[2], W[0])), {0, 0, 0}), sign * (sq > 0.5f ? 1.0f : 0.25f));
+}
+
+bool shell_facing(V3 n, V3 oa, V3 ob) {                              // :1324-1337
+    float sa = len2(oa), sb = len2(ob);
+    return (sa < 0.01f || (sa > 0.5f ? dot(n, oa) < -1.0e-3f : std::fabs(dot(n, oa)) > 2.5e-4f)) &&
+           (sb < 0.01f || (sb > 0.5f ? dot(n, ob) > 1.0e-3f : std::fabs(dot(n, ob)) > 2.5e-4f));
+}
+
+bool point_inside_convex(const Shape &m, const PmoState &st, V3 world) {   // :1339-1357
+    if (!m.has_planes) return false;
+    V3 l = inv_rotate(st.orientation, sub(world, st.position));
+    uint32_t tri = m.index_count() / 3u;
+    uint32_t count = m.unique_planes != 0u ? m.unique_planes : tri;
+    uint32_t first = m.unique_planes != 0u ? tri : 0u;
+    for (uint32_t f = 0; f < count; ++f) {
+        const Plane &pl = m.planes[first + f];
+        if (dot(pl.normal, l) > pl.offset + 1.0e-5f) return false;
+    }
+    return true;
+}
+
+V3 triangle_feature_normal(const V3 A[3], const V3 B[3], V3 pa, V3 pb, V3 fallback) {   // :1855-1908
+    V3 delta = sub(pa, pb);
+    V3 normal = normalized_or(delta, fallback);
+    V3 faces[2] = {normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), normal),
+                   normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), normal)};
+    const V3 *tris[2] = {A, B};
+    V3 pts[2] = {pa, pb};
+    float align_best = -1.0f;
+    V3 interior_normal = {0, 0, 0};
+    for (uint32_t side = 0; side < 2; ++side) {
+        const V3 *T = tris[side];
+        V3 raw = cross(sub(T[1], T[0]), sub(T[2], T[0]));
+        float tol = 1.0e-5f * len2(raw);
+        bool interior = tol > 0.0f;
+        for (uint32_t e = 0; e < 3; ++e)
+            interior = interior && dot(cross(sub(T[(e + 1) % 3], T[e]), sub(pts[side], T[e])), raw) > tol;
+        float alignment = std::fabs(dot(faces[side], normal));
+        if (interior && alignment > align_best) {
+            align_best = alignment;
+            float proj = dot(delta, faces[side]);
+            float dir = std::fabs(proj) > kEpsilon ? proj : dot(fallback, faces[side]);
+            interior_normal = dir >= 0.0f ? faces[side] : mul(faces[side], -1.0f);
+        }
+    }
+    if (align_best >= 0.0f) return interior_normal;
+    float best = 4.0f * kEpsilon * kEpsilon;
+    if (len2(delta) > kEpsilon * kEpsilon) {
+        for (uint32_t side = 0; side < 2; ++side) {
+            float proj = dot(delta, faces[side]);
+            float err = len2(sub(delta, mul(faces[side], proj)));
+            if (err < best) {
+                best = err;
+                normal = proj >= 0.0f ? faces[side] : mul(faces[side], -1.0f);
+            }
+        }
+    }
+    return normal;
+}
+
+// The per-pair inputs of the triangle path.  Hinge frames carry only the
+// static flag: no joints act on this path (rigid_hinge_contact_frame,
+// geometry.slang:404-491, with no incident constraint), and the body
+// reference point is the body position.
+struct TrianglePair {
+    const Shape *am, *bm;
+    PmoState ap, a, bp, b;          // previous and predicted states
+    PmoHingeFrame ha, hb;
+    float margin, dt;
+};
+
+inline void world_triangle(const Shape &m, const PmoState &st, uint32_t t, V3 W[3]) {
+    for (uint32_t c = 0; c < 3; ++c) W[c] = transform_point(st, m.vertices[m.indices[3 * t + c]]);
+}
+
+// rigid_collide_triangle_ranges, geometry.slang:1359-1584.
+void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count, uint32_t b_first,
+                             uint32_t b_count, PmoManifold &m) {
+    const Shape &am = *P.am, &bm = *P.bm;
+    const float margin = P.margin;
+    const float rest = fmin_(margin, 1.0e-3f);
+    const V3 body_ref = P.a.position;
+    const bool inside_a = P.ha.static_body != 0u && point_inside_convex(am, P.ap, P.bp.position);
+    const bool inside_b = P.hb.static_body != 0u && point_inside_convex(bm, P.bp, P.ap.position);
+    for (uint32_t ai = 0; ai < a_count; ++ai) {
+        const uint32_t at = a_first + ai;
+        if (3u * at + 2u >= am.index_count()) break;
+        V3 A[3];
+        world_triangle(am, P.a, at, A);
+        for (uint32_t bi = 0; bi < b_count; ++bi) {
+            const uint32_t bt = b_first + bi;
+            if (3u * bt + 2u >= bm.index_count()) break;
+            V3 B[3];
+            world_triangle(bm, P.b, bt, B);
+            if (!triangle_bounds_overlap(A, B, margin)) continue;
+            V3 oa = shell_outward(am, at, A), ob = shell_outward(bm, bt, B);
+            V3 coa = P.ha.static_body != 0u && inside_a ? V3{0, 0, 0} : oa;
+            V3 cob = P.hb.static_body != 0u && inside_b ? V3{0, 0, 0} : ob;
+            if (dot(oa, ob) > 0.9999f) continue;
+            V3 pa, pb;
+            closest_triangle_pair(A, B, pa, pb);
+            V3 delta = sub(pa, pb);
+            float sq = len2(delta);
+            if (sq > margin * margin) continue;
+            V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+            V3 fallback = dot(cn, sub(body_ref, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+            if (len2(cob) > 0.5f) fallback = cob;
+            float distance = std::sqrt(fmax_(sq, 0.0f));
+            V3 point = mul(add(pa, pb), 0.5f);
+            V3 proj = sub(pa, mul(cn, dot(sub(pa, B[0]), cn)));
+            bool on_face = len2(sub(proj, closest_on_triangle(proj, B[0], B[1], B[2]))) <= 1.0e-10f;
+            V3 normal = (distance <= 1.0e-5f || on_face) ? fallback : normalized_or(delta, fallback);
+            if (on_face && len2(oa) < 0.01f && len2(ob) > 0.5f && bm.has_planes) {
+                V3 face_a = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0, 0, 0});
+                if (std::fabs(dot(face_a, normal)) < 1.0e-3f) {
+                    float depth = fmax_(0.0f, -fmin_(dot(sub(A[0], B[0]), normal),
+                                                     fmin_(dot(sub(A[1], B[0]), normal), dot(sub(A[2], B[0]), normal))));
+                    V3 axis = inv_rotate(P.b.orientation, face_a);
+                    float offset = dot(sub(P.b.position, A[0]), face_a);
+                    float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+                    for (uint32_t v = 0; v < bm.vertex_count(); ++v) {
+                        float pr = dot(bm.vertices[v], axis) + offset;
+                        lo = fmin_(lo, pr);
+                        hi = fmax_(hi, pr);
+                    }
+                    float alternate = fmax_(0.0f, fmin_(-lo, hi));
+                    if (alternate + 1.0e-5f < depth) continue;
+                }
+            }
+            float separation = on_face && len2(cob) > 0.5f ? dot(delta, normal) : distance;
+            if (!reaches_rest_offset(P.a, P.b, point, normal, separation, rest, P.dt)) continue;
+            PmoContact c;
+            std::memset(&c, 0, sizeof c);
+            c.normal = normal;
+            c.point = point;
+            c.body_hinge = P.ha;
+            c.collider_hinge = P.hb;
+            c.penetration = rest - separation + 1.0e-5f;
+            bool recovered = false;
+            if (!am.has_planes && !bm.has_planes && distance <= 1.0e-5f) {
+                V3 oldA[3], oldB[3];
+                world_triangle(am, P.ap, at, oldA);
+                world_triangle(bm, P.bp, bt, oldB);
+                V3 opa, opb;
+                closest_triangle_pair(oldA, oldB, opa, opb);
+                V3 od = sub(opa, opb);
+                if (len2(od) > 1.0e-10f) {
+                    V3 IA[3], IB[3];
+                    float lo = 0.0f, hi = 1.0f;
+                    float gap = fmin_(rest, 0.5f * std::sqrt(len2(od)));
+                    for (uint32_t k = 0; k < 10u; ++k) {
+                        float time = 0.5f * (lo + hi);
+                        for (uint32_t v = 0; v < 3; ++v) {
+                            IA[v] = add(oldA[v], mul(sub(A[v], oldA[v]), time));
+                            IB[v] = add(oldB[v], mul(sub(B[v], oldB[v]), time));
+                        }
+                        V3 u, w;
+                        closest_triangle_pair(IA, IB, u, w);
+                        if (len2(sub(u, w)) > gap * gap) lo = time;
+                        else hi = time;
+                    }
+                    for (uint32_t v = 0; v < 3; ++v) {
+                        IA[v] = add(oldA[v], mul(sub(A[v], oldA[v]), lo));
+                        IB[v] = add(oldB[v], mul(sub(B[v], oldB[v]), lo));
+                    }
+                    closest_triangle_pair(IA, IB, opa, opb);
+                    od = sub(opa, opb);
+                    c.normal = triangle_feature_normal(IA, IB, opa, opb, normalized_or(od, fallback));
+                    V3 ca = advect_triangle_point(opa, IA, A), cb = advect_triangle_point(opb, IB, B);
+                    c.point = mul(add(ca, cb), 0.5f);
+                    c.penetration = rest - dot(sub(ca, cb), c.normal) + 1.0e-5f;
+                    recovered = true;
+                }
+            }
+            if (distance <= 1.0e-5f) {
+                if (!recovered && (P.ha.fixed_member != 0u || P.hb.fixed_member != 0u || on_face)) {
+                    float depth = fmax_(0.0f, -fmin_(dot(sub(A[0], pb), normal),
+                                                     fmin_(dot(sub(A[1], pb), normal), dot(sub(A[2], pb), normal))));
+                    c.penetration = fmin_(margin, depth + rest) + 1.0e-5f;
+                } else if (!recovered) {
+                    c.penetration = margin + 1.0e-5f;
+                }
+            }
+            if (!shell_facing(c.normal, coa, cob)) continue;
+            c.impact_fraction = 1.0f;
+            c.initial_normal_speed = contact_normal_speed(P.a, P.b, c.point, c.normal);
+            add_contact(m, c, fmax_(2.0f * margin, 1.0e-4f));
+        }
+    }
+}
+
+// rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154
+// (unguided: no hinge joints act on this path).
+void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first, uint32_t a_count, uint32_t b_first,
+                                   uint32_t b_count, PmoManifold &m) {
+    const Shape &am = *P.am, &bm = *P.bm;
+    const float margin = P.margin;
+    const float rest = fmin_(margin, 1.0e-3f);
+    const bool inside_a = P.ha.static_body != 0u && point_inside_convex(am, P.ap, P.bp.position);
+    const bool inside_b = P.hb.static_body != 0u && point_inside_convex(bm, P.bp, P.ap.position);
+    for (uint32_t ai = 0; ai < a_count; ++ai) {
+        const uint32_t at = a_first + ai;
+        if (3u * at + 2u >= am.index_count()) break;
+        V3 pA[3], cA[3];
+        world_triangle(am, P.ap, at, pA);
+        world_triangle(am, P.a, at, cA);
+        V3 amin = cmin(pA[0], cA[0]), amax = cmax(pA[0], cA[0]);
+        for (uint32_t v = 1; v < 3; ++v) {
+            amin = cmin(amin, cmin(pA[v], cA[v]));
+            amax = cmax(amax, cmax(pA[v], cA[v]));
+        }
+        for (uint32_t bi = 0; bi < b_count; ++bi) {
+            const uint32_t bt = b_first + bi;
+            if (3u * bt + 2u >= bm.index_count()) break;
+            V3 pB[3], cB[3];
+            world_triangle(bm, P.bp, bt, pB);
+            world_triangle(bm, P.b, bt, cB);
+            V3 bmin = cmin(pB[0], cB[0]), bmax = cmax(pB[0], cB[0]);
+            for (uint32_t v = 1; v < 3; ++v) {
+                bmin = cmin(bmin, cmin(pB[v], cB[v]));
+                bmax = cmax(bmax, cmax(pB[v], cB[v]));
+            }
+            V3 e = {margin, margin, margin};
+            if (!boxes_touch(sub(amin, e), add(amax, e), bmin, bmax)) continue;
+            V3 dA[3], dB[3];
+            float speed_abs = 0.0f;
+            for (uint32_t v = 0; v < 3; ++v) {
+                dA[v] = sub(cA[v], pA[v]);
+                dB[v] = sub(cB[v], pB[v]);
+                speed_abs = fmax_(speed_abs, len(dA[v]));
+            }
+            float speed_b = 0.0f;
+            for (uint32_t v = 0; v < 3; ++v) speed_b = fmax_(speed_b, len(dB[v]));
+            speed_abs += speed_b;
+            V3 common = dB[0];
+            float rel_a = 0.0f, rel_b = 0.0f;
+            for (uint32_t v = 0; v < 3; ++v) {
+                rel_a = fmax_(rel_a, len(sub(dA[v], common)));
+                rel_b = fmax_(rel_b, len(sub(dB[v], common)));
+            }
+            float bound = fmin_(speed_abs, rel_a + rel_b);
+            if (bound <= kEpsilon) continue;
+            float time = 0.0f;
+            bool starts_near = false;
+            V3 sep_normal = {0, 0, 0};
+            bool have_sep = false;
+            for (uint32_t it = 0; it < 32u; ++it) {
+                V3 A[3], B[3];
+                for (uint32_t v = 0; v < 3; ++v) {
+                    A[v] = add(pA[v], mul(dA[v], time));
+                    B[v] = add(pB[v], mul(dB[v], time));
+                }
+                V3 pa, pb;
+                closest_triangle_pair(A, B, pa, pb);
+                V3 delta = sub(pa, pb);
+                float distance = std::sqrt(fmax_(0.0f, len2(delta)));
+                const float offset = rest;
+                if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
+                if (distance <= offset + 1.0e-5f) {
+                    V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+                    V3 center = add(P.ap.position, mul(sub(P.a.position, P.ap.position), time));
+                    V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+                    V3 oa = shell_outward(am, at, A), ob = shell_outward(bm, bt, B);
+                    V3 coa = P.ha.static_body != 0u && inside_a ? V3{0, 0, 0} : oa;
+                    V3 cob = P.hb.static_body != 0u && inside_b ? V3{0, 0, 0} : ob;
+                    if (dot(oa, ob) > 0.9999f) break;
+                    if (len2(cob) > 0.5f) fallback = cob;
+                    else if (len2(coa) > 0.5f) fallback = mul(coa, -1.0f);
+                    V3 normal = triangle_feature_normal(A, B, pa, pb, have_sep ? sep_normal : fallback);
+                    if (!shell_facing(normal, coa, cob)) break;
+                    V3 point = mul(add(pa, pb), 0.5f);
+                    float vn = contact_normal_speed(P.a, P.b, point, normal);
+                    if (vn > 1.0e-5f) break;
+                    float remaining = -vn * P.dt * (1.0f - time) - distance;
+                    PmoContact c;
+                    std::memset(&c, 0, sizeof c);
+                    c.normal = normal;
+                    c.point = point;
+                    c.penetration = fmax_(0.0f, remaining + offset) + 1.0e-5f;
+                    c.body_hinge = P.ha;
+                    c.collider_hinge = P.hb;
+                    c.impact_fraction = starts_near ? 0.0f : time;
+                    c.initial_normal_speed = vn;
+                    add_contact(m, c, fmax_(margin * 2.0f, 1.0e-4f));
+                    break;
+                }
+                sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
+                have_sep = true;
+                float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+                time += fmax_(advance, 1.0e-5f);
+                if (time > 1.0f) break;
+            }
+        }
+    }
+}
+
+// BoundsTransform and transformed_(motion_)bounds, math.slang:142-190.
+struct BoundsFrame { V3 position, ax, ay, az; };
+inline BoundsFrame bounds_frame(const PmoState &s) {
+    return {s.position, rotate(s.orientation, {1, 0, 0}), rotate(s.orientation, {0, 1, 0}),
+            rotate(s.orientation, {0, 0, 1})};
+}
+inline void frame_bounds(V3 lmin, V3 lmax, const BoundsFrame &f, float margin, V3 &mn, V3 &mx) {
+    V3 c = mul(add(lmin, lmax), 0.5f), h = mul(sub(lmax, lmin), 0.5f);
+    V3 wc = add(f.position, add(mul(f.ax, c.x), add(mul(f.ay, c.y), mul(f.az, c.z))));
+    V3 wh = {std::fabs(f.ax.x) * h.x + std::fabs(f.ay.x) * h.y + std::fabs(f.az.x) * h.z,
+             std::fabs(f.ax.y) * h.x + std::fabs(f.ay.y) * h.y + std::fabs(f.az.y) * h.z,
+             std::fabs(f.ax.z) * h.x + std::fabs(f.ay.z) * h.y + std::fabs(f.az.z) * h.z};
+    V3 e = {margin, margin, margin};
+    mn = sub(sub(wc, wh), e);
+    mx = add(add(wc, wh), e);
+}
+inline void motion_bounds(V3 lmin, V3 lmax, const BoundsFrame &prev, const BoundsFrame &cur, bool swept,
+                          float margin, V3 &mn, V3 &mx) {
+    frame_bounds(lmin, lmax, cur, margin, mn, mx);
+    if (!swept) return;
+    V3 pmn, pmx;
+    frame_bounds(lmin, lmax, prev, margin, pmn, pmx);
+    mn = cmin(mn, pmn);
+    mx = cmax(mx, pmx);
+}
+
+// One leaf range pair: the discrete pass, then the swept pass when the
+// pair moves past the margin.  Shared by the leaf-pair evaluation and the
+// BVH traversal.
+inline void collide_leaf_ranges(const TrianglePair &P, bool swept, const BvhNode &na, const BvhNode &nb,
+                                PmoManifold &m) {
+    collide_triangle_ranges(P, na.first, na.count, nb.first, nb.count, m);
+    if (swept) collide_triangle_ranges_swept(P, na.first, na.count, nb.first, nb.count, m);
+}
+
+// rigid_collide_meshes, geometry.slang:2156-2285: simultaneous BVH descent
+// with a 64-entry stack, all triangles when the stack would overflow.
+void collide_meshes(const TrianglePair &P, bool swept, PmoManifold &m) {
+    clear_manifold(m);
+    const Shape &am = *P.am, &bm = *P.bm;
+    BoundsFrame fa = bounds_frame(P.a), fb = bounds_frame(P.b);
+    BoundsFrame pfa = swept ? bounds_frame(P.ap) : BoundsFrame{};
+    BoundsFrame pfb = swept ? bounds_frame(P.bp) : BoundsFrame{};
+    V3 amn, amx, bmn, bmx;
+    motion_bounds(am.minimum, am.maximum, pfa, fa, swept, P.margin, amn, amx);
+    motion_bounds(bm.minimum, bm.maximum, pfb, fb, swept, 0.0f, bmn, bmx);
+    if (!boxes_touch(amn, amx, bmn, bmx)) return;
+    constexpr uint32_t kStack = 64;
+    uint32_t sa[kStack] = {}, sb[kStack] = {};
+    uint32_t top = 1;
+    bool overflow = am.nodes.empty() || bm.nodes.empty();
+    while (top > 0u && !overflow) {
+        --top;
+        const uint32_t ia = sa[top], ib = sb[top];
+        const BvhNode &na = am.nodes[ia], &nb = bm.nodes[ib];
+        motion_bounds(na.minimum, na.maximum, pfa, fa, swept, P.margin, amn, amx);
+        motion_bounds(nb.minimum, nb.maximum, pfb, fb, swept, 0.0f, bmn, bmx);
+        if (!boxes_touch(amn, amx, bmn, bmx)) continue;
+        const bool la = na.count != 0u, lb = nb.count != 0u;
+        if (la && lb) {
+            collide_leaf_ranges(P, swept, na, nb, m);
+            continue;
+        }
+        const uint32_t need = la || lb ? 2u : 4u;
+        if (top + need > kStack) { overflow = true; break; }
+        if (la) {
+            sa[top] = ia; sb[top++] = nb.right;
+            sa[top] = ia; sb[top++] = nb.left;
+        } else if (lb) {
+            sa[top] = na.right; sb[top++] = ib;
+            sa[top] = na.left; sb[top++] = ib;
+        } else {
+            sa[top] = na.right; sb[top++] = nb.right;
+            sa[top] = na.right; sb[top++] = nb.left;
+            sa[top] = na.left; sb[top++] = nb.right;
+            sa[top] = na.left; sb[top++] = nb.left;
+        }
+    }
+    if (overflow) {
+        clear_manifold(m);
+        collide_triangle_ranges(P, 0, am.index_count() / 3u, 0, bm.index_count() / 3u, m);
+        if (swept) collide_triangle_ranges_swept(P, 0, am.index_count() / 3u, 0, bm.index_count() / 3u, m);
+    }
+}
+
+// pm_generate_rigid_leaf_pairs (geometry.slang:2303-2492): overlapping
+// (body leaf, collider leaf) pairs of motion bounds.  128 workers each take
+// every 128th entry of the row-major leaf-pair grid and emit their hits
+// contiguously in worker order; that order is the order in which
+// candidates later meet add_contact.  Returns false when the pair has more
+// than `limit` candidates.
+bool leaf_candidates(const TrianglePair &P, bool swept, uint32_t limit,
+                     std::vector<std::pair<uint32_t, uint32_t>> &out) {
+    out.clear();
+    const Shape &am = *P.am, &bm = *P.bm;
+    BoundsFrame fa = bounds_frame(P.a), fb = bounds_frame(P.b);
+    BoundsFrame pfa = swept ? bounds_frame(P.ap) : BoundsFrame{};
+    BoundsFrame pfb = swept ? bounds_frame(P.bp) : BoundsFrame{};
+    const uint32_t la = uint32_t(am.leaves.size()), lb = uint32_t(bm.leaves.size());
+    const uint64_t cells = uint64_t(la) * lb;
+    constexpr uint32_t kWorkers = 128;
+    for (uint32_t w = 0; w < kWorkers; ++w)
+        for (uint64_t f = w; f < cells; f += kWorkers) {
+            const BvhNode &na = am.nodes[am.leaves[f / lb]], &nb = bm.nodes[bm.leaves[f % lb]];
+            V3 amn, amx, bmn, bmx;
+            motion_bounds(na.minimum, na.maximum, pfa, fa, swept, P.margin, amn, amx);
+            motion_bounds(nb.minimum, nb.maximum, pfb, fb, swept, 0.0f, bmn, bmx);
+            if (boxes_touch(amn, amx, bmn, bmx)) out.push_back({am.leaves[f / lb], bm.leaves[f % lb]});
+        }
+    return out.size() <= limit;
+}
+
 // ---------------------------------------------------------------------
 // Analytic spheres and capsules: no reference counterpart.  Contacts
 // follow the face path's conventions: the normal points from the collider
@@ -815,13 +1507,6 @@ bool analytic_manifold(const PmoState &as, const Shape &am, const PmoState &bs, 
         else push_sphere_contact(out, mul(n, -1.0f), sub(ends[k], mul(n, cap.sphere_radius)), gap, margin, reach);
     }
     return true;
-}
-
-// rigid_contact_normal_speed, geometry.slang:1272-1280.
-float contact_normal_speed(const PmoState &b, const PmoState &c, V3 p, V3 n) {
-    V3 bv = add(b.linear_velocity, cross(b.angular_velocity, sub(p, b.position)));
-    V3 cv = add(c.linear_velocity, cross(c.angular_velocity, sub(p, c.position)));
-    return dot(sub(bv, cv), n);
 }
 
 // ---------------------------------------------------------------------
@@ -1517,32 +2202,76 @@ void find_pairs(World &w, const std::vector<PmoState> &pred) {
     (void)pred;
 }
 
-// Per pair narrowphase: pm_prepare_rigid_face_manifolds (geometry.slang:
-// 2741-2803) for convex meshes, analytic round shapes otherwise, then
-// pm_finalize_rigid_contact_manifolds (:2805-2873).
+uint32_t slot_capacity(const World &w);
+
+// Per pair narrowphase, in the reference's dispatch order
+// (production_realtime_backend.cu.inc:7710-7776): leaf-pair candidates
+// (pm_generate_rigid_leaf_pairs, geometry.slang:2303-2492), the convex face
+// path (pm_prepare_rigid_face_manifolds, :2743-2803) which claims the pairs
+// it handles, the triangle path over the remaining pairs' leaf candidates
+// in candidate order (pm_evaluate_rigid_leaf_pairs, :2525-2706), and
+// pm_finalize_rigid_contact_manifolds (:2807-2873), which runs the BVH
+// traversal for pairs without a candidate slot or with too many
+// candidates.  Round shapes (no reference counterpart) take the analytic
+// path instead.
 void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<PmoState> &pred) {
     const uint32_t n = uint32_t(w.params.size());
+    const uint32_t slots = slot_capacity(w);
+    const uint32_t capacity = w.desc.body_capacity ? w.desc.body_capacity : n;
+    const uint32_t per_slot = capacity <= 8u ? 4096u : 512u;          // geometry_storage.hpp:165-166
     w.manifolds.assign(w.pairs.size(), PmoManifold{});
     w.stats.unsupported_pairs = 0;
+    w.stats.triangle_pairs = 0;
+    w.stats.fallback_pairs = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> candidates;
     for (size_t active = 0; active < w.pairs.size(); ++active) {
         uint32_t a = w.pairs[active] / n, b = w.pairs[active] % n;
         const Shape &sa = w.shapes[w.params[a].mesh_index], &sb = w.shapes[w.params[b].mesh_index];
         float margin = w.params[a].collision_margin + w.params[b].collision_margin;
         float cr = correction_reach(w.params[a], sa) + correction_reach(w.params[b], sb);
+        PmoHingeFrame ha, hb;
+        std::memset(&ha, 0, sizeof ha);
+        std::memset(&hb, 0, sizeof hb);
+        ha.static_body = w.params[a].motion == kStatic ? 1u : 0u;
+        hb.static_body = w.params[b].motion == kStatic ? 1u : 0u;
         PmoManifold m;
         clear_manifold(m);
-        if (convex_face_shape(sa) && convex_face_shape(sb)) {
+        if (sa.kind != kMesh || sb.kind != kMesh) {
+            if (!analytic_manifold(pred[a], sa, pred[b], sb, margin, margin + cr, m)) {
+                clear_manifold(m);
+                w.stats.unsupported_pairs++;
+            }
+        } else {
             bool swept = requires_swept(prev[a], pred[a], sa, prev[b], pred[b], sb, margin);
             bool face = false;
-            if (swept) face = swept_convex_face_manifold(prev[a], pred[a], sa, prev[b], pred[b], sb, margin, m);
-            if (!face) {
-                face = convex_face_manifold(pred[a], sa, pred[b], sb, margin, cr, m);
-                if (swept && m.count == 0u) face = false;
+            if (convex_face_shape(sa) && convex_face_shape(sb)) {
+                if (swept) face = swept_convex_face_manifold(prev[a], pred[a], sa, prev[b], pred[b], sb, margin, m);
+                if (!face) {
+                    face = convex_face_manifold(pred[a], sa, pred[b], sb, margin, cr, m);
+                    if (swept && m.count == 0u) face = false;
+                }
             }
-            if (!face) { clear_manifold(m); w.stats.unsupported_pairs++; }
-        } else if (!analytic_manifold(pred[a], sa, pred[b], sb, margin, margin + cr, m)) {
-            clear_manifold(m);
-            w.stats.unsupported_pairs++;
+            if (!face) {
+                clear_manifold(m);
+                TrianglePair P{&sa, &sb, prev[a], pred[a], prev[b], pred[b], ha, hb, margin, w.desc.dt};
+                w.stats.triangle_pairs++;
+                if (active < slots && leaf_candidates(P, swept, per_slot, candidates)) {
+                    const float spacing = fmax_(margin * 2.0f, 1.0e-4f);
+                    PmoManifold local;
+                    for (auto &c : candidates) {
+                        clear_manifold(local);
+                        collide_leaf_ranges(P, swept, sa.nodes[c.first], sb.nodes[c.second], local);
+                        for (uint32_t p = 0; p < local.count; ++p) add_contact(m, local.contacts[p], spacing);
+                    }
+                } else {
+                    w.stats.fallback_pairs++;
+                    collide_meshes(P, swept, m);
+                }
+            }
+        }
+        for (uint32_t p = 0; p < m.count; ++p) {
+            m.contacts[p].body_hinge = ha;
+            m.contacts[p].collider_hinge = hb;
         }
         bool persistent = (sa.kind != kMesh || (sa.has_planes && sa.index_count() <= 96u)) &&
             (w.params[b].motion != kDynamic || sb.kind != kMesh || (sb.has_planes && sb.index_count() <= 96u));
@@ -1851,6 +2580,69 @@ void pmo_world_get_mesh_planes(void *world, uint32_t shape, float *out) {
         out[4 * i] = s.planes[i].normal.x; out[4 * i + 1] = s.planes[i].normal.y;
         out[4 * i + 2] = s.planes[i].normal.z; out[4 * i + 3] = s.planes[i].offset;
     }
+}
+
+
+uint32_t pmo_world_get_mesh_bvh(void *world, uint32_t shape, uint32_t *nodes, float *bounds) {
+    auto &w = *static_cast<World *>(world);
+    if (shape >= w.shapes.size()) return 0;
+    const Shape &s = w.shapes[shape];
+    for (size_t k = 0; k < s.nodes.size(); ++k) {
+        const BvhNode &nd = s.nodes[k];
+        if (nodes) {
+            nodes[4 * k] = nd.left;
+            nodes[4 * k + 1] = nd.right;
+            nodes[4 * k + 2] = nd.first;
+            nodes[4 * k + 3] = nd.count;
+        }
+        if (bounds) {
+            const float b[6] = {nd.minimum.x, nd.minimum.y, nd.minimum.z, nd.maximum.x, nd.maximum.y, nd.maximum.z};
+            std::memcpy(bounds + 6 * k, b, sizeof b);
+        }
+    }
+    return uint32_t(s.nodes.size());
+}
+
+uint32_t pmo_world_get_mesh_shell(void *world, uint32_t shape, float *out) {
+    auto &w = *static_cast<World *>(world);
+    if (shape >= w.shapes.size() || w.shapes[shape].shell.empty()) return 0;
+    const Shape &s = w.shapes[shape];
+    if (out)
+        for (size_t t = 0; t < s.shell.size(); ++t) {
+            out[3 * t] = s.shell[t].x;
+            out[3 * t + 1] = s.shell[t].y;
+            out[3 * t + 2] = s.shell[t].z;
+        }
+    return uint32_t(s.shell.size());
+}
+
+static V3 v3_in(const float *f) { return {f[0], f[1], f[2]}; }
+static void v3_out(V3 v, float *f) { f[0] = v.x; f[1] = v.y; f[2] = v.z; }
+
+void pmo_closest_segments(const float p0[3], const float p1[3], const float q0[3], const float q1[3],
+                          float on_p[3], float on_q[3]) {
+    V3 u, w;
+    closest_segments(v3_in(p0), v3_in(p1), v3_in(q0), v3_in(q1), u, w);
+    v3_out(u, on_p);
+    v3_out(w, on_q);
+}
+
+void pmo_closest_on_triangle(const float p[3], const float tri[9], float out[3]) {
+    v3_out(closest_on_triangle(v3_in(p), v3_in(tri), v3_in(tri + 3), v3_in(tri + 6)), out);
+}
+
+void pmo_closest_triangle_pair(const float a[9], const float b[9], float on_a[3], float on_b[3]) {
+    V3 A[3] = {v3_in(a), v3_in(a + 3), v3_in(a + 6)}, B[3] = {v3_in(b), v3_in(b + 3), v3_in(b + 6)};
+    V3 u, w;
+    closest_triangle_pair(A, B, u, w);
+    v3_out(u, on_a);
+    v3_out(w, on_b);
+}
+
+void pmo_triangle_feature_normal(const float a[9], const float b[9], const float on_a[3], const float on_b[3],
+                                 const float fallback[3], float out[3]) {
+    V3 A[3] = {v3_in(a), v3_in(a + 3), v3_in(a + 6)}, B[3] = {v3_in(b), v3_in(b + 3), v3_in(b + 6)};
+    v3_out(triangle_feature_normal(A, B, v3_in(on_a), v3_in(on_b), v3_in(fallback)), out);
 }
 
 }  // extern "C"

@@ -174,9 +174,11 @@ typedef struct PmoWorldStats {
     uint32_t islands;
     uint32_t impact;               /* schedule counts[0] */
     uint32_t budget;
-    uint32_t unsupported_pairs;    /* would take the triangle-pair path */
+    uint32_t unsupported_pairs;    /* round-shape pairs with no analytic path */
     uint32_t cache_hits;           /* contact points restored from cache */
     uint32_t phases;
+    uint32_t triangle_pairs;       /* mesh pairs on the triangle-pair path */
+    uint32_t fallback_pairs;       /* of those, solved by BVH traversal */
 } PmoWorldStats;
 
 void *pmo_world_create(const PmoWorldDesc *desc);
@@ -215,6 +217,22 @@ void pmo_world_get_shape(void *world, uint32_t shape, float out[16]);
 /* Mesh indices after BVH reordering and per-triangle planes (4 floats). */
 void pmo_world_get_mesh_indices(void *world, uint32_t shape, uint32_t *out);
 void pmo_world_get_mesh_planes(void *world, uint32_t shape, float *out);
+
+/* Mesh BVH after upload preparation: per node 4 uints (left, right, first
+ * triangle, triangle count; leaves have count != 0) and 6 floats of local
+ * bounds.  Returns the node count; either output may be NULL. */
+uint32_t pmo_world_get_mesh_bvh(void *world, uint32_t shape, uint32_t *nodes, float *bounds);
+/* Certified shell normals, 3 floats per triangle in the reordered index
+ * order; returns 0 (and writes nothing) when the mesh has none. */
+uint32_t pmo_world_get_mesh_shell(void *world, uint32_t shape, float *out);
+
+/* ---- geometric kernels of the triangle-pair path (test hooks) -------- */
+void pmo_closest_segments(const float p0[3], const float p1[3], const float q0[3],
+                          const float q1[3], float on_p[3], float on_q[3]);
+void pmo_closest_on_triangle(const float p[3], const float tri[9], float out[3]);
+void pmo_closest_triangle_pair(const float a[9], const float b[9], float on_a[3], float on_b[3]);
+void pmo_triangle_feature_normal(const float a[9], const float b[9], const float on_a[3],
+                                 const float on_b[3], const float fallback[3], float out[3]);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,8 @@ class WorldDesc(ctypes.Structure):
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact",
-                "budget", "unsupported_pairs", "cache_hits", "phases"]
+                "budget", "unsupported_pairs", "cache_hits", "phases", "triangle_pairs",
+                "fallback_pairs"]
 
 
 class WorldStats(ctypes.Structure):
@@ -87,6 +88,14 @@ def lib():
         _lib.pmo_update_dual.argtypes = [_P, _F, _F, _F, _F, _F, _P]
         _lib.pmo_project_contact.argtypes = [_P, _F, _P]
         _lib.pmo_stiffness_scales.argtypes = [_P, _F, _P]
+        _lib.pmo_world_get_mesh_bvh.argtypes = [_P, _U, _P, _P]
+        _lib.pmo_world_get_mesh_bvh.restype = _U
+        _lib.pmo_world_get_mesh_shell.argtypes = [_P, _U, _P]
+        _lib.pmo_world_get_mesh_shell.restype = _U
+        _lib.pmo_closest_segments.argtypes = [_P] * 6
+        _lib.pmo_closest_on_triangle.argtypes = [_P] * 3
+        _lib.pmo_closest_triangle_pair.argtypes = [_P] * 4
+        _lib.pmo_triangle_feature_normal.argtypes = [_P] * 6
     return _lib
 
 
@@ -198,10 +207,56 @@ class OracleWorld:
         lib().pmo_world_get_mesh_indices(self._h, index, _ptr(out))
         return out
 
+    def mesh_bvh(self, index):
+        n = lib().pmo_world_get_mesh_bvh(self._h, index, None, None)
+        nodes = np.zeros((n, 4), np.uint32)
+        bounds = np.zeros((n, 6), np.float32)
+        lib().pmo_world_get_mesh_bvh(self._h, index, _ptr(nodes), _ptr(bounds))
+        return nodes, bounds
+
+    def mesh_shell(self, index):
+        n = lib().pmo_world_get_mesh_shell(self._h, index, None)
+        out = np.zeros((n, 3), np.float32)
+        if n:
+            lib().pmo_world_get_mesh_shell(self._h, index, _ptr(out))
+        return out
+
     def mesh_planes(self, index, count):
         out = np.zeros((count, 4), dtype=np.float32)
         lib().pmo_world_get_mesh_planes(self._h, index, _ptr(out))
         return out
+
+
+def closest_segments(p0, p1, q0, q1):
+    on_p, on_q = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    args = [np.ascontiguousarray(v, dtype=np.float32) for v in (p0, p1, q0, q1)]
+    lib().pmo_closest_segments(*[_ptr(a) for a in args], _ptr(on_p), _ptr(on_q))
+    return on_p, on_q
+
+
+def closest_on_triangle(p, tri):
+    out = np.zeros(3, np.float32)
+    pp = np.ascontiguousarray(p, dtype=np.float32)
+    t = np.ascontiguousarray(tri, dtype=np.float32).reshape(9)
+    lib().pmo_closest_on_triangle(_ptr(pp), _ptr(t), _ptr(out))
+    return out
+
+
+def closest_triangle_pair(a, b):
+    on_a, on_b = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    ta = np.ascontiguousarray(a, dtype=np.float32).reshape(9)
+    tb = np.ascontiguousarray(b, dtype=np.float32).reshape(9)
+    lib().pmo_closest_triangle_pair(_ptr(ta), _ptr(tb), _ptr(on_a), _ptr(on_b))
+    return on_a, on_b
+
+
+def triangle_feature_normal(a, b, on_a, on_b, fallback):
+    out = np.zeros(3, np.float32)
+    args = [np.ascontiguousarray(a, dtype=np.float32).reshape(9),
+            np.ascontiguousarray(b, dtype=np.float32).reshape(9)]
+    args += [np.ascontiguousarray(v, dtype=np.float32) for v in (on_a, on_b, fallback)]
+    lib().pmo_triangle_feature_normal(*[_ptr(x) for x in args], _ptr(out))
+    return out
 
 
 def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
