@@ -156,9 +156,12 @@ typedef struct PmhWorldDesc {
 
 typedef struct PmhWorldStats {
     uint32_t pairs, manifolds, points, colors, islands, impact, budget,
-        unsupported_pairs, cache_hits, phases;
+        unsupported_pairs,       /* round-shape pairs with no analytic path */
+        cache_hits, phases;
     uint32_t status;             /* sticky overflow bits: 1 pairs of a body, 2 points, 4 colours */
     uint32_t pair_capacity, point_capacity, points_allocated;
+    uint32_t triangle_pairs;     /* mesh pairs on the triangle-pair path */
+    uint32_t fallback_pairs;     /* of those, solved by BVH traversal (no candidate slot or too many) */
 } PmhWorldStats;
 
 /* Phase timings of the last step (milliseconds, HIP events on the

@@ -1180,6 +1180,7 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     }
     R(code, pairs); R(ab, pairs); R(m_count, pairs + 1); R(m_start, pairs + 1); R(m_slot, pairs + 1);
     R(m_impact, pairs); R(m_face, pairs); R(m_init_rel, pairs); R(adj, 2 * (size_t)pairs);
+    R(tri_list, pairs);
     R(c_np, points); R(c_pt, points); R(c_aa, points); R(c_ab, points); R(c_err, points);
     R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
     R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
@@ -1205,7 +1206,7 @@ void Solver::release() {
     deg.release(); adj_start.release(); adj_cursor.release(); color_bodies.release();
     bp_start.release(); anchored.release(); mean.release(); code.release(); ab.release();
     m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
-    m_init_rel.release(); adj.release(); c_np.release(); c_pt.release(); c_aa.release();
+    m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();

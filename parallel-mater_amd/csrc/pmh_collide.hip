@@ -17,8 +17,9 @@
 // Narrowphase: one thread per pair; convex meshes with <= 96 indices take
 // the reference's SAT + face clipping path (rigid_convex_face_manifold and
 // its swept variant, geometry.slang:1606-1853); analytic spheres/capsules
-// use closed forms.  Pairs the reference would send to its triangle-pair
-// path are counted and left without contacts.
+// use closed forms.  Mesh pairs the face path declines take the
+// triangle-pair path (leaf-pair candidates or BVH traversal) in a second
+// kernel over a compact list of those pairs.
 #include "pmh_internal.hpp"
 #include "pmh_world.hpp"
 
@@ -283,9 +284,14 @@ struct ShapeView {
     const V3 *v;
     const uint32_t *idx;
     const float4 *planes;
+    const BvhDev *nodes = nullptr;
+    const uint32_t *leaves = nullptr;
+    const float4 *shell = nullptr;
     __device__ float4 plane(uint32_t f) const { return planes[s.plane_off + f]; }
     __device__ V3 vert(uint32_t k) const { return v[s.vtx_off + k]; }
     __device__ uint32_t index(uint32_t k) const { return idx[s.idx_off + k]; }
+    __device__ const BvhDev &node(uint32_t k) const { return nodes[s.node_off + k]; }
+    __device__ uint32_t leaf(uint32_t k) const { return leaves[s.leaf_off + k]; }
 };
 
 // rigid_add_contact, geometry.slang:1129-1220 (unguided, no swept-witness
@@ -621,15 +627,593 @@ __device__ bool analytic_manifold(const PmhRigidBodyState &as, const ShapeDev &a
     return true;
 }
 
+// ---------------------------------------------------------------------
+// Triangle-pair narrowphase: mesh pairs the convex face path does not take
+// (edge-on-edge contacts of convex meshes, non-convex or large meshes).
+// Same steps and float operations as rigid_collide_triangle_ranges and its
+// swept variant (geometry.slang:949-1117, 1272-1584, 1855-2154), the
+// leaf-pair candidate order of pm_generate_rigid_leaf_pairs (:2303-2492),
+// the per-candidate merge of pm_evaluate_rigid_leaf_pairs (:2525-2706)
+// and the BVH traversal of rigid_collide_meshes (:2156-2285).  One thread
+// per listed pair.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ float normal_speed(const PmhRigidBodyState &b, const PmhRigidBodyState &c, V3 p, V3 n) {
+    const V3 bv = add(sv(b.linear_velocity), cross(sv(b.angular_velocity), sub(p, sv(b.position))));
+    const V3 cv = add(sv(c.linear_velocity), cross(sv(c.angular_velocity), sub(p, sv(c.position))));
+    return dot(sub(bv, cv), n);
+}
+
+__device__ void closest_segments(V3 p0, V3 p1, V3 q0, V3 q1, V3 &on_p, V3 &on_q) {
+    const V3 d0 = sub(p1, p0), d1 = sub(q1, q0), r = sub(p0, q0);
+    const float a = dot(d0, d0), e = dot(d1, d1);
+    float s = 0.0f, t = 0.0f;
+    if (a <= 1.0e-12f && e <= 1.0e-12f) { on_p = p0; on_q = q0; return; }
+    if (a <= 1.0e-12f) {
+        t = clampf(dot(d1, r) / e, 0.0f, 1.0f);
+    } else {
+        const float c = dot(d0, r);
+        if (e <= 1.0e-12f) {
+            s = clampf(-c / a, 0.0f, 1.0f);
+        } else {
+            const float b = dot(d0, d1), f = dot(d1, r);
+            const float den = a * e - b * b;
+            if (fabsf(den) > kEpsilon * a * e) s = clampf((b * f - c * e) / den, 0.0f, 1.0f);
+            t = (b * s + f) / e;
+            if (t < 0.0f) { t = 0.0f; s = clampf(-c / a, 0.0f, 1.0f); }
+            else if (t > 1.0f) { t = 1.0f; s = clampf((b - c) / a, 0.0f, 1.0f); }
+        }
+    }
+    on_p = add(p0, mul(d0, s));
+    on_q = add(q0, mul(d1, t));
+}
+
+__device__ V3 closest_on_triangle(V3 p, V3 a, V3 b, V3 c) {
+    const V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    const V3 bp = sub(p, b);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) return add(a, mul(ab, d1 / (d1 - d3)));
+    const V3 cp = sub(p, c);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) return add(a, mul(ac, d2 / (d2 - d6)));
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && d4 - d3 >= 0.0f && d5 - d6 >= 0.0f)
+        return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    const V3 n = cross(ab, ac);
+    return sub(p, mul(n, dot(ap, n) / len2(n)));
+}
+
+__device__ bool segment_hits_triangle(V3 p, V3 q, V3 a, V3 b, V3 c, V3 &hit) {
+    hit = {0.0f, 0.0f, 0.0f};
+    const V3 n = cross(sub(b, a), sub(c, a));
+    const V3 dir = sub(q, p);
+    const float den = dot(n, dir);
+    if (len2(n) <= kEpsilon * kEpsilon || fabsf(den) <= kEpsilon) return false;
+    const float amount = dot(n, sub(a, p)) / den;
+    if (amount < 0.0f || amount > 1.0f) return false;
+    hit = add(p, mul(dir, amount));
+    const float tol = -1.0e-5f * len2(n);
+    return dot(cross(sub(b, a), sub(hit, a)), n) >= tol && dot(cross(sub(c, b), sub(hit, b)), n) >= tol &&
+           dot(cross(sub(a, c), sub(hit, c)), n) >= tol;
+}
+
+__device__ __forceinline__ void keep_closer(V3 a, V3 b, float &best, V3 &best_a, V3 &best_b) {
+    const float d = len2(sub(a, b));
+    if (d < best) { best = d; best_a = a; best_b = b; }
+}
+
+__device__ __noinline__ void closest_triangle_pair(const V3 *A, const V3 *B, V3 &pa, V3 &pb) {
+    pa = {0.0f, 0.0f, 0.0f};
+    pb = {0.0f, 0.0f, 0.0f};
+    V3 hit;
+    for (uint32_t e = 0; e < 3; ++e)
+        if (segment_hits_triangle(A[e], A[(e + 1) % 3], B[0], B[1], B[2], hit)) { pa = hit; pb = hit; return; }
+    for (uint32_t e = 0; e < 3; ++e)
+        if (segment_hits_triangle(B[e], B[(e + 1) % 3], A[0], A[1], A[2], hit)) { pa = hit; pb = hit; return; }
+    float best = 3.402823466e+38f;
+    for (uint32_t i = 0; i < 3; ++i) keep_closer(A[i], closest_on_triangle(A[i], B[0], B[1], B[2]), best, pa, pb);
+    for (uint32_t i = 0; i < 3; ++i) keep_closer(closest_on_triangle(B[i], A[0], A[1], A[2]), B[i], best, pa, pb);
+    for (uint32_t i = 0; i < 3; ++i)
+        for (uint32_t j = 0; j < 3; ++j) {
+            V3 u, w;
+            closest_segments(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3], u, w);
+            keep_closer(u, w, best, pa, pb);
+        }
+}
+
+__device__ __forceinline__ bool boxes_touch(V3 amin, V3 amax, V3 bmin, V3 bmax) {
+    return amin.x <= bmax.x && amax.x >= bmin.x && amin.y <= bmax.y && amax.y >= bmin.y && amin.z <= bmax.z &&
+           amax.z >= bmin.z;
+}
+
+__device__ V3 advect_triangle_point(V3 q, const V3 *from, const V3 *to) {
+    const V3 ab = sub(from[1], from[0]), ac = sub(from[2], from[0]);
+    const V3 n = cross(ab, ac);
+    const float sq2 = len2(n);
+    if (sq2 <= 1.0e-20f) return to[0];
+    const V3 rel = sub(q, from[0]);
+    const float v = dot(cross(rel, ac), n) / sq2;
+    const float w = dot(cross(ab, rel), n) / sq2;
+    return add(to[0], add(mul(sub(to[1], to[0]), v), mul(sub(to[2], to[0]), w)));
+}
+
+// Certified outward direction of a (possibly swept) world triangle: unit
+// for exterior faces, a quarter for two-sided interior features, zero when
+// uncertified (rigid_shell_outward, geometry.slang:1308-1322).
+__device__ V3 shell_outward(const ShapeView &m, uint32_t t, const V3 *W) {
+    if (!m.s.has_shell) return {0.0f, 0.0f, 0.0f};
+    const V3 o = v3(m.shell[m.s.shell_off + t]);
+    const float sq2 = len2(o);
+    if (sq2 < 0.01f) return {0.0f, 0.0f, 0.0f};
+    const V3 la = m.vert(m.index(3 * t)), lb = m.vert(m.index(3 * t + 1)), lc = m.vert(m.index(3 * t + 2));
+    const float sign = dot(cross(sub(lb, la), sub(lc, la)), o) >= 0.0f ? 1.0f : -1.0f;
+    return mul(normalized_or(cross(sub(W[1], W[0]), sub(W[2], W[0])), {0.0f, 0.0f, 0.0f}),
+               sign * (sq2 > 0.5f ? 1.0f : 0.25f));
+}
+
+__device__ __forceinline__ bool shell_facing(V3 n, V3 oa, V3 ob) {
+    const float sa = len2(oa), sb = len2(ob);
+    return (sa < 0.01f || (sa > 0.5f ? dot(n, oa) < -1.0e-3f : fabsf(dot(n, oa)) > 2.5e-4f)) &&
+           (sb < 0.01f || (sb > 0.5f ? dot(n, ob) > 1.0e-3f : fabsf(dot(n, ob)) > 2.5e-4f));
+}
+
+__device__ bool point_inside_convex(const ShapeView &m, const PmhRigidBodyState &st, V3 world) {
+    if (!m.s.has_planes) return false;
+    const V3 l = inv_rotate(sq(st.orientation), sub(world, sv(st.position)));
+    const uint32_t tris = m.s.idx_count / 3u;
+    const uint32_t unique = m.s.unique_planes;
+    const uint32_t count = unique != 0u ? unique : tris;
+    const uint32_t first = unique != 0u ? tris : 0u;
+    for (uint32_t f = 0; f < count; ++f) {
+        const float4 pl = m.plane(first + f);
+        if (dot(v3(pl), l) > pl.w + 1.0e-5f) return false;
+    }
+    return true;
+}
+
+__device__ __noinline__ V3 triangle_feature_normal(const V3 *A, const V3 *B, V3 pa, V3 pb, V3 fallback) {
+    const V3 delta = sub(pa, pb);
+    V3 normal = normalized_or(delta, fallback);
+    const V3 faces[2] = {normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), normal),
+                         normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), normal)};
+    float align_best = -1.0f;
+    V3 interior_normal = {0.0f, 0.0f, 0.0f};
+    for (uint32_t side = 0; side < 2; ++side) {
+        const V3 *T = side == 0 ? A : B;
+        const V3 pt = side == 0 ? pa : pb;
+        const V3 raw = cross(sub(T[1], T[0]), sub(T[2], T[0]));
+        const float tol = 1.0e-5f * len2(raw);
+        bool interior = tol > 0.0f;
+        for (uint32_t e = 0; e < 3; ++e)
+            interior = interior && dot(cross(sub(T[(e + 1) % 3], T[e]), sub(pt, T[e])), raw) > tol;
+        const float alignment = fabsf(dot(faces[side], normal));
+        if (interior && alignment > align_best) {
+            align_best = alignment;
+            const float proj = dot(delta, faces[side]);
+            const float dir = fabsf(proj) > kEpsilon ? proj : dot(fallback, faces[side]);
+            interior_normal = dir >= 0.0f ? faces[side] : mul(faces[side], -1.0f);
+        }
+    }
+    if (align_best >= 0.0f) return interior_normal;
+    float best = 4.0f * kEpsilon * kEpsilon;
+    if (len2(delta) > kEpsilon * kEpsilon) {
+        for (uint32_t side = 0; side < 2; ++side) {
+            const float proj = dot(delta, faces[side]);
+            const float err = len2(sub(delta, mul(faces[side], proj)));
+            if (err < best) {
+                best = err;
+                normal = proj >= 0.0f ? faces[side] : mul(faces[side], -1.0f);
+            }
+        }
+    }
+    return normal;
+}
+
+// Inputs of one pair on the triangle path.  Without joints the hinge
+// frames carry only the static flag and the body reference is the body
+// position (rigid_hinge_contact_frame, geometry.slang:404-491).
+struct TrianglePair {
+    ShapeView am, bm;
+    PmhRigidBodyState ap, a, bp, b;   // previous and predicted states
+    uint32_t a_static, b_static;
+    float margin, dt;
+};
+
+__device__ __forceinline__ void world_triangle(const ShapeView &m, const PmhRigidBodyState &st, uint32_t t, V3 *W) {
+    for (uint32_t c = 0; c < 3; ++c) W[c] = transform_point(st, m.vert(m.index(3 * t + c)));
+}
+
+__device__ __noinline__ void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count,
+                                                     uint32_t b_first, uint32_t b_count, Manifold &m) {
+    const float margin = P.margin;
+    const float rest = smin(margin, 1.0e-3f);
+    const V3 body_ref = sv(P.a.position);
+    const bool inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
+    const bool inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+    for (uint32_t ai = 0; ai < a_count; ++ai) {
+        const uint32_t at = a_first + ai;
+        if (3u * at + 2u >= P.am.s.idx_count) break;
+        V3 A[3];
+        world_triangle(P.am, P.a, at, A);
+        for (uint32_t bi = 0; bi < b_count; ++bi) {
+            const uint32_t bt = b_first + bi;
+            if (3u * bt + 2u >= P.bm.s.idx_count) break;
+            V3 B[3];
+            world_triangle(P.bm, P.b, bt, B);
+            {
+                const V3 e = {margin, margin, margin};
+                const V3 amin = sub(cmin(A[0], cmin(A[1], A[2])), e), amax = add(cmax(A[0], cmax(A[1], A[2])), e);
+                const V3 bmin = cmin(B[0], cmin(B[1], B[2])), bmax = cmax(B[0], cmax(B[1], B[2]));
+                if (!boxes_touch(amin, amax, bmin, bmax)) continue;
+            }
+            const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
+            const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
+            const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
+            if (dot(oa, ob) > 0.9999f) continue;
+            V3 pa, pb;
+            closest_triangle_pair(A, B, pa, pb);
+            const V3 delta = sub(pa, pb);
+            const float sq2 = len2(delta);
+            if (sq2 > margin * margin) continue;
+            const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+            V3 fallback = dot(cn, sub(body_ref, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+            if (len2(cob) > 0.5f) fallback = cob;
+            const float distance = sqrtf(smax(sq2, 0.0f));
+            const V3 point = mul(add(pa, pb), 0.5f);
+            const V3 proj = sub(pa, mul(cn, dot(sub(pa, B[0]), cn)));
+            const bool on_face = len2(sub(proj, closest_on_triangle(proj, B[0], B[1], B[2]))) <= 1.0e-10f;
+            const V3 normal = (distance <= 1.0e-5f || on_face) ? fallback : normalized_or(delta, fallback);
+            if (on_face && len2(oa) < 0.01f && len2(ob) > 0.5f && P.bm.s.has_planes) {
+                const V3 face_a = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0.0f, 0.0f, 0.0f});
+                if (fabsf(dot(face_a, normal)) < 1.0e-3f) {
+                    const float depth = smax(0.0f, -smin(dot(sub(A[0], B[0]), normal),
+                                                         smin(dot(sub(A[1], B[0]), normal), dot(sub(A[2], B[0]), normal))));
+                    const V3 axis = inv_rotate(sq(P.b.orientation), face_a);
+                    const float offset = dot(sub(sv(P.b.position), A[0]), face_a);
+                    float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+                    for (uint32_t k = 0; k < P.bm.s.vtx_count; ++k) {
+                        const float pr = dot(P.bm.vert(k), axis) + offset;
+                        lo = smin(lo, pr);
+                        hi = smax(hi, pr);
+                    }
+                    const float alternate = smax(0.0f, smin(-lo, hi));
+                    if (alternate + 1.0e-5f < depth) continue;
+                }
+            }
+            const float separation = on_face && len2(cob) > 0.5f ? dot(delta, normal) : distance;
+            // rigid_contact_reaches_rest_offset (:1282-1290)
+            if (!(separation <= rest + 1.0e-5f)) {
+                const float speed = normal_speed(P.a, P.b, point, normal);
+                if (speed > 1.0e-5f || !(-speed * P.dt + rest + 1.0e-5f >= separation)) continue;
+            }
+            Contact c;
+            c.normal = normal;
+            c.point = point;
+            c.pen = rest - separation + 1.0e-5f;
+            bool recovered = false;
+            if (!P.am.s.has_planes && !P.bm.s.has_planes && distance <= 1.0e-5f) {
+                V3 oldA[3], oldB[3];
+                world_triangle(P.am, P.ap, at, oldA);
+                world_triangle(P.bm, P.bp, bt, oldB);
+                V3 opa, opb;
+                closest_triangle_pair(oldA, oldB, opa, opb);
+                V3 od = sub(opa, opb);
+                if (len2(od) > 1.0e-10f) {
+                    V3 IA[3], IB[3];
+                    float lo = 0.0f, hi = 1.0f;
+                    const float gap = smin(rest, 0.5f * sqrtf(len2(od)));
+                    for (uint32_t k = 0; k < 10u; ++k) {
+                        const float time = 0.5f * (lo + hi);
+                        for (uint32_t q = 0; q < 3; ++q) {
+                            IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), time));
+                            IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), time));
+                        }
+                        V3 u, w;
+                        closest_triangle_pair(IA, IB, u, w);
+                        if (len2(sub(u, w)) > gap * gap) lo = time;
+                        else hi = time;
+                    }
+                    for (uint32_t q = 0; q < 3; ++q) {
+                        IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), lo));
+                        IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), lo));
+                    }
+                    closest_triangle_pair(IA, IB, opa, opb);
+                    od = sub(opa, opb);
+                    c.normal = triangle_feature_normal(IA, IB, opa, opb, normalized_or(od, fallback));
+                    const V3 ca = advect_triangle_point(opa, IA, A), cb = advect_triangle_point(opb, IB, B);
+                    c.point = mul(add(ca, cb), 0.5f);
+                    c.pen = rest - dot(sub(ca, cb), c.normal) + 1.0e-5f;
+                    recovered = true;
+                }
+            }
+            if (distance <= 1.0e-5f) {
+                if (!recovered && on_face) {   // no fixed joints act on this path
+                    const float depth = smax(0.0f, -smin(dot(sub(A[0], pb), normal),
+                                                         smin(dot(sub(A[1], pb), normal), dot(sub(A[2], pb), normal))));
+                    c.pen = smin(margin, depth + rest) + 1.0e-5f;
+                } else if (!recovered) {
+                    c.pen = margin + 1.0e-5f;
+                }
+            }
+            if (!shell_facing(c.normal, coa, cob)) continue;
+            c.impact_fraction = 1.0f;
+            add_contact(m, c, smax(2.0f * margin, 1.0e-4f));
+        }
+    }
+}
+
+__device__ __noinline__ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first,
+                                                           uint32_t a_count, uint32_t b_first, uint32_t b_count,
+                                                           Manifold &m) {
+    const float margin = P.margin;
+    const float rest = smin(margin, 1.0e-3f);
+    const bool inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
+    const bool inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+    for (uint32_t ai = 0; ai < a_count; ++ai) {
+        const uint32_t at = a_first + ai;
+        if (3u * at + 2u >= P.am.s.idx_count) break;
+        V3 pA[3], cA[3];
+        world_triangle(P.am, P.ap, at, pA);
+        world_triangle(P.am, P.a, at, cA);
+        V3 amin = cmin(pA[0], cA[0]), amax = cmax(pA[0], cA[0]);
+        for (uint32_t q = 1; q < 3; ++q) {
+            amin = cmin(amin, cmin(pA[q], cA[q]));
+            amax = cmax(amax, cmax(pA[q], cA[q]));
+        }
+        for (uint32_t bi = 0; bi < b_count; ++bi) {
+            const uint32_t bt = b_first + bi;
+            if (3u * bt + 2u >= P.bm.s.idx_count) break;
+            V3 pB[3], cB[3];
+            world_triangle(P.bm, P.bp, bt, pB);
+            world_triangle(P.bm, P.b, bt, cB);
+            V3 bmin = cmin(pB[0], cB[0]), bmax = cmax(pB[0], cB[0]);
+            for (uint32_t q = 1; q < 3; ++q) {
+                bmin = cmin(bmin, cmin(pB[q], cB[q]));
+                bmax = cmax(bmax, cmax(pB[q], cB[q]));
+            }
+            const V3 e = {margin, margin, margin};
+            if (!boxes_touch(sub(amin, e), add(amax, e), bmin, bmax)) continue;
+            V3 dA[3], dB[3];
+            float speed_abs = 0.0f;
+            for (uint32_t q = 0; q < 3; ++q) {
+                dA[q] = sub(cA[q], pA[q]);
+                dB[q] = sub(cB[q], pB[q]);
+                speed_abs = smax(speed_abs, len(dA[q]));
+            }
+            float speed_b = 0.0f;
+            for (uint32_t q = 0; q < 3; ++q) speed_b = smax(speed_b, len(dB[q]));
+            speed_abs += speed_b;
+            const V3 common = dB[0];
+            float rel_a = 0.0f, rel_b = 0.0f;
+            for (uint32_t q = 0; q < 3; ++q) {
+                rel_a = smax(rel_a, len(sub(dA[q], common)));
+                rel_b = smax(rel_b, len(sub(dB[q], common)));
+            }
+            const float bound = smin(speed_abs, rel_a + rel_b);
+            if (bound <= kEpsilon) continue;
+            float time = 0.0f;
+            bool starts_near = false;
+            V3 sep_normal = {0.0f, 0.0f, 0.0f};
+            bool have_sep = false;
+            for (uint32_t it = 0; it < 32u; ++it) {
+                V3 A[3], B[3];
+                for (uint32_t q = 0; q < 3; ++q) {
+                    A[q] = add(pA[q], mul(dA[q], time));
+                    B[q] = add(pB[q], mul(dB[q], time));
+                }
+                V3 pa, pb;
+                closest_triangle_pair(A, B, pa, pb);
+                const V3 delta = sub(pa, pb);
+                const float distance = sqrtf(smax(0.0f, len2(delta)));
+                const float offset = rest;
+                if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
+                if (distance <= offset + 1.0e-5f) {
+                    const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+                    const V3 center = add(sv(P.ap.position), mul(sub(sv(P.a.position), sv(P.ap.position)), time));
+                    V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+                    const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
+                    const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
+                    const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
+                    if (dot(oa, ob) > 0.9999f) break;
+                    if (len2(cob) > 0.5f) fallback = cob;
+                    else if (len2(coa) > 0.5f) fallback = mul(coa, -1.0f);
+                    const V3 normal = triangle_feature_normal(A, B, pa, pb, have_sep ? sep_normal : fallback);
+                    if (!shell_facing(normal, coa, cob)) break;
+                    const V3 point = mul(add(pa, pb), 0.5f);
+                    const float vn = normal_speed(P.a, P.b, point, normal);
+                    if (vn > 1.0e-5f) break;
+                    const float remaining = -vn * P.dt * (1.0f - time) - distance;
+                    Contact c;
+                    c.normal = normal;
+                    c.point = point;
+                    c.pen = smax(0.0f, remaining + offset) + 1.0e-5f;
+                    c.impact_fraction = starts_near ? 0.0f : time;
+                    add_contact(m, c, smax(margin * 2.0f, 1.0e-4f));
+                    break;
+                }
+                sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
+                have_sep = true;
+                const float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+                time += smax(advance, 1.0e-5f);
+                if (time > 1.0f) break;
+            }
+        }
+    }
+}
+
+// BoundsTransform and transformed_(motion_)bounds, math.slang:142-190.
+struct BoundsFrame {
+    V3 position, ax, ay, az;
+};
+__device__ __forceinline__ BoundsFrame bounds_frame(const PmhRigidBodyState &s) {
+    const Q q = sq(s.orientation);
+    return {sv(s.position), rotate(q, {1.0f, 0.0f, 0.0f}), rotate(q, {0.0f, 1.0f, 0.0f}), rotate(q, {0.0f, 0.0f, 1.0f})};
+}
+__device__ __forceinline__ void frame_bounds(V3 lmin, V3 lmax, const BoundsFrame &f, float margin, V3 &mn, V3 &mx) {
+    const V3 c = mul(add(lmin, lmax), 0.5f), h = mul(sub(lmax, lmin), 0.5f);
+    const V3 wc = add(f.position, add(mul(f.ax, c.x), add(mul(f.ay, c.y), mul(f.az, c.z))));
+    const V3 wh = {fabsf(f.ax.x) * h.x + fabsf(f.ay.x) * h.y + fabsf(f.az.x) * h.z,
+                   fabsf(f.ax.y) * h.x + fabsf(f.ay.y) * h.y + fabsf(f.az.y) * h.z,
+                   fabsf(f.ax.z) * h.x + fabsf(f.ay.z) * h.y + fabsf(f.az.z) * h.z};
+    const V3 e = {margin, margin, margin};
+    mn = sub(sub(wc, wh), e);
+    mx = add(add(wc, wh), e);
+}
+__device__ __forceinline__ void motion_bounds(V3 lmin, V3 lmax, const BoundsFrame &prev, const BoundsFrame &cur,
+                                              bool swept, float margin, V3 &mn, V3 &mx) {
+    frame_bounds(lmin, lmax, cur, margin, mn, mx);
+    if (!swept) return;
+    V3 pmn, pmx;
+    frame_bounds(lmin, lmax, prev, margin, pmn, pmx);
+    mn = cmin(mn, pmn);
+    mx = cmax(mx, pmx);
+}
+__device__ __forceinline__ void node_bounds(const BvhDev &nd, const BoundsFrame &prev, const BoundsFrame &cur,
+                                            bool swept, float margin, V3 &mn, V3 &mx) {
+    motion_bounds(sarr(nd.mn), sarr(nd.mx), prev, cur, swept, margin, mn, mx);
+}
+
+__device__ __forceinline__ void collide_leaf_ranges(const TrianglePair &P, bool swept, const BvhDev &na,
+                                                    const BvhDev &nb, Manifold &m) {
+    collide_triangle_ranges(P, na.first, na.count, nb.first, nb.count, m);
+    if (swept) collide_triangle_ranges_swept(P, na.first, na.count, nb.first, nb.count, m);
+}
+
+// rigid_collide_meshes: simultaneous descent with a 64-entry stack, all
+// triangles when the stack would overflow.
+__device__ __noinline__ void collide_meshes(const TrianglePair &P, bool swept, Manifold &m) {
+    m.count = 0;
+    m.face_patch = 0;
+    const BoundsFrame fa = bounds_frame(P.a), fb = bounds_frame(P.b);
+    const BoundsFrame pfa = swept ? bounds_frame(P.ap) : BoundsFrame{};
+    const BoundsFrame pfb = swept ? bounds_frame(P.bp) : BoundsFrame{};
+    V3 amn, amx, bmn, bmx;
+    motion_bounds(sarr(P.am.s.mn), sarr(P.am.s.mx), pfa, fa, swept, P.margin, amn, amx);
+    motion_bounds(sarr(P.bm.s.mn), sarr(P.bm.s.mx), pfb, fb, swept, 0.0f, bmn, bmx);
+    if (!boxes_touch(amn, amx, bmn, bmx)) return;
+    constexpr uint32_t kStack = 64;
+    uint32_t sa[kStack], sb[kStack];
+    sa[0] = 0;
+    sb[0] = 0;
+    uint32_t top = 1;
+    bool overflow = P.am.s.node_count == 0u || P.bm.s.node_count == 0u;
+    while (top > 0u && !overflow) {
+        --top;
+        const uint32_t ia = sa[top], ib = sb[top];
+        const BvhDev na = P.am.node(ia), nb = P.bm.node(ib);
+        node_bounds(na, pfa, fa, swept, P.margin, amn, amx);
+        node_bounds(nb, pfb, fb, swept, 0.0f, bmn, bmx);
+        if (!boxes_touch(amn, amx, bmn, bmx)) continue;
+        const bool la = na.count != 0u, lb = nb.count != 0u;
+        if (la && lb) {
+            collide_leaf_ranges(P, swept, na, nb, m);
+            continue;
+        }
+        const uint32_t need = la || lb ? 2u : 4u;
+        if (top + need > kStack) { overflow = true; break; }
+        if (la) {
+            sa[top] = ia; sb[top++] = nb.right;
+            sa[top] = ia; sb[top++] = nb.left;
+        } else if (lb) {
+            sa[top] = na.right; sb[top++] = ib;
+            sa[top] = na.left; sb[top++] = ib;
+        } else {
+            sa[top] = na.right; sb[top++] = nb.right;
+            sa[top] = na.right; sb[top++] = nb.left;
+            sa[top] = na.left; sb[top++] = nb.right;
+            sa[top] = na.left; sb[top++] = nb.left;
+        }
+    }
+    if (overflow) {
+        m.count = 0;
+        m.face_patch = 0;
+        collide_triangle_ranges(P, 0, P.am.s.idx_count / 3u, 0, P.bm.s.idx_count / 3u, m);
+        if (swept) collide_triangle_ranges_swept(P, 0, P.am.s.idx_count / 3u, 0, P.bm.s.idx_count / 3u, m);
+    }
+}
+
+// Number of (body leaf, collider leaf) pairs whose motion bounds overlap.
+__device__ uint32_t count_leaf_candidates(const TrianglePair &P, bool swept, const BoundsFrame &fa,
+                                          const BoundsFrame &fb, const BoundsFrame &pfa, const BoundsFrame &pfb) {
+    const uint32_t la = P.am.s.leaf_count, lb = P.bm.s.leaf_count;
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < la; ++i) {
+        V3 amn, amx;
+        node_bounds(P.am.node(P.am.leaf(i)), pfa, fa, swept, P.margin, amn, amx);
+        for (uint32_t j = 0; j < lb; ++j) {
+            V3 bmn, bmx;
+            node_bounds(P.bm.node(P.bm.leaf(j)), pfb, fb, swept, 0.0f, bmn, bmx);
+            total += boxes_touch(amn, amx, bmn, bmx) ? 1u : 0u;
+        }
+    }
+    return total;
+}
+
+// Narrowphase outputs: per-pair records and the point arrays, into which
+// each non-empty manifold writes a contiguous range.
+struct NarrowOut {
+    uint32_t *m_count, *m_start, *m_face;
+    float4 *m_init_rel;
+    float4 *c_np, *c_pt, *c_aa, *c_ab, *c_err, *c_lam, *c_pen, *c_acc, *c_rep;
+    uint32_t *c_flags, *c_pair;
+    uint32_t point_cap;
+    StepCounters *ctr;
+};
+
+// pm_finalize_rigid_contact_manifolds (geometry.slang:2805-2873): the
+// initial relative position, persistence and the initial normal speed, and
+// the point range.  Where the range lies does not affect any result.
+__device__ void emit_manifold(const NarrowOut &o, uint32_t m, Manifold &man, const PmhRigidBodyState &A,
+                              const PmhRigidBodyState &B, bool persistent) {
+    uint32_t start = 0;
+    if (man.count) {
+        start = atomicAdd(&o.ctr->point_alloc, man.count);
+        if (start + man.count > o.point_cap) {
+            atomicOr(&o.ctr->status, kStatusPointOverflow);
+            man.count = 0;
+        }
+    }
+    o.m_count[m] = man.count;
+    o.m_start[m] = start;
+    o.m_face[m] = man.face_patch;
+    o.m_init_rel[m] = f4(sub(sv(A.position), sv(B.position)), 0.0f);
+    for (uint32_t k = 0; k < man.count; ++k) {
+        const Contact &c = man.c[k];
+        const float speed = normal_speed(A, B, c.point, c.normal);
+        const uint32_t i = start + k;
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        o.c_np[i] = f4(c.normal, c.pen);
+        o.c_pt[i] = f4(c.point, speed);
+        o.c_aa[i] = make_float4(0.0f, 0.0f, 0.0f, c.impact_fraction);   // w: impact fraction
+        o.c_ab[i] = z;
+        o.c_err[i] = z;
+        o.c_lam[i] = z;
+        o.c_pen[i] = z;
+        o.c_acc[i] = z;
+        o.c_rep[i] = z;
+        o.c_flags[i] = persistent ? 2u : 0u;
+        o.c_pair[i] = m;
+    }
+}
+
+__device__ __forceinline__ bool persistent_pair(const ShapeDev &a, const ShapeDev &b, const PmhBodyParameters &pb) {
+    const auto sup = [](const ShapeDev &s) { return s.kind != kShapeMesh || (s.has_planes && s.idx_count <= 96u); };
+    return sup(a) && (pb.motion != kDynamic || sup(b));
+}
+
+// One thread per pair: the convex face path (and its swept variant) for
+// convex meshes, closed forms for round shapes.  Mesh pairs the face path
+// declines are queued for the triangle-pair kernel.
 __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, const PmhBodyParameters *P,
                                                      const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
                                                      const float4 *planes, const PmhRigidBodyState *prev,
-                                                     const PmhRigidBodyState *pred, const uint2 *ab,
-                                                     uint32_t *m_count, uint32_t *m_start, uint32_t *m_face,
-                                                     float4 *m_init_rel, float4 *c_np, float4 *c_pt, float4 *c_aa,
-                                                     float4 *c_ab, float4 *c_err, float4 *c_lam, float4 *c_pen,
-                                                     float4 *c_acc, float4 *c_rep, uint32_t *c_flags,
-                                                     uint32_t *c_pair, uint32_t point_cap, StepCounters *ctr) {
+                                                     const PmhRigidBodyState *pred, const uint2 *ab, NarrowOut o,
+                                                     uint32_t *tri_list) {
     const uint32_t M = *npairs;
     uint32_t unsupported = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
@@ -643,58 +1227,90 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
         Manifold man;
         man.count = 0;
         man.face_patch = 0;
-        if (sa.s.convex_face && sb.s.convex_face) {
-            const bool swept = requires_swept(A0, A, sa.s, B0, B, sb.s, margin);
-            bool face = false;
-            if (swept) face = swept_convex_face_manifold(A0, A, sa, B0, B, sb, margin, man);
-            if (!face) {
-                face = convex_face_manifold(A, sa, B, sb, margin, cr, man);
-                if (swept && man.count == 0u) face = false;
-            }
-            if (!face) { man.count = 0; man.face_patch = 0; ++unsupported; }
-        } else if (!analytic_manifold(A, sa.s, B, sb.s, margin, margin + cr, man)) {
-            man.count = 0;
-            man.face_patch = 0;
-            ++unsupported;
-        }
-        const auto sup = [](const ShapeDev &s) { return s.kind != kShapeMesh || (s.has_planes && s.idx_count <= 96u); };
-        const bool persistent = sup(sa.s) && (pb.motion != kDynamic || sup(sb.s));
-        // Each manifold owns a contiguous range of the point arrays; where
-        // that range lies does not affect any result.
-        uint32_t start = 0;
-        if (man.count) {
-            start = atomicAdd(&ctr->point_alloc, man.count);
-            if (start + man.count > point_cap) {
-                atomicOr(&ctr->status, kStatusPointOverflow);
+        if (sa.s.kind != kShapeMesh || sb.s.kind != kShapeMesh) {
+            if (!analytic_manifold(A, sa.s, B, sb.s, margin, margin + cr, man)) {
                 man.count = 0;
+                man.face_patch = 0;
+                ++unsupported;
+            }
+        } else {
+            bool face = false;
+            if (sa.s.convex_face && sb.s.convex_face) {
+                const bool swept = requires_swept(A0, A, sa.s, B0, B, sb.s, margin);
+                if (swept) face = swept_convex_face_manifold(A0, A, sa, B0, B, sb, margin, man);
+                if (!face) {
+                    face = convex_face_manifold(A, sa, B, sb, margin, cr, man);
+                    if (swept && man.count == 0u) face = false;
+                }
+            }
+            if (!face) {
+                tri_list[atomicAdd(&o.ctr->tri_pairs, 1u)] = m;
+                continue;
             }
         }
-        m_count[m] = man.count;
-        m_start[m] = start;
-        m_face[m] = man.face_patch;
-        m_init_rel[m] = f4(sub(sv(A.position), sv(B.position)), 0.0f);
-        for (uint32_t k = 0; k < man.count; ++k) {
-            const Contact &c = man.c[k];
-            // pm_finalize_rigid_contact_manifolds (geometry.slang:2805-2873)
-            const V3 av = add(sv(A.linear_velocity), cross(sv(A.angular_velocity), sub(c.point, sv(A.position))));
-            const V3 bv = add(sv(B.linear_velocity), cross(sv(B.angular_velocity), sub(c.point, sv(B.position))));
-            const float speed = dot(sub(av, bv), c.normal);
-            const uint32_t i = start + k;
-            const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            c_np[i] = f4(c.normal, c.pen);
-            c_pt[i] = f4(c.point, speed);
-            c_aa[i] = make_float4(0.0f, 0.0f, 0.0f, c.impact_fraction);   // w: impact fraction
-            c_ab[i] = z;
-            c_err[i] = z;
-            c_lam[i] = z;
-            c_pen[i] = z;
-            c_acc[i] = z;
-            c_rep[i] = z;
-            c_flags[i] = persistent ? 2u : 0u;
-            c_pair[i] = m;
-        }
+        emit_manifold(o, m, man, A, B, persistent_pair(sa.s, sb.s, pb));
     }
-    if (unsupported) atomicAdd(&ctr->unsupported, unsupported);
+    if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
+}
+
+// The triangle-pair path for the queued pairs.  A pair with a candidate
+// slot (active index below `slots`) and at most `limit` overlapping leaf
+// pairs evaluates its candidates in the order 128 workers emit them (each
+// worker takes every 128th cell of the row-major leaf-pair grid) and
+// merges each candidate's manifold in that order; any other pair runs the
+// BVH traversal.
+__global__ void __launch_bounds__(64) k_triangle_pairs(const uint32_t *tri_list, const PmhBodyParameters *P,
+                                                       const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
+                                                       const float4 *planes, const BvhDev *nodes,
+                                                       const uint32_t *leaves, const float4 *shell,
+                                                       const PmhRigidBodyState *prev, const PmhRigidBodyState *pred,
+                                                       const uint2 *ab, NarrowOut o, uint32_t slots, uint32_t limit,
+                                                       float dt) {
+    const uint32_t T = o.ctr->tri_pairs;
+    uint32_t fallback = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        const uint32_t m = tri_list[t];
+        const uint2 p = ab[m];
+        const PmhBodyParameters pa = P[p.x], pb = P[p.y];
+        TrianglePair tp{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                        ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                        prev[p.x], pred[p.x], prev[p.y], pred[p.y],
+                        pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
+                        pa.collision_margin + pb.collision_margin, dt};
+        const bool swept = requires_swept(tp.ap, tp.a, tp.am.s, tp.bp, tp.b, tp.bm.s, tp.margin);
+        const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
+        const BoundsFrame pfa = swept ? bounds_frame(tp.ap) : BoundsFrame{};
+        const BoundsFrame pfb = swept ? bounds_frame(tp.bp) : BoundsFrame{};
+        Manifold man;
+        man.count = 0;
+        man.face_patch = 0;
+        bool fits = m < slots;
+        if (fits) fits = count_leaf_candidates(tp, swept, fa, fb, pfa, pfb) <= limit;
+        if (fits) {
+            const uint32_t lb = tp.bm.s.leaf_count;
+            const uint64_t cells = uint64_t(tp.am.s.leaf_count) * lb;
+            const float spacing = smax(tp.margin * 2.0f, 1.0e-4f);
+            for (uint32_t w = 0; w < 128u && w < cells; ++w)
+                for (uint64_t f = w; f < cells; f += 128u) {
+                    const BvhDev na = tp.am.node(tp.am.leaf(uint32_t(f / lb)));
+                    const BvhDev nb = tp.bm.node(tp.bm.leaf(uint32_t(f % lb)));
+                    V3 amn, amx, bmn, bmx;
+                    node_bounds(na, pfa, fa, swept, tp.margin, amn, amx);
+                    node_bounds(nb, pfb, fb, swept, 0.0f, bmn, bmx);
+                    if (!boxes_touch(amn, amx, bmn, bmx)) continue;
+                    Manifold local;
+                    local.count = 0;
+                    local.face_patch = 0;
+                    collide_leaf_ranges(tp, swept, na, nb, local);
+                    for (uint32_t k = 0; k < local.count; ++k) add_contact(man, local.c[k], spacing);
+                }
+        } else {
+            ++fallback;
+            collide_meshes(tp, swept, man);
+        }
+        emit_manifold(o, m, man, tp.a, tp.b, persistent_pair(tp.am.s, tp.bm.s, pb));
+    }
+    if (fallback) atomicAdd(&o.ctr->fallback_pairs, fallback);
 }
 
 __global__ void k_nonempty(const uint32_t *npairs, const uint32_t *m_count, uint32_t *flag) {
@@ -828,6 +1444,8 @@ __global__ void k_step_begin(StepCounters *ctr) {
     ctr->large_count = 0;
     ctr->color_ticket = 0;
     ctr->point_alloc = 0;
+    ctr->tri_pairs = 0;
+    ctr->fallback_pairs = 0;
     ctr->cell_size_bits = 0.0f;
 }
 
@@ -873,10 +1491,15 @@ PmhResult World::step_once(uint32_t k_step) {
                                                   cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr);
     mark(2);
     // Narrowphase + point compaction.
+    const NarrowOut out{v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
+                        v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p,
+                        v.c_pair.p, v.point_cap, ctr};
     k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
-        &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p,
-        v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p,
-        v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p, v.c_pair.p, v.point_cap, ctr);
+        &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p, out,
+        v.tri_list.p);
+    k_triangle_pairs<<<blocks_for(v.pair_cap, 64, 2048), 64, 0, s>>>(
+        v.tri_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, nodes_dev.p, leaves_dev.p,
+        shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt);
     k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap);
     mark(3);
     // Response slots and the contact cache.

@@ -39,6 +39,19 @@ struct ShapeDev {
     float center[3], half_length;
     float box_half[3];
     uint32_t is_box;   // centred axis-aligned 8-vertex box
+    uint32_t node_off, node_count, leaf_off, leaf_count;   // BVH nodes and leaf-node list
+    uint32_t shell_off, has_shell;                         // certified outward normal per triangle
+    uint32_t unique_planes;                                // deduplicated planes after the per-triangle ones
+};
+
+// Median-split BVH node over the reordered triangles: leaves (count != 0)
+// hold triangles [first, first + count); inner nodes their two children.
+struct BvhDev {
+    float mn[3];
+    uint32_t first;
+    float mx[3];
+    uint32_t count;
+    uint32_t left, right, pad0, pad1;
 };
 
 struct HostShape {
@@ -46,6 +59,9 @@ struct HostShape {
     std::vector<V3> vertices;
     std::vector<uint32_t> indices;
     std::vector<float> planes;   // 4 per triangle plane, then the unique tail
+    std::vector<BvhDev> nodes;   // preorder
+    std::vector<uint32_t> leaves;
+    std::vector<V3> shell;       // empty when nothing is certified
     uint32_t unique_planes = 0;
     float unit_inertia[3]{};
 };
@@ -67,7 +83,7 @@ struct StepCounters {
     uint32_t colors;           // colour count
     uint32_t islands;
     uint32_t impact;
-    uint32_t unsupported;      // pairs the triangle-pair path would take
+    uint32_t unsupported;      // round-shape pairs with no analytic path
     uint32_t cache_hits;
     uint32_t status;
     uint32_t large_count;      // bodies outside the hash grid
@@ -82,7 +98,9 @@ struct StepCounters {
     uint32_t dyn_bodies, impact_bodies;
     uint32_t m1, m2, p1, p2, m1i, m2i, p1i, p2i;
     uint32_t point_alloc;      // narrowphase point-range allocator
-    uint32_t pad[6];
+    uint32_t tri_pairs;        // mesh pairs on the triangle-pair path
+    uint32_t fallback_pairs;   // of those, solved by BVH traversal
+    uint32_t pad[4];
 };
 
 // Running totals over every step since the last reset.
@@ -130,6 +148,7 @@ struct Solver {
     DBuf<uint32_t> m_count, m_start, m_slot, m_impact, m_face;
     DBuf<float4> m_init_rel;
     DBuf<uint32_t> adj;                  // 2 per pair
+    DBuf<uint32_t> tri_list;             // pairs for the triangle-pair path
     // points
     DBuf<float4> c_np, c_pt, c_aa, c_ab, c_err, c_lam, c_pen, c_acc, c_rep;
     DBuf<uint32_t> c_flags, c_pair;

@@ -73,6 +73,117 @@ static std::vector<std::array<float, 4>> convex_planes(const std::vector<V3> &v,
     return out;
 }
 
+// Outward normals certified per triangle at upload, as closed_shell_normals
+// (runtime_domains.cu.inc:1111-1245): vertices welded by position are
+// grouped into connected components; a convex component takes its face
+// planes' normals, a closed orientable one the winding normal signed by its
+// volume, an open volumetric one the side of each support plane (or a
+// quarter-length two-sided marker on interior triangles).  Empty when no
+// triangle is certified.
+static std::vector<V3> shell_normals(const std::vector<V3> &v, const std::vector<uint32_t> &idx) {
+    const uint32_t nv = uint32_t(v.size());
+    std::map<std::array<float, 3>, uint32_t> weld;
+    std::vector<uint32_t> rep(nv), parent(nv);
+    for (uint32_t i = 0; i < nv; ++i) {
+        rep[i] = weld.emplace(std::array<float, 3>{v[i].x, v[i].y, v[i].z}, i).first->second;
+        parent[i] = i;
+    }
+    auto root = [&](uint32_t i) {
+        while (parent[i] != i) { parent[i] = parent[parent[i]]; i = parent[i]; }
+        return i;
+    };
+    for (size_t t = 0; t < idx.size(); t += 3)
+        for (size_t c = 1; c < 3; ++c) {
+            const uint32_t r0 = root(rep[idx[t]]), rc = root(rep[idx[t + c]]);
+            parent[rc] = r0;
+        }
+    const uint32_t tris = uint32_t(idx.size() / 3);
+    std::map<uint32_t, std::vector<uint32_t>> parts;
+    for (uint32_t t = 0; t < tris; ++t) parts[root(rep[idx[3 * t]])].push_back(t);
+    std::vector<V3> out(tris, V3{0.0f, 0.0f, 0.0f});
+    bool certified = false;
+    for (const auto &part : parts) {
+        const std::vector<uint32_t> &ts = part.second;
+        std::map<uint32_t, uint32_t> local;
+        std::vector<V3> pts;
+        std::vector<uint32_t> faces;
+        for (uint32_t t : ts)
+            for (uint32_t c = 0; c < 3; ++c) {
+                const uint32_t g = rep[idx[3 * t + c]];
+                auto ins = local.emplace(g, uint32_t(pts.size()));
+                if (ins.second) pts.push_back(v[g]);
+                faces.push_back(ins.first->second);
+            }
+        const auto planes = convex_planes(pts, faces);
+        if (!planes.empty()) {
+            certified = true;
+            for (size_t i = 0; i < ts.size(); ++i) out[ts[i]] = {planes[i][0], planes[i][1], planes[i][2]};
+            continue;
+        }
+        std::map<std::pair<uint32_t, uint32_t>, std::pair<uint32_t, int>> edges;
+        double volume = 0.0;
+        for (size_t f = 0; f < faces.size(); f += 3) {
+            for (size_t c = 0; c < 3; ++c) {
+                const uint32_t e0 = faces[f + c], e1 = faces[f + (c + 1) % 3];
+                auto &use = edges[{std::min(e0, e1), std::max(e0, e1)}];
+                use.first += 1;
+                use.second += e0 < e1 ? 1 : -1;
+            }
+            double d[3][3];
+            for (size_t c = 0; c < 3; ++c) {
+                const V3 q = pts[faces[f + c]];
+                d[c][0] = double(q.x) - pts[0].x;
+                d[c][1] = double(q.y) - pts[0].y;
+                d[c][2] = double(q.z) - pts[0].z;
+            }
+            volume += d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) +
+                      d[0][1] * (d[1][2] * d[2][0] - d[1][0] * d[2][2]) +
+                      d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+        }
+        bool closed = volume != 0.0 && std::isfinite(volume);
+        for (const auto &e : edges) closed = closed && e.second.first == 2u && e.second.second == 0;
+        if (closed) {
+            certified = true;
+            for (size_t i = 0; i < ts.size(); ++i) {
+                const V3 a = pts[faces[3 * i]], b = pts[faces[3 * i + 1]], c = pts[faces[3 * i + 2]];
+                out[ts[i]] = mul(normalized_or(cross(sub(b, a), sub(c, a)), {0.0f, 0.0f, 0.0f}),
+                                 volume > 0.0 ? 1.0f : -1.0f);
+            }
+            continue;
+        }
+        float scale = 0.0f;
+        for (const V3 &q : pts) scale = std::max(scale, len(sub(q, pts[0])));
+        const float tol = std::max(1.0e-7f, scale * 1.0e-5f);
+        for (size_t i = 0; i < ts.size(); ++i) {
+            const V3 a = pts[faces[3 * i]];
+            const V3 nrm = normalized_or(cross(sub(pts[faces[3 * i + 1]], a), sub(pts[faces[3 * i + 2]], a)),
+                                         {0.0f, 0.0f, 0.0f});
+            float lo = 0.0f, hi = 0.0f;
+            for (const V3 &q : pts) {
+                const float d = dot(nrm, sub(q, a));
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
+            }
+            if (hi <= tol && lo < -tol) {
+                out[ts[i]] = nrm;
+            } else if (lo >= -tol && hi > tol) {
+                out[ts[i]] = mul(nrm, -1.0f);
+            } else {
+                bool interior = true;
+                for (size_t c = 0; c < 3; ++c) {
+                    const uint32_t e0 = faces[3 * i + c], e1 = faces[3 * i + (c + 1) % 3];
+                    const auto use = edges.at({std::min(e0, e1), std::max(e0, e1)});
+                    interior = interior && use.first == 2u && use.second == 0;
+                }
+                if (!interior) continue;
+                out[ts[i]] = mul(nrm, 0.25f);
+            }
+            certified = true;
+        }
+    }
+    return certified ? out : std::vector<V3>{};
+}
+
 static PmhResult prepare_mesh(const float *vp, uint32_t nv, const uint32_t *ip, uint32_t ni, HostShape &sh) {
     if (!vp || !ip || nv < 3 || ni < 3 || ni % 3) return PMH_ERROR_INVALID_ARGUMENT;
     std::vector<V3> v(nv);
@@ -95,11 +206,22 @@ static PmhResult prepare_mesh(const float *vp, uint32_t nv, const uint32_t *ip, 
     auto centroid = [&](uint32_t t) {
         return mul(add(add(v[in[3 * t]], v[in[3 * t + 1]]), v[in[3 * t + 2]]), 1.0f / 3.0f);
     };
-    std::function<void(uint32_t, uint32_t)> split = [&](uint32_t b, uint32_t e) {
+    sh.nodes.clear();
+    sh.nodes.reserve(2 * tris);
+    std::function<uint32_t(uint32_t, uint32_t)> split = [&](uint32_t b, uint32_t e) -> uint32_t {
         V3 lo{FLT_MAX, FLT_MAX, FLT_MAX}, hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
         for (uint32_t k = b; k < e; ++k)
             for (uint32_t c = 0; c < 3; ++c) { lo = cmin(lo, v[in[3 * order[k] + c]]); hi = cmax(hi, v[in[3 * order[k] + c]]); }
-        if (e - b <= 4) return;
+        const uint32_t self = uint32_t(sh.nodes.size());
+        BvhDev node{};
+        node.mn[0] = lo.x; node.mn[1] = lo.y; node.mn[2] = lo.z;
+        node.mx[0] = hi.x; node.mx[1] = hi.y; node.mx[2] = hi.z;
+        sh.nodes.push_back(node);
+        if (e - b <= 4) {
+            sh.nodes[self].first = b;
+            sh.nodes[self].count = e - b;
+            return self;
+        }
         const V3 ext = sub(hi, lo);
         const int axis = ext.x >= ext.y && ext.x >= ext.z ? 0 : (ext.y >= ext.z ? 1 : 2);
         std::stable_sort(order.begin() + b, order.begin() + e, [&](uint32_t x, uint32_t y) {
@@ -107,15 +229,22 @@ static PmhResult prepare_mesh(const float *vp, uint32_t nv, const uint32_t *ip, 
             return fx < fy || (fx == fy && x < y);
         });
         const uint32_t mid = b + (e - b) / 2;
-        split(b, mid);
-        split(mid, e);
+        const uint32_t l = split(b, mid);
+        const uint32_t r = split(mid, e);
+        sh.nodes[self].left = l;
+        sh.nodes[self].right = r;
+        return self;
     };
     split(0, tris);
+    sh.leaves.clear();
+    for (uint32_t k = 0; k < sh.nodes.size(); ++k)
+        if (sh.nodes[k].count != 0u) sh.leaves.push_back(k);
     sh.indices.resize(ni);
     for (uint32_t t = 0; t < tris; ++t)
         for (uint32_t c = 0; c < 3; ++c) sh.indices[3 * t + c] = in[3 * order[t] + c];
     sh.vertices = v;
     auto planes = convex_planes(v, sh.indices);
+    sh.shell = shell_normals(v, sh.indices);
     sh.planes.clear();
     sh.unique_planes = 0;
     if (!planes.empty()) {
@@ -132,6 +261,7 @@ static PmhResult prepare_mesh(const float *vp, uint32_t nv, const uint32_t *ip, 
     d.vtx_count = nv;
     d.idx_count = ni;
     d.has_planes = planes.empty() ? 0u : 1u;
+    d.unique_planes = sh.unique_planes;
     d.convex_face = d.has_planes && ni <= 96u;
     d.mn[0] = mn.x; d.mn[1] = mn.y; d.mn[2] = mn.z;
     d.mx[0] = mx.x; d.mx[1] = mx.y; d.mx[2] = mx.z;
@@ -174,21 +304,37 @@ PmhResult World::ensure_device() {
         std::vector<ShapeDev> info;
         std::vector<V3> verts;
         std::vector<uint32_t> idx;
-        std::vector<float4> planes;
+        std::vector<float4> planes, shell;
+        std::vector<BvhDev> nodes;
+        std::vector<uint32_t> leaves;
         for (HostShape &s : shapes) {
             ShapeDev d = s.info;
             d.vtx_off = uint32_t(verts.size());
             d.idx_off = uint32_t(idx.size());
             d.plane_off = uint32_t(planes.size());
+            d.node_off = uint32_t(nodes.size());
+            d.node_count = uint32_t(s.nodes.size());
+            d.leaf_off = uint32_t(leaves.size());
+            d.leaf_count = uint32_t(s.leaves.size());
+            d.shell_off = uint32_t(shell.size());
+            d.has_shell = s.shell.empty() ? 0u : 1u;
             verts.insert(verts.end(), s.vertices.begin(), s.vertices.end());
             idx.insert(idx.end(), s.indices.begin(), s.indices.end());
+            nodes.insert(nodes.end(), s.nodes.begin(), s.nodes.end());
+            leaves.insert(leaves.end(), s.leaves.begin(), s.leaves.end());
+            for (const V3 &o : s.shell) shell.push_back(make_float4(o.x, o.y, o.z, 0.0f));
             for (size_t k = 0; k < s.planes.size(); k += 4)
                 planes.push_back(make_float4(s.planes[k], s.planes[k + 1], s.planes[k + 2], s.planes[k + 3]));
             info.push_back(d);
         }
         if ((r = shapes_dev.reserve(info.size())) || (r = verts_dev.reserve(verts.size())) ||
-            (r = idx_dev.reserve(idx.size())) || (r = planes_dev.reserve(planes.size())))
+            (r = idx_dev.reserve(idx.size())) || (r = planes_dev.reserve(planes.size())) ||
+            (r = nodes_dev.reserve(nodes.size())) || (r = leaves_dev.reserve(leaves.size())) ||
+            (r = shell_dev.reserve(shell.size())))
             return r;
+        if (!nodes.empty()) PMH_CHECK(hipMemcpy(nodes_dev.p, nodes.data(), nodes.size() * sizeof(BvhDev), hipMemcpyHostToDevice));
+        if (!leaves.empty()) PMH_CHECK(hipMemcpy(leaves_dev.p, leaves.data(), leaves.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (!shell.empty()) PMH_CHECK(hipMemcpy(shell_dev.p, shell.data(), shell.size() * sizeof(float4), hipMemcpyHostToDevice));
         if (!info.empty()) PMH_CHECK(hipMemcpy(shapes_dev.p, info.data(), info.size() * sizeof(ShapeDev), hipMemcpyHostToDevice));
         if (!verts.empty()) PMH_CHECK(hipMemcpy(verts_dev.p, verts.data(), verts.size() * sizeof(V3), hipMemcpyHostToDevice));
         if (!idx.empty()) PMH_CHECK(hipMemcpy(idx_dev.p, idx.data(), idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -223,6 +369,9 @@ PmhResult World::ensure_device() {
         }
         const uint64_t cap_n = body_capacity ? body_capacity : n;
         cache_capacity = uint32_t(std::min<uint64_t>(cap_n * (cap_n - 1) / 2, std::max<uint64_t>(4096, 8 * cap_n)));
+        // resource_plan.hpp:151-164, geometry_storage.hpp:164-166
+        leaf_slots = cache_capacity;
+        leaf_limit = cap_n <= 8 ? 4096u : 512u;
         cache_capacity = std::max<uint32_t>(1, cache_capacity);
         if ((r = params.reserve(n)) || (r = state_cur.reserve(n)) || (r = state_prev.reserve(n)) ||
             (r = state_next.reserve(n)) || (r = proxy_mn.reserve(n)) || (r = proxy_mx.reserve(n)) ||
@@ -250,6 +399,7 @@ PmhResult World::ensure_device() {
 void World::release() {
     if (stream) (void)hipStreamSynchronize(stream);
     shapes_dev.release(); verts_dev.release(); idx_dev.release(); planes_dev.release();
+    nodes_dev.release(); leaves_dev.release(); shell_dev.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -486,6 +636,8 @@ PmhResult pmhWorldGetStats(PmhWorld world, PmhWorldStats *stats) {
     stats->pair_capacity = w->solver.pair_cap;
     stats->point_capacity = w->solver.point_cap;
     stats->points_allocated = c.point_alloc;
+    stats->triangle_pairs = c.tri_pairs;
+    stats->fallback_pairs = c.fallback_pairs;
     return PMH_SUCCESS;
 }
 
@@ -554,8 +706,10 @@ PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *out) {
     std::vector<uint32_t> count, start, slot, face;
     std::vector<float4> init, np, pt, aa, ab, err, lam, pen, acc, rep;
     std::vector<uint32_t> flags;
+    std::vector<uint2> pair_ab;
     PmhResult r;
-    if ((r = fetch(count, v.m_count.p, M)) || (r = fetch(start, v.m_start.p, M)) || (r = fetch(slot, v.m_slot.p, M)) ||
+    if ((r = fetch(pair_ab, v.ab.p, M)) ||
+        (r = fetch(count, v.m_count.p, M)) || (r = fetch(start, v.m_start.p, M)) || (r = fetch(slot, v.m_slot.p, M)) ||
         (r = fetch(face, v.m_face.p, M)) || (r = fetch(init, v.m_init_rel.p, M)) || (r = fetch(np, v.c_np.p, P)) ||
         (r = fetch(pt, v.c_pt.p, P)) || (r = fetch(aa, v.c_aa.p, P)) || (r = fetch(ab, v.c_ab.p, P)) ||
         (r = fetch(err, v.c_err.p, P)) || (r = fetch(lam, v.c_lam.p, P)) || (r = fetch(pen, v.c_pen.p, P)) ||
@@ -588,6 +742,9 @@ PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *out) {
             t.reported_friction_impulse = {rep[i].y, rep[i].z, rep[i].w};
             t.avbd_cached = flags[i] & 1u;
             t.persistent = (flags[i] >> 1) & 1u;
+            // Hinge frames without joints carry only the static flag.
+            t.body_hinge.static_body = w->host_params[pair_ab[m].x].motion == 0u ? 1u : 0u;
+            t.collider_hinge.static_body = w->host_params[pair_ab[m].y].motion == 0u ? 1u : 0u;
         }
     }
     return PMH_SUCCESS;

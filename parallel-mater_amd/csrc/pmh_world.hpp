@@ -28,6 +28,9 @@ struct World {
     DBuf<V3> verts_dev;
     DBuf<uint32_t> idx_dev;
     DBuf<float4> planes_dev;
+    DBuf<BvhDev> nodes_dev;
+    DBuf<uint32_t> leaves_dev;
+    DBuf<float4> shell_dev;
     DBuf<PmhBodyParameters> params;
     DBuf<PmhRigidBodyState> state_cur, state_prev, state_next;
 
@@ -40,6 +43,7 @@ struct World {
 
     // contact cache, double buffered
     uint32_t cache_capacity = 0;
+    uint32_t leaf_slots = 0, leaf_limit = 512;   // triangle-path candidate slots and per-pair limit
     int cache_parity = 0;
     DBuf<uint32_t> cache_code[2], cache_count[2];
     DBuf<float4> cache_data[2];

@@ -49,7 +49,7 @@ class WorldDesc(ctypes.Structure):
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
                 "unsupported_pairs", "cache_hits", "phases", "status", "pair_capacity",
-                "point_capacity", "points_allocated"]
+                "point_capacity", "points_allocated", "triangle_pairs", "fallback_pairs"]
 
 
 class WorldStats(ctypes.Structure):

@@ -40,7 +40,7 @@ def cube_sphere_mesh(radius):
 
 def params(n, motion=L.MOTION_DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0),
            friction=0.5, restitution=0.0, margin=0.01, shape=0,
-           max_linear=1000.0, max_angular=1000.0):
+           max_linear=1000.0, max_angular=1000.0, damping=0.0):
     p = np.zeros(n, dtype=L.PARAMS)
     p["motion"] = motion
     p["mesh_index"] = shape
@@ -54,6 +54,8 @@ def params(n, motion=L.MOTION_DYNAMIC, mass=1.0, inertia=(1.0, 1.0, 1.0),
     p["friction"] = friction
     p["restitution"] = restitution
     p["collision_margin"] = margin
+    p["linear_damping"] = damping
+    p["angular_damping"] = damping
     p["maximum_linear_speed"] = max_linear
     p["maximum_angular_speed"] = max_angular
     return p
@@ -97,10 +99,19 @@ def box_inertia(mass, half):
             m * (hx * hx + hy * hy) / np.float32(3))
 
 
-def pyramid(rows=10, seed=1, jitter=1e-3, margin=0.01, friction=0.5):
-    """55 unit boxes (10 rows), density 1000,
-    friction 0.5 on a static ground box (half 50, 0.5, 50) at y = -0.5,
-    +-1 mm seeded position jitter.  Body 0 is the ground."""
+# The reference's default rigid-body options (RigidBodyOptions,
+# src/simulation/types.hpp:643-660): collision margin 5 mm, linear and
+# angular damping 0.05, speed limits 100 m/s and 100 rad/s.  Scenes that
+# state no other value use these.
+REF_MARGIN, REF_DAMPING, REF_MAX_SPEED = 0.005, 0.05, 100.0
+
+
+def pyramid(rows=10, seed=1, jitter=1e-3, margin=REF_MARGIN, friction=0.5, damping=REF_DAMPING,
+            max_speed=REF_MAX_SPEED):
+    """Pyramid of 55 unit boxes (10 rows), density 1000, friction 0.5 on a
+    static ground box (half 50, 0.5, 50) at y = -0.5, +-1 mm seeded
+    position jitter, reference default body options.  Body 0 is the
+    ground."""
     rng = np.random.default_rng(seed)
     shapes = [("mesh", *box_mesh(50.0, 0.5, 50.0)), ("mesh", *box_mesh(0.5, 0.5, 0.5))]
     pos = [(0.0, -0.5, 0.0)]
@@ -114,18 +125,19 @@ def pyramid(rows=10, seed=1, jitter=1e-3, margin=0.01, friction=0.5):
     motion = np.full(n, L.MOTION_DYNAMIC, dtype=np.uint32)
     motion[0] = L.MOTION_STATIC
     p = params(n, motion=motion, mass=1000.0, inertia=box_inertia(1000.0, (0.5, 0.5, 0.5)),
-               friction=friction, margin=margin, shape=np.where(motion == 0, 0, 1))
+               friction=friction, margin=margin, shape=np.where(motion == 0, 0, 1), damping=damping,
+               max_linear=max_speed, max_angular=max_speed)
     return Scene(shapes, p, states(n, pos), f"pyramid{rows}")
 
 
 def pile(side=32, seed=2, spacing=0.8, jitter=0.05, analytic_spheres=True,
-         margin=0.01, height=0.6):
-    """BASELINE configuration 2 at side**3 bodies: half boxes (edges
-    U[0.2, 0.6] m) and half spheres (r U[0.1, 0.3] m) on a jittered grid of
-    `spacing`, density log-uniform [200, 5000] kg/m^3, friction U[0.3, 0.8],
-    above a static floor and inside four static walls.  Bodies 0-4 are the
-    bin (floor first).  Spheres are analytic or the reference's polyhedral
-    mesh."""
+         margin=REF_MARGIN, height=0.6, damping=REF_DAMPING, max_speed=REF_MAX_SPEED):
+    """Pile of side**3 bodies: half boxes (edges U[0.2, 0.6] m) and half
+    spheres (r U[0.1, 0.3] m) on a jittered grid of `spacing`, density
+    log-uniform [200, 5000] kg/m^3, friction U[0.3, 0.8], reference default
+    body options, above a static floor and inside four static walls.
+    Bodies 0-4 are the bin (floor first).  Spheres are analytic or the
+    reference's polyhedral mesh."""
     rng = np.random.default_rng(seed)
     count = side ** 3
     extent = side * spacing
@@ -176,7 +188,8 @@ def pile(side=32, seed=2, spacing=0.8, jitter=0.05, analytic_spheres=True,
     all_inertia = np.concatenate([np.ones((5, 3), np.float32), inertia])
     all_mu = np.concatenate([np.full(5, 0.5, np.float32), friction])
     p = params(n, motion=motion, mass=all_mass, inertia=all_inertia, friction=all_mu,
-               margin=margin, shape=np.concatenate([np.array(bin_shape, np.uint32), body_shape]))
+               margin=margin, shape=np.concatenate([np.array(bin_shape, np.uint32), body_shape]),
+               damping=damping, max_linear=max_speed, max_angular=max_speed)
     s = states(n, np.concatenate([np.array(bin_pos, np.float32), pos]))
     return Scene(shapes, p, s, f"pile{side}")
 
@@ -341,3 +354,84 @@ def check_stack_window(out):
     assert out["rot"] <= 0.035
     assert out["energy_growth"] <= 0.05
     assert out["tunnel"] == 0
+
+
+def l_prism(flip=False):
+    """Closed concave L-shaped prism: cross-section (0,0) (1,0) (2,0) (2,1)
+    (1,1) (1,2) (0,2) in xy, extruded over z in [-1, 1], outward wound
+    (inward when `flip`)."""
+    poly = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [1, 1], [1, 2], [0, 2]], np.float32)
+    m = len(poly)
+    v = np.zeros((2 * m, 3), np.float32)
+    v[:m, :2] = poly
+    v[:m, 2] = 1.0
+    v[m:, :2] = poly
+    v[m:, 2] = -1.0
+    cap = [(0, 1, 4), (0, 4, 6), (4, 5, 6), (1, 2, 3), (1, 3, 4)]
+    idx = []
+    for t in cap:
+        idx += list(t)                                   # front, counter-clockwise from +z
+        idx += [m + t[0], m + t[2], m + t[1]]            # back
+    for i in range(m):
+        j = (i + 1) % m
+        idx += [i, m + i, m + j, i, m + j, j]
+    idx = np.array(idx, np.uint32).reshape(-1, 3)
+    if flip:
+        idx = idx[:, ::-1]
+    return v, np.ascontiguousarray(idx.reshape(-1))
+
+
+def quat(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    s = np.sin(angle / 2)
+    return np.array([axis[0] * s, axis[1] * s, axis[2] * s, np.cos(angle / 2)], np.float32)
+
+
+def concave_drop(count=12, seed=5, speed=8.0):
+    """Boxes (half 0.1-0.25 m) at random orientations thrown down at up to
+    `speed` m/s onto a static concave L-shaped prism (l_prism) and into its
+    inner corner: every pair takes the triangle-pair path, fast ones its
+    swept pass.  Body 0 is the prism, body 1 a static floor box under it
+    (top face at y = 0)."""
+    rng = np.random.default_rng(seed)
+    shapes = [("mesh", *l_prism()), ("mesh", *box_mesh(6.0, 0.5, 6.0))]
+    halves = rng.uniform(0.1, 0.25, size=(count, 3)).astype(np.float32)
+    for h in halves:
+        shapes.append(("mesh", *box_mesh(*h)))
+    n = count + 2
+    motion = np.full(n, L.MOTION_DYNAMIC, dtype=np.uint32)
+    motion[:2] = L.MOTION_STATIC
+    mass = np.ones(n, np.float32)
+    inertia = np.ones((n, 3), np.float32)
+    for k, h in enumerate(halves):
+        mass[k + 2] = 8.0 * h[0] * h[1] * h[2] * 1000.0
+        inertia[k + 2] = box_inertia(mass[k + 2], h)
+    p = params(n, motion=motion, mass=mass, inertia=inertia, margin=0.01,
+               shape=np.arange(n))
+    pos = np.zeros((n, 3), np.float32)
+    pos[1] = (0.0, -0.5, 0.0)
+    pos[2:, 0] = rng.uniform(0.2, 1.8, size=count)
+    pos[2:, 1] = rng.uniform(2.4, 3.6, size=count)
+    pos[2:, 2] = rng.uniform(-0.8, 0.8, size=count)
+    st = states(n, pos)
+    for k in range(2, n):
+        st["orientation"][k] = quat(rng.normal(size=3), rng.uniform(0, np.pi))
+    st["linear_velocity"][2:, 1] = -rng.uniform(0, speed, size=count).astype(np.float32)
+    st["angular_velocity"][2:] = rng.uniform(-3, 3, size=(count, 3)).astype(np.float32)
+    return Scene(shapes, p, st, "concave_drop")
+
+
+def crossing_edges(gap=5e-4, half=0.5):
+    """Two boxes meeting edge on edge (the convex face path declines this
+    pair): the static one turned 45 degrees about z, the dynamic one 45
+    degrees about x and `gap` above it."""
+    h = np.float32(half)
+    top = h * np.float32(np.sqrt(2.0))
+    shapes = [("mesh", *box_mesh(h, h, h))]
+    motion = np.array([L.MOTION_STATIC, L.MOTION_DYNAMIC], np.uint32)
+    p = params(2, motion=motion, mass=1000.0, inertia=box_inertia(1000.0, (h, h, h)), shape=np.array([0, 0]))
+    st = states(2, np.array([[0, 0, 0], [0, 2 * top + np.float32(gap), 0]], np.float32))
+    st["orientation"][0] = quat((0, 0, 1), np.pi / 4)
+    st["orientation"][1] = quat((1, 0, 0), np.pi / 4)
+    return Scene(shapes, p, st, "crossing_edges")

@@ -87,14 +87,7 @@ def test_pyramid_single_step(profile):
     g.step(1)
     o.step(1)
     np.testing.assert_array_equal(g.pairs(), o.pairs())
-    gm, om = g.manifolds(), o.manifolds()
-    np.testing.assert_array_equal(gm["count"], om["count"])
-    # Only slots below `count` are contacts: face-patch pruning leaves the
-    # tail of the fixed 8-slot record stale in the reference layout.
-    valid = np.arange(8)[None, :] < gm["count"][:, None]
-    for f in ("normal", "point", "penetration", "impact_fraction", "avbd_dual", "avbd_anchor_a",
-              "avbd_anchor_b"):
-        np.testing.assert_array_equal(gm["contacts"][f][valid], om["contacts"][f][valid], err_msg=f)
+    assert_manifolds_equal(g.manifolds(), o.manifolds())
     gb, ob = g.bodies(), o.bodies()
     np.testing.assert_array_equal(gb["color"], ob["color"])
     np.testing.assert_array_equal(gb["parent"], ob["parent"])
@@ -103,11 +96,76 @@ def test_pyramid_single_step(profile):
     say(f"bit-exact state fraction {exact_fraction(gs.view(np.float32), os_.view(np.float32)):.4f}")
     assert_states_close(gs, os_)
     assert g.stats()["colors"] == o.stats()["colors"]
-    assert g.stats()["unsupported_pairs"] == o.stats()["unsupported_pairs"]
+    for f in ("unsupported_pairs", "triangle_pairs", "fallback_pairs"):
+        assert g.stats()[f] == o.stats()[f], f
     # Reference accumulation order stays within the single-step tolerance.
     r = oracle_world(scene, lanes=0, profile=profile, base_iterations=10)
     r.step(1)
     assert_states_close(gs, r.states(), rtol=1e-4, atol=1e-4)
+
+
+def assert_manifolds_equal(gm, om):
+    np.testing.assert_array_equal(gm["count"], om["count"])
+    # Only slots below `count` are contacts: face-patch pruning leaves the
+    # tail of the fixed 8-slot record stale in the reference layout.
+    valid = np.arange(8)[None, :] < gm["count"][:, None]
+    for f in ("normal", "point", "penetration", "impact_fraction", "avbd_dual", "avbd_anchor_a",
+              "avbd_anchor_b"):
+        np.testing.assert_array_equal(gm["contacts"][f][valid], om["contacts"][f][valid], err_msg=f)
+    for h in ("body_hinge", "collider_hinge"):
+        np.testing.assert_array_equal(gm["contacts"][h]["static_body"][valid],
+                                      om["contacts"][h]["static_body"][valid], err_msg=h)
+
+
+def test_crossing_edges_take_the_triangle_path():
+    say("crossing box edges: triangle-pair path")
+    scene = S.crossing_edges()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    g.step(1)
+    o.step(1)
+    assert g.stats()["triangle_pairs"] == o.stats()["triangle_pairs"] == 1
+    assert_manifolds_equal(g.manifolds(), o.manifolds())
+    assert_states_close(g.states(), o.states())
+
+
+def test_concave_drop_trajectory():
+    say("boxes thrown onto a concave prism, 90 steps")
+    scene = S.concave_drop()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    worst = 0.0
+    for k in range(90):
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs(), err_msg=f"step {k}")
+        gs, os_ = g.stats(), o.stats()
+        assert gs["triangle_pairs"] == os_["triangle_pairs"], k
+        if k == 0:
+            assert_manifolds_equal(g.manifolds(), o.manifolds())
+        worst = max(worst, float(np.abs(g.states()["position"] - o.states()["position"]).max()))
+    say(f"worst position difference {worst:.3e}")
+    assert worst < 1e-3
+
+
+def test_triangle_path_fallback_matches_oracle():
+    say("mesh pile side 8: triangle pairs beyond the candidate slots take the BVH traversal")
+    scene = S.pile(side=8, seed=3, analytic_spheres=False)
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    fallback = 0
+    for k in range(40):
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs(), err_msg=f"step {k}")
+        gs, os_ = g.stats(), o.stats()
+        assert (gs["triangle_pairs"], gs["fallback_pairs"]) == (os_["triangle_pairs"], os_["fallback_pairs"]), k
+        fallback += gs["fallback_pairs"]
+        if gs["fallback_pairs"] and fallback == gs["fallback_pairs"]:
+            assert_manifolds_equal(g.manifolds(), o.manifolds())
+    say(f"fallback pairs over 40 steps: {fallback}")
+    assert fallback > 0
+    assert_states_close(g.states(), o.states(), rtol=1e-3, atol=1e-3)
 
 
 @pytest.mark.parametrize("lanes", [8, 16, 64])

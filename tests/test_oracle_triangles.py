@@ -174,31 +174,6 @@ def test_feature_normal_face_and_edge_cases():
 
 
 # ---- meshes ------------------------------------------------------------------
-def l_prism(flip=False):
-    """Closed concave L-shaped prism: cross-section (0,0) (1,0) (2,0) (2,1)
-    (1,1) (1,2) (0,2) in xy, extruded over z in [-1, 1], outward wound
-    (inward when `flip`)."""
-    poly = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [1, 1], [1, 2], [0, 2]], np.float32)
-    m = len(poly)
-    v = np.zeros((2 * m, 3), np.float32)
-    v[:m, :2] = poly
-    v[:m, 2] = 1.0
-    v[m:, :2] = poly
-    v[m:, 2] = -1.0
-    cap = [(0, 1, 4), (0, 4, 6), (4, 5, 6), (1, 2, 3), (1, 3, 4)]
-    idx = []
-    for t in cap:
-        idx += list(t)                                   # front, counter-clockwise from +z
-        idx += [m + t[0], m + t[2], m + t[1]]            # back
-    for i in range(m):
-        j = (i + 1) % m
-        idx += [i, m + i, m + j, i, m + j, j]
-    idx = np.array(idx, np.uint32).reshape(-1, 3)
-    if flip:
-        idx = idx[:, ::-1]
-    return v, np.ascontiguousarray(idx.reshape(-1))
-
-
 def world_with(meshes):
     w = O.OracleWorld(gravity=(0, 0, 0), profile=0)
     for v, i in meshes:
@@ -240,7 +215,7 @@ def outward_normals_of_l(tris):
 
 @pytest.mark.parametrize("flip", [False, True])
 def test_shell_normals_of_closed_concave_prism(flip):
-    v, i = l_prism(flip)
+    v, i = S.l_prism(flip)
     w = world_with([(v, i)])
     assert w.shape(0)[15] == 0                      # not convex: no solid planes
     shell = w.mesh_shell(0)
@@ -376,7 +351,7 @@ def test_crossing_box_edges_take_the_triangle_path():
 def test_box_resting_on_concave_mesh():
     h = np.float32(0.1)
     gap = np.float32(5e-4)
-    w = two_body_world(S.box_mesh(h, h, h), l_prism(), (0.5, 2 + h + gap, 0.0), rot((0, 1, 0), 0.0),
+    w = two_body_world(S.box_mesh(h, h, h), S.l_prism(), (0.5, 2 + h + gap, 0.0), rot((0, 1, 0), 0.0),
                        (0, 0, 0), rot((0, 1, 0), 0.0))
     w.step(1)
     assert w.stats()["triangle_pairs"] == 1
@@ -398,7 +373,7 @@ def test_box_resting_on_concave_mesh():
 
 def test_fast_box_does_not_tunnel_into_concave_mesh():
     h = np.float32(0.1)
-    w = two_body_world(S.box_mesh(h, h, h), l_prism(), (0.5, 2.6, 0.0), rot((0, 1, 0), 0.3),
+    w = two_body_world(S.box_mesh(h, h, h), S.l_prism(), (0.5, 2.6, 0.0), rot((0, 1, 0), 0.3),
                        (0, 0, 0), rot((0, 1, 0), 0.0), vel_a=(0, -30.0, 0))
     lowest = np.inf
     for k in range(20):
