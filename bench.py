@@ -4,21 +4,24 @@
 
 Workload: a 32768-body pile (half boxes with edges U[0.2, 0.6] m, half
 analytic spheres with radius U[0.1, 0.3] m, densities log-uniform in
-[200, 5000] kg/m^3, friction U[0.3, 0.8]) dropped from a jittered 32^3
-grid into a static five-wall bin; dt = 1/60, legacy AVBD profile with 10
-iterations (pose phase and velocity phase), FP32.  W untimed steps, then
-exactly K timed steps (defaults 60 + 540 = 600 steps).
+[200, 5000] kg/m^3, friction U[0.3, 0.8], the reference's default body
+options: margin 5 mm, damping 0.05, speed limits 100 m/s and 100 rad/s)
+dropped from a jittered 32^3 grid into a static five-wall bin; dt = 1/60,
+legacy AVBD profile with 10 iterations (pose phase and velocity phase),
+FP32.  W untimed steps, then exactly K timed steps (defaults 60 + 540 =
+600 steps).
 
-value   = AVBD body-iterations per second of wall time over the K steps
-          (whole steps: predictor, broadphase, narrowphase, contact cache,
-          prepare, solve);
-roofline = the dominant kernel group, the AVBD solve phase of a step (per
-          iteration one primal kernel per colour and one dual kernel, for the
-          pose and velocity phases), algorithmic bytes per step over the
-          HIP-event time between the phase's first and last launch;
+value    = AVBD body-iterations per second of wall time over the K steps
+           (whole steps: predictor, broadphase, narrowphase, contact cache,
+           prepare, solve);
+roofline = the dominant kernel, the colour sweep k_stage_color (one launch
+           per colour per iteration): its algorithmic bytes per launch over
+           its average launch duration, both from the launches of every
+           4th timed step, each bracketed by HIP events on the world's
+           stream;
 cpu_baseline = the oracle (same algorithm, serial, one thread) on a bounded
-          sample of the same workload, body-iterations per second of its
-          solve phase.
+           sample of the same workload, body-iterations per second of its
+           solve phase.
 """
 import argparse
 import json
@@ -36,10 +39,14 @@ METRIC = "AVBD body-iters/s @32k bodies; sim-steps/s to 1e-4 residual, MLP vs pl
 HBM_PEAK_GBS = 8000.0
 
 
+KERNEL_TIMING_STRIDE = 4
+
+
 def traffic_from_profiles():
-    """HBM bytes per solve launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by a copy-kernel calibration) when recorded."""
-    path = os.path.join(ROOT, "profiles", "solve_traffic.json")
+    """HBM bytes per colour-sweep launch measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) when
+    recorded."""
+    path = os.path.join(ROOT, "profiles", "color_sweep_traffic.json")
     try:
         with open(path) as f:
             return json.load(f).get("bytes_per_launch")
@@ -82,7 +89,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body in the colour sweep (8, 16, 32, 64; 0 = library default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": n * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "
               "multi-GPU sharding of this path is a design note, not a measured configuration.")
@@ -93,7 +112,8 @@ def main():
 
     scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
-                                    cooperative=args.cooperative, solve_lanes=args.lanes))
+                                    cooperative=args.cooperative, solve_lanes=args.lanes,
+                                    kernel_timing_stride=KERNEL_TIMING_STRIDE))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     def advance(steps):
         try:
@@ -108,6 +128,7 @@ def main():
     start_states = world.states()
     world.totals(reset=True)
     world.solve_timing(reset=True)
+    world.kernel_timing(reset=True)
     pmamd.lib().pmhDeviceSynchronize()
     t0 = time.perf_counter()
     advance(args.steps)
@@ -115,14 +136,20 @@ def main():
     wall = time.perf_counter() - t0
     totals = world.totals()
     solve_ms, launches = world.solve_timing()
+    kt = world.kernel_timing()
     stats = world.stats()
     phases = world.phase_times()
     body_iters = totals["body_iterations"]
-    # One unit per step: the whole solve phase (a single launch when
-    # cooperative, a sequence of stream-ordered launches otherwise).
-    units = max(1, totals["steps"])
-    per_launch_bytes = totals["algorithmic_bytes"] / units
-    per_launch_s = solve_ms / 1e3 / units
+    if args.cooperative or kt["launches"] == 0:
+        # One cooperative launch per step: the unit is the whole solve.
+        unit_launches = max(1, totals["steps"])
+        per_launch_bytes = totals["algorithmic_bytes"] / unit_launches
+        per_launch_s = solve_ms / 1e3 / unit_launches
+        kernel = "k_solve_all (one cooperative launch per step)"
+    else:
+        per_launch_bytes = kt["algorithmic_bytes"] / kt["launches"]
+        per_launch_s = kt["milliseconds"] / 1e3 / kt["launches"]
+        kernel = f"k_stage_color<{args.lanes or 32}> (colour sweep, one launch per colour per iteration)"
     achieved = per_launch_bytes / per_launch_s / 1e9
     traffic = traffic_from_profiles()
     name, cus = pmamd.device_info()
@@ -152,6 +179,7 @@ def main():
             "device": f"{name} ({cus} CUs)",
         },
         "solve_phase_body_iters_per_s": body_iters / (solve_ms / 1e3),
+        "solve_phase_gbs": totals["algorithmic_bytes"] / (solve_ms / 1e3) / 1e9,
         "steps_per_s": args.steps / wall,
         "solve_ms_per_step": solve_ms / args.steps,
         "velocity_phases": totals["velocity_phases"],
@@ -164,10 +192,11 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": ("k_solve_all, one cooperative launch per step" if args.cooperative else
-                       "solve phase per step: k_stage_color per colour and k_stage_duals per iteration"),
+            "kernel": kernel,
             "algorithmic_bytes_per_launch": per_launch_bytes,
             "avg_launch_ms": per_launch_s * 1e3,
+            "timed_launches": kt["launches"],
+            "timed_every_nth_step": KERNEL_TIMING_STRIDE,
         },
     }
     if not args.no_cpu_baseline:

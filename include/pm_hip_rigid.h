@@ -150,6 +150,8 @@ typedef struct PmhWorldDesc {
     uint32_t flags;                  /* PMH_WORLD_* */
     uint32_t solve_lanes;            /* lanes per body in the colour sweep: 8, 16, 32, 64; 0 = 32.
                                         Sets the order in which a body's contact rows are summed. */
+    uint32_t kernel_timing_stride;   /* every Nth step brackets each colour-sweep launch with HIP
+                                        events (pmhWorldKernelTiming); 0 = off */
 } PmhWorldDesc;
 
 #define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */
@@ -229,6 +231,19 @@ typedef struct PmhWorldTotals {
     double algorithmic_bytes;
 } PmhWorldTotals;
 PmhResult pmhWorldGetTotals(PmhWorld world, PmhWorldTotals *totals, int reset);
+
+/* Per-launch timing of the dominant kernel, the colour sweep of the staged
+ * solve (one launch per colour per iteration), on the steps sampled by
+ * kernel_timing_stride: summed HIP-event milliseconds, the number of timed
+ * launches and their algorithmic bytes (per iteration: 136 B per body,
+ * 84 B per contact point and dynamic endpoint, 64 B per manifold and
+ * dynamic endpoint). */
+typedef struct PmhKernelTiming {
+    double milliseconds;
+    uint64_t launches;
+    double algorithmic_bytes;
+} PmhKernelTiming;
+PmhResult pmhWorldKernelTiming(PmhWorld world, PmhKernelTiming *timing, int reset);
 
 /* Results of the last step in the reference layouts (for parity checks
  * and pmCmdCopyContactEvents-style consumers). */

@@ -1167,6 +1167,25 @@ static uint32_t grid_for(uint32_t items, uint32_t block = 256, uint32_t cap = 40
     return std::max<uint32_t>(1, std::min(g, cap));
 }
 
+PmhResult KernelTimer::collect() {
+    for (uint32_t j = 0; j < pending; ++j) {
+        float t = 0.0f;
+        PMH_CHECK(hipEventElapsedTime(&t, events[2 * j], events[2 * j + 1]));
+        ms += t;
+    }
+    launches += pending;
+    if (pending) bytes += pending_bytes;
+    pending = 0;
+    pending_bytes = 0.0;
+    return PMH_SUCCESS;
+}
+
+void KernelTimer::release() {
+    for (hipEvent_t e : events) if (e) (void)hipEventDestroy(e);
+    events.clear();
+    pending = 0;
+}
+
 PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     PmhResult r = PMH_SUCCESS;
 #define R(buf, cnt) if ((r = buf.reserve(cnt)) != PMH_SUCCESS) return r
@@ -1327,6 +1346,14 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         PMH_CHECK(hipStreamSynchronize(s));
         const StepCounters host = *v.host_ctr;
         const uint32_t lanes = sp.lanes;
+        KernelTimer *kt = timing ? timing->kernels : nullptr;
+        bool time_colors = false;
+        if (kt && kt->stride) {
+            PmhResult r = kt->collect();   // the previous step's launches are complete
+            if (r != PMH_SUCCESS) return r;
+            time_colors = (kt->step++ % kt->stride) == 0;
+        }
+        uint32_t timed = 0;
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const uint32_t colors = std::min(host.colors, kMaxColors);
         const uint32_t pts_blocks = std::max<uint32_t>(1, (host.points + 255) / 256);
@@ -1346,10 +1373,22 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                     const uint32_t per_wave = 64 / lanes;
                     const uint32_t waves = (bodies + per_wave - 1) / per_wave;
                     const uint32_t blocks = std::max<uint32_t>(1, (waves + 3) / 4);
+                    if (time_colors) {
+                        if (kt->events.size() < 2 * size_t(timed + 1)) {
+                            const size_t old = kt->events.size();
+                            kt->events.resize(2 * size_t(timed + 1) + 64, nullptr);
+                            for (size_t e = old; e < kt->events.size(); ++e) PMH_CHECK(hipEventCreate(&kt->events[e]));
+                        }
+                        PMH_CHECK(hipEventRecord(kt->events[2 * timed], s));
+                    }
                     if (lanes == 64) k_stage_color<64><<<blocks, 256, 0, s>>>(a, c, phase);
                     else if (lanes == 32) k_stage_color<32><<<blocks, 256, 0, s>>>(a, c, phase);
                     else if (lanes == 8) k_stage_color<8><<<blocks, 256, 0, s>>>(a, c, phase);
                     else k_stage_color<16><<<blocks, 256, 0, s>>>(a, c, phase);
+                    if (time_colors) {
+                        PMH_CHECK(hipEventRecord(kt->events[2 * timed + 1], s));
+                        ++timed;
+                    }
                 }
                 k_stage_duals<<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 launches += colors + 1;
@@ -1359,6 +1398,18 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             launches += 2;
         }
         if (timing) timing->launches += launches;
+        if (time_colors) {
+            // Algorithmic bytes of the colour sweeps: per iteration 136 B per
+            // body, 84 B per contact point and dynamic endpoint, 64 B per
+            // manifold and dynamic endpoint (pair record + other body).
+            auto sweep = [](double d, double m1, double m2, double p1, double p2) {
+                return 136.0 * d + 84.0 * (p1 + 2.0 * p2) + 64.0 * (m1 + 2.0 * m2);
+            };
+            double bytes = budget * sweep(host.dyn_bodies, host.m1, host.m2, host.p1, host.p2);
+            if (phases > 1) bytes += budget * sweep(host.impact_bodies, host.m1i, host.m2i, host.p1i, host.p2i);
+            kt->pending = timed;
+            kt->pending_bytes = bytes;
+        }
         k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, 0);
     }
     if (timing && timing->stop) PMH_CHECK(hipEventRecord(timing->stop, s));

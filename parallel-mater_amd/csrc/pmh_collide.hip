@@ -1514,6 +1514,7 @@ PmhResult World::step_once(uint32_t k_step) {
     mark(4);
     SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes};
     SolveTiming st;
+    st.kernels = &kernel_timer;
     if (timing_enabled) {
         st.start = solve_events[2 * k_step];
         st.stop = solve_events[2 * k_step + 1];

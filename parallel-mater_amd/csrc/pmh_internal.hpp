@@ -174,10 +174,26 @@ struct SolveParams {
     uint32_t lanes;                      // lanes of a wavefront per body in the colour sweep: 8, 16, 32, 64
 };
 
+// Per-launch HIP-event timing of the colour-sweep kernel on sampled steps.
+// A step's events are read once the stream has passed them (at the next
+// step's schedule read-back or at the end of a batch).
+struct KernelTimer {
+    uint32_t stride = 0;                  // 0: off
+    uint32_t step = 0;
+    std::vector<hipEvent_t> events;       // start/stop pairs
+    uint32_t pending = 0;                 // recorded pairs not yet read
+    double pending_bytes = 0.0;
+    double ms = 0.0, bytes = 0.0;
+    unsigned long long launches = 0;
+    PmhResult collect();
+    void release();
+};
+
 struct SolveTiming {
     hipEvent_t start = nullptr, stop = nullptr;   // recorded around the solve launch
     float total_ms = 0.0f;
     uint32_t launches = 0;
+    KernelTimer *kernels = nullptr;
 };
 
 // Device-wide exclusive scan of `count_ptr[0]` (or `count` when the

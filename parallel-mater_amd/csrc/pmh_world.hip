@@ -410,6 +410,7 @@ void World::release() {
     for (auto &e : events) if (e) { (void)hipEventDestroy(e); e = nullptr; }
     for (auto &e : solve_events) if (e) (void)hipEventDestroy(e);
     solve_events.clear();
+    kernel_timer.release();
     if (solve_timing.start) (void)hipEventDestroy(solve_timing.start);
     if (solve_timing.stop) (void)hipEventDestroy(solve_timing.stop);
     if (stream) (void)hipStreamDestroy(stream);
@@ -454,6 +455,7 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
         return PMH_ERROR_INVALID_ARGUMENT;
     }
     w->solve_lanes = desc->solve_lanes ? desc->solve_lanes : pmh::kDefaultLanes;
+    w->kernel_timer.stride = desc->kernel_timing_stride;
     w->device = int(desc->device);
     w->timing_enabled = true;
     *out = reinterpret_cast<PmhWorld>(w);
@@ -579,6 +581,8 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
             PMH_CHECK(hipEventElapsedTime(&w->last_solve_ms, w->solve_events[2 * (count - 1)],
                                           w->solve_events[2 * (count - 1) + 1]));
         }
+        r = w->kernel_timer.collect();
+        if (r != PMH_SUCCESS) return r;
     }
     w->has_stepped = true;
     pmh::StepCounters c;
@@ -663,6 +667,20 @@ PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds, uint32_t *lau
     if (milliseconds) *milliseconds = w->solve_timing.total_ms;
     if (launches) *launches = w->solve_timing.launches;
     if (reset) { w->solve_timing.total_ms = 0.0f; w->solve_timing.launches = 0; }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldKernelTiming(PmhWorld world, PmhKernelTiming *timing, int reset) {
+    if (!world || !timing) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    timing->milliseconds = w->kernel_timer.ms;
+    timing->launches = w->kernel_timer.launches;
+    timing->algorithmic_bytes = w->kernel_timer.bytes;
+    if (reset) {
+        w->kernel_timer.ms = 0.0;
+        w->kernel_timer.bytes = 0.0;
+        w->kernel_timer.launches = 0;
+    }
     return PMH_SUCCESS;
 }
 

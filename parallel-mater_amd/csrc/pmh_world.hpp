@@ -55,6 +55,7 @@ struct World {
     bool timing_enabled = false;
     hipEvent_t events[8] = {};
     SolveTiming solve_timing;
+    KernelTimer kernel_timer;
     std::vector<hipEvent_t> solve_events;   // start/stop per step of a batch
     float last_solve_ms = 0.0f;
     bool has_stepped = false;

@@ -44,7 +44,8 @@ class WorldDesc(ctypes.Structure):
     _fields_ = [("gravity", _F * 3), ("timestep", _F), ("profile_version", _U),
                 ("base_iterations", _U), ("requested_iterations", _U),
                 ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_body", _U),
-                ("device", _U), ("flags", _U), ("solve_lanes", _U)]
+                ("device", _U), ("flags", _U), ("solve_lanes", _U),
+                ("kernel_timing_stride", _U)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
@@ -54,6 +55,11 @@ STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "
 
 class WorldStats(ctypes.Structure):
     _fields_ = [(n, _U) for n in STATS_FIELDS]
+
+
+class KernelTiming(ctypes.Structure):
+    _fields_ = [("milliseconds", ctypes.c_double), ("launches", ctypes.c_uint64),
+                ("algorithmic_bytes", ctypes.c_double)]
 
 
 class WorldTotals(ctypes.Structure):
@@ -94,6 +100,7 @@ def lib():
         "pmhWorldGetPhaseTimes": (i32, [_P, _P]),
         "pmhWorldSolveTiming": (i32, [_P, _P, _P, ctypes.c_int]),
         "pmhWorldGetTotals": (i32, [_P, _P, ctypes.c_int]),
+        "pmhWorldKernelTiming": (i32, [_P, _P, ctypes.c_int]),
         "pmhWorldPairCount": (_U, [_P]),
         "pmhWorldGetPairs": (i32, [_P, _P]),
         "pmhWorldGetManifolds": (i32, [_P, _P]),
@@ -139,11 +146,11 @@ class World:
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
                  pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
-                 solve_lanes=0):
+                 solve_lanes=0, kernel_timing_stride=0):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
                          points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0,
-                         solve_lanes)
+                         solve_lanes, kernel_timing_stride)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h
@@ -217,6 +224,14 @@ class World:
         _check(lib().pmhWorldSolveTiming(self._h, ctypes.byref(ms), ctypes.byref(launches),
                                          1 if reset else 0), "pmhWorldSolveTiming")
         return ms.value, launches.value
+
+    def kernel_timing(self, reset=False):
+        """Colour-sweep launches timed with HIP events on the sampled steps:
+        milliseconds, launches, algorithmic bytes."""
+        t = KernelTiming()
+        _check(lib().pmhWorldKernelTiming(self._h, ctypes.byref(t), 1 if reset else 0),
+               "pmhWorldKernelTiming")
+        return {f: getattr(t, f) for f, _ in KernelTiming._fields_}
 
     def totals(self, reset=False):
         t = WorldTotals()
