@@ -205,6 +205,27 @@ uint32_t pmhWorldBodyCount(PmhWorld world);
  * contact cache, AVBD prepare and solve, speed clamp, cache save
  * (production_realtime_backend.cu.inc:2642-2687, 7664-7861).  Enqueues
  * `steps` steps and waits for them. */
+/* PmRigidInput (include/parallel_mater/parallel_mater.h:421-427). */
+typedef struct PmhRigidInput {
+    uint32_t body_index;
+    float force[3], torque[3], linear_impulse[3], angular_impulse[3];
+} PmhRigidInput;
+
+/* pmCmdApplyRigidInputs (src/api/resources.cpp:908) ->
+ * apply_rigid_input (src/simulation/runtime_rigid.cu.inc:265-289): forces,
+ * torques and impulses accumulate on dynamic bodies and act during the
+ * next step only.  PM_ERROR_INVALID_ARGUMENT for a body that is not
+ * dynamic or non-finite values (nothing is applied then). */
+PmhResult pmhWorldApplyRigidInputs(PmhWorld world, uint32_t count, const PmhRigidInput *inputs);
+
+/* pmCmdSetKinematicTargets (src/api/resources.cpp:967) ->
+ * set_kinematic_target (runtime_rigid.cu.inc:197-230): poses that
+ * kinematic bodies reach during the next step (velocities follow from the
+ * motion); kinematic bodies only, finite, non-zero orientation (stored
+ * normalised). */
+PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t count,
+                                      const PmhRigidBodyState *targets);
+
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
 
 /* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;

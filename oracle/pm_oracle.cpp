@@ -2120,23 +2120,45 @@ struct World {
     std::map<uint32_t, CachedPair> cache;
     uint32_t cache_bodies = 0;
     uint64_t epoch = 0;
+    // One-step inputs (BodyAccumulator, KinematicTarget), cleared by every
+    // step as pm_clear_rigid_inputs does (geometry.slang:2877-2886).
+    std::vector<std::array<V3, 4>> accumulators;   // force, torque, impulse, angular impulse
+    std::vector<PmoState> targets;
+    std::vector<uint8_t> target_active;
 };
 
-// pm_integrate_rigid_bodies, geometry.slang:524-612 (no force/impulse
-// accumulators and no kinematic targets are set on this path).
-PmoState integrate(const PmoParams &b, const PmoState &prev, V3 g, float dt) {
+// pm_integrate_rigid_bodies, geometry.slang:524-612, one substep
+// (remaining_substeps = 1, impulses applied).
+PmoState integrate(const PmoParams &b, const PmoState &prev, const std::array<V3, 4> &acc, bool target_active,
+                   const PmoState &target, V3 g, float dt) {
     PmoState next = prev;
-    if (b.motion == kStatic || b.motion == kKinematic) {
+    if (b.motion == kStatic) {
         next.linear_velocity = {0, 0, 0};
         next.angular_velocity = {0, 0, 0};
         return next;
     }
-    V3 force{0, 0, 0}, torque{0, 0, 0};
+    if (b.motion == kKinematic) {
+        if (target_active) {
+            const float fraction = 1.0f / float(1u);
+            next.position = add(prev.position, mul(sub(target.position, prev.position), fraction));
+            Q t = target.orientation;
+            const Q o = prev.orientation;
+            if (o.x * t.x + o.y * t.y + o.z * t.z + o.w * t.w < 0.0f) t = {-t.x, -t.y, -t.z, -t.w};
+            next.orientation = qnormalize({o.x + (t.x - o.x) * fraction, o.y + (t.y - o.y) * fraction,
+                                           o.z + (t.z - o.z) * fraction, o.w + (t.w - o.w) * fraction});
+            next.linear_velocity = mul(sub(next.position, prev.position), 1.0f / dt);
+            next.angular_velocity = quaternion_delta_velocity(prev.orientation, next.orientation, dt);
+        } else {
+            next.linear_velocity = {0, 0, 0};
+            next.angular_velocity = {0, 0, 0};
+        }
+        return next;
+    }
+    const V3 force = acc[0], torque = acc[1], imp = acc[2], aimp = acc[3];
     next.linear_velocity = add(next.linear_velocity, mul(add(g, mul(force, b.inverse_mass)), dt));
     next.angular_velocity = add(next.angular_velocity, mul(inverse_inertia_world(b, next, torque), dt));
-    V3 imp{0, 0, 0};
     next.linear_velocity = add(next.linear_velocity, mul(imp, b.inverse_mass));
-    next.angular_velocity = add(next.angular_velocity, inverse_inertia_world(b, next, imp));
+    next.angular_velocity = add(next.angular_velocity, inverse_inertia_world(b, next, aimp));
     next.linear_velocity = mul(next.linear_velocity, 1.0f / (1.0f + b.linear_damping * dt));
     next.angular_velocity = mul(next.angular_velocity, 1.0f / (1.0f + b.angular_damping * dt));
     next.linear_velocity = clamp_length(next.linear_velocity, b.maximum_linear_speed);
@@ -2365,7 +2387,13 @@ void world_step(World &w) {
     const float dt = w.desc.dt;
     if (w.cache_bodies != n) { w.cache.clear(); w.cache_bodies = n; }
     std::vector<PmoState> prev = w.states, pred(n);
-    for (uint32_t i = 0; i < n; ++i) pred[i] = integrate(w.params[i], prev[i], g, dt);
+    w.accumulators.resize(n, std::array<V3, 4>{});
+    w.targets.resize(n, PmoState{});
+    w.target_active.resize(n, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        pred[i] = integrate(w.params[i], prev[i], w.accumulators[i], w.target_active[i] != 0, w.targets[i], g, dt);
+    std::fill(w.accumulators.begin(), w.accumulators.end(), std::array<V3, 4>{});
+    std::fill(w.target_active.begin(), w.target_active.end(), 0);
     w.proxies.resize(n);
     for (uint32_t i = 0; i < n; ++i)
         w.proxies[i] = make_proxy(w.params[i], w.shapes[w.params[i].mesh_index], prev[i], pred[i]);
@@ -2510,6 +2538,52 @@ int pmo_world_add_bodies(void *world, uint32_t count, const PmoParams *p, const 
     w.states.insert(w.states.end(), s, s + count);
     return 0;
 }
+// apply_rigid_input (runtime_rigid.cu.inc:265-289): inputs accumulate on
+// dynamic bodies until the next step.
+int pmo_world_apply_inputs(void *world, uint32_t count, const PmoRigidInput *inputs) {
+    World &w = *static_cast<World *>(world);
+    const uint32_t n = uint32_t(w.params.size());
+    w.accumulators.resize(n, std::array<V3, 4>{});
+    for (uint32_t k = 0; k < count; ++k) {
+        const PmoRigidInput &in = inputs[k];
+        if (in.body_index >= n || w.params[in.body_index].motion != kDynamic) return -1;
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(in.force[c]) || !std::isfinite(in.torque[c]) || !std::isfinite(in.linear_impulse[c]) ||
+                !std::isfinite(in.angular_impulse[c]))
+                return -1;
+        auto &a = w.accumulators[in.body_index];
+        a[0] = add(a[0], V3{in.force[0], in.force[1], in.force[2]});
+        a[1] = add(a[1], V3{in.torque[0], in.torque[1], in.torque[2]});
+        a[2] = add(a[2], V3{in.linear_impulse[0], in.linear_impulse[1], in.linear_impulse[2]});
+        a[3] = add(a[3], V3{in.angular_impulse[0], in.angular_impulse[1], in.angular_impulse[2]});
+    }
+    return 0;
+}
+
+// set_kinematic_target (runtime_rigid.cu.inc:197-230): kinematic bodies
+// only, finite, non-zero orientation, stored normalised, for the next step.
+int pmo_world_set_kinematic_targets(void *world, uint32_t first, uint32_t count, const PmoState *targets) {
+    World &w = *static_cast<World *>(world);
+    const uint32_t n = uint32_t(w.params.size());
+    if (uint64_t(first) + count > n) return -1;
+    w.targets.resize(n, PmoState{});
+    w.target_active.resize(n, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+        PmoState t = targets[k];
+        const uint32_t i = first + k;
+        if (w.params[i].motion != kKinematic) return -1;
+        const float *f = reinterpret_cast<const float *>(&t);
+        for (size_t c = 0; c < sizeof(PmoState) / sizeof(float); ++c)
+            if (!std::isfinite(f[c])) return -1;
+        const Q o = t.orientation;
+        if (o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w <= kEpsilon * kEpsilon) return -1;
+        t.orientation = qnormalize(o);
+        w.targets[i] = t;
+        w.target_active[i] = 1;
+    }
+    return 0;
+}
+
 int pmo_world_step(void *world, uint32_t steps) {
     World &w = *static_cast<World *>(world);
     uint64_t n = w.params.size();

@@ -193,6 +193,16 @@ int pmo_world_add_sphere(void *world, float radius);
 int pmo_world_add_capsule(void *world, float radius, float half_length);
 int pmo_world_add_bodies(void *world, uint32_t count,
                          const PmoParams *parameters, const PmoState *states);
+/* PmRigidInput (include/parallel_mater/parallel_mater.h:421-427). */
+typedef struct PmoRigidInput {
+    uint32_t body_index;
+    float force[3], torque[3], linear_impulse[3], angular_impulse[3];
+} PmoRigidInput;
+/* Forces, torques and impulses for the next step on dynamic bodies
+ * (accumulating); kinematic targets for the next step on kinematic
+ * bodies.  Both return 0 or -1 (invalid body, motion type or values). */
+int pmo_world_apply_inputs(void *world, uint32_t count, const PmoRigidInput *inputs);
+int pmo_world_set_kinematic_targets(void *world, uint32_t first, uint32_t count, const PmoState *targets);
 int pmo_world_step(void *world, uint32_t steps);
 uint32_t pmo_world_body_count(void *world);
 /* Cumulative wall time of world steps, of AVBD prepare and of the AVBD

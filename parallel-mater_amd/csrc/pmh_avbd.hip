@@ -161,9 +161,12 @@ __global__ void k_body_init(uint32_t n, const PmhBodyParameters *P, const PmhRig
                             const PmhRigidBodyState *st, float dt, V3 g, float4 *tpos, float4 *tq,
                             float4 *tv, float4 *tw, float4 *disp, float4 *dq, float4 *inertia,
                             float4 *prev_vel, uint32_t *color, uint32_t *parent, uint32_t *impact,
-                            uint32_t *deg, uint32_t *anchored, uint32_t *adj_cursor) {
+                            uint32_t *deg, uint32_t *anchored, uint32_t *adj_cursor, uint32_t *members,
+                            float4 *mean) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    members[i] = 0;
+    mean[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const PmhRigidBodyState s = st[i];
     tpos[i] = f4(sv(s.position), 0.0f);
     tq[i] = f4q(sq(s.orientation));
@@ -386,15 +389,22 @@ __global__ void k_impact_roots(const uint32_t *npairs, const PmhBodyParameters *
     }
 }
 
+// Also counts each island's dynamic members and lists the roots whose
+// islands take the free-motion mean correction (unanchored, with contacts).
 __global__ void k_impact_bodies(uint32_t n, const PmhBodyParameters *P, const uint32_t *parent, uint32_t *impact,
-                                const uint32_t *deg, StepCounters *ctr) {
+                                const uint32_t *deg, const uint32_t *anchored, uint32_t *members,
+                                uint32_t *mean_list, StepCounters *ctr) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     if (P[i].motion != kDynamic) { impact[i] = 0; return; }
     const uint32_t r = parent[i];
     const uint32_t flag = ld_relaxed(&impact[r]);
     if (r != i) impact[i] = flag;
-    if (r == i && deg[i] != 0) atomicAdd(&ctr->islands, 1u);
+    if (r == i && deg[i] != 0) {
+        atomicAdd(&ctr->islands, 1u);
+        if (anchored[i] == 0) mean_list[atomicAdd(&ctr->mean_roots, 1u)] = i;
+    }
+    atomicAdd(&members[r], 1u);
     atomicAdd(&ctr->dyn_bodies, 1u);
     if (flag) atomicAdd(&ctr->impact_bodies, 1u);
 }
@@ -612,6 +622,7 @@ struct SolveArgs {
     const uint32_t *c_pair;
     const uint2 *ab;
     const uint32_t *parent, *impact, *deg, *anchored;
+    const uint32_t *members, *mean_list;
     float4 *mean;
     PmhContactSchedule *schedule;
     StepCounters *ctr;
@@ -867,29 +878,48 @@ __device__ void advance_duals(const SolveArgs &a, uint32_t phase, uint32_t itera
     }
 }
 
-// free_mean (per unanchored island root, members in ascending index) and
-// free_subtract (solve.slang:617-698).
-__device__ void free_mean(const SolveArgs &a, uint32_t phase, uint32_t tid, uint32_t nthreads) {
-    const bool vel = phase != 0;
-    for (uint32_t r = tid; r < a.n; r += nthreads) {
-        if (a.P[r].motion != kDynamic || a.parent[r] != r) continue;
-        float4 mean = make_float4(0, 0, 0, 0);
-        if (!((vel && a.impact[r] == 0) || a.deg[r] == 0) && a.anchored[r] == 0) {
+// free_mean (solve.slang:617-675): the mass-weighted mean translation of
+// each unanchored island with contacts, its members summed in ascending
+// index order.  One wavefront per listed root scans 64 indices at a time
+// from the root (the smallest index of its island) and stops once it has
+// met every member; the sum itself stays sequential in index order.
+__device__ void free_mean(const SolveArgs &a, uint32_t phase, uint32_t gw, uint32_t nw) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t roots = a.ctr->mean_roots;
+    for (uint32_t k = gw; k < roots; k += nw) {
+        const uint32_t r = a.mean_list[k];
+        float4 mean = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!(phase != 0 && a.impact[r] == 0)) {
+            const uint32_t want = a.members[r];
+            uint32_t found = 0;
             V3 w = {0.0f, 0.0f, 0.0f};
             float total = 0.0f;
-            for (uint32_t i = r; i < a.n; ++i) {
-                if (a.parent[i] != r || a.P[i].motion != kDynamic) continue;
-                const float mass = 1.0f / smax(a.P[i].inverse_mass, 1.0e-12f);
-                const float4 d = a.disp[2 * i];
-                w = add(w, mul({d.x, d.y, d.z}, mass));
-                total += mass;
+            for (uint32_t base = r; found < want && base < a.n; base += 64) {
+                const uint32_t i = base + lane;
+                const bool member = i < a.n && a.parent[i] == r && a.P[i].motion == kDynamic;
+                V3 prod = {0.0f, 0.0f, 0.0f};
+                float mass = 0.0f;
+                if (member) {
+                    mass = 1.0f / smax(a.P[i].inverse_mass, 1.0e-12f);
+                    const float4 d = a.disp[2 * i];
+                    prod = mul({d.x, d.y, d.z}, mass);
+                }
+                uint64_t bal = __ballot(member);
+                while (bal) {
+                    const int b = __builtin_ctzll(bal);
+                    bal &= bal - 1ull;
+                    const V3 p = {__shfl(prod.x, b, 64), __shfl(prod.y, b, 64), __shfl(prod.z, b, 64)};
+                    w = add(w, p);
+                    total += __shfl(mass, b, 64);
+                    ++found;
+                }
             }
             if (total > 0.0f) {
                 w = mul(w, 1.0f / total);
                 mean = make_float4(w.x, w.y, w.z, 0.0f);
             }
         }
-        a.mean[r] = mean;
+        if (lane == 0) a.mean[r] = mean;
     }
 }
 
@@ -1076,7 +1106,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
             advance_duals(a, phase, it, tid, nthreads);
             grid.sync();
         }
-        free_mean(a, phase, tid, nthreads);
+        free_mean(a, phase, gw, nw);
         grid.sync();
         free_subtract_apply(a, phase, tid, nthreads);
         grid.sync();
@@ -1110,7 +1140,7 @@ __global__ void k_stage_velocity_jacobians(SolveArgs a) {
     velocity_jacobians(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 __global__ void k_stage_mean(SolveArgs a, uint32_t phase) {
-    free_mean(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    free_mean(a, phase, global_wave(), (gridDim.x * blockDim.x) >> 6);
 }
 __global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
     free_subtract_apply(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1136,6 +1166,7 @@ __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32
     ctr->impact = 0;
     ctr->color_ticket = 0;
     ctr->dyn_bodies = ctr->impact_bodies = 0;
+    ctr->mean_roots = 0;
     ctr->m1 = ctr->m2 = ctr->p1 = ctr->p2 = 0;
     ctr->m1i = ctr->m2i = ctr->p1i = ctr->p2i = 0;
 }
@@ -1193,6 +1224,7 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     R(dq, bodies); R(inertia, 2 * (size_t)bodies); R(color, bodies); R(parent, bodies);
     R(impact, bodies); R(deg, bodies + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
     R(color_bodies, bodies); R(bp_start, bodies + 1); R(anchored, bodies); R(mean, bodies);
+    R(members, bodies); R(mean_list, bodies);
     if (prev_vel.n < bodies) {
         R(prev_vel, bodies);
         PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
@@ -1223,7 +1255,7 @@ void Solver::release() {
     tpos.release(); tq.release(); tv.release(); tw.release(); disp.release(); dq.release();
     inertia.release(); prev_vel.release(); color.release(); parent.release(); impact.release();
     deg.release(); adj_start.release(); adj_cursor.release(); color_bodies.release();
-    bp_start.release(); anchored.release(); mean.release(); code.release(); ab.release();
+    bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release(); code.release(); ab.release();
     m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
     m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
@@ -1256,7 +1288,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
     k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
                                    v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
-                                   v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p);
+                                   v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p, v.members.p, v.mean.p);
     k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
                                       v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
                                       v.m_impact.p, dt, sp.profile);
@@ -1269,7 +1301,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
     k_impact_roots<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_impact.p, v.m_count.p, v.parent.p, v.impact.p,
                                       v.anchored.p, ctr);
-    k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, ctr);
+    k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, v.anchored.p, v.members.p,
+                                       v.mean_list.p, ctr);
     if (!realtime)
         k_count_impact_pairs<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.impact.p, ctr);
     const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
@@ -1304,6 +1337,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.c_rep = v.c_rep.p; a.c_err = v.c_err.p; a.c_np = v.c_np.p; a.c_pt = v.c_pt.p;
     a.c_pair = v.c_pair.p; a.ab = v.ab.p;
     a.parent = v.parent.p; a.impact = v.impact.p; a.deg = v.adj_cursor.p; a.anchored = v.anchored.p;
+    a.members = v.members.p; a.mean_list = v.mean_list.p;
     a.mean = v.mean.p;
     a.schedule = v.schedule.p;
     a.ctr = ctr;
@@ -1393,9 +1427,12 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                 k_stage_duals<<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 launches += colors + 1;
             }
-            k_stage_mean<<<bg, 256, 0, s>>>(a, phase);
+            if (host.mean_roots) {
+                k_stage_mean<<<(host.mean_roots + 3) / 4, 256, 0, s>>>(a, phase);
+                launches += 1;
+            }
             k_stage_apply<<<bg, 256, 0, s>>>(a, phase);
-            launches += 2;
+            launches += 1;
         }
         if (timing) timing->launches += launches;
         if (time_colors) {

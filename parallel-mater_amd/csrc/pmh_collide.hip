@@ -56,6 +56,7 @@ __device__ __forceinline__ V3 inverse_inertia_world(const PmhBodyParameters &p, 
 __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const ShapeDev *shapes,
                                   const PmhRigidBodyState *cur, PmhRigidBodyState *prev,
                                   PmhRigidBodyState *pred, float4 *pmn, float4 *pmx, V3 g, float dt,
+                                  float4 *inputs, const PmhRigidBodyState *targets, uint32_t *target_flags,
                                   StepCounters *ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -63,15 +64,37 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
     const PmhRigidBodyState s0 = cur[i];
     prev[i] = s0;
     PmhRigidBodyState s = s0;
-    if (b.motion == kStatic || b.motion == kKinematic) {
+    const uint32_t target_active = target_flags[i];
+    if (b.motion == kStatic) {
         s.linear_velocity = {0.0f, 0.0f, 0.0f};
         s.angular_velocity = {0.0f, 0.0f, 0.0f};
+    } else if (b.motion == kKinematic) {
+        if (target_active) {
+            // One substep: the target is reached (fraction 1).
+            const PmhRigidBodyState t = targets[i];
+            const float fraction = 1.0f / float(1u);
+            const V3 p0 = sv(s0.position);
+            const V3 p1 = add(p0, mul(sub(sv(t.position), p0), fraction));
+            const Q o = sq(s0.orientation);
+            Q q = sq(t.orientation);
+            if (o.x * q.x + o.y * q.y + o.z * q.z + o.w * q.w < 0.0f) q = {-q.x, -q.y, -q.z, -q.w};
+            const Q no = qnormalize({o.x + (q.x - o.x) * fraction, o.y + (q.y - o.y) * fraction,
+                                     o.z + (q.z - o.z) * fraction, o.w + (q.w - o.w) * fraction});
+            s.position = pv(p1);
+            s.orientation = pq(no);
+            s.linear_velocity = pv(mul(sub(p1, p0), 1.0f / dt));
+            s.angular_velocity = pv(quaternion_delta_velocity(o, no, dt));
+        } else {
+            s.linear_velocity = {0.0f, 0.0f, 0.0f};
+            s.angular_velocity = {0.0f, 0.0f, 0.0f};
+        }
     } else {
-        const V3 zero = {0.0f, 0.0f, 0.0f};
-        V3 v = add(sv(s.linear_velocity), mul(add(g, mul(zero, b.inverse_mass)), dt));
-        V3 w = add(sv(s.angular_velocity), mul(inverse_inertia_world(b, sq(s.orientation), zero), dt));
-        v = add(v, mul(zero, b.inverse_mass));
-        w = add(w, inverse_inertia_world(b, sq(s.orientation), zero));
+        const V3 force = v3(inputs[4 * i]), torque = v3(inputs[4 * i + 1]);
+        const V3 imp = v3(inputs[4 * i + 2]), aimp = v3(inputs[4 * i + 3]);
+        V3 v = add(sv(s.linear_velocity), mul(add(g, mul(force, b.inverse_mass)), dt));
+        V3 w = add(sv(s.angular_velocity), mul(inverse_inertia_world(b, sq(s.orientation), torque), dt));
+        v = add(v, mul(imp, b.inverse_mass));
+        w = add(w, inverse_inertia_world(b, sq(s.orientation), aimp));
         v = mul(v, 1.0f / (1.0f + b.linear_damping * dt));
         w = mul(w, 1.0f / (1.0f + b.angular_damping * dt));
         v = clamp_length(v, b.maximum_linear_speed);
@@ -86,6 +109,13 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
         s.linear_velocity = pv(v);
         s.angular_velocity = pv(w);
     }
+    // pm_clear_rigid_inputs (geometry.slang:2877-2886): inputs last one step.
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    inputs[4 * i] = z;
+    inputs[4 * i + 1] = z;
+    inputs[4 * i + 2] = z;
+    inputs[4 * i + 3] = z;
+    if (target_active) target_flags[i] = 0u;
     pred[i] = s;
     // Proxy: production arguments path_deviation 0, correction_budget 1,
     // epsilon 1e-5 (production_realtime_backend.cu.inc:2663-2684).
@@ -223,7 +253,9 @@ __global__ void k_set_pairs(StepCounters *ctr, const uint32_t *offset, uint32_t 
 }
 
 // Large bodies: one block scans every body in index order; a block-wide
-// prefix sum places the hits, so the segment comes out ascending.
+// prefix sum places the hits, so the segment comes out ascending.  Blocks
+// beyond the number of large bodies return at once.
+constexpr uint32_t kLargeBlocks = 1024;
 template <int kPass>
 __global__ void __launch_bounds__(256) k_bp_large(uint32_t n, const PmhBodyParameters *P, const float4 *pmn,
                                                   const float4 *pmx, const uint32_t *large, uint32_t *cand_count,
@@ -828,220 +860,241 @@ __device__ __forceinline__ void world_triangle(const ShapeView &m, const PmhRigi
     for (uint32_t c = 0; c < 3; ++c) W[c] = transform_point(st, m.vert(m.index(3 * t + c)));
 }
 
-__device__ __noinline__ void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count,
-                                                     uint32_t b_first, uint32_t b_count, Manifold &m) {
+// The discrete pass for one triangle pair (the body of the triangle loops
+// of rigid_collide_triangle_ranges, geometry.slang:1359-1584): at most one
+// contact, independent of every other triangle pair.
+__device__ __noinline__ bool triangle_pair_contact(const TrianglePair &P, uint32_t at, uint32_t bt, bool inside_a,
+                                                   bool inside_b, Contact &c) {
     const float margin = P.margin;
     const float rest = smin(margin, 1.0e-3f);
     const V3 body_ref = sv(P.a.position);
-    const bool inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
-    const bool inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
-    for (uint32_t ai = 0; ai < a_count; ++ai) {
-        const uint32_t at = a_first + ai;
-        if (3u * at + 2u >= P.am.s.idx_count) break;
-        V3 A[3];
-        world_triangle(P.am, P.a, at, A);
-        for (uint32_t bi = 0; bi < b_count; ++bi) {
-            const uint32_t bt = b_first + bi;
-            if (3u * bt + 2u >= P.bm.s.idx_count) break;
-            V3 B[3];
-            world_triangle(P.bm, P.b, bt, B);
-            {
-                const V3 e = {margin, margin, margin};
-                const V3 amin = sub(cmin(A[0], cmin(A[1], A[2])), e), amax = add(cmax(A[0], cmax(A[1], A[2])), e);
-                const V3 bmin = cmin(B[0], cmin(B[1], B[2])), bmax = cmax(B[0], cmax(B[1], B[2]));
-                if (!boxes_touch(amin, amax, bmin, bmax)) continue;
+    V3 A[3], B[3];
+    world_triangle(P.am, P.a, at, A);
+    world_triangle(P.bm, P.b, bt, B);
+    {
+        const V3 e = {margin, margin, margin};
+        const V3 amin = sub(cmin(A[0], cmin(A[1], A[2])), e), amax = add(cmax(A[0], cmax(A[1], A[2])), e);
+        const V3 bmin = cmin(B[0], cmin(B[1], B[2])), bmax = cmax(B[0], cmax(B[1], B[2]));
+        if (!boxes_touch(amin, amax, bmin, bmax)) return false;
+    }
+    const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
+    const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
+    const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
+    if (dot(oa, ob) > 0.9999f) return false;
+    V3 pa, pb;
+    closest_triangle_pair(A, B, pa, pb);
+    const V3 delta = sub(pa, pb);
+    const float sq2 = len2(delta);
+    if (sq2 > margin * margin) return false;
+    const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+    V3 fallback = dot(cn, sub(body_ref, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+    if (len2(cob) > 0.5f) fallback = cob;
+    const float distance = sqrtf(smax(sq2, 0.0f));
+    const V3 point = mul(add(pa, pb), 0.5f);
+    const V3 proj = sub(pa, mul(cn, dot(sub(pa, B[0]), cn)));
+    const bool on_face = len2(sub(proj, closest_on_triangle(proj, B[0], B[1], B[2]))) <= 1.0e-10f;
+    const V3 normal = (distance <= 1.0e-5f || on_face) ? fallback : normalized_or(delta, fallback);
+    if (on_face && len2(oa) < 0.01f && len2(ob) > 0.5f && P.bm.s.has_planes) {
+        const V3 face_a = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0.0f, 0.0f, 0.0f});
+        if (fabsf(dot(face_a, normal)) < 1.0e-3f) {
+            const float depth = smax(0.0f, -smin(dot(sub(A[0], B[0]), normal),
+                                                 smin(dot(sub(A[1], B[0]), normal), dot(sub(A[2], B[0]), normal))));
+            const V3 axis = inv_rotate(sq(P.b.orientation), face_a);
+            const float offset = dot(sub(sv(P.b.position), A[0]), face_a);
+            float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
+            for (uint32_t k = 0; k < P.bm.s.vtx_count; ++k) {
+                const float pr = dot(P.bm.vert(k), axis) + offset;
+                lo = smin(lo, pr);
+                hi = smax(hi, pr);
             }
+            const float alternate = smax(0.0f, smin(-lo, hi));
+            if (alternate + 1.0e-5f < depth) return false;
+        }
+    }
+    const float separation = on_face && len2(cob) > 0.5f ? dot(delta, normal) : distance;
+    // rigid_contact_reaches_rest_offset (:1282-1290)
+    if (!(separation <= rest + 1.0e-5f)) {
+        const float speed = normal_speed(P.a, P.b, point, normal);
+        if (speed > 1.0e-5f || !(-speed * P.dt + rest + 1.0e-5f >= separation)) return false;
+    }
+    c.normal = normal;
+    c.point = point;
+    c.pen = rest - separation + 1.0e-5f;
+    bool recovered = false;
+    if (!P.am.s.has_planes && !P.bm.s.has_planes && distance <= 1.0e-5f) {
+        V3 oldA[3], oldB[3];
+        world_triangle(P.am, P.ap, at, oldA);
+        world_triangle(P.bm, P.bp, bt, oldB);
+        V3 opa, opb;
+        closest_triangle_pair(oldA, oldB, opa, opb);
+        V3 od = sub(opa, opb);
+        if (len2(od) > 1.0e-10f) {
+            V3 IA[3], IB[3];
+            float lo = 0.0f, hi = 1.0f;
+            const float gap = smin(rest, 0.5f * sqrtf(len2(od)));
+            for (uint32_t k = 0; k < 10u; ++k) {
+                const float time = 0.5f * (lo + hi);
+                for (uint32_t q = 0; q < 3; ++q) {
+                    IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), time));
+                    IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), time));
+                }
+                V3 u, w;
+                closest_triangle_pair(IA, IB, u, w);
+                if (len2(sub(u, w)) > gap * gap) lo = time;
+                else hi = time;
+            }
+            for (uint32_t q = 0; q < 3; ++q) {
+                IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), lo));
+                IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), lo));
+            }
+            closest_triangle_pair(IA, IB, opa, opb);
+            od = sub(opa, opb);
+            c.normal = triangle_feature_normal(IA, IB, opa, opb, normalized_or(od, fallback));
+            const V3 ca = advect_triangle_point(opa, IA, A), cb = advect_triangle_point(opb, IB, B);
+            c.point = mul(add(ca, cb), 0.5f);
+            c.pen = rest - dot(sub(ca, cb), c.normal) + 1.0e-5f;
+            recovered = true;
+        }
+    }
+    if (distance <= 1.0e-5f) {
+        if (!recovered && on_face) {   // no fixed joints act on this path
+            const float depth = smax(0.0f, -smin(dot(sub(A[0], pb), normal),
+                                                 smin(dot(sub(A[1], pb), normal), dot(sub(A[2], pb), normal))));
+            c.pen = smin(margin, depth + rest) + 1.0e-5f;
+        } else if (!recovered) {
+            c.pen = margin + 1.0e-5f;
+        }
+    }
+    if (!shell_facing(c.normal, coa, cob)) return false;
+    c.impact_fraction = 1.0f;
+    return true;
+}
+
+// The swept pass for one triangle pair (conservative advancement, the body
+// of rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154, for
+// unguided pairs): at most one contact.
+__device__ __noinline__ bool triangle_pair_swept_contact(const TrianglePair &P, uint32_t at, uint32_t bt,
+                                                         bool inside_a, bool inside_b, Contact &c) {
+    const float margin = P.margin;
+    const float rest = smin(margin, 1.0e-3f);
+    V3 pA[3], cA[3], pB[3], cB[3];
+    world_triangle(P.am, P.ap, at, pA);
+    world_triangle(P.am, P.a, at, cA);
+    world_triangle(P.bm, P.bp, bt, pB);
+    world_triangle(P.bm, P.b, bt, cB);
+    V3 amin = cmin(pA[0], cA[0]), amax = cmax(pA[0], cA[0]);
+    for (uint32_t q = 1; q < 3; ++q) {
+        amin = cmin(amin, cmin(pA[q], cA[q]));
+        amax = cmax(amax, cmax(pA[q], cA[q]));
+    }
+    V3 bmin = cmin(pB[0], cB[0]), bmax = cmax(pB[0], cB[0]);
+    for (uint32_t q = 1; q < 3; ++q) {
+        bmin = cmin(bmin, cmin(pB[q], cB[q]));
+        bmax = cmax(bmax, cmax(pB[q], cB[q]));
+    }
+    const V3 e = {margin, margin, margin};
+    if (!boxes_touch(sub(amin, e), add(amax, e), bmin, bmax)) return false;
+    V3 dA[3], dB[3];
+    float speed_abs = 0.0f;
+    for (uint32_t q = 0; q < 3; ++q) {
+        dA[q] = sub(cA[q], pA[q]);
+        dB[q] = sub(cB[q], pB[q]);
+        speed_abs = smax(speed_abs, len(dA[q]));
+    }
+    float speed_b = 0.0f;
+    for (uint32_t q = 0; q < 3; ++q) speed_b = smax(speed_b, len(dB[q]));
+    speed_abs += speed_b;
+    const V3 common = dB[0];
+    float rel_a = 0.0f, rel_b = 0.0f;
+    for (uint32_t q = 0; q < 3; ++q) {
+        rel_a = smax(rel_a, len(sub(dA[q], common)));
+        rel_b = smax(rel_b, len(sub(dB[q], common)));
+    }
+    const float bound = smin(speed_abs, rel_a + rel_b);
+    if (bound <= kEpsilon) return false;
+    float time = 0.0f;
+    bool starts_near = false;
+    V3 sep_normal = {0.0f, 0.0f, 0.0f};
+    bool have_sep = false;
+    for (uint32_t it = 0; it < 32u; ++it) {
+        V3 A[3], B[3];
+        for (uint32_t q = 0; q < 3; ++q) {
+            A[q] = add(pA[q], mul(dA[q], time));
+            B[q] = add(pB[q], mul(dB[q], time));
+        }
+        V3 pa, pb;
+        closest_triangle_pair(A, B, pa, pb);
+        const V3 delta = sub(pa, pb);
+        const float distance = sqrtf(smax(0.0f, len2(delta)));
+        const float offset = rest;
+        if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
+        if (distance <= offset + 1.0e-5f) {
+            const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
+            const V3 center = add(sv(P.ap.position), mul(sub(sv(P.a.position), sv(P.ap.position)), time));
+            V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
             const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
             const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
             const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
-            if (dot(oa, ob) > 0.9999f) continue;
-            V3 pa, pb;
-            closest_triangle_pair(A, B, pa, pb);
-            const V3 delta = sub(pa, pb);
-            const float sq2 = len2(delta);
-            if (sq2 > margin * margin) continue;
-            const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
-            V3 fallback = dot(cn, sub(body_ref, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+            if (dot(oa, ob) > 0.9999f) return false;
             if (len2(cob) > 0.5f) fallback = cob;
-            const float distance = sqrtf(smax(sq2, 0.0f));
+            else if (len2(coa) > 0.5f) fallback = mul(coa, -1.0f);
+            const V3 normal = triangle_feature_normal(A, B, pa, pb, have_sep ? sep_normal : fallback);
+            if (!shell_facing(normal, coa, cob)) return false;
             const V3 point = mul(add(pa, pb), 0.5f);
-            const V3 proj = sub(pa, mul(cn, dot(sub(pa, B[0]), cn)));
-            const bool on_face = len2(sub(proj, closest_on_triangle(proj, B[0], B[1], B[2]))) <= 1.0e-10f;
-            const V3 normal = (distance <= 1.0e-5f || on_face) ? fallback : normalized_or(delta, fallback);
-            if (on_face && len2(oa) < 0.01f && len2(ob) > 0.5f && P.bm.s.has_planes) {
-                const V3 face_a = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0.0f, 0.0f, 0.0f});
-                if (fabsf(dot(face_a, normal)) < 1.0e-3f) {
-                    const float depth = smax(0.0f, -smin(dot(sub(A[0], B[0]), normal),
-                                                         smin(dot(sub(A[1], B[0]), normal), dot(sub(A[2], B[0]), normal))));
-                    const V3 axis = inv_rotate(sq(P.b.orientation), face_a);
-                    const float offset = dot(sub(sv(P.b.position), A[0]), face_a);
-                    float lo = 3.402823466e+38f, hi = -3.402823466e+38f;
-                    for (uint32_t k = 0; k < P.bm.s.vtx_count; ++k) {
-                        const float pr = dot(P.bm.vert(k), axis) + offset;
-                        lo = smin(lo, pr);
-                        hi = smax(hi, pr);
-                    }
-                    const float alternate = smax(0.0f, smin(-lo, hi));
-                    if (alternate + 1.0e-5f < depth) continue;
-                }
-            }
-            const float separation = on_face && len2(cob) > 0.5f ? dot(delta, normal) : distance;
-            // rigid_contact_reaches_rest_offset (:1282-1290)
-            if (!(separation <= rest + 1.0e-5f)) {
-                const float speed = normal_speed(P.a, P.b, point, normal);
-                if (speed > 1.0e-5f || !(-speed * P.dt + rest + 1.0e-5f >= separation)) continue;
-            }
-            Contact c;
+            const float vn = normal_speed(P.a, P.b, point, normal);
+            if (vn > 1.0e-5f) return false;
+            const float remaining = -vn * P.dt * (1.0f - time) - distance;
             c.normal = normal;
             c.point = point;
-            c.pen = rest - separation + 1.0e-5f;
-            bool recovered = false;
-            if (!P.am.s.has_planes && !P.bm.s.has_planes && distance <= 1.0e-5f) {
-                V3 oldA[3], oldB[3];
-                world_triangle(P.am, P.ap, at, oldA);
-                world_triangle(P.bm, P.bp, bt, oldB);
-                V3 opa, opb;
-                closest_triangle_pair(oldA, oldB, opa, opb);
-                V3 od = sub(opa, opb);
-                if (len2(od) > 1.0e-10f) {
-                    V3 IA[3], IB[3];
-                    float lo = 0.0f, hi = 1.0f;
-                    const float gap = smin(rest, 0.5f * sqrtf(len2(od)));
-                    for (uint32_t k = 0; k < 10u; ++k) {
-                        const float time = 0.5f * (lo + hi);
-                        for (uint32_t q = 0; q < 3; ++q) {
-                            IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), time));
-                            IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), time));
-                        }
-                        V3 u, w;
-                        closest_triangle_pair(IA, IB, u, w);
-                        if (len2(sub(u, w)) > gap * gap) lo = time;
-                        else hi = time;
-                    }
-                    for (uint32_t q = 0; q < 3; ++q) {
-                        IA[q] = add(oldA[q], mul(sub(A[q], oldA[q]), lo));
-                        IB[q] = add(oldB[q], mul(sub(B[q], oldB[q]), lo));
-                    }
-                    closest_triangle_pair(IA, IB, opa, opb);
-                    od = sub(opa, opb);
-                    c.normal = triangle_feature_normal(IA, IB, opa, opb, normalized_or(od, fallback));
-                    const V3 ca = advect_triangle_point(opa, IA, A), cb = advect_triangle_point(opb, IB, B);
-                    c.point = mul(add(ca, cb), 0.5f);
-                    c.pen = rest - dot(sub(ca, cb), c.normal) + 1.0e-5f;
-                    recovered = true;
-                }
-            }
-            if (distance <= 1.0e-5f) {
-                if (!recovered && on_face) {   // no fixed joints act on this path
-                    const float depth = smax(0.0f, -smin(dot(sub(A[0], pb), normal),
-                                                         smin(dot(sub(A[1], pb), normal), dot(sub(A[2], pb), normal))));
-                    c.pen = smin(margin, depth + rest) + 1.0e-5f;
-                } else if (!recovered) {
-                    c.pen = margin + 1.0e-5f;
-                }
-            }
-            if (!shell_facing(c.normal, coa, cob)) continue;
-            c.impact_fraction = 1.0f;
-            add_contact(m, c, smax(2.0f * margin, 1.0e-4f));
+            c.pen = smax(0.0f, remaining + offset) + 1.0e-5f;
+            c.impact_fraction = starts_near ? 0.0f : time;
+            return true;
+        }
+        sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
+        have_sep = true;
+        const float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+        time += smax(advance, 1.0e-5f);
+        if (time > 1.0f) return false;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void inside_flags(const TrianglePair &P, bool &inside_a, bool &inside_b) {
+    inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
+    inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+}
+
+// rigid_collide_triangle_ranges: triangles in order, body-major.
+__device__ void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count,
+                                        uint32_t b_first, uint32_t b_count, Manifold &m) {
+    bool inside_a, inside_b;
+    inside_flags(P, inside_a, inside_b);
+    const float spacing = smax(2.0f * P.margin, 1.0e-4f);
+    for (uint32_t ai = 0; ai < a_count; ++ai) {
+        const uint32_t at = a_first + ai;
+        if (3u * at + 2u >= P.am.s.idx_count) break;
+        for (uint32_t bi = 0; bi < b_count; ++bi) {
+            const uint32_t bt = b_first + bi;
+            if (3u * bt + 2u >= P.bm.s.idx_count) break;
+            Contact c;
+            if (triangle_pair_contact(P, at, bt, inside_a, inside_b, c)) add_contact(m, c, spacing);
         }
     }
 }
 
-__device__ __noinline__ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first,
-                                                           uint32_t a_count, uint32_t b_first, uint32_t b_count,
-                                                           Manifold &m) {
-    const float margin = P.margin;
-    const float rest = smin(margin, 1.0e-3f);
-    const bool inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
-    const bool inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+__device__ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first, uint32_t a_count,
+                                              uint32_t b_first, uint32_t b_count, Manifold &m) {
+    bool inside_a, inside_b;
+    inside_flags(P, inside_a, inside_b);
+    const float spacing = smax(P.margin * 2.0f, 1.0e-4f);
     for (uint32_t ai = 0; ai < a_count; ++ai) {
         const uint32_t at = a_first + ai;
         if (3u * at + 2u >= P.am.s.idx_count) break;
-        V3 pA[3], cA[3];
-        world_triangle(P.am, P.ap, at, pA);
-        world_triangle(P.am, P.a, at, cA);
-        V3 amin = cmin(pA[0], cA[0]), amax = cmax(pA[0], cA[0]);
-        for (uint32_t q = 1; q < 3; ++q) {
-            amin = cmin(amin, cmin(pA[q], cA[q]));
-            amax = cmax(amax, cmax(pA[q], cA[q]));
-        }
         for (uint32_t bi = 0; bi < b_count; ++bi) {
             const uint32_t bt = b_first + bi;
             if (3u * bt + 2u >= P.bm.s.idx_count) break;
-            V3 pB[3], cB[3];
-            world_triangle(P.bm, P.bp, bt, pB);
-            world_triangle(P.bm, P.b, bt, cB);
-            V3 bmin = cmin(pB[0], cB[0]), bmax = cmax(pB[0], cB[0]);
-            for (uint32_t q = 1; q < 3; ++q) {
-                bmin = cmin(bmin, cmin(pB[q], cB[q]));
-                bmax = cmax(bmax, cmax(pB[q], cB[q]));
-            }
-            const V3 e = {margin, margin, margin};
-            if (!boxes_touch(sub(amin, e), add(amax, e), bmin, bmax)) continue;
-            V3 dA[3], dB[3];
-            float speed_abs = 0.0f;
-            for (uint32_t q = 0; q < 3; ++q) {
-                dA[q] = sub(cA[q], pA[q]);
-                dB[q] = sub(cB[q], pB[q]);
-                speed_abs = smax(speed_abs, len(dA[q]));
-            }
-            float speed_b = 0.0f;
-            for (uint32_t q = 0; q < 3; ++q) speed_b = smax(speed_b, len(dB[q]));
-            speed_abs += speed_b;
-            const V3 common = dB[0];
-            float rel_a = 0.0f, rel_b = 0.0f;
-            for (uint32_t q = 0; q < 3; ++q) {
-                rel_a = smax(rel_a, len(sub(dA[q], common)));
-                rel_b = smax(rel_b, len(sub(dB[q], common)));
-            }
-            const float bound = smin(speed_abs, rel_a + rel_b);
-            if (bound <= kEpsilon) continue;
-            float time = 0.0f;
-            bool starts_near = false;
-            V3 sep_normal = {0.0f, 0.0f, 0.0f};
-            bool have_sep = false;
-            for (uint32_t it = 0; it < 32u; ++it) {
-                V3 A[3], B[3];
-                for (uint32_t q = 0; q < 3; ++q) {
-                    A[q] = add(pA[q], mul(dA[q], time));
-                    B[q] = add(pB[q], mul(dB[q], time));
-                }
-                V3 pa, pb;
-                closest_triangle_pair(A, B, pa, pb);
-                const V3 delta = sub(pa, pb);
-                const float distance = sqrtf(smax(0.0f, len2(delta)));
-                const float offset = rest;
-                if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
-                if (distance <= offset + 1.0e-5f) {
-                    const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
-                    const V3 center = add(sv(P.ap.position), mul(sub(sv(P.a.position), sv(P.ap.position)), time));
-                    V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
-                    const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
-                    const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
-                    const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
-                    if (dot(oa, ob) > 0.9999f) break;
-                    if (len2(cob) > 0.5f) fallback = cob;
-                    else if (len2(coa) > 0.5f) fallback = mul(coa, -1.0f);
-                    const V3 normal = triangle_feature_normal(A, B, pa, pb, have_sep ? sep_normal : fallback);
-                    if (!shell_facing(normal, coa, cob)) break;
-                    const V3 point = mul(add(pa, pb), 0.5f);
-                    const float vn = normal_speed(P.a, P.b, point, normal);
-                    if (vn > 1.0e-5f) break;
-                    const float remaining = -vn * P.dt * (1.0f - time) - distance;
-                    Contact c;
-                    c.normal = normal;
-                    c.point = point;
-                    c.pen = smax(0.0f, remaining + offset) + 1.0e-5f;
-                    c.impact_fraction = starts_near ? 0.0f : time;
-                    add_contact(m, c, smax(margin * 2.0f, 1.0e-4f));
-                    break;
-                }
-                sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
-                have_sep = true;
-                const float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
-                time += smax(advance, 1.0e-5f);
-                if (time > 1.0f) break;
-            }
+            Contact c;
+            if (triangle_pair_swept_contact(P, at, bt, inside_a, inside_b, c)) add_contact(m, c, spacing);
         }
     }
 }
@@ -1253,64 +1306,266 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
     if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
 }
 
-// The triangle-pair path for the queued pairs.  A pair with a candidate
-// slot (active index below `slots`) and at most `limit` overlapping leaf
-// pairs evaluates its candidates in the order 128 workers emit them (each
-// worker takes every 128th cell of the row-major leaf-pair grid) and
-// merges each candidate's manifold in that order; any other pair runs the
-// BVH traversal.
-__global__ void __launch_bounds__(64) k_triangle_pairs(const uint32_t *tri_list, const PmhBodyParameters *P,
-                                                       const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
-                                                       const float4 *planes, const BvhDev *nodes,
-                                                       const uint32_t *leaves, const float4 *shell,
-                                                       const PmhRigidBodyState *prev, const PmhRigidBodyState *pred,
-                                                       const uint2 *ab, NarrowOut o, uint32_t slots, uint32_t limit,
-                                                       float dt) {
+// ---- triangle-pair kernel: one wavefront per queued pair ----------------
+constexpr uint32_t kTriWaves = 4;          // wavefronts per block
+constexpr uint32_t kLeafListCap = 512;     // ordered leaf pairs held in LDS per wavefront
+constexpr uint32_t kListOverflow = 0xffffffffu;
+
+struct TriWaveShared {
+    uint2 list[kLeafListCap];              // (body leaf node, collider leaf node) in evaluation order
+    float4 disc[64][2], swept[64][2];      // per lane: normal+pen, point+impact fraction
+    Manifold man, local;
+    uint32_t count;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= uint32_t(d)) x += y;
+    }
+    return x - v;
+}
+
+__device__ __forceinline__ bool leaf_cell_overlaps(const TrianglePair &tp, bool swept, const BoundsFrame &fa,
+                                                   const BoundsFrame &fb, const BoundsFrame &pfa,
+                                                   const BoundsFrame &pfb, uint64_t f, uint2 &nodes) {
+    const uint32_t lb = tp.bm.s.leaf_count;
+    nodes = make_uint2(tp.am.leaf(uint32_t(f / lb)), tp.bm.leaf(uint32_t(f % lb)));
+    V3 amn, amx, bmn, bmx;
+    node_bounds(tp.am.node(nodes.x), pfa, fa, swept, tp.margin, amn, amx);
+    node_bounds(tp.bm.node(nodes.y), pfb, fb, swept, 0.0f, bmn, bmx);
+    return boxes_touch(amn, amx, bmn, bmx);
+}
+
+// The leaf pairs rigid_collide_meshes visits, in visiting order, or
+// kListOverflow when its stack would overflow (the reference then runs
+// every triangle pair) or the list does not fit.
+__device__ uint32_t traversal_list(const TrianglePair &P, bool swept, uint2 *list) {
+    const BoundsFrame fa = bounds_frame(P.a), fb = bounds_frame(P.b);
+    const BoundsFrame pfa = swept ? bounds_frame(P.ap) : BoundsFrame{};
+    const BoundsFrame pfb = swept ? bounds_frame(P.bp) : BoundsFrame{};
+    V3 amn, amx, bmn, bmx;
+    motion_bounds(sarr(P.am.s.mn), sarr(P.am.s.mx), pfa, fa, swept, P.margin, amn, amx);
+    motion_bounds(sarr(P.bm.s.mn), sarr(P.bm.s.mx), pfb, fb, swept, 0.0f, bmn, bmx);
+    if (!boxes_touch(amn, amx, bmn, bmx)) return 0;
+    if (P.am.s.node_count == 0u || P.bm.s.node_count == 0u) return kListOverflow;
+    constexpr uint32_t kStack = 64;
+    uint32_t sa[kStack], sb[kStack];
+    sa[0] = 0;
+    sb[0] = 0;
+    uint32_t top = 1, count = 0;
+    while (top > 0u) {
+        --top;
+        const uint32_t ia = sa[top], ib = sb[top];
+        const BvhDev na = P.am.node(ia), nb = P.bm.node(ib);
+        node_bounds(na, pfa, fa, swept, P.margin, amn, amx);
+        node_bounds(nb, pfb, fb, swept, 0.0f, bmn, bmx);
+        if (!boxes_touch(amn, amx, bmn, bmx)) continue;
+        const bool la = na.count != 0u, lb = nb.count != 0u;
+        if (la && lb) {
+            if (count == kLeafListCap) return kListOverflow;
+            list[count++] = make_uint2(ia, ib);
+            continue;
+        }
+        const uint32_t need = la || lb ? 2u : 4u;
+        if (top + need > kStack) return kListOverflow;
+        if (la) {
+            sa[top] = ia; sb[top++] = nb.right;
+            sa[top] = ia; sb[top++] = nb.left;
+        } else if (lb) {
+            sa[top] = na.right; sb[top++] = ib;
+            sa[top] = na.left; sb[top++] = ib;
+        } else {
+            sa[top] = na.right; sb[top++] = nb.right;
+            sa[top] = na.right; sb[top++] = nb.left;
+            sa[top] = na.left; sb[top++] = nb.right;
+            sa[top] = na.left; sb[top++] = nb.left;
+        }
+    }
+    return count;
+}
+
+// Serial evaluation for the rare pairs whose leaf list does not fit in
+// LDS: the same candidate order and merge rule on one lane.
+__device__ void triangle_pair_serial(const TrianglePair &tp, bool swept, bool leaf_mode, Manifold &man,
+                                     Manifold &local) {
+    man.count = 0;
+    man.face_patch = 0;
+    if (!leaf_mode) {
+        collide_meshes(tp, swept, man);
+        return;
+    }
+    const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
+    const BoundsFrame pfa = swept ? bounds_frame(tp.ap) : BoundsFrame{};
+    const BoundsFrame pfb = swept ? bounds_frame(tp.bp) : BoundsFrame{};
+    const uint64_t cells = uint64_t(tp.am.s.leaf_count) * tp.bm.s.leaf_count;
+    const float spacing = smax(tp.margin * 2.0f, 1.0e-4f);
+    for (uint32_t w = 0; w < 128u && w < cells; ++w)
+        for (uint64_t f = w; f < cells; f += 128u) {
+            uint2 nodes;
+            if (!leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nodes)) continue;
+            local.count = 0;
+            local.face_patch = 0;
+            collide_leaf_ranges(tp, swept, tp.am.node(nodes.x), tp.bm.node(nodes.y), local);
+            for (uint32_t k = 0; k < local.count; ++k) add_contact(man, local.c[k], spacing);
+        }
+}
+
+// The triangle-pair path for the queued pairs, one wavefront per pair.
+// A pair with a candidate slot (active index below `slots`) and at most
+// `limit` overlapping leaf pairs takes its candidates in the order 128
+// workers emit them (worker w takes cells w, w + 128, ... of the row-major
+// leaf-pair grid) and merges each candidate's own manifold in that order;
+// any other pair takes the leaf pairs of the BVH traversal in visiting
+// order and merges contacts directly.  The lanes evaluate up to 4 leaf
+// pairs x 16 triangle pairs at once (leaves hold at most 4 triangles); one
+// lane then replays the contacts through the merge rule in order: per leaf
+// pair the discrete contacts body-triangle-major, then the swept ones.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_triangle_pairs(const uint32_t *tri_list, const PmhBodyParameters *P,
+                                                        const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
+                                                        const float4 *planes, const BvhDev *nodes,
+                                                        const uint32_t *leaves, const float4 *shell,
+                                                        const PmhRigidBodyState *prev,
+                                                        const PmhRigidBodyState *pred, const uint2 *ab,
+                                                        NarrowOut o, uint32_t slots, uint32_t limit, float dt) {
+    __shared__ TriWaveShared shared_waves[kTriWaves];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    TriWaveShared &S = shared_waves[wv];
     const uint32_t T = o.ctr->tri_pairs;
+    const uint32_t nwaves = gridDim.x * kTriWaves;
     uint32_t fallback = 0;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x * kTriWaves + wv; t < T; t += nwaves) {
         const uint32_t m = tri_list[t];
         const uint2 p = ab[m];
         const PmhBodyParameters pa = P[p.x], pb = P[p.y];
-        TrianglePair tp{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
-                        ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
-                        prev[p.x], pred[p.x], prev[p.y], pred[p.y],
-                        pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
-                        pa.collision_margin + pb.collision_margin, dt};
+        const TrianglePair tp{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                              ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                              prev[p.x], pred[p.x], prev[p.y], pred[p.y],
+                              pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
+                              pa.collision_margin + pb.collision_margin, dt};
         const bool swept = requires_swept(tp.ap, tp.a, tp.am.s, tp.bp, tp.b, tp.bm.s, tp.margin);
         const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
         const BoundsFrame pfa = swept ? bounds_frame(tp.ap) : BoundsFrame{};
         const BoundsFrame pfb = swept ? bounds_frame(tp.bp) : BoundsFrame{};
-        Manifold man;
-        man.count = 0;
-        man.face_patch = 0;
-        bool fits = m < slots;
-        if (fits) fits = count_leaf_candidates(tp, swept, fa, fb, pfa, pfb) <= limit;
-        if (fits) {
-            const uint32_t lb = tp.bm.s.leaf_count;
-            const uint64_t cells = uint64_t(tp.am.s.leaf_count) * lb;
-            const float spacing = smax(tp.margin * 2.0f, 1.0e-4f);
-            for (uint32_t w = 0; w < 128u && w < cells; ++w)
-                for (uint64_t f = w; f < cells; f += 128u) {
-                    const BvhDev na = tp.am.node(tp.am.leaf(uint32_t(f / lb)));
-                    const BvhDev nb = tp.bm.node(tp.bm.leaf(uint32_t(f % lb)));
-                    V3 amn, amx, bmn, bmx;
-                    node_bounds(na, pfa, fa, swept, tp.margin, amn, amx);
-                    node_bounds(nb, pfb, fb, swept, 0.0f, bmn, bmx);
-                    if (!boxes_touch(amn, amx, bmn, bmx)) continue;
-                    Manifold local;
-                    local.count = 0;
-                    local.face_patch = 0;
-                    collide_leaf_ranges(tp, swept, na, nb, local);
-                    for (uint32_t k = 0; k < local.count; ++k) add_contact(man, local.c[k], spacing);
-                }
-        } else {
-            ++fallback;
-            collide_meshes(tp, swept, man);
+        bool inside_a, inside_b;
+        inside_flags(tp, inside_a, inside_b);
+        const uint64_t cells = uint64_t(tp.am.s.leaf_count) * tp.bm.s.leaf_count;
+        // 1. Leaf pairs in evaluation order.
+        bool leaf_mode = m < slots;
+        uint32_t total = 0;
+        if (leaf_mode) {
+            uint32_t mine = 0;
+            for (uint64_t f = lane; f < cells; f += 64u) {
+                uint2 nd;
+                mine += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
+            }
+            total = wave_sum(mine);
+            leaf_mode = total <= limit;
         }
-        emit_manifold(o, m, man, tp.a, tp.b, persistent_pair(tp.am.s, tp.bm.s, pb));
+        uint32_t nlist = 0;
+        if (leaf_mode) {
+            if (total > kLeafListCap) {
+                nlist = kListOverflow;
+            } else {
+                uint32_t c0 = 0, c1 = 0;
+                uint2 nd;
+                for (uint64_t f = lane; f < cells; f += 128u) c0 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
+                for (uint64_t f = lane + 64u; f < cells; f += 128u) c1 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
+                uint32_t at0 = wave_exclusive_scan(c0, lane);
+                uint32_t at1 = wave_sum(c0) + wave_exclusive_scan(c1, lane);
+                for (uint64_t f = lane; f < cells; f += 128u)
+                    if (leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd)) S.list[at0++] = nd;
+                for (uint64_t f = lane + 64u; f < cells; f += 128u)
+                    if (leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd)) S.list[at1++] = nd;
+                nlist = total;
+            }
+        } else {
+            if (lane == 0) {
+                ++fallback;
+                S.count = traversal_list(tp, swept, S.list);
+            }
+            wave_sync();
+            nlist = S.count;
+        }
+        wave_sync();
+        // 2. Contacts.
+        if (nlist == kListOverflow) {
+            if (lane == 0) triangle_pair_serial(tp, swept, leaf_mode, S.man, S.local);
+        } else {
+            if (lane == 0) {
+                S.man.count = 0;
+                S.man.face_patch = 0;
+            }
+            const float spacing = smax(tp.margin * 2.0f, 1.0e-4f);
+            for (uint32_t c0 = 0; c0 < nlist; c0 += 4u) {
+                const uint32_t j = c0 + lane / 16u, q = lane % 16u;
+                bool hd = false, hs = false;
+                Contact cd, cs;
+                if (j < nlist) {
+                    const uint2 e = S.list[j];
+                    const BvhDev na = tp.am.node(e.x), nb = tp.bm.node(e.y);
+                    const uint32_t ai = q / 4u, bi = q % 4u;
+                    if (ai < na.count && bi < nb.count) {
+                        const uint32_t at = na.first + ai, bt = nb.first + bi;
+                        if (3u * at + 2u < tp.am.s.idx_count && 3u * bt + 2u < tp.bm.s.idx_count) {
+                            hd = triangle_pair_contact(tp, at, bt, inside_a, inside_b, cd);
+                            if (swept) hs = triangle_pair_swept_contact(tp, at, bt, inside_a, inside_b, cs);
+                        }
+                    }
+                }
+                if (hd) {
+                    S.disc[lane][0] = f4(cd.normal, cd.pen);
+                    S.disc[lane][1] = f4(cd.point, cd.impact_fraction);
+                }
+                if (hs) {
+                    S.swept[lane][0] = f4(cs.normal, cs.pen);
+                    S.swept[lane][1] = f4(cs.point, cs.impact_fraction);
+                }
+                const uint64_t md = __ballot(hd), ms = __ballot(hs);
+                wave_sync();
+                if (lane == 0) {
+                    for (uint32_t g = 0; g < 4u && c0 + g < nlist; ++g) {
+                        Manifold &dst = leaf_mode ? S.local : S.man;
+                        if (leaf_mode) {
+                            S.local.count = 0;
+                            S.local.face_patch = 0;
+                        }
+                        for (uint32_t k = 16u * g; k < 16u * g + 16u; ++k) {
+                            if (!((md >> k) & 1ull)) continue;
+                            const Contact c{v3(S.disc[k][0]), v3(S.disc[k][1]), S.disc[k][0].w, S.disc[k][1].w};
+                            add_contact(dst, c, spacing);
+                        }
+                        for (uint32_t k = 16u * g; k < 16u * g + 16u; ++k) {
+                            if (!((ms >> k) & 1ull)) continue;
+                            const Contact c{v3(S.swept[k][0]), v3(S.swept[k][1]), S.swept[k][0].w, S.swept[k][1].w};
+                            add_contact(dst, c, spacing);
+                        }
+                        if (leaf_mode)
+                            for (uint32_t k = 0; k < S.local.count; ++k) add_contact(S.man, S.local.c[k], spacing);
+                    }
+                }
+                wave_sync();
+            }
+        }
+        // 3. Emit.
+        if (lane == 0) {
+            Manifold man = S.man;
+            emit_manifold(o, m, man, tp.a, tp.b, persistent_pair(tp.am.s, tp.bm.s, pb));
+        }
+        wave_sync();
     }
-    if (fallback) atomicAdd(&o.ctr->fallback_pairs, fallback);
+    if (lane == 0 && fallback) atomicAdd(&o.ctr->fallback_pairs, fallback);
 }
 
 __global__ void k_nonempty(const uint32_t *npairs, const uint32_t *m_count, uint32_t *flag) {
@@ -1466,7 +1721,7 @@ PmhResult World::step_once(uint32_t k_step) {
     k_step_begin<<<1, 64, 0, s>>>(ctr);
     PmhRigidBodyState *cur = state_cur.p, *prev = state_prev.p, *pred = state_next.p;
     k_integrate_proxy<<<bg, 256, 0, s>>>(n, params.p, shapes_dev.p, cur, prev, pred, proxy_mn.p, proxy_mx.p,
-                                         gravity, dt, ctr);
+                                         gravity, dt, inputs_dev.p, targets_dev.p, target_flags_dev.p, ctr);
     mark(1);
     // Broadphase.
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * (table_size + 1), s));
@@ -1479,12 +1734,12 @@ PmhResult World::step_once(uint32_t k_step) {
     k_bp_query<0><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
                                                   cand_count.p, nullptr, nullptr, nullptr, ctr);
-    k_bp_large<0><<<64, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, nullptr, nullptr,
+    k_bp_large<0><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, nullptr, nullptr,
                                      nullptr, ctr);
     r = exclusive_scan(s, cand_count.p, cand_offset.p, n, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     k_set_pairs<<<1, 64, 0, s>>>(ctr, cand_offset.p, n, v.pair_cap);
-    k_bp_large<1><<<64, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, cand_offset.p,
+    k_bp_large<1><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, cand_offset.p,
                                      v.code.p, v.ab.p, ctr);
     k_bp_query<1><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
@@ -1497,7 +1752,7 @@ PmhResult World::step_once(uint32_t k_step) {
     k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
         &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p, out,
         v.tri_list.p);
-    k_triangle_pairs<<<blocks_for(v.pair_cap, 64, 2048), 64, 0, s>>>(
+    k_triangle_pairs<<<2048, 256, 0, s>>>(
         v.tri_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, nodes_dev.p, leaves_dev.p,
         shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt);
     k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap);

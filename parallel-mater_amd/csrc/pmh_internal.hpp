@@ -100,7 +100,8 @@ struct StepCounters {
     uint32_t point_alloc;      // narrowphase point-range allocator
     uint32_t tri_pairs;        // mesh pairs on the triangle-pair path
     uint32_t fallback_pairs;   // of those, solved by BVH traversal
-    uint32_t pad[4];
+    uint32_t mean_roots;       // unanchored islands with contacts (free-motion mean)
+    uint32_t pad[3];
 };
 
 // Running totals over every step since the last reset.
@@ -142,6 +143,7 @@ struct Solver {
     DBuf<uint32_t> color, parent, impact, deg, adj_start, adj_cursor;
     DBuf<uint32_t> color_bodies, bp_start, anchored;
     DBuf<float4> mean;                   // free-motion mean per root
+    DBuf<uint32_t> members, mean_list;   // dynamic members per root; roots taking the mean
     // pairs
     DBuf<uint32_t> code;                 // a*n+b
     DBuf<uint2> ab;

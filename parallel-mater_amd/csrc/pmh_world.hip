@@ -379,8 +379,11 @@ PmhResult World::ensure_device() {
             (r = bucket_start.reserve(table_size + 1)) || (r = bucket_entries.reserve(n)) ||
             (r = large.reserve(n)) ||
             (r = cand_count.reserve(n + 1)) || (r = cand_offset.reserve(n + 1)) ||
-            (r = cache_n.reserve(2)))
+            (r = cache_n.reserve(2)) || (r = inputs_dev.reserve(4 * size_t(n))) ||
+            (r = targets_dev.reserve(n)) || (r = target_flags_dev.reserve(n)))
             return r;
+        PMH_CHECK(hipMemset(inputs_dev.p, 0, sizeof(float4) * 4 * size_t(n)));
+        PMH_CHECK(hipMemset(target_flags_dev.p, 0, sizeof(uint32_t) * n));
         for (int k = 0; k < 2; ++k)
             if ((r = cache_code[k].reserve(cache_capacity)) || (r = cache_count[k].reserve(cache_capacity)) ||
                 (r = cache_data[k].reserve(size_t(cache_capacity) * 8 * kCacheFloat4PerPoint)))
@@ -393,6 +396,19 @@ PmhResult World::ensure_device() {
         PMH_CHECK(hipMemset(cache_n.p, 0, sizeof(uint32_t) * 2));
         bodies_dirty = false;
     }
+    if (inputs_dirty) {
+        // The step consumes and clears them (pm_clear_rigid_inputs).
+        PMH_CHECK(hipMemcpyAsync(inputs_dev.p, host_inputs.data(), sizeof(float4) * host_inputs.size(),
+                                 hipMemcpyHostToDevice, stream));
+        PMH_CHECK(hipMemcpyAsync(targets_dev.p, host_targets.data(), sizeof(PmhRigidBodyState) * host_targets.size(),
+                                 hipMemcpyHostToDevice, stream));
+        PMH_CHECK(hipMemcpyAsync(target_flags_dev.p, host_target_flags.data(),
+                                 sizeof(uint32_t) * host_target_flags.size(), hipMemcpyHostToDevice, stream));
+        PMH_CHECK(hipStreamSynchronize(stream));
+        std::fill(host_inputs.begin(), host_inputs.end(), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        std::fill(host_target_flags.begin(), host_target_flags.end(), 0u);
+        inputs_dirty = false;
+    }
     return PMH_SUCCESS;
 }
 
@@ -400,6 +416,7 @@ void World::release() {
     if (stream) (void)hipStreamSynchronize(stream);
     shapes_dev.release(); verts_dev.release(); idx_dev.release(); planes_dev.release();
     nodes_dev.release(); leaves_dev.release(); shell_dev.release();
+    inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -590,6 +607,65 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     if (c.status & (pmh::kStatusPairOverflow | pmh::kStatusPointOverflow | pmh::kStatusColorOverflow))
         return PMH_ERROR_CAPACITY_EXCEEDED;
     if (c.status & pmh::kStatusColorTimeout) return PMH_ERROR_INVALID_STATE;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldApplyRigidInputs(PmhWorld world, uint32_t count, const PmhRigidInput *inputs) {
+    if (!world || (count && !inputs)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    const uint32_t n = w->body_count;
+    for (uint32_t k = 0; k < count; ++k) {
+        const PmhRigidInput &in = inputs[k];
+        if (in.body_index >= n || w->host_params[in.body_index].motion != pmh::kDynamic)
+            return PMH_ERROR_INVALID_ARGUMENT;
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(in.force[c]) || !std::isfinite(in.torque[c]) ||
+                !std::isfinite(in.linear_impulse[c]) || !std::isfinite(in.angular_impulse[c]))
+                return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    w->host_inputs.resize(4 * size_t(n), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    w->host_targets.resize(n);
+    w->host_target_flags.resize(n, 0u);
+    for (uint32_t k = 0; k < count; ++k) {
+        const PmhRigidInput &in = inputs[k];
+        const float *v[4] = {in.force, in.torque, in.linear_impulse, in.angular_impulse};
+        for (int q = 0; q < 4; ++q) {
+            float4 &a = w->host_inputs[4 * size_t(in.body_index) + q];
+            a.x = a.x + v[q][0];
+            a.y = a.y + v[q][1];
+            a.z = a.z + v[q][2];
+        }
+    }
+    w->inputs_dirty = true;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t count,
+                                      const PmhRigidBodyState *targets) {
+    if (!world || (count && !targets)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    const uint32_t n = w->body_count;
+    if (uint64_t(first) + count > n) return PMH_ERROR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (w->host_params[first + k].motion != pmh::kKinematic) return PMH_ERROR_INVALID_ARGUMENT;
+        const float *f = reinterpret_cast<const float *>(&targets[k]);
+        for (size_t c = 0; c < sizeof(PmhRigidBodyState) / sizeof(float); ++c)
+            if (!std::isfinite(f[c])) return PMH_ERROR_INVALID_ARGUMENT;
+        const PmhQuat o = targets[k].orientation;
+        if (o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w <= pmh::kEpsilon * pmh::kEpsilon)
+            return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    w->host_inputs.resize(4 * size_t(n), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    w->host_targets.resize(n);
+    w->host_target_flags.resize(n, 0u);
+    for (uint32_t k = 0; k < count; ++k) {
+        PmhRigidBodyState t = targets[k];
+        const pmh::Q q = pmh::qnormalize({t.orientation.x, t.orientation.y, t.orientation.z, t.orientation.w});
+        t.orientation = {q.x, q.y, q.z, q.w};
+        w->host_targets[first + k] = t;
+        w->host_target_flags[first + k] = 1u;
+    }
+    w->inputs_dirty = true;
     return PMH_SUCCESS;
 }
 

@@ -21,6 +21,12 @@ struct World {
     std::vector<PmhBodyParameters> host_params;
     std::vector<PmhRigidBodyState> host_states;
     bool shapes_dirty = true, bodies_dirty = true;
+    // One-step inputs: 4 float4 per body (force, torque, linear and angular
+    // impulse) and kinematic targets, uploaded before the next step.
+    std::vector<float4> host_inputs;
+    std::vector<PmhRigidBodyState> host_targets;
+    std::vector<uint32_t> host_target_flags;
+    bool inputs_dirty = false;
     uint32_t body_count = 0;
 
     // device scene
@@ -33,6 +39,9 @@ struct World {
     DBuf<float4> shell_dev;
     DBuf<PmhBodyParameters> params;
     DBuf<PmhRigidBodyState> state_cur, state_prev, state_next;
+    DBuf<float4> inputs_dev;
+    DBuf<PmhRigidBodyState> targets_dev;
+    DBuf<uint32_t> target_flags_dev;
 
     // broadphase
     DBuf<float4> proxy_mn, proxy_mx;

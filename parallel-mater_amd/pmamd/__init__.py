@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 from . import layouts  # noqa: F401
-from .layouts import (AVBD_BODY, MANIFOLD, PARAMS, SCHEDULE, STATE)
+from .layouts import (AVBD_BODY, MANIFOLD, PARAMS, RIGID_INPUT, SCHEDULE, STATE)
 
 HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(HERE, "lib", "libpm_hip_rigid.so")
@@ -94,6 +94,8 @@ def lib():
         "pmhWorldAddBodies": (i32, [_P, _U, _P, _P]),
         "pmhWorldBodyCount": (_U, [_P]),
         "pmhWorldStep": (i32, [_P, _U]),
+        "pmhWorldApplyRigidInputs": (i32, [_P, _U, _P]),
+        "pmhWorldSetKinematicTargets": (i32, [_P, _U, _U, _P]),
         "pmhWorldCopyStates": (i32, [_P, _P]),
         "pmhWorldSetStates": (i32, [_P, _P]),
         "pmhWorldGetStats": (i32, [_P, _P]),
@@ -195,6 +197,17 @@ class World:
     @property
     def body_count(self):
         return lib().pmhWorldBodyCount(self._h)
+
+    def apply_inputs(self, inputs):
+        """Forces, torques and impulses (layouts.RIGID_INPUT records) for
+        the next step."""
+        a = np.ascontiguousarray(inputs, dtype=RIGID_INPUT)
+        _check(lib().pmhWorldApplyRigidInputs(self._h, len(a), _ptr(a)), "pmhWorldApplyRigidInputs")
+
+    def set_kinematic_targets(self, first, targets):
+        a = np.ascontiguousarray(targets, dtype=STATE)
+        _check(lib().pmhWorldSetKinematicTargets(self._h, first, len(a), _ptr(a)),
+               "pmhWorldSetKinematicTargets")
 
     def step(self, steps=1):
         _check(lib().pmhWorldStep(self._h, steps), "pmhWorldStep")

@@ -20,6 +20,10 @@ assert STATE.itemsize == 52
 
 MOTION_STATIC, MOTION_KINEMATIC, MOTION_DYNAMIC = 0, 1, 2
 
+RIGID_INPUT = np.dtype([                      # PmRigidInput, 52 bytes
+    ("body_index", "<u4"), ("force", "<f4", 3), ("torque", "<f4", 3),
+    ("linear_impulse", "<f4", 3), ("angular_impulse", "<f4", 3)])
+
 PARAMS = np.dtype({
     "names": ["motion", "mesh_index", "mesh_generation", "inverse_mass",
               "inverse_inertia_local", "friction", "restitution",

@@ -58,6 +58,8 @@ def lib():
         _lib.pmo_world_add_capsule.argtypes = [_P, _F, _F]
         _lib.pmo_world_add_bodies.argtypes = [_P, _U, _P, _P]
         _lib.pmo_world_step.argtypes = [_P, _U]
+        _lib.pmo_world_apply_inputs.argtypes = [_P, _U, _P]
+        _lib.pmo_world_set_kinematic_targets.argtypes = [_P, _U, _U, _P]
         _lib.pmo_world_body_count.argtypes = [_P]
         _lib.pmo_world_body_count.restype = _U
         _lib.pmo_world_timing.argtypes = [_P, _P, ctypes.c_int]
@@ -142,6 +144,16 @@ class OracleWorld:
         s = np.ascontiguousarray(states, dtype=L.STATE)
         if lib().pmo_world_add_bodies(self._h, len(p), _ptr(p), _ptr(s)) != 0:
             raise ValueError("bodies rejected")
+
+    def apply_inputs(self, inputs):
+        a = np.ascontiguousarray(inputs, dtype=L.RIGID_INPUT)
+        if lib().pmo_world_apply_inputs(self._h, len(a), _ptr(a)) != 0:
+            raise ValueError("invalid rigid inputs")
+
+    def set_kinematic_targets(self, first, targets):
+        a = np.ascontiguousarray(targets, dtype=L.STATE)
+        if lib().pmo_world_set_kinematic_targets(self._h, first, len(a), _ptr(a)) != 0:
+            raise ValueError("invalid kinematic targets")
 
     def step(self, steps=1):
         r = lib().pmo_world_step(self._h, steps)

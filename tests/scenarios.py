@@ -435,3 +435,15 @@ def crossing_edges(gap=5e-4, half=0.5):
     st["orientation"][0] = quat((0, 0, 1), np.pi / 4)
     st["orientation"][1] = quat((1, 0, 0), np.pi / 4)
     return Scene(shapes, p, st, "crossing_edges")
+
+
+def pusher(rows=5, seed=1):
+    """A pyramid of unit boxes (rows) with a kinematic unit box beside it
+    (last body) that kinematic targets drive into the stack."""
+    base = pyramid(rows=rows, seed=seed)
+    n = base.n + 1
+    p = np.concatenate([base.params, params(1, motion=np.array([L.MOTION_KINEMATIC], np.uint32), shape=1,
+                                            margin=REF_MARGIN, damping=REF_DAMPING, max_linear=REF_MAX_SPEED,
+                                            max_angular=REF_MAX_SPEED)])
+    st = np.concatenate([base.states, states(1, np.array([[-rows / 2.0 - 1.5, 0.5, 0.0]], np.float32))])
+    return Scene(base.shapes, p, st, "pusher")

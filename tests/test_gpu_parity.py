@@ -168,6 +168,34 @@ def test_triangle_path_fallback_matches_oracle():
     assert_states_close(g.states(), o.states(), rtol=1e-3, atol=1e-3)
 
 
+def test_inputs_and_kinematic_targets_follow_the_oracle():
+    say("pusher: kinematic targets and rigid inputs, 40 steps")
+    scene = S.pusher()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    k = scene.n - 1
+    x0 = float(scene.states["position"][k][0])
+    for step in range(40):
+        t = S.states(1, np.array([[x0 + 0.04 * (step + 1), 0.5, 0.0]], np.float32))
+        t["orientation"][0] = S.quat((0, 1, 0), 0.01 * step)
+        g.set_kinematic_targets(k, t)
+        o.set_kinematic_targets(k, t)
+        if step % 10 == 0:
+            inp = np.zeros(2, L.RIGID_INPUT)
+            inp["body_index"] = [scene.n - 2, 1]
+            inp["linear_impulse"][0] = (0.0, 500.0, 100.0)
+            inp["torque"][1] = (0.0, 2000.0, 0.0)
+            g.apply_inputs(inp)
+            o.apply_inputs(inp)
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs(), err_msg=f"step {step}")
+    gs, os_ = g.states(), o.states()
+    say(f"bit-exact state fraction {exact_fraction(gs.view(np.float32), os_.view(np.float32)):.4f}")
+    np.testing.assert_array_equal(gs[k]["position"], os_[k]["position"])
+    assert_states_close(gs, os_, rtol=1e-4, atol=1e-4)
+
+
 @pytest.mark.parametrize("lanes", [8, 16, 64])
 def test_lane_counts_follow_the_oracle(lanes):
     say(f"pyramid 5 steps with {lanes} lanes per body")
