@@ -39,6 +39,9 @@ static_assert(sizeof(PmoContact) == 284, "Contact layout");
 static_assert(sizeof(PmoManifold) == 2304, "ContactManifold layout");
 static_assert(sizeof(PmoAvbdBody) == 236, "AvbdBody layout");
 static_assert(sizeof(PmoSchedule) == 264, "ContactSchedule layout");
+static_assert(sizeof(PmoRow) == 88, "RigidAvbdRow layout");
+static_assert(sizeof(PmoJointOptions) == 228, "RigidConstraintOptions layout");
+static_assert(sizeof(PmoJoint) == 1852, "RigidConstraintResource layout");
 
 namespace {
 
@@ -1601,6 +1604,8 @@ struct Ctx {
     PmoSchedule *S;
     float dt;
     uint32_t lanes;
+    PmoJoint *J = nullptr;
+    uint32_t nj = 0;
 };
 
 inline bool dyn(const Ctx &x, uint32_t i) { return x.P[i].motion == kDynamic; }
@@ -1620,6 +1625,245 @@ bool contact_impact(const Ctx &x, uint32_t active) {
         impact = impact || fresh || prev_pen > skin + 1.0e-4f;
     }
     return impact;
+}
+
+// ---------------------------------------------------------------------
+// Joints: rigid_avbd_visit_joint(_row), geometry_constraints.slang:
+// 289-445, 626-926.  One routine builds every row of a joint for the four
+// visits (initialise in prepare, accumulate in the colour sweep, advance
+// in the dual update, the velocity-phase rows).  Pinned by closed forms
+// (a pendulum's anchor drift and period, a hinge's free axis, a fixed
+// weld, limits and breaking) in the tests; no reference-held vectors.
+// ---------------------------------------------------------------------
+enum JointType : uint32_t { kFixed = 0, kPoint = 1, kHinge = 2, kSlider = 3, kPiston = 4, kGeneric = 5,
+                            kGenericSpring = 6, kMotor = 7 };
+enum JointVisit : uint32_t { kInitialize = 0, kAccumulate = 1, kAdvance = 2, kImpact = 3 };
+constexpr float kLimitStabilization = 0.8f;                   // geometry_constraints.slang:32
+
+inline V3 basis_axis(uint32_t a) { return a == 0 ? V3{1, 0, 0} : (a == 1 ? V3{0, 1, 0} : V3{0, 0, 1}); }
+inline Vec6 jac6(V3 l, V3 a) { return {{l.x, l.y, l.z, a.x, a.y, a.z}}; }
+inline V3 relative_rotation_vector(Q a, Q b) {                 // :312-316
+    return quaternion_rotation_vector(relative_quaternion(a, b, true));
+}
+// rigid_avbd_log_axis, :401-422: the derivative of the rotation
+// logarithm along a body axis (series below one radian).
+V3 log_axis(Q frame, V3 lg, uint32_t axis) {
+    float sq = len2(lg);
+    float coef = 1.0f / 12.0f + sq * (1.0f / 720.0f + sq * (1.0f / 30240.0f + sq * (1.0f / 1209600.0f +
+                 sq / 47900160.0f)));
+    if (sq > 1.0f) {
+        float angle = std::sqrt(sq);
+        float half = 0.5f * angle;
+        coef = (1.0f - half * std::cos(half) / std::sin(half)) / sq;
+    }
+    V3 unit = basis_axis(axis);
+    V3 local = add(add(unit, mul(cross(lg, unit), 0.5f)), mul(cross(lg, cross(lg, unit)), coef));
+    return rotate(frame, local);
+}
+
+struct JointRowArgs {
+    Vec6 ja, jb;
+    float current, previous, penalty;
+    bool angular;
+    float stiffness, damping, lower, upper;
+    bool motor;
+    float motor_speed;
+    V3 curvature_a, curvature_b;
+    float stabilization;
+};
+
+// rigid_avbd_visit_joint_row, :626-713.
+void visit_joint_row(Ctx &x, PmoJoint &j, Block &blk, uint32_t &index, float &impulse, bool first,
+                     const JointRowArgs &r, uint32_t visit) {
+    if (index >= 18u) return;
+    const uint32_t ri = index++;
+    Row row = row_from(reinterpret_cast<const float *>(&j.rows[ri]));
+    const uint32_t a = j.dense_a, b = j.dense_b;
+    const Vec6 da = disp(x.B[a]), db = disp(x.B[b]);
+    const float linearized = dot6(r.ja, da) + dot6(r.jb, db);
+    row.a = r.ja;
+    row.b = r.jb;
+    row.stiffness = r.stiffness;
+    row.damping = r.damping;
+    if (visit == kInitialize) {
+        if (!(row.dual.penalty > kMinPenalty)) row.dual.penalty = r.penalty;
+        row.dual = warm_start(row.dual, r.stiffness);
+        if (r.stiffness < kMaxPenalty) row.dual.lambda = 0.0f;
+        row.beta = r.penalty * (r.angular ? 10.0f : 100.0f);
+        row.reference_force = 0.0f;
+    }
+    row.error = r.current - (r.stiffness < kMaxPenalty ? 0.0f : r.stabilization * r.previous) - linearized;
+    row.velocity = r.motor ? r.motor_speed : (r.current - r.previous - linearized) / x.dt;
+    row.lower = r.lower;
+    row.upper = r.upper;
+    if (visit == kAccumulate) {
+        row_accumulate(blk, row, first, da, db, x.dt);
+        if (!r.angular) {
+            // Geometric stiffness of a linear row on the angular diagonal.
+            V3 axis = {r.jb.v[0], r.jb.v[1], r.jb.v[2]};
+            V3 arm = first ? r.curvature_a : r.curvature_b;
+            float force = row_force(row, da, db, x.dt);
+            for (uint32_t c = 0; c < 3; ++c) {
+                V3 value = sub(mul(add(mul(axis, comp(arm, c)), mul(arm, comp(axis, c))), 0.5f),
+                               mul(basis_axis(c), dot(axis, arm)));
+                blk.h[bidx(3 + c, 3 + c)] += std::fabs(force) * std::sqrt(len2(value));
+            }
+        }
+        return;
+    }
+    if (visit == kAdvance) {
+        impulse += std::fabs(row_force(row, da, db, x.dt)) * x.dt;
+        row_advance(row, da, db);
+    } else if (visit == kImpact) {
+        if (r.stiffness < kMaxPenalty) {
+            row.reference_force = row_force(row, da, db, x.dt);
+        } else {
+            float gap = r.lower == 0.0f ? fmin_(0.0f, r.current) : (r.upper == 0.0f ? fmax_(0.0f, r.current) : 0.0f);
+            Vec6 va = jac6(mul(x.B[a].target.linear_velocity, x.dt), mul(x.B[a].target.angular_velocity, x.dt));
+            Vec6 vb = jac6(mul(x.B[b].target.linear_velocity, x.dt), mul(x.B[b].target.angular_velocity, x.dt));
+            row.reference_force = row.dual.lambda;
+            row.error = dot6(r.ja, va) + dot6(r.jb, vb) + gap - (r.motor ? r.motor_speed * x.dt : 0.0f);
+        }
+    }
+    row_to(row, reinterpret_cast<float *>(&j.rows[ri]));
+}
+
+// rigid_avbd_visit_joint, :715-926.  Returns the row impulses of an
+// advance visit.
+float visit_joint(Ctx &x, PmoJoint &j, Block &blk, bool first, uint32_t visit) {
+    uint32_t index = 0;
+    float impulse = 0.0f;
+    const PmoJointOptions &o = j.options;
+    const uint32_t a = j.dense_a, b = j.dense_b;
+    const PmoState sa = avbd_pose(x.B[a]), sb = avbd_pose(x.B[b]);
+    const V3 armA = rotate(sa.orientation, o.local_anchor_a), armB = rotate(sb.orientation, o.local_anchor_b);
+    const V3 error = sub(add(sb.position, armB), add(sa.position, armA));
+    const V3 old_error = sub(add(x.prev[b].position, rotate(x.prev[b].orientation, o.local_anchor_b)),
+                             add(x.prev[a].position, rotate(x.prev[a].orientation, o.local_anchor_a)));
+    const Q fA = qnormalize(qmul(sa.orientation, o.local_orientation_a));
+    const Q fB = qnormalize(qmul(sb.orientation, o.local_orientation_b));
+    const Q ofA = qnormalize(qmul(x.prev[a].orientation, o.local_orientation_a));
+    const Q ofB = qnormalize(qmul(x.prev[b].orientation, o.local_orientation_b));
+    const V3 twist = relative_rotation_vector(fA, fB);
+    const V3 old_twist = relative_rotation_vector(ofA, ofB);
+    const uint32_t type = o.type;
+    const bool swing = type == kHinge || type == kPiston || (type == kMotor && o.motor.angular_enabled != 0u);
+    const V3 free_axis = type == kHinge ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 swing_axis = inv_rotate(fA, rotate(fB, free_axis));
+    const V3 old_swing_axis = inv_rotate(ofA, rotate(ofB, free_axis));
+    const V3 rotation = swing ? cross(free_axis, swing_axis) : twist;
+    const V3 old_rotation = swing ? cross(free_axis, old_swing_axis) : old_twist;
+    const float linear_penalty = pair_stiffness(x.P[a], x.P[b], x.dt);
+    const bool generic = type == kGeneric || type == kGenericSpring;
+    const bool motor_type = type == kMotor;
+    for (uint32_t axis = 0; axis < 3; ++axis) {
+        const V3 direction = rotate(fA, basis_axis(axis));
+        const V3 old_direction = rotate(ofA, basis_axis(axis));
+        for (uint32_t av = 0; av < 2; ++av) {
+            const bool angular = av != 0;
+            const bool lock = angular
+                ? type == kFixed || type == kSlider || (type == kHinge && axis != 2u) ||
+                      (type == kPiston && axis != 0u) ||
+                      (motor_type && (axis != 0u || o.motor.angular_enabled == 0u))
+                : type == kFixed || type == kPoint || type == kHinge ||
+                      ((type == kSlider || type == kPiston) && axis != 0u) ||
+                      (motor_type && (axis != 0u || o.motor.linear_enabled == 0u));
+            const PmoJointLimits &limits = angular ? o.angular_limits : o.linear_limits;
+            const PmoJointSprings &springs = angular ? o.angular_springs : o.linear_springs;
+            const float current = angular ? comp(rotation, axis) : dot(error, direction);
+            const float old_current = angular ? comp(old_rotation, axis) : dot(old_error, old_direction);
+            const V3 angular_axis = swing
+                ? rotate(fA, sub(mul(basis_axis(axis), dot(free_axis, swing_axis)),
+                                 mul(free_axis, comp(swing_axis, axis))))
+                : log_axis(fA, twist, axis);
+            const V3 zero = {0.0f, 0.0f, 0.0f};
+            const Vec6 ja = angular ? jac6(zero, mul(angular_axis, -1.0f))
+                                    : jac6(mul(direction, -1.0f), add(mul(cross(armA, direction), -1.0f),
+                                                                     cross(direction, error)));
+            const Vec6 jb = angular ? jac6(zero, angular_axis) : jac6(direction, cross(armB, direction));
+            const float penalty = angular
+                ? 1.0f / (fmax_(dot(direction, add(inverse_inertia_world(x.P[a], sa, direction),
+                                                   inverse_inertia_world(x.P[b], sb, direction))),
+                                1.0e-6f) * x.dt * x.dt)
+                : linear_penalty;
+            const float lock_penalty = angular
+                ? fmax_(penalty, 2.0f * linear_penalty * fmax_(len2(cross(direction, armA)), len2(cross(direction, armB))))
+                : penalty;
+            const V3 curv_a = add(error, armA);
+            if (lock) {
+                if (visit == kInitialize && angular && index < 18u)
+                    j.rows[index].penalty = fmax_(j.rows[index].penalty, lock_penalty / kGamma);
+                visit_joint_row(x, j, blk, index, impulse, first,
+                                {ja, jb, current, old_current, lock_penalty, angular, kMaxPenalty, 0.0f,
+                                 -kMaxPenalty, kMaxPenalty, false, 0.0f, curv_a, armB, kAlpha},
+                                visit);
+            }
+            const bool free_limit = angular ? (type == kHinge && axis == 2u) || (type == kPiston && axis == 0u)
+                                            : (type == kSlider || type == kPiston) && axis == 0u;
+            if ((generic || free_limit) && (limits.axes & (1u << axis)) != 0u) {
+                const float limit_current = angular ? comp(twist, axis) : current;
+                const float old_limit_current = angular ? comp(old_twist, axis) : old_current;
+                const V3 limit_axis = log_axis(fA, twist, axis);
+                const Vec6 la = angular ? jac6(zero, mul(limit_axis, -1.0f)) : ja;
+                const Vec6 lb = angular ? jac6(zero, limit_axis) : jb;
+                const float lower = comp(limits.lower, axis), upper = comp(limits.upper, axis);
+                if (lower == upper) {
+                    const float equal_penalty = angular && generic ? lock_penalty : penalty;
+                    if (visit == kInitialize && angular && generic && index < 18u)
+                        j.rows[index].penalty = fmax_(j.rows[index].penalty, equal_penalty / kGamma);
+                    visit_joint_row(x, j, blk, index, impulse, first,
+                                    {la, lb, limit_current - lower, old_limit_current - lower, equal_penalty, angular,
+                                     kMaxPenalty, 0.0f, -kMaxPenalty, kMaxPenalty, false, 0.0f, curv_a, armB, kAlpha},
+                                    visit);
+                } else if (lower < upper) {
+                    visit_joint_row(x, j, blk, index, impulse, first,
+                                    {la, lb, limit_current - upper, fmax_(old_limit_current - upper, 0.0f), penalty,
+                                     angular, kMaxPenalty, 0.0f, 0.0f, kMaxPenalty, false, 0.0f, curv_a, armB,
+                                     kLimitStabilization},
+                                    visit);
+                    visit_joint_row(x, j, blk, index, impulse, first,
+                                    {la, lb, limit_current - lower, fmin_(old_limit_current - lower, 0.0f), penalty,
+                                     angular, kMaxPenalty, 0.0f, -kMaxPenalty, 0.0f, false, 0.0f, curv_a, armB,
+                                     kLimitStabilization},
+                                    visit);
+                }
+            }
+            if (type == kGenericSpring && (springs.axes & (1u << axis)) != 0u)
+                visit_joint_row(x, j, blk, index, impulse, first,
+                                {ja, jb, current, old_current, penalty, angular, comp(springs.stiffness, axis),
+                                 comp(springs.damping, axis), -kMaxPenalty, kMaxPenalty, false, 0.0f, curv_a, armB,
+                                 kAlpha},
+                                visit);
+            if (motor_type && axis == 0u && (angular ? o.motor.angular_enabled != 0u : o.motor.linear_enabled != 0u)) {
+                const float speed = angular ? o.motor.angular_target_velocity : o.motor.linear_target_velocity;
+                const float bound = (angular ? o.motor.angular_maximum_impulse : o.motor.linear_maximum_impulse) / x.dt;
+                const Q rel = relative_quaternion(fA, fB, true);
+                const Q old_rel = relative_quaternion(ofA, ofB, true);
+                const V3 motor_rotation = relative_rotation_vector({0.0f, 0.0f, 0.0f, 1.0f},
+                                                                   relative_quaternion(old_rel, rel, false));
+                const V3 motor_axis = log_axis(fA, motor_rotation, axis);
+                const Vec6 ma = angular ? jac6(zero, mul(motor_axis, -1.0f)) : ja;
+                const Vec6 mb = angular ? jac6(zero, motor_axis) : jb;
+                const float travel = angular ? comp(motor_rotation, axis) : current - old_current;
+                visit_joint_row(x, j, blk, index, impulse, first,
+                                {ma, mb, travel - speed * x.dt, 0.0f, penalty, angular, kMaxPenalty, 0.0f, -bound,
+                                 bound, true, speed, curv_a, armB, kAlpha},
+                                visit);
+            }
+        }
+    }
+    if (visit == kInitialize) j.avbd_count = index;
+    return impulse;
+}
+
+// Joints incident to body i in list order (push-front: descending index).
+template <class F>
+void for_each_joint(const Ctx &x, uint32_t i, F f) {
+    for (uint32_t k = x.B[i].joint_head; k != kInvalid;) {
+        const bool first = i == x.J[k].dense_a;
+        f(k, first);
+        k = first ? x.J[k].avbd_next_a : x.J[k].avbd_next_b;
+    }
 }
 
 void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, uint32_t profile) {
@@ -1704,6 +1948,23 @@ void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, u
             }
         }
     }
+    // Joints, geometry_constraints.slang:1391-1417 (body ids are
+    // {dense index, 1}: rigid_avbd_find_dense over this world's ids).
+    for (uint32_t k = 0; k < x.nj; ++k) {
+        PmoJoint &j = x.J[k];
+        j.avbd_count = 0u;
+        if (j.alive == 0u) continue;
+        j.state_enabled = j.options.enabled != 0u && j.state_broken == 0u ? 1u : 0u;
+        if (j.state_enabled == 0u) continue;
+        j.applied_impulse = 0.0f;
+        auto find = [&](PmoBodyId id) { return id.index < x.n && id.generation == 1u ? id.index : kInvalid; };
+        j.dense_a = find(j.options.body_a);
+        j.dense_b = find(j.options.body_b);
+        if (j.dense_a != kInvalid && j.dense_b != kInvalid) {
+            Block scratch = zero_block();
+            visit_joint(x, j, scratch, true, kInitialize);
+        }
+    }
     // Schedule and graph (lane 0), geometry_constraints.slang:1432-1560.
     PmoSchedule s;
     std::memset(&s, 0, sizeof s);
@@ -1730,6 +1991,17 @@ void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, u
         x.M[active].avbd_next_b = x.B[b].contact_head;
         x.B[b].contact_head = active;
     }
+    for (uint32_t k = 0; k < x.nj; ++k) {
+        PmoJoint &j = x.J[k];
+        if (j.avbd_count == 0u) continue;
+        uint32_t a = j.dense_a, b = j.dense_b;
+        if (!realtime && requested == 0u) s.budget = std::max(s.budget, j.options.solver_iterations);
+        unite(a, b);
+        j.avbd_next_a = x.B[a].joint_head;
+        x.B[a].joint_head = k;
+        j.avbd_next_b = x.B[b].joint_head;
+        x.B[b].joint_head = k;
+    }
     for (uint32_t i = 0; i < x.n; ++i) parent[i] = root(i);
     std::vector<uint32_t> impact(x.n, 0u);
     for (uint32_t active = 0; active < x.np; ++active) {
@@ -1743,7 +2015,7 @@ void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, u
     for (uint32_t i = 0; i < x.n; ++i) {
         if (!dyn(x, i)) continue;
         impact[i] = impact[parent[i]];
-        if (parent[i] == i && x.B[i].contact_head != kInvalid) s.island_count++;
+        if (parent[i] == i && (x.B[i].contact_head != kInvalid || x.B[i].joint_head != kInvalid)) s.island_count++;
     }
     for (uint32_t i = 0; i < x.n; ++i) {
         if (!dyn(x, i)) continue;
@@ -1756,6 +2028,9 @@ void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, u
                 conflict = conflict || x.B[first ? b : a].color == color;
                 active = first ? x.M[active].avbd_next_a : x.M[active].avbd_next_b;
             }
+            for_each_joint(x, i, [&](uint32_t k, bool first) {
+                conflict = conflict || x.B[first ? x.J[k].dense_b : x.J[k].dense_a].color == color;
+            });
             if (!conflict) break;
         }
         x.B[i].color = color;
@@ -1861,7 +2136,21 @@ void solve_body(Ctx &x, uint32_t i, uint32_t phase, std::vector<PointRef> &pts) 
         for (size_t k = 0; k < pts.size(); ++k) contact_point_rows(x, i, pts[k], phase, parts[k % x.lanes]);
         blk = tree_reduce(parts);
     }
-    // Joint rows would follow here (rigid_avbd_visit_joint): no joints.
+    // Joint rows (solve.slang:401-430): recomputed in the pose phase,
+    // stored hard rows in the velocity phase.
+    for_each_joint(x, i, [&](uint32_t k, bool first) {
+        PmoJoint &j = x.J[k];
+        if (j.state_enabled == 0u) return;
+        if (phase == 0u) {
+            visit_joint(x, j, blk, first, kAccumulate);
+            return;
+        }
+        const Vec6 da = disp(x.B[j.dense_a]), db = disp(x.B[j.dense_b]);
+        for (uint32_t r = 0; r < j.avbd_count; ++r) {
+            Row row = row_from(reinterpret_cast<const float *>(&j.rows[r]));
+            if (row.stiffness >= kMaxPenalty) row_accumulate(blk, row, first, da, db, x.dt);
+        }
+    });
     Vec6 update = zero6();
     if (!block_solve(blk, update)) return;
     bool crossing = false;
@@ -1928,6 +2217,33 @@ void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
             }
         }
     }
+    // Joints (solve.slang:566-614): dual updates, applied impulse, breaking.
+    for (uint32_t k = 0; k < x.nj; ++k) {
+        PmoJoint &j = x.J[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u) continue;
+        const uint32_t a = j.dense_a, b = j.dense_b;
+        if (phase != 0u && x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
+        float impulse = 0.0f;
+        if (phase == 0u) {
+            Block scratch = zero_block();
+            impulse = visit_joint(x, j, scratch, true, kAdvance);
+        } else {
+            const Vec6 da = disp(x.B[a]), db = disp(x.B[b]);
+            for (uint32_t r = 0; r < j.avbd_count; ++r) {
+                Row row = row_from(reinterpret_cast<const float *>(&j.rows[r]));
+                impulse += std::fabs(row.stiffness < kMaxPenalty ? row.reference_force : row_force(row, da, db, x.dt)) * x.dt;
+                if (row.stiffness >= kMaxPenalty) {
+                    row_advance(row, da, db);
+                    row_to(row, reinterpret_cast<float *>(&j.rows[r]));
+                }
+            }
+        }
+        j.applied_impulse = impulse;
+        if (j.options.breaking_impulse_threshold > 0.0f && impulse > j.options.breaking_impulse_threshold) {
+            j.state_broken = 1u;
+            j.state_enabled = 0u;
+        }
+    }
 }
 
 // free_mean / free_subtract / free_clear, :617-698, with each island's
@@ -1956,6 +2272,9 @@ void free_motion(Ctx &x, uint32_t phase) {
                 anchored = anchored || x.P[first ? b : a].motion != kDynamic;
                 active = first ? x.M[active].avbd_next_a : x.M[active].avbd_next_b;
             }
+            for_each_joint(x, i, [&](uint32_t k, bool first) {
+                anchored = anchored || x.P[first ? x.J[k].dense_b : x.J[k].dense_a].motion != kDynamic;
+            });
         }
         if (!anchored && total > 0.0f) {
             weighted = mul(weighted, 1.0f / total);
@@ -2019,7 +2338,14 @@ void impact_stages(Ctx &x) {
             x.B[i].orientation_delta = {0, 0, 0, 1};
         }
     }
-    for (uint32_t active = 0; active < x.np; ++active) {                // impact_constraints
+    for (uint32_t k = 0; k < x.nj; ++k) {                               // impact_constraints: joints
+        PmoJoint &j = x.J[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u) continue;
+        if (x.B[j.dense_a].impact == 0u && x.B[j.dense_b].impact == 0u) continue;
+        Block scratch = zero_block();
+        visit_joint(x, j, scratch, true, kImpact);
+    }
+    for (uint32_t active = 0; active < x.np; ++active) {                // impact_constraints: contacts
         uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
         if (x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
         Vec6 va = jac(mul(x.states[a].linear_velocity, x.dt), mul(x.states[a].angular_velocity, x.dt));
@@ -2039,6 +2365,27 @@ void impact_stages(Ctx &x) {
                             dot6(ja[1], va) + dot6(jb[1], vb),
                             dot6(ja[2], va) + dot6(jb[2], vb)};
         }
+    }
+}
+
+// rigid_avbd_project_hinge_anchors (solve.slang:174-209): after the pose
+// phase, enabled hinges move their bodies' anchors together, split by
+// inverse mass, in joint index order.
+void project_hinges(Ctx &x) {
+    for (uint32_t k = 0; k < x.nj; ++k) {
+        const PmoJoint &j = x.J[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u || j.options.type != kHinge) continue;
+        const uint32_t a = j.dense_a, b = j.dense_b;
+        const float ima = x.P[a].motion == kDynamic ? x.P[a].inverse_mass : 0.0f;
+        const float imb = x.P[b].motion == kDynamic ? x.P[b].inverse_mass : 0.0f;
+        const float sum = ima + imb;
+        if (sum <= kEpsilon) continue;
+        const PmoState sa = x.states[a], sb = x.states[b];
+        const V3 pa = add(sa.position, rotate(sa.orientation, j.options.local_anchor_a));
+        const V3 pb = add(sb.position, rotate(sb.orientation, j.options.local_anchor_b));
+        const V3 e = sub(pb, pa);
+        x.states[a].position = add(x.states[a].position, mul(e, ima / sum));
+        x.states[b].position = sub(x.states[b].position, mul(e, imb / sum));
     }
 }
 
@@ -2072,6 +2419,7 @@ int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t request
         }
         free_motion(x, phase);
         apply_states(x, phase, realtime);
+        if (phase == 0u) project_hinges(x);
         x.S->counts[phase == 0u ? 6u : 7u] = x.S->budget;
         x.S->maximum_passes += x.S->budget;
         if (clock) {
@@ -2122,6 +2470,7 @@ struct World {
     uint64_t epoch = 0;
     // One-step inputs (BodyAccumulator, KinematicTarget), cleared by every
     // step as pm_clear_rigid_inputs does (geometry.slang:2877-2886).
+    std::vector<PmoJoint> joints;
     std::vector<std::array<V3, 4>> accumulators;   // force, torque, impulse, angular impulse
     std::vector<PmoState> targets;
     std::vector<uint8_t> target_active;
@@ -2195,9 +2544,34 @@ inline bool overlap(const Proxy &a, const Proxy &b) {          // spatial_build.
 void find_pairs(World &w, const std::vector<PmoState> &pred) {
     const uint32_t n = uint32_t(w.params.size());
     w.pairs.clear();
+    // Collision exclusions of joints (geometry.slang:494-511, 3108-3136):
+    // enabled, unbroken joints with disable_collisions exclude their pair;
+    // fixed ones also join their bodies' weld group, whose members never
+    // collide.
+    std::vector<uint32_t> weld(n);
+    for (uint32_t i = 0; i < n; ++i) weld[i] = i;
+    auto weld_root = [&](uint32_t v) { while (weld[v] != v) v = weld[v]; return v; };
+    std::vector<uint64_t> disabled;
+    for (const PmoJoint &j : w.joints) {
+        const PmoJointOptions &o = j.options;
+        if (j.alive == 0u || o.enabled == 0u || j.state_broken != 0u || o.disable_collisions == 0u) continue;
+        const uint32_t a = o.body_a.index < n && o.body_a.generation == 1u ? o.body_a.index : kInvalid;
+        const uint32_t b = o.body_b.index < n && o.body_b.generation == 1u ? o.body_b.index : kInvalid;
+        if (a == kInvalid || b == kInvalid) continue;
+        disabled.push_back((uint64_t(std::min(a, b)) << 32) | std::max(a, b));
+        if (o.type != kFixed) continue;
+        const uint32_t ra = weld_root(a), rb = weld_root(b);
+        if (ra != rb) weld[std::max(ra, rb)] = std::min(ra, rb);
+    }
+    for (uint32_t i = 0; i < n; ++i) weld[i] = weld_root(i);
+    std::sort(disabled.begin(), disabled.end());
     auto keep = [&](uint32_t a, uint32_t b) {
         if (w.params[a].motion != kDynamic || a == b) return false;
         if (w.params[b].motion == kDynamic && !(a < b)) return false;
+        if (weld[a] == weld[b]) return false;
+        if (!disabled.empty() &&
+            std::binary_search(disabled.begin(), disabled.end(), (uint64_t(std::min(a, b)) << 32) | std::max(a, b)))
+            return false;
         return true;
     };
     if (w.desc.flags & 1u) {
@@ -2403,6 +2777,8 @@ void world_step(World &w) {
     if (w.bodies.size() != n) w.bodies.assign(n, PmoAvbdBody{});
     Ctx x{w.params.data(), prev.data(), pred.data(), n, w.manifolds.data(), w.pairs.data(),
           uint32_t(w.pairs.size()), w.bodies.data(), &w.schedule, dt, w.desc.lanes};
+    x.J = w.joints.data();
+    x.nj = uint32_t(w.joints.size());
     prepare_solve(x, g, w.desc.base_iterations, w.desc.requested_iterations, w.desc.profile_version, &w.clock);
     for (uint32_t i = 0; i < n; ++i) {                               // pm_clamp_rigid_speeds, :3086-3100
         if (w.params[i].motion != kDynamic) continue;
@@ -2582,6 +2958,28 @@ int pmo_world_set_kinematic_targets(void *world, uint32_t first, uint32_t count,
         w.target_active[i] = 1;
     }
     return 0;
+}
+
+int pmo_world_add_joints(void *world, uint32_t count, const PmoJointOptions *options) {
+    World &w = *static_cast<World *>(world);
+    for (uint32_t k = 0; k < count; ++k)
+        if (options[k].type > kMotor) return -1;
+    for (uint32_t k = 0; k < count; ++k) {
+        PmoJoint j;
+        std::memset(&j, 0, sizeof j);
+        j.options = options[k];
+        j.generation = 1u;
+        j.alive = 1u;
+        j.dense_a = j.dense_b = kInvalid;
+        j.avbd_next_a = j.avbd_next_b = kInvalid;
+        w.joints.push_back(j);
+    }
+    return 0;
+}
+uint32_t pmo_world_joint_count(void *world) { return uint32_t(static_cast<World *>(world)->joints.size()); }
+void pmo_world_get_joints(void *world, PmoJoint *out) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.joints.begin(), w.joints.end(), out);
 }
 
 int pmo_world_step(void *world, uint32_t steps) {

@@ -113,6 +113,46 @@ typedef struct PmoSchedule {
         maximum_passes, blocks, candidates, contacts;
 } PmoSchedule;
 
+/* ---- joints (geometry.slang:24-141) ----------------------------------
+ * RigidAvbdRow 88 B, RigidConstraintOptions 228 B, RigidConstraintResource
+ * 1852 B.  Joint types: 0 fixed, 1 point, 2 hinge, 3 slider, 4 piston,
+ * 5 generic, 6 generic spring, 7 motor.  Body ids are {dense index, 1}. */
+typedef struct PmoRow {
+    float a[6], b[6];
+    float lambda, penalty;
+    float error, velocity, lower, upper, stiffness, damping, beta,
+        reference_force;
+} PmoRow;
+typedef struct PmoBodyId { uint32_t index, generation; } PmoBodyId;
+typedef struct PmoJointLimits { uint32_t axes; PmoVec3 lower, upper; } PmoJointLimits;
+typedef struct PmoJointSprings { uint32_t axes; PmoVec3 stiffness, damping; } PmoJointSprings;
+typedef struct PmoJointMotor {
+    uint32_t linear_enabled, angular_enabled;
+    float linear_target_velocity, linear_maximum_impulse,
+        angular_target_velocity, angular_maximum_impulse;
+} PmoJointMotor;
+typedef struct PmoJointOptions {
+    uint32_t type;
+    PmoBodyId body_a, body_b;
+    PmoVec3 local_anchor_a, local_anchor_b;
+    PmoQuat local_orientation_a, local_orientation_b;
+    PmoJointLimits linear_limits, angular_limits;
+    PmoJointSprings linear_springs, angular_springs;
+    PmoJointMotor motor;
+    uint32_t enabled, disable_collisions;
+    float breaking_impulse_threshold;
+    uint32_t solver_iterations;
+} PmoJointOptions;
+typedef struct PmoJoint {
+    PmoJointOptions options;
+    uint32_t state_enabled, state_broken;
+    float applied_impulse;
+    uint32_t generation, alive;
+    uint32_t dense_a, dense_b;
+    PmoRow rows[18];
+    uint32_t avbd_count, avbd_next_a, avbd_next_b;
+} PmoJoint;
+
 /* ---- numerical core (constraint_math.slang:16-345) ------------------- */
 /* Row record, 22 floats: a[6] b[6] lambda penalty error velocity lower
  * upper stiffness damping beta reference_force. */
@@ -203,6 +243,11 @@ typedef struct PmoRigidInput {
  * bodies.  Both return 0 or -1 (invalid body, motion type or values). */
 int pmo_world_apply_inputs(void *world, uint32_t count, const PmoRigidInput *inputs);
 int pmo_world_set_kinematic_targets(void *world, uint32_t first, uint32_t count, const PmoState *targets);
+/* Joints between bodies of the world (alive, generation 1); returns 0 or
+ * -1 for an invalid type or body. */
+int pmo_world_add_joints(void *world, uint32_t count, const PmoJointOptions *options);
+uint32_t pmo_world_joint_count(void *world);
+void pmo_world_get_joints(void *world, PmoJoint *out);
 int pmo_world_step(void *world, uint32_t steps);
 uint32_t pmo_world_body_count(void *world);
 /* Cumulative wall time of world steps, of AVBD prepare and of the AVBD

@@ -84,3 +84,56 @@ SCHEDULE = np.dtype([
 assert SCHEDULE.itemsize == 264
 
 INVALID = 0xFFFFFFFF
+
+# Joints (geometry.slang:24-141).
+JOINT_FIXED, JOINT_POINT, JOINT_HINGE, JOINT_SLIDER, JOINT_PISTON = 0, 1, 2, 3, 4
+JOINT_GENERIC, JOINT_GENERIC_SPRING, JOINT_MOTOR = 5, 6, 7
+
+AVBD_ROW = np.dtype([                          # RigidAvbdRow, 88 bytes
+    ("a", "<f4", 6), ("b", "<f4", 6), ("lambda", "<f4"), ("penalty", "<f4"),
+    ("error", "<f4"), ("velocity", "<f4"), ("lower", "<f4"), ("upper", "<f4"),
+    ("stiffness", "<f4"), ("damping", "<f4"), ("beta", "<f4"), ("reference_force", "<f4")])
+
+_LIMITS = [("axes", "<u4"), ("lower", "<f4", 3), ("upper", "<f4", 3)]
+_SPRINGS = [("axes", "<u4"), ("stiffness", "<f4", 3), ("damping", "<f4", 3)]
+
+JOINT_OPTIONS = np.dtype([                     # RigidConstraintOptions, 228 bytes
+    ("type", "<u4"), ("body_a", "<u4", 2), ("body_b", "<u4", 2),
+    ("local_anchor_a", "<f4", 3), ("local_anchor_b", "<f4", 3),
+    ("local_orientation_a", "<f4", 4), ("local_orientation_b", "<f4", 4),
+    ("linear_limits", _LIMITS), ("angular_limits", _LIMITS),
+    ("linear_springs", _SPRINGS), ("angular_springs", _SPRINGS),
+    ("motor", [("linear_enabled", "<u4"), ("angular_enabled", "<u4"),
+               ("linear_target_velocity", "<f4"), ("linear_maximum_impulse", "<f4"),
+               ("angular_target_velocity", "<f4"), ("angular_maximum_impulse", "<f4")]),
+    ("enabled", "<u4"), ("disable_collisions", "<u4"),
+    ("breaking_impulse_threshold", "<f4"), ("solver_iterations", "<u4")])
+
+JOINT = np.dtype([                             # RigidConstraintResource, 1852 bytes
+    ("options", JOINT_OPTIONS), ("enabled", "<u4"), ("broken", "<u4"),
+    ("applied_impulse", "<f4"), ("generation", "<u4"), ("alive", "<u4"),
+    ("dense_a", "<u4"), ("dense_b", "<u4"), ("rows", AVBD_ROW, 18),
+    ("avbd_count", "<u4"), ("avbd_next_a", "<u4"), ("avbd_next_b", "<u4")])
+
+assert JOINT_OPTIONS.itemsize == 228 and JOINT.itemsize == 1852 and AVBD_ROW.itemsize == 88
+
+
+def joint_options(n, kind, a, b, anchor_a=(0, 0, 0), anchor_b=(0, 0, 0), disable_collisions=1,
+                  breaking=0.0, iterations=0):
+    """Joint options between dense bodies a and b (identity local frames)."""
+    o = np.zeros(n, dtype=JOINT_OPTIONS)
+    o["type"] = kind
+    o["body_a"][:, 0] = a
+    o["body_a"][:, 1] = 1
+    o["body_b"][:, 0] = b
+    o["body_b"][:, 1] = 1
+    o["local_anchor_a"] = anchor_a
+    o["local_anchor_b"] = anchor_b
+    o["local_orientation_a"] = (0, 0, 0, 1)
+    o["local_orientation_b"] = (0, 0, 0, 1)
+    o["enabled"] = 1
+    o["disable_collisions"] = disable_collisions
+    o["breaking_impulse_threshold"] = breaking
+    o["solver_iterations"] = iterations
+    return o
+

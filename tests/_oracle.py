@@ -59,6 +59,10 @@ def lib():
         _lib.pmo_world_add_bodies.argtypes = [_P, _U, _P, _P]
         _lib.pmo_world_step.argtypes = [_P, _U]
         _lib.pmo_world_apply_inputs.argtypes = [_P, _U, _P]
+        _lib.pmo_world_add_joints.argtypes = [_P, _U, _P]
+        _lib.pmo_world_joint_count.argtypes = [_P]
+        _lib.pmo_world_joint_count.restype = _U
+        _lib.pmo_world_get_joints.argtypes = [_P, _P]
         _lib.pmo_world_set_kinematic_targets.argtypes = [_P, _U, _U, _P]
         _lib.pmo_world_body_count.argtypes = [_P]
         _lib.pmo_world_body_count.restype = _U
@@ -144,6 +148,17 @@ class OracleWorld:
         s = np.ascontiguousarray(states, dtype=L.STATE)
         if lib().pmo_world_add_bodies(self._h, len(p), _ptr(p), _ptr(s)) != 0:
             raise ValueError("bodies rejected")
+
+    def add_joints(self, options):
+        a = np.ascontiguousarray(options, dtype=L.JOINT_OPTIONS)
+        if lib().pmo_world_add_joints(self._h, len(a), _ptr(a)) != 0:
+            raise ValueError("invalid joint options")
+
+    def joints(self):
+        out = np.zeros(lib().pmo_world_joint_count(self._h), dtype=L.JOINT)
+        if len(out):
+            lib().pmo_world_get_joints(self._h, _ptr(out))
+        return out
 
     def apply_inputs(self, inputs):
         a = np.ascontiguousarray(inputs, dtype=L.RIGID_INPUT)
