@@ -149,7 +149,7 @@ def main():
     else:
         per_launch_bytes = kt["algorithmic_bytes"] / kt["launches"]
         per_launch_s = kt["milliseconds"] / 1e3 / kt["launches"]
-        kernel = f"k_stage_color<{args.lanes or 32}> (colour sweep, one launch per colour per iteration)"
+        kernel = f"k_stage_color<{args.lanes or 32}, false> (colour sweep, one launch per colour per iteration)"
     achieved = per_launch_bytes / per_launch_s / 1e9
     traffic = traffic_from_profiles()
     name, cus = pmamd.device_info()

@@ -205,6 +205,56 @@ uint32_t pmhWorldBodyCount(PmhWorld world);
  * contact cache, AVBD prepare and solve, speed clamp, cache save
  * (production_realtime_backend.cu.inc:2642-2687, 7664-7861).  Enqueues
  * `steps` steps and waits for them. */
+/* ---- joints ---------------------------------------------------------------
+ * RigidAvbdRow, RigidConstraintOptions and RigidConstraintResource
+ * (src/slang/geometry.slang:24-141; 88, 228 and 1852 bytes).  Types:
+ * 0 fixed, 1 point, 2 hinge, 3 slider, 4 piston, 5 generic, 6 generic
+ * spring, 7 motor.  A body id is {dense index, 1} in a world. */
+typedef struct PmhAvbdRow {
+    float a[6], b[6];
+    float lambda, penalty;
+    float error, velocity, lower, upper, stiffness, damping, beta, reference_force;
+} PmhAvbdRow;
+typedef struct PmhBodyId { uint32_t index, generation; } PmhBodyId;
+typedef struct PmhJointLimits { uint32_t axes; PmhVec3 lower, upper; } PmhJointLimits;
+typedef struct PmhJointSprings { uint32_t axes; PmhVec3 stiffness, damping; } PmhJointSprings;
+typedef struct PmhJointMotor {
+    uint32_t linear_enabled, angular_enabled;
+    float linear_target_velocity, linear_maximum_impulse;
+    float angular_target_velocity, angular_maximum_impulse;
+} PmhJointMotor;
+typedef struct PmhRigidConstraintOptions {
+    uint32_t type;
+    PmhBodyId body_a, body_b;
+    PmhVec3 local_anchor_a, local_anchor_b;
+    PmhQuat local_orientation_a, local_orientation_b;
+    PmhJointLimits linear_limits, angular_limits;
+    PmhJointSprings linear_springs, angular_springs;
+    PmhJointMotor motor;
+    uint32_t enabled, disable_collisions;
+    float breaking_impulse_threshold;
+    uint32_t solver_iterations;
+} PmhRigidConstraintOptions;
+typedef struct PmhRigidConstraint {
+    PmhRigidConstraintOptions options;
+    uint32_t state_enabled, state_broken;
+    float applied_impulse;
+    uint32_t generation, alive;
+    uint32_t dense_a, dense_b;
+    PmhAvbdRow rows[18];
+    uint32_t avbd_count, avbd_next_a, avbd_next_b;
+} PmhRigidConstraint;
+
+/* pmCmdWriteRigidConstraints (src/api/resources.cpp:998) -> the runtime's
+ * constraint resources: appends joints (state, rows and lists start
+ * empty).  PM_ERROR_INVALID_ARGUMENT for an unknown type. */
+PmhResult pmhWorldAddJoints(PmhWorld world, uint32_t count, const PmhRigidConstraintOptions *options);
+uint32_t pmhWorldJointCount(PmhWorld world);
+/* The joint records after the last step (pmCmdCopyConstraintStates,
+ * src/api/domains.cpp:1585):
+ * enabled/broken state, applied impulse, dense ids, rows and list links. */
+PmhResult pmhWorldGetJoints(PmhWorld world, PmhRigidConstraint *joints);
+
 /* PmRigidInput (include/parallel_mater/parallel_mater.h:421-427). */
 typedef struct PmhRigidInput {
     uint32_t body_index;

@@ -43,6 +43,21 @@ __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Sum over a converged wavefront (every lane active).
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Adds each lane's value to one counter with one atomic per wavefront
+// (every lane active).  Same-address atomics from every lane serialise in
+// one L2 channel; the counters here are read only after the kernel.
+__device__ __forceinline__ void wave_counter_add(uint32_t *counter, uint32_t v) {
+    const uint32_t t = wave_total(v);
+    if ((threadIdx.x & 63) == 0 && t) atomicAdd(counter, t);
+}
+
 __device__ __forceinline__ V3 v3(float4 f) { return {f.x, f.y, f.z}; }
 __device__ __forceinline__ Q q4(float4 f) { return {f.x, f.y, f.z, f.w}; }
 __device__ __forceinline__ float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
@@ -162,10 +177,12 @@ __global__ void k_body_init(uint32_t n, const PmhBodyParameters *P, const PmhRig
                             float4 *tv, float4 *tw, float4 *disp, float4 *dq, float4 *inertia,
                             float4 *prev_vel, uint32_t *color, uint32_t *parent, uint32_t *impact,
                             uint32_t *deg, uint32_t *anchored, uint32_t *adj_cursor, uint32_t *members,
-                            float4 *mean) {
+                            float4 *mean, uint32_t *jdeg, uint32_t *jcursor) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     members[i] = 0;
+    jdeg[i] = 0;
+    jcursor[i] = 0;
     mean[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const PmhRigidBodyState s = st[i];
     tpos[i] = f4(sv(s.position), 0.0f);
@@ -301,11 +318,11 @@ __global__ void k_degree(const uint32_t *npairs, const PmhBodyParameters *P, con
         if (db) atomicAdd(&deg[p.y], 1u);
         if (da && db) { ++m2; p2 += m_count[m]; } else { ++m1; p1 += m_count[m]; }
     }
-    if (local) {
-        atomicAdd(&ctr->manifolds, local);
-        atomicAdd(&ctr->m1, m1); atomicAdd(&ctr->m2, m2);
-        atomicAdd(&ctr->p1, p1); atomicAdd(&ctr->p2, p2);
-    }
+    wave_counter_add(&ctr->manifolds, local);
+    wave_counter_add(&ctr->m1, m1);
+    wave_counter_add(&ctr->m2, m2);
+    wave_counter_add(&ctr->p1, p1);
+    wave_counter_add(&ctr->p2, p2);
     (void)anchored_edge;
 }
 
@@ -334,9 +351,19 @@ __global__ void k_adj_sort(uint32_t n, const uint32_t *adj_start, uint32_t *adj)
     }
 }
 
+// Root of v with path halving: a non-root's parent is replaced by its
+// grandparent.  Parents only ever decrease and a non-root never becomes a
+// root again, so the write cannot undo a hook (hooks CAS roots only) and
+// every pointer still leads to the root; a racing halving may leave a
+// longer path, never a wrong one.
 __device__ uint32_t uf_find(uint32_t *parent, uint32_t v) {
     uint32_t p = ld_relaxed(&parent[v]);
-    while (p != v) { v = p; p = ld_relaxed(&parent[v]); }
+    while (p != v) {
+        const uint32_t g = ld_relaxed(&parent[p]);
+        if (g != p) st_relaxed(&parent[v], g);
+        v = p;
+        p = g;
+    }
     return v;
 }
 
@@ -375,16 +402,19 @@ __global__ void k_impact_roots(const uint32_t *npairs, const PmhBodyParameters *
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         const uint2 p = ab[m];
         const bool da = P[p.x].motion == kDynamic, db = P[p.y].motion == kDynamic;
+        // Flags are read before the atomic: most islands are flagged by
+        // their first pairs, and a pile's one large island would otherwise
+        // take one same-address atomic per pair.
         if (m_impact[m]) {
-            if (da) atomicOr(&impact[parent[p.x]], 1u);
-            if (db) atomicOr(&impact[parent[p.y]], 1u);
+            if (da && ld_relaxed(&impact[parent[p.x]]) == 0u) atomicOr(&impact[parent[p.x]], 1u);
+            if (db && ld_relaxed(&impact[parent[p.y]]) == 0u) atomicOr(&impact[parent[p.y]], 1u);
             ctr->impact = 1u;
         }
         // An island touching any non-dynamic body keeps its momentum
         // correction (free_mean's `anchored`, solve.slang:641-652).
         if (m_count[m] != 0) {
-            if (da && !db) atomicOr(&anchored[parent[p.x]], 1u);
-            if (db && !da) atomicOr(&anchored[parent[p.y]], 1u);
+            if (da && !db && ld_relaxed(&anchored[parent[p.x]]) == 0u) atomicOr(&anchored[parent[p.x]], 1u);
+            if (db && !da && ld_relaxed(&anchored[parent[p.y]]) == 0u) atomicOr(&anchored[parent[p.y]], 1u);
         }
     }
 }
@@ -392,21 +422,43 @@ __global__ void k_impact_roots(const uint32_t *npairs, const PmhBodyParameters *
 // Also counts each island's dynamic members and lists the roots whose
 // islands take the free-motion mean correction (unanchored, with contacts).
 __global__ void k_impact_bodies(uint32_t n, const PmhBodyParameters *P, const uint32_t *parent, uint32_t *impact,
-                                const uint32_t *deg, const uint32_t *anchored, uint32_t *members,
-                                uint32_t *mean_list, StepCounters *ctr) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (P[i].motion != kDynamic) { impact[i] = 0; return; }
-    const uint32_t r = parent[i];
-    const uint32_t flag = ld_relaxed(&impact[r]);
-    if (r != i) impact[i] = flag;
-    if (r == i && deg[i] != 0) {
-        atomicAdd(&ctr->islands, 1u);
-        if (anchored[i] == 0) mean_list[atomicAdd(&ctr->mean_roots, 1u)] = i;
+                                const uint32_t *deg, const uint32_t *jdeg, const uint32_t *anchored,
+                                uint32_t *members, uint32_t *mean_list, StepCounters *ctr) {
+    // Every lane stays active to the end: the counters are wave-aggregated.
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const bool dyn = i < n && P[i].motion == kDynamic;
+    if (i < n && !dyn) impact[i] = 0;
+    uint32_t r = kInvalid, flag = 0;
+    bool root_listed = false, mean_root = false;
+    if (dyn) {
+        r = parent[i];
+        flag = ld_relaxed(&impact[r]);
+        if (r != i) impact[i] = flag;
+        root_listed = r == i && (deg[i] != 0 || jdeg[i] != 0);
+        mean_root = root_listed && anchored[i] == 0;
     }
-    atomicAdd(&members[r], 1u);
-    atomicAdd(&ctr->dyn_bodies, 1u);
-    if (flag) atomicAdd(&ctr->impact_bodies, 1u);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t roots = __ballot(root_listed), means = __ballot(mean_root);
+    const uint64_t dyns = __ballot(dyn), flagged = __ballot(dyn && flag != 0);
+    uint32_t mean_base = 0;
+    if (lane == 0) {
+        if (roots) atomicAdd(&ctr->islands, uint32_t(__popcll(roots)));
+        if (means) mean_base = atomicAdd(&ctr->mean_roots, uint32_t(__popcll(means)));
+        if (dyns) atomicAdd(&ctr->dyn_bodies, uint32_t(__popcll(dyns)));
+        if (flagged) atomicAdd(&ctr->impact_bodies, uint32_t(__popcll(flagged)));
+    }
+    mean_base = __shfl(mean_base, 0, 64);
+    if (mean_root) mean_list[mean_base + __popcll(means & below)] = i;
+    // Island member counts: lanes sharing the first dynamic lane's root
+    // add together (a large island covers whole wavefronts).
+    if (dyns) {
+        const uint32_t lead = __builtin_ctzll(dyns);
+        const uint32_t r0 = __shfl(r, lead, 64);
+        const uint64_t same = __ballot(dyn && r == r0);
+        if (lane == lead) atomicAdd(&members[r0], uint32_t(__popcll(same)));
+        if (dyn && r != r0) atomicAdd(&members[r], 1u);
+    }
 }
 
 // Velocity-phase work: non-empty manifolds touching an impact body.
@@ -421,10 +473,10 @@ __global__ void k_count_impact_pairs(const uint32_t *npairs, const PmhBodyParame
         const bool da = P[p.x].motion == kDynamic, db = P[p.y].motion == kDynamic;
         if (da && db) { ++m2; p2 += m_count[m]; } else { ++m1; p1 += m_count[m]; }
     }
-    if (m1 + m2) {
-        atomicAdd(&ctr->m1i, m1); atomicAdd(&ctr->m2i, m2);
-        atomicAdd(&ctr->p1i, p1); atomicAdd(&ctr->p2i, p2);
-    }
+    wave_counter_add(&ctr->m1i, m1);
+    wave_counter_add(&ctr->m2i, m2);
+    wave_counter_add(&ctr->p1i, p1);
+    wave_counter_add(&ctr->p2i, p2);
 }
 
 // Algorithmic bytes of one AVBD iteration: per dynamic body 136 B
@@ -444,7 +496,8 @@ __device__ __forceinline__ double iteration_bytes(uint32_t d, uint32_t m1, uint3
 // colour: the wait chain always ends.
 __global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
                                                const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
-                                               StepCounters *ctr) {
+                                               const uint32_t *jstart, const uint32_t *jadj,
+                                               const PmhRigidConstraint *joints, StepCounters *ctr) {
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         uint32_t t = 0;
@@ -467,6 +520,18 @@ __global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParamete
             if (c == kInvalid) { atomicOr(&ctr->status, kStatusColorTimeout); continue; }
             if (c < kMaxColors) used[c >> 6] |= 1ull << (c & 63);
         }
+        if (joints) {
+            for (uint32_t k = jstart[i] + lane; k < jstart[i + 1]; k += 64) {
+                const PmhRigidConstraint &jt = joints[jadj[k]];
+                const uint32_t j = jt.dense_a == i ? jt.dense_b : jt.dense_a;
+                if (j >= i || P[j].motion != kDynamic) continue;
+                uint32_t c;
+                uint32_t spins = 0;
+                while ((c = ld_relaxed(&color[j])) == kInvalid && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
+                if (c == kInvalid) { atomicOr(&ctr->status, kStatusColorTimeout); continue; }
+                if (c < kMaxColors) used[c >> 6] |= 1ull << (c & 63);
+            }
+        }
         uint32_t chosen = kMaxColors;
 #pragma unroll
         for (uint32_t w = 0; w < kMaxColors / 64; ++w) {
@@ -478,7 +543,7 @@ __global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParamete
         if (lane == 0) {
             if (chosen >= kMaxColors) { atomicOr(&ctr->status, kStatusColorOverflow); chosen = kMaxColors - 1; }
             st_relaxed(&color[i], chosen);
-            atomicMax(&ctr->colors, chosen + 1);
+            if (ld_relaxed(&ctr->colors) < chosen + 1) atomicMax(&ctr->colors, chosen + 1);
         }
     }
 }
@@ -630,6 +695,11 @@ struct SolveArgs {
     const uint32_t *npts, *npairs;
     float dt;
     uint32_t budget, realtime;
+    // joints
+    PmhRigidConstraint *joints;
+    uint32_t nj;
+    const uint32_t *jstart, *jadj;       // per-body incident joints, descending index
+    const PmhRigidBodyState *prev;
 };
 
 __device__ __forceinline__ void load_disp(const float4 *disp, uint32_t i, float *d) {
@@ -645,6 +715,8 @@ __device__ __forceinline__ float row_error(float e0, float4 axis, float4 angA, f
     const float jb[6] = {axis.x, axis.y, axis.z, angB.x, angB.y, angB.z};
     return e0 + dot6(ja, dA) + dot6(jb, dB);
 }
+
+__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk);
 
 template <int L>
 __device__ __forceinline__ void group_reduce(Block &b) {
@@ -676,7 +748,9 @@ __device__ __forceinline__ void apply_pose_update(float4 *disp, float4 *dqa, uin
 // One colour sweep for the bodies of `color` (solve_color,
 // geometry_constraints_solve.slang:331-491).  `gw`/`nw` index the
 // wavefront within the launch.
-template <int L>
+// kJoints: the world has joints.  Joint-free worlds run the false
+// instance, which carries none of the joint code (registers, stack).
+template <int L, bool kJoints>
 __device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, uint32_t gw, uint32_t nw) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t grp = lane / L, sub_lane = lane % L;
@@ -748,6 +822,9 @@ __device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, 
             }
         }
         group_reduce<L>(blk);
+        // Joint rows follow the contact rows (solve.slang:401-430); every
+        // lane of the group adds the same rows to its copy of the block.
+        if (kJoints && active) body_joint_rows(a, i, phase, blk);
         float update[6];
         const bool solved = active && block_solve(blk, update);
         // Crossing safeguard (solve.slang:433-475).
@@ -818,6 +895,337 @@ __device__ __forceinline__ PmhRigidBodyState avbd_pose(const SolveArgs &a, uint3
     if (dq.x != 0.0f || dq.y != 0.0f || dq.z != 0.0f) o = qnormalize(qmul(dq, o));
     s.orientation = pq(o);
     return s;
+}
+
+// ---------------------------------------------------------------------
+// Joints: rigid_avbd_visit_joint(_row), geometry_constraints.slang:
+// 289-445, 626-926.  One routine builds a joint's rows for the four visits
+// (initialise in prepare, accumulate in the colour sweep, advance in the
+// dual update, the velocity-phase rows).  Same operations and order as the
+// reference.
+// ---------------------------------------------------------------------
+enum : uint32_t { kVisitInitialize = 0, kVisitAccumulate = 1, kVisitAdvance = 2, kVisitImpact = 3 };
+constexpr float kLimitStabilization = 0.8f;
+
+__device__ __forceinline__ float aabs_(float v) { return v < 0.0f ? -v : v; }
+
+__device__ __forceinline__ float jrow_trial(const PmhAvbdRow &r, const float *da, const float *db, float dt) {
+    const float d = dot6(r.a, da) + dot6(r.b, db);
+    return r.lambda + r.penalty * (r.error + d) + r.damping * (r.velocity + d / dt);
+}
+__device__ __forceinline__ float jrow_force(const PmhAvbdRow &r, const float *da, const float *db, float dt) {
+    return aclamp(jrow_trial(r, da, db, dt), r.lower, r.upper);
+}
+__device__ __forceinline__ void jrow_accumulate(Block &blk, const PmhAvbdRow &r, bool first, const float *da,
+                                                const float *db, float dt) {
+    const float trial = jrow_trial(r, da, db, dt);
+    const bool two_sided = r.lower < 0.0f && r.upper > 0.0f;
+    const float k = two_sided || (trial > r.lower && trial < r.upper) ? r.penalty + r.damping / dt : 0.0f;
+    add_row(blk, first ? r.a : r.b, k, aclamp(trial, r.lower, r.upper) - r.reference_force);
+}
+__device__ __forceinline__ void jrow_advance(PmhAvbdRow &r, const float *da, const float *db) {
+    const float e = r.error + dot6(r.a, da) + dot6(r.b, db);
+    const float u = r.lambda + r.penalty * e;
+    r.lambda = r.stiffness < kMaxPenalty ? 0.0f : aclamp(u, r.lower, r.upper);
+    if (u > r.lower && u < r.upper) r.penalty = amin(r.stiffness, amin(kMaxPenalty, r.penalty + r.beta * aabs_(e)));
+}
+
+__device__ __forceinline__ V3 iiw(const PmhBodyParameters &p, Q o, V3 w) {
+    const V3 l = inv_rotate(o, w);
+    return rotate(o, {l.x * p.inverse_inertia_local.x, l.y * p.inverse_inertia_local.y,
+                      l.z * p.inverse_inertia_local.z});
+}
+
+// rigid_avbd_log_axis, :401-422.
+__device__ V3 log_axis(Q frame, V3 lg, uint32_t axis) {
+    const float sq2 = len2(lg);
+    float coef = 1.0f / 12.0f + sq2 * (1.0f / 720.0f + sq2 * (1.0f / 30240.0f + sq2 * (1.0f / 1209600.0f +
+                 sq2 / 47900160.0f)));
+    if (sq2 > 1.0f) {
+        const float angle = sqrtf(sq2);
+        const float half = 0.5f * angle;
+        coef = (1.0f - half * cosf(half) / sinf(half)) / sq2;
+    }
+    const V3 unit = basis_axis(axis);
+    const V3 local = add(add(unit, mul(cross(lg, unit), 0.5f)), mul(cross(lg, cross(lg, unit)), coef));
+    return rotate(frame, local);
+}
+
+__device__ __forceinline__ V3 rel_rotation_vector(Q a, Q b) {
+    return quaternion_rotation_vector(relative_quaternion(a, b, true));
+}
+
+struct JointRowArgs {
+    float ja[6], jb[6];
+    float current, previous, penalty;
+    bool angular;
+    float stiffness, damping, lower, upper;
+    bool motor;
+    float motor_speed;
+    V3 curv_a, curv_b;
+    float stabilization;
+};
+
+__device__ __forceinline__ void set6(float *d, V3 l, V3 r) {
+    d[0] = l.x; d[1] = l.y; d[2] = l.z; d[3] = r.x; d[4] = r.y; d[5] = r.z;
+}
+
+// rigid_avbd_visit_joint_row, :626-713.
+__device__ void visit_joint_row(const SolveArgs &a, PmhRigidConstraint &j, Block &blk, uint32_t &index,
+                                float &impulse, bool first, const JointRowArgs &r, uint32_t visit) {
+    if (index >= 18u) return;
+    const uint32_t ri = index++;
+    PmhAvbdRow row = j.rows[ri];
+    const uint32_t ia = j.dense_a, ib = j.dense_b;
+    float da[6], db[6];
+    load_disp(a.disp, ia, da);
+    load_disp(a.disp, ib, db);
+    const float linearized = dot6(r.ja, da) + dot6(r.jb, db);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) { row.a[k] = r.ja[k]; row.b[k] = r.jb[k]; }
+    row.stiffness = r.stiffness;
+    row.damping = r.damping;
+    if (visit == kVisitInitialize) {
+        if (!(row.penalty > kMinPenalty)) row.penalty = r.penalty;
+        warm_start(row.lambda, row.penalty, r.stiffness);
+        if (r.stiffness < kMaxPenalty) row.lambda = 0.0f;
+        row.beta = r.penalty * (r.angular ? 10.0f : 100.0f);
+        row.reference_force = 0.0f;
+    }
+    row.error = r.current - (r.stiffness < kMaxPenalty ? 0.0f : r.stabilization * r.previous) - linearized;
+    row.velocity = r.motor ? r.motor_speed : (r.current - r.previous - linearized) / a.dt;
+    row.lower = r.lower;
+    row.upper = r.upper;
+    if (visit == kVisitAccumulate) {
+        jrow_accumulate(blk, row, first, da, db, a.dt);
+        if (!r.angular) {
+            const V3 axis = {r.jb[0], r.jb[1], r.jb[2]};
+            const V3 arm = first ? r.curv_a : r.curv_b;
+            const float force = jrow_force(row, da, db, a.dt);
+            for (uint32_t c = 0; c < 3; ++c) {
+                const V3 value = sub(mul(add(mul(axis, comp(arm, c)), mul(arm, comp(axis, c))), 0.5f),
+                                     mul(basis_axis(c), dot(axis, arm)));
+                const uint32_t d = 3 + c;
+                blk.h[d * (d + 1) / 2 + d] += fabsf(force) * sqrtf(len2(value));
+            }
+        }
+        return;
+    }
+    if (visit == kVisitAdvance) {
+        impulse += fabsf(jrow_force(row, da, db, a.dt)) * a.dt;
+        jrow_advance(row, da, db);
+    } else if (visit == kVisitImpact) {
+        if (r.stiffness < kMaxPenalty) {
+            row.reference_force = jrow_force(row, da, db, a.dt);
+        } else {
+            const float gap = r.lower == 0.0f ? smin(0.0f, r.current) : (r.upper == 0.0f ? smax(0.0f, r.current) : 0.0f);
+            float va[6], vb[6];
+            set6(va, mul(v3(a.tv[ia]), a.dt), mul(v3(a.tw[ia]), a.dt));
+            set6(vb, mul(v3(a.tv[ib]), a.dt), mul(v3(a.tw[ib]), a.dt));
+            row.reference_force = row.lambda;
+            row.error = dot6(r.ja, va) + dot6(r.jb, vb) + gap - (r.motor ? r.motor_speed * a.dt : 0.0f);
+        }
+    }
+    j.rows[ri] = row;
+}
+
+// rigid_avbd_visit_joint, :715-926.
+__device__ __noinline__ float visit_joint(const SolveArgs &a, PmhRigidConstraint &j, Block &blk, bool first,
+                                          uint32_t visit) {
+    uint32_t index = 0;
+    float impulse = 0.0f;
+    const PmhRigidConstraintOptions &o = j.options;
+    const uint32_t ia = j.dense_a, ib = j.dense_b;
+    const PmhRigidBodyState sa = avbd_pose(a, ia), sb = avbd_pose(a, ib);
+    const PmhRigidBodyState pa = a.prev[ia], pb = a.prev[ib];
+    const Q qa = sq(sa.orientation), qb = sq(sb.orientation);
+    const V3 armA = rotate(qa, sv(o.local_anchor_a)), armB = rotate(qb, sv(o.local_anchor_b));
+    const V3 error = sub(add(sv(sb.position), armB), add(sv(sa.position), armA));
+    const V3 old_error = sub(add(sv(pb.position), rotate(sq(pb.orientation), sv(o.local_anchor_b))),
+                             add(sv(pa.position), rotate(sq(pa.orientation), sv(o.local_anchor_a))));
+    const Q fA = qnormalize(qmul(qa, sq(o.local_orientation_a)));
+    const Q fB = qnormalize(qmul(qb, sq(o.local_orientation_b)));
+    const Q ofA = qnormalize(qmul(sq(pa.orientation), sq(o.local_orientation_a)));
+    const Q ofB = qnormalize(qmul(sq(pb.orientation), sq(o.local_orientation_b)));
+    const V3 twist = rel_rotation_vector(fA, fB);
+    const V3 old_twist = rel_rotation_vector(ofA, ofB);
+    const uint32_t type = o.type;
+    const bool swing = type == kJointHinge || type == kJointPiston ||
+                       (type == kJointMotor && o.motor.angular_enabled != 0u);
+    const V3 free_axis = type == kJointHinge ? V3{0.0f, 0.0f, 1.0f} : V3{1.0f, 0.0f, 0.0f};
+    const V3 swing_axis = inv_rotate(fA, rotate(fB, free_axis));
+    const V3 old_swing_axis = inv_rotate(ofA, rotate(ofB, free_axis));
+    const V3 rotation = swing ? cross(free_axis, swing_axis) : twist;
+    const V3 old_rotation = swing ? cross(free_axis, old_swing_axis) : old_twist;
+    const PmhBodyParameters Pa = a.P[ia], Pb = a.P[ib];
+    const float linear_penalty = pair_stiffness(Pa, Pb, a.dt);
+    const bool generic = type == kJointGeneric || type == kJointGenericSpring;
+    const bool motor_type = type == kJointMotor;
+    const V3 zero = {0.0f, 0.0f, 0.0f};
+    for (uint32_t axis = 0; axis < 3; ++axis) {
+        const V3 direction = rotate(fA, basis_axis(axis));
+        const V3 old_direction = rotate(ofA, basis_axis(axis));
+        for (uint32_t av = 0; av < 2; ++av) {
+            const bool angular = av != 0;
+            const bool lock = angular
+                ? type == kJointFixed || type == kJointSlider || (type == kJointHinge && axis != 2u) ||
+                      (type == kJointPiston && axis != 0u) ||
+                      (motor_type && (axis != 0u || o.motor.angular_enabled == 0u))
+                : type == kJointFixed || type == kJointPoint || type == kJointHinge ||
+                      ((type == kJointSlider || type == kJointPiston) && axis != 0u) ||
+                      (motor_type && (axis != 0u || o.motor.linear_enabled == 0u));
+            const PmhJointLimits &limits = angular ? o.angular_limits : o.linear_limits;
+            const PmhJointSprings &springs = angular ? o.angular_springs : o.linear_springs;
+            const float current = angular ? comp(rotation, axis) : dot(error, direction);
+            const float old_current = angular ? comp(old_rotation, axis) : dot(old_error, old_direction);
+            const V3 angular_axis = swing
+                ? rotate(fA, sub(mul(basis_axis(axis), dot(free_axis, swing_axis)),
+                                 mul(free_axis, comp(swing_axis, axis))))
+                : log_axis(fA, twist, axis);
+            JointRowArgs r;
+            if (angular) {
+                set6(r.ja, zero, mul(angular_axis, -1.0f));
+                set6(r.jb, zero, angular_axis);
+            } else {
+                set6(r.ja, mul(direction, -1.0f), add(mul(cross(armA, direction), -1.0f), cross(direction, error)));
+                set6(r.jb, direction, cross(armB, direction));
+            }
+            const float penalty = angular
+                ? 1.0f / (smax(dot(direction, add(iiw(Pa, qa, direction), iiw(Pb, qb, direction))), 1.0e-6f) *
+                          a.dt * a.dt)
+                : linear_penalty;
+            const float lock_penalty = angular
+                ? smax(penalty, 2.0f * linear_penalty * smax(len2(cross(direction, armA)), len2(cross(direction, armB))))
+                : penalty;
+            r.angular = angular;
+            r.motor = false;
+            r.motor_speed = 0.0f;
+            r.curv_a = add(error, armA);
+            r.curv_b = armB;
+            if (lock) {
+                if (visit == kVisitInitialize && angular && index < 18u)
+                    j.rows[index].penalty = smax(j.rows[index].penalty, lock_penalty / kGamma);
+                r.current = current; r.previous = old_current; r.penalty = lock_penalty;
+                r.stiffness = kMaxPenalty; r.damping = 0.0f; r.lower = -kMaxPenalty; r.upper = kMaxPenalty;
+                r.stabilization = kAlpha;
+                visit_joint_row(a, j, blk, index, impulse, first, r, visit);
+            }
+            const bool free_limit = angular ? (type == kJointHinge && axis == 2u) || (type == kJointPiston && axis == 0u)
+                                            : (type == kJointSlider || type == kJointPiston) && axis == 0u;
+            if ((generic || free_limit) && (limits.axes & (1u << axis)) != 0u) {
+                JointRowArgs l = r;
+                const float limit_current = angular ? comp(twist, axis) : current;
+                const float old_limit_current = angular ? comp(old_twist, axis) : old_current;
+                if (angular) {
+                    const V3 limit_axis = log_axis(fA, twist, axis);
+                    set6(l.ja, zero, mul(limit_axis, -1.0f));
+                    set6(l.jb, zero, limit_axis);
+                }
+                const float lower = comp(sv(limits.lower), axis), upper = comp(sv(limits.upper), axis);
+                l.stiffness = kMaxPenalty;
+                l.damping = 0.0f;
+                if (lower == upper) {
+                    const float equal_penalty = angular && generic ? lock_penalty : penalty;
+                    if (visit == kVisitInitialize && angular && generic && index < 18u)
+                        j.rows[index].penalty = smax(j.rows[index].penalty, equal_penalty / kGamma);
+                    l.current = limit_current - lower; l.previous = old_limit_current - lower;
+                    l.penalty = equal_penalty; l.lower = -kMaxPenalty; l.upper = kMaxPenalty;
+                    l.stabilization = kAlpha;
+                    visit_joint_row(a, j, blk, index, impulse, first, l, visit);
+                } else if (lower < upper) {
+                    l.current = limit_current - upper; l.previous = smax(old_limit_current - upper, 0.0f);
+                    l.penalty = penalty; l.lower = 0.0f; l.upper = kMaxPenalty;
+                    l.stabilization = kLimitStabilization;
+                    visit_joint_row(a, j, blk, index, impulse, first, l, visit);
+                    l.current = limit_current - lower; l.previous = smin(old_limit_current - lower, 0.0f);
+                    l.lower = -kMaxPenalty; l.upper = 0.0f;
+                    visit_joint_row(a, j, blk, index, impulse, first, l, visit);
+                }
+            }
+            if (type == kJointGenericSpring && (springs.axes & (1u << axis)) != 0u) {
+                JointRowArgs sr = r;
+                sr.current = current; sr.previous = old_current; sr.penalty = penalty;
+                sr.stiffness = comp(sv(springs.stiffness), axis); sr.damping = comp(sv(springs.damping), axis);
+                sr.lower = -kMaxPenalty; sr.upper = kMaxPenalty; sr.stabilization = kAlpha;
+                visit_joint_row(a, j, blk, index, impulse, first, sr, visit);
+            }
+            if (motor_type && axis == 0u && (angular ? o.motor.angular_enabled != 0u : o.motor.linear_enabled != 0u)) {
+                const float speed = angular ? o.motor.angular_target_velocity : o.motor.linear_target_velocity;
+                const float bound = (angular ? o.motor.angular_maximum_impulse : o.motor.linear_maximum_impulse) / a.dt;
+                const Q rel = relative_quaternion(fA, fB, true);
+                const Q old_rel = relative_quaternion(ofA, ofB, true);
+                const V3 motor_rotation = rel_rotation_vector({0.0f, 0.0f, 0.0f, 1.0f},
+                                                              relative_quaternion(old_rel, rel, false));
+                JointRowArgs mr = r;
+                if (angular) {
+                    const V3 motor_axis = log_axis(fA, motor_rotation, axis);
+                    set6(mr.ja, zero, mul(motor_axis, -1.0f));
+                    set6(mr.jb, zero, motor_axis);
+                }
+                const float travel = angular ? comp(motor_rotation, axis) : current - old_current;
+                mr.current = travel - speed * a.dt; mr.previous = 0.0f; mr.penalty = penalty;
+                mr.stiffness = kMaxPenalty; mr.damping = 0.0f; mr.lower = -bound; mr.upper = bound;
+                mr.motor = true; mr.motor_speed = speed; mr.stabilization = kAlpha;
+                visit_joint_row(a, j, blk, index, impulse, first, mr, visit);
+            }
+        }
+    }
+    if (visit == kVisitInitialize) j.avbd_count = index;
+    return impulse;
+}
+
+// The joint rows of body i in its list order, added to its block: the
+// pose phase recomputes the rows, the velocity phase adds the stored hard
+// rows (solve.slang:401-430).
+__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk) {
+    for (uint32_t e = a.jstart[i]; e < a.jstart[i + 1]; ++e) {
+        PmhRigidConstraint &j = a.joints[a.jadj[e]];
+        if (j.state_enabled == 0u) continue;
+        const bool first = i == j.dense_a;
+        if (phase == 0u) {
+            visit_joint(a, j, blk, first, kVisitAccumulate);
+            continue;
+        }
+        float da[6], db[6];
+        load_disp(a.disp, j.dense_a, da);
+        load_disp(a.disp, j.dense_b, db);
+        for (uint32_t r = 0; r < j.avbd_count; ++r) {
+            const PmhAvbdRow row = j.rows[r];
+            if (row.stiffness >= kMaxPenalty) jrow_accumulate(blk, row, first, da, db, a.dt);
+        }
+    }
+}
+
+// Joint part of advance_duals (solve.slang:566-614): dual updates, the
+// applied impulse and breaking, one thread per joint.
+__device__ void advance_joint(const SolveArgs &a, uint32_t k, uint32_t phase) {
+    PmhRigidConstraint &j = a.joints[k];
+    if (j.avbd_count == 0u || j.state_enabled == 0u) return;
+    const uint32_t ia = j.dense_a, ib = j.dense_b;
+    if (phase != 0u && a.impact[ia] == 0u && a.impact[ib] == 0u) return;
+    float impulse = 0.0f;
+    if (phase == 0u) {
+        Block scratch;
+        block_zero(scratch);
+        impulse = visit_joint(a, j, scratch, true, kVisitAdvance);
+    } else {
+        float da[6], db[6];
+        load_disp(a.disp, ia, da);
+        load_disp(a.disp, ib, db);
+        for (uint32_t r = 0; r < j.avbd_count; ++r) {
+            PmhAvbdRow row = j.rows[r];
+            impulse += fabsf(row.stiffness < kMaxPenalty ? row.reference_force : jrow_force(row, da, db, a.dt)) * a.dt;
+            if (row.stiffness >= kMaxPenalty) {
+                jrow_advance(row, da, db);
+                j.rows[r] = row;
+            }
+        }
+    }
+    j.applied_impulse = impulse;
+    if (j.options.breaking_impulse_threshold > 0.0f && impulse > j.options.breaking_impulse_threshold) {
+        j.state_broken = 1u;
+        j.state_enabled = 0u;
+    }
 }
 
 // advance_duals for contacts (solve.slang:492-566): one thread per point.
@@ -1100,7 +1508,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
         }
         for (uint32_t it = 0; it < a.budget; ++it) {
             for (uint32_t c = 0; c < colors; ++c) {
-                solve_color<L>(a, c, phase, gw, nw);
+                solve_color<L, false>(a, c, phase, gw, nw);
                 grid.sync();
             }
             advance_duals(a, phase, it, tid, nthreads);
@@ -1119,13 +1527,17 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
 }
 
 // Staged variant: one launch per stage (no grid barrier).
-template <int L>
+template <int L, bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t color, uint32_t phase) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    solve_color<L>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
+    solve_color<L, kJoints>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
+template <bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
     advance_duals(a, phase, it, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    if (kJoints)
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.nj; k += gridDim.x * blockDim.x)
+            advance_joint(a, k, phase);
 }
 __global__ void k_stage_impact_gaps(SolveArgs a) {
     impact_gaps(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1133,8 +1545,18 @@ __global__ void k_stage_impact_gaps(SolveArgs a) {
 __global__ void k_stage_impact_bodies(SolveArgs a) {
     impact_bodies(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
+template <bool kJoints>
 __global__ void k_stage_velocity_errors(SolveArgs a, const uint32_t *c_flags) {
     velocity_errors(a, c_flags, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    // Joint rows of the velocity phase (impact_constraints, solve.slang:277-291).
+    if (kJoints) for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.nj; k += gridDim.x * blockDim.x) {
+        PmhRigidConstraint &j = a.joints[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u) continue;
+        if (a.impact[j.dense_a] == 0u && a.impact[j.dense_b] == 0u) continue;
+        Block scratch;
+        block_zero(scratch);
+        visit_joint(a, j, scratch, true, kVisitImpact);
+    }
 }
 __global__ void k_stage_velocity_jacobians(SolveArgs a) {
     velocity_jacobians(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1148,6 +1570,118 @@ __global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
         a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
         a.schedule->maximum_passes += a.budget;
         account_phase(a, phase);
+    }
+}
+
+// ---- joint preparation (geometry_constraints.slang:1391-1417, 1462-1475) --
+// State, dense ids and initial rows; one thread per joint.  Body ids are
+// {dense index, 1}.
+__global__ void k_joint_init(SolveArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nj) return;
+    PmhRigidConstraint &j = a.joints[k];
+    j.avbd_count = 0u;
+    j.avbd_next_a = j.avbd_next_b = kInvalid;
+    if (j.alive == 0u) return;
+    j.state_enabled = j.options.enabled != 0u && j.state_broken == 0u ? 1u : 0u;
+    if (j.state_enabled == 0u) return;
+    j.applied_impulse = 0.0f;
+    const PmhBodyId ia = j.options.body_a, ib = j.options.body_b;
+    j.dense_a = ia.index < a.n && ia.generation == 1u ? ia.index : kInvalid;
+    j.dense_b = ib.index < a.n && ib.generation == 1u ? ib.index : kInvalid;
+    if (j.dense_a == kInvalid || j.dense_b == kInvalid) return;
+    Block scratch;
+    block_zero(scratch);
+    visit_joint(a, j, scratch, true, kVisitInitialize);
+}
+
+// Incident-joint counts, the joints' iteration floor, island unions and
+// anchoring by non-dynamic bodies.
+__global__ void k_joint_degree(SolveArgs a, uint32_t *jdeg, uint32_t raise_budget, StepCounters *ctr) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nj) return;
+    const PmhRigidConstraint &j = a.joints[k];
+    if (j.avbd_count == 0u) return;
+    atomicAdd(&jdeg[j.dense_a], 1u);
+    atomicAdd(&jdeg[j.dense_b], 1u);
+    if (raise_budget) {
+        // Legacy profile without a requested count: the scene budget rises
+        // to the largest joint iteration count (:1466-1470).
+        atomicMax(&ctr->joint_budget, j.options.solver_iterations);
+        atomicMax(&a.schedule->budget, j.options.solver_iterations);
+    }
+}
+
+__global__ void k_joint_fill(SolveArgs a, const uint32_t *jstart, uint32_t *jcursor, uint32_t *jadj) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nj) return;
+    const PmhRigidConstraint &j = a.joints[k];
+    if (j.avbd_count == 0u) return;
+    jadj[jstart[j.dense_a] + atomicAdd(&jcursor[j.dense_a], 1u)] = k;
+    jadj[jstart[j.dense_b] + atomicAdd(&jcursor[j.dense_b], 1u)] = k;
+}
+
+// List links of the reference layout (push-front in joint order, so each
+// body's list runs in descending joint index) from the sorted segments.
+__global__ void k_joint_links(SolveArgs a, uint32_t *joint_head) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t b = a.jstart[i], e = a.jstart[i + 1];
+    joint_head[i] = b < e ? a.jadj[b] : kInvalid;
+    for (uint32_t p = b; p < e; ++p) {
+        PmhRigidConstraint &j = a.joints[a.jadj[p]];
+        const uint32_t next = p + 1 < e ? a.jadj[p + 1] : kInvalid;
+        if (j.dense_a == i) j.avbd_next_a = next;
+        else j.avbd_next_b = next;
+    }
+}
+
+__global__ void k_joint_hook(SolveArgs a, uint32_t *parent) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nj) return;
+    const PmhRigidConstraint &j = a.joints[k];
+    if (j.avbd_count == 0u) return;
+    if (a.P[j.dense_a].motion != kDynamic || a.P[j.dense_b].motion != kDynamic) return;
+    uint32_t u = uf_find(parent, j.dense_a), v = uf_find(parent, j.dense_b);
+    while (u != v) {
+        if (u > v) { uint32_t t = u; u = v; v = t; }
+        const uint32_t old = atomicCAS(&parent[v], v, u);
+        if (old == v) break;
+        u = uf_find(parent, u);
+        v = uf_find(parent, old);
+    }
+}
+
+__global__ void k_joint_anchor(SolveArgs a, const uint32_t *parent, uint32_t *anchored) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.nj) return;
+    const PmhRigidConstraint &j = a.joints[k];
+    if (j.avbd_count == 0u) return;
+    const bool da = a.P[j.dense_a].motion == kDynamic, db = a.P[j.dense_b].motion == kDynamic;
+    if (da && !db) atomicOr(&anchored[parent[j.dense_a]], 1u);
+    if (db && !da) atomicOr(&anchored[parent[j.dense_b]], 1u);
+}
+
+// rigid_avbd_project_hinge_anchors (solve.slang:174-209): after the pose
+// phase, each enabled hinge moves its anchors together, split by inverse
+// mass, one joint after the other in index order (a single thread: chained
+// hinges share bodies).
+__global__ void k_project_hinges(SolveArgs a) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (uint32_t k = 0; k < a.nj; ++k) {
+        const PmhRigidConstraint &j = a.joints[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u || j.options.type != kJointHinge) continue;
+        const uint32_t ia = j.dense_a, ib = j.dense_b;
+        const float ima = a.P[ia].motion == kDynamic ? a.P[ia].inverse_mass : 0.0f;
+        const float imb = a.P[ib].motion == kDynamic ? a.P[ib].inverse_mass : 0.0f;
+        const float sum = ima + imb;
+        if (sum <= kEpsilon) continue;
+        const PmhRigidBodyState sa = a.states[ia], sb = a.states[ib];
+        const V3 pa = add(sv(sa.position), rotate(sq(sa.orientation), sv(j.options.local_anchor_a)));
+        const V3 pb = add(sv(sb.position), rotate(sq(sb.orientation), sv(j.options.local_anchor_b)));
+        const V3 e = sub(pb, pa);
+        a.states[ia].position = pv(add(sv(a.states[ia].position), mul(e, ima / sum)));
+        a.states[ib].position = pv(sub(sv(a.states[ib].position), mul(e, imb / sum)));
     }
 }
 
@@ -1167,6 +1701,7 @@ __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32
     ctr->color_ticket = 0;
     ctr->dyn_bodies = ctr->impact_bodies = 0;
     ctr->mean_roots = 0;
+    ctr->joint_budget = 0;
     ctr->m1 = ctr->m2 = ctr->p1 = ctr->p2 = 0;
     ctr->m1i = ctr->m2i = ctr->p1i = ctr->p2i = 0;
 }
@@ -1217,6 +1752,12 @@ void KernelTimer::release() {
     pending = 0;
 }
 
+PmhResult Solver::reserve_joints(uint32_t count) {
+    PmhResult r = PMH_SUCCESS;
+    if ((r = jadj.reserve(2 * size_t(std::max<uint32_t>(1, count)))) != PMH_SUCCESS) return r;
+    return PMH_SUCCESS;
+}
+
 PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     PmhResult r = PMH_SUCCESS;
 #define R(buf, cnt) if ((r = buf.reserve(cnt)) != PMH_SUCCESS) return r
@@ -1225,6 +1766,7 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     R(impact, bodies); R(deg, bodies + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
     R(color_bodies, bodies); R(bp_start, bodies + 1); R(anchored, bodies); R(mean, bodies);
     R(members, bodies); R(mean_list, bodies);
+    R(jdeg, bodies + 1); R(jstart, bodies + 1); R(jcursor, bodies); R(joint_head, bodies);
     if (prev_vel.n < bodies) {
         R(prev_vel, bodies);
         PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
@@ -1255,7 +1797,8 @@ void Solver::release() {
     tpos.release(); tq.release(); tv.release(); tw.release(); disp.release(); dq.release();
     inertia.release(); prev_vel.release(); color.release(); parent.release(); impact.release();
     deg.release(); adj_start.release(); adj_cursor.release(); color_bodies.release();
-    bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release(); code.release(); ab.release();
+    bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release();
+    jdeg.release(); jstart.release(); jcursor.release(); joint_head.release(); jadj.release(); code.release(); ab.release();
     m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
     m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
@@ -1285,45 +1828,6 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     const uint32_t pg = grid_for(v.pair_cap, 256, 2048);
     const uint32_t cg_ = grid_for(v.point_cap, 256, 4096);
 
-    k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
-    k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
-                                   v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
-                                   v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p, v.members.p, v.mean.p);
-    k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
-                                      v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
-                                      v.m_impact.p, dt, sp.profile);
-    k_degree<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.deg.p, nullptr, ctr);
-    PmhResult r = exclusive_scan(s, v.deg.p, v.adj_start.p, n, nullptr, nullptr, v.scan_tmp);
-    if (r != PMH_SUCCESS) return r;
-    k_adj_fill<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.adj_start.p, v.adj_cursor.p, v.adj.p);
-    k_adj_sort<<<bg, 256, 0, s>>>(n, v.adj_start.p, v.adj.p);
-    k_uf_hook<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.parent.p);
-    k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
-    k_impact_roots<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_impact.p, v.m_count.p, v.parent.p, v.impact.p,
-                                      v.anchored.p, ctr);
-    k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, v.anchored.p, v.members.p,
-                                       v.mean_list.p, ctr);
-    if (!realtime)
-        k_count_impact_pairs<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.impact.p, ctr);
-    const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
-    k_color<<<color_blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, ctr);
-    PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
-    k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
-    k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr);
-    k_color_place<<<bg, 256, 0, s>>>(n, v.color.p, v.color_start.p, v.color_count.p, v.color_bodies.p);
-    // colour-ordered point lists (deg array reused as the per-position count)
-    // The degree array is free again once the adjacency is built (the
-    // fill cursor keeps each body's degree): it holds the point counts.
-    k_bp_count<<<bg, 256, 0, s>>>(n, ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p,
-                                  v.m_count.p, v.deg.p);
-    r = exclusive_scan(s, v.deg.p, v.bp_start.p, n, nullptr, nullptr, v.scan_tmp);
-    if (r != PMH_SUCCESS) return r;
-    k_bp_fill<<<bg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
-                                 v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p));
-    k_jacobians<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.tpos.p, v.tq.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
-                                    v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p);
-    PMH_CHECK(hipGetLastError());
-
     SolveArgs a;
     a.n = n;
     a.P = P;
@@ -1346,8 +1850,67 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.dt = dt;
     a.budget = budget;
     a.realtime = realtime ? 1u : 0u;
+    a.joints = v.joints;
+    a.nj = v.nj;
+    a.jstart = v.jstart.p;
+    a.jadj = v.jadj.p;
+    a.prev = previous;
+    const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj));
+    k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
+    k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
+                                   v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
+                                   v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p, v.members.p, v.mean.p,
+                                   v.jdeg.p, v.jcursor.p);
+    k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
+                                      v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
+                                      v.m_impact.p, dt, sp.profile);
+    if (v.nj) k_joint_init<<<jg, 256, 0, s>>>(a);
+    k_degree<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.deg.p, nullptr, ctr);
+    PmhResult r = exclusive_scan(s, v.deg.p, v.adj_start.p, n, nullptr, nullptr, v.scan_tmp);
+    if (r != PMH_SUCCESS) return r;
+    k_adj_fill<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.adj_start.p, v.adj_cursor.p, v.adj.p);
+    k_adj_sort<<<bg, 256, 0, s>>>(n, v.adj_start.p, v.adj.p);
+    if (v.nj) {
+        // Joint lists, geometry_constraints.slang:1462-1475.
+        k_joint_degree<<<jg, 256, 0, s>>>(a, v.jdeg.p, !realtime && sp.requested_iterations == 0u ? 1u : 0u, ctr);
+        r = exclusive_scan(s, v.jdeg.p, v.jstart.p, n, nullptr, nullptr, v.scan_tmp);
+        if (r != PMH_SUCCESS) return r;
+        k_joint_fill<<<jg, 256, 0, s>>>(a, v.jstart.p, v.jcursor.p, v.jadj.p);
+        k_adj_sort<<<bg, 256, 0, s>>>(n, v.jstart.p, v.jadj.p);
+        k_joint_links<<<bg, 256, 0, s>>>(a, v.joint_head.p);
+    }
+    k_uf_hook<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.parent.p);
+    if (v.nj) k_joint_hook<<<jg, 256, 0, s>>>(a, v.parent.p);
+    k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
+    k_impact_roots<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_impact.p, v.m_count.p, v.parent.p, v.impact.p,
+                                      v.anchored.p, ctr);
+    if (v.nj) k_joint_anchor<<<jg, 256, 0, s>>>(a, v.parent.p, v.anchored.p);
+    k_impact_bodies<<<bg, 256, 0, s>>>(n, P, v.parent.p, v.impact.p, v.deg.p, v.jdeg.p, v.anchored.p, v.members.p,
+                                       v.mean_list.p, ctr);
+    if (!realtime)
+        k_count_impact_pairs<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.impact.p, ctr);
+    const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
+    k_color<<<color_blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
+                                         v.nj ? v.joints : nullptr, ctr);
+    PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
+    k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
+    k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr);
+    k_color_place<<<bg, 256, 0, s>>>(n, v.color.p, v.color_start.p, v.color_count.p, v.color_bodies.p);
+    // colour-ordered point lists (deg array reused as the per-position count)
+    // The degree array is free again once the adjacency is built (the
+    // fill cursor keeps each body's degree): it holds the point counts.
+    k_bp_count<<<bg, 256, 0, s>>>(n, ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p,
+                                  v.m_count.p, v.deg.p);
+    r = exclusive_scan(s, v.deg.p, v.bp_start.p, n, nullptr, nullptr, v.scan_tmp);
+    if (r != PMH_SUCCESS) return r;
+    k_bp_fill<<<bg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
+                                 v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p));
+    k_jacobians<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.tpos.p, v.tq.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
+                                    v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p);
+    PMH_CHECK(hipGetLastError());
 
-    if (sp.cooperative) {
+
+    if (sp.cooperative && v.nj == 0) {
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const void *kernel = sp.lanes == 64 ? reinterpret_cast<const void *>(k_solve_all<64>)
                            : sp.lanes == 32 ? reinterpret_cast<const void *>(k_solve_all<32>)
@@ -1380,6 +1943,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         PMH_CHECK(hipStreamSynchronize(s));
         const StepCounters host = *v.host_ctr;
         const uint32_t lanes = sp.lanes;
+        if (!realtime && sp.requested_iterations == 0u) a.budget = std::max(budget, host.joint_budget);
+        const uint32_t budget = a.budget;
         KernelTimer *kt = timing ? timing->kernels : nullptr;
         bool time_colors = false;
         if (kt && kt->stride) {
@@ -1390,14 +1955,29 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         uint32_t timed = 0;
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const uint32_t colors = std::min(host.colors, kMaxColors);
-        const uint32_t pts_blocks = std::max<uint32_t>(1, (host.points + 255) / 256);
+        const uint32_t pts_blocks = std::max<uint32_t>(1, (std::max(host.points, v.nj) + 255) / 256);
         const uint32_t phases = realtime ? 1u : (host.impact ? 2u : 1u);
+        const bool joints = v.nj != 0;
+        auto launch_color = [&](uint32_t blocks, uint32_t c, uint32_t phase) {
+            if (joints) {
+                if (lanes == 64) k_stage_color<64, true><<<blocks, 256, 0, s>>>(a, c, phase);
+                else if (lanes == 32) k_stage_color<32, true><<<blocks, 256, 0, s>>>(a, c, phase);
+                else if (lanes == 8) k_stage_color<8, true><<<blocks, 256, 0, s>>>(a, c, phase);
+                else k_stage_color<16, true><<<blocks, 256, 0, s>>>(a, c, phase);
+            } else {
+                if (lanes == 64) k_stage_color<64, false><<<blocks, 256, 0, s>>>(a, c, phase);
+                else if (lanes == 32) k_stage_color<32, false><<<blocks, 256, 0, s>>>(a, c, phase);
+                else if (lanes == 8) k_stage_color<8, false><<<blocks, 256, 0, s>>>(a, c, phase);
+                else k_stage_color<16, false><<<blocks, 256, 0, s>>>(a, c, phase);
+            }
+        };
         uint32_t launches = 0;
         for (uint32_t phase = 0; phase < phases; ++phase) {
             if (phase != 0) {
                 k_stage_impact_gaps<<<pts_blocks, 256, 0, s>>>(a);
                 k_stage_impact_bodies<<<bg, 256, 0, s>>>(a);
-                k_stage_velocity_errors<<<pts_blocks, 256, 0, s>>>(a, v.c_flags.p);
+                if (joints) k_stage_velocity_errors<true><<<pts_blocks, 256, 0, s>>>(a, v.c_flags.p);
+                else k_stage_velocity_errors<false><<<pts_blocks, 256, 0, s>>>(a, v.c_flags.p);
                 k_stage_velocity_jacobians<<<pts_blocks, 256, 0, s>>>(a);
                 launches += 4;
             }
@@ -1415,16 +1995,14 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                         }
                         PMH_CHECK(hipEventRecord(kt->events[2 * timed], s));
                     }
-                    if (lanes == 64) k_stage_color<64><<<blocks, 256, 0, s>>>(a, c, phase);
-                    else if (lanes == 32) k_stage_color<32><<<blocks, 256, 0, s>>>(a, c, phase);
-                    else if (lanes == 8) k_stage_color<8><<<blocks, 256, 0, s>>>(a, c, phase);
-                    else k_stage_color<16><<<blocks, 256, 0, s>>>(a, c, phase);
+                    launch_color(blocks, c, phase);
                     if (time_colors) {
                         PMH_CHECK(hipEventRecord(kt->events[2 * timed + 1], s));
                         ++timed;
                     }
                 }
-                k_stage_duals<<<pts_blocks, 256, 0, s>>>(a, phase, it);
+                if (joints) k_stage_duals<true><<<pts_blocks, 256, 0, s>>>(a, phase, it);
+                else k_stage_duals<false><<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 launches += colors + 1;
             }
             if (host.mean_roots) {
@@ -1433,6 +2011,10 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             }
             k_stage_apply<<<bg, 256, 0, s>>>(a, phase);
             launches += 1;
+            if (phase == 0 && v.hinges) {
+                k_project_hinges<<<1, 64, 0, s>>>(a);
+                launches += 1;
+            }
         }
         if (timing) timing->launches += launches;
         if (time_colors) {

@@ -179,6 +179,70 @@ __device__ __forceinline__ bool overlap(float4 amn, float4 amx, float4 bmn, floa
            amx.z >= bmn.z;
 }
 
+// Collision exclusions of joints (geometry.slang:494-511, 3108-3136): bodies
+// of one weld group never collide, and a pair joined by a joint with
+// disable_collisions does not while that joint is alive, enabled and
+// unbroken.  A null `weld` or `excl_start` turns that test off.
+struct JointFilter {
+    const uint32_t *weld = nullptr;
+    const uint32_t *excl_start = nullptr;
+    const uint2 *excl_list = nullptr;   // (partner, joint)
+    const PmhRigidConstraint *joints = nullptr;
+};
+
+__device__ __forceinline__ bool joint_excluded(const JointFilter &f, uint32_t a, uint32_t b) {
+    if (f.weld && f.weld[a] == f.weld[b]) return true;
+    if (f.excl_start) {
+        for (uint32_t e = f.excl_start[a]; e < f.excl_start[a + 1]; ++e) {
+            const uint2 x = f.excl_list[e];
+            if (x.x == b && f.joints[x.y].state_broken == 0u) return true;
+        }
+    }
+    return false;
+}
+
+__device__ uint32_t weld_find(const uint32_t *weld, uint32_t v) {
+    uint32_t p = __hip_atomic_load(&weld[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != v) {
+        v = p;
+        p = __hip_atomic_load(&weld[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return v;
+}
+
+__global__ void k_weld_init(uint32_t n, uint32_t *weld) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) weld[i] = i;
+}
+
+// Fixed joints that disable collisions join their bodies' weld group; the
+// larger root hooks under the smaller, so each group's root is its
+// smallest index whatever the order of the hooks.
+__global__ void k_weld_hook(uint32_t n, uint32_t nj, const PmhRigidConstraint *joints, uint32_t *weld) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nj) return;
+    const PmhRigidConstraint &j = joints[k];
+    const PmhRigidConstraintOptions &o = j.options;
+    if (j.alive == 0u || o.enabled == 0u || j.state_broken != 0u || o.disable_collisions == 0u ||
+        o.type != kJointFixed)
+        return;
+    if (o.body_a.index >= n || o.body_a.generation != 1u || o.body_b.index >= n || o.body_b.generation != 1u)
+        return;
+    uint32_t u = weld_find(weld, o.body_a.index), v = weld_find(weld, o.body_b.index);
+    while (u != v) {
+        if (u > v) { const uint32_t t = u; u = v; v = t; }
+        const uint32_t old = atomicCAS(&weld[v], v, u);
+        if (old == v) break;
+        u = weld_find(weld, u);
+        v = weld_find(weld, old);
+    }
+}
+
+__global__ void k_weld_compress(uint32_t n, uint32_t *weld) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) weld[i] = weld_find(weld, i);
+}
+
 // Kept candidates of dynamic body a.  Pass 0 counts them; pass 1 writes
 // them at the body's offset (from an exclusive scan of the counts) and
 // sorts the segment ascending (identity order = index), so a body may own
@@ -187,7 +251,8 @@ template <int kPass>
 __global__ void k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 *pmn, const float4 *pmx,
                            const int4 *cell_of, const uint32_t *bucket_start, const uint32_t *entries,
                            uint32_t table_mask, const uint32_t *large, uint32_t *cand_count,
-                           const uint32_t *offset, uint32_t *code, uint2 *ab, const StepCounters *ctr) {
+                           const uint32_t *offset, uint32_t *code, uint2 *ab, const StepCounters *ctr,
+                           JointFilter jf) {
     const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     if (P[a].motion != kDynamic) {
@@ -202,6 +267,7 @@ __global__ void k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 
         if (b == a) return;
         if (P[b].motion == kDynamic && !(a < b)) return;
         if (!overlap(amn, amx, pmn[b], pmx[b])) return;
+        if (joint_excluded(jf, a, b)) return;
         if (kPass == 1) code[base + count] = b;
         ++count;
     };
@@ -260,7 +326,7 @@ template <int kPass>
 __global__ void __launch_bounds__(256) k_bp_large(uint32_t n, const PmhBodyParameters *P, const float4 *pmn,
                                                   const float4 *pmx, const uint32_t *large, uint32_t *cand_count,
                                                   const uint32_t *offset, uint32_t *code, uint2 *ab,
-                                                  const StepCounters *ctr) {
+                                                  const StepCounters *ctr, JointFilter jf) {
     __shared__ uint32_t warp_tot[4];
     const uint32_t nl = ctr->large_count;
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -277,7 +343,7 @@ __global__ void __launch_bounds__(256) k_bp_large(uint32_t n, const PmhBodyParam
             const uint32_t b = base + threadIdx.x;
             bool hit = false;
             if (b < n && b != a && !(P[b].motion == kDynamic && !(a < b)))
-                hit = overlap(amn, amx, pmn[b], pmx[b]);
+                hit = overlap(amn, amx, pmn[b], pmx[b]) && !joint_excluded(jf, a, b);
             const uint64_t bal = __ballot(hit);
             if (lane == 0) warp_tot[wid] = __popcll(bal);
             __syncthreads();
@@ -1218,14 +1284,35 @@ struct NarrowOut {
     StepCounters *ctr;
 };
 
+// Reserves `count` (< 16) consecutive slots of *counter per active lane
+// with one atomic per wavefront: each lane's offset is the sum of the
+// counts of the active lanes below it, built from one ballot per bit.
+// Works in divergent code (ballots and the broadcast see active lanes only).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t count) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t active = __ballot(1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t offset = 0, total = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) {
+        const uint64_t bits = __ballot((count >> b) & 1u);
+        offset += uint32_t(__popcll(bits & below)) << b;
+        total += uint32_t(__popcll(bits)) << b;
+    }
+    const uint32_t leader = __builtin_ctzll(active);
+    uint32_t base = 0;
+    if (lane == leader && total) base = atomicAdd(counter, total);
+    base = __shfl(base, leader, 64);
+    return base + offset;
+}
+
 // pm_finalize_rigid_contact_manifolds (geometry.slang:2805-2873): the
 // initial relative position, persistence and the initial normal speed, and
 // the point range.  Where the range lies does not affect any result.
 __device__ void emit_manifold(const NarrowOut &o, uint32_t m, Manifold &man, const PmhRigidBodyState &A,
                               const PmhRigidBodyState &B, bool persistent) {
-    uint32_t start = 0;
+    uint32_t start = wave_reserve(&o.ctr->point_alloc, man.count);
     if (man.count) {
-        start = atomicAdd(&o.ctr->point_alloc, man.count);
         if (start + man.count > o.point_cap) {
             atomicOr(&o.ctr->status, kStatusPointOverflow);
             man.count = 0;
@@ -1297,7 +1384,7 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
                 }
             }
             if (!face) {
-                tri_list[atomicAdd(&o.ctr->tri_pairs, 1u)] = m;
+                tri_list[wave_reserve(&o.ctr->tri_pairs, 1u)] = m;
                 continue;
             }
         }
@@ -1723,6 +1810,18 @@ PmhResult World::step_once(uint32_t k_step) {
     k_integrate_proxy<<<bg, 256, 0, s>>>(n, params.p, shapes_dev.p, cur, prev, pred, proxy_mn.p, proxy_mx.p,
                                          gravity, dt, inputs_dev.p, targets_dev.p, target_flags_dev.p, ctr);
     mark(1);
+    JointFilter jf;
+    if (solver.nj && weld_joints) {
+        k_weld_init<<<bg, 256, 0, s>>>(n, weld.p);
+        k_weld_hook<<<blocks_for(solver.nj), 256, 0, s>>>(n, solver.nj, joints_dev.p, weld.p);
+        k_weld_compress<<<bg, 256, 0, s>>>(n, weld.p);
+        jf.weld = weld.p;
+    }
+    if (solver.nj && excl_count) {
+        jf.excl_start = excl_start.p;
+        jf.excl_list = excl_list.p;
+        jf.joints = joints_dev.p;
+    }
     // Broadphase.
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * (table_size + 1), s));
     k_bp_cells<<<bg, 256, 0, s>>>(n, proxy_mn.p, proxy_mx.p, cell_of.p, bucket_count.p, table_size - 1,
@@ -1733,17 +1832,17 @@ PmhResult World::step_once(uint32_t k_step) {
     k_bp_scatter<<<bg, 256, 0, s>>>(n, cell_of.p, bucket_start.p, bucket_count.p, bucket_entries.p);
     k_bp_query<0><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
-                                                  cand_count.p, nullptr, nullptr, nullptr, ctr);
+                                                  cand_count.p, nullptr, nullptr, nullptr, ctr, jf);
     k_bp_large<0><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, nullptr, nullptr,
-                                     nullptr, ctr);
+                                     nullptr, ctr, jf);
     r = exclusive_scan(s, cand_count.p, cand_offset.p, n, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     k_set_pairs<<<1, 64, 0, s>>>(ctr, cand_offset.p, n, v.pair_cap);
     k_bp_large<1><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, cand_offset.p,
-                                     v.code.p, v.ab.p, ctr);
+                                     v.code.p, v.ab.p, ctr, jf);
     k_bp_query<1><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
-                                                  cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr);
+                                                  cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr, jf);
     mark(2);
     // Narrowphase + point compaction.
     const NarrowOut out{v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p,

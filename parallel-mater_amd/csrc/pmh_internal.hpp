@@ -101,7 +101,8 @@ struct StepCounters {
     uint32_t tri_pairs;        // mesh pairs on the triangle-pair path
     uint32_t fallback_pairs;   // of those, solved by BVH traversal
     uint32_t mean_roots;       // unanchored islands with contacts (free-motion mean)
-    uint32_t pad[3];
+    uint32_t joint_budget;     // largest joint iteration count (legacy budget floor)
+    uint32_t pad[2];
 };
 
 // Running totals over every step since the last reset.
@@ -112,6 +113,9 @@ struct SolveTotals {
 
 constexpr uint32_t kMaxColors = 256;
 constexpr uint32_t kDefaultLanes = 32;   // lanes per body in the colour sweep
+// Joint types (RigidConstraintType, src/simulation/types.hpp:662).
+enum : uint32_t { kJointFixed = 0, kJointPoint = 1, kJointHinge = 2, kJointSlider = 3, kJointPiston = 4,
+                  kJointGeneric = 5, kJointGenericSpring = 6, kJointMotor = 7 };
 constexpr uint32_t kCacheFloat4PerPoint = 6;
 
 template <class T>
@@ -144,6 +148,11 @@ struct Solver {
     DBuf<uint32_t> color_bodies, bp_start, anchored;
     DBuf<float4> mean;                   // free-motion mean per root
     DBuf<uint32_t> members, mean_list;   // dynamic members per root; roots taking the mean
+    // joints (records owned by the caller: the world or the kernel seam)
+    PmhRigidConstraint *joints = nullptr;
+    uint32_t nj = 0;
+    bool hinges = false;                 // some joint is a hinge (anchor projection)
+    DBuf<uint32_t> jdeg, jstart, jcursor, jadj, joint_head;
     // pairs
     DBuf<uint32_t> code;                 // a*n+b
     DBuf<uint2> ab;
@@ -165,6 +174,7 @@ struct Solver {
     StepCounters *host_ctr = nullptr;        // pinned read-back of the schedule
     uint32_t *host_color_start = nullptr;
     PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
+    PmhResult reserve_joints(uint32_t count);
     void release();
 };
 

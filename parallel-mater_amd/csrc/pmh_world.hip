@@ -395,7 +395,9 @@ PmhResult World::ensure_device() {
         PMH_CHECK(hipMemcpy(state_cur.p, host_states.data(), sizeof(PmhRigidBodyState) * n, hipMemcpyHostToDevice));
         PMH_CHECK(hipMemset(cache_n.p, 0, sizeof(uint32_t) * 2));
         bodies_dirty = false;
+        joints_dirty = true;   // exclusion lists are sized by the body count
     }
+    if (joints_dirty && (r = upload_joints()) != PMH_SUCCESS) return r;
     if (inputs_dirty) {
         // The step consumes and clears them (pm_clear_rigid_inputs).
         PMH_CHECK(hipMemcpyAsync(inputs_dev.p, host_inputs.data(), sizeof(float4) * host_inputs.size(),
@@ -412,11 +414,72 @@ PmhResult World::ensure_device() {
     return PMH_SUCCESS;
 }
 
+// Joint records and their collision exclusions (geometry.slang:494-511,
+// 3108-3136).  Body ids resolve to {dense index, generation 1}; a joint
+// excludes its pair while it is alive, enabled, unbroken and has
+// disable_collisions, and a fixed one also welds its bodies together.
+PmhResult World::upload_joints() {
+    const uint32_t nj = uint32_t(host_joints.size());
+    const uint32_t n = body_count;
+    PmhResult r;
+    if ((r = joints_dev.reserve(std::max<uint32_t>(1, nj))) || (r = solver.reserve_joints(nj)) ||
+        (r = weld.reserve(std::max<uint32_t>(1, n))) || (r = excl_start.reserve(n + 1)))
+        return r;
+    if (nj) PMH_CHECK(hipMemcpy(joints_dev.p, host_joints.data(), sizeof(PmhRigidConstraint) * nj,
+                                hipMemcpyHostToDevice));
+    std::vector<uint32_t> start(n + 1, 0u);
+    std::vector<uint2> list;
+    weld_joints = 0;
+    bool hinges = false;
+    auto dense = [&](PmhBodyId id) { return id.index < n && id.generation == 1u ? id.index : kInvalid; };
+    for (const PmhRigidConstraint &j : host_joints) {
+        if (j.options.type == kJointHinge) hinges = true;
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || j.options.enabled == 0u || j.options.disable_collisions == 0u) continue;
+        if (a == kInvalid || b == kInvalid) continue;
+        start[a]++;
+        start[b]++;
+        if (j.options.type == kJointFixed) weld_joints++;
+    }
+    uint32_t total = 0;
+    for (uint32_t i = 0; i <= n; ++i) { const uint32_t c = start[i]; start[i] = total; total += c; }
+    list.resize(std::max<uint32_t>(1, total));
+    std::vector<uint32_t> cursor(start.begin(), start.end());
+    for (uint32_t k = 0; k < nj; ++k) {
+        const PmhRigidConstraint &j = host_joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || j.options.enabled == 0u || j.options.disable_collisions == 0u) continue;
+        if (a == kInvalid || b == kInvalid) continue;
+        list[cursor[a]++] = make_uint2(b, k);
+        list[cursor[b]++] = make_uint2(a, k);
+    }
+    excl_count = total;
+    if ((r = excl_list.reserve(list.size()))) return r;
+    PMH_CHECK(hipMemcpy(excl_start.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    PMH_CHECK(hipMemcpy(excl_list.p, list.data(), sizeof(uint2) * list.size(), hipMemcpyHostToDevice));
+    solver.joints = joints_dev.p;
+    solver.nj = nj;
+    solver.hinges = hinges;
+    joints_dirty = false;
+    joints_on_device = nj != 0;
+    return PMH_SUCCESS;
+}
+
+// The device records (state, rows, duals) back into the host copy.
+PmhResult World::pull_joints() {
+    if (!joints_on_device) return PMH_SUCCESS;
+    PMH_CHECK(hipStreamSynchronize(stream));
+    PMH_CHECK(hipMemcpy(host_joints.data(), joints_dev.p, sizeof(PmhRigidConstraint) * solver.nj,
+                        hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
 void World::release() {
     if (stream) (void)hipStreamSynchronize(stream);
     shapes_dev.release(); verts_dev.release(); idx_dev.release(); planes_dev.release();
     nodes_dev.release(); leaves_dev.release(); shell_dev.release();
     inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
+    joints_dev.release(); weld.release(); excl_start.release(); excl_list.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -669,6 +732,40 @@ PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t c
     return PMH_SUCCESS;
 }
 
+PmhResult pmhWorldAddJoints(PmhWorld world, uint32_t count, const PmhRigidConstraintOptions *options) {
+    if (!world || (count && !options)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    for (uint32_t k = 0; k < count; ++k)
+        if (options[k].type > pmh::kJointMotor) return PMH_ERROR_INVALID_ARGUMENT;
+    PmhResult r = w->pull_joints();
+    if (r != PMH_SUCCESS) return r;
+    for (uint32_t k = 0; k < count; ++k) {
+        PmhRigidConstraint j;
+        std::memset(&j, 0, sizeof j);
+        j.options = options[k];
+        j.generation = 1u;
+        j.alive = 1u;
+        j.dense_a = j.dense_b = pmh::kInvalid;
+        j.avbd_next_a = j.avbd_next_b = pmh::kInvalid;
+        w->host_joints.push_back(j);
+    }
+    w->joints_dirty = true;
+    return PMH_SUCCESS;
+}
+
+uint32_t pmhWorldJointCount(PmhWorld world) { return world ? uint32_t(W(world)->host_joints.size()) : 0u; }
+
+PmhResult pmhWorldGetJoints(PmhWorld world, PmhRigidConstraint *joints) {
+    if (!world || !joints) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (!w->joints_dirty) {
+        PmhResult r = w->pull_joints();
+        if (r != PMH_SUCCESS) return r;
+    }
+    std::copy(w->host_joints.begin(), w->host_joints.end(), joints);
+    return PMH_SUCCESS;
+}
+
 PmhResult pmhWorldCopyStates(PmhWorld world, PmhRigidBodyState *states) {
     if (!world || !states) return PMH_ERROR_INVALID_ARGUMENT;
     World *w = W(world);
@@ -850,8 +947,9 @@ PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *out) {
     pmh::Solver &v = w->solver;
     const uint32_t n = w->body_count;
     std::vector<float4> tpos, tq, tv, tw, disp, dq, inertia, pvel;
-    std::vector<uint32_t> color, parent, impact;
+    std::vector<uint32_t> color, parent, impact, joint_head;
     PmhResult r;
+    if (v.nj && w->has_stepped && (r = fetch(joint_head, v.joint_head.p, n))) return r;
     if ((r = fetch(tpos, v.tpos.p, n)) || (r = fetch(tq, v.tq.p, n)) || (r = fetch(tv, v.tv.p, n)) ||
         (r = fetch(tw, v.tw.p, n)) || (r = fetch(disp, v.disp.p, 2 * size_t(n))) || (r = fetch(dq, v.dq.p, n)) ||
         (r = fetch(inertia, v.inertia.p, 2 * size_t(n))) || (r = fetch(pvel, v.prev_vel.p, n)) ||
@@ -873,6 +971,7 @@ PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *out) {
         b.inertia_h[9] = i0.y; b.inertia_h[13] = i0.z; b.inertia_h[14] = i0.w;
         b.inertia_h[18] = i1.x; b.inertia_h[19] = i1.y; b.inertia_h[20] = i1.z;
         b.contact_head = b.joint_head = 0xffffffffu;
+        if (v.nj && w->has_stepped) b.joint_head = joint_head[i];
         b.color = color[i];
         b.parent = parent[i];
         b.impact = impact[i];

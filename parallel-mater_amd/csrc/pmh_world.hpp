@@ -28,6 +28,10 @@ struct World {
     std::vector<uint32_t> host_target_flags;
     bool inputs_dirty = false;
     uint32_t body_count = 0;
+    // Joints: the host copy is authoritative until the first upload; after
+    // a step the device copy is (breaking, rows and warm-start duals).
+    std::vector<PmhRigidConstraint> host_joints;
+    bool joints_dirty = false, joints_on_device = false;
 
     // device scene
     DBuf<ShapeDev> shapes_dev;
@@ -42,6 +46,13 @@ struct World {
     DBuf<float4> inputs_dev;
     DBuf<PmhRigidBodyState> targets_dev;
     DBuf<uint32_t> target_flags_dev;
+    DBuf<PmhRigidConstraint> joints_dev;
+    // Collision exclusions of joints: weld roots (fixed joints that disable
+    // collisions) and, per body, its disable_collisions partners with the
+    // joint that excludes them.
+    DBuf<uint32_t> weld, excl_start;
+    DBuf<uint2> excl_list;
+    uint32_t weld_joints = 0, excl_count = 0;
 
     // broadphase
     DBuf<float4> proxy_mn, proxy_mx;
@@ -70,6 +81,8 @@ struct World {
     bool has_stepped = false;
 
     PmhResult ensure_device();
+    PmhResult upload_joints();
+    PmhResult pull_joints();
     PmhResult step_once(uint32_t k);
     void release();
 };

@@ -16,7 +16,7 @@ import os
 import numpy as np
 
 from . import layouts  # noqa: F401
-from .layouts import (AVBD_BODY, MANIFOLD, PARAMS, RIGID_INPUT, SCHEDULE, STATE)
+from .layouts import (AVBD_BODY, JOINT, JOINT_OPTIONS, MANIFOLD, PARAMS, RIGID_INPUT, SCHEDULE, STATE)
 
 HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(HERE, "lib", "libpm_hip_rigid.so")
@@ -96,6 +96,9 @@ def lib():
         "pmhWorldStep": (i32, [_P, _U]),
         "pmhWorldApplyRigidInputs": (i32, [_P, _U, _P]),
         "pmhWorldSetKinematicTargets": (i32, [_P, _U, _U, _P]),
+        "pmhWorldAddJoints": (i32, [_P, _U, _P]),
+        "pmhWorldJointCount": (_U, [_P]),
+        "pmhWorldGetJoints": (i32, [_P, _P]),
         "pmhWorldCopyStates": (i32, [_P, _P]),
         "pmhWorldSetStates": (i32, [_P, _P]),
         "pmhWorldGetStats": (i32, [_P, _P]),
@@ -208,6 +211,22 @@ class World:
         a = np.ascontiguousarray(targets, dtype=STATE)
         _check(lib().pmhWorldSetKinematicTargets(self._h, first, len(a), _ptr(a)),
                "pmhWorldSetKinematicTargets")
+
+    def add_joints(self, options):
+        """Appends joints (layouts.JOINT_OPTIONS records)."""
+        a = np.ascontiguousarray(options, dtype=JOINT_OPTIONS)
+        _check(lib().pmhWorldAddJoints(self._h, len(a), _ptr(a)), "pmhWorldAddJoints")
+
+    @property
+    def joint_count(self):
+        return lib().pmhWorldJointCount(self._h)
+
+    def joints(self):
+        """The joint records (layouts.JOINT) after the last step."""
+        out = np.zeros(self.joint_count, dtype=JOINT)
+        if len(out):
+            _check(lib().pmhWorldGetJoints(self._h, _ptr(out)), "pmhWorldGetJoints")
+        return out
 
     def step(self, steps=1):
         _check(lib().pmhWorldStep(self._h, steps), "pmhWorldStep")

@@ -73,8 +73,9 @@ class Scene:
     """Shapes (("mesh", v, i) | ("sphere", r) | ("capsule", r, h)), body
     parameters and initial states."""
 
-    def __init__(self, shapes, params, states, name=""):
+    def __init__(self, shapes, params, states, name="", joints=None):
         self.shapes, self.params, self.states, self.name = shapes, params, states, name
+        self.joints = joints
 
     @property
     def n(self):
@@ -89,6 +90,8 @@ class Scene:
             else:
                 world.add_capsule(shape[1], shape[2])
         world.add_bodies(self.params, self.states)
+        if self.joints is not None:
+            world.add_joints(self.joints)
         return world
 
 
@@ -447,3 +450,95 @@ def pusher(rows=5, seed=1):
                                             max_angular=REF_MAX_SPEED)])
     st = np.concatenate([base.states, states(1, np.array([[-rows / 2.0 - 1.5, 0.5, 0.0]], np.float32))])
     return Scene(base.shapes, p, st, "pusher")
+
+
+def joint_rig():
+    """Every joint type in one scene over a static floor: point and hinge
+    pendulums, a four-link chain, a welded pair under a falling cube, a
+    slider with a travel limit, a piston, a sprung cube, a motor and a
+    joint that breaks under its load.  Jointed pairs do not collide
+    (disable_collisions); the weld also joins a weld group."""
+    shapes = [("mesh",) + box_mesh(0.1, 0.1, 0.1), ("mesh",) + box_mesh(0.1, 0.5, 0.1),
+              ("mesh",) + box_mesh(8.0, 0.5, 8.0), ("mesh",) + box_mesh(0.25, 0.25, 0.25)]
+    motion, shape, mass, inertia, pos, rot = [], [], [], [], [], []
+
+    def body(m, sh, ms, p, q=(0, 0, 0, 1)):
+        motion.append(m)
+        shape.append(sh)
+        mass.append(ms)
+        half = {0: (0.1, 0.1, 0.1), 1: (0.1, 0.5, 0.1), 2: (8.0, 0.5, 8.0), 3: (0.25, 0.25, 0.25)}[sh]
+        inertia.append(box_inertia(ms, half))
+        pos.append(p)
+        rot.append(q)
+        return len(pos) - 1
+
+    S_, D_ = L.MOTION_STATIC, L.MOTION_DYNAMIC
+    floor = body(S_, 2, 1.0, (0, -0.5, 0))
+    anchor = body(S_, 0, 1.0, (0, 4.0, 0))
+    joints = []
+
+    def joint(kind, a, b, aa, ab, **kw):
+        o = L.joint_options(1, kind, a, b, anchor_a=aa, anchor_b=ab)
+        for k, v in kw.items():
+            o[k] = v
+        joints.append(o)
+        return o
+
+    def hanging_rod(x, angle, m=5.0):
+        c = (x + np.sin(angle) * 0.5, 4.0 - np.cos(angle) * 0.5, 0.0)
+        return body(D_, 1, m, c, quat((0, 0, 1), angle))
+
+    # pendulums
+    r = hanging_rod(-3.0, 0.6)
+    joint(L.JOINT_POINT, anchor, r, (-3.0, 0, 0), (0, 0.5, 0))
+    r = hanging_rod(-2.0, 0.4)
+    joint(L.JOINT_HINGE, anchor, r, (-2.0, 0, 0), (0, 0.5, 0))
+    # a four-link chain
+    prev, top = anchor, (0.0, 0.0, 0.0)
+    for k in range(4):
+        ang = 0.3
+        cx = 1.0 + k * np.sin(ang) * 1.0 + np.sin(ang) * 0.5
+        cy = 4.0 - k * np.cos(ang) * 1.0 - np.cos(ang) * 0.5
+        link = body(D_, 1, 2.0, (cx, cy, 0.0), quat((0, 0, 1), ang))
+        joint(L.JOINT_POINT, prev, link, (1.0, 0, 0) if k == 0 else (0, -0.5, 0), (0, 0.5, 0))
+        prev = link
+    # a welded pair resting on the floor, a cube dropping onto it
+    wa = body(D_, 3, 2.0, (-1.0, 0.25, 2.0))
+    wb = body(D_, 3, 2.0, (-0.5, 0.25, 2.0))
+    joint(L.JOINT_FIXED, wa, wb, (0.25, 0, 0), (-0.25, 0, 0))
+    body(D_, 3, 1.0, (-0.75, 1.0, 2.0), quat((1, 0, 1), 0.3))
+    # a slider along the floor's x axis with a travel limit
+    sl = body(D_, 3, 1.0, (2.0, 0.3, 2.0))
+    lim = np.zeros((), L.JOINT_OPTIONS)["linear_limits"]
+    lim["axes"] = 1
+    lim["lower"] = (-0.3, 0, 0)
+    lim["upper"] = (0.3, 0, 0)
+    joint(L.JOINT_SLIDER, floor, sl, (2.0, 0.8, 2.0), (0, 0, 0), linear_limits=lim)
+    # a piston hanging from the anchor: free travel and twist about x
+    pi = hanging_rod(3.0, 0.0, m=3.0)
+    joint(L.JOINT_PISTON, anchor, pi, (3.0, 0, 0), (0, 0.5, 0))
+    # a sprung cube under the anchor
+    sp = body(D_, 0, 1.0, (4.0, 3.0, 0.0))
+    spr = np.zeros((), L.JOINT_OPTIONS)["linear_springs"]
+    spr["axes"] = 7
+    spr["stiffness"] = (200.0, 200.0, 200.0)
+    spr["damping"] = (2.0, 2.0, 2.0)
+    joint(L.JOINT_GENERIC_SPRING, anchor, sp, (4.0, -0.8, 0), (0, 0, 0), linear_springs=spr)
+    # a motor turning a rod about its x axis
+    mo = hanging_rod(5.0, 0.0, m=2.0)
+    mot = np.zeros((), L.JOINT_OPTIONS)["motor"]
+    mot["angular_enabled"] = 1
+    mot["angular_target_velocity"] = 2.0
+    mot["angular_maximum_impulse"] = 5.0
+    joint(L.JOINT_MOTOR, anchor, mo, (5.0, 0, 0), (0, 0.5, 0), motor=mot)
+    # an overloaded joint: a heavy cube that tears loose
+    hv = body(D_, 3, 50.0, (-4.5, 3.4, 0.0))
+    joint(L.JOINT_POINT, anchor, hv, (-4.5, -0.35, 0), (0, 0.25, 0), breaking_impulse_threshold=2.0)
+    n = len(pos)
+    p = params(n, motion=np.array(motion, np.uint32), mass=np.array(mass, np.float32),
+               inertia=np.array(inertia, np.float32), shape=np.array(shape), margin=REF_MARGIN,
+               damping=REF_DAMPING, max_linear=REF_MAX_SPEED, max_angular=REF_MAX_SPEED)
+    st = states(n, np.array(pos, np.float32))
+    st["orientation"] = np.array(rot, np.float32)
+    st["linear_velocity"][sl] = (2.0, 0.0, 0.0)     # runs into the slider's limit
+    return Scene(shapes, p, st, name="joint_rig", joints=np.concatenate(joints))

@@ -196,6 +196,64 @@ def test_inputs_and_kinematic_targets_follow_the_oracle():
     assert_states_close(gs, os_, rtol=1e-4, atol=1e-4)
 
 
+def assert_joints_match(gj, oj, rtol=1e-5, atol=1e-5):
+    for f in ("enabled", "broken", "dense_a", "dense_b", "avbd_count", "avbd_next_a", "avbd_next_b"):
+        np.testing.assert_array_equal(gj[f], oj[f], err_msg=f)
+    np.testing.assert_allclose(gj["applied_impulse"], oj["applied_impulse"], rtol=rtol, atol=atol)
+    valid = np.arange(18)[None, :] < oj["avbd_count"][:, None]
+    for f in ("lambda", "penalty", "error", "a", "b"):
+        np.testing.assert_allclose(gj["rows"][f][valid], oj["rows"][f][valid], rtol=rtol, atol=atol, err_msg=f)
+
+
+@pytest.mark.parametrize("profile", [0, 1])
+def test_joint_rig_single_step(profile):
+    say(f"joint rig single step, profile {profile}")
+    scene = S.joint_rig()
+    g = gpu_world(scene, profile=profile, base_iterations=10)
+    o = oracle_world(scene, profile=profile, base_iterations=10)
+    g.step(1)
+    o.step(1)
+    np.testing.assert_array_equal(g.pairs(), o.pairs())
+    gb, ob = g.bodies(), o.bodies()
+    for f in ("color", "parent", "impact", "joint_head"):
+        np.testing.assert_array_equal(gb[f], ob[f], err_msg=f)
+    assert g.schedule()["budget"] == o.schedule()["budget"]
+    assert g.stats()["islands"] == o.stats()["islands"]
+    assert_states_close(g.states(), o.states())
+    assert_joints_match(g.joints(), o.joints())
+
+
+def test_joint_rig_trajectory():
+    say("joint rig, 90 steps, profile 0")
+    scene = S.joint_rig()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    for k in range(90):
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs(), err_msg=f"step {k}")
+    gj, oj = g.joints(), o.joints()
+    np.testing.assert_array_equal(gj["broken"], oj["broken"])
+    assert oj["broken"][-1] == 1
+    assert_states_close(g.states(), o.states(), rtol=1e-3, atol=1e-3)
+
+
+def test_weld_group_and_disabled_pairs_are_excluded():
+    say("joint collision exclusions")
+    scene = S.joint_rig()
+    g = gpu_world(scene, profile=0)
+    o = oracle_world(scene, profile=0)
+    g.step(2)
+    o.step(2)
+    gp = g.pairs()
+    np.testing.assert_array_equal(gp, o.pairs())
+    n = scene.n
+    codes = set(int(c) for c in gp)
+    for j in scene.joints:
+        a, b = int(j["body_a"][0]), int(j["body_b"][0])
+        assert a * n + b not in codes and b * n + a not in codes
+
+
 @pytest.mark.parametrize("lanes", [8, 16, 64])
 def test_lane_counts_follow_the_oracle(lanes):
     say(f"pyramid 5 steps with {lanes} lanes per body")
