@@ -351,20 +351,43 @@ __global__ void k_adj_sort(uint32_t n, const uint32_t *adj_start, uint32_t *adj)
     }
 }
 
-// Root of v with path halving: a non-root's parent is replaced by its
-// grandparent.  Parents only ever decrease and a non-root never becomes a
-// root again, so the write cannot undo a hook (hooks CAS roots only) and
-// every pointer still leads to the root; a racing halving may leave a
-// longer path, never a wrong one.
 __device__ uint32_t uf_find(uint32_t *parent, uint32_t v) {
     uint32_t p = ld_relaxed(&parent[v]);
-    while (p != v) {
-        const uint32_t g = ld_relaxed(&parent[p]);
-        if (g != p) st_relaxed(&parent[v], g);
-        v = p;
-        p = g;
-    }
+    while (p != v) { v = p; p = ld_relaxed(&parent[v]); }
     return v;
+}
+
+// Joins the trees of u and v: the larger root is hooked under the smaller
+// (parents only decrease), retrying when another hook moved a root first.
+__device__ void uf_union(uint32_t *parent, uint32_t u, uint32_t v) {
+    u = uf_find(parent, u);
+    v = uf_find(parent, v);
+    while (u != v) {
+        if (u > v) { const uint32_t t = u; u = v; v = t; }
+        const uint32_t old = atomicCAS(&parent[v], v, u);
+        if (old == v) break;
+        u = uf_find(parent, u);
+        v = uf_find(parent, old);
+    }
+}
+
+// First hooking pass over one edge per body: each dynamic body joins its
+// smallest-index dynamic neighbour.  After a compression the trees are
+// flat and cover most of every island, so the pass over all edges
+// (k_uf_hook) mostly finds both ends under one root at once.  The final
+// roots do not depend on the order of the hooks: each island's root is
+// its smallest index.
+__global__ void k_uf_sample(uint32_t n, const PmhBodyParameters *P, const uint2 *ab, const uint32_t *adj_start,
+                            const uint32_t *adj, uint32_t *parent) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || P[i].motion != kDynamic) return;
+    uint32_t best = kInvalid;
+    for (uint32_t e = adj_start[i]; e < adj_start[i + 1]; ++e) {
+        const uint2 p = ab[adj[e]];
+        const uint32_t j = p.x == i ? p.y : p.x;
+        if (j < best && P[j].motion == kDynamic) best = j;
+    }
+    if (best < i) uf_union(parent, i, best);
 }
 
 // Hook the larger root under the smaller one (parents only decrease), so
@@ -377,14 +400,7 @@ __global__ void k_uf_hook(const uint32_t *npairs, const PmhBodyParameters *P, co
         if (m_count[m] == 0) continue;
         const uint2 p = ab[m];
         if (P[p.x].motion != kDynamic || P[p.y].motion != kDynamic) continue;
-        uint32_t u = uf_find(parent, p.x), v = uf_find(parent, p.y);
-        while (u != v) {
-            if (u > v) { uint32_t t = u; u = v; v = t; }
-            uint32_t old = atomicCAS(&parent[v], v, u);
-            if (old == v) break;
-            u = uf_find(parent, u);
-            v = uf_find(parent, old);
-        }
+        uf_union(parent, p.x, p.y);
     }
 }
 
@@ -543,7 +559,7 @@ __global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParamete
         if (lane == 0) {
             if (chosen >= kMaxColors) { atomicOr(&ctr->status, kStatusColorOverflow); chosen = kMaxColors - 1; }
             st_relaxed(&color[i], chosen);
-            if (ld_relaxed(&ctr->colors) < chosen + 1) atomicMax(&ctr->colors, chosen + 1);
+            atomicMax(&ctr->colors, chosen + 1);
         }
     }
 }
@@ -1642,14 +1658,7 @@ __global__ void k_joint_hook(SolveArgs a, uint32_t *parent) {
     const PmhRigidConstraint &j = a.joints[k];
     if (j.avbd_count == 0u) return;
     if (a.P[j.dense_a].motion != kDynamic || a.P[j.dense_b].motion != kDynamic) return;
-    uint32_t u = uf_find(parent, j.dense_a), v = uf_find(parent, j.dense_b);
-    while (u != v) {
-        if (u > v) { uint32_t t = u; u = v; v = t; }
-        const uint32_t old = atomicCAS(&parent[v], v, u);
-        if (old == v) break;
-        u = uf_find(parent, u);
-        v = uf_find(parent, old);
-    }
+    uf_union(parent, j.dense_a, j.dense_b);
 }
 
 __global__ void k_joint_anchor(SolveArgs a, const uint32_t *parent, uint32_t *anchored) {
@@ -1879,6 +1888,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         k_adj_sort<<<bg, 256, 0, s>>>(n, v.jstart.p, v.jadj.p);
         k_joint_links<<<bg, 256, 0, s>>>(a, v.joint_head.p);
     }
+    k_uf_sample<<<bg, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.parent.p);
+    k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
     k_uf_hook<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.parent.p);
     if (v.nj) k_joint_hook<<<jg, 256, 0, s>>>(a, v.parent.p);
     k_uf_compress<<<bg, 256, 0, s>>>(n, v.parent.p);
