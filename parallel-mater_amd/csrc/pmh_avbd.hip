@@ -630,7 +630,8 @@ __global__ void k_bp_count(uint32_t n, const StepCounters *ctr, const uint32_t *
 
 __global__ void k_bp_fill(const StepCounters *ctr, const uint32_t *start, const uint32_t *bodies,
                           const uint32_t *adj_start, const uint32_t *adj, const uint32_t *m_count,
-                          const uint32_t *m_start, const uint2 *ab, const uint32_t *bp_start, uint2 *entries) {
+                          const uint32_t *m_start, const uint2 *ab, const uint32_t *bp_start, uint2 *entries,
+                          uint32_t *c_entry) {
     const uint32_t D = start[ctr->colors];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= D) return;
@@ -641,8 +642,29 @@ __global__ void k_bp_fill(const StepCounters *ctr, const uint32_t *start, const 
         const uint2 p = ab[m];
         const bool first = p.x == i;
         const uint32_t other = first ? p.y : p.x;
-        for (uint32_t q = 0; q < m_count[m]; ++q)
+        for (uint32_t q = 0; q < m_count[m]; ++q) {
+            c_entry[2 * (m_start[m] + q) + (first ? 0u : 1u)] = o;
             entries[o++] = make_uint2((m_start[m] + q) | (first ? kSideFirst : 0u), other);
+        }
+    }
+}
+
+// Stores a point's nine row records in its Jacobian slot and in the
+// colour-ordered slot of each dynamic endpoint.
+__device__ __forceinline__ void store_rows(float4 *jac, float4 *erec, size_t ecap, const uint32_t *c_entry,
+                                           uint32_t c, bool dyn_a, bool dyn_b, const float4 *J) {
+    float4 *dst = jac + 9 * (size_t)c;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q] = J[q];
+    if (dyn_a) {
+        const size_t e = c_entry[2 * c];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) erec[q * ecap + e] = J[q];
+    }
+    if (dyn_b) {
+        const size_t e = c_entry[2 * c + 1];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) erec[q * ecap + e] = J[q];
     }
 }
 
@@ -652,7 +674,8 @@ __global__ void k_bp_fill(const StepCounters *ctr, const uint32_t *start, const 
 __global__ void k_jacobians(const uint32_t *npts, const PmhBodyParameters *P, const uint32_t *c_pair,
                             const uint2 *ab, const float4 *tpos, const float4 *tq, const float4 *c_np,
                             const float4 *c_pt, const float4 *c_aa, const float4 *c_ab, const float4 *c_err,
-                            const float4 *c_acc, float4 *jac, float dt, uint32_t phase, const uint32_t *impact) {
+                            const float4 *c_acc, float4 *jac, float dt, uint32_t phase, const uint32_t *impact,
+                            float4 *erec, size_t ecap, const uint32_t *c_entry) {
     const uint32_t Pn = *npts;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < Pn; c += gridDim.x * blockDim.x) {
         const uint2 p = ab[c_pair[c]];
@@ -675,7 +698,7 @@ __global__ void k_jacobians(const uint32_t *npts, const PmhBodyParameters *P, co
         }
         const float mu = sqrtf(P[p.x].friction * P[p.y].friction);
         const float beta = pair_stiffness(P[p.x], P[p.y], dt) * 100.0f;
-        float4 *J = jac + 9 * (size_t)c;
+        float4 J[9];
         for (uint32_t ax = 0; ax < 3; ++ax) {
             const V3 ra = ax == 0 ? armA : ra_t;
             const V3 rb = ax == 0 ? armB : rb_t;
@@ -683,6 +706,7 @@ __global__ void k_jacobians(const uint32_t *npts, const PmhBodyParameters *P, co
             J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
             J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
         }
+        store_rows(jac, erec, ecap, c_entry, c, P[p.x].motion == kDynamic, P[p.y].motion == kDynamic, J);
     }
 }
 
@@ -698,6 +722,9 @@ struct SolveArgs {
     const uint32_t *color_bodies, *color_start, *bp_start;
     const uint2 *entries;
     const float4 *jac;
+    const float4 *erec;                  // contact rows in colour order (plane stride ecap)
+    size_t ecap;
+    const uint32_t *c_entry;
     float4 *c_lam, *c_pen, *c_aa, *c_ab, *c_acc, *c_rep, *c_err;
     const float4 *c_np, *c_pt;
     const uint32_t *c_pair;
@@ -767,11 +794,11 @@ __device__ __forceinline__ void apply_pose_update(float4 *disp, float4 *dqa, uin
 // kJoints: the world has joints.  Joint-free worlds run the false
 // instance, which carries none of the joint code (registers, stack).
 template <int L, bool kJoints>
-__device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, uint32_t gw, uint32_t nw) {
+__device__ void solve_color(const SolveArgs &a, uint32_t begin, uint32_t end, uint32_t phase, uint32_t gw,
+                            uint32_t nw) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t grp = lane / L, sub_lane = lane % L;
     constexpr uint32_t groups = 64 / L;
-    const uint32_t begin = a.color_start[color], end = a.color_start[color + 1];
     for (uint32_t base = begin + gw * groups; base < end; base += nw * groups) {
         const uint32_t k = base + grp;
         bool active = k < end;
@@ -814,11 +841,10 @@ __device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, 
             load_disp(a.disp, ent.y, dO);
             const float *dA = first ? dI : dO;
             const float *dB = first ? dO : dI;
-            const float4 *J = a.jac + 9 * (size_t)c;
             const float4 lam = a.c_lam[c], pen = a.c_pen[c];
             float4 Jx[9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q) Jx[q] = J[q];
+            for (int q = 0; q < 9; ++q) Jx[q] = a.erec[q * a.ecap + e];
             const float err0 = row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB);
             const float err1 = row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB);
             const float err2 = row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB);
@@ -856,8 +882,7 @@ __device__ void solve_color(const SolveArgs &a, uint32_t color, uint32_t phase, 
                 load_disp(a.disp, ent.y, dO);
                 const float *dA = first ? dI : dO;
                 const float *dB = first ? dO : dI;
-                const float4 *J = a.jac + 9 * (size_t)c;
-                const float4 axis = J[0], angA = J[3], angB = J[6];
+                const float4 axis = a.erec[e], angA = a.erec[3 * a.ecap + e], angB = a.erec[6 * a.ecap + e];
                 const float err = row_error(axis.w, axis, angA, angB, dA, dB);
                 const float lam0 = a.c_lam[c].x, k0 = a.c_pen[c].x;
                 const float4 ang = first ? angA : angB;
@@ -1474,14 +1499,17 @@ __device__ void velocity_jacobians(const SolveArgs &a, uint32_t tid, uint32_t nt
         const float4 acc = a.c_acc[c];
         const V3 af = {acc.y, acc.z, acc.w};
         const float ref[3] = {acc.x / a.dt, dot(af, t0) / a.dt, dot(af, t1) / a.dt};
-        float4 *J = const_cast<float4 *>(a.jac) + 9 * (size_t)c;
-        const float mu = J[6].w, beta = J[7].w;
+        const float4 *old = a.jac + 9 * (size_t)c;
+        const float mu = old[6].w, beta = old[7].w;
+        float4 J[9];
         for (uint32_t ax = 0; ax < 3; ++ax) {
             const V3 ra = ax == 0 ? armA : ra_t, rb = ax == 0 ? armB : rb_t;
             J[ax] = f4(axes[ax], e0[ax]);
             J[3 + ax] = f4(mul(cross(ra, axes[ax]), -1.0f), ref[ax]);
             J[6 + ax] = f4(cross(rb, axes[ax]), ax == 0 ? mu : (ax == 1 ? beta : 0.0f));
         }
+        store_rows(const_cast<float4 *>(a.jac), const_cast<float4 *>(a.erec), a.ecap, a.c_entry, c,
+                   a.P[p.x].motion == kDynamic, a.P[p.y].motion == kDynamic, J);
     }
 }
 
@@ -1524,7 +1552,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
         }
         for (uint32_t it = 0; it < a.budget; ++it) {
             for (uint32_t c = 0; c < colors; ++c) {
-                solve_color<L, false>(a, c, phase, gw, nw);
+                solve_color<L, false>(a, a.color_start[c], a.color_start[c + 1], phase, gw, nw);
                 grid.sync();
             }
             advance_duals(a, phase, it, tid, nthreads);
@@ -1544,9 +1572,9 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
 
 // Staged variant: one launch per stage (no grid barrier).
 template <int L, bool kJoints>
-__global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t color, uint32_t phase) {
+__global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t begin, uint32_t end, uint32_t phase) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    solve_color<L, kJoints>(a, color, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
+    solve_color<L, kJoints>(a, begin, end, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
 template <bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
@@ -1786,6 +1814,8 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     R(c_np, points); R(c_pt, points); R(c_aa, points); R(c_ab, points); R(c_err, points);
     R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
     R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
+    R(erec, 18 * (size_t)points); R(c_entry, 2 * (size_t)points);
+    ecap = 2 * (size_t)points;
     R(schedule, 1); R(color_count, kMaxColors + 1); R(color_start, kMaxColors + 1);
     if (!counters.p) {
         R(counters, 1);
@@ -1812,6 +1842,7 @@ void Solver::release() {
     m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
+    erec.release(); c_entry.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
     if (host_ctr) (void)hipHostFree(host_ctr);
@@ -1846,6 +1877,9 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.color_bodies = v.color_bodies.p; a.color_start = v.color_start.p; a.bp_start = v.bp_start.p;
     a.entries = reinterpret_cast<const uint2 *>(v.bp_entries.p);
     a.jac = v.jac.p;
+    a.erec = v.erec.p;
+    a.ecap = v.ecap;
+    a.c_entry = v.c_entry.p;
     a.c_lam = v.c_lam.p; a.c_pen = v.c_pen.p; a.c_aa = v.c_aa.p; a.c_ab = v.c_ab.p; a.c_acc = v.c_acc.p;
     a.c_rep = v.c_rep.p; a.c_err = v.c_err.p; a.c_np = v.c_np.p; a.c_pt = v.c_pt.p;
     a.c_pair = v.c_pair.p; a.ab = v.ab.p;
@@ -1915,9 +1949,11 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     r = exclusive_scan(s, v.deg.p, v.bp_start.p, n, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     k_bp_fill<<<bg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
-                                 v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p));
+                                 v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p),
+                                 v.c_entry.p);
     k_jacobians<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.tpos.p, v.tq.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
-                                    v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p);
+                                    v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p, v.erec.p, v.ecap,
+                                    v.c_entry.p);
     PMH_CHECK(hipGetLastError());
 
 
@@ -1970,16 +2006,17 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         const uint32_t phases = realtime ? 1u : (host.impact ? 2u : 1u);
         const bool joints = v.nj != 0;
         auto launch_color = [&](uint32_t blocks, uint32_t c, uint32_t phase) {
+            const uint32_t b = v.host_color_start[c], e = v.host_color_start[c + 1];
             if (joints) {
-                if (lanes == 64) k_stage_color<64, true><<<blocks, 256, 0, s>>>(a, c, phase);
-                else if (lanes == 32) k_stage_color<32, true><<<blocks, 256, 0, s>>>(a, c, phase);
-                else if (lanes == 8) k_stage_color<8, true><<<blocks, 256, 0, s>>>(a, c, phase);
-                else k_stage_color<16, true><<<blocks, 256, 0, s>>>(a, c, phase);
+                if (lanes == 64) k_stage_color<64, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else if (lanes == 32) k_stage_color<32, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else if (lanes == 8) k_stage_color<8, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else k_stage_color<16, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
             } else {
-                if (lanes == 64) k_stage_color<64, false><<<blocks, 256, 0, s>>>(a, c, phase);
-                else if (lanes == 32) k_stage_color<32, false><<<blocks, 256, 0, s>>>(a, c, phase);
-                else if (lanes == 8) k_stage_color<8, false><<<blocks, 256, 0, s>>>(a, c, phase);
-                else k_stage_color<16, false><<<blocks, 256, 0, s>>>(a, c, phase);
+                if (lanes == 64) k_stage_color<64, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else if (lanes == 32) k_stage_color<32, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else if (lanes == 8) k_stage_color<8, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                else k_stage_color<16, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
             }
         };
         uint32_t launches = 0;

@@ -153,6 +153,13 @@ struct Solver {
     uint32_t nj = 0;
     bool hinges = false;                 // some joint is a hinge (anchor projection)
     DBuf<uint32_t> jdeg, jstart, jcursor, jadj, joint_head;
+    // Contact rows in colour order: plane q (q = 0..8) of entry e at
+    // erec[q * ecap + e], the same nine float4 as the point's Jacobian record,
+    // so the lanes of a body read consecutive slots.  c_entry holds each
+    // point's entry on its first and second body.
+    DBuf<float4> erec;
+    DBuf<uint32_t> c_entry;
+    size_t ecap = 0;
     // pairs
     DBuf<uint32_t> code;                 // a*n+b
     DBuf<uint2> ab;
