@@ -152,6 +152,9 @@ typedef struct PmhWorldDesc {
                                         Sets the order in which a body's contact rows are summed. */
     uint32_t kernel_timing_stride;   /* every Nth step brackets each colour-sweep launch with HIP
                                         events (pmhWorldKernelTiming); 0 = off */
+    float residual_tolerance;        /* > 0: each solve phase iterates until the constraint residual
+                                        (pmhWorldGetResiduals) is at most this, the iteration budget
+                                        being the cap; 0 = the fixed budget of the profile */
 } PmhWorldDesc;
 
 #define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */
@@ -314,6 +317,17 @@ typedef struct PmhKernelTiming {
     uint64_t launches;
     double algorithmic_bytes;
 } PmhKernelTiming;
+/* Constraint residual after each dual update of the last step, per phase:
+ * RMS in out[phase * 64 + iteration] and maximum in maximum[...] (NULL to
+ * skip; NaN where no update ran), and the iterations each phase ran.  No
+ * counterpart in the reference, which runs a fixed budget
+ * (geometry_constraints.slang:1441-1444).  Each contact point contributes
+ * |C_n| if its normal force is positive, else max(C_n, 0), raised to |C_t|
+ * when its friction sticks; each joint the largest |C| of its hard equality
+ * rows; all at the displacements the update sees.  Squares are summed in
+ * 2^-40 fixed point (exact in any order), each clamped at r = 4 m. */
+PmhResult pmhWorldGetResiduals(PmhWorld world, float *out, float *maximum, uint32_t *executed);
+
 PmhResult pmhWorldKernelTiming(PmhWorld world, PmhKernelTiming *timing, int reset);
 
 /* Results of the last step in the reference layouts (for parity checks

@@ -1592,6 +1592,8 @@ float solve_contact_normal_row(const PmoContact &c, const PmoAvbdBody &A, const 
     return c.avbd_error.x + dot6(ja, da) + dot6(jb, db);
 }
 
+constexpr uint32_t kResidualSlots = 64;
+
 struct Ctx {
     const PmoParams *P;
     const PmoState *prev;
@@ -1606,6 +1608,15 @@ struct Ctx {
     uint32_t lanes;
     PmoJoint *J = nullptr;
     uint32_t nj = 0;
+    // Constraint residual after each dual update, per phase and iteration
+    // (kResidualSlots each; NaN = not run), and the tolerance of the
+    // to-residual mode (0: fixed budget).
+    float *residual = nullptr;           // RMS per phase and iteration
+    float *residual_max = nullptr;       // max per phase and iteration
+    float tolerance = 0.0f;
+    float last_residual = 0.0f;
+    uint32_t executed[2] = {0u, 0u};     // iterations run per phase
+    std::vector<uint8_t> stick{};        // per point of the last dual update
 };
 
 inline bool dyn(const Ctx &x, uint32_t i) { return x.P[i].motion == kDynamic; }
@@ -2175,10 +2186,90 @@ void solve_body(Ctx &x, uint32_t i, uint32_t phase, std::vector<PointRef> &pts) 
 }
 
 // advance_duals, geometry_constraints_solve.slang:492-566.
-void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
+// Residual of a joint's hard equality rows (locked axes, equal limits):
+// |C| at the current displacements.
+float joint_residual(const Ctx &x, const PmoJoint &j) {
+    if (j.state_enabled == 0u) return 0.0f;
+    const Vec6 da = disp(x.B[j.dense_a]), db = disp(x.B[j.dense_b]);
+    float r = 0.0f;
+    for (uint32_t k = 0; k < j.avbd_count; ++k) {
+        Row row = row_from(reinterpret_cast<const float *>(&j.rows[k]));
+        if (row.stiffness < kMaxPenalty || row.lower != -kMaxPenalty || row.upper != kMaxPenalty) continue;
+        r = fmax_(r, std::fabs(row.error + dot6(row.a, da) + dot6(row.b, db)));
+    }
+    return r;
+}
+
+// The impulses published after the last dual update and the re-anchoring
+// of sliding contacts (solve.slang:540-562), from the stored forces and
+// stick flags of that update.
+void publish_impulses(Ctx &x, uint32_t phase) {
+    size_t slot = 0;
     for (uint32_t active = 0; active < x.np; ++active) {
         uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
-        if (phase != 0u && x.B[a].impact == 0u && x.B[b].impact == 0u) continue;
+        PmoManifold &m = x.M[active];
+        const bool skip = phase != 0u && x.B[a].impact == 0u && x.B[b].impact == 0u;
+        for (uint32_t p = 0; p < m.count; ++p, ++slot) {
+            if (skip) continue;
+            PmoContact &c = m.contacts[p];
+            V3 t0, t1;
+            basis(c.normal, t0, t1);
+            c.accumulated_normal_impulse = c.avbd_dual[0].lambda * x.dt;
+            c.reported_normal_impulse = c.accumulated_normal_impulse;
+            c.accumulated_friction_impulse = mul(add(mul(t0, c.avbd_dual[1].lambda), mul(t1, c.avbd_dual[2].lambda)), x.dt);
+            c.reported_friction_impulse = c.accumulated_friction_impulse;
+            if (!x.stick[slot]) {
+                PmoState pa = phase != 0u ? x.B[a].target : avbd_pose(x.B[a]);
+                PmoState pb = phase != 0u ? x.B[b].target : avbd_pose(x.B[b]);
+                c.avbd_anchor_a = inv_rotate(pa.orientation, sub(c.point, pa.position));
+                c.avbd_anchor_b = inv_rotate(pb.orientation, sub(c.point, pb.position));
+            }
+        }
+    }
+}
+
+// Constraint residual of one contact point at the primal state the dual
+// update sees (no reference counterpart): an active normal row (force > 0)
+// must hold exactly, an inactive one must not penetrate, and a sticking
+// friction row must not slip.
+inline float contact_residual(const float err[3], float normal_force, bool stick) {
+    float r = normal_force > 0.0f ? std::fabs(err[0]) : fmax_(err[0], 0.0f);
+    if (stick) r = fmax_(r, fmax_(std::fabs(err[1]), std::fabs(err[2])));
+    return r;
+}
+
+// The residual of an update is the RMS of the per-point and per-joint
+// values (the tolerance test) and their maximum (reported).  Squares are
+// summed in 2^-40 fixed point (1e-6 m resolves), each clamped to 2^44
+// units (r = 4 m), so the sum is exact and order-free; a million rows at
+// the clamp still fit 64 bits.
+inline uint64_t residual_square(float r) {
+    const double q = double(r) * double(r) * 1099511627776.0 + 0.5;
+    return !(q < 17592186044416.0) ? uint64_t(17592186044416ull) : uint64_t(q);
+}
+inline float residual_rms(uint64_t sum, uint32_t count) {
+    return count ? float(std::sqrt(double(sum) / 1099511627776.0 / double(count))) : 0.0f;
+}
+struct ResidualSum {
+    uint64_t sum = 0;
+    uint32_t count = 0;
+    float max = 0.0f;
+    void add(float r) { sum += residual_square(r); ++count; max = fmax_(max, r); }
+};
+
+void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
+    // Points in manifold order; the stick flags feed publish_impulses.
+    size_t slot = 0;
+    ResidualSum residual;
+    const bool fixed_last = x.tolerance <= 0.0f && iteration + 1u == x.S->budget;
+    x.stick.clear();
+    for (uint32_t active = 0; active < x.np; ++active) x.stick.resize(x.stick.size() + x.M[active].count, 1u);
+    for (uint32_t active = 0; active < x.np; ++active) {
+        uint32_t a = x.pairs[active] / x.n, b = x.pairs[active] % x.n;
+        if (phase != 0u && x.B[a].impact == 0u && x.B[b].impact == 0u) {
+            slot += x.M[active].count;
+            continue;
+        }
         float mu = std::sqrt(x.P[a].friction * x.P[b].friction);
         float beta = pair_stiffness(x.P[a], x.P[b], x.dt) * 100.0f;
         PmoManifold &m = x.M[active];
@@ -2201,22 +2292,12 @@ void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
             for (uint32_t ax = 0; ax < 3; ++ax)
                 if (ax == 0 ? f.normal > 0.0f : stick)
                     c.avbd_dual[ax].penalty = fmin_(kMaxPenalty, c.avbd_dual[ax].penalty + beta * std::fabs(err[ax]));
-            if (iteration + 1u == x.S->budget) {
-                V3 t0, t1;
-                basis(c.normal, t0, t1);
-                c.accumulated_normal_impulse = f.normal * x.dt;
-                c.reported_normal_impulse = c.accumulated_normal_impulse;
-                c.accumulated_friction_impulse = mul(add(mul(t0, f.t0), mul(t1, f.t1)), x.dt);
-                c.reported_friction_impulse = c.accumulated_friction_impulse;
-                if (!stick) {
-                    PmoState pa = phase != 0u ? x.B[a].target : avbd_pose(x.B[a]);
-                    PmoState pb = phase != 0u ? x.B[b].target : avbd_pose(x.B[b]);
-                    c.avbd_anchor_a = inv_rotate(pa.orientation, sub(c.point, pa.position));
-                    c.avbd_anchor_b = inv_rotate(pb.orientation, sub(c.point, pb.position));
-                }
-            }
+            residual.add(contact_residual(err, f.normal, stick));
+            x.stick[slot + p] = stick ? 1u : 0u;
         }
+        slot += m.count;
     }
+    if (fixed_last) publish_impulses(x, phase);
     // Joints (solve.slang:566-614): dual updates, applied impulse, breaking.
     for (uint32_t k = 0; k < x.nj; ++k) {
         PmoJoint &j = x.J[k];
@@ -2243,6 +2324,12 @@ void advance_duals(Ctx &x, uint32_t phase, uint32_t iteration) {
             j.state_broken = 1u;
             j.state_enabled = 0u;
         }
+        residual.add(joint_residual(x, j));
+    }
+    x.last_residual = residual_rms(residual.sum, residual.count);
+    if (iteration < kResidualSlots && x.residual) {
+        x.residual[phase * kResidualSlots + iteration] = x.last_residual;
+        x.residual_max[phase * kResidualSlots + iteration] = residual.max;
     }
 }
 
@@ -2407,6 +2494,10 @@ int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t request
     for (uint32_t phase = 0; phase < phases; ++phase) {
         if (phase != 0u && x.S->counts[0] == 0u) break;
         if (phase != 0u) impact_stages(x);
+        // Fixed budget, or (to-residual mode) until the residual after a
+        // dual update is at most the tolerance, the budget being the cap;
+        // the impulses are then published from that update.
+        uint32_t executed = 0;
         for (uint32_t it = 0; it < x.S->budget; ++it) {
             for (uint32_t color = 0; color < x.S->remaining; ++color) {
                 for (uint32_t i = 0; i < x.n; ++i) {
@@ -2416,17 +2507,23 @@ int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t request
                 }
             }
             advance_duals(x, phase, it);
+            executed = it + 1u;
+            if (x.tolerance > 0.0f && (x.last_residual <= x.tolerance || executed == x.S->budget)) {
+                publish_impulses(x, phase);
+                break;
+            }
         }
+        x.executed[phase] = executed;
         free_motion(x, phase);
         apply_states(x, phase, realtime);
         if (phase == 0u) project_hinges(x);
-        x.S->counts[phase == 0u ? 6u : 7u] = x.S->budget;
-        x.S->maximum_passes += x.S->budget;
+        x.S->counts[phase == 0u ? 6u : 7u] = executed;
+        x.S->maximum_passes += executed;
         if (clock) {
             uint64_t bodies = 0;
             for (uint32_t i = 0; i < x.n; ++i)
                 bodies += dyn(x, i) && (phase == 0u || x.B[i].impact != 0u);
-            clock->body_iterations += bodies * x.S->budget;
+            clock->body_iterations += bodies * executed;
         }
     }
     if (clock) {
@@ -2471,6 +2568,9 @@ struct World {
     // One-step inputs (BodyAccumulator, KinematicTarget), cleared by every
     // step as pm_clear_rigid_inputs does (geometry.slang:2877-2886).
     std::vector<PmoJoint> joints;
+    std::vector<float> residual = std::vector<float>(2 * kResidualSlots, std::nanf(""));
+    std::vector<float> residual_max = std::vector<float>(2 * kResidualSlots, std::nanf(""));
+    uint32_t executed[2] = {0u, 0u};
     std::vector<std::array<V3, 4>> accumulators;   // force, torque, impulse, angular impulse
     std::vector<PmoState> targets;
     std::vector<uint8_t> target_active;
@@ -2779,7 +2879,14 @@ void world_step(World &w) {
           uint32_t(w.pairs.size()), w.bodies.data(), &w.schedule, dt, w.desc.lanes};
     x.J = w.joints.data();
     x.nj = uint32_t(w.joints.size());
+    std::fill(w.residual.begin(), w.residual.end(), std::nanf(""));
+    std::fill(w.residual_max.begin(), w.residual_max.end(), std::nanf(""));
+    x.residual = w.residual.data();
+    x.residual_max = w.residual_max.data();
+    x.tolerance = w.desc.residual_tolerance;
     prepare_solve(x, g, w.desc.base_iterations, w.desc.requested_iterations, w.desc.profile_version, &w.clock);
+    w.executed[0] = x.executed[0];
+    w.executed[1] = x.executed[1];
     for (uint32_t i = 0; i < n; ++i) {                               // pm_clamp_rigid_speeds, :3086-3100
         if (w.params[i].motion != kDynamic) continue;
         pred[i].linear_velocity = clamp_length(pred[i].linear_velocity, w.params[i].maximum_linear_speed);
@@ -2975,6 +3082,13 @@ int pmo_world_add_joints(void *world, uint32_t count, const PmoJointOptions *opt
         w.joints.push_back(j);
     }
     return 0;
+}
+void pmo_world_residuals(void *world, float *out, float *maximum, uint32_t *executed) {
+    World &w = *static_cast<World *>(world);
+    std::copy(w.residual.begin(), w.residual.end(), out);
+    std::copy(w.residual_max.begin(), w.residual_max.end(), maximum);
+    executed[0] = w.executed[0];
+    executed[1] = w.executed[1];
 }
 uint32_t pmo_world_joint_count(void *world) { return uint32_t(static_cast<World *>(world)->joints.size()); }
 void pmo_world_get_joints(void *world, PmoJoint *out) {

@@ -204,6 +204,8 @@ typedef struct PmoWorldDesc {
     uint32_t body_capacity;        /* 0: the body count at first step */
     uint32_t lanes;                /* accumulation order, see above */
     uint32_t flags;                /* bit 0: brute-force broadphase */
+    float residual_tolerance;      /* > 0: iterate each phase until the constraint residual
+                                      is at most this (the budget caps); 0: fixed budget */
 } PmoWorldDesc;
 
 typedef struct PmoWorldStats {
@@ -222,6 +224,10 @@ typedef struct PmoWorldStats {
 } PmoWorldStats;
 
 void *pmo_world_create(const PmoWorldDesc *desc);
+/* Constraint residual after each dual update of the last step: RMS in out[phase * 64 +
+ * iteration] and maximum in maximum[...] (NaN where no update ran), and the iterations
+ * run per phase. */
+void pmo_world_residuals(void *world, float *out, float *maximum, uint32_t *executed);
 void pmo_world_destroy(void *world);
 /* Returns the shape index or -1. Meshes run the reference's upload
  * preparation (runtime_deformables.cu.inc:4-304). */

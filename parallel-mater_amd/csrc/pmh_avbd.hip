@@ -722,6 +722,9 @@ struct SolveArgs {
     const uint32_t *color_bodies, *color_start, *bp_start;
     const uint2 *entries;
     const float4 *jac;
+    ResidualLog *log;                    // per-iteration constraint residual
+    uint32_t *c_flags;
+    float tol;                           // > 0: to-residual mode
     const float4 *erec;                  // contact rows in colour order (plane stride ecap)
     size_t ecap;
     const uint32_t *c_entry;
@@ -1269,10 +1272,92 @@ __device__ void advance_joint(const SolveArgs &a, uint32_t k, uint32_t phase) {
     }
 }
 
+// Residual of a joint's hard equality rows: |C| at the current
+// displacements.
+__device__ float joint_residual(const SolveArgs &a, const PmhRigidConstraint &j) {
+    if (j.avbd_count == 0u || j.state_enabled == 0u) return 0.0f;
+    float da[6], db[6];
+    load_disp(a.disp, j.dense_a, da);
+    load_disp(a.disp, j.dense_b, db);
+    float r = 0.0f;
+    for (uint32_t k = 0; k < j.avbd_count; ++k) {
+        const PmhAvbdRow &row = j.rows[k];
+        if (row.stiffness < kMaxPenalty || row.lower != -kMaxPenalty || row.upper != kMaxPenalty) continue;
+        r = smax(r, fabsf(row.error + dot6(row.a, da) + dot6(row.b, db)));
+    }
+    return r;
+}
+
+// Point flag: the point slid at the last dual update (to-residual mode,
+// where the impulses are published after the loop).
+constexpr uint32_t kFlagSlipping = 4u;
+
+// The impulses of the last dual update and the re-anchoring of a sliding
+// contact (solve.slang:540-562).
+__device__ void publish_point(const SolveArgs &a, uint32_t c, uint32_t phase, uint2 p, float fn, float ft0,
+                              float ft1, bool stick, V3 t0, V3 t1) {
+    const float an = fn * a.dt;
+    const V3 af = mul(add(mul(t0, ft0), mul(t1, ft1)), a.dt);
+    a.c_acc[c] = make_float4(an, af.x, af.y, af.z);
+    a.c_rep[c] = make_float4(an, af.x, af.y, af.z);
+    if (!stick) {
+        PmhRigidBodyState pa, pb;
+        if (phase != 0) {
+            pa.position = pv(v3(a.tpos[p.x])); pa.orientation = pq(q4(a.tq[p.x]));
+            pb.position = pv(v3(a.tpos[p.y])); pb.orientation = pq(q4(a.tq[p.y]));
+        } else {
+            pa = avbd_pose(a, p.x);
+            pb = avbd_pose(a, p.y);
+        }
+        const V3 point = v3(a.c_pt[c]);
+        const V3 na = inv_rotate(sq(pa.orientation), sub(point, sv(pa.position)));
+        const V3 nb = inv_rotate(sq(pb.orientation), sub(point, sv(pb.position)));
+        a.c_aa[c] = f4(na, a.c_aa[c].w);
+        a.c_ab[c] = f4(nb, 0.0f);
+    }
+}
+
+// Constraint residual of a contact point (no reference counterpart): an
+// active normal row must hold, an inactive one must not penetrate, and a
+// sticking friction row must not slip.
+__device__ __forceinline__ float contact_residual(const float *err, float fn, bool stick) {
+    float r = fn > 0.0f ? fabsf(err[0]) : smax(err[0], 0.0f);
+    if (stick) r = smax(r, smax(fabsf(err[1]), fabsf(err[2])));
+    return r;
+}
+
+// A lane's share of one update's residual: maximum, fixed-point sum of
+// squares and count of its points or joints.
+struct ResidualAcc {
+    float max = 0.0f;
+    unsigned long long sum = 0;
+    uint32_t count = 0;
+    __device__ void add(float r) { max = smax(max, r); sum += residual_square(r); ++count; }
+};
+
+// Combines the lanes of a converged wavefront into the log's slot with one
+// atomic per field (non-negative floats order like their bit patterns).
+__device__ __forceinline__ void residual_commit(ResidualLog *log, uint32_t slot, const ResidualAcc &acc) {
+    uint32_t bits = __float_as_uint(acc.max), count = acc.count;
+    unsigned long long sum = acc.sum;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        bits = max(bits, uint32_t(__shfl_xor(bits, d, 64)));
+        count += uint32_t(__shfl_xor(count, d, 64));
+        sum += (unsigned long long)__shfl_xor((long long)sum, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && count) {
+        atomicMax(&log->bits[slot], bits);
+        atomicAdd(&log->sum[slot], sum);
+        atomicAdd(&log->count[slot], count);
+    }
+}
+
 // advance_duals for contacts (solve.slang:492-566): one thread per point.
 __device__ void advance_duals(const SolveArgs &a, uint32_t phase, uint32_t iteration, uint32_t tid, uint32_t nthreads) {
     const uint32_t Pn = *a.npts;
-    const bool last = iteration + 1u == a.budget;
+    const bool last = a.tol <= 0.0f && iteration + 1u == a.budget;
+    ResidualAcc residual;
     for (uint32_t c = tid; c < Pn; c += nthreads) {
         const uint2 p = a.ab[a.c_pair[c]];
         if (phase != 0 && a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
@@ -1302,29 +1387,11 @@ __device__ void advance_duals(const SolveArgs &a, uint32_t phase, uint32_t itera
         }
         a.c_lam[c] = lam;
         a.c_pen[c] = pen;
-        if (last) {
-            const V3 t0 = v3(Jx[1]), t1 = v3(Jx[2]);
-            const float an = f.n * a.dt;
-            const V3 af = mul(add(mul(t0, f.t0), mul(t1, f.t1)), a.dt);
-            a.c_acc[c] = make_float4(an, af.x, af.y, af.z);
-            a.c_rep[c] = make_float4(an, af.x, af.y, af.z);
-            if (!stick) {
-                PmhRigidBodyState pa, pb;
-                if (phase != 0) {
-                    pa.position = pv(v3(a.tpos[p.x])); pa.orientation = pq(q4(a.tq[p.x]));
-                    pb.position = pv(v3(a.tpos[p.y])); pb.orientation = pq(q4(a.tq[p.y]));
-                } else {
-                    pa = avbd_pose(a, p.x);
-                    pb = avbd_pose(a, p.y);
-                }
-                const V3 point = v3(a.c_pt[c]);
-                const V3 na = inv_rotate(sq(pa.orientation), sub(point, sv(pa.position)));
-                const V3 nb = inv_rotate(sq(pb.orientation), sub(point, sv(pb.position)));
-                a.c_aa[c] = f4(na, a.c_aa[c].w);
-                a.c_ab[c] = f4(nb, 0.0f);
-            }
-        }
+        residual.add(contact_residual(err, f.n, stick));
+        if (last) publish_point(a, c, phase, p, f.n, f.t0, f.t1, stick, v3(Jx[1]), v3(Jx[2]));
+        else if (a.tol > 0.0f) a.c_flags[c] = (a.c_flags[c] & ~kFlagSlipping) | (stick ? 0u : kFlagSlipping);
     }
+    if (iteration < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + iteration, residual);
 }
 
 // free_mean (solve.slang:617-675): the mass-weighted mean translation of
@@ -1513,17 +1580,24 @@ __device__ void velocity_jacobians(const SolveArgs &a, uint32_t tid, uint32_t nt
     }
 }
 
+// Iterations a phase ran: the budget, or in to-residual mode the update
+// that met the tolerance.
+__device__ __forceinline__ uint32_t phase_iterations(const SolveArgs &a, uint32_t phase) {
+    return a.tol > 0.0f && a.log->executed[phase] != 0u ? a.log->executed[phase] : a.budget;
+}
+
 __device__ void account_phase(const SolveArgs &a, uint32_t phase) {
     const StepCounters *c = a.ctr;
     SolveTotals *t = a.totals;
+    const uint32_t its = phase_iterations(a, phase);
     if (phase == 0) {
         t->steps += 1;
-        t->body_iterations += (unsigned long long)c->dyn_bodies * a.budget;
-        t->algorithmic_bytes += a.budget * iteration_bytes(c->dyn_bodies, c->m1, c->m2, c->p1, c->p2);
+        t->body_iterations += (unsigned long long)c->dyn_bodies * its;
+        t->algorithmic_bytes += its * iteration_bytes(c->dyn_bodies, c->m1, c->m2, c->p1, c->p2);
     } else {
         t->velocity_phases += 1;
-        t->body_iterations += (unsigned long long)c->impact_bodies * a.budget;
-        t->algorithmic_bytes += a.budget * iteration_bytes(c->impact_bodies, c->m1i, c->m2i, c->p1i, c->p2i);
+        t->body_iterations += (unsigned long long)c->impact_bodies * its;
+        t->algorithmic_bytes += its * iteration_bytes(c->impact_bodies, c->m1i, c->m2i, c->p1i, c->p2i);
     }
 }
 
@@ -1563,6 +1637,7 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
         free_subtract_apply(a, phase, tid, nthreads);
         grid.sync();
         if (tid == 0) {
+            a.log->executed[phase] = a.budget;
             a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
             a.schedule->maximum_passes += a.budget;
             account_phase(a, phase);
@@ -1573,15 +1648,46 @@ __global__ void __launch_bounds__(256) k_solve_all(SolveArgs a, const uint32_t *
 // Staged variant: one launch per stage (no grid barrier).
 template <int L, bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t begin, uint32_t end, uint32_t phase) {
+    if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     solve_color<L, kJoints>(a, begin, end, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
 template <bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
+    if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
     advance_duals(a, phase, it, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-    if (kJoints)
-        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.nj; k += gridDim.x * blockDim.x)
+    if (kJoints) {
+        ResidualAcc r;
+        for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.nj; k += gridDim.x * blockDim.x) {
+            const PmhRigidConstraint &j = a.joints[k];
+            if (j.avbd_count == 0u || j.state_enabled == 0u) continue;
+            if (phase != 0u && a.impact[j.dense_a] == 0u && a.impact[j.dense_b] == 0u) continue;
             advance_joint(a, k, phase);
+            r.add(joint_residual(a, j));
+        }
+        if (it < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + it, r);
+    }
+}
+
+// To-residual mode: the phase stops at the first update whose residual is
+// at most the tolerance, or at the budget.
+__global__ void k_stage_check(SolveArgs a, uint32_t phase, uint32_t it) {
+    if (threadIdx.x != 0 || a.log->executed[phase] != 0u) return;
+    const uint32_t slot = phase * kResidualSlots + it;
+    const float r = it < kResidualSlots ? residual_rms(a.log->sum[slot], a.log->count[slot]) : 0.0f;
+    if (r <= a.tol || it + 1u == a.budget) a.log->executed[phase] = it + 1u;
+}
+
+// To-residual mode: impulses and re-anchoring from the last update.
+__global__ void k_stage_publish(SolveArgs a, uint32_t phase) {
+    const uint32_t Pn = *a.npts;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < Pn; c += gridDim.x * blockDim.x) {
+        const uint2 p = a.ab[a.c_pair[c]];
+        if (phase != 0 && a.impact[p.x] == 0 && a.impact[p.y] == 0) continue;
+        const float4 lam = a.c_lam[c];
+        const float4 *J = a.jac + 9 * (size_t)c;
+        publish_point(a, c, phase, p, lam.x, lam.y, lam.z, (a.c_flags[c] & kFlagSlipping) == 0u, v3(J[1]), v3(J[2]));
+    }
 }
 __global__ void k_stage_impact_gaps(SolveArgs a) {
     impact_gaps(a, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
@@ -1611,8 +1717,10 @@ __global__ void k_stage_mean(SolveArgs a, uint32_t phase) {
 __global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
     free_subtract_apply(a, phase, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.schedule->counts[phase == 0 ? 6 : 7] = a.budget;
-        a.schedule->maximum_passes += a.budget;
+        const uint32_t its = phase_iterations(a, phase);
+        if (a.log->executed[phase] == 0u) a.log->executed[phase] = its;
+        a.schedule->counts[phase == 0 ? 6 : 7] = its;
+        a.schedule->maximum_passes += its;
         account_phase(a, phase);
     }
 }
@@ -1722,6 +1830,15 @@ __global__ void k_project_hinges(SolveArgs a) {
     }
 }
 
+__global__ void k_residual_reset(ResidualLog *log) {
+    for (uint32_t k = threadIdx.x; k < 2 * kResidualSlots; k += blockDim.x) {
+        log->bits[k] = 0u;
+        log->sum[k] = 0ull;
+        log->count[k] = 0u;
+    }
+    if (threadIdx.x == 0) log->executed[0] = log->executed[1] = 0u;
+}
+
 __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32_t budget, uint32_t profile,
                                 uint32_t base_iterations) {
     if (threadIdx.x != 0) return;
@@ -1814,7 +1931,7 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     R(c_np, points); R(c_pt, points); R(c_aa, points); R(c_ab, points); R(c_err, points);
     R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
     R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
-    R(erec, 18 * (size_t)points); R(c_entry, 2 * (size_t)points);
+    R(erec, 18 * (size_t)points); R(c_entry, 2 * (size_t)points); R(residual, 1);
     ecap = 2 * (size_t)points;
     R(schedule, 1); R(color_count, kMaxColors + 1); R(color_start, kMaxColors + 1);
     if (!counters.p) {
@@ -1842,7 +1959,7 @@ void Solver::release() {
     m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
-    erec.release(); c_entry.release();
+    erec.release(); c_entry.release(); residual.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
     if (host_ctr) (void)hipHostFree(host_ctr);
@@ -1898,8 +2015,12 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.jstart = v.jstart.p;
     a.jadj = v.jadj.p;
     a.prev = previous;
+    a.log = v.residual.p;
+    a.tol = sp.tolerance;
+    a.c_flags = v.c_flags.p;
     const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj));
     k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
+    k_residual_reset<<<1, 128, 0, s>>>(v.residual.p);
     k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
                                    v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
                                    v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p, v.members.p, v.mean.p,
@@ -1957,7 +2078,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     PMH_CHECK(hipGetLastError());
 
 
-    if (sp.cooperative && v.nj == 0) {
+    if (sp.cooperative && v.nj == 0 && sp.tolerance <= 0.0f) {
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const void *kernel = sp.lanes == 64 ? reinterpret_cast<const void *>(k_solve_all<64>)
                            : sp.lanes == 32 ? reinterpret_cast<const void *>(k_solve_all<32>)
@@ -2052,6 +2173,14 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                 if (joints) k_stage_duals<true><<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 else k_stage_duals<false><<<pts_blocks, 256, 0, s>>>(a, phase, it);
                 launches += colors + 1;
+                if (sp.tolerance > 0.0f) {
+                    k_stage_check<<<1, 64, 0, s>>>(a, phase, it);
+                    launches += 1;
+                }
+            }
+            if (sp.tolerance > 0.0f) {
+                k_stage_publish<<<pts_blocks, 256, 0, s>>>(a, phase);
+                launches += 1;
             }
             if (host.mean_roots) {
                 k_stage_mean<<<(host.mean_roots + 3) / 4, 256, 0, s>>>(a, phase);

@@ -1866,7 +1866,8 @@ PmhResult World::step_once(uint32_t k_step) {
                                     v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_lam.p, v.c_pen.p, v.c_acc.p,
                                     v.c_flags.p, dt, ctr);
     mark(4);
-    SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes};
+    SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes,
+                   residual_tolerance};
     SolveTiming st;
     st.kernels = &kernel_timer;
     if (timing_enabled) {

@@ -134,6 +134,28 @@ struct DBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Constraint residual after each dual update of the step, 64 slots per
+// phase: the maximum (float bits), the sum of squares in 2^-40 fixed point
+// and the number of contributing points and joints; and the iterations
+// each phase ran.
+constexpr uint32_t kResidualSlots = 64;
+struct ResidualLog {
+    unsigned long long sum[2 * kResidualSlots];
+    uint32_t bits[2 * kResidualSlots];
+    uint32_t count[2 * kResidualSlots];
+    uint32_t executed[2];
+    uint32_t pad[2];
+};
+
+// Square of a residual in 2^-40 fixed point, clamped at 2^44 (r = 4 m).
+__host__ __device__ inline unsigned long long residual_square(float r) {
+    const double q = double(r) * double(r) * 1099511627776.0 + 0.5;
+    return !(q < 17592186044416.0) ? 17592186044416ull : (unsigned long long)q;
+}
+__host__ __device__ inline float residual_rms(unsigned long long sum, uint32_t count) {
+    return count ? float(sqrt(double(sum) / 1099511627776.0 / double(count))) : 0.0f;
+}
+
 // Everything AVBD needs after the narrowphase: per-body, per-pair and
 // per-point solver state.  Shared by the world step and the kernel seam.
 struct Solver {
@@ -160,6 +182,7 @@ struct Solver {
     DBuf<float4> erec;
     DBuf<uint32_t> c_entry;
     size_t ecap = 0;
+    DBuf<ResidualLog> residual;
     // pairs
     DBuf<uint32_t> code;                 // a*n+b
     DBuf<uint2> ab;
@@ -191,7 +214,9 @@ struct SolveParams {
     uint32_t base_iterations, requested_iterations, profile;
     uint32_t cooperative;                // one cooperative launch with grid barriers instead of per-stage launches
     uint32_t lanes;                      // lanes of a wavefront per body in the colour sweep: 8, 16, 32, 64
+    float tolerance = 0.0f;              // > 0: to-residual mode, the budget caps each phase
 };
+
 
 // Per-launch HIP-event timing of the colour-sweep kernel on sampled steps.
 // A step's events are read once the stream has passed them (at the next

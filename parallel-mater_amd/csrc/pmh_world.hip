@@ -535,6 +535,11 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
         return PMH_ERROR_INVALID_ARGUMENT;
     }
     w->solve_lanes = desc->solve_lanes ? desc->solve_lanes : pmh::kDefaultLanes;
+    if (!(desc->residual_tolerance >= 0.0f) || !std::isfinite(desc->residual_tolerance)) {
+        delete w;
+        return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    w->residual_tolerance = desc->residual_tolerance;
     w->kernel_timer.stride = desc->kernel_timing_stride;
     w->device = int(desc->device);
     w->timing_enabled = true;
@@ -840,6 +845,36 @@ PmhResult pmhWorldSolveTiming(PmhWorld world, float *milliseconds, uint32_t *lau
     if (milliseconds) *milliseconds = w->solve_timing.total_ms;
     if (launches) *launches = w->solve_timing.launches;
     if (reset) { w->solve_timing.total_ms = 0.0f; w->solve_timing.launches = 0; }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetResiduals(PmhWorld world, float *out, float *maximum, uint32_t *executed) {
+    if (!world || !out || !executed) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    const float nan = std::nanf("");
+    if (!w->has_stepped) {
+        std::fill(out, out + 2 * pmh::kResidualSlots, nan);
+        if (maximum) std::fill(maximum, maximum + 2 * pmh::kResidualSlots, nan);
+        executed[0] = executed[1] = 0u;
+        return PMH_SUCCESS;
+    }
+    pmh::ResidualLog log;
+    PMH_CHECK(hipStreamSynchronize(w->stream));
+    PMH_CHECK(hipMemcpy(&log, w->solver.residual.p, sizeof log, hipMemcpyDeviceToHost));
+    pmh::StepCounters c;
+    PMH_CHECK(hipMemcpy(&c, w->solver.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    // A phase that did not run (realtime profile, or no impact) reports 0.
+    const bool second = w->profile != 1u && c.impact != 0u;   // legacy profile with an impact
+    executed[0] = log.executed[0];
+    executed[1] = second ? log.executed[1] : 0u;
+    for (uint32_t ph = 0; ph < 2; ++ph)
+        for (uint32_t it = 0; it < pmh::kResidualSlots; ++it) {
+            const uint32_t k = ph * pmh::kResidualSlots + it;
+            float v;
+            std::memcpy(&v, &log.bits[k], sizeof v);
+            out[k] = it < executed[ph] ? pmh::residual_rms(log.sum[k], log.count[k]) : nan;
+            if (maximum) maximum[k] = it < executed[ph] ? v : nan;
+        }
     return PMH_SUCCESS;
 }
 

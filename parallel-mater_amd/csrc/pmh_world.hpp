@@ -13,6 +13,7 @@ struct World {
     uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
     bool cooperative = false;
     uint32_t solve_lanes = kDefaultLanes;
+    float residual_tolerance = 0.0f;
     int device = 0;
     hipStream_t stream = nullptr;
 

@@ -45,7 +45,7 @@ class WorldDesc(ctypes.Structure):
                 ("base_iterations", _U), ("requested_iterations", _U),
                 ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_body", _U),
                 ("device", _U), ("flags", _U), ("solve_lanes", _U),
-                ("kernel_timing_stride", _U)]
+                ("kernel_timing_stride", _U), ("residual_tolerance", _F)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
@@ -106,6 +106,7 @@ def lib():
         "pmhWorldSolveTiming": (i32, [_P, _P, _P, ctypes.c_int]),
         "pmhWorldGetTotals": (i32, [_P, _P, ctypes.c_int]),
         "pmhWorldKernelTiming": (i32, [_P, _P, ctypes.c_int]),
+        "pmhWorldGetResiduals": (i32, [_P, _P, _P, _P]),
         "pmhWorldPairCount": (_U, [_P]),
         "pmhWorldGetPairs": (i32, [_P, _P]),
         "pmhWorldGetManifolds": (i32, [_P, _P]),
@@ -151,11 +152,11 @@ class World:
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
                  pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
-                 solve_lanes=0, kernel_timing_stride=0):
+                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
                          points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0,
-                         solve_lanes, kernel_timing_stride)
+                         solve_lanes, kernel_timing_stride, residual_tolerance)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h
@@ -230,6 +231,18 @@ class World:
 
     def step(self, steps=1):
         _check(lib().pmhWorldStep(self._h, steps), "pmhWorldStep")
+
+    def residuals(self, maximum=False):
+        """(per-iteration RMS constraint residuals [2, 64], NaN where no
+        update ran; iterations run per phase) of the last step; with
+        maximum=True also the per-iteration maxima."""
+        out = np.zeros(128, dtype=np.float32)
+        mx = np.zeros(128, dtype=np.float32)
+        ex = np.zeros(2, dtype=np.uint32)
+        _check(lib().pmhWorldGetResiduals(self._h, _ptr(out), _ptr(mx), _ptr(ex)), "pmhWorldGetResiduals")
+        if maximum:
+            return out.reshape(2, 64), ex, mx.reshape(2, 64)
+        return out.reshape(2, 64), ex
 
     def states(self):
         out = np.zeros(self.body_count, dtype=STATE)

@@ -27,7 +27,8 @@ def _ptr(a):
 class WorldDesc(ctypes.Structure):
     _fields_ = [("gravity", _F * 3), ("dt", _F), ("profile_version", _U),
                 ("base_iterations", _U), ("requested_iterations", _U),
-                ("body_capacity", _U), ("lanes", _U), ("flags", _U)]
+                ("body_capacity", _U), ("lanes", _U), ("flags", _U),
+                ("residual_tolerance", _F)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact",
@@ -60,6 +61,7 @@ def lib():
         _lib.pmo_world_step.argtypes = [_P, _U]
         _lib.pmo_world_apply_inputs.argtypes = [_P, _U, _P]
         _lib.pmo_world_add_joints.argtypes = [_P, _U, _P]
+        _lib.pmo_world_residuals.argtypes = [_P, _P, _P, _P]
         _lib.pmo_world_joint_count.argtypes = [_P]
         _lib.pmo_world_joint_count.restype = _U
         _lib.pmo_world_get_joints.argtypes = [_P, _P]
@@ -116,10 +118,10 @@ class OracleWorld:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0,
-                 body_capacity=0, lanes=0, brute_force=False):
+                 body_capacity=0, lanes=0, brute_force=False, residual_tolerance=0.0):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, lanes,
-                         1 if brute_force else 0)
+                         1 if brute_force else 0, residual_tolerance)
         self._h = lib().pmo_world_create(ctypes.byref(desc))
         if not self._h:
             raise ValueError("invalid world description")
@@ -153,6 +155,18 @@ class OracleWorld:
         a = np.ascontiguousarray(options, dtype=L.JOINT_OPTIONS)
         if lib().pmo_world_add_joints(self._h, len(a), _ptr(a)) != 0:
             raise ValueError("invalid joint options")
+
+    def residuals(self, maximum=False):
+        """(per-iteration RMS residuals [2, 64] with NaN where no update
+        ran, iterations run per phase) of the last step; with maximum=True
+        also the per-iteration maxima."""
+        out = np.zeros(128, dtype=np.float32)
+        mx = np.zeros(128, dtype=np.float32)
+        ex = np.zeros(2, dtype=np.uint32)
+        lib().pmo_world_residuals(self._h, _ptr(out), _ptr(mx), _ptr(ex))
+        if maximum:
+            return out.reshape(2, 64), ex, mx.reshape(2, 64)
+        return out.reshape(2, 64), ex
 
     def joints(self):
         out = np.zeros(lib().pmo_world_joint_count(self._h), dtype=L.JOINT)

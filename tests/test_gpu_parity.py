@@ -383,3 +383,44 @@ def test_contact_rich_graph_matches_recomputation():
     colour = first_fit_colours(n, dynamic, zip(a[live].tolist(), b[live].tolist()))
     np.testing.assert_array_equal(bodies["color"][dynamic], colour[dynamic])
 
+
+def assert_residuals_equal(g, o):
+    gr, gx, gm = g.residuals(maximum=True)
+    orr, ox, om = o.residuals(maximum=True)
+    np.testing.assert_array_equal(gx, ox)
+    np.testing.assert_array_equal(np.isnan(gr), np.isnan(orr))
+    m = ~np.isnan(orr)
+    np.testing.assert_array_equal(gr[m].view(np.uint32), orr[m].view(np.uint32))
+    np.testing.assert_array_equal(gm[m].view(np.uint32), om[m].view(np.uint32))
+
+
+@pytest.mark.parametrize("scene_name", ["pyramid", "joint_rig"])
+def test_residuals_match_the_oracle(scene_name):
+    """The residual is a max over points and rows of values both sides
+    compute in the same order, so it is equal bit for bit."""
+    say(f"residual log, {scene_name}")
+    scene = S.pyramid(rows=6) if scene_name == "pyramid" else S.joint_rig()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    for k in range(30):
+        g.step(1)
+        o.step(1)
+        assert_residuals_equal(g, o)
+
+
+@pytest.mark.parametrize("scene_name", ["pyramid", "joint_rig"])
+def test_to_residual_mode_matches_the_oracle(scene_name):
+    say(f"to-residual mode (1e-4, cap 40), {scene_name}")
+    scene = S.pyramid(rows=6) if scene_name == "pyramid" else S.joint_rig()
+    g = gpu_world(scene, profile=0, base_iterations=40, residual_tolerance=1e-4)
+    o = oracle_world(scene, profile=0, base_iterations=40, residual_tolerance=1e-4)
+    ran = []
+    for k in range(40):
+        g.step(1)
+        o.step(1)
+        assert_residuals_equal(g, o)
+        ran.append(int(g.residuals()[1][0]))
+    assert_states_close(g.states(), o.states(), rtol=1e-4, atol=1e-4)
+    say(f"pose-phase iterations per step: {ran}")
+    assert min(ran) >= 1 and max(ran) <= 40
+
