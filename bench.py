@@ -82,7 +82,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=540)
     ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--workload", choices=["pile", "chains"], default="pile",
+                    help="pile: 32^3 boxes and spheres in a bin; chains: capsule chains with ball and hinge joints")
     ap.add_argument("--side", type=int, default=32)
+    ap.add_argument("--chains", type=int, default=4096, help="chain count of the chains workload (64 links each)")
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--profile", type=int, default=0, help="0 legacy (10 iterations), 1 realtime v1")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -110,10 +113,22 @@ def main():
     import pmamd
     import scenarios as S
 
-    scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
+    if args.workload == "chains":
+        scene = S.chains(count=args.chains, links=64)
+        dynamic = args.chains * 64
+        workload = (f"{args.chains} chains of 64 capsule links (r 0.05 m, length 0.3 m) joined by ball and hinge "
+                    f"joints in turn, every 8th link 100x heavier, each chain held by a ball joint to a static "
+                    f"anchor, swinging into each other onto a ground box")
+    else:
+        scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
+        dynamic = args.side ** 3
+        workload = (f"pile of {args.side ** 3} bodies (half boxes 0.2-0.6 m, half analytic spheres "
+                    f"r 0.1-0.3 m) dropped into a static 5-wall bin")
+    # Chains touch few bodies each: smaller pools (per body on average).
+    pools = dict(pairs_per_body=48, points_per_body=32) if args.workload == "chains" else {}
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
                                     cooperative=args.cooperative, solve_lanes=args.lanes,
-                                    kernel_timing_stride=KERNEL_TIMING_STRIDE))
+                                    kernel_timing_stride=KERNEL_TIMING_STRIDE, **pools))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     def advance(steps):
         try:
@@ -149,9 +164,10 @@ def main():
     else:
         per_launch_bytes = kt["algorithmic_bytes"] / kt["launches"]
         per_launch_s = kt["milliseconds"] / 1e3 / kt["launches"]
-        kernel = f"k_stage_color<{args.lanes or 32}, false> (colour sweep, one launch per colour per iteration)"
+        kernel = (f"k_stage_color<{args.lanes or 32}, {'true' if scene.joints is not None else 'false'}> "
+                  f"(colour sweep, one launch per colour per iteration)")
     achieved = per_launch_bytes / per_launch_s / 1e9
-    traffic = traffic_from_profiles()
+    traffic = traffic_from_profiles() if args.workload == "pile" and args.side == 32 else None
     name, cus = pmamd.device_info()
     line = {
         "metric": METRIC,
@@ -167,12 +183,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"pile of {args.side ** 3} bodies (half boxes 0.2-0.6 m, half analytic spheres "
-                        f"r 0.1-0.3 m) dropped into a static 5-wall bin, dt 1/60, "
+            "workload": f"{workload}, dt 1/60, "
                         f"{'legacy AVBD, ' + str(args.iterations) + ' iterations' if args.profile == 0 else 'realtime v1, 8 sweeps'}, "
                         f"steps {args.warmup}..{args.warmup + args.steps}",
             "bodies": scene.n,
-            "dynamic_bodies": args.side ** 3,
+            "dynamic_bodies": dynamic,
+            "joints": 0 if scene.joints is None else len(scene.joints),
             "iterations": args.iterations if args.profile == 0 else 8,
             "profile": "legacy" if args.profile == 0 else "realtime-v1",
             "solve_lanes": args.lanes or 32,
@@ -201,8 +217,17 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("# bench: cpu baseline (oracle, one thread)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(scene, start_states, args.cpu_seconds, args.profile,
-                                            args.iterations)
+        if args.workload == "chains":
+            # The oracle runs a 16-chain sample of the same construction
+            # from rest (the full workload takes minutes per oracle step).
+            sample = S.chains(count=16, links=64)
+            line["cpu_baseline"] = cpu_baseline(sample, sample.states, args.cpu_seconds, args.profile,
+                                                args.iterations)
+            line["cpu_baseline"]["sample"] = ("16 chains of the same construction from rest, serial oracle, "
+                                              "solve phase only; " + line["cpu_baseline"]["sample"].split("; ")[-1])
+        else:
+            line["cpu_baseline"] = cpu_baseline(scene, start_states, args.cpu_seconds, args.profile,
+                                                args.iterations)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

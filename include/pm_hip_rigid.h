@@ -333,6 +333,9 @@ PmhResult pmhWorldKernelTiming(PmhWorld world, PmhKernelTiming *timing, int rese
 /* Results of the last step in the reference layouts (for parity checks
  * and pmCmdCopyContactEvents-style consumers). */
 uint32_t pmhWorldPairCount(PmhWorld world);
+/* Pair codes a*n+b, ascending.  The world itself keys pairs by (a, b) and
+ * runs any body count; the 32-bit codes exist while n*n < 2^32, and for a
+ * larger world this call returns PMH_ERROR_CAPACITY_EXCEEDED. */
 PmhResult pmhWorldGetPairs(PmhWorld world, uint32_t *codes);
 PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *manifolds);
 PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *bodies);

@@ -1747,30 +1747,61 @@ __global__ void k_joint_init(SolveArgs a) {
     visit_joint(a, j, scratch, true, kVisitInitialize);
 }
 
-// Incident-joint counts, the joints' iteration floor, island unions and
-// anchoring by non-dynamic bodies.
-__global__ void k_joint_degree(SolveArgs a, uint32_t *jdeg, uint32_t raise_budget, StepCounters *ctr) {
+// The joints' iteration floor: the legacy profile without a requested
+// count raises the scene budget to the largest joint iteration count
+// (:1466-1470); one atomic per wavefront.
+__global__ void k_joint_budget(SolveArgs a, StepCounters *ctr) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nj) return;
-    const PmhRigidConstraint &j = a.joints[k];
-    if (j.avbd_count == 0u) return;
-    atomicAdd(&jdeg[j.dense_a], 1u);
-    atomicAdd(&jdeg[j.dense_b], 1u);
-    if (raise_budget) {
-        // Legacy profile without a requested count: the scene budget rises
-        // to the largest joint iteration count (:1466-1470).
-        atomicMax(&ctr->joint_budget, j.options.solver_iterations);
-        atomicMax(&a.schedule->budget, j.options.solver_iterations);
+    uint32_t want = 0;
+    if (k < a.nj && a.joints[k].avbd_count != 0u) want = a.joints[k].options.solver_iterations;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) want = max(want, uint32_t(__shfl_xor(want, d, 64)));
+    if ((threadIdx.x & 63) == 0 && want) {
+        atomicMax(&ctr->joint_budget, want);
+        atomicMax(&a.schedule->budget, want);
     }
 }
 
-__global__ void k_joint_fill(SolveArgs a, const uint32_t *jstart, uint32_t *jcursor, uint32_t *jadj) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= a.nj) return;
-    const PmhRigidConstraint &j = a.joints[k];
-    if (j.avbd_count == 0u) return;
-    jadj[jstart[j.dense_a] + atomicAdd(&jcursor[j.dense_a], 1u)] = k;
-    jadj[jstart[j.dense_b] + atomicAdd(&jcursor[j.dense_b], 1u)] = k;
+// Per-body lists of the joints with rows (geometry_constraints.slang:
+// 1462-1475): each body's candidate list, already in descending joint
+// index, filtered in order.  Pass 0 counts, pass 1 writes at the scanned
+// offsets.  One thread per body; bodies with long lists go to the
+// wavefront kernel below.
+template <int kPass>
+__global__ void k_joint_lists(SolveArgs a, const uint32_t *jlist_start, const uint32_t *jlist, uint32_t *jdeg,
+                              uint32_t *jadj) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t b = jlist_start[i], e = jlist_start[i + 1];
+    if (e - b > 64u) return;
+    uint32_t o = kPass == 1 ? a.jstart[i] : 0u;
+    for (uint32_t p = b; p < e; ++p) {
+        const uint32_t k = jlist[p];
+        if (a.joints[k].avbd_count == 0u) continue;
+        if (kPass == 1) jadj[o] = k;
+        ++o;
+    }
+    if (kPass == 0) jdeg[i] = o;
+}
+
+template <int kPass>
+__global__ void __launch_bounds__(64) k_joint_lists_heavy(SolveArgs a, const uint32_t *heavy, uint32_t nheavy,
+                                                          const uint32_t *jlist_start, const uint32_t *jlist,
+                                                          uint32_t *jdeg, uint32_t *jadj) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t h = blockIdx.x; h < nheavy; h += gridDim.x) {
+        const uint32_t i = heavy[h];
+        const uint32_t b = jlist_start[i], e = jlist_start[i + 1];
+        uint32_t o = kPass == 1 ? a.jstart[i] : 0u;
+        for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t p = base + lane;
+            const bool keep = p < e && a.joints[jlist[p]].avbd_count != 0u;
+            const uint64_t mask = __ballot(keep);
+            if (kPass == 1 && keep) jadj[o + __popcll(mask & ((1ull << lane) - 1ull))] = jlist[p];
+            o += uint32_t(__popcll(mask));
+        }
+        if (kPass == 0 && lane == 0) jdeg[i] = o;
+    }
 }
 
 // List links of the reference layout (push-front in joint order, so each
@@ -1809,24 +1840,51 @@ __global__ void k_joint_anchor(SolveArgs a, const uint32_t *parent, uint32_t *an
 
 // rigid_avbd_project_hinge_anchors (solve.slang:174-209): after the pose
 // phase, each enabled hinge moves its anchors together, split by inverse
-// mass, one joint after the other in index order (a single thread: chained
-// hinges share bodies).
-__global__ void k_project_hinges(SolveArgs a) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (uint32_t k = 0; k < a.nj; ++k) {
-        const PmhRigidConstraint &j = a.joints[k];
-        if (j.avbd_count == 0u || j.state_enabled == 0u || j.options.type != kJointHinge) continue;
-        const uint32_t ia = j.dense_a, ib = j.dense_b;
-        const float ima = a.P[ia].motion == kDynamic ? a.P[ia].inverse_mass : 0.0f;
-        const float imb = a.P[ib].motion == kDynamic ? a.P[ib].inverse_mass : 0.0f;
-        const float sum = ima + imb;
-        if (sum <= kEpsilon) continue;
-        const PmhRigidBodyState sa = a.states[ia], sb = a.states[ib];
-        const V3 pa = add(sv(sa.position), rotate(sq(sa.orientation), sv(j.options.local_anchor_a)));
-        const V3 pb = add(sv(sb.position), rotate(sq(sb.orientation), sv(j.options.local_anchor_b)));
-        const V3 e = sub(pb, pa);
-        a.states[ia].position = pv(add(sv(a.states[ia].position), mul(e, ima / sum)));
-        a.states[ib].position = pv(sub(sv(a.states[ib].position), mul(e, imb / sum)));
+// mass, one joint after the other in index order.  One wavefront walks the
+// joints 64 at a time: when no two hinges of a window share a body their
+// projections commute and run together; otherwise the window's hinges run
+// one after the other in index order.
+__device__ void project_hinge(const SolveArgs &a, const PmhRigidConstraint &j) {
+    const uint32_t ia = j.dense_a, ib = j.dense_b;
+    const float ima = a.P[ia].motion == kDynamic ? a.P[ia].inverse_mass : 0.0f;
+    const float imb = a.P[ib].motion == kDynamic ? a.P[ib].inverse_mass : 0.0f;
+    const float sum = ima + imb;
+    if (sum <= kEpsilon) return;
+    const PmhRigidBodyState sa = a.states[ia], sb = a.states[ib];
+    const V3 pa = add(sv(sa.position), rotate(sq(sa.orientation), sv(j.options.local_anchor_a)));
+    const V3 pb = add(sv(sb.position), rotate(sq(sb.orientation), sv(j.options.local_anchor_b)));
+    const V3 e = sub(pb, pa);
+    a.states[ia].position = pv(add(sv(sa.position), mul(e, ima / sum)));
+    a.states[ib].position = pv(sub(sv(sb.position), mul(e, imb / sum)));
+}
+
+__global__ void __launch_bounds__(64) k_project_hinges(SolveArgs a) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t base = 0; base < a.nj; base += 64) {
+        const uint32_t k = base + lane;
+        bool active = false;
+        uint32_t ia = kInvalid, ib = kInvalid;
+        if (k < a.nj) {
+            const PmhRigidConstraint &j = a.joints[k];
+            active = j.avbd_count != 0u && j.state_enabled != 0u && j.options.type == kJointHinge;
+            if (active) { ia = j.dense_a; ib = j.dense_b; }
+        }
+        // Does an earlier active hinge of the window share a body?
+        bool conflict = false;
+        for (uint32_t l = 0; l < 64; ++l) {
+            const uint32_t oa = __shfl(ia, l, 64), ob = __shfl(ib, l, 64);
+            if (l < lane && active && oa != kInvalid && (oa == ia || oa == ib || ob == ia || ob == ib))
+                conflict = true;
+        }
+        if (__ballot(conflict) == 0ull) {
+            if (active) project_hinge(a, a.joints[k]);
+        } else {
+            for (uint32_t l = 0; l < 64; ++l) {
+                if (lane == l && active) project_hinge(a, a.joints[k]);
+                __threadfence_block();
+            }
+        }
+        __threadfence_block();
     }
 }
 
@@ -2036,11 +2094,18 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     k_adj_sort<<<bg, 256, 0, s>>>(n, v.adj_start.p, v.adj.p);
     if (v.nj) {
         // Joint lists, geometry_constraints.slang:1462-1475.
-        k_joint_degree<<<jg, 256, 0, s>>>(a, v.jdeg.p, !realtime && sp.requested_iterations == 0u ? 1u : 0u, ctr);
+        if (!realtime && sp.requested_iterations == 0u) k_joint_budget<<<jg, 256, 0, s>>>(a, ctr);
+        const uint32_t hg = std::max<uint32_t>(1, std::min<uint32_t>(v.nheavy, 4096));
+        k_joint_lists<0><<<bg, 256, 0, s>>>(a, v.jlist_start, v.jlist, v.jdeg.p, v.jadj.p);
+        if (v.nheavy)
+            k_joint_lists_heavy<0><<<hg, 64, 0, s>>>(a, v.jheavy, v.nheavy, v.jlist_start, v.jlist, v.jdeg.p,
+                                                     v.jadj.p);
         r = exclusive_scan(s, v.jdeg.p, v.jstart.p, n, nullptr, nullptr, v.scan_tmp);
         if (r != PMH_SUCCESS) return r;
-        k_joint_fill<<<jg, 256, 0, s>>>(a, v.jstart.p, v.jcursor.p, v.jadj.p);
-        k_adj_sort<<<bg, 256, 0, s>>>(n, v.jstart.p, v.jadj.p);
+        k_joint_lists<1><<<bg, 256, 0, s>>>(a, v.jlist_start, v.jlist, v.jdeg.p, v.jadj.p);
+        if (v.nheavy)
+            k_joint_lists_heavy<1><<<hg, 64, 0, s>>>(a, v.jheavy, v.nheavy, v.jlist_start, v.jlist, v.jdeg.p,
+                                                     v.jadj.p);
         k_joint_links<<<bg, 256, 0, s>>>(a, v.joint_head.p);
     }
     k_uf_sample<<<bg, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.parent.p);

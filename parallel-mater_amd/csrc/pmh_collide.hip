@@ -1677,8 +1677,14 @@ __global__ void k_guard_points(StepCounters *ctr, uint32_t point_cap) {
 
 // pm_load_rigid_contact_cache (geometry.slang:2941-3009) against the
 // previous step's saved pairs, found by binary search on their sorted codes.
+// Cache key of a pair: (a, b) as one 64-bit number, ordered like the
+// reference's pair code a*n+b (ascending a, then b) for any body count.
+__device__ __forceinline__ unsigned long long pair_key(uint2 p) {
+    return ((unsigned long long)p.x << 32) | p.y;
+}
+
 __global__ void k_cache_load(const uint32_t *npairs, const uint32_t *code, const uint2 *ab, const uint32_t *m_count,
-                             const uint32_t *m_start, const PmhRigidBodyState *pred, const uint32_t *saved_code,
+                             const uint32_t *m_start, const PmhRigidBodyState *pred, const unsigned long long *saved_code,
                              const uint32_t *saved_count, const float4 *saved, const uint32_t *n_saved_ptr,
                              float4 *c_np, float4 *c_pt, float4 *c_aa, float4 *c_ab, float4 *c_lam,
                              float4 *c_pen, float4 *c_acc, uint32_t *c_flags, float dt, StepCounters *ctr) {
@@ -1687,7 +1693,7 @@ __global__ void k_cache_load(const uint32_t *npairs, const uint32_t *code, const
     uint32_t hits = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         if (m_count[m] == 0) continue;
-        const uint32_t key = code[m];
+        const unsigned long long key = pair_key(ab[m]);
         uint32_t lo = 0, hi = ns;
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
@@ -1738,7 +1744,7 @@ __global__ void k_cache_load(const uint32_t *npairs, const uint32_t *code, const
 __global__ void k_cache_save(const uint32_t *npairs, const uint32_t *code, const uint2 *ab, const uint32_t *m_count,
                              const uint32_t *m_start, const uint32_t *m_slot, const PmhRigidBodyState *states,
                              const float4 *c_np, const float4 *c_pt, const float4 *c_aa, const float4 *c_ab,
-                             const float4 *c_lam, const float4 *c_pen, const float4 *c_acc, uint32_t *saved_code,
+                             const float4 *c_lam, const float4 *c_pen, const float4 *c_acc, unsigned long long *saved_code,
                              uint32_t *saved_count, float4 *saved, uint32_t *n_saved, uint32_t cap,
                              const StepCounters *ctr) {
     const uint32_t M = *npairs;
@@ -1746,7 +1752,7 @@ __global__ void k_cache_save(const uint32_t *npairs, const uint32_t *code, const
         if (m_count[m] == 0) continue;
         const uint32_t s = m_slot[m];
         if (s >= cap) continue;
-        saved_code[s] = code[m];
+        saved_code[s] = pair_key(ab[m]);
         saved_count[s] = m_count[m];
         const PmhRigidBodyState body = states[ab[m].x];
         float4 *rec = saved + (size_t)s * 8 * kCacheFloat4PerPoint;

@@ -175,6 +175,12 @@ struct Solver {
     uint32_t nj = 0;
     bool hinges = false;                 // some joint is a hinge (anchor projection)
     DBuf<uint32_t> jdeg, jstart, jcursor, jadj, joint_head;
+    // Every body's candidate joints (alive, both ids valid) in descending
+    // joint index, built by the owner at upload; each step keeps the joints
+    // that initialised rows.  Bodies with more than 64 are listed apart
+    // (one wavefront each).
+    const uint32_t *jlist_start = nullptr, *jlist = nullptr, *jheavy = nullptr;
+    uint32_t nheavy = 0;
     // Contact rows in colour order: plane q (q = 0..8) of entry e at
     // erec[q * ecap + e], the same nine float4 as the point's Jacobian record,
     // so the lanes of a body read consecutive slots.  c_entry holds each

@@ -343,7 +343,7 @@ PmhResult World::ensure_device() {
     }
     if (bodies_dirty) {
         const uint32_t n = body_count;
-        if (uint64_t(n) * n > 0xffffffffull) return PMH_ERROR_CAPACITY_EXCEEDED;   // pair codes a*n+b
+        if (n > 0x7fffffffu) return PMH_ERROR_CAPACITY_EXCEEDED;
         // Per-body candidate lists hold up to pairs_per_body entries; the
         // pair and point pools are sized by their realistic totals.
         const uint32_t pair_cap = std::max<uint32_t>(256, n * pairs_per_body);
@@ -453,6 +453,42 @@ PmhResult World::upload_joints() {
         list[cursor[a]++] = make_uint2(b, k);
         list[cursor[b]++] = make_uint2(a, k);
     }
+    // Candidate joint lists: each body's alive joints with valid ids, in
+    // descending joint index (the reference pushes joints to the front of
+    // a body's list in index order).
+    std::vector<uint32_t> lstart(n + 1, 0u), lcursor;
+    for (const PmhRigidConstraint &j : host_joints) {
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
+        lstart[a]++;
+        lstart[b]++;
+    }
+    uint32_t ltotal = 0;
+    for (uint32_t i = 0; i <= n; ++i) { const uint32_t c = lstart[i]; lstart[i] = ltotal; ltotal += c; }
+    std::vector<uint32_t> llist(std::max<uint32_t>(1, ltotal));
+    lcursor.assign(lstart.begin(), lstart.end());
+    for (uint32_t k = nj; k-- > 0;) {
+        const PmhRigidConstraint &j = host_joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
+        llist[lcursor[a]++] = k;
+        llist[lcursor[b]++] = k;
+    }
+    std::vector<uint32_t> heavy;
+    for (uint32_t i = 0; i < n; ++i)
+        if (lstart[i + 1] - lstart[i] > 64u) heavy.push_back(i);
+    if ((r = jlist_start.reserve(n + 1)) || (r = jlist.reserve(llist.size())) ||
+        (r = jheavy.reserve(std::max<size_t>(1, heavy.size()))))
+        return r;
+    PMH_CHECK(hipMemcpy(jlist_start.p, lstart.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    PMH_CHECK(hipMemcpy(jlist.p, llist.data(), sizeof(uint32_t) * llist.size(), hipMemcpyHostToDevice));
+    if (!heavy.empty())
+        PMH_CHECK(hipMemcpy(jheavy.p, heavy.data(), sizeof(uint32_t) * heavy.size(), hipMemcpyHostToDevice));
+    solver.jlist_start = jlist_start.p;
+    solver.jlist = jlist.p;
+    solver.jheavy = jheavy.p;
+    solver.nheavy = uint32_t(heavy.size());
+    if ((r = solver.jadj.reserve(std::max<size_t>(1, ltotal)))) return r;
     excl_count = total;
     if ((r = excl_list.reserve(list.size()))) return r;
     PMH_CHECK(hipMemcpy(excl_start.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
@@ -480,6 +516,7 @@ void World::release() {
     nodes_dev.release(); leaves_dev.release(); shell_dev.release();
     inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
     joints_dev.release(); weld.release(); excl_start.release(); excl_list.release();
+    jlist_start.release(); jlist.release(); jheavy.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -918,6 +955,10 @@ uint32_t pmhWorldPairCount(PmhWorld world) {
 PmhResult pmhWorldGetPairs(PmhWorld world, uint32_t *codes) {
     if (!world || !codes) return PMH_ERROR_INVALID_ARGUMENT;
     const uint32_t m = pmhWorldPairCount(world);
+    // Codes a*n+b are the reference's 32-bit pair codes: defined while
+    // n*n fits 32 bits.
+    const uint64_t nb = W(world)->body_count;
+    if (nb * nb > 0xffffffffull) return PMH_ERROR_CAPACITY_EXCEEDED;
     if (m) PMH_CHECK(hipMemcpy(codes, W(world)->solver.code.p, sizeof(uint32_t) * m, hipMemcpyDeviceToHost));
     return PMH_SUCCESS;
 }

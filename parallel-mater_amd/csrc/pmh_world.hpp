@@ -51,6 +51,7 @@ struct World {
     // Collision exclusions of joints: weld roots (fixed joints that disable
     // collisions) and, per body, its disable_collisions partners with the
     // joint that excludes them.
+    DBuf<uint32_t> jlist_start, jlist, jheavy;   // candidate joint lists (descending index)
     DBuf<uint32_t> weld, excl_start;
     DBuf<uint2> excl_list;
     uint32_t weld_joints = 0, excl_count = 0;
@@ -66,7 +67,8 @@ struct World {
     uint32_t cache_capacity = 0;
     uint32_t leaf_slots = 0, leaf_limit = 512;   // triangle-path candidate slots and per-pair limit
     int cache_parity = 0;
-    DBuf<uint32_t> cache_code[2], cache_count[2];
+    DBuf<unsigned long long> cache_code[2];   // pair keys (a << 32 | b), ascending
+    DBuf<uint32_t> cache_count[2];
     DBuf<float4> cache_data[2];
     DBuf<uint32_t> cache_n;
 

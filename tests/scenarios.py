@@ -542,3 +542,88 @@ def joint_rig():
     st["orientation"] = np.array(rot, np.float32)
     st["linear_velocity"][sl] = (2.0, 0.0, 0.0)     # runs into the slider's limit
     return Scene(shapes, p, st, name="joint_rig", joints=np.concatenate(joints))
+
+
+def capsule_inertia(mass, radius, half_length):
+    """Solid capsule along local y (cylinder of length 2h plus two
+    hemispheres), principal moments (x, y, z)."""
+    r, h = float(radius), float(half_length)
+    vc = np.pi * r * r * 2.0 * h
+    vs = 4.0 / 3.0 * np.pi * r ** 3
+    mc = mass * vc / (vc + vs)
+    ms = mass - mc
+    iy = mc * r * r / 2.0 + ms * 2.0 * r * r / 5.0
+    ix = mc * (3.0 * r * r + 4.0 * h * h) / 12.0 + ms * (2.0 * r * r / 5.0 + h * h + 3.0 * h * r / 4.0)
+    return (np.float32(ix), np.float32(iy), np.float32(ix))
+
+
+def chains(count=4, links=64, spacing=0.45, height=None, seed=4, radius=0.05, length=0.3,
+           heavy_every=8, heavy_ratio=100.0, density=1000.0, max_tilt=0.35):
+    """`count` chains (a square grid of anchors, `spacing` apart) of
+    `links` capsule links (radius 0.05 m, total length 0.3 m) joined end to
+    end by ball (point) and hinge joints in turn, the first link held by a
+    ball joint to a static anchor body high above; every `heavy_every`-th
+    link is `heavy_ratio` times heavier.  Each chain starts straight but
+    tilted by up to `max_tilt` rad in a seeded direction, so the chains
+    swing into each other and, hanging down, reach a static ground box
+    (the anchors sit at 0.99 of a chain's length).  Reference
+    default body options.  Bodies: 0 ground, 1 anchor, then the links chain
+    by chain; joint j of chain c joins link j-1 (or the anchor) and link j."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(count)))
+    if height is None:
+        height = 0.99 * links * length              # a chain swung down reaches the ground
+    h = length / 2.0 - radius                       # capsule half segment
+    half_link = length / 2.0
+    extent = side * spacing
+    shapes = [("mesh",) + box_mesh(extent / 2.0 + 25.0, 0.5, extent / 2.0 + 25.0),
+              ("mesh",) + box_mesh(0.05, 0.05, 0.05), ("capsule", radius, h)]
+    anchor_pos = np.array([0.0, height + 5.0, 0.0], np.float32)
+    n = 2 + count * links
+    pos = np.zeros((n, 3), np.float64)
+    rot = np.zeros((n, 4), np.float32)
+    rot[:, 3] = 1.0
+    pos[0] = (0.0, -0.5, 0.0)
+    pos[1] = anchor_pos
+    motion = np.full(n, L.MOTION_DYNAMIC, np.uint32)
+    motion[:2] = L.MOTION_STATIC
+    shape = np.full(n, 2)
+    shape[:2] = (0, 1)
+    base_mass = density * (np.pi * radius ** 2 * 2.0 * h + 4.0 / 3.0 * np.pi * radius ** 3)
+    k = np.arange(links)
+    link_mass = np.where((k + 1) % heavy_every == 0, base_mass * heavy_ratio, base_mass)
+    mass = np.ones(n, np.float64)
+    mass[2:] = np.tile(link_mass, count)
+    inertia = np.ones((n, 3), np.float32)
+    light, heavy = capsule_inertia(base_mass, radius, h), capsule_inertia(base_mass * heavy_ratio, radius, h)
+    inertia[2:] = np.tile(np.where(((k + 1) % heavy_every == 0)[:, None], heavy, light), (count, 1))
+    jo = L.joint_options(count * links, np.tile(np.where(k % 2 == 0, L.JOINT_POINT, L.JOINT_HINGE), count),
+                         0, 0, anchor_a=(0, -half_link, 0), anchor_b=(0, half_link, 0))
+    for c in range(count):
+        gx, gz = c % side, c // side
+        top = np.array([(gx - (side - 1) / 2.0) * spacing, height, (gz - (side - 1) / 2.0) * spacing])
+        tilt = rng.uniform(0.0, max_tilt)
+        heading = rng.uniform(0.0, 2.0 * np.pi)
+        q = quat(np.array([np.cos(heading), 0.0, np.sin(heading)]), tilt)   # tilts the chain's down direction
+        down = _rotate_np(q, np.array([0.0, -1.0, 0.0]))
+        first = 2 + c * links
+        pos[first:first + links] = top[None, :] + down[None, :] * ((2.0 * k + 1.0) * half_link)[:, None]
+        rot[first:first + links] = q
+        j = slice(c * links, (c + 1) * links)
+        jo["body_a"][j, 0] = np.concatenate([[1], first + k[:-1]])
+        jo["body_b"][j, 0] = first + k
+        # the first joint holds the top link to the anchor body, whose frame
+        # is turned with the chain so that the rest pose is the joint's zero
+        jo["local_anchor_a"][c * links] = top - anchor_pos
+        jo["local_orientation_a"][c * links] = q
+    p = params(n, motion=motion, mass=mass.astype(np.float32), inertia=inertia, shape=shape, margin=REF_MARGIN,
+               damping=REF_DAMPING, max_linear=REF_MAX_SPEED, max_angular=REF_MAX_SPEED)
+    st = states(n, pos.astype(np.float32))
+    st["orientation"] = rot
+    return Scene(shapes, p, st, name=f"chains{count}x{links}", joints=jo)
+
+
+def _rotate_np(q, v):
+    u = np.asarray(q[:3], np.float64)
+    t = 2.0 * np.cross(u, v)
+    return v + q[3] * t + np.cross(u, t)

@@ -424,3 +424,52 @@ def test_to_residual_mode_matches_the_oracle(scene_name):
     say(f"pose-phase iterations per step: {ran}")
     assert min(ran) >= 1 and max(ran) <= 40
 
+
+def joint_gaps(scene, states):
+    o = scene.joints
+    qa = states["orientation"][o["body_a"][:, 0]].astype(np.float64)
+    qb = states["orientation"][o["body_b"][:, 0]].astype(np.float64)
+
+    def rot(q, v):
+        u = q[:, :3]
+        t = 2.0 * np.cross(u, v)
+        return v + q[:, 3:4] * t + np.cross(u, t)
+    pa = states["position"][o["body_a"][:, 0]] + rot(qa, o["local_anchor_a"].astype(np.float64))
+    pb = states["position"][o["body_b"][:, 0]] + rot(qb, o["local_anchor_b"].astype(np.float64))
+    return np.linalg.norm(pa - pb, axis=1)
+
+
+def test_chains_single_step_and_trajectory():
+    """Four 64-link capsule chains (ball and hinge joints in turn, heavy
+    links): one step to the single-step tolerance; 20 steps with equal pair
+    sets at every step and states within 1e-3; then, at step 60, the
+    aggregates the long-run criterion names (joint drift and kinetic
+    energy) within 10% -- the chains whip, so ulp differences of the
+    rotation logarithms' atan2f/sinf grow into millimetres by then."""
+    say("4 chains x 64 capsule links")
+    scene = S.chains(count=4, links=64)
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    g.step(1)
+    o.step(1)
+    gb, ob = g.bodies(), o.bodies()
+    for f in ("color", "parent", "impact", "joint_head"):
+        np.testing.assert_array_equal(gb[f], ob[f], err_msg=f)
+    assert_states_close(g.states(), o.states())
+    assert_joints_match(g.joints(), o.joints())
+    for k in range(19):
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs(), err_msg=f"step {k + 2}")
+    assert_states_close(g.states(), o.states(), rtol=1e-3, atol=1e-3)
+    g.step(40)
+    o.step(40)
+    gs, os_ = g.states(), o.states()
+    m = scene.params["inverse_mass"][2:]
+    ke = [float(np.sum(0.5 / m * np.sum(s["linear_velocity"][2:].astype(np.float64) ** 2, axis=1)))
+          for s in (gs, os_)]
+    dg, do = joint_gaps(scene, gs), joint_gaps(scene, os_)
+    say(f"step 60: kinetic energy {ke[0]:.4g} / {ke[1]:.4g} J, mean joint gap {dg.mean():.3g} / {do.mean():.3g} m")
+    assert abs(ke[0] - ke[1]) <= 0.1 * ke[1]
+    assert abs(dg.mean() - do.mean()) <= 0.1 * do.mean()
+
