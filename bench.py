@@ -29,6 +29,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for sub in ("parallel-mater_amd", "tests"):
     path = os.path.join(ROOT, sub)
@@ -82,8 +84,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=540)
     ap.add_argument("--warmup", type=int, default=60)
-    ap.add_argument("--workload", choices=["pile", "chains"], default="pile",
-                    help="pile: 32^3 boxes and spheres in a bin; chains: capsule chains with ball and hinge joints")
+    ap.add_argument("--workload", choices=["pile", "chains", "batch"], default="pile",
+                    help="pile: 32^3 boxes and spheres in a bin; chains: capsule chains with ball and hinge joints; "
+                         "batch: independent scenes of 8-512 bodies in one world")
+    ap.add_argument("--scenes", type=int, default=4096, help="scene count of the batch workload")
     ap.add_argument("--side", type=int, default=32)
     ap.add_argument("--chains", type=int, default=4096, help="chain count of the chains workload (64 links each)")
     ap.add_argument("--iterations", type=int, default=10)
@@ -113,7 +117,12 @@ def main():
     import pmamd
     import scenarios as S
 
-    if args.workload == "chains":
+    if args.workload == "batch":
+        scene = S.batch(count=args.scenes, seed=7)
+        dynamic = scene.n - args.scenes
+        workload = (f"{args.scenes} independent scenes in one world, bodies per scene log-uniform in 8-512 "
+                    f"(half boxes, half analytic spheres, pile materials), drops and stacks on a ground box each")
+    elif args.workload == "chains":
         scene = S.chains(count=args.chains, links=64)
         dynamic = args.chains * 64
         workload = (f"{args.chains} chains of 64 capsule links (r 0.05 m, length 0.3 m) joined by ball and hinge "
@@ -125,11 +134,19 @@ def main():
         workload = (f"pile of {args.side ** 3} bodies (half boxes 0.2-0.6 m, half analytic spheres "
                     f"r 0.1-0.3 m) dropped into a static 5-wall bin")
     # Chains touch few bodies each: smaller pools (per body on average).
-    pools = dict(pairs_per_body=48, points_per_body=32) if args.workload == "chains" else {}
+    pools = {"chains": dict(pairs_per_body=48, points_per_body=32),
+             "batch": dict(pairs_per_body=40, points_per_body=40)}.get(args.workload, {})
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
                                     cooperative=args.cooperative, solve_lanes=args.lanes,
                                     kernel_timing_stride=KERNEL_TIMING_STRIDE, **pools))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
+    tuned = None
+    if args.workload == "batch" and not args.lanes:
+        # Lane tile of the colour sweep picked by timing each on this world
+        # (part of the warm-up).
+        world.step(max(1, args.warmup // 2))
+        tuned = pmamd.autotune_lanes(world, steps=4)
+        print(f"# bench: autotuned lane tile {tuned[0]} ({tuned[1]})", file=sys.stderr, flush=True)
     def advance(steps):
         try:
             world.step(steps)
@@ -145,8 +162,17 @@ def main():
     world.solve_timing(reset=True)
     world.kernel_timing(reset=True)
     pmamd.lib().pmhDeviceSynchronize()
+    latencies = []
     t0 = time.perf_counter()
-    advance(args.steps)
+    if args.workload == "batch":
+        # One batch step per call: its latency as a caller sees it.
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            advance(1)
+            pmamd.lib().pmhDeviceSynchronize()
+            latencies.append((time.perf_counter() - t1) * 1e3)
+    else:
+        advance(args.steps)
     pmamd.lib().pmhDeviceSynchronize()
     wall = time.perf_counter() - t0
     totals = world.totals()
@@ -164,7 +190,8 @@ def main():
     else:
         per_launch_bytes = kt["algorithmic_bytes"] / kt["launches"]
         per_launch_s = kt["milliseconds"] / 1e3 / kt["launches"]
-        kernel = (f"k_stage_color<{args.lanes or 32}, {'true' if scene.joints is not None else 'false'}> "
+        lanes_used = args.lanes or (tuned[0] if tuned else 32)
+        kernel = (f"k_stage_color<{lanes_used}, {'true' if scene.joints is not None else 'false'}> "
                   f"(colour sweep, one launch per colour per iteration)")
     achieved = per_launch_bytes / per_launch_s / 1e9
     traffic = traffic_from_profiles() if args.workload == "pile" and args.side == 32 else None
@@ -191,7 +218,7 @@ def main():
             "joints": 0 if scene.joints is None else len(scene.joints),
             "iterations": args.iterations if args.profile == 0 else 8,
             "profile": "legacy" if args.profile == 0 else "realtime-v1",
-            "solve_lanes": args.lanes or 32,
+            "solve_lanes": args.lanes or (tuned[0] if tuned else 32),
             "device": f"{name} ({cus} CUs)",
         },
         "solve_phase_body_iters_per_s": body_iters / (solve_ms / 1e3),
@@ -201,6 +228,11 @@ def main():
         "velocity_phases": totals["velocity_phases"],
         "solve_launches_per_step": launches / max(1, args.steps),
         "last_step": {"phase_ms": {k: round(v, 4) for k, v in phases.items()}, **stats},
+        **({"scenes": args.scenes, "scene_steps_per_s": args.scenes * args.steps / wall,
+            "batch_latency_ms": {"p50": float(np.percentile(latencies, 50)),
+                                 "p99": float(np.percentile(latencies, 99))},
+            "autotune_ms_per_step": tuned[1] if tuned else None}
+           if args.workload == "batch" else {}),
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -217,7 +249,30 @@ def main():
     }
     if not args.no_cpu_baseline:
         print("# bench: cpu baseline (oracle, one thread)", file=sys.stderr, flush=True)
-        if args.workload == "chains":
+        if args.workload == "batch":
+            # The oracle steps a sample of 8 of the scenes, one after the
+            # other, from rest.
+            sample = S.Batch(scene.shapes, scene.scenes[:8], "sample")
+            total = {"steps": 0, "body_iterations": 0, "solve_ms": 0.0, "step_ms": 0.0}
+            import _oracle as O
+            start = time.process_time()
+            for s_ in range(8):
+                w = sample.build_one(O.OracleWorld(profile=args.profile, base_iterations=args.iterations, lanes=0), s_)
+                w.timing(reset=True)
+                for _ in range(30):
+                    w.step(1)
+                t = w.timing()
+                for k in total:
+                    total[k] += t[k] if k != "steps" else 30
+                if time.process_time() - start > args.cpu_seconds:
+                    break
+            line["cpu_baseline"] = {
+                "value": total["body_iterations"] / (total["solve_ms"] / 1e3), "unit": "body-iters/s",
+                "cores": 1, "kind": "port",
+                "sample": f"the first {s_ + 1} scenes of the batch, 30 steps each from rest, one after the other, "
+                          f"serial oracle, solve phase only",
+                "scene_steps_per_s": total["steps"] / (total["step_ms"] / 1e3)}
+        elif args.workload == "chains":
             # The oracle runs a 16-chain sample of the same construction
             # from rest (the full workload takes minutes per oracle step).
             sample = S.chains(count=16, links=64)

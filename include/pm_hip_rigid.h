@@ -203,6 +203,21 @@ PmhResult pmhWorldAddBodies(PmhWorld world, uint32_t count,
                             const PmhRigidBodyState *states);
 uint32_t pmhWorldBodyCount(PmhWorld world);
 
+/* Batched scenes (no counterpart in the reference, whose simulations each
+ * own one scene): appends a scene of `count` bodies as a new contiguous body
+ * range and returns its index.  Bodies of different scenes never pair, so
+ * every scene steps as it would alone, with its own body count for the
+ * triangle-path slot and limit rules.  Plain pmhWorldAddBodies appends to
+ * the last scene. */
+PmhResult pmhWorldAddScene(PmhWorld world, uint32_t count, const PmhBodyParameters *parameters,
+                           const PmhRigidBodyState *states, uint32_t *scene);
+uint32_t pmhWorldSceneCount(PmhWorld world);
+
+/* Lanes per body of the colour sweep for the following steps (8, 16, 32 or
+ * 64): the tile the autotuner picks.  It sets the order in which a body's
+ * contact rows are summed. */
+PmhResult pmhWorldSetSolveLanes(PmhWorld world, uint32_t lanes);
+
 /* pmCmdStepSimulation (src/api/resources.cpp:1230-1288) -> step_async ->
  * the production rigid order: predict, proxies, broadphase, narrowphase,
  * contact cache, AVBD prepare and solve, speed clamp, cache save

@@ -183,14 +183,22 @@ __device__ __forceinline__ bool overlap(float4 amn, float4 amx, float4 bmn, floa
 // of one weld group never collide, and a pair joined by a joint with
 // disable_collisions does not while that joint is alive, enabled and
 // unbroken.  A null `weld` or `excl_start` turns that test off.
+// A batch of independent scenes adds the scene filter: bodies of
+// different scenes never pair (null `scene`: one scene).
 struct JointFilter {
     const uint32_t *weld = nullptr;
     const uint32_t *excl_start = nullptr;
     const uint2 *excl_list = nullptr;   // (partner, joint)
     const PmhRigidConstraint *joints = nullptr;
+    const uint32_t *scene = nullptr;    // scene of each body
+    // Batch: each scene's large bodies (a segment of scene_large per scene)
+    // and each scene's body range.
+    const uint32_t *scene_large_start = nullptr, *scene_large = nullptr;
+    const uint32_t *scene_first = nullptr, *scene_count = nullptr;
 };
 
 __device__ __forceinline__ bool joint_excluded(const JointFilter &f, uint32_t a, uint32_t b) {
+    if (f.scene && f.scene[a] != f.scene[b]) return true;
     if (f.weld && f.weld[a] == f.weld[b]) return true;
     if (f.excl_start) {
         for (uint32_t e = f.excl_start[a]; e < f.excl_start[a + 1]; ++e) {
@@ -243,6 +251,19 @@ __global__ void k_weld_compress(uint32_t n, uint32_t *weld) {
     if (i < n) weld[i] = weld_find(weld, i);
 }
 
+// A batch's large bodies by scene: counts, then (after a scan over the
+// scenes) placement.  The order within a scene's segment is free: every
+// query sorts its candidates.
+__global__ void k_scene_large(const StepCounters *ctr, const uint32_t *large, const uint32_t *scene,
+                              uint32_t *count, const uint32_t *start, uint32_t *out) {
+    const uint32_t nl = ctr->large_count;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nl; k += gridDim.x * blockDim.x) {
+        const uint32_t b = large[k];
+        const uint32_t slot = atomicAdd(&count[scene[b]], 1u);
+        if (start) out[start[scene[b]] + slot] = b;
+    }
+}
+
 // Kept candidates of dynamic body a.  Pass 0 counts them; pass 1 writes
 // them at the body's offset (from an exclusive scan of the counts) and
 // sorts the segment ascending (identity order = index), so a body may own
@@ -287,8 +308,13 @@ __global__ void k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 
                     }
                 }
     }
-    const uint32_t nl = ctr->large_count;
-    for (uint32_t k = 0; k < nl; ++k) consider(large[k]);
+    if (jf.scene_large) {
+        const uint32_t sc = jf.scene[a];
+        for (uint32_t k = jf.scene_large_start[sc]; k < jf.scene_large_start[sc + 1]; ++k) consider(jf.scene_large[k]);
+    } else {
+        const uint32_t nl = ctr->large_count;
+        for (uint32_t k = 0; k < nl; ++k) consider(large[k]);
+    }
     if (kPass == 0) {
         cand_count[a] = count;
         return;
@@ -339,10 +365,13 @@ __global__ void __launch_bounds__(256) k_bp_large(uint32_t n, const PmhBodyParam
         if (kPass == 1 && ctr->pairs == 0) continue;
         const float4 amn = pmn[a], amx = pmx[a];
         uint32_t running = kPass == 1 ? offset[a] : 0u;
-        for (uint32_t base = 0; base < n; base += 256) {
+        // In a batch only the body's own scene can pair with it.
+        const uint32_t first = jf.scene_first ? jf.scene_first[jf.scene[a]] : 0u;
+        const uint32_t last = jf.scene_first ? first + jf.scene_count[jf.scene[a]] : n;
+        for (uint32_t base = first; base < last; base += 256) {
             const uint32_t b = base + threadIdx.x;
             bool hit = false;
-            if (b < n && b != a && !(P[b].motion == kDynamic && !(a < b)))
+            if (b < last && b != a && !(P[b].motion == kDynamic && !(a < b)))
                 hit = overlap(amn, amx, pmn[b], pmx[b]) && !joint_excluded(jf, a, b);
             const uint64_t bal = __ballot(hit);
             if (lane == 0) warp_tot[wid] = __popcll(bal);
@@ -1393,6 +1422,31 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
     if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
 }
 
+// Scenes of a batch: each body's scene, each scene's body count and the
+// index of its first pair (pairs are sorted by body, scenes are contiguous
+// body ranges).
+struct SceneRules {
+    const uint32_t *scene = nullptr;
+    const uint32_t *count = nullptr;
+    const uint32_t *pair_start = nullptr;
+};
+
+// Candidate slots of a world of n bodies: the contact-cache slot rule,
+// min(n(n-1)/2, max(4096, 8n)) (resource_plan.hpp:151-164).
+__host__ __device__ inline uint32_t leaf_slot_rule(uint64_t n) {
+    const uint64_t pairs = n * (n - 1) / 2;
+    const uint64_t cap = 8 * n > 4096 ? 8 * n : 4096;
+    return uint32_t(pairs < cap ? pairs : cap);
+}
+
+__global__ void k_scene_pairs(const uint32_t *npairs, const uint2 *ab, const uint32_t *scene, uint32_t *pair_start) {
+    const uint32_t M = *npairs;
+    for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint32_t s = scene[ab[m].x];
+        if (m == 0 || scene[ab[m - 1].x] != s) pair_start[s] = m;
+    }
+}
+
 // ---- triangle-pair kernel: one wavefront per queued pair ----------------
 constexpr uint32_t kTriWaves = 4;          // wavefronts per block
 constexpr uint32_t kLeafListCap = 512;     // ordered leaf pairs held in LDS per wavefront
@@ -1525,7 +1579,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                                                         const uint32_t *leaves, const float4 *shell,
                                                         const PmhRigidBodyState *prev,
                                                         const PmhRigidBodyState *pred, const uint2 *ab,
-                                                        NarrowOut o, uint32_t slots, uint32_t limit, float dt) {
+                                                        NarrowOut o, uint32_t slots, uint32_t limit, float dt,
+                                                        SceneRules scenes) {
     __shared__ TriWaveShared shared_waves[kTriWaves];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     TriWaveShared &S = shared_waves[wv];
@@ -1548,8 +1603,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         bool inside_a, inside_b;
         inside_flags(tp, inside_a, inside_b);
         const uint64_t cells = uint64_t(tp.am.s.leaf_count) * tp.bm.s.leaf_count;
-        // 1. Leaf pairs in evaluation order.
-        bool leaf_mode = m < slots;
+        // 1. Leaf pairs in evaluation order.  In a batch the slot and
+        // limit rules are those of the pair's own scene.
+        bool leaf_mode;
+        if (scenes.scene) {
+            const uint32_t sc = scenes.scene[p.x];
+            const uint64_t ns = scenes.count[sc];
+            const uint32_t local = m - scenes.pair_start[sc];
+            leaf_mode = local < leaf_slot_rule(ns);
+            limit = ns <= 8u ? 4096u : 512u;
+        } else {
+            leaf_mode = m < slots;
+        }
         uint32_t total = 0;
         if (leaf_mode) {
             uint32_t mine = 0;
@@ -1817,6 +1882,13 @@ PmhResult World::step_once(uint32_t k_step) {
                                          gravity, dt, inputs_dev.p, targets_dev.p, target_flags_dev.p, ctr);
     mark(1);
     JointFilter jf;
+    SceneRules rules;
+    if (scene_first.size() > 1) {
+        jf.scene = scene_of.p;
+        rules.scene = scene_of.p;
+        rules.count = scene_count.p;
+        rules.pair_start = scene_pair_start.p;
+    }
     if (solver.nj && weld_joints) {
         k_weld_init<<<bg, 256, 0, s>>>(n, weld.p);
         k_weld_hook<<<blocks_for(solver.nj), 256, 0, s>>>(n, solver.nj, joints_dev.p, weld.p);
@@ -1836,6 +1908,20 @@ PmhResult World::step_once(uint32_t k_step) {
     if (r != PMH_SUCCESS) return r;
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * table_size, s));
     k_bp_scatter<<<bg, 256, 0, s>>>(n, cell_of.p, bucket_start.p, bucket_count.p, bucket_entries.p);
+    if (jf.scene) {
+        const uint32_t ns = uint32_t(scene_first.size());
+        PMH_CHECK(hipMemsetAsync(scene_large_count.p, 0, sizeof(uint32_t) * (ns + 1), s));
+        k_scene_large<<<64, 256, 0, s>>>(ctr, large.p, scene_of.p, scene_large_count.p, nullptr, nullptr);
+        r = exclusive_scan(s, scene_large_count.p, scene_large_start.p, ns, nullptr, nullptr, v.scan_tmp);
+        if (r != PMH_SUCCESS) return r;
+        PMH_CHECK(hipMemsetAsync(scene_large_count.p, 0, sizeof(uint32_t) * (ns + 1), s));
+        k_scene_large<<<64, 256, 0, s>>>(ctr, large.p, scene_of.p, scene_large_count.p, scene_large_start.p,
+                                         scene_large.p);
+        jf.scene_large_start = scene_large_start.p;
+        jf.scene_large = scene_large.p;
+        jf.scene_first = scene_first_dev.p;
+        jf.scene_count = scene_count.p;
+    }
     k_bp_query<0><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
                                                   cand_count.p, nullptr, nullptr, nullptr, ctr, jf);
@@ -1850,6 +1936,7 @@ PmhResult World::step_once(uint32_t k_step) {
                                                   bucket_start.p, bucket_entries.p, table_size - 1, large.p,
                                                   cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr, jf);
     mark(2);
+    if (rules.scene) k_scene_pairs<<<pg, 256, 0, s>>>(&ctr->pairs, v.ab.p, scene_of.p, scene_pair_start.p);
     // Narrowphase + point compaction.
     const NarrowOut out{v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
                         v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p,
@@ -1859,7 +1946,7 @@ PmhResult World::step_once(uint32_t k_step) {
         v.tri_list.p);
     k_triangle_pairs<<<2048, 256, 0, s>>>(
         v.tri_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, nodes_dev.p, leaves_dev.p,
-        shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt);
+        shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt, rules);
     k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap);
     mark(3);
     // Response slots and the contact cache.

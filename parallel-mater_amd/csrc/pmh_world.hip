@@ -394,6 +394,22 @@ PmhResult World::ensure_device() {
         PMH_CHECK(hipMemcpy(params.p, host_params.data(), sizeof(PmhBodyParameters) * n, hipMemcpyHostToDevice));
         PMH_CHECK(hipMemcpy(state_cur.p, host_states.data(), sizeof(PmhRigidBodyState) * n, hipMemcpyHostToDevice));
         PMH_CHECK(hipMemset(cache_n.p, 0, sizeof(uint32_t) * 2));
+        if (scene_first.size() > 1) {
+            const uint32_t ns = uint32_t(scene_first.size());
+            std::vector<uint32_t> of(n), cnt(ns);
+            for (uint32_t sc = 0; sc < ns; ++sc) {
+                const uint32_t b = scene_first[sc], e = sc + 1 < ns ? scene_first[sc + 1] : n;
+                cnt[sc] = e - b;
+                for (uint32_t i = b; i < e; ++i) of[i] = sc;
+            }
+            if ((r = scene_of.reserve(n)) || (r = scene_count.reserve(ns)) || (r = scene_pair_start.reserve(ns)) ||
+                (r = scene_first_dev.reserve(ns)) || (r = scene_large_count.reserve(ns + 1)) ||
+                (r = scene_large_start.reserve(ns + 1)) || (r = scene_large.reserve(n)))
+                return r;
+            PMH_CHECK(hipMemcpy(scene_of.p, of.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+            PMH_CHECK(hipMemcpy(scene_count.p, cnt.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+            PMH_CHECK(hipMemcpy(scene_first_dev.p, scene_first.data(), sizeof(uint32_t) * ns, hipMemcpyHostToDevice));
+        }
         bodies_dirty = false;
         joints_dirty = true;   // exclusion lists are sized by the body count
     }
@@ -517,6 +533,8 @@ void World::release() {
     inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
     joints_dev.release(); weld.release(); excl_start.release(); excl_list.release();
     jlist_start.release(); jlist.release(); jheavy.release();
+    scene_of.release(); scene_count.release(); scene_pair_start.release(); scene_first_dev.release();
+    scene_large_count.release(); scene_large_start.release(); scene_large.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -669,6 +687,31 @@ PmhResult pmhWorldAddBodies(PmhWorld world, uint32_t count, const PmhBodyParamet
     w->bodies_dirty = true;
     return PMH_SUCCESS;
 }
+
+PmhResult pmhWorldSetSolveLanes(PmhWorld world, uint32_t lanes) {
+    if (!world || (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)) return PMH_ERROR_INVALID_ARGUMENT;
+    W(world)->solve_lanes = lanes;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldAddScene(PmhWorld world, uint32_t count, const PmhBodyParameters *parameters,
+                           const PmhRigidBodyState *states, uint32_t *scene) {
+    if (!world || !scene) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    // The first scene is the one plain pmhWorldAddBodies fills; an empty
+    // first scene is reused.
+    const bool reuse = w->scene_first.size() == 1 && w->body_count == 0;
+    if (!reuse) w->scene_first.push_back(w->body_count);
+    PmhResult r = pmhWorldAddBodies(world, count, parameters, states);
+    if (r != PMH_SUCCESS) {
+        if (!reuse) w->scene_first.pop_back();
+        return r;
+    }
+    *scene = uint32_t(w->scene_first.size() - 1);
+    return PMH_SUCCESS;
+}
+
+uint32_t pmhWorldSceneCount(PmhWorld world) { return world ? uint32_t(W(world)->scene_first.size()) : 0u; }
 
 uint32_t pmhWorldBodyCount(PmhWorld world) { return world ? W(world)->body_count : 0; }
 

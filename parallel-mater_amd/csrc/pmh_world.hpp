@@ -29,6 +29,11 @@ struct World {
     std::vector<uint32_t> host_target_flags;
     bool inputs_dirty = false;
     uint32_t body_count = 0;
+    // Scenes: contiguous body ranges starting at scene_first[s]; pairs never
+    // join bodies of different scenes.
+    std::vector<uint32_t> scene_first{0u};
+    DBuf<uint32_t> scene_of, scene_count, scene_pair_start, scene_first_dev;
+    DBuf<uint32_t> scene_large_count, scene_large_start, scene_large;
     // Joints: the host copy is authoritative until the first upload; after
     // a step the device copy is (breaking, rows and warm-start duals).
     std::vector<PmhRigidConstraint> host_joints;

@@ -93,6 +93,9 @@ def lib():
         "pmhWorldGetShapeInfo": (i32, [_P, _U, _P]),
         "pmhWorldAddBodies": (i32, [_P, _U, _P, _P]),
         "pmhWorldBodyCount": (_U, [_P]),
+        "pmhWorldAddScene": (i32, [_P, _U, _P, _P, _P]),
+        "pmhWorldSceneCount": (_U, [_P]),
+        "pmhWorldSetSolveLanes": (i32, [_P, _U]),
         "pmhWorldStep": (i32, [_P, _U]),
         "pmhWorldApplyRigidInputs": (i32, [_P, _U, _P]),
         "pmhWorldSetKinematicTargets": (i32, [_P, _U, _U, _P]),
@@ -197,6 +200,23 @@ class World:
         p = np.ascontiguousarray(params, dtype=PARAMS)
         s = np.ascontiguousarray(states, dtype=STATE)
         _check(lib().pmhWorldAddBodies(self._h, len(p), _ptr(p), _ptr(s)), "pmhWorldAddBodies")
+
+    def add_scene(self, params, states):
+        """Appends an independent scene (its bodies never pair with other
+        scenes'); returns its index."""
+        p = np.ascontiguousarray(params, dtype=PARAMS)
+        st = np.ascontiguousarray(states, dtype=STATE)
+        out = _U()
+        _check(lib().pmhWorldAddScene(self._h, len(p), _ptr(p), _ptr(st), ctypes.byref(out)), "pmhWorldAddScene")
+        return out.value
+
+    @property
+    def scene_count(self):
+        return lib().pmhWorldSceneCount(self._h)
+
+    def set_solve_lanes(self, lanes):
+        _check(lib().pmhWorldSetSolveLanes(self._h, lanes), "pmhWorldSetSolveLanes")
+
 
     @property
     def body_count(self):
@@ -362,3 +382,22 @@ def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
     manifolds[...] = d_m.read()
     bodies[...] = d_b.read()
     return d_sched.read()[0]
+
+
+def autotune_lanes(world, steps=8, candidates=(8, 16, 32, 64)):
+    """Times `steps` steps of `world` with each lane tile of the colour
+    sweep (the world keeps simulating meanwhile) and leaves it on the
+    fastest.  Returns (best, {lanes: ms per step})."""
+    import time
+    times = {}
+    for lanes in candidates:
+        world.set_solve_lanes(lanes)
+        world.step(1)
+        lib().pmhDeviceSynchronize()
+        t0 = time.perf_counter()
+        world.step(steps)
+        lib().pmhDeviceSynchronize()
+        times[lanes] = (time.perf_counter() - t0) * 1e3 / steps
+    best = min(times, key=times.get)
+    world.set_solve_lanes(best)
+    return best, times

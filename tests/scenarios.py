@@ -627,3 +627,115 @@ def _rotate_np(q, v):
     u = np.asarray(q[:3], np.float64)
     t = 2.0 * np.cross(u, v)
     return v + q[3] * t + np.cross(u, t)
+
+
+class Batch:
+    """Independent scenes sharing one shape palette: `scenes[s]` is a
+    (params, states) pair whose shape indices point into `shapes`."""
+
+    joints = None
+
+    def __init__(self, shapes, scenes, name=""):
+        self.shapes, self.scenes, self.name = shapes, scenes, name
+
+    @property
+    def n(self):
+        return sum(len(p) for p, _ in self.scenes)
+
+    def add_shapes(self, world):
+        for shape in self.shapes:
+            if shape[0] == "mesh":
+                world.add_mesh(shape[1], shape[2])
+            elif shape[0] == "sphere":
+                world.add_sphere(shape[1])
+            else:
+                world.add_capsule(shape[1], shape[2])
+
+    def build(self, world):
+        """All scenes in one world (pmhWorldAddScene each)."""
+        self.add_shapes(world)
+        for p, st in self.scenes:
+            world.add_scene(p, st)
+        return world
+
+    def build_one(self, world, s):
+        """Scene `s` alone, with the same shape palette."""
+        self.add_shapes(world)
+        p, st = self.scenes[s]
+        world.add_bodies(p, st)
+        return world
+
+
+def batch(count=4096, seed=7, min_bodies=8, max_bodies=512, palette=64, region=40.0):
+    """`count` independent scenes for one batched launch.  Per scene: a body
+    count log-uniform in [min_bodies, max_bodies]; half boxes (edges
+    U[0.2, 0.6] m, drawn from a palette of `palette` sizes) and half analytic
+    spheres (r U[0.1, 0.3] m, same palette rule), density log-uniform
+    [200, 5000] kg/m^3, friction U[0.3, 0.8], reference default body
+    options; either a drop (a jittered grid 0.8 m apart above the ground) or
+    stacks (columns of bodies resting on each other).  Each scene has its
+    own static ground box and is laid out in its own `region`-metre square,
+    so the batch also stays apart in space.  Body 0 of a scene is its
+    ground."""
+    rng = np.random.default_rng(seed)
+    box_h = rng.uniform(0.1, 0.3, size=(palette, 3)).astype(np.float32)
+    radii = rng.uniform(0.1, 0.3, size=palette).astype(np.float32)
+    shapes = [("mesh",) + box_mesh(region * 0.45, 0.5, region * 0.45)]
+    shapes += [("mesh",) + box_mesh(*h) for h in box_h]
+    shapes += [("sphere", float(r)) for r in radii]
+    grid = int(np.ceil(np.sqrt(count)))
+    scenes = []
+    for sc in range(count):
+        nb = int(np.exp(rng.uniform(np.log(min_bodies), np.log(max_bodies + 1))))
+        nb = max(min_bodies, min(max_bodies, nb))
+        ox = (sc % grid - grid / 2.0) * region
+        oz = (sc // grid - grid / 2.0) * region
+        is_box = rng.permutation(np.arange(nb) % 2 == 0)
+        pick = rng.integers(0, palette, size=nb)
+        density = np.exp(rng.uniform(np.log(200.0), np.log(5000.0), size=nb)).astype(np.float32)
+        friction = rng.uniform(0.3, 0.8, size=nb).astype(np.float32)
+        shape = np.where(is_box, 1 + pick, 1 + palette + pick)
+        mass = np.zeros(nb, np.float32)
+        inertia = np.zeros((nb, 3), np.float32)
+        size = np.zeros(nb, np.float32)                  # vertical half extent
+        for i in range(nb):
+            if is_box[i]:
+                h = box_h[pick[i]]
+                mass[i] = density[i] * np.float32(8.0) * h.prod(dtype=np.float32)
+                inertia[i] = box_inertia(mass[i], h)
+                size[i] = h[1]
+            else:
+                r = radii[pick[i]]
+                mass[i] = density[i] * np.float32(4.0 / 3.0 * np.pi) * r * r * r
+                inertia[i] = np.float32(0.4) * mass[i] * r * r
+                size[i] = r
+        pos = np.zeros((nb, 3), np.float32)
+        if rng.random() < 0.5:                           # drop
+            side = int(np.ceil(nb ** (1.0 / 3.0)))
+            g = np.stack(np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij"),
+                         -1).reshape(-1, 3)[:nb, ::-1]
+            pos[:] = (g - (side - 1) / 2.0) * 0.8
+            pos[:, 1] += (side - 1) * 0.4 + 1.0
+            pos += rng.uniform(-0.05, 0.05, size=pos.shape).astype(np.float32)
+        else:                                            # stacks of up to 8
+            cols = int(np.ceil(nb / 8.0))
+            cside = int(np.ceil(np.sqrt(cols)))
+            height = np.zeros(cols, np.float32)
+            for i in range(nb):
+                c = i % cols
+                pos[i] = ((c % cside - cside / 2.0) * 0.8, height[c] + size[i] + 0.002, (c // cside - cside / 2.0) * 0.8)
+                height[c] += 2.0 * size[i] + 0.004
+            pos[:, :2] += rng.uniform(-0.01, 0.01, size=(nb, 2)).astype(np.float32)
+        pos[:, 0] += ox
+        pos[:, 2] += oz
+        n = nb + 1
+        motion = np.full(n, L.MOTION_DYNAMIC, np.uint32)
+        motion[0] = L.MOTION_STATIC
+        p = params(n, motion=motion, mass=np.concatenate([[1.0], mass]).astype(np.float32),
+                   inertia=np.concatenate([np.ones((1, 3), np.float32), inertia]),
+                   friction=np.concatenate([[0.5], friction]).astype(np.float32),
+                   shape=np.concatenate([[0], shape]), margin=REF_MARGIN, damping=REF_DAMPING,
+                   max_linear=REF_MAX_SPEED, max_angular=REF_MAX_SPEED)
+        st = states(n, np.concatenate([np.array([[ox, -0.5, oz]], np.float32), pos]))
+        scenes.append((p, st))
+    return Batch(shapes, scenes, f"batch{count}")

@@ -473,3 +473,37 @@ def test_chains_single_step_and_trajectory():
     assert abs(ke[0] - ke[1]) <= 0.1 * ke[1]
     assert abs(dg.mean() - do.mean()) <= 0.1 * do.mean()
 
+
+def test_batched_scenes_step_as_they_would_alone():
+    """Eight scenes of 9-300 bodies in one world step bit for bit as each
+    does alone: pairs never cross scenes and the triangle-path rules are
+    the scene's own."""
+    say("batch of 8 scenes vs each alone, 90 steps")
+    b = S.batch(count=8, seed=11, max_bodies=300)
+    g = b.build(pmamd.World(profile=0, base_iterations=10))
+    assert g.scene_count == 8
+    alone = [b.build_one(pmamd.World(profile=0, base_iterations=10), s) for s in range(8)]
+    for k in range(90):
+        g.step(1)
+        for w in alone:
+            w.step(1)
+    st = g.states()
+    off = 0
+    for s, w in enumerate(alone):
+        n = len(b.scenes[s][0])
+        np.testing.assert_array_equal(st[off:off + n].view(np.uint32), w.states().view(np.uint32),
+                                      err_msg=f"scene {s}")
+        off += n
+    assert g.stats()["manifolds"] == sum(w.stats()["manifolds"] for w in alone)
+
+
+def test_autotuner_picks_a_lane_tile_and_keeps_simulating():
+    b = S.batch(count=16, seed=3, max_bodies=64)
+    w = b.build(pmamd.World(profile=0, base_iterations=10))
+    w.step(5)
+    best, times = pmamd.autotune_lanes(w, steps=2)
+    say(f"lane tile {best}: {times}")
+    assert best in (8, 16, 32, 64) and set(times) == {8, 16, 32, 64}
+    w.step(2)
+    assert np.all(np.isfinite(w.states()["position"]))
+
