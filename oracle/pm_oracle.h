@@ -259,6 +259,12 @@ uint32_t pmo_world_body_count(void *world);
 /* Cumulative wall time of world steps, of AVBD prepare and of the AVBD
  * solve (milliseconds), and body iterations solved; optionally reset. */
 void pmo_world_timing(void *world, double out[4], int reset);
+
+/* The learned warm start's network (pm_oracle_mlp.cpp): forward pass and
+ * mean-squared-error gradients, with (round_bf16 != 0) or without the
+ * device path's bf16 rounding points. */
+void pmo_mlp_forward(const float *params, const float *x, uint32_t n, float *y, int round_bf16);
+float pmo_mlp_gradients(const float *params, const float *samples, uint32_t count, float *grads, int round_bf16);
 void pmo_world_get_states(void *world, PmoState *out);
 void pmo_world_set_states(void *world, const PmoState *states);
 void pmo_world_get_stats(void *world, PmoWorldStats *out);

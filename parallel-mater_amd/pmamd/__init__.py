@@ -118,6 +118,9 @@ def lib():
         "pmhWorldGetProxies": (i32, [_P, _P]),
         "pmhPrepareSolveGeometryConstraints": (
             i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
+        "pmhWarmstartForward": (i32, [_P, _P, _U, _P]),
+        "pmhWarmstartGradients": (i32, [_P, _U, _P, _P, _P]),
+        "pmhWarmstartTrain": (i32, [_P, _U, _U, _F, _U, _P, _P]),
         "pmhDeviceAlloc": (i32, [ctypes.c_uint64, _P]),
         "pmhDeviceFree": (i32, [_P]),
         "pmhCopyToDevice": (i32, [_P, _P, ctypes.c_uint64]),
@@ -125,6 +128,8 @@ def lib():
         "pmhDeviceSynchronize": (i32, []),
         "pmhDeviceInfo": (i32, [_P, _U, _P]),
     }
+    L.pmhWarmstartInit.restype = None
+    L.pmhWarmstartInit.argtypes = [_U, _P]
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
         fn.restype = res
@@ -401,3 +406,48 @@ def autotune_lanes(world, steps=8, candidates=(8, 16, 32, 64)):
     best = min(times, key=times.get)
     world.set_solve_lanes(best)
     return best, times
+
+
+# ---------------------------------------------------------------------------
+# Learned warm start: a 16 -> 64 -> 64 -> 2 network on MFMA (BF16 operands,
+# FP32 accumulation); parameters packed W1[64][16], b1, W2[64][64], b2,
+# W3[2][64], b3 (5378 floats).
+# ---------------------------------------------------------------------------
+MLP_PARAMS = 5378
+MLP_FEATURES = 16
+MLP_SAMPLE = 18
+
+
+def warmstart_init(seed=0):
+    p = np.zeros(MLP_PARAMS, dtype=np.float32)
+    lib().pmhWarmstartInit(seed, _ptr(p))
+    return p
+
+
+def warmstart_forward(params, features):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, MLP_FEATURES)
+    y = np.zeros((len(x), 2), dtype=np.float32)
+    _check(lib().pmhWarmstartForward(_ptr(p), _ptr(x), len(x), _ptr(y)), "pmhWarmstartForward")
+    return y
+
+
+def warmstart_gradients(params, samples):
+    """Mean-squared-error loss and gradients of at most 64 samples."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    s = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, MLP_SAMPLE)
+    g = np.zeros(MLP_PARAMS, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    _check(lib().pmhWarmstartGradients(_ptr(s), len(s), _ptr(p), _ptr(g), _ptr(loss)), "pmhWarmstartGradients")
+    return float(loss[0]), g
+
+
+def warmstart_train(samples, epochs=4, learning_rate=1e-3, seed=0, params=None):
+    """Adam over minibatches of 64 on the GPU; returns (params, loss per epoch)."""
+    s = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, MLP_SAMPLE)
+    p = warmstart_init(seed) if params is None else np.array(params, dtype=np.float32)
+    loss = np.zeros(epochs, dtype=np.float32)
+    _check(lib().pmhWarmstartTrain(_ptr(s), len(s), epochs, learning_rate, seed, _ptr(p), _ptr(loss)),
+           "pmhWarmstartTrain")
+    return p, loss
+

@@ -62,6 +62,10 @@ def lib():
         _lib.pmo_world_apply_inputs.argtypes = [_P, _U, _P]
         _lib.pmo_world_add_joints.argtypes = [_P, _U, _P]
         _lib.pmo_world_residuals.argtypes = [_P, _P, _P, _P]
+        _lib.pmo_mlp_forward.argtypes = [_P, _P, _U, _P, ctypes.c_int]
+        _lib.pmo_mlp_forward.restype = None
+        _lib.pmo_mlp_gradients.argtypes = [_P, _P, _U, _P, ctypes.c_int]
+        _lib.pmo_mlp_gradients.restype = ctypes.c_float
         _lib.pmo_world_joint_count.argtypes = [_P]
         _lib.pmo_world_joint_count.restype = _U
         _lib.pmo_world_get_joints.argtypes = [_P, _P]
@@ -318,3 +322,20 @@ def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
     if r != 0:
         raise RuntimeError("prepare_solve failed")
     return schedule[0]
+
+
+def mlp_forward(params, features, round_bf16=True):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 16)
+    y = np.zeros((len(x), 2), dtype=np.float32)
+    lib().pmo_mlp_forward(_ptr(p), _ptr(x), len(x), _ptr(y), 1 if round_bf16 else 0)
+    return y
+
+
+def mlp_gradients(params, samples, round_bf16=True):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    s = np.ascontiguousarray(samples, dtype=np.float32).reshape(-1, 18)
+    g = np.zeros(len(p), dtype=np.float32)
+    loss = lib().pmo_mlp_gradients(_ptr(p), _ptr(s), len(s), _ptr(g), 1 if round_bf16 else 0)
+    return float(loss), g
+
