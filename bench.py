@@ -79,12 +79,92 @@ def cpu_baseline(scene, states, seconds, profile, iterations):
     }
 
 
+def warmstart_bench(args, tolerance=1e-4, cap=64, samples_wanted=1_000_000):
+    """The learned warm start against plain AVBD, both iterating each phase
+    to an RMS constraint residual of 1e-4 (the budget capping at 64): log
+    cold-contact samples from plain runs of four seeded 32768-body piles
+    (seeds 11-14), train the network on 10^6 of them on the GPU, then run the
+    bench pile (seed 2) with each warm start (plain; the predicted penalty;
+    with --warm-lambda also the predicted penalty and, where a contact starts
+    in violation, lambda): warm-up steps, then timed steps, each step checked
+    for the fallback. value = sim-steps/s with the predicted penalty."""
+    import pmamd
+    import scenarios as S
+    t0 = time.perf_counter()
+    logs = []
+    per_seed = samples_wanted // 4
+    for seed in (11, 12, 13, 14):
+        # every cold contact of the 240 steps, then a uniform sample of them
+        # (falling, impact and settling phases alike)
+        w = S.pile(side=args.side, seed=seed).build(pmamd.World(profile=0, base_iterations=args.iterations))
+        w.set_contact_log(8_000_000)
+        w.step(240)
+        log = w.contact_log()
+        w.close()
+        keep = np.random.default_rng(seed).choice(len(log), size=min(per_seed, len(log)), replace=False)
+        logs.append(log[np.sort(keep)])
+        print(f"# bench: seed {seed}: {len(log)} cold-contact samples, {len(logs[-1])} kept", file=sys.stderr,
+              flush=True)
+    samples = np.concatenate(logs)
+    t1 = time.perf_counter()
+    params, loss = pmamd.warmstart_train(samples, epochs=6, learning_rate=1e-3, seed=1)
+    t2 = time.perf_counter()
+    var = float(np.mean((samples[:, 16:] - samples[:, 16:].mean(0)) ** 2))
+    print(f"# bench: trained on {len(samples)} samples in {t2 - t1:.1f} s, loss {loss[-1]:.4g} "
+          f"(target variance {var:.4g})", file=sys.stderr, flush=True)
+    runs = {}
+    scene = S.pile(side=args.side, seed=2, analytic_spheres=True)
+    modes = ("plain", "learned", "learned_lambda") if args.warm_lambda else ("plain", "learned")
+    for mode in modes:
+        w = scene.build(pmamd.World(profile=0, base_iterations=cap, residual_tolerance=tolerance))
+        if mode != "plain":
+            w.set_warmstart(params, penalty=True, lam=mode == "learned_lambda")
+        w.step(args.warmup)
+        pmamd.lib().pmhDeviceSynchronize()
+        its = []
+        start = w.warmstart_stats()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            w.step(1)
+            its.append(w.residuals()[1].copy())
+        pmamd.lib().pmhDeviceSynchronize()
+        wall = time.perf_counter() - ts
+        a = np.array(its, dtype=np.float64)
+        st = w.warmstart_stats()
+        runs[mode] = {"steps_per_s": args.steps / wall, "pose_iterations": float(a[:, 0].mean()),
+                      "velocity_iterations": float(a[:, 1].mean()),
+                      "capped_steps": int((a[:, 0] >= cap).sum()),
+                      "fallback_rate": (st[1] - start[1]) / max(1, st[0] - start[0]) if mode != "plain" else None}
+        print(f"# bench: {mode}: {runs[mode]}", file=sys.stderr, flush=True)
+        w.close()
+    name, cus = pmamd.device_info()
+    p, l_ = runs["plain"], runs["learned"]
+    return {
+        "metric": METRIC, "value": l_["steps_per_s"], "unit": "sim-steps/s to 1e-4 residual",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 / l_["steps_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (network bf16 on MFMA, fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": f"learned warm start vs plain AVBD on the {args.side ** 3}-body pile (seed 2), "
+                               f"each phase iterated to an RMS constraint residual of {tolerance} (cap {cap}); "
+                               f"network trained on {len(samples)} cold-contact samples logged from plain runs of "
+                               f"four other seeded piles",
+                   "device": f"{name} ({cus} CUs)"},
+        "plain": p, "learned": l_, "learned_lambda": runs.get("learned_lambda"),
+        "iteration_ratio": p["pose_iterations"] / l_["pose_iterations"],
+        "steps_per_s_ratio": l_["steps_per_s"] / p["steps_per_s"],
+        "training": {"samples": int(len(samples)), "epochs": 6, "seconds": t2 - t1,
+                     "loss_per_epoch": [float(x) for x in loss], "target_variance": var,
+                     "logging_seconds": t1 - t0},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=540)
     ap.add_argument("--warmup", type=int, default=60)
-    ap.add_argument("--workload", choices=["pile", "chains", "batch"], default="pile",
+    ap.add_argument("--workload", choices=["pile", "chains", "batch", "warmstart"], default="pile",
                     help="pile: 32^3 boxes and spheres in a bin; chains: capsule chains with ball and hinge joints; "
                          "batch: independent scenes of 8-512 bodies in one world")
     ap.add_argument("--scenes", type=int, default=4096, help="scene count of the batch workload")
@@ -108,6 +188,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": n * 4}))
+        return
+    if args.workload == "warmstart":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only.")
+            sys.exit(2)
+        print(json.dumps(warmstart_bench(args)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); "

@@ -213,6 +213,43 @@ PmhResult pmhWorldAddScene(PmhWorld world, uint32_t count, const PmhBodyParamete
                            const PmhRigidBodyState *states, uint32_t *scene);
 uint32_t pmhWorldSceneCount(PmhWorld world);
 
+/* Learned warm start (no counterpart in the reference; its cold contacts
+ * start from lambda = 0 and the pair stiffness, geometry_constraints.slang:
+ * 1366-1373).  A 16 -> 64 -> 64 -> 2 network (ReLU; BF16 weights and
+ * activations on MFMA, FP32 accumulation) predicts the normal row's
+ * (lambda, penalty) of every cold contact point from 16 features.  The
+ * 5380 parameters are packed W1[64][16], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2], then the largest training target of each output
+ * (set by pmhWarmstartTrain; predictions are clamped to [0, it]).
+ *   pmhWorldSetWarmstart: use `params` from the next step (NULL: plain),
+ *     applying the predicted penalty (PMH_WARMSTART_PENALTY) and/or the
+ *     predicted lambda (PMH_WARMSTART_LAMBDA, only on cold points whose
+ *     initial normal error is positive: those active from the first
+ *     iteration anyway).  In
+ *     to-residual mode (residual_tolerance > 0) a step whose residual ends
+ *     above the tolerance is taken again with the plain warm start;
+ *     pmhWorldWarmstartStats counts steps and such fallbacks.
+ *   pmhWorldSetContactLog: log every cold point's 16 features and its duals
+ *     at the end of its step (log10(1 + lambda / (m_eff |g|)), log10(k / k0)) as 18
+ *     floats, up to `capacity` samples (0: off); pmhWorldGetContactLog reads
+ *     them.
+ *   pmhWarmstartTrain: Adam over minibatches of 64 on the GPU, mean squared
+ *     error, `epochs` passes in seeded orders; params in/out.
+ *   pmhWarmstartForward / pmhWarmstartGradients: the network's outputs and
+ *     one minibatch's loss and gradients (for checks). */
+#define PMH_WARMSTART_PENALTY 1u
+#define PMH_WARMSTART_LAMBDA 2u
+PmhResult pmhWorldSetWarmstart(PmhWorld world, const float *params, uint32_t policy);
+PmhResult pmhWorldSetContactLog(PmhWorld world, uint32_t capacity);
+PmhResult pmhWorldGetContactLog(PmhWorld world, float *samples, uint32_t capacity, uint32_t *count);
+PmhResult pmhWorldWarmstartStats(PmhWorld world, uint64_t *steps, uint64_t *fallbacks);
+void pmhWarmstartInit(uint32_t seed, float *params);
+PmhResult pmhWarmstartTrain(const float *samples, uint32_t n, uint32_t epochs, float learning_rate, uint32_t seed,
+                            float *params, float *epoch_loss);
+PmhResult pmhWarmstartForward(const float *params, const float *features, uint32_t n, float *outputs);
+PmhResult pmhWarmstartGradients(const float *samples, uint32_t count, const float *params, float *grads,
+                                float *loss);
+
 /* Lanes per body of the colour sweep for the following steps (8, 16, 32 or
  * 64): the tile the autotuner picks.  It sets the order in which a body's
  * contact rows are summed. */

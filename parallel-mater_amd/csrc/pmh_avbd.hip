@@ -19,6 +19,7 @@
 // then solved in registers by an unrolled LDL^T.  Per-point Jacobians are
 // computed once per phase (they only depend on the target poses).
 #include "pmh_internal.hpp"
+#include "pmh_warmstart.hpp"
 
 #include <hip/hip_cooperative_groups.h>
 
@@ -1725,6 +1726,69 @@ __global__ void k_stage_apply(SolveArgs a, uint32_t phase) {
     }
 }
 
+// ---- learned warm start: features of cold points and the contact log -----
+// 16 features of a cold contact point (no reference counterpart): depth,
+// closing motion over the step, normal, initial normal error, log pair
+// stiffness, log effective mass, friction, both-dynamic flag, support
+// against gravity, manifold size, face-patch flag, impact fraction,
+// persistence, and 1.  Lengths are in centimetres.
+__global__ void k_warm_features(const uint32_t *npts, const PmhBodyParameters *P, const uint32_t *c_pair,
+                                const uint2 *ab, const uint32_t *m_count, const uint32_t *m_face,
+                                const float4 *c_np, const float4 *c_pt, const float4 *c_err, const float4 *c_aa,
+                                const uint32_t *c_flags, V3 gravity, float dt, uint32_t cap, uint32_t *list,
+                                float *feat, float *scale, StepCounters *ctr) {
+    const uint32_t Pn = *npts;
+    const uint32_t lane = threadIdx.x & 63;
+    const float g = sqrtf(dot(gravity, gravity));
+    const V3 up = g > 0.0f ? mul(gravity, -1.0f / g) : V3{0.0f, 0.0f, 0.0f};
+    for (uint32_t base = blockIdx.x * blockDim.x; base < Pn; base += gridDim.x * blockDim.x) {
+        const uint32_t c = base + threadIdx.x;
+        const bool cold = c < Pn && (c_flags[c] & 1u) == 0u;
+        const uint64_t mask = __ballot(cold);
+        uint32_t first = 0;
+        if (lane == 0 && mask) first = atomicAdd(&ctr->cold_points, uint32_t(__popcll(mask)));
+        first = __shfl(first, 0, 64);
+        if (!cold) continue;
+        const uint32_t idx = first + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+        if (idx >= cap) continue;
+        const uint32_t m = c_pair[c];
+        const uint2 p = ab[m];
+        const PmhBodyParameters pa = P[p.x], pb = P[p.y];
+        const float ia = pa.motion == kDynamic ? pa.inverse_mass : 0.0f;
+        const float ib = pb.motion == kDynamic ? pb.inverse_mass : 0.0f;
+        const float meff = ia + ib > 0.0f ? 1.0f / (ia + ib) : 1.0f;
+        const float k0 = pair_stiffness(pa, pb, dt);
+        const float4 np = c_np[c];
+        const float f[kMlpIn] = {np.w * 100.0f, c_pt[c].w * dt * 100.0f, np.x, np.y, np.z, c_err[c].x * 100.0f,
+                                 log10f(k0) * 0.1f, log10f(meff) * 0.25f, sqrtf(pa.friction * pb.friction),
+                                 pa.motion == kDynamic && pb.motion == kDynamic ? 1.0f : 0.0f,
+                                 dot(V3{np.x, np.y, np.z}, up), float(m_count[m]) * 0.125f,
+                                 m_face[m] ? 1.0f : 0.0f, c_aa[c].w, (c_flags[c] & 2u) ? 1.0f : 0.0f, 1.0f};
+        list[idx] = c;
+#pragma unroll
+        for (int q = 0; q < kMlpIn; ++q) feat[size_t(idx) * kMlpIn + q] = f[q];
+        scale[2 * size_t(idx)] = g > 0.0f ? meff * g : meff;
+        scale[2 * size_t(idx) + 1] = k0;
+    }
+}
+
+// Appends each listed cold point's features and its duals at the end of
+// the step (log10(1 + lambda / (m_eff |g|)), log10(k / k0)) to the log.
+__global__ void k_warm_log(StepCounters *ctr, uint32_t cap_list, const uint32_t *list, const float *feat,
+                           const float *scale, const float4 *c_lam, const float4 *c_pen, float *log,
+                           uint32_t log_cap) {
+    const uint32_t n = min(ctr->cold_points, cap_list);
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t slot = atomicAdd(&ctr->log_count, 1u);
+        if (slot >= log_cap) return;
+        const uint32_t c = list[k];
+        float *row = log + size_t(slot) * kMlpSample;
+        for (int q = 0; q < kMlpIn; ++q) row[q] = feat[size_t(k) * kMlpIn + q];
+        row[kMlpIn] = log10f(1.0f + fmaxf(c_lam[c].x, 0.0f) / scale[2 * size_t(k)]);
+        row[kMlpIn + 1] = log10f(fmaxf(c_pen[c].x, 1.0e-30f) / scale[2 * size_t(k) + 1]);
+    }
+}
+
 // ---- joint preparation (geometry_constraints.slang:1391-1417, 1462-1475) --
 // State, dense ids and initial rows; one thread per joint.  Body ids are
 // {dense index, 1}.
@@ -1914,6 +1978,7 @@ __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32
     ctr->dyn_bodies = ctr->impact_bodies = 0;
     ctr->mean_roots = 0;
     ctr->joint_budget = 0;
+    ctr->cold_points = 0;
     ctr->m1 = ctr->m2 = ctr->p1 = ctr->p2 = 0;
     ctr->m1i = ctr->m2i = ctr->p1i = ctr->p2i = 0;
 }
@@ -2018,6 +2083,7 @@ void Solver::release() {
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
     erec.release(); c_entry.release(); residual.release();
+    cold_list.release(); cold_feat.release(); cold_scale.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
     if (host_ctr) (void)hipHostFree(host_ctr);
@@ -2086,6 +2152,18 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
                                       v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
                                       v.m_impact.p, dt, sp.profile);
+    const uint32_t cold_cap = uint32_t(v.cold_list.n);
+    const bool warm = (v.mlp && sp.warm_infer) || v.log;
+    if (warm && cold_cap) {
+        k_warm_features<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.m_count.p, v.m_face.p, v.c_np.p, v.c_pt.p,
+                                            v.c_err.p, v.c_aa.p, v.c_flags.p, sp.gravity, dt, cold_cap,
+                                            v.cold_list.p, v.cold_feat.p, v.cold_scale.p, ctr);
+        if (v.mlp && sp.warm_infer) {
+            PmhResult rr = mlp_infer(s, &ctr->cold_points, v.cold_list.p, v.cold_feat.p, v.cold_scale.p, v.mlp,
+                                     v.c_lam.p, v.c_pen.p, cold_cap, v.mlp_policy);
+            if (rr != PMH_SUCCESS) return rr;
+        }
+    }
     if (v.nj) k_joint_init<<<jg, 256, 0, s>>>(a);
     k_degree<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.deg.p, nullptr, ctr);
     PmhResult r = exclusive_scan(s, v.deg.p, v.adj_start.p, n, nullptr, nullptr, v.scan_tmp);
@@ -2273,6 +2351,9 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         }
         k_schedule_finish<<<1, 64, 0, s>>>(v.schedule.p, ctr, 0);
     }
+    if (v.log && cold_cap)
+        k_warm_log<<<grid_for(cold_cap), 256, 0, s>>>(ctr, cold_cap, v.cold_list.p, v.cold_feat.p, v.cold_scale.p,
+                                                      v.c_lam.p, v.c_pen.p, v.log, v.log_cap);
     if (timing && timing->stop) PMH_CHECK(hipEventRecord(timing->stop, s));
     PMH_CHECK(hipGetLastError());
     return PMH_SUCCESS;

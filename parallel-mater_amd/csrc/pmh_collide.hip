@@ -1867,7 +1867,7 @@ static uint32_t blocks_for(uint32_t items, uint32_t block = 256, uint32_t cap = 
     return std::max<uint32_t>(1, std::min(g, cap));
 }
 
-PmhResult World::step_once(uint32_t k_step) {
+PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     hipStream_t s = stream;
     const uint32_t n = body_count;
     Solver &v = solver;
@@ -1960,7 +1960,7 @@ PmhResult World::step_once(uint32_t k_step) {
                                     v.c_flags.p, dt, ctr);
     mark(4);
     SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes,
-                   residual_tolerance};
+                   residual_tolerance, warm_infer && warm_enabled};
     SolveTiming st;
     st.kernels = &kernel_timer;
     if (timing_enabled) {

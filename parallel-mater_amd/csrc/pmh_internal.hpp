@@ -102,7 +102,8 @@ struct StepCounters {
     uint32_t fallback_pairs;   // of those, solved by BVH traversal
     uint32_t mean_roots;       // unanchored islands with contacts (free-motion mean)
     uint32_t joint_budget;     // largest joint iteration count (legacy budget floor)
-    uint32_t pad[2];
+    uint32_t cold_points;      // cold contact points listed for the learned warm start
+    uint32_t log_count;        // contact samples logged so far (learned warm start)
 };
 
 // Running totals over every step since the last reset.
@@ -189,6 +190,15 @@ struct Solver {
     DBuf<uint32_t> c_entry;
     size_t ecap = 0;
     DBuf<ResidualLog> residual;
+    // Learned warm start: cold points of the step, their 16 features and
+    // (lambda scale, k0); network parameters (null: plain warm start); the
+    // contact log (features and end-of-step duals of cold points).
+    DBuf<uint32_t> cold_list;
+    DBuf<float> cold_feat, cold_scale;
+    const float *mlp = nullptr;
+    uint32_t mlp_policy = 1u;            // predictions applied (penalty; lambda)
+    float *log = nullptr;
+    uint32_t log_cap = 0;
     // pairs
     DBuf<uint32_t> code;                 // a*n+b
     DBuf<uint2> ab;
@@ -221,6 +231,7 @@ struct SolveParams {
     uint32_t cooperative;                // one cooperative launch with grid barriers instead of per-stage launches
     uint32_t lanes;                      // lanes of a wavefront per body in the colour sweep: 8, 16, 32, 64
     float tolerance = 0.0f;              // > 0: to-residual mode, the budget caps each phase
+    bool warm_infer = false;             // predict cold points' normal duals
 };
 
 

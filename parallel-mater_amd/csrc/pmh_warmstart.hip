@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(64) k_mlp_train(const float *samples, uint32_t
             __syncthreads();
             ++t;
             const float c1 = 1.0f - powf(0.9f, float(t)), c2 = 1.0f - powf(0.999f, float(t));
-            for (int k = l; k < kMlpParams; k += 64) {
+            for (int k = l; k < kMlpBound; k += 64) {
                 const float gk = grads[k];
                 m[k] = 0.9f * m[k] + 0.1f * gk;
                 v[k] = 0.999f * v[k] + 0.001f * gk * gk;
@@ -265,14 +265,14 @@ __global__ void __launch_bounds__(64) k_mlp_train(const float *samples, uint32_t
     }
 }
 
-// Inference for listed cold points: y0 scales lambda by m_eff |g|, y1 is
-// log10(k / k0); lambda >= 0, k0 <= k <= the penalty cap.
+// Inference for listed cold points: y0 = log10(1 + lambda / (m_eff |g|)),
+// y1 = log10(k / k0); lambda >= 0, k0 <= k <= the penalty cap.
 __global__ void __launch_bounds__(64) k_mlp_infer(const uint32_t *count_ptr, const uint32_t *list, const float *feat,
                                                   const float *scale, const float *params, float4 *c_lam,
-                                                  float4 *c_pen) {
+                                                  float4 *c_pen, uint32_t cap, uint32_t policy) {
     __shared__ MlpShared S;
     load_weights(S, params);
-    const uint32_t total = *count_ptr;
+    const uint32_t total = min(*count_ptr, cap);
     const int l = threadIdx.x;
     for (uint32_t first = blockIdx.x * 64; first < total; first += gridDim.x * 64) {
         const uint32_t count = min(64u, total - first);
@@ -283,23 +283,34 @@ __global__ void __launch_bounds__(64) k_mlp_infer(const uint32_t *count_ptr, con
         if (live) {
             const uint32_t c = list[first + l];
             const float lam_scale = scale[2 * size_t(first + l)], k0 = scale[2 * size_t(first + l) + 1];
-            const float y0 = S.y[l * kMlpOut], y1 = S.y[l * kMlpOut + 1];
-            const float lam = fmaxf(y0, 0.0f) * lam_scale;
-            const float k = fminf(k0 * exp10f(fminf(fmaxf(y1, 0.0f), 10.0f)), kMaxPenalty);
-            float4 a = c_lam[c], b = c_pen[c];
-            a.x = lam;
-            b.x = fmaxf(k, k0);
-            c_lam[c] = a;
-            c_pen[c] = b;
+            // predictions stay within the range of the training targets
+            const float y0 = fminf(fmaxf(S.y[l * kMlpOut], 0.0f), params[kMlpBound]);
+            const float y1 = fminf(fmaxf(S.y[l * kMlpOut + 1], 0.0f), params[kMlpBound + 1]);
+            const float lam = (exp10f(y0) - 1.0f) * lam_scale;
+            const float k = fminf(k0 * exp10f(y1), kMaxPenalty);
+            // The penalty always; lambda only where the contact starts in
+            // violation (initial normal error > 0, feature 5), i.e. where it
+            // is active from the first iteration anyway.
+            if (policy & kWarmPenalty) {
+                float4 b = c_pen[c];
+                b.x = fmaxf(k, k0);
+                c_pen[c] = b;
+            }
+            if ((policy & kWarmLambda) && feat[size_t(first + l) * kMlpIn + 5] > 0.0f) {
+                float4 a = c_lam[c];
+                a.x = lam;
+                c_lam[c] = a;
+            }
         }
         __syncthreads();
     }
 }
 
 PmhResult mlp_infer(hipStream_t s, const uint32_t *count, const uint32_t *list, const float *feat,
-                    const float *scale, const float *params, float4 *c_lam, float4 *c_pen, uint32_t cap) {
+                    const float *scale, const float *params, float4 *c_lam, float4 *c_pen, uint32_t cap,
+                    uint32_t policy) {
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((cap + 63) / 64, 1024));
-    k_mlp_infer<<<blocks, 64, 0, s>>>(count, list, feat, scale, params, c_lam, c_pen);
+    k_mlp_infer<<<blocks, 64, 0, s>>>(count, list, feat, scale, params, c_lam, c_pen, cap, policy);
     PMH_CHECK(hipGetLastError());
     return PMH_SUCCESS;
 }
@@ -336,6 +347,7 @@ void pmhWarmstartInit(uint32_t seed, float *params) {
     for (int k = 0; k < kMlpHidden * kMlpIn; ++k) params[kMlpW1 + k] = (2.0f * uni() - 1.0f) * a1;
     for (int k = 0; k < kMlpHidden * kMlpHidden; ++k) params[kMlpW2 + k] = (2.0f * uni() - 1.0f) * a2;
     for (int k = 0; k < kMlpOut * kMlpHidden; ++k) params[kMlpW3 + k] = (2.0f * uni() - 1.0f) * a3;
+    params[kMlpBound] = params[kMlpBound + 1] = 0.0f;
 }
 
 PmhResult pmhWarmstartForward(const float *params, const float *features, uint32_t n, float *outputs) {
@@ -363,6 +375,7 @@ PmhResult pmhWarmstartGradients(const float *samples, uint32_t count, const floa
         return r;
     PMH_CHECK(hipMemcpy(ds.p, samples, sizeof(float) * count * kMlpSample, hipMemcpyHostToDevice));
     PMH_CHECK(hipMemcpy(dp.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+    PMH_CHECK(hipMemset(dg.p, 0, sizeof(float) * kMlpParams));
     k_mlp_gradients<<<1, 64>>>(ds.p, count, dp.p, dg.p, dl.p);
     PMH_CHECK(hipGetLastError());
     PMH_CHECK(hipMemcpy(grads, dg.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
@@ -400,11 +413,17 @@ PmhResult pmhWarmstartTrain(const float *samples, uint32_t n, uint32_t epochs, f
     PMH_CHECK(hipMemcpy(dp.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     PMH_CHECK(hipMemset(dm.p, 0, sizeof(float) * kMlpParams));
     PMH_CHECK(hipMemset(dv.p, 0, sizeof(float) * kMlpParams));
+    PMH_CHECK(hipMemset(dg.p, 0, sizeof(float) * kMlpParams));
     PMH_CHECK(hipMemcpy(dorder.p, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice));
     k_mlp_train<<<1, 64>>>(ds.p, n, dorder.p, epochs, learning_rate, dp.p, dm.p, dv.p, dg.p, dl.p, dst.p);
     PMH_CHECK(hipGetLastError());
     PMH_CHECK(hipMemcpy(params, dp.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
     PMH_CHECK(hipMemcpy(epoch_loss, dl.p, sizeof(float) * epochs, hipMemcpyDeviceToHost));
+    for (int o = 0; o < kMlpOut; ++o) {
+        float hi = 0.0f;
+        for (uint32_t k = 0; k < n; ++k) hi = std::max(hi, samples[size_t(k) * kMlpSample + kMlpIn + o]);
+        params[kMlpBound + o] = hi;
+    }
     ds.release(); dp.release(); dm.release(); dv.release(); dg.release(); dl.release(); dst.release(); dorder.release();
     return PMH_SUCCESS;
 }

@@ -2,6 +2,7 @@
 // World API: shape upload preparation, device buffers, stepping and
 // result read-back in the reference's record layouts.
 #include "pmh_world.hpp"
+#include "pmh_warmstart.hpp"
 
 #include <algorithm>
 #include <array>
@@ -414,6 +415,7 @@ PmhResult World::ensure_device() {
         joints_dirty = true;   // exclusion lists are sized by the body count
     }
     if (joints_dirty && (r = upload_joints()) != PMH_SUCCESS) return r;
+    if ((r = reserve_warm()) != PMH_SUCCESS) return r;
     if (inputs_dirty) {
         // The step consumes and clears them (pm_clear_rigid_inputs).
         PMH_CHECK(hipMemcpyAsync(inputs_dev.p, host_inputs.data(), sizeof(float4) * host_inputs.size(),
@@ -517,6 +519,65 @@ PmhResult World::upload_joints() {
     return PMH_SUCCESS;
 }
 
+// Cold-point buffers of the learned warm start, when weights or the
+// contact log are set: up to 2^21 cold points a step.
+PmhResult World::reserve_warm() {
+    solver.mlp = warm_enabled ? mlp_dev.p : nullptr;
+    solver.mlp_policy = warm_policy;
+    solver.log = log_cap ? log_dev.p : nullptr;
+    solver.log_cap = log_cap;
+    if (!warm_enabled && !log_cap) return PMH_SUCCESS;
+    const uint32_t cap = std::max<uint32_t>(64, std::min<uint32_t>(solver.point_cap, 1u << 21));
+    PmhResult r;
+    if ((r = solver.cold_list.reserve(cap)) || (r = solver.cold_feat.reserve(size_t(cap) * kMlpIn)) ||
+        (r = solver.cold_scale.reserve(2 * size_t(cap))))
+        return r;
+    return PMH_SUCCESS;
+}
+
+// One step with the learned warm start in to-residual mode: when a phase
+// ends above the tolerance, the step is taken again from the same inputs
+// with the plain warm start (the step reads its pre-step states and never
+// writes them; the velocity history, the one-step inputs and the joints it
+// does write are restored from copies).
+PmhResult World::step_with_fallback(uint32_t k) {
+    const uint32_t n = body_count, nj = solver.nj;
+    PmhResult r;
+    if ((r = snap_vel.reserve(n)) || (r = snap_inputs.reserve(4 * size_t(n))) || (r = snap_flags.reserve(n)) ||
+        (nj && (r = snap_joints.reserve(nj))))
+        return r;
+    PMH_CHECK(hipMemcpyAsync(snap_vel.p, solver.prev_vel.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+    PMH_CHECK(hipMemcpyAsync(snap_inputs.p, inputs_dev.p, sizeof(float4) * 4 * n, hipMemcpyDeviceToDevice, stream));
+    PMH_CHECK(hipMemcpyAsync(snap_flags.p, target_flags_dev.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
+    if (nj)
+        PMH_CHECK(hipMemcpyAsync(snap_joints.p, joints_dev.p, sizeof(PmhRigidConstraint) * nj, hipMemcpyDeviceToDevice,
+                                 stream));
+    const int parity = cache_parity;
+    if ((r = step_once(k, true)) != PMH_SUCCESS) return r;
+    ResidualLog log;
+    PMH_CHECK(hipMemcpyAsync(&log, solver.residual.p, sizeof log, hipMemcpyDeviceToHost, stream));
+    PMH_CHECK(hipStreamSynchronize(stream));
+    bool failed = false;
+    for (uint32_t ph = 0; ph < 2; ++ph) {
+        const uint32_t ex = log.executed[ph];
+        if (ex == 0 || ex > kResidualSlots) continue;
+        const uint32_t slot = ph * kResidualSlots + ex - 1;
+        failed = failed || residual_rms(log.sum[slot], log.count[slot]) > residual_tolerance;
+    }
+    ++warm_steps;
+    if (!failed) return PMH_SUCCESS;
+    ++warm_fallbacks;
+    std::swap(state_cur.p, state_next.p);
+    cache_parity = parity;
+    PMH_CHECK(hipMemcpyAsync(solver.prev_vel.p, snap_vel.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
+    PMH_CHECK(hipMemcpyAsync(inputs_dev.p, snap_inputs.p, sizeof(float4) * 4 * n, hipMemcpyDeviceToDevice, stream));
+    PMH_CHECK(hipMemcpyAsync(target_flags_dev.p, snap_flags.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
+    if (nj)
+        PMH_CHECK(hipMemcpyAsync(joints_dev.p, snap_joints.p, sizeof(PmhRigidConstraint) * nj, hipMemcpyDeviceToDevice,
+                                 stream));
+    return step_once(k, false);
+}
+
 // The device records (state, rows, duals) back into the host copy.
 PmhResult World::pull_joints() {
     if (!joints_on_device) return PMH_SUCCESS;
@@ -535,6 +596,8 @@ void World::release() {
     jlist_start.release(); jlist.release(); jheavy.release();
     scene_of.release(); scene_count.release(); scene_pair_start.release(); scene_first_dev.release();
     scene_large_count.release(); scene_large_start.release(); scene_large.release();
+    mlp_dev.release(); log_dev.release(); snap_vel.release(); snap_inputs.release(); snap_flags.release();
+    snap_joints.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
     bucket_start.release(); bucket_entries.release(); large.release();
@@ -688,6 +751,64 @@ PmhResult pmhWorldAddBodies(PmhWorld world, uint32_t count, const PmhBodyParamet
     return PMH_SUCCESS;
 }
 
+PmhResult pmhWorldSetWarmstart(PmhWorld world, const float *params, uint32_t policy) {
+    if (!world || (params && (policy == 0u || (policy & ~3u) != 0u))) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    w->warm_policy = policy;
+    if (!params) {
+        w->warm_enabled = false;
+        return PMH_SUCCESS;
+    }
+    for (int k = 0; k < pmh::kMlpParams; ++k)
+        if (!std::isfinite(params[k])) return PMH_ERROR_INVALID_ARGUMENT;
+    if (hipSetDevice(w->device) != hipSuccess) return PMH_ERROR_DEVICE_LOST;
+    PmhResult r = w->mlp_dev.reserve(pmh::kMlpParams);
+    if (r != PMH_SUCCESS) return r;
+    PMH_CHECK(hipMemcpy(w->mlp_dev.p, params, sizeof(float) * pmh::kMlpParams, hipMemcpyHostToDevice));
+    w->warm_enabled = true;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSetContactLog(PmhWorld world, uint32_t capacity) {
+    if (!world) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (hipSetDevice(w->device) != hipSuccess) return PMH_ERROR_DEVICE_LOST;
+    w->log_cap = capacity;
+    if (capacity) {
+        PmhResult r = w->log_dev.reserve(size_t(capacity) * pmh::kMlpSample);
+        if (r != PMH_SUCCESS) return r;
+    }
+    if (w->solver.counters.p) {
+        PMH_CHECK(hipDeviceSynchronize());
+        const uint32_t zero = 0;
+        PMH_CHECK(hipMemcpy(&w->solver.counters.p->log_count, &zero, sizeof zero, hipMemcpyHostToDevice));
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetContactLog(PmhWorld world, float *samples, uint32_t capacity, uint32_t *count) {
+    if (!world || !count || (capacity && !samples)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    *count = 0;
+    if (!w->has_stepped || !w->log_cap) return PMH_SUCCESS;
+    PMH_CHECK(hipStreamSynchronize(w->stream));
+    uint32_t logged = 0;
+    PMH_CHECK(hipMemcpy(&logged, &w->solver.counters.p->log_count, sizeof logged, hipMemcpyDeviceToHost));
+    logged = std::min(logged, w->log_cap);
+    *count = logged;
+    const uint32_t copy = std::min(logged, capacity);
+    if (copy)
+        PMH_CHECK(hipMemcpy(samples, w->log_dev.p, sizeof(float) * size_t(copy) * pmh::kMlpSample, hipMemcpyDeviceToHost));
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldWarmstartStats(PmhWorld world, uint64_t *steps, uint64_t *fallbacks) {
+    if (!world || !steps || !fallbacks) return PMH_ERROR_INVALID_ARGUMENT;
+    *steps = W(world)->warm_steps;
+    *fallbacks = W(world)->warm_fallbacks;
+    return PMH_SUCCESS;
+}
+
 PmhResult pmhWorldSetSolveLanes(PmhWorld world, uint32_t lanes) {
     if (!world || (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)) return PMH_ERROR_INVALID_ARGUMENT;
     W(world)->solve_lanes = lanes;
@@ -731,8 +852,9 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
             w->solve_events.resize(2 * size_t(count), nullptr);
             for (size_t k = old; k < w->solve_events.size(); ++k) PMH_CHECK(hipEventCreate(&w->solve_events[k]));
         }
+        const bool fallback = w->warm_enabled && w->residual_tolerance > 0.0f;
         for (uint32_t k = 0; k < count; ++k) {
-            r = w->step_once(k);
+            r = fallback ? w->step_with_fallback(k) : w->step_once(k);
             if (r != PMH_SUCCESS) return r;
         }
         PMH_CHECK(hipStreamSynchronize(w->stream));

@@ -79,6 +79,16 @@ struct World {
 
     Solver solver;
 
+    // learned warm start
+    DBuf<float> mlp_dev, log_dev;
+    bool warm_enabled = false;
+    uint32_t warm_policy = 1u;
+    uint32_t log_cap = 0;
+    uint64_t warm_steps = 0, warm_fallbacks = 0;
+    DBuf<float4> snap_vel, snap_inputs;
+    DBuf<uint32_t> snap_flags;
+    DBuf<PmhRigidConstraint> snap_joints;
+
     // timing
     bool timing_enabled = false;
     hipEvent_t events[8] = {};
@@ -91,7 +101,9 @@ struct World {
     PmhResult ensure_device();
     PmhResult upload_joints();
     PmhResult pull_joints();
-    PmhResult step_once(uint32_t k);
+    PmhResult step_once(uint32_t k, bool warm_infer = true);
+    PmhResult step_with_fallback(uint32_t k);
+    PmhResult reserve_warm();
     void release();
 };
 

@@ -118,6 +118,10 @@ def lib():
         "pmhWorldGetProxies": (i32, [_P, _P]),
         "pmhPrepareSolveGeometryConstraints": (
             i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
+        "pmhWorldSetWarmstart": (i32, [_P, _P, _U]),
+        "pmhWorldSetContactLog": (i32, [_P, _U]),
+        "pmhWorldGetContactLog": (i32, [_P, _P, _U, _P]),
+        "pmhWorldWarmstartStats": (i32, [_P, _P, _P]),
         "pmhWarmstartForward": (i32, [_P, _P, _U, _P]),
         "pmhWarmstartGradients": (i32, [_P, _U, _P, _P, _P]),
         "pmhWarmstartTrain": (i32, [_P, _U, _U, _F, _U, _P, _P]),
@@ -222,6 +226,36 @@ class World:
     def set_solve_lanes(self, lanes):
         _check(lib().pmhWorldSetSolveLanes(self._h, lanes), "pmhWorldSetSolveLanes")
 
+    def set_warmstart(self, params, penalty=True, lam=False):
+        """Learned warm start for cold contacts (None: the plain one):
+        the predicted penalty and/or the predicted lambda (the latter only
+        where a contact starts in violation)."""
+        if params is None:
+            _check(lib().pmhWorldSetWarmstart(self._h, None, 0), "pmhWorldSetWarmstart")
+            return
+        self._warm = np.ascontiguousarray(params, dtype=np.float32)
+        policy = (WARMSTART_PENALTY if penalty else 0) | (WARMSTART_LAMBDA if lam else 0)
+        _check(lib().pmhWorldSetWarmstart(self._h, _ptr(self._warm), policy), "pmhWorldSetWarmstart")
+
+    def set_contact_log(self, capacity):
+        _check(lib().pmhWorldSetContactLog(self._h, capacity), "pmhWorldSetContactLog")
+
+    def contact_log(self, capacity=None):
+        """Logged cold-contact samples: 16 features,
+        log10(1 + lambda / (m_eff |g|)), log10(k / k0)."""
+        count = _U()
+        _check(lib().pmhWorldGetContactLog(self._h, None, 0, ctypes.byref(count)), "pmhWorldGetContactLog")
+        n = count.value if capacity is None else min(count.value, capacity)
+        out = np.zeros((n, 18), dtype=np.float32)
+        if n:
+            _check(lib().pmhWorldGetContactLog(self._h, _ptr(out), n, ctypes.byref(count)), "pmhWorldGetContactLog")
+        return out
+
+    def warmstart_stats(self):
+        steps, falls = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().pmhWorldWarmstartStats(self._h, ctypes.byref(steps), ctypes.byref(falls)),
+               "pmhWorldWarmstartStats")
+        return steps.value, falls.value
 
     @property
     def body_count(self):
@@ -411,9 +445,11 @@ def autotune_lanes(world, steps=8, candidates=(8, 16, 32, 64)):
 # ---------------------------------------------------------------------------
 # Learned warm start: a 16 -> 64 -> 64 -> 2 network on MFMA (BF16 operands,
 # FP32 accumulation); parameters packed W1[64][16], b1, W2[64][64], b2,
-# W3[2][64], b3 (5378 floats).
+# W3[2][64], b3, then the largest training target of each output, which
+# bounds the predictions (5380 floats).
 # ---------------------------------------------------------------------------
-MLP_PARAMS = 5378
+MLP_PARAMS = 5380
+WARMSTART_PENALTY, WARMSTART_LAMBDA = 1, 2
 MLP_FEATURES = 16
 MLP_SAMPLE = 18
 
