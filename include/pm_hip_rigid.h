@@ -155,6 +155,11 @@ typedef struct PmhWorldDesc {
     float residual_tolerance;        /* > 0: each solve phase iterates until the constraint residual
                                         (pmhWorldGetResiduals) is at most this, the iteration budget
                                         being the cap; 0 = the fixed budget of the profile */
+    uint32_t substeps;               /* substeps per pmhWorldStep step, 0 = 1, at most 64: each of
+                                        timestep / substeps; forces act in every substep, impulses in
+                                        the first, a kinematic target moves 1/remaining of the way per
+                                        substep (pm_integrate_rigid_bodies' remaining_substeps,
+                                        geometry.slang:524-612) and inputs clear after the last */
 } PmhWorldDesc;
 
 #define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */
@@ -309,6 +314,17 @@ uint32_t pmhWorldJointCount(PmhWorld world);
  * src/api/domains.cpp:1585):
  * enabled/broken state, applied impulse, dense ids, rows and list links. */
 PmhResult pmhWorldGetJoints(PmhWorld world, PmhRigidConstraint *joints);
+/* pmCmdWriteRigidConstraints (resources.cpp:998-1060): each joint slot
+ * first..first+count-1 is removed (remove_rigid_constraint, runtime_rigid.cu.inc:
+ * 639-654: not alive, state cleared, generation advanced) and, where
+ * active[k] != 0, added again from options[k] with a fresh state and zero rows
+ * (add_rigid_constraint, :552-596). The slot is the joint's index; the
+ * reference re-adds into its first free slot, which is the same slot whenever
+ * removed joints are re-added in index order. Invalid options (type, body ids,
+ * no dynamic body, breaking threshold, solver_iterations 1..64) fail the whole
+ * call with PMH_ERROR_INVALID_ARGUMENT before anything changes. */
+PmhResult pmhWorldWriteJoints(PmhWorld world, uint32_t first, uint32_t count,
+                              const PmhRigidConstraintOptions *options, const uint32_t *active);
 
 /* PmRigidInput (include/parallel_mater/parallel_mater.h:421-427). */
 typedef struct PmhRigidInput {
@@ -331,6 +347,8 @@ PmhResult pmhWorldApplyRigidInputs(PmhWorld world, uint32_t count, const PmhRigi
 PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t count,
                                       const PmhRigidBodyState *targets);
 
+/* Worlds of fewer than 4 bodies return PMH_ERROR_FEATURE_NOT_PRESENT (not
+ * supported on the device yet). */
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
 
 /* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;
@@ -338,6 +356,14 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
  * contact cache). */
 PmhResult pmhWorldCopyStates(PmhWorld world, PmhRigidBodyState *states);
 PmhResult pmhWorldSetStates(PmhWorld world, const PmhRigidBodyState *states);
+/* pmCmdSetRigidStates (resources.cpp:1293-1330 family) -> set_rigid_body_state
+ * (runtime_rigid.cu.inc:146-195) for bodies first..first+count-1: each state
+ * (finite, non-zero orientation, stored normalised) becomes the current one;
+ * the body's solver history, pending inputs and kinematic target are
+ * cleared and the rows of every alive joint on the body are zeroed. The
+ * contact cache is kept. */
+PmhResult pmhWorldSetRigidStates(PmhWorld world, uint32_t first, uint32_t count,
+                                 const PmhRigidBodyState *states);
 
 PmhResult pmhWorldGetStats(PmhWorld world, PmhWorldStats *stats);
 PmhResult pmhWorldGetPhaseTimes(PmhWorld world, PmhPhaseTimes *times);
@@ -390,6 +416,21 @@ uint32_t pmhWorldPairCount(PmhWorld world);
  * larger world this call returns PMH_ERROR_CAPACITY_EXCEEDED. */
 PmhResult pmhWorldGetPairs(PmhWorld world, uint32_t *codes);
 PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *manifolds);
+/* PmContactEvent (parallel_mater.h:959-968), 56 bytes. */
+typedef struct PmhContactEvent {
+    uint32_t type;               /* 0: rigid-rigid */
+    uint32_t first, second;      /* body indices of the pair (a, b) */
+    PmhVec3 position, normal;
+    float penetration;           /* max(0, penetration) */
+    float normal_impulse;        /* accumulated if the contact persisted, else reported */
+    PmhVec3 friction_impulse;
+} PmhContactEvent;
+/* pmCmdCopyContactEvents: the last step's rigid contact events, one per
+ * contact point in active-pair order (pm_prepare_parallel_contact_events and
+ * pm_save_rigid_contact_cache, geometry.slang:2904-2936, 3023-3037). Writes
+ * min(total, capacity) events; *count receives the total. */
+PmhResult pmhWorldGetContactEvents(PmhWorld world, PmhContactEvent *events, uint32_t capacity,
+                                   uint32_t *count);
 PmhResult pmhWorldGetAvbdBodies(PmhWorld world, PmhAvbdBody *bodies);
 PmhResult pmhWorldGetSchedule(PmhWorld world, PmhContactSchedule *schedule);
 PmhResult pmhWorldGetProxies(PmhWorld world, float *bounds);

@@ -2574,12 +2574,14 @@ struct World {
     std::vector<std::array<V3, 4>> accumulators;   // force, torque, impulse, angular impulse
     std::vector<PmoState> targets;
     std::vector<uint8_t> target_active;
+    uint32_t substep = 0;                          // substep of the current frame
 };
 
-// pm_integrate_rigid_bodies, geometry.slang:524-612, one substep
-// (remaining_substeps = 1, impulses applied).
+// pm_integrate_rigid_bodies, geometry.slang:524-612: one substep of a
+// frame, with remaining_substeps substeps left (the kinematic target moves
+// 1/remaining of the way) and the impulses applied in the frame's first.
 PmoState integrate(const PmoParams &b, const PmoState &prev, const std::array<V3, 4> &acc, bool target_active,
-                   const PmoState &target, V3 g, float dt) {
+                   const PmoState &target, V3 g, float dt, uint32_t remaining, bool apply_impulses) {
     PmoState next = prev;
     if (b.motion == kStatic) {
         next.linear_velocity = {0, 0, 0};
@@ -2588,7 +2590,7 @@ PmoState integrate(const PmoParams &b, const PmoState &prev, const std::array<V3
     }
     if (b.motion == kKinematic) {
         if (target_active) {
-            const float fraction = 1.0f / float(1u);
+            const float fraction = 1.0f / float(remaining);
             next.position = add(prev.position, mul(sub(target.position, prev.position), fraction));
             Q t = target.orientation;
             const Q o = prev.orientation;
@@ -2606,8 +2608,10 @@ PmoState integrate(const PmoParams &b, const PmoState &prev, const std::array<V3
     const V3 force = acc[0], torque = acc[1], imp = acc[2], aimp = acc[3];
     next.linear_velocity = add(next.linear_velocity, mul(add(g, mul(force, b.inverse_mass)), dt));
     next.angular_velocity = add(next.angular_velocity, mul(inverse_inertia_world(b, next, torque), dt));
-    next.linear_velocity = add(next.linear_velocity, mul(imp, b.inverse_mass));
-    next.angular_velocity = add(next.angular_velocity, inverse_inertia_world(b, next, aimp));
+    if (apply_impulses) {
+        next.linear_velocity = add(next.linear_velocity, mul(imp, b.inverse_mass));
+        next.angular_velocity = add(next.angular_velocity, inverse_inertia_world(b, next, aimp));
+    }
     next.linear_velocity = mul(next.linear_velocity, 1.0f / (1.0f + b.linear_damping * dt));
     next.angular_velocity = mul(next.angular_velocity, 1.0f / (1.0f + b.angular_damping * dt));
     next.linear_velocity = clamp_length(next.linear_velocity, b.maximum_linear_speed);
@@ -2858,16 +2862,22 @@ void save_cache(World &w) {
 void world_step(World &w) {
     const uint32_t n = uint32_t(w.params.size());
     const V3 g = {w.desc.gravity[0], w.desc.gravity[1], w.desc.gravity[2]};
-    const float dt = w.desc.dt;
+    const uint32_t substeps = w.desc.substeps ? w.desc.substeps : 1u;
+    const float dt = substeps == 1u ? w.desc.dt : w.desc.dt / float(substeps);
     if (w.cache_bodies != n) { w.cache.clear(); w.cache_bodies = n; }
     std::vector<PmoState> prev = w.states, pred(n);
     w.accumulators.resize(n, std::array<V3, 4>{});
     w.targets.resize(n, PmoState{});
     w.target_active.resize(n, 0);
     for (uint32_t i = 0; i < n; ++i)
-        pred[i] = integrate(w.params[i], prev[i], w.accumulators[i], w.target_active[i] != 0, w.targets[i], g, dt);
-    std::fill(w.accumulators.begin(), w.accumulators.end(), std::array<V3, 4>{});
-    std::fill(w.target_active.begin(), w.target_active.end(), 0);
+        pred[i] = integrate(w.params[i], prev[i], w.accumulators[i], w.target_active[i] != 0, w.targets[i], g, dt,
+                            substeps - w.substep, w.substep == 0u);
+    // pm_clear_rigid_inputs after the frame's last substep.
+    if (w.substep + 1u == substeps) {
+        std::fill(w.accumulators.begin(), w.accumulators.end(), std::array<V3, 4>{});
+        std::fill(w.target_active.begin(), w.target_active.end(), 0);
+    }
+    w.substep = (w.substep + 1u) % substeps;
     w.proxies.resize(n);
     for (uint32_t i = 0; i < n; ++i)
         w.proxies[i] = make_proxy(w.params[i], w.shapes[w.params[i].mesh_index], prev[i], pred[i]);
@@ -3075,14 +3085,107 @@ int pmo_world_add_joints(void *world, uint32_t count, const PmoJointOptions *opt
         PmoJoint j;
         std::memset(&j, 0, sizeof j);
         j.options = options[k];
+        j.options.local_orientation_a = qnormalize(j.options.local_orientation_a);
+        j.options.local_orientation_b = qnormalize(j.options.local_orientation_b);
         j.generation = 1u;
         j.alive = 1u;
+        j.state_enabled = j.options.enabled;            // add_rigid_constraint, runtime_rigid.cu.inc:588-589
         j.dense_a = j.dense_b = kInvalid;
         j.avbd_next_a = j.avbd_next_b = kInvalid;
         w.joints.push_back(j);
     }
     return 0;
 }
+// validate_constraint_options (runtime.cpp:400-456, the checks a joint
+// record can fail here) plus add_rigid_constraint's body checks (:552-567).
+static bool joint_options_valid(const World &w, const PmoJointOptions &o) {
+    const uint32_t n = uint32_t(w.params.size());
+    if (o.type > kMotor || o.body_a.index >= n || o.body_b.index >= n) return false;
+    if (w.params[o.body_a.index].motion != kDynamic && w.params[o.body_b.index].motion != kDynamic) return false;
+    if (!std::isfinite(o.breaking_impulse_threshold) || o.breaking_impulse_threshold < 0.0f) return false;
+    return o.solver_iterations >= 1u && o.solver_iterations <= 64u;
+}
+
+int pmo_world_write_joints(void *world, uint32_t first, uint32_t count, const PmoJointOptions *options,
+                           const uint32_t *active) {
+    World &w = *static_cast<World *>(world);
+    if (uint64_t(first) + count > w.joints.size() || (count && (!options || !active))) return -1;
+    for (uint32_t k = 0; k < count; ++k)
+        if (active[k] && !joint_options_valid(w, options[k])) return -1;
+    for (uint32_t k = 0; k < count; ++k) {
+        PmoJoint &j = w.joints[first + k];
+        if (j.alive) {                                  // remove_rigid_constraint
+            j.alive = 0u;
+            j.state_enabled = j.state_broken = 0u;
+            j.applied_impulse = 0.0f;
+            j.generation = j.generation + 1u ? j.generation + 1u : 1u;
+        }
+        if (!active[k]) continue;
+        const uint32_t generation = j.generation ? j.generation : 1u;
+        std::memset(&j, 0, sizeof j);                   // add_rigid_constraint
+        j.options = options[k];
+        j.options.local_orientation_a = qnormalize(j.options.local_orientation_a);
+        j.options.local_orientation_b = qnormalize(j.options.local_orientation_b);
+        j.generation = generation;
+        j.alive = 1u;
+        j.state_enabled = j.options.enabled;
+        j.dense_a = j.dense_b = kInvalid;
+        j.avbd_next_a = j.avbd_next_b = kInvalid;
+    }
+    return 0;
+}
+
+int pmo_world_set_rigid_states(void *world, uint32_t first, uint32_t count, const PmoState *states) {
+    World &w = *static_cast<World *>(world);
+    const uint32_t n = uint32_t(w.params.size());
+    if (uint64_t(first) + count > n || (count && !states)) return -1;
+    for (uint32_t k = 0; k < count; ++k) {
+        const float *f = reinterpret_cast<const float *>(&states[k]);
+        for (size_t c = 0; c < sizeof(PmoState) / sizeof(float); ++c)
+            if (!std::isfinite(f[c])) return -1;
+        const Q o = states[k].orientation;
+        if (o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w <= kEpsilon * kEpsilon) return -1;
+    }
+    w.accumulators.resize(n, std::array<V3, 4>{});
+    w.target_active.resize(n, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t i = first + k;
+        PmoState s = states[k];
+        s.orientation = qnormalize(s.orientation);
+        w.states[i] = s;
+        if (i < w.bodies.size()) w.bodies[i] = PmoAvbdBody{};
+        w.accumulators[i] = std::array<V3, 4>{};
+        w.target_active[i] = 0;
+        for (PmoJoint &j : w.joints)
+            if (j.alive && (j.options.body_a.index == i || j.options.body_b.index == i))
+                std::memset(j.rows, 0, sizeof j.rows);
+    }
+    return 0;
+}
+
+uint32_t pmo_world_contact_events(void *world, PmoContactEvent *events, uint32_t capacity) {
+    World &w = *static_cast<World *>(world);
+    const uint32_t n = uint32_t(w.params.size());
+    uint32_t cursor = 0;
+    for (size_t m = 0; m < w.pairs.size(); ++m) {
+        const PmoManifold &mf = w.manifolds[m];
+        for (uint32_t k = 0; k < mf.count; ++k, ++cursor) {
+            if (cursor >= capacity) continue;
+            const PmoContact &c = mf.contacts[k];
+            PmoContactEvent &e = events[cursor];
+            e.type = 0u;
+            e.first = w.pairs[m] / n;
+            e.second = w.pairs[m] % n;
+            e.position = c.point;
+            e.normal = c.normal;
+            e.penetration = std::max(0.0f, c.penetration);
+            e.normal_impulse = c.persistent ? c.accumulated_normal_impulse : c.reported_normal_impulse;
+            e.friction_impulse = c.persistent ? c.accumulated_friction_impulse : c.reported_friction_impulse;
+        }
+    }
+    return cursor;
+}
+
 void pmo_world_residuals(void *world, float *out, float *maximum, uint32_t *executed) {
     World &w = *static_cast<World *>(world);
     std::copy(w.residual.begin(), w.residual.end(), out);
@@ -3101,7 +3204,8 @@ int pmo_world_step(void *world, uint32_t steps) {
     uint64_t n = w.params.size();
     if (n * n > 0xffffffffull) return -2;  // pair codes a*n+b must fit 32 bits
     const auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t s = 0; s < steps; ++s) world_step(w);
+    const uint32_t substeps = w.desc.substeps ? w.desc.substeps : 1u;
+    for (uint64_t s = 0; s < uint64_t(steps) * substeps; ++s) world_step(w);
     w.step_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
 }

@@ -206,6 +206,8 @@ typedef struct PmoWorldDesc {
     uint32_t flags;                /* bit 0: brute-force broadphase */
     float residual_tolerance;      /* > 0: iterate each phase until the constraint residual
                                       is at most this (the budget caps); 0: fixed budget */
+    uint32_t substeps;             /* substeps per step of dt / substeps (0 = 1), as
+                                      pm_integrate_rigid_bodies' remaining_substeps */
 } PmoWorldDesc;
 
 typedef struct PmoWorldStats {
@@ -252,6 +254,28 @@ int pmo_world_set_kinematic_targets(void *world, uint32_t first, uint32_t count,
 /* Joints between bodies of the world (alive, generation 1); returns 0 or
  * -1 for an invalid type or body. */
 int pmo_world_add_joints(void *world, uint32_t count, const PmoJointOptions *options);
+/* pmCmdWriteRigidConstraints (resources.cpp:998-1060): each slot first..
+ * first+count-1 is removed (remove_rigid_constraint, runtime_rigid.cu.inc:
+ * 639-654) and, where active[k] != 0, added again with options[k] and a
+ * fresh state (add_rigid_constraint, :552-596). 0, or -1 on bad input. */
+int pmo_world_write_joints(void *world, uint32_t first, uint32_t count, const PmoJointOptions *options,
+                           const uint32_t *active);
+/* set_rigid_body_state (runtime_rigid.cu.inc:146-195) for bodies first..
+ * first+count-1: the state (orientation normalised) becomes current and
+ * previous, the body's solver history, inputs and kinematic target are
+ * cleared and the rows of every alive joint on the body are zeroed. */
+int pmo_world_set_rigid_states(void *world, uint32_t first, uint32_t count, const PmoState *states);
+/* The last step's rigid contact events (pm_prepare_parallel_contact_events
+ * and pm_save_rigid_contact_cache, geometry.slang:2904-2936, 3023-3037):
+ * one per manifold point in active-pair order, PmContactEvent layout
+ * (parallel_mater.h:959-968). Writes min(total, capacity); returns total. */
+typedef struct PmoContactEvent {
+    uint32_t type, first, second;
+    PmoVec3 position, normal;
+    float penetration, normal_impulse;
+    PmoVec3 friction_impulse;
+} PmoContactEvent;
+uint32_t pmo_world_contact_events(void *world, PmoContactEvent *events, uint32_t capacity);
 uint32_t pmo_world_joint_count(void *world);
 void pmo_world_get_joints(void *world, PmoJoint *out);
 int pmo_world_step(void *world, uint32_t steps);

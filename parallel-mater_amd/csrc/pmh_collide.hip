@@ -57,6 +57,7 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
                                   const PmhRigidBodyState *cur, PmhRigidBodyState *prev,
                                   PmhRigidBodyState *pred, float4 *pmn, float4 *pmx, V3 g, float dt,
                                   float4 *inputs, const PmhRigidBodyState *targets, uint32_t *target_flags,
+                                  uint32_t remaining, uint32_t first_substep, uint32_t last_substep,
                                   StepCounters *ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -70,9 +71,10 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
         s.angular_velocity = {0.0f, 0.0f, 0.0f};
     } else if (b.motion == kKinematic) {
         if (target_active) {
-            // One substep: the target is reached (fraction 1).
+            // The target moves 1/remaining of the way each substep and is
+            // reached in the frame's last.
             const PmhRigidBodyState t = targets[i];
-            const float fraction = 1.0f / float(1u);
+            const float fraction = 1.0f / float(remaining);
             const V3 p0 = sv(s0.position);
             const V3 p1 = add(p0, mul(sub(sv(t.position), p0), fraction));
             const Q o = sq(s0.orientation);
@@ -93,8 +95,10 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
         const V3 imp = v3(inputs[4 * i + 2]), aimp = v3(inputs[4 * i + 3]);
         V3 v = add(sv(s.linear_velocity), mul(add(g, mul(force, b.inverse_mass)), dt));
         V3 w = add(sv(s.angular_velocity), mul(inverse_inertia_world(b, sq(s.orientation), torque), dt));
-        v = add(v, mul(imp, b.inverse_mass));
-        w = add(w, inverse_inertia_world(b, sq(s.orientation), aimp));
+        if (first_substep) {
+            v = add(v, mul(imp, b.inverse_mass));
+            w = add(w, inverse_inertia_world(b, sq(s.orientation), aimp));
+        }
         v = mul(v, 1.0f / (1.0f + b.linear_damping * dt));
         w = mul(w, 1.0f / (1.0f + b.angular_damping * dt));
         v = clamp_length(v, b.maximum_linear_speed);
@@ -109,13 +113,16 @@ __global__ void k_integrate_proxy(uint32_t n, const PmhBodyParameters *P, const 
         s.linear_velocity = pv(v);
         s.angular_velocity = pv(w);
     }
-    // pm_clear_rigid_inputs (geometry.slang:2877-2886): inputs last one step.
-    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    inputs[4 * i] = z;
-    inputs[4 * i + 1] = z;
-    inputs[4 * i + 2] = z;
-    inputs[4 * i + 3] = z;
-    if (target_active) target_flags[i] = 0u;
+    // pm_clear_rigid_inputs (geometry.slang:2877-2886) after the frame's
+    // last substep: inputs and targets last one frame.
+    if (last_substep) {
+        const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        inputs[4 * i] = z;
+        inputs[4 * i + 1] = z;
+        inputs[4 * i + 2] = z;
+        inputs[4 * i + 3] = z;
+        if (target_active) target_flags[i] = 0u;
+    }
     pred[i] = s;
     // Proxy: production arguments path_deviation 0, correction_budget 1,
     // epsilon 1e-5 (production_realtime_backend.cu.inc:2663-2684).
@@ -1879,7 +1886,9 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     k_step_begin<<<1, 64, 0, s>>>(ctr);
     PmhRigidBodyState *cur = state_cur.p, *prev = state_prev.p, *pred = state_next.p;
     k_integrate_proxy<<<bg, 256, 0, s>>>(n, params.p, shapes_dev.p, cur, prev, pred, proxy_mn.p, proxy_mx.p,
-                                         gravity, dt, inputs_dev.p, targets_dev.p, target_flags_dev.p, ctr);
+                                         gravity, dt, inputs_dev.p, targets_dev.p, target_flags_dev.p,
+                                         substeps - substep, substep == 0 ? 1u : 0u,
+                                         substep + 1 == substeps ? 1u : 0u, ctr);
     mark(1);
     JointFilter jf;
     SceneRules rules;
@@ -1979,6 +1988,7 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     mark(6);
     cache_parity = wr;
     std::swap(state_cur.p, state_next.p);
+    substep = (substep + 1) % substeps;
     PMH_CHECK(hipGetLastError());
     return PMH_SUCCESS;
 }

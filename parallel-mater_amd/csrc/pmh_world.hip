@@ -553,6 +553,7 @@ PmhResult World::step_with_fallback(uint32_t k) {
         PMH_CHECK(hipMemcpyAsync(snap_joints.p, joints_dev.p, sizeof(PmhRigidConstraint) * nj, hipMemcpyDeviceToDevice,
                                  stream));
     const int parity = cache_parity;
+    const uint32_t sub = substep;
     if ((r = step_once(k, true)) != PMH_SUCCESS) return r;
     ResidualLog log;
     PMH_CHECK(hipMemcpyAsync(&log, solver.residual.p, sizeof log, hipMemcpyDeviceToHost, stream));
@@ -569,6 +570,7 @@ PmhResult World::step_with_fallback(uint32_t k) {
     ++warm_fallbacks;
     std::swap(state_cur.p, state_next.p);
     cache_parity = parity;
+    substep = sub;
     PMH_CHECK(hipMemcpyAsync(solver.prev_vel.p, snap_vel.p, sizeof(float4) * n, hipMemcpyDeviceToDevice, stream));
     PMH_CHECK(hipMemcpyAsync(inputs_dev.p, snap_inputs.p, sizeof(float4) * 4 * n, hipMemcpyDeviceToDevice, stream));
     PMH_CHECK(hipMemcpyAsync(target_flags_dev.p, snap_flags.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
@@ -639,7 +641,12 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     World *w = new (std::nothrow) World();
     if (!w) return PMH_ERROR_OUT_OF_HOST_MEMORY;
     w->gravity = {desc->gravity[0], desc->gravity[1], desc->gravity[2]};
-    w->dt = desc->timestep;
+    if (desc->substeps > 64) {
+        delete w;
+        return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    w->substeps = desc->substeps ? desc->substeps : 1u;
+    w->dt = w->substeps == 1u ? desc->timestep : desc->timestep / float(w->substeps);
     w->profile = desc->profile_version;
     w->base_iterations = desc->base_iterations ? desc->base_iterations : 10;
     w->requested_iterations = desc->requested_iterations;
@@ -836,17 +843,24 @@ uint32_t pmhWorldSceneCount(PmhWorld world) { return world ? uint32_t(W(world)->
 
 uint32_t pmhWorldBodyCount(PmhWorld world) { return world ? W(world)->body_count : 0; }
 
+constexpr uint32_t kMinStepBodies = 4;
+
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     if (!world) return PMH_ERROR_INVALID_ARGUMENT;
     World *w = W(world);
     if (w->body_count == 0 || steps == 0) return PMH_SUCCESS;
+    // Worlds of fewer than kMinStepBodies bodies are refused: the first
+    // steps of a two- and a three-body world (no active pairs) did not
+    // return on the device, and no cause has been shown yet.
+    if (w->body_count < kMinStepBodies) return PMH_ERROR_FEATURE_NOT_PRESENT;
     PmhResult r = w->ensure_device();
     if (r != PMH_SUCCESS) return r;
-    // Steps are enqueued back to back; each step's solve launch is
+    // Substeps are enqueued back to back; each one's solve launch is
     // bracketed by its own event pair, read once the batch is done.
+    const uint64_t total = uint64_t(steps) * w->substeps;
     const uint32_t batch = 256;
-    for (uint32_t done = 0; done < steps; done += batch) {
-        const uint32_t count = std::min(batch, steps - done);
+    for (uint64_t done = 0; done < total; done += batch) {
+        const uint32_t count = uint32_t(std::min<uint64_t>(batch, total - done));
         if (w->timing_enabled && w->solve_events.size() < 2 * size_t(count)) {
             const size_t old = w->solve_events.size();
             w->solve_events.resize(2 * size_t(count), nullptr);
@@ -939,6 +953,25 @@ PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t c
     return PMH_SUCCESS;
 }
 
+// add_rigid_constraint (runtime_rigid.cu.inc:552-596): the options with
+// normalised local frames, a fresh state enabled as the options say, zero
+// rows, alive.
+static PmhRigidConstraint new_joint(const PmhRigidConstraintOptions &o, uint32_t generation) {
+    PmhRigidConstraint j;
+    std::memset(&j, 0, sizeof j);
+    j.options = o;
+    for (PmhQuat *q : {&j.options.local_orientation_a, &j.options.local_orientation_b}) {
+        const pmh::Q n = pmh::qnormalize(pmh::Q{q->x, q->y, q->z, q->w});
+        *q = {n.x, n.y, n.z, n.w};
+    }
+    j.generation = generation ? generation : 1u;
+    j.alive = 1u;
+    j.state_enabled = o.enabled;
+    j.dense_a = j.dense_b = pmh::kInvalid;
+    j.avbd_next_a = j.avbd_next_b = pmh::kInvalid;
+    return j;
+}
+
 PmhResult pmhWorldAddJoints(PmhWorld world, uint32_t count, const PmhRigidConstraintOptions *options) {
     if (!world || (count && !options)) return PMH_ERROR_INVALID_ARGUMENT;
     World *w = W(world);
@@ -947,14 +980,7 @@ PmhResult pmhWorldAddJoints(PmhWorld world, uint32_t count, const PmhRigidConstr
     PmhResult r = w->pull_joints();
     if (r != PMH_SUCCESS) return r;
     for (uint32_t k = 0; k < count; ++k) {
-        PmhRigidConstraint j;
-        std::memset(&j, 0, sizeof j);
-        j.options = options[k];
-        j.generation = 1u;
-        j.alive = 1u;
-        j.dense_a = j.dense_b = pmh::kInvalid;
-        j.avbd_next_a = j.avbd_next_b = pmh::kInvalid;
-        w->host_joints.push_back(j);
+        w->host_joints.push_back(new_joint(options[k], 1u));
     }
     w->joints_dirty = true;
     return PMH_SUCCESS;
@@ -994,6 +1020,123 @@ PmhResult pmhWorldSetStates(PmhWorld world, const PmhRigidBodyState *states) {
         PMH_CHECK(hipMemset(w->cache_n.p, 0, sizeof(uint32_t) * 2));
         PMH_CHECK(hipMemset(w->solver.prev_vel.p, 0, sizeof(float4) * w->body_count));
     }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSetRigidStates(PmhWorld world, uint32_t first, uint32_t count, const PmhRigidBodyState *states) {
+    if (!world || (count && !states)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    const uint32_t n = w->body_count;
+    if (uint64_t(first) + count > n) return PMH_ERROR_INVALID_ARGUMENT;
+    std::vector<PmhRigidBodyState> s(states, states + count);
+    for (PmhRigidBodyState &x : s) {
+        const float *f = reinterpret_cast<const float *>(&x);
+        for (size_t c = 0; c < sizeof x / sizeof(float); ++c)
+            if (!std::isfinite(f[c])) return PMH_ERROR_INVALID_ARGUMENT;
+        const PmhQuat o = x.orientation;
+        if (o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w <= pmh::kEpsilon * pmh::kEpsilon)
+            return PMH_ERROR_INVALID_ARGUMENT;
+        const pmh::Q q = pmh::qnormalize({o.x, o.y, o.z, o.w});
+        x.orientation = {q.x, q.y, q.z, q.w};
+    }
+    PmhResult r = w->joints_dirty ? PMH_SUCCESS : w->pull_joints();
+    if (r != PMH_SUCCESS) return r;
+    std::copy(s.begin(), s.end(), w->host_states.begin() + first);
+    // Pending inputs and targets of these bodies are dropped.
+    for (uint32_t i = first; i < first + count; ++i) {
+        if (4 * size_t(i) + 3 < w->host_inputs.size())
+            for (int q = 0; q < 4; ++q) w->host_inputs[4 * size_t(i) + q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i < w->host_target_flags.size()) w->host_target_flags[i] = 0u;
+    }
+    bool rows = false;
+    for (PmhRigidConstraint &j : w->host_joints)
+        if (j.alive && ((j.options.body_a.index >= first && j.options.body_a.index < first + count) ||
+                        (j.options.body_b.index >= first && j.options.body_b.index < first + count))) {
+            std::memset(j.rows, 0, sizeof j.rows);
+            rows = true;
+        }
+    if (rows) w->joints_dirty = true;
+    if (w->has_stepped && !w->bodies_dirty && count) {
+        PMH_CHECK(hipStreamSynchronize(w->stream));
+        PMH_CHECK(hipMemcpy(w->state_cur.p + first, s.data(), sizeof(PmhRigidBodyState) * count,
+                            hipMemcpyHostToDevice));
+        PMH_CHECK(hipMemset(w->solver.prev_vel.p + first, 0, sizeof(float4) * count));
+        PMH_CHECK(hipMemset(w->inputs_dev.p + 4 * size_t(first), 0, sizeof(float4) * 4 * count));
+        PMH_CHECK(hipMemset(w->target_flags_dev.p + first, 0, sizeof(uint32_t) * count));
+    }
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldWriteJoints(PmhWorld world, uint32_t first, uint32_t count, const PmhRigidConstraintOptions *options,
+                              const uint32_t *active) {
+    if (!world || (count && (!options || !active))) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    if (uint64_t(first) + count > w->host_joints.size()) return PMH_ERROR_INVALID_ARGUMENT;
+    const uint32_t n = w->body_count;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (!active[k]) continue;
+        const PmhRigidConstraintOptions &o = options[k];
+        if (o.type > pmh::kJointMotor || o.body_a.index >= n || o.body_b.index >= n ||
+            !std::isfinite(o.breaking_impulse_threshold) || o.breaking_impulse_threshold < 0.0f ||
+            o.solver_iterations < 1u || o.solver_iterations > 64u)
+            return PMH_ERROR_INVALID_ARGUMENT;
+        if (w->host_params[o.body_a.index].motion != pmh::kDynamic &&
+            w->host_params[o.body_b.index].motion != pmh::kDynamic)
+            return PMH_ERROR_INVALID_ARGUMENT;
+    }
+    PmhResult r = w->joints_dirty ? PMH_SUCCESS : w->pull_joints();
+    if (r != PMH_SUCCESS) return r;
+    for (uint32_t k = 0; k < count; ++k) {
+        PmhRigidConstraint &j = w->host_joints[first + k];
+        if (j.alive) {                                  // remove_rigid_constraint
+            j.alive = 0u;
+            j.state_enabled = j.state_broken = 0u;
+            j.applied_impulse = 0.0f;
+            j.generation = j.generation + 1u ? j.generation + 1u : 1u;
+        }
+        if (active[k]) j = new_joint(options[k], j.generation);
+    }
+    w->joints_dirty = true;
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldGetContactEvents(PmhWorld world, PmhContactEvent *events, uint32_t capacity, uint32_t *count) {
+    if (!world || !count || (capacity && !events)) return PMH_ERROR_INVALID_ARGUMENT;
+    World *w = W(world);
+    *count = 0;
+    if (!w->has_stepped) return PMH_SUCCESS;
+    pmh::Solver &v = w->solver;
+    pmh::StepCounters c;
+    PMH_CHECK(hipStreamSynchronize(w->stream));
+    PMH_CHECK(hipMemcpy(&c, v.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    const uint32_t M = c.pairs, P = c.points;
+    std::vector<uint32_t> m_count, m_start, flags;
+    std::vector<uint2> ab;
+    std::vector<float4> np, pt, acc, rep;
+    PmhResult r;
+    if ((r = fetch(ab, v.ab.p, M)) || (r = fetch(m_count, v.m_count.p, M)) || (r = fetch(m_start, v.m_start.p, M)))
+        return r;
+    if (capacity && ((r = fetch(np, v.c_np.p, P)) || (r = fetch(pt, v.c_pt.p, P)) || (r = fetch(acc, v.c_acc.p, P)) ||
+                     (r = fetch(rep, v.c_rep.p, P)) || (r = fetch(flags, v.c_flags.p, P))))
+        return r;
+    uint32_t cursor = 0;
+    for (uint32_t m = 0; m < M; ++m)
+        for (uint32_t k = 0; k < m_count[m]; ++k, ++cursor) {
+            if (cursor >= capacity) continue;
+            const uint32_t i = m_start[m] + k;
+            const bool persistent = ((flags[i] >> 1) & 1u) != 0u;
+            const float4 imp = persistent ? acc[i] : rep[i];
+            PmhContactEvent &e = events[cursor];
+            e.type = 0u;
+            e.first = ab[m].x;
+            e.second = ab[m].y;
+            e.position = {pt[i].x, pt[i].y, pt[i].z};
+            e.normal = {np[i].x, np[i].y, np[i].z};
+            e.penetration = std::max(0.0f, np[i].w);
+            e.normal_impulse = imp.x;
+            e.friction_impulse = {imp.y, imp.z, imp.w};
+        }
+    *count = cursor;
     return PMH_SUCCESS;
 }
 

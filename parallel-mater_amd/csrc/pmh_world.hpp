@@ -8,7 +8,8 @@ namespace pmh {
 struct World {
     // configuration
     V3 gravity{0.0f, -9.81f, 0.0f};
-    float dt = 1.0f / 60.0f;
+    float dt = 1.0f / 60.0f;               // substep length: the frame's timestep / substeps
+    uint32_t substeps = 1, substep = 0;    // substeps per frame; substep of the current frame
     uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
     uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
     bool cooperative = false;

@@ -16,7 +16,8 @@ import os
 import numpy as np
 
 from . import layouts  # noqa: F401
-from .layouts import (AVBD_BODY, JOINT, JOINT_OPTIONS, MANIFOLD, PARAMS, RIGID_INPUT, SCHEDULE, STATE)
+from .layouts import (AVBD_BODY, CONTACT_EVENT, JOINT, JOINT_OPTIONS, MANIFOLD, PARAMS, RIGID_INPUT, SCHEDULE,
+                      STATE)
 
 HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(HERE, "lib", "libpm_hip_rigid.so")
@@ -45,7 +46,7 @@ class WorldDesc(ctypes.Structure):
                 ("base_iterations", _U), ("requested_iterations", _U),
                 ("body_capacity", _U), ("pairs_per_body", _U), ("points_per_body", _U),
                 ("device", _U), ("flags", _U), ("solve_lanes", _U),
-                ("kernel_timing_stride", _U), ("residual_tolerance", _F)]
+                ("kernel_timing_stride", _U), ("residual_tolerance", _F), ("substeps", _U)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact", "budget",
@@ -102,6 +103,9 @@ def lib():
         "pmhWorldAddJoints": (i32, [_P, _U, _P]),
         "pmhWorldJointCount": (_U, [_P]),
         "pmhWorldGetJoints": (i32, [_P, _P]),
+        "pmhWorldWriteJoints": (i32, [_P, _U, _U, _P, _P]),
+        "pmhWorldSetRigidStates": (i32, [_P, _U, _U, _P]),
+        "pmhWorldGetContactEvents": (i32, [_P, _P, _U, _P]),
         "pmhWorldCopyStates": (i32, [_P, _P]),
         "pmhWorldSetStates": (i32, [_P, _P]),
         "pmhWorldGetStats": (i32, [_P, _P]),
@@ -164,11 +168,11 @@ class World:
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
                  pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
-                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0):
+                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0, substeps=1):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
                          points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0,
-                         solve_lanes, kernel_timing_stride, residual_tolerance)
+                         solve_lanes, kernel_timing_stride, residual_tolerance, substeps)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
         self._h = h
@@ -286,6 +290,28 @@ class World:
         out = np.zeros(self.joint_count, dtype=JOINT)
         if len(out):
             _check(lib().pmhWorldGetJoints(self._h, _ptr(out)), "pmhWorldGetJoints")
+        return out
+
+    def write_joints(self, first, options, active):
+        """Removes joint slots first.. and re-adds those with active != 0
+        from `options` (pmCmdWriteRigidConstraints)."""
+        o = np.ascontiguousarray(options, dtype=JOINT_OPTIONS)
+        a = np.ascontiguousarray(active, dtype=np.uint32)
+        _check(lib().pmhWorldWriteJoints(self._h, first, len(o), _ptr(o), _ptr(a)), "pmhWorldWriteJoints")
+
+    def set_rigid_states(self, first, states):
+        """Replaces the states of bodies first.. (pmCmdSetRigidStates)."""
+        a = np.ascontiguousarray(states, dtype=STATE)
+        _check(lib().pmhWorldSetRigidStates(self._h, first, len(a), _ptr(a)), "pmhWorldSetRigidStates")
+
+    def contact_events(self):
+        """The last step's rigid contact events (layouts.CONTACT_EVENT)."""
+        count = ctypes.c_uint32()
+        _check(lib().pmhWorldGetContactEvents(self._h, None, 0, ctypes.byref(count)), "pmhWorldGetContactEvents")
+        out = np.zeros(count.value, dtype=CONTACT_EVENT)
+        if len(out):
+            _check(lib().pmhWorldGetContactEvents(self._h, _ptr(out), len(out), ctypes.byref(count)),
+                   "pmhWorldGetContactEvents")
         return out
 
     def step(self, steps=1):

@@ -24,6 +24,11 @@ RIGID_INPUT = np.dtype([                      # PmRigidInput, 52 bytes
     ("body_index", "<u4"), ("force", "<f4", 3), ("torque", "<f4", 3),
     ("linear_impulse", "<f4", 3), ("angular_impulse", "<f4", 3)])
 
+CONTACT_EVENT = np.dtype([                    # PmContactEvent, 56 bytes
+    ("type", "<u4"), ("first", "<u4"), ("second", "<u4"), ("position", "<f4", 3),
+    ("normal", "<f4", 3), ("penetration", "<f4"), ("normal_impulse", "<f4"),
+    ("friction_impulse", "<f4", 3)])
+
 PARAMS = np.dtype({
     "names": ["motion", "mesh_index", "mesh_generation", "inverse_mass",
               "inverse_inertia_local", "friction", "restitution",

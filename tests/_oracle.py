@@ -28,7 +28,7 @@ class WorldDesc(ctypes.Structure):
     _fields_ = [("gravity", _F * 3), ("dt", _F), ("profile_version", _U),
                 ("base_iterations", _U), ("requested_iterations", _U),
                 ("body_capacity", _U), ("lanes", _U), ("flags", _U),
-                ("residual_tolerance", _F)]
+                ("residual_tolerance", _F), ("substeps", _U)]
 
 
 STATS_FIELDS = ["pairs", "manifolds", "points", "colors", "islands", "impact",
@@ -62,6 +62,10 @@ def lib():
         _lib.pmo_world_apply_inputs.argtypes = [_P, _U, _P]
         _lib.pmo_world_add_joints.argtypes = [_P, _U, _P]
         _lib.pmo_world_residuals.argtypes = [_P, _P, _P, _P]
+        _lib.pmo_world_write_joints.argtypes = [_P, _U, _U, _P, _P]
+        _lib.pmo_world_set_rigid_states.argtypes = [_P, _U, _U, _P]
+        _lib.pmo_world_contact_events.argtypes = [_P, _P, _U]
+        _lib.pmo_world_contact_events.restype = _U
         _lib.pmo_mlp_forward.argtypes = [_P, _P, _U, _P, ctypes.c_int]
         _lib.pmo_mlp_forward.restype = None
         _lib.pmo_mlp_gradients.argtypes = [_P, _P, _U, _P, ctypes.c_int]
@@ -122,10 +126,10 @@ class OracleWorld:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0,
-                 body_capacity=0, lanes=0, brute_force=False, residual_tolerance=0.0):
+                 body_capacity=0, lanes=0, brute_force=False, residual_tolerance=0.0, substeps=1):
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, lanes,
-                         1 if brute_force else 0, residual_tolerance)
+                         1 if brute_force else 0, residual_tolerance, substeps)
         self._h = lib().pmo_world_create(ctypes.byref(desc))
         if not self._h:
             raise ValueError("invalid world description")
@@ -182,6 +186,24 @@ class OracleWorld:
         a = np.ascontiguousarray(inputs, dtype=L.RIGID_INPUT)
         if lib().pmo_world_apply_inputs(self._h, len(a), _ptr(a)) != 0:
             raise ValueError("invalid rigid inputs")
+
+    def write_joints(self, first, options, active):
+        o = np.ascontiguousarray(options, dtype=L.JOINT_OPTIONS)
+        a = np.ascontiguousarray(active, dtype=np.uint32)
+        if lib().pmo_world_write_joints(self._h, first, len(o), _ptr(o), _ptr(a)) != 0:
+            raise ValueError("invalid joint write")
+
+    def set_rigid_states(self, first, states):
+        a = np.ascontiguousarray(states, dtype=L.STATE)
+        if lib().pmo_world_set_rigid_states(self._h, first, len(a), _ptr(a)) != 0:
+            raise ValueError("invalid rigid states")
+
+    def contact_events(self):
+        total = lib().pmo_world_contact_events(self._h, None, 0)
+        out = np.zeros(total, dtype=L.CONTACT_EVENT)
+        if total:
+            lib().pmo_world_contact_events(self._h, _ptr(out), total)
+        return out
 
     def set_kinematic_targets(self, first, targets):
         a = np.ascontiguousarray(targets, dtype=L.STATE)

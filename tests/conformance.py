@@ -1,0 +1,280 @@
+# SPDX-License-Identifier: MIT
+"""The reference's analytic rigid conformance cases, run on a world with the
+oracle's or the library's surface and compared with the reference's CUDA
+captures (fixtures under tests/golden/conformance, written by
+tests/golden/make_conformance_fixtures.py).
+
+What is restated from the reference:
+- the scenes: direct_scene and weld_scene of
+  conformance/v1/src/command_runner.cpp:243-309 (box meshes :230-242), with
+  the gallery's RigidBodyOptions / RigidConstraintOptions defaults
+  (include/parallel_mater_scene/authoring.hpp:127-141, 187-208) mapped as
+  make_parameters does (src/simulation/runtime.cpp:459-482: a zero inertia
+  diagonal becomes the mesh's unit inertia times the mass);
+- the commands (command_runner.cpp:1121-1220) and the run loop (:2484-2518):
+  checkpoint 0 before any step, each frame's commands, one step, a
+  checkpoint after the frames the case lists;
+- the records (rigid_resource / constraint_resource, :1316-1348; contacts,
+  :2162-2190) and the comparator's rule (conformance/v1/compare.py:248-452):
+  |expected - actual| <= abs + rel * max(|expected|, |actual|) per number,
+  quaternions by the angle 2 acos|<q_e, q_a>| <= abs + rel, booleans and
+  counts exactly, contacts matched one-to-one within each (body, collider)
+  identity by records that pass every field.
+
+The captures are older than the reference's current step: the reference's
+own record of the rigid-direct capture (docs/roadmap/evidence/
+F5-04-rigid-direct-divergence.json) gives 4 substeps per 1/60 s frame, and
+the rigid solver's iteration count is the pipeline's 10
+(examples/gallery/scene_builder_create.cpp.inc:531-532), i.e. the legacy
+profile (profile_version 0, base_iterations 10).
+"""
+import json
+import math
+import pathlib
+
+import numpy as np
+
+import scenarios as S
+from pmamd import layouts as L
+
+FIXTURES = pathlib.Path(__file__).resolve().parent / "golden" / "conformance"
+CASES = ("rigid-direct", "constraint-breaking", "compound-weld-lifecycle")
+SUBSTEPS = 4
+ITERATIONS = 10
+
+
+def load(name):
+    return json.loads((FIXTURES / f"{name}.json").read_text())
+
+
+def _options(count, motion=L.MOTION_DYNAMIC, shape=0, mass=1.0, inertia=None, damping=0.05, unit=None):
+    """RigidBodyOptions defaults: friction 0.5, restitution 0, damping 0.05,
+    speed limits 100, collision margin 5 mm, mass 1, inertia from the mesh."""
+    if inertia is None:
+        inertia = tuple(float(np.float32(u) * np.float32(mass)) for u in unit)
+    return S.params(count, motion=motion, mass=mass, inertia=inertia, friction=0.5, restitution=0.0,
+                    margin=0.005, shape=shape, max_linear=100.0, max_angular=100.0, damping=damping)
+
+
+def _unit_inertia(world, shape):
+    return [float(x) for x in world.shape(shape)[10:13]]
+
+
+def build(name, world):
+    """Adds the case's meshes, bodies and joints to `world`; returns the body
+    names, constraint names and the joint options (for add_constraints)."""
+    if name == "rigid-direct":                                  # direct_scene, :243-275
+        for half in ((0.25, 0.25, 0.25), (3.0, 0.05, 3.0), (0.2, 0.2, 0.2)):
+            world.add_mesh(*S.box_mesh(*half))
+        p = np.concatenate([
+            _options(1, shape=0, mass=2.0, inertia=(1.0 / 12.0,) * 3, damping=0.0),
+            _options(1, motion=L.MOTION_STATIC, shape=1, unit=_unit_inertia(world, 1)),
+            _options(1, motion=L.MOTION_KINEMATIC, shape=2, unit=_unit_inertia(world, 2))])
+        st = S.states(3, np.array([[0, 2, 0], [0, -0.05, 0], [-1, 1, 0]], dtype=np.float32))
+        world.add_bodies(p, st)
+        return ["dynamic-box", "ground", "kinematic-box"], [], None
+    breaking = name == "constraint-breaking"                   # weld_scene, :277-309
+    world.add_mesh(*S.box_mesh(0.2, 0.2, 0.2))
+    n = 2 if breaking else 3
+    p = _options(n, shape=0, damping=0.0, unit=_unit_inertia(world, 0))
+    st = S.states(n, np.array([[-0.4 + 0.4 * k, 1.0, 0.0] for k in range(n)], dtype=np.float32))
+    world.add_bodies(p, st)
+    joints = np.concatenate([
+        L.joint_options(1, L.JOINT_FIXED, k, k + 1, anchor_a=(0.2, 0, 0), anchor_b=(-0.2, 0, 0),
+                        disable_collisions=1, breaking=0.05 if breaking else 0.0, iterations=8)
+        for k in range(n - 1)])
+    world.add_joints(joints)
+    bodies = ["break-a", "break-b"] if breaking else [f"member-{k}" for k in range(n)]
+    constraints = ["break-link"] if breaking else [f"weld-link-{k}" for k in range(n - 1)]
+    return bodies, constraints, joints
+
+
+def run(name, world):
+    """Runs the case on `world` (built with dt 1/60, SUBSTEPS substeps and
+    the legacy profile); returns {frame: record}."""
+    fixture = load(name)
+    case = fixture["case"]
+    bodies, constraints, joints = build(name, world)
+    initial = world.states()
+    present = [True] * len(constraints)
+    frames = case["world"]["frames"]
+    wanted = set(case["checkpoints"])
+    out = {}
+
+    def record(frame):
+        s = world.states()
+        rec = {"rigid": {}, "constraints": {}, "contacts": []}
+        for k, body in enumerate(bodies):
+            rec["rigid"][body] = {"position": s["position"][k].astype(float).tolist(),
+                                  "orientation": s["orientation"][k].astype(float).tolist(),
+                                  "linear_velocity": s["linear_velocity"][k].astype(float).tolist(),
+                                  "angular_velocity": s["angular_velocity"][k].astype(float).tolist()}
+        if constraints:
+            js = world.joints()
+            for k, c in enumerate(constraints):
+                on = present[k]
+                rec["constraints"][c] = {"present": on,
+                                         "applied_impulse": float(js["applied_impulse"][k]) if on else 0.0,
+                                         "broken": bool(on and js["broken"][k]),
+                                         "enabled": bool(on and js["enabled"][k])}
+        if frame > 0:
+            for e in world.contact_events():
+                rec["contacts"].append({
+                    "body": bodies[e["first"]], "collider": bodies[e["second"]],
+                    "friction_impulse": e["friction_impulse"].astype(float).tolist(),
+                    "normal": e["normal"].astype(float).tolist(),
+                    "normal_impulse": float(e["normal_impulse"]),
+                    "penetration": float(e["penetration"]),
+                    "position": e["position"].astype(float).tolist()})
+        out[frame] = rec
+
+    if 0 in wanted:
+        record(0)
+    for frame in range(frames):
+        for command in case["commands"]:
+            if command["frame"] != frame:
+                continue
+            op, value = command["operation"], command.get("value")
+            if op == "apply_impulse":
+                inp = np.zeros(1, dtype=L.RIGID_INPUT)
+                inp["body_index"] = bodies.index(command["target"])
+                inp["linear_impulse"] = value
+                world.apply_inputs(inp)
+            elif op == "set_kinematic_target":
+                k = bodies.index(command["target"])
+                target = initial[k:k + 1].copy()
+                target["position"] = value
+                world.set_kinematic_targets(k, target)
+            elif op == "replace_body":
+                k = bodies.index(command["target"])
+                state = initial[k:k + 1].copy()
+                state["position"] = value
+                world.set_rigid_states(k, state)
+            elif op in ("remove_constraints", "add_constraints"):
+                active = op == "add_constraints"
+                world.write_joints(0, joints, np.full(len(joints), 1 if active else 0, dtype=np.uint32))
+                present = [active] * len(constraints)
+            else:
+                raise ValueError(f"unsupported command {op}")
+        world.step(1)
+        if frame + 1 in wanted:
+            record(frame + 1)
+    return out
+
+
+def _limit(case, tolerance_class, leaf):
+    values = case["tolerances"]["values"]
+    if leaf == "orientation":
+        q = values["quaternion_angular"]
+        return q["abs"] + q["rel"], None
+    profile = values.get(tolerance_class) or values[case["tolerances"]["profile"]]
+    quantity = "velocity" if ("velocity" in leaf or "speed" in leaf or "momentum" in leaf) else "position"
+    return profile[quantity]["abs"], profile[quantity]["rel"]
+
+
+def _close(expected, actual, absolute, relative):
+    return abs(expected - actual) <= absolute + relative * max(abs(expected), abs(actual))
+
+
+def quaternion_angle(expected, actual):
+    e, a = np.asarray(expected, dtype=float), np.asarray(actual, dtype=float)
+    ne, na = np.linalg.norm(e), np.linalg.norm(a)
+    if ne == 0.0 or na == 0.0:
+        return math.inf
+    return 2.0 * math.acos(min(1.0, max(0.0, abs(float(e @ a)) / (ne * na))))
+
+
+def _contact_passes(case, expected, actual):
+    if expected["body"] != actual["body"] or expected["collider"] != actual["collider"]:
+        return False
+    for key in ("friction_impulse", "normal", "normal_impulse", "penetration", "position"):
+        absolute, relative = _limit(case, case["tolerances"]["profile"], key)
+        e, a = np.atleast_1d(expected[key]), np.atleast_1d(actual[key])
+        if not all(_close(float(x), float(y), absolute, relative) for x, y in zip(e, a)):
+            return False
+    return True
+
+
+def _match_contacts(case, expected, actual):
+    """One-to-one matching by augmenting paths over the pairs whose every
+    field passes; returns the expected indices left unmatched."""
+    edges = [[j for j, a in enumerate(actual) if _contact_passes(case, e, a)] for e in expected]
+    owner = {}
+
+    def augment(i, seen):
+        for j in edges[i]:
+            if j in seen:
+                continue
+            seen.add(j)
+            if j not in owner or augment(owner[j], seen):
+                owner[j] = i
+                return True
+        return False
+
+    return [i for i in range(len(expected)) if not augment(i, set())]
+
+
+def compare(name, result, frames=None):
+    """Failures (strings) of `result` against the capture at the listed
+    checkpoint frames (all by default)."""
+    fixture = load(name)
+    case = fixture["case"]
+    failures = []
+    for cp in fixture["checkpoints"]:
+        frame = cp["frame"]
+        if frames is not None and frame not in frames:
+            continue
+        got = result[frame]
+        for r in cp["resources"]:
+            rid = r["id"]
+            if r["type"] == "rigid_body":
+                actual = got["rigid"][rid]
+                for leaf in ("position", "linear_velocity", "angular_velocity"):
+                    absolute, relative = _limit(case, r["tolerance_class"], leaf)
+                    for axis, (e, a) in enumerate(zip(r[leaf], actual[leaf])):
+                        if not _close(e, a, absolute, relative):
+                            failures.append(f"frame {frame} {rid}.{leaf}[{axis}]: expected {e:.7g}, got {a:.7g}")
+                limit, _ = _limit(case, r["tolerance_class"], "orientation")
+                angle = quaternion_angle(r["orientation"], actual["orientation"])
+                if angle > limit:
+                    failures.append(f"frame {frame} {rid}.orientation: angle {angle:.3g} > {limit}")
+            else:
+                actual = got["constraints"][rid]
+                for key in ("present", "broken", "enabled"):
+                    if bool(r[key]) != actual[key]:
+                        failures.append(f"frame {frame} {rid}.{key}: expected {r[key]}, got {actual[key]}")
+                absolute, relative = _limit(case, r["tolerance_class"], "applied_impulse")
+                if not _close(r["applied_impulse"], actual["applied_impulse"], absolute, relative):
+                    failures.append(f"frame {frame} {rid}.applied_impulse: expected {r['applied_impulse']:.7g}, "
+                                    f"got {actual['applied_impulse']:.7g}")
+        if cp["rigid_contact_count"] != len(got["contacts"]):
+            failures.append(f"frame {frame} rigid_contact_count: expected {cp['rigid_contact_count']}, "
+                            f"got {len(got['contacts'])}")
+        else:
+            for i in _match_contacts(case, cp["rigid_contacts"], got["contacts"]):
+                failures.append(f"frame {frame} contact {cp['rigid_contacts'][i]} has no match")
+    return failures
+
+
+# rigid-direct's frame 40 follows the kinematic box's 120 m/s sweep through
+# the dynamic box (frame 21). The reference's own later captures of this case
+# (docs/roadmap/evidence/F5-04-rigid-direct-3acb413-cuda.json and
+# -603fee-cuda.json) differ from the CUDA capture there by up to 22% in the
+# launched box's spin (omega_z 12.89 and -0.009 against 16.47), so this
+# frame is held to a stated band instead of the 1e-5 direct class.
+IMPACT_FRAME = 40
+EXACT_FRAMES = {"rigid-direct": {0, 1, 20, 60}}
+
+
+def impact_band(result, relative=0.01, absolute=1e-3, angle=0.02):
+    cp = {c["frame"]: c for c in load("rigid-direct")["checkpoints"]}[IMPACT_FRAME]
+    failures = []
+    for r in cp["resources"]:
+        actual = result[IMPACT_FRAME]["rigid"][r["id"]]
+        for leaf in ("position", "linear_velocity", "angular_velocity"):
+            for axis, (e, a) in enumerate(zip(r[leaf], actual[leaf])):
+                if not _close(e, a, absolute, relative):
+                    failures.append(f"{r['id']}.{leaf}[{axis}]: expected {e:.7g}, got {a:.7g}")
+        if quaternion_angle(r["orientation"], actual["orientation"]) > angle:
+            failures.append(f"{r['id']}.orientation beyond {angle} rad")
+    return failures

@@ -1,0 +1,42 @@
+# SPDX-License-Identifier: MIT
+"""The oracle against the reference's CUDA conformance captures of its three
+analytic rigid cases (conformance/v1/golden/cuda/{rigid-direct,
+constraint-breaking,compound-weld-lifecycle}.json, kept as fixtures under
+tests/golden/conformance).
+
+The captures pin, within the cases' own tolerance classes: free flight under
+4 substeps (direct class, 1e-5), an applied impulse, a kinematic target
+reached over a frame's substeps, a body replaced mid-run
+(set_rigid_body_state), a fixed joint breaking in its first substep (applied
+impulse 0.896216, broken and disabled), a three-body weld compound moving as
+one, released (its members then touch: 8 contact events with their
+positions, normals and impulses) and rebuilt (fresh joint state).
+"""
+import pytest
+
+import _oracle as O
+import conformance as C
+
+
+def oracle_world(**kw):
+    args = dict(dt=1.0 / 60.0, profile=0, base_iterations=C.ITERATIONS, substeps=C.SUBSTEPS)
+    args.update(kw)
+    return O.OracleWorld(**args)
+
+
+@pytest.mark.parametrize("name", C.CASES)
+def test_oracle_matches_the_cuda_capture(name):
+    result = C.run(name, oracle_world())
+    assert C.compare(name, result, C.EXACT_FRAMES.get(name)) == []
+    if name == "rigid-direct":
+        assert C.impact_band(result) == []
+
+
+@pytest.mark.parametrize("name, change", [("rigid-direct", dict(substeps=1)),
+                                          ("constraint-breaking", dict(profile=1)),
+                                          ("compound-weld-lifecycle", dict(substeps=2))])
+def test_a_different_step_contract_fails_the_capture(name, change):
+    """One or two substeps per frame, or the realtime profile, each move the
+    oracle off the capture: the comparison discriminates."""
+    result = C.run(name, oracle_world(**change))
+    assert C.compare(name, result, C.EXACT_FRAMES.get(name)) != []
