@@ -227,7 +227,7 @@ def main():
                                     kernel_timing_stride=KERNEL_TIMING_STRIDE, **pools))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     tuned = None
-    if args.workload == "batch" and not args.lanes:
+    if args.workload in ("batch", "chains") and not args.lanes:
         # Lane tile of the colour sweep picked by timing each on this world
         # (part of the warm-up).
         world.step(max(1, args.warmup // 2))

@@ -2137,31 +2137,38 @@ void solve_body(Ctx &x, uint32_t i, uint32_t phase, std::vector<PointRef> &pts) 
     for (uint32_t r = 0; r < 6; ++r)
         for (uint32_t c = 0; c < 6; ++c) base.g[r] += base.h[bidx(r, c)] * x.B[i].displacement[c];
     body_points(x, i, pts);
+    // Joint rows (solve.slang:401-430) follow the contact rows: recomputed
+    // in the pose phase, stored hard rows in the velocity phase. In lane
+    // order the k-th enabled joint of the body's list goes to partial k mod L.
+    auto joint_rows = [&](auto &&target) {
+        uint32_t pos = 0;
+        for_each_joint(x, i, [&](uint32_t k, bool first) {
+            PmoJoint &j = x.J[k];
+            if (j.state_enabled == 0u) return;
+            Block &b = target(pos++);
+            if (phase == 0u) {
+                visit_joint(x, j, b, first, kAccumulate);
+                return;
+            }
+            const Vec6 da = disp(x.B[j.dense_a]), db = disp(x.B[j.dense_b]);
+            for (uint32_t r = 0; r < j.avbd_count; ++r) {
+                Row row = row_from(reinterpret_cast<const float *>(&j.rows[r]));
+                if (row.stiffness >= kMaxPenalty) row_accumulate(b, row, first, da, db, x.dt);
+            }
+        });
+    };
     Block blk;
     if (x.lanes == 0u) {
         blk = base;
         for (auto &r : pts) contact_point_rows(x, i, r, phase, blk);
+        joint_rows([&](uint32_t) -> Block & { return blk; });
     } else {
         std::vector<Block> parts(x.lanes, zero_block());
         parts[0] = base;
         for (size_t k = 0; k < pts.size(); ++k) contact_point_rows(x, i, pts[k], phase, parts[k % x.lanes]);
+        joint_rows([&](uint32_t pos) -> Block & { return parts[pos % x.lanes]; });
         blk = tree_reduce(parts);
     }
-    // Joint rows (solve.slang:401-430): recomputed in the pose phase,
-    // stored hard rows in the velocity phase.
-    for_each_joint(x, i, [&](uint32_t k, bool first) {
-        PmoJoint &j = x.J[k];
-        if (j.state_enabled == 0u) return;
-        if (phase == 0u) {
-            visit_joint(x, j, blk, first, kAccumulate);
-            return;
-        }
-        const Vec6 da = disp(x.B[j.dense_a]), db = disp(x.B[j.dense_b]);
-        for (uint32_t r = 0; r < j.avbd_count; ++r) {
-            Row row = row_from(reinterpret_cast<const float *>(&j.rows[r]));
-            if (row.stiffness >= kMaxPenalty) row_accumulate(blk, row, first, da, db, x.dt);
-        }
-    });
     Vec6 update = zero6();
     if (!block_solve(blk, update)) return;
     bool crossing = false;

@@ -763,7 +763,8 @@ __device__ __forceinline__ float row_error(float e0, float4 axis, float4 angA, f
     return e0 + dot6(ja, dA) + dot6(jb, dB);
 }
 
-__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk);
+__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk, uint32_t sub_lane,
+                                uint32_t L);
 
 template <int L>
 __device__ __forceinline__ void group_reduce(Block &b) {
@@ -867,10 +868,10 @@ __device__ void solve_color(const SolveArgs &a, uint32_t begin, uint32_t end, ui
                 add_row(blk, row, stiff[ax], force[ax]);
             }
         }
+        // Joint rows follow the contact rows (solve.slang:401-430), each
+        // joint in one lane's partial block.
+        if (kJoints && active) body_joint_rows(a, i, phase, blk, sub_lane, L);
         group_reduce<L>(blk);
-        // Joint rows follow the contact rows (solve.slang:401-430); every
-        // lane of the group adds the same rows to its copy of the block.
-        if (kJoints && active) body_joint_rows(a, i, phase, blk);
         float update[6];
         const bool solved = active && block_solve(blk, update);
         // Crossing safeguard (solve.slang:433-475).
@@ -1075,7 +1076,7 @@ __device__ void visit_joint_row(const SolveArgs &a, PmhRigidConstraint &j, Block
 }
 
 // rigid_avbd_visit_joint, :715-926.
-__device__ __noinline__ float visit_joint(const SolveArgs &a, PmhRigidConstraint &j, Block &blk, bool first,
+__device__ __forceinline__ float visit_joint(const SolveArgs &a, PmhRigidConstraint &j, Block &blk, bool first,
                                           uint32_t visit) {
     uint32_t index = 0;
     float impulse = 0.0f;
@@ -1222,10 +1223,16 @@ __device__ __noinline__ float visit_joint(const SolveArgs &a, PmhRigidConstraint
 // The joint rows of body i in its list order, added to its block: the
 // pose phase recomputes the rows, the velocity phase adds the stored hard
 // rows (solve.slang:401-430).
-__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk) {
+// The body's enabled joints are dealt round-robin over the L lanes of its
+// group (the k-th enabled joint of the list to lane k mod L) and added to
+// that lane's partial block before the group reduction.
+__device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, Block &blk, uint32_t sub_lane,
+                                uint32_t L) {
+    uint32_t pos = 0;
     for (uint32_t e = a.jstart[i]; e < a.jstart[i + 1]; ++e) {
         PmhRigidConstraint &j = a.joints[a.jadj[e]];
         if (j.state_enabled == 0u) continue;
+        if (pos++ % L != sub_lane) continue;
         const bool first = i == j.dense_a;
         if (phase == 0u) {
             visit_joint(a, j, blk, first, kVisitAccumulate);
@@ -1348,9 +1355,10 @@ __device__ __forceinline__ void residual_commit(ResidualLog *log, uint32_t slot,
         sum += (unsigned long long)__shfl_xor((long long)sum, d, 64);
     }
     if ((threadIdx.x & 63) == 0 && count) {
-        atomicMax(&log->bits[slot], bits);
-        atomicAdd(&log->sum[slot], sum);
-        atomicAdd(&log->count[slot], count);
+        const uint32_t bank = blockIdx.x & (kResidualBanks - 1);
+        atomicMax(&log->bits[slot][bank], bits);
+        atomicAdd(&log->sum[slot][bank], sum);
+        atomicAdd(&log->count[slot][bank], count);
     }
 }
 
@@ -1672,10 +1680,20 @@ __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase
 
 // To-residual mode: the phase stops at the first update whose residual is
 // at most the tolerance, or at the budget.
+// One wavefront: lane b folds bank b of the slot.
 __global__ void k_stage_check(SolveArgs a, uint32_t phase, uint32_t it) {
-    if (threadIdx.x != 0 || a.log->executed[phase] != 0u) return;
+    if (a.log->executed[phase] != 0u) return;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t slot = phase * kResidualSlots + it;
-    const float r = it < kResidualSlots ? residual_rms(a.log->sum[slot], a.log->count[slot]) : 0.0f;
+    unsigned long long sum = it < kResidualSlots ? a.log->sum[slot][lane] : 0ull;
+    uint32_t count = it < kResidualSlots ? a.log->count[slot][lane] : 0u;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        sum += (unsigned long long)__shfl_xor((long long)sum, d, 64);
+        count += uint32_t(__shfl_xor(count, d, 64));
+    }
+    if (lane != 0) return;
+    const float r = it < kResidualSlots ? residual_rms(sum, count) : 0.0f;
     if (r <= a.tol || it + 1u == a.budget) a.log->executed[phase] = it + 1u;
 }
 
@@ -1953,12 +1971,14 @@ __global__ void __launch_bounds__(64) k_project_hinges(SolveArgs a) {
 }
 
 __global__ void k_residual_reset(ResidualLog *log) {
-    for (uint32_t k = threadIdx.x; k < 2 * kResidualSlots; k += blockDim.x) {
-        log->bits[k] = 0u;
-        log->sum[k] = 0ull;
-        log->count[k] = 0u;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * kResidualSlots * kResidualBanks;
+         k += gridDim.x * blockDim.x) {
+        const uint32_t slot = k / kResidualBanks, bank = k % kResidualBanks;
+        log->bits[slot][bank] = 0u;
+        log->sum[slot][bank] = 0ull;
+        log->count[slot][bank] = 0u;
     }
-    if (threadIdx.x == 0) log->executed[0] = log->executed[1] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) log->executed[0] = log->executed[1] = 0u;
 }
 
 __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32_t budget, uint32_t profile,
@@ -2144,7 +2164,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.c_flags = v.c_flags.p;
     const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj));
     k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
-    k_residual_reset<<<1, 128, 0, s>>>(v.residual.p);
+    k_residual_reset<<<32, 256, 0, s>>>(v.residual.p);
     k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
                                    v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
                                    v.impact.p, v.deg.p, v.anchored.p, v.adj_cursor.p, v.members.p, v.mean.p,

@@ -140,13 +140,33 @@ struct DBuf {
 // and the number of contributing points and joints; and the iterations
 // each phase ran.
 constexpr uint32_t kResidualSlots = 64;
+// Per update slot, kResidualBanks partial sums: a wavefront commits to the
+// bank of its block (blockIdx & 63), so the atomics of one update spread
+// over 64 addresses per field instead of meeting at one; readers fold the
+// banks (integer sums and maxima, so the fold is exact in any order).
+constexpr uint32_t kResidualBanks = 64;
 struct ResidualLog {
-    unsigned long long sum[2 * kResidualSlots];
-    uint32_t bits[2 * kResidualSlots];
-    uint32_t count[2 * kResidualSlots];
+    unsigned long long sum[2 * kResidualSlots][kResidualBanks];
+    uint32_t bits[2 * kResidualSlots][kResidualBanks];
+    uint32_t count[2 * kResidualSlots][kResidualBanks];
     uint32_t executed[2];
     uint32_t pad[2];
 };
+
+// Folded totals of one slot.
+struct ResidualTotal {
+    unsigned long long sum = 0;
+    uint32_t bits = 0, count = 0;
+};
+inline ResidualTotal residual_fold(const ResidualLog &log, uint32_t slot) {
+    ResidualTotal t;
+    for (uint32_t b = 0; b < kResidualBanks; ++b) {
+        t.sum += log.sum[slot][b];
+        t.count += log.count[slot][b];
+        t.bits = log.bits[slot][b] > t.bits ? log.bits[slot][b] : t.bits;
+    }
+    return t;
+}
 
 // Square of a residual in 2^-40 fixed point, clamped at 2^44 (r = 4 m).
 __host__ __device__ inline unsigned long long residual_square(float r) {

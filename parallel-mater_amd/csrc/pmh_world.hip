@@ -11,6 +11,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <numeric>
 #include <unordered_map>
@@ -555,7 +556,8 @@ PmhResult World::step_with_fallback(uint32_t k) {
     const int parity = cache_parity;
     const uint32_t sub = substep;
     if ((r = step_once(k, true)) != PMH_SUCCESS) return r;
-    ResidualLog log;
+    std::unique_ptr<ResidualLog> held(new ResidualLog);
+    ResidualLog &log = *held;
     PMH_CHECK(hipMemcpyAsync(&log, solver.residual.p, sizeof log, hipMemcpyDeviceToHost, stream));
     PMH_CHECK(hipStreamSynchronize(stream));
     bool failed = false;
@@ -563,7 +565,8 @@ PmhResult World::step_with_fallback(uint32_t k) {
         const uint32_t ex = log.executed[ph];
         if (ex == 0 || ex > kResidualSlots) continue;
         const uint32_t slot = ph * kResidualSlots + ex - 1;
-        failed = failed || residual_rms(log.sum[slot], log.count[slot]) > residual_tolerance;
+        const ResidualTotal t = residual_fold(log, slot);
+        failed = failed || residual_rms(t.sum, t.count) > residual_tolerance;
     }
     ++warm_steps;
     if (!failed) return PMH_SUCCESS;
@@ -1203,7 +1206,8 @@ PmhResult pmhWorldGetResiduals(PmhWorld world, float *out, float *maximum, uint3
         executed[0] = executed[1] = 0u;
         return PMH_SUCCESS;
     }
-    pmh::ResidualLog log;
+    std::unique_ptr<pmh::ResidualLog> held(new pmh::ResidualLog);
+    pmh::ResidualLog &log = *held;
     PMH_CHECK(hipStreamSynchronize(w->stream));
     PMH_CHECK(hipMemcpy(&log, w->solver.residual.p, sizeof log, hipMemcpyDeviceToHost));
     pmh::StepCounters c;
@@ -1215,9 +1219,10 @@ PmhResult pmhWorldGetResiduals(PmhWorld world, float *out, float *maximum, uint3
     for (uint32_t ph = 0; ph < 2; ++ph)
         for (uint32_t it = 0; it < pmh::kResidualSlots; ++it) {
             const uint32_t k = ph * pmh::kResidualSlots + it;
+            const pmh::ResidualTotal t = pmh::residual_fold(log, k);
             float v;
-            std::memcpy(&v, &log.bits[k], sizeof v);
-            out[k] = it < executed[ph] ? pmh::residual_rms(log.sum[k], log.count[k]) : nan;
+            std::memcpy(&v, &t.bits, sizeof v);
+            out[k] = it < executed[ph] ? pmh::residual_rms(t.sum, t.count) : nan;
             if (maximum) maximum[k] = it < executed[ph] ? v : nan;
         }
     return PMH_SUCCESS;
