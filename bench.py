@@ -173,6 +173,8 @@ def main():
     ap.add_argument("--iterations", type=int, default=10)
     ap.add_argument("--profile", type=int, default=0, help="0 legacy (10 iterations), 1 realtime v1")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--warm-lambda", action="store_true",
+                    help="warmstart workload: also run the predicted-lambda mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body in the colour sweep (8, 16, 32, 64; 0 = library default)")

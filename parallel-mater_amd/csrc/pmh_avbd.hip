@@ -1922,10 +1922,8 @@ __global__ void k_joint_anchor(SolveArgs a, const uint32_t *parent, uint32_t *an
 
 // rigid_avbd_project_hinge_anchors (solve.slang:174-209): after the pose
 // phase, each enabled hinge moves its anchors together, split by inverse
-// mass, one joint after the other in index order.  One wavefront walks the
-// joints 64 at a time: when no two hinges of a window share a body their
-// projections commute and run together; otherwise the window's hinges run
-// one after the other in index order.
+// mass, one joint after the other in index order.  Here the hinges run by
+// levels of a schedule built at upload (below).
 __device__ void project_hinge(const SolveArgs &a, const PmhRigidConstraint &j) {
     const uint32_t ia = j.dense_a, ib = j.dense_b;
     const float ima = a.P[ia].motion == kDynamic ? a.P[ia].inverse_mass : 0.0f;
@@ -1940,33 +1938,30 @@ __device__ void project_hinge(const SolveArgs &a, const PmhRigidConstraint &j) {
     a.states[ib].position = pv(sub(sv(sb.position), mul(e, imb / sum)));
 }
 
-__global__ void __launch_bounds__(64) k_project_hinges(SolveArgs a) {
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t base = 0; base < a.nj; base += 64) {
-        const uint32_t k = base + lane;
-        bool active = false;
-        uint32_t ia = kInvalid, ib = kInvalid;
-        if (k < a.nj) {
-            const PmhRigidConstraint &j = a.joints[k];
-            active = j.avbd_count != 0u && j.state_enabled != 0u && j.options.type == kJointHinge;
-            if (active) { ia = j.dense_a; ib = j.dense_b; }
+constexpr uint32_t kHingeLevelLaunches = 64;   // deeper schedules take the one-block walk
+
+// One thread per hinge of one schedule level: the hinges of a level share
+// no body, so their projections commute; the levels run in order, which
+// gives every hinge the bodies as the serial pass in index order leaves them.
+__global__ void k_project_hinge_level(SolveArgs a, const uint32_t *list, uint32_t begin, uint32_t end) {
+    const uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= end) return;
+    const PmhRigidConstraint &j = a.joints[list[t]];
+    if (j.avbd_count == 0u || j.state_enabled == 0u) return;
+    project_hinge(a, j);
+}
+
+// Deep schedules: one block walks the levels, with an agent-scope fence
+// (writes done, stale cache lines dropped) and a block barrier between them.
+__global__ void __launch_bounds__(1024) k_project_hinge_levels(SolveArgs a, const uint32_t *list,
+                                                               const uint32_t *level_start, uint32_t levels) {
+    for (uint32_t l = 0; l < levels; ++l) {
+        for (uint32_t t = level_start[l] + threadIdx.x; t < level_start[l + 1]; t += blockDim.x) {
+            const PmhRigidConstraint &j = a.joints[list[t]];
+            if (j.avbd_count != 0u && j.state_enabled != 0u) project_hinge(a, j);
         }
-        // Does an earlier active hinge of the window share a body?
-        bool conflict = false;
-        for (uint32_t l = 0; l < 64; ++l) {
-            const uint32_t oa = __shfl(ia, l, 64), ob = __shfl(ib, l, 64);
-            if (l < lane && active && oa != kInvalid && (oa == ia || oa == ib || ob == ia || ob == ib))
-                conflict = true;
-        }
-        if (__ballot(conflict) == 0ull) {
-            if (active) project_hinge(a, a.joints[k]);
-        } else {
-            for (uint32_t l = 0; l < 64; ++l) {
-                if (lane == l && active) project_hinge(a, a.joints[k]);
-                __threadfence_block();
-            }
-        }
-        __threadfence_block();
+        __threadfence();
+        __syncthreads();
     }
 }
 
@@ -2352,8 +2347,18 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             k_stage_apply<<<bg, 256, 0, s>>>(a, phase);
             launches += 1;
             if (phase == 0 && v.hinges) {
-                k_project_hinges<<<1, 64, 0, s>>>(a);
-                launches += 1;
+                const std::vector<uint32_t> &ls = v.hinge_level_start;
+                const uint32_t levels = uint32_t(ls.size()) - 1u;
+                if (levels <= kHingeLevelLaunches) {
+                    for (uint32_t l = 0; l < levels; ++l) {
+                        const uint32_t count = ls[l + 1] - ls[l];
+                        if (count) k_project_hinge_level<<<(count + 255) / 256, 256, 0, s>>>(a, v.hinge_list, ls[l], ls[l + 1]);
+                    }
+                    launches += levels;
+                } else {
+                    k_project_hinge_levels<<<1, 1024, 0, s>>>(a, v.hinge_list, v.hinge_level_dev, levels);
+                    launches += 1;
+                }
             }
         }
         if (timing) timing->launches += launches;

@@ -202,6 +202,11 @@ struct Solver {
     // (one wavefront each).
     const uint32_t *jlist_start = nullptr, *jlist = nullptr, *jheavy = nullptr;
     uint32_t nheavy = 0;
+    // Hinge anchor projection schedule: alive hinges by (level, index); the
+    // hinges of one level share no body, and every hinge's level is above
+    // those of the earlier hinges on its bodies.
+    const uint32_t *hinge_list = nullptr, *hinge_level_dev = nullptr;
+    std::vector<uint32_t> hinge_level_start;   // host copy: levels + 1 offsets
     // Contact rows in colour order: plane q (q = 0..8) of entry e at
     // erec[q * ecap + e], the same nine float4 as the point's Jacobian record,
     // so the lanes of a body read consecutive slots.  c_entry holds each

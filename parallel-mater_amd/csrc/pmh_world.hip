@@ -512,9 +512,41 @@ PmhResult World::upload_joints() {
     if ((r = excl_list.reserve(list.size()))) return r;
     PMH_CHECK(hipMemcpy(excl_start.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
     PMH_CHECK(hipMemcpy(excl_list.p, list.data(), sizeof(uint2) * list.size(), hipMemcpyHostToDevice));
+    // Hinge projection levels, in joint index order: a hinge's level is one
+    // above the highest level already on either of its bodies.
+    std::vector<uint32_t> body_level(n, 0u), hinge_level;
+    std::vector<uint32_t> hinge_ids;
+    uint32_t levels = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        const PmhRigidConstraint &j = host_joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || j.options.type != kJointHinge || a == kInvalid || b == kInvalid) continue;
+        const uint32_t level = std::max(body_level[a], body_level[b]) + 1u;
+        body_level[a] = body_level[b] = level;
+        hinge_ids.push_back(k);
+        hinge_level.push_back(level);
+        levels = std::max(levels, level);
+    }
+    std::vector<uint32_t> level_start(levels + 2u, 0u), ordered(hinge_ids.size());
+    for (uint32_t level : hinge_level) level_start[level + 1u]++;
+    for (uint32_t l = 1; l <= levels + 1u; ++l) level_start[l] += level_start[l - 1];
+    {
+        std::vector<uint32_t> cur(level_start.begin(), level_start.end());
+        for (size_t h = 0; h < hinge_ids.size(); ++h) ordered[cur[hinge_level[h]]++] = hinge_ids[h];
+    }
+    if ((r = hinge_list.reserve(std::max<size_t>(1, ordered.size())))) return r;
+    if (!ordered.empty())
+        PMH_CHECK(hipMemcpy(hinge_list.p, ordered.data(), sizeof(uint32_t) * ordered.size(), hipMemcpyHostToDevice));
+    solver.hinge_list = hinge_list.p;
+    // level_start[l] .. level_start[l + 1] holds level l (levels start at 1).
+    solver.hinge_level_start.assign(level_start.begin() + 1, level_start.end());
+    if ((r = hinge_levels.reserve(solver.hinge_level_start.size()))) return r;
+    PMH_CHECK(hipMemcpy(hinge_levels.p, solver.hinge_level_start.data(),
+                        sizeof(uint32_t) * solver.hinge_level_start.size(), hipMemcpyHostToDevice));
+    solver.hinge_level_dev = hinge_levels.p;
     solver.joints = joints_dev.p;
     solver.nj = nj;
-    solver.hinges = hinges;
+    solver.hinges = hinges && !ordered.empty();
     joints_dirty = false;
     joints_on_device = nj != 0;
     return PMH_SUCCESS;
@@ -598,7 +630,7 @@ void World::release() {
     nodes_dev.release(); leaves_dev.release(); shell_dev.release();
     inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
     joints_dev.release(); weld.release(); excl_start.release(); excl_list.release();
-    jlist_start.release(); jlist.release(); jheavy.release();
+    jlist_start.release(); jlist.release(); jheavy.release(); hinge_list.release(); hinge_levels.release();
     scene_of.release(); scene_count.release(); scene_pair_start.release(); scene_first_dev.release();
     scene_large_count.release(); scene_large_start.release(); scene_large.release();
     mlp_dev.release(); log_dev.release(); snap_vel.release(); snap_inputs.release(); snap_flags.release();

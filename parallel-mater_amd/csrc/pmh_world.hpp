@@ -58,6 +58,7 @@ struct World {
     // collisions) and, per body, its disable_collisions partners with the
     // joint that excludes them.
     DBuf<uint32_t> jlist_start, jlist, jheavy;   // candidate joint lists (descending index)
+    DBuf<uint32_t> hinge_list, hinge_levels;     // hinge projection schedule and its level offsets
     DBuf<uint32_t> weld, excl_start;
     DBuf<uint2> excl_list;
     uint32_t weld_joints = 0, excl_count = 0;

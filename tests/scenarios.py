@@ -558,7 +558,7 @@ def capsule_inertia(mass, radius, half_length):
 
 
 def chains(count=4, links=64, spacing=0.45, height=None, seed=4, radius=0.05, length=0.3,
-           heavy_every=8, heavy_ratio=100.0, density=1000.0, max_tilt=0.35):
+           heavy_every=8, heavy_ratio=100.0, density=1000.0, max_tilt=0.35, hinges_only=False):
     """`count` chains (a square grid of anchors, `spacing` apart) of
     `links` capsule links (radius 0.05 m, total length 0.3 m) joined end to
     end by ball (point) and hinge joints in turn, the first link held by a
@@ -568,7 +568,8 @@ def chains(count=4, links=64, spacing=0.45, height=None, seed=4, radius=0.05, le
     swing into each other and, hanging down, reach a static ground box
     (the anchors sit at 0.99 of a chain's length).  Reference
     default body options.  Bodies: 0 ground, 1 anchor, then the links chain
-    by chain; joint j of chain c joins link j-1 (or the anchor) and link j."""
+    by chain; joint j of chain c joins link j-1 (or the anchor) and link j.
+    With `hinges_only` every joint is a hinge."""
     rng = np.random.default_rng(seed)
     side = int(np.ceil(np.sqrt(count)))
     if height is None:
@@ -597,7 +598,8 @@ def chains(count=4, links=64, spacing=0.45, height=None, seed=4, radius=0.05, le
     inertia = np.ones((n, 3), np.float32)
     light, heavy = capsule_inertia(base_mass, radius, h), capsule_inertia(base_mass * heavy_ratio, radius, h)
     inertia[2:] = np.tile(np.where(((k + 1) % heavy_every == 0)[:, None], heavy, light), (count, 1))
-    jo = L.joint_options(count * links, np.tile(np.where(k % 2 == 0, L.JOINT_POINT, L.JOINT_HINGE), count),
+    kinds = np.full(links, L.JOINT_HINGE) if hinges_only else np.where(k % 2 == 0, L.JOINT_POINT, L.JOINT_HINGE)
+    jo = L.joint_options(count * links, np.tile(kinds, count),
                          0, 0, anchor_a=(0, -half_link, 0), anchor_b=(0, half_link, 0))
     for c in range(count):
         gx, gz = c % side, c // side

@@ -439,6 +439,23 @@ def joint_gaps(scene, states):
     return np.linalg.norm(pa - pb, axis=1)
 
 
+def test_deep_hinge_schedule_follows_the_oracle():
+    """One 80-link chain of hinges only: each hinge shares a body with the
+    next, so the anchor projection's schedule has 80 levels and runs as the
+    one-block walk; 10 steps against the oracle (serial in joint order)."""
+    say("one 80-link hinge chain (80 projection levels)")
+    scene = S.chains(count=1, links=80, hinges_only=True)
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    g.step(1)
+    o.step(1)
+    assert_states_close(g.states(), o.states())
+    g.step(9)
+    o.step(9)
+    assert_states_close(g.states(), o.states(), rtol=1e-4, atol=1e-4)
+    assert_joints_match(g.joints(), o.joints(), rtol=1e-3, atol=1e-3)
+
+
 def test_chains_single_step_and_trajectory():
     """Four 64-link capsule chains (ball and hinge joints in turn, heavy
     links): one step to the single-step tolerance; 20 steps with equal pair
