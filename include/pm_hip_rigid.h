@@ -347,8 +347,6 @@ PmhResult pmhWorldApplyRigidInputs(PmhWorld world, uint32_t count, const PmhRigi
 PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t count,
                                       const PmhRigidBodyState *targets);
 
-/* Worlds of fewer than 4 bodies return PMH_ERROR_FEATURE_NOT_PRESENT (not
- * supported on the device yet). */
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
 
 /* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;

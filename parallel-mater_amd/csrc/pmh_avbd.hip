@@ -507,11 +507,106 @@ __device__ __forceinline__ double iteration_bytes(uint32_t d, uint32_t m1, uint3
            (128.0 + 104.0) * m2;
 }
 
-// First-fit colouring in index order.  One wavefront per body ticket;
-// tickets are issued in index order, so every lower-index neighbour was
-// taken earlier by a resident wavefront and eventually publishes its
-// colour: the wait chain always ends.
-__global__ void __launch_bounds__(256) k_color(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
+// First-fit colouring in index order, as a dataflow: a body's colour is
+// the smallest not taken by its lower-index neighbours (contacts and
+// joints), so any schedule that colours a body after those neighbours gives
+// the sequential result.  Each lane takes one body ticket (a wavefront 64
+// consecutive ones, issued in index order) and walks its lower-index
+// neighbours with a cursor, waiting at the first one not yet coloured;
+// the wavefront loops until all its lanes have published.  Every lower
+// ticket is held by a resident wavefront whose lanes progress on their own,
+// so the waits end; a bounded loop turns a logic error into a timeout
+// status instead of a hang.
+constexpr uint32_t kColorSpinLimit = 1u << 22;
+constexpr float kLaneColorNeighbours = 6.0f;   // below this many neighbours per body: per-lane tickets
+
+__device__ __forceinline__ bool color_wait(const uint32_t *color, uint32_t j, uint64_t &u0, uint64_t &u1,
+                                           uint64_t &u2, uint64_t &u3) {
+    const uint32_t c = ld_relaxed(&color[j]);
+    if (c == kInvalid) return false;
+    if (c < kMaxColors) {
+        const uint64_t bit = 1ull << (c & 63u);
+        const uint32_t w = c >> 6;
+        u0 |= w == 0u ? bit : 0ull;
+        u1 |= w == 1u ? bit : 0ull;
+        u2 |= w == 2u ? bit : 0ull;
+        u3 |= w == 3u ? bit : 0ull;
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
+                                               const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
+                                               const uint32_t *jstart, const uint32_t *jadj,
+                                               const PmhRigidConstraint *joints, StepCounters *ctr) {
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ctr->color_ticket, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= n) break;
+        const uint32_t i = base + lane;
+        bool pending = i < n && P[i].motion == kDynamic;
+        uint32_t k = 0, ke = 0, jk = 0, je = 0;
+        if (pending) {
+            k = adj_start[i];
+            ke = adj_start[i + 1];
+            if (joints) {
+                jk = jstart[i];
+                je = jstart[i + 1];
+            }
+        }
+        uint64_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;
+        uint32_t top = 0;   // largest colour published by this lane, plus one
+        for (uint32_t spins = 0;; ++spins) {
+            if (pending) {
+                bool blocked = false;
+                for (; k < ke; ++k) {
+                    const uint2 p = ab[adj[k]];
+                    const uint32_t j = p.x == i ? p.y : p.x;
+                    if (j >= i || P[j].motion != kDynamic) continue;
+                    if (!color_wait(color, j, u0, u1, u2, u3)) { blocked = true; break; }
+                }
+                if (!blocked)
+                    for (; jk < je; ++jk) {
+                        const PmhRigidConstraint &jt = joints[jadj[jk]];
+                        const uint32_t j = jt.dense_a == i ? jt.dense_b : jt.dense_a;
+                        if (j >= i || P[j].motion != kDynamic) continue;
+                        if (!color_wait(color, j, u0, u1, u2, u3)) { blocked = true; break; }
+                    }
+                if (!blocked) {
+                    uint32_t chosen = ~u0 ? __builtin_ctzll(~u0)
+                                    : ~u1 ? 64u + __builtin_ctzll(~u1)
+                                    : ~u2 ? 128u + __builtin_ctzll(~u2)
+                                    : ~u3 ? 192u + __builtin_ctzll(~u3) : kMaxColors;
+                    if (chosen >= kMaxColors) {
+                        atomicOr(&ctr->status, kStatusColorOverflow);
+                        chosen = kMaxColors - 1;
+                    }
+                    st_relaxed(&color[i], chosen);
+                    top = chosen + 1;
+                    pending = false;
+                }
+            }
+            if (__ballot(pending) == 0ull) break;
+            if (spins >= kColorSpinLimit) {
+                if (lane == 0) atomicOr(&ctr->status, kStatusColorTimeout);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) top = max(top, uint32_t(__shfl_xor(top, d, 64)));
+        if (lane == 0 && top) atomicMax(&ctr->colors, top);
+    }
+}
+
+// The same colouring with one wavefront per body ticket, its lanes sharing
+// the body's neighbour scan: the faster form when bodies have many
+// neighbours.  Tickets are issued in index order, so every lower-index
+// neighbour was taken earlier by a resident wavefront and eventually
+// publishes its colour: the wait chain always ends.
+__global__ void __launch_bounds__(256) k_color_wave(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
                                                const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
                                                const uint32_t *jstart, const uint32_t *jadj,
                                                const PmhRigidConstraint *joints, StepCounters *ctr) {
@@ -2213,9 +2308,17 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                                        v.mean_list.p, ctr);
     if (!realtime)
         k_count_impact_pairs<<<pg, 256, 0, s>>>(npairs, P, v.ab.p, v.m_count.p, v.impact.p, ctr);
-    const uint32_t color_blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
-    k_color<<<color_blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
-                                         v.nj ? v.joints : nullptr, ctr);
+    // Per-lane tickets when bodies have few neighbours (the previous step's
+    // contact and joint ends per body), one wavefront per body otherwise.
+    if (v.neighbour_hint < kLaneColorNeighbours) {
+        const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 4 * device_compute_units()));
+        k_color_lane<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
+                                            v.nj ? v.joints : nullptr, ctr);
+    } else {
+        const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
+        k_color_wave<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
+                                            v.nj ? v.joints : nullptr, ctr);
+    }
     PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
     k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
     k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr);
@@ -2268,6 +2371,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                                  hipMemcpyDeviceToHost, s));
         PMH_CHECK(hipStreamSynchronize(s));
         const StepCounters host = *v.host_ctr;
+        v.neighbour_hint = 2.0f * float(uint64_t(host.manifolds) + v.nj) / float(std::max<uint32_t>(1, n));
         const uint32_t lanes = sp.lanes;
         if (!realtime && sp.requested_iterations == 0u) a.budget = std::max(budget, host.joint_budget);
         const uint32_t budget = a.budget;

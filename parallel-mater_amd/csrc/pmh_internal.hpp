@@ -206,6 +206,9 @@ struct Solver {
     // hinges of one level share no body, and every hinge's level is above
     // those of the earlier hinges on its bodies.
     const uint32_t *hinge_list = nullptr, *hinge_level_dev = nullptr;
+    // Neighbours per body (2 x (manifolds + joints) / bodies) of the last
+    // staged solve: picks the colouring kernel's form; infinite before one.
+    float neighbour_hint = 1.0e30f;
     std::vector<uint32_t> hinge_level_start;   // host copy: levels + 1 offsets
     // Contact rows in colour order: plane q (q = 0..8) of entry e at
     // erec[q * ecap + e], the same nine float4 as the point's Jacobian record,

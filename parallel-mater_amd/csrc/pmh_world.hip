@@ -878,16 +878,10 @@ uint32_t pmhWorldSceneCount(PmhWorld world) { return world ? uint32_t(W(world)->
 
 uint32_t pmhWorldBodyCount(PmhWorld world) { return world ? W(world)->body_count : 0; }
 
-constexpr uint32_t kMinStepBodies = 4;
-
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     if (!world) return PMH_ERROR_INVALID_ARGUMENT;
     World *w = W(world);
     if (w->body_count == 0 || steps == 0) return PMH_SUCCESS;
-    // Worlds of fewer than kMinStepBodies bodies are refused: the first
-    // steps of a two- and a three-body world (no active pairs) did not
-    // return on the device, and no cause has been shown yet.
-    if (w->body_count < kMinStepBodies) return PMH_ERROR_FEATURE_NOT_PRESENT;
     PmhResult r = w->ensure_device();
     if (r != PMH_SUCCESS) return r;
     // Substeps are enqueued back to back; each one's solve launch is
