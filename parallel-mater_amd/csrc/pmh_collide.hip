@@ -1464,6 +1464,9 @@ struct TriWaveShared {
     float4 disc[64][2], swept[64][2];      // per lane: normal+pen, point+impact fraction
     Manifold man, local;
     uint32_t count;
+    // The wavefront's triangle pair (one per wavefront, read by every lane):
+    // held here rather than copied into each lane's private memory.
+    alignas(16) unsigned char pair[sizeof(TrianglePair)];
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -1598,11 +1601,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         const uint32_t m = tri_list[t];
         const uint2 p = ab[m];
         const PmhBodyParameters pa = P[p.x], pb = P[p.y];
-        const TrianglePair tp{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
-                              ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
-                              prev[p.x], pred[p.x], prev[p.y], pred[p.y],
-                              pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
-                              pa.collision_margin + pb.collision_margin, dt};
+        TrianglePair &shared_pair = *reinterpret_cast<TrianglePair *>(S.pair);
+        wave_sync();   // every lane is done with the previous pair
+        if (lane == 0)
+            shared_pair = TrianglePair{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                                       ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
+                                       prev[p.x], pred[p.x], prev[p.y], pred[p.y],
+                                       pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
+                                       pa.collision_margin + pb.collision_margin, dt};
+        wave_sync();
+        const TrianglePair &tp = shared_pair;
         const bool swept = requires_swept(tp.ap, tp.a, tp.am.s, tp.bp, tp.b, tp.bm.s, tp.margin);
         const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
         const BoundsFrame pfa = swept ? bounds_frame(tp.ap) : BoundsFrame{};
