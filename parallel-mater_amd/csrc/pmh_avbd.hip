@@ -22,6 +22,7 @@
 #include "pmh_warmstart.hpp"
 
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstring>
@@ -2388,19 +2389,27 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         const uint32_t pts_blocks = std::max<uint32_t>(1, (std::max(host.points, v.nj) + 255) / 256);
         const uint32_t phases = realtime ? 1u : (host.impact ? 2u : 1u);
         const bool joints = v.nj != 0;
-        auto launch_color = [&](uint32_t blocks, uint32_t c, uint32_t phase) {
-            const uint32_t b = v.host_color_start[c], e = v.host_color_start[c + 1];
+        // A timed launch goes through hipExtLaunchKernel, which writes the
+        // dispatch's own start and end timestamps into the two events: no
+        // separate event commands on the stream around the kernel.
+        auto launch_color = [&](uint32_t blocks, uint32_t c, uint32_t phase, hipEvent_t start,
+                                hipEvent_t stop) -> hipError_t {
+            uint32_t b = v.host_color_start[c], e = v.host_color_start[c + 1];
+            const void *kernel;
             if (joints) {
-                if (lanes == 64) k_stage_color<64, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else if (lanes == 32) k_stage_color<32, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else if (lanes == 8) k_stage_color<8, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else k_stage_color<16, true><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                kernel = lanes == 64   ? reinterpret_cast<const void *>(&k_stage_color<64, true>)
+                         : lanes == 32 ? reinterpret_cast<const void *>(&k_stage_color<32, true>)
+                         : lanes == 8  ? reinterpret_cast<const void *>(&k_stage_color<8, true>)
+                                       : reinterpret_cast<const void *>(&k_stage_color<16, true>);
             } else {
-                if (lanes == 64) k_stage_color<64, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else if (lanes == 32) k_stage_color<32, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else if (lanes == 8) k_stage_color<8, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
-                else k_stage_color<16, false><<<blocks, 256, 0, s>>>(a, b, e, phase);
+                kernel = lanes == 64   ? reinterpret_cast<const void *>(&k_stage_color<64, false>)
+                         : lanes == 32 ? reinterpret_cast<const void *>(&k_stage_color<32, false>)
+                         : lanes == 8  ? reinterpret_cast<const void *>(&k_stage_color<8, false>)
+                                       : reinterpret_cast<const void *>(&k_stage_color<16, false>);
             }
+            void *args[] = {&a, &b, &e, &phase};
+            if (start) return hipExtLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s, start, stop, 0);
+            return hipLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s);
         };
         uint32_t launches = 0;
         for (uint32_t phase = 0; phase < phases; ++phase) {
@@ -2424,12 +2433,10 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                             kt->events.resize(2 * size_t(timed + 1) + 64, nullptr);
                             for (size_t e = old; e < kt->events.size(); ++e) PMH_CHECK(hipEventCreate(&kt->events[e]));
                         }
-                        PMH_CHECK(hipEventRecord(kt->events[2 * timed], s));
-                    }
-                    launch_color(blocks, c, phase);
-                    if (time_colors) {
-                        PMH_CHECK(hipEventRecord(kt->events[2 * timed + 1], s));
+                        PMH_CHECK(launch_color(blocks, c, phase, kt->events[2 * timed], kt->events[2 * timed + 1]));
                         ++timed;
+                    } else {
+                        PMH_CHECK(launch_color(blocks, c, phase, nullptr, nullptr));
                     }
                 }
                 if (joints) k_stage_duals<true><<<pts_blocks, 256, 0, s>>>(a, phase, it);

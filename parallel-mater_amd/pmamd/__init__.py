@@ -246,13 +246,23 @@ class World:
 
     def contact_log(self, capacity=None):
         """Logged cold-contact samples: 16 features,
-        log10(1 + lambda / (m_eff |g|)), log10(k / k0)."""
+        log10(1 + lambda / (m_eff |g|)), log10(k / k0).
+
+        The device appends rows through atomic slots, so their order varies
+        from run to run; the rows come back sorted by a 64-bit FNV-1a hash
+        of their bits, an order that depends only on the rows, so that a
+        seeded sample of the log is reproducible."""
         count = _U()
         _check(lib().pmhWorldGetContactLog(self._h, None, 0, ctypes.byref(count)), "pmhWorldGetContactLog")
         n = count.value if capacity is None else min(count.value, capacity)
         out = np.zeros((n, 18), dtype=np.float32)
         if n:
             _check(lib().pmhWorldGetContactLog(self._h, _ptr(out), n, ctypes.byref(count)), "pmhWorldGetContactLog")
+            bits = out.view(np.uint32)
+            h = np.full(n, 0xCBF29CE484222325, dtype=np.uint64)
+            for k in range(out.shape[1]):
+                h = (h ^ bits[:, k].astype(np.uint64)) * np.uint64(0x100000001B3)
+            out = out[np.argsort(h, kind="stable")]
         return out
 
     def warmstart_stats(self):

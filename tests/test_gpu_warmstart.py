@@ -91,7 +91,7 @@ def test_learned_penalty_warm_start_takes_fewer_iterations():
         w.close()
         logs.append(log[np.random.default_rng(seed).choice(len(log), size=min(300000, len(log)), replace=False)])
     samples = np.concatenate(logs)
-    params, loss = pmamd.warmstart_train(samples, epochs=4, learning_rate=1e-3, seed=3)
+    params, loss = pmamd.warmstart_train(samples, epochs=6, learning_rate=1e-3, seed=3)
     say(f"  {len(samples)} samples, loss per epoch {np.array2string(loss, precision=4)}")
     scene = S.pile(side=16, seed=33)
     result = {}
