@@ -2216,7 +2216,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     StepCounters *ctr = v.counters.p;
     const uint32_t *npairs = &ctr->pairs;
     const uint32_t *npts = &ctr->points;
-    const uint32_t bg = grid_for(n);
+    // One thread per body or joint: these grids are never capped.
+    const uint32_t bg = grid_for(n, 256, 0xffffffffu);
     const uint32_t pg = grid_for(v.pair_cap, 256, 2048);
     const uint32_t cg_ = grid_for(v.point_cap, 256, 4096);
 
@@ -2253,7 +2254,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.log = v.residual.p;
     a.tol = sp.tolerance;
     a.c_flags = v.c_flags.p;
-    const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj));
+    const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj), 256, 0xffffffffu);
     k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
     k_residual_reset<<<32, 256, 0, s>>>(v.residual.p);
     k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,

@@ -172,13 +172,21 @@ __global__ void k_bp_cells(uint32_t n, const float4 *pmn, const float4 *pmx, int
     atomicAdd(&bucket_count[b], 1u);
 }
 
-__global__ void k_bp_scatter(uint32_t n, const int4 *cell_of, const uint32_t *bucket_start, uint32_t *cursor,
-                             uint32_t *entries) {
+// Places each body in its bucket and copies, in entry order, what a query
+// tests of a candidate (cell, dynamic flag, box), so the query reads those
+// records by entry index instead of gathering them by body index.
+__global__ void k_bp_scatter(uint32_t n, const PmhBodyParameters *P, const float4 *pmn, const float4 *pmx,
+                             const int4 *cell_of, const uint32_t *bucket_start, uint32_t *cursor, int4 *entry_cell,
+                             float4 *entry_mn, float4 *entry_mx) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int4 c = cell_of[i];
     if (c.x == kLargeCell) return;
-    entries[bucket_start[c.w] + atomicAdd(&cursor[c.w], 1u)] = i;
+    const uint32_t e = bucket_start[c.w] + atomicAdd(&cursor[c.w], 1u);
+    const float4 mn = pmn[i];
+    entry_cell[e] = make_int4(c.x, c.y, c.z, P[i].motion == kDynamic ? 1 : 0);
+    entry_mn[e] = make_float4(mn.x, mn.y, mn.z, __uint_as_float(i));
+    entry_mx[e] = pmx[i];
 }
 
 __device__ __forceinline__ bool overlap(float4 amn, float4 amx, float4 bmn, float4 bmx) {
@@ -271,69 +279,93 @@ __global__ void k_scene_large(const StepCounters *ctr, const uint32_t *large, co
     }
 }
 
-// Kept candidates of dynamic body a.  Pass 0 counts them; pass 1 writes
-// them at the body's offset (from an exclusive scan of the counts) and
-// sorts the segment ascending (identity order = index), so a body may own
-// any number of pairs.
+// Kept candidates of dynamic body a, found by a group of 32 lanes: lane
+// q < 27 walks neighbour cell q of the 3x3x3 block around the body's cell,
+// and every lane takes every 32nd entry of the large-body list.  Pass 0
+// counts them; pass 1 writes each lane's candidates at the body's offset
+// (from an exclusive scan of the counts) plus the lane's prefix within the
+// group, then one lane sorts the segment ascending (identity order =
+// index), so a body may own any number of pairs.
+constexpr uint32_t kQueryLanes = 32;
+
 template <int kPass>
-__global__ void k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 *pmn, const float4 *pmx,
-                           const int4 *cell_of, const uint32_t *bucket_start, const uint32_t *entries,
-                           uint32_t table_mask, const uint32_t *large, uint32_t *cand_count,
-                           const uint32_t *offset, uint32_t *code, uint2 *ab, const StepCounters *ctr,
-                           JointFilter jf) {
-    const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_bp_query(uint32_t n, const PmhBodyParameters *P, const float4 *pmn,
+                                                  const float4 *pmx, const int4 *cell_of, const uint32_t *bucket_start,
+                                                  const int4 *entry_cell, const float4 *entry_mn,
+                                                  const float4 *entry_mx, uint32_t table_mask, const uint32_t *large,
+                                                  uint32_t *cand_count, const uint32_t *offset, uint32_t *code,
+                                                  uint2 *ab, const StepCounters *ctr, JointFilter jf) {
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t a = uint32_t(t / kQueryLanes), q = uint32_t(t % kQueryLanes);
+    // Every exit above the shuffles is uniform over the body's group.
     if (a >= n) return;
     if (P[a].motion != kDynamic) {
-        if (kPass == 0) cand_count[a] = 0;
+        if (kPass == 0 && q == 0) cand_count[a] = 0;
         return;
     }
     if (kPass == 1 && ctr->pairs == 0) return;   // pair pool overflow: nothing is emitted
-    const float4 amn = pmn[a], amx = pmx[a];
-    const uint32_t base = kPass == 1 ? offset[a] : 0;
-    uint32_t count = 0;
-    auto consider = [&](uint32_t b) {
-        if (b == a) return;
-        if (P[b].motion == kDynamic && !(a < b)) return;
-        if (!overlap(amn, amx, pmn[b], pmx[b])) return;
-        if (joint_excluded(jf, a, b)) return;
-        if (kPass == 1) code[base + count] = b;
-        ++count;
-    };
     const int4 c = cell_of[a];
     if (c.x == kLargeCell) return;   // k_bp_large owns this body
-    {
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int x = c.x + dx, y = c.y + dy, z = c.z + dz;
-                    const uint32_t bk = cell_hash(x, y, z) & table_mask;
-                    for (uint32_t e = bucket_start[bk]; e < bucket_start[bk + 1]; ++e) {
-                        const uint32_t b = entries[e];
-                        const int4 cb = cell_of[b];
-                        if (cb.x != x || cb.y != y || cb.z != z) continue;
-                        consider(b);
-                    }
-                }
+    const float4 amn = pmn[a], amx = pmx[a];
+    uint32_t count = 0;
+    auto walk = [&](auto &&emit) {
+        if (q < 27u) {
+            const int x = c.x + int(q % 3u) - 1, y = c.y + int((q / 3u) % 3u) - 1, z = c.z + int(q / 9u) - 1;
+            const uint32_t bk = cell_hash(x, y, z) & table_mask;
+            for (uint32_t e = bucket_start[bk]; e < bucket_start[bk + 1]; ++e) {
+                const int4 cb = entry_cell[e];
+                if (cb.x != x || cb.y != y || cb.z != z) continue;
+                const float4 bmn = entry_mn[e];
+                const uint32_t b = __float_as_uint(bmn.w);
+                if (b == a || (cb.w != 0 && !(a < b))) continue;
+                if (!overlap(amn, amx, bmn, entry_mx[e])) continue;
+                if (joint_excluded(jf, a, b)) continue;
+                emit(b);
+            }
+        }
+        auto consider = [&](uint32_t b) {
+            if (b == a) return;
+            if (P[b].motion == kDynamic && !(a < b)) return;
+            if (!overlap(amn, amx, pmn[b], pmx[b])) return;
+            if (joint_excluded(jf, a, b)) return;
+            emit(b);
+        };
+        if (jf.scene_large) {
+            const uint32_t sc = jf.scene[a];
+            for (uint32_t k = jf.scene_large_start[sc] + q; k < jf.scene_large_start[sc + 1]; k += kQueryLanes)
+                consider(jf.scene_large[k]);
+        } else {
+            const uint32_t nl = ctr->large_count;
+            for (uint32_t k = q; k < nl; k += kQueryLanes) consider(large[k]);
+        }
+    };
+    walk([&](uint32_t) { ++count; });   // pass 1 counts again: its lanes need their own prefixes
+    // Inclusive scan of the lane counts within the group.
+    uint32_t incl = count;
+    for (uint32_t d = 1; d < kQueryLanes; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, kQueryLanes);
+        if (q >= d) incl += y;
     }
-    if (jf.scene_large) {
-        const uint32_t sc = jf.scene[a];
-        for (uint32_t k = jf.scene_large_start[sc]; k < jf.scene_large_start[sc + 1]; ++k) consider(jf.scene_large[k]);
-    } else {
-        const uint32_t nl = ctr->large_count;
-        for (uint32_t k = 0; k < nl; ++k) consider(large[k]);
-    }
+    const uint32_t total = __shfl(incl, kQueryLanes - 1, kQueryLanes);
     if (kPass == 0) {
-        cand_count[a] = count;
+        if (q == 0) cand_count[a] = total;
         return;
     }
+    const uint32_t base = offset[a];
     uint32_t *out = code + base;
-    for (uint32_t k = 1; k < count; ++k) {
+    uint32_t at = incl - count;
+    walk([&](uint32_t b) { out[at++] = b; });
+    // The segment's writes of every lane are visible to lane 0 of the group.
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+    if (q != 0) return;
+    for (uint32_t k = 1; k < total; ++k) {
         const uint32_t v = out[k];
         uint32_t j = k;
         while (j > 0 && out[j - 1] > v) { out[j] = out[j - 1]; --j; }
         out[j] = v;
     }
-    for (uint32_t k = 0; k < count; ++k) {
+    for (uint32_t k = 0; k < total; ++k) {
         const uint32_t b = out[k];
         ab[base + k] = make_uint2(a, b);
         out[k] = a * n + b;
@@ -1887,7 +1919,9 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     const uint32_t n = body_count;
     Solver &v = solver;
     StepCounters *ctr = v.counters.p;
-    const uint32_t bg = blocks_for(n);
+    // Grids of one thread per body or joint are never capped: those kernels
+    // take one item per thread.
+    const uint32_t bg = blocks_for(n, 256, 0xffffffffu);
     const uint32_t pg = blocks_for(v.pair_cap, 256, 2048);
     auto mark = [&](int k) { if (timing_enabled) (void)hipEventRecord(events[k], s); };
     mark(0);
@@ -1908,7 +1942,7 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     }
     if (solver.nj && weld_joints) {
         k_weld_init<<<bg, 256, 0, s>>>(n, weld.p);
-        k_weld_hook<<<blocks_for(solver.nj), 256, 0, s>>>(n, solver.nj, joints_dev.p, weld.p);
+        k_weld_hook<<<blocks_for(solver.nj, 256, 0xffffffffu), 256, 0, s>>>(n, solver.nj, joints_dev.p, weld.p);
         k_weld_compress<<<bg, 256, 0, s>>>(n, weld.p);
         jf.weld = weld.p;
     }
@@ -1924,7 +1958,8 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     PmhResult r = exclusive_scan(s, bucket_count.p, bucket_start.p, table_size, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     PMH_CHECK(hipMemsetAsync(bucket_count.p, 0, sizeof(uint32_t) * table_size, s));
-    k_bp_scatter<<<bg, 256, 0, s>>>(n, cell_of.p, bucket_start.p, bucket_count.p, bucket_entries.p);
+    k_bp_scatter<<<bg, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p, bucket_start.p, bucket_count.p,
+                                    entry_cell.p, entry_mn.p, entry_mx.p);
     if (jf.scene) {
         const uint32_t ns = uint32_t(scene_first.size());
         PMH_CHECK(hipMemsetAsync(scene_large_count.p, 0, sizeof(uint32_t) * (ns + 1), s));
@@ -1939,8 +1974,9 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
         jf.scene_first = scene_first_dev.p;
         jf.scene_count = scene_count.p;
     }
-    k_bp_query<0><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
-                                                  bucket_start.p, bucket_entries.p, table_size - 1, large.p,
+    const uint32_t query_blocks = uint32_t((uint64_t(n) * kQueryLanes + 255) / 256);
+    k_bp_query<0><<<query_blocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
+                                                  bucket_start.p, entry_cell.p, entry_mn.p, entry_mx.p, table_size - 1, large.p,
                                                   cand_count.p, nullptr, nullptr, nullptr, ctr, jf);
     k_bp_large<0><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, nullptr, nullptr,
                                      nullptr, ctr, jf);
@@ -1949,8 +1985,8 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     k_set_pairs<<<1, 64, 0, s>>>(ctr, cand_offset.p, n, v.pair_cap);
     k_bp_large<1><<<kLargeBlocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, large.p, cand_count.p, cand_offset.p,
                                      v.code.p, v.ab.p, ctr, jf);
-    k_bp_query<1><<<blocks_for(n, 64), 64, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
-                                                  bucket_start.p, bucket_entries.p, table_size - 1, large.p,
+    k_bp_query<1><<<query_blocks, 256, 0, s>>>(n, params.p, proxy_mn.p, proxy_mx.p, cell_of.p,
+                                                  bucket_start.p, entry_cell.p, entry_mn.p, entry_mx.p, table_size - 1, large.p,
                                                   cand_count.p, cand_offset.p, v.code.p, v.ab.p, ctr, jf);
     mark(2);
     if (rules.scene) k_scene_pairs<<<pg, 256, 0, s>>>(&ctr->pairs, v.ab.p, scene_of.p, scene_pair_start.p);

@@ -378,8 +378,8 @@ PmhResult World::ensure_device() {
         if ((r = params.reserve(n)) || (r = state_cur.reserve(n)) || (r = state_prev.reserve(n)) ||
             (r = state_next.reserve(n)) || (r = proxy_mn.reserve(n)) || (r = proxy_mx.reserve(n)) ||
             (r = cell_of.reserve(n)) || (r = bucket_count.reserve(table_size + 1)) ||
-            (r = bucket_start.reserve(table_size + 1)) || (r = bucket_entries.reserve(n)) ||
-            (r = large.reserve(n)) ||
+            (r = bucket_start.reserve(table_size + 1)) || (r = entry_cell.reserve(n)) ||
+            (r = entry_mn.reserve(n)) || (r = entry_mx.reserve(n)) || (r = large.reserve(n)) ||
             (r = cand_count.reserve(n + 1)) || (r = cand_offset.reserve(n + 1)) ||
             (r = cache_n.reserve(2)) || (r = inputs_dev.reserve(4 * size_t(n))) ||
             (r = targets_dev.reserve(n)) || (r = target_flags_dev.reserve(n)))
@@ -637,7 +637,8 @@ void World::release() {
     snap_joints.release();
     params.release(); state_cur.release(); state_prev.release(); state_next.release();
     proxy_mn.release(); proxy_mx.release(); cell_of.release(); bucket_count.release();
-    bucket_start.release(); bucket_entries.release(); large.release();
+    bucket_start.release(); large.release();
+    entry_cell.release(); entry_mn.release(); entry_mx.release();
     cand_count.release(); cand_offset.release();
     for (int k = 0; k < 2; ++k) { cache_code[k].release(); cache_count[k].release(); cache_data[k].release(); }
     cache_n.release();

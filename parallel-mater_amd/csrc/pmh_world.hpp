@@ -68,7 +68,11 @@ struct World {
     DBuf<int4> cell_of;
     uint32_t table_size = 0;
     float grid_cell = 1.0f;
-    DBuf<uint32_t> bucket_count, bucket_start, bucket_entries, large, cand_count, cand_offset;
+    DBuf<uint32_t> bucket_count, bucket_start, large, cand_count, cand_offset;
+    // Per bucket entry, in entry order: the body's cell (w: 1 if dynamic), its
+    // box minimum (w: the body index's bits) and maximum.
+    DBuf<int4> entry_cell;
+    DBuf<float4> entry_mn, entry_mx;
 
     // contact cache, double buffered
     uint32_t cache_capacity = 0;
