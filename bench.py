@@ -8,8 +8,9 @@ analytic spheres with radius U[0.1, 0.3] m, densities log-uniform in
 options: margin 5 mm, damping 0.05, speed limits 100 m/s and 100 rad/s)
 dropped from a jittered 32^3 grid into a static five-wall bin; dt = 1/60,
 legacy AVBD profile with 10 iterations (pose phase and velocity phase),
-FP32.  W untimed steps, then exactly K timed steps (defaults 60 + 540 =
-600 steps).
+FP32, 600 steps.  The timed window is the last K steps of the 600-step run
+(the settled pile), after max(W, 600 - K) untimed steps (defaults: K = 60,
+W = 5, so steps 540..600 are timed).
 
 value    = AVBD body-iterations per second of wall time over the K steps
            (whole steps: predictor, broadphase, narrowphase, contact cache,
@@ -44,16 +45,33 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_TIMING_STRIDE = 4
 
 
-def traffic_from_profiles():
+RUN_STEPS = 600   # the pile workload's run length
+
+
+def traffic_from_profiles(window):
     """HBM bytes per colour-sweep launch measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) when
-    recorded."""
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) over the
+    same timed window of the same command, when recorded; None otherwise."""
     path = os.path.join(ROOT, "profiles", "color_sweep_traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            rec = json.load(f)
     except (OSError, ValueError):
         return None
+    if list(rec.get("window", [])) != list(window):
+        return None
+    return rec.get("bytes_per_launch")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(scene, states, seconds, profile, iterations):
@@ -71,6 +89,7 @@ def cpu_baseline(scene, states, seconds, profile, iterations):
         "value": t["body_iterations"] / (t["solve_ms"] / 1e3),
         "unit": "body-iters/s",
         "cores": 1,
+        "cpu": cpu_model(),
         "kind": "port",
         "sample": f"{steps} steps of the same pile from the GPU's state at the timed window start "
                   f"(cold contact cache), serial oracle, solve phase only; "
@@ -162,8 +181,8 @@ def warmstart_bench(args, tolerance=1e-4, cap=64, samples_wanted=1_000_000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=540)
-    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["pile", "chains", "batch", "warmstart"], default="pile",
                     help="pile: 32^3 boxes and spheres in a bin; chains: capsule chains with ball and hinge joints; "
                          "batch: independent scenes of 8-512 bodies in one world")
@@ -242,8 +261,11 @@ def main():
             print(f"# bench: step failed, stats {world.stats()}", file=sys.stderr, flush=True)
             raise
 
-    if args.warmup:
-        advance(args.warmup)
+    # The pile is timed over the last K steps of its 600-step run (settled);
+    # the other workloads after W warm-up steps.
+    settle = max(args.warmup, RUN_STEPS - args.steps) if args.workload == "pile" else args.warmup
+    if settle:
+        advance(settle)
     pmamd.lib().pmhDeviceSynchronize()
     start_states = world.states()
     world.totals(reset=True)
@@ -282,7 +304,8 @@ def main():
         kernel = (f"k_stage_color<{lanes_used}, {'true' if scene.joints is not None else 'false'}> "
                   f"(colour sweep, one launch per colour per iteration)")
     achieved = per_launch_bytes / per_launch_s / 1e9
-    traffic = traffic_from_profiles() if args.workload == "pile" and args.side == 32 else None
+    window = (settle, settle + args.steps)
+    traffic = traffic_from_profiles(window) if args.workload == "pile" and args.side == 32 else None
     name, cus = pmamd.device_info()
     line = {
         "metric": METRIC,
@@ -291,6 +314,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "untimed_steps": settle,
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -300,7 +324,8 @@ def main():
         "config": {
             "workload": f"{workload}, dt 1/60, "
                         f"{'legacy AVBD, ' + str(args.iterations) + ' iterations' if args.profile == 0 else 'realtime v1, 8 sweeps'}, "
-                        f"steps {args.warmup}..{args.warmup + args.steps}",
+                        f"timed steps {settle}..{settle + args.steps}"
+                        + (f" of a {RUN_STEPS}-step run (settled pile)" if args.workload == "pile" else ""),
             "bodies": scene.n,
             "dynamic_bodies": dynamic,
             "joints": 0 if scene.joints is None else len(scene.joints),

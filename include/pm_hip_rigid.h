@@ -163,12 +163,17 @@ typedef struct PmhWorldDesc {
 } PmhWorldDesc;
 
 #define PMH_WORLD_COOPERATIVE_SOLVE 1u  /* whole solve in one cooperative launch with grid barriers */
+#define PMH_WORLD_PHASE_TIMING 2u       /* HIP events around each phase and the solve of every step
+                                           (pmhWorldGetPhaseTimes, pmhWorldSolveTiming, and the per-launch
+                                           colour-sweep timing of kernel_timing_stride); off by default */
 
 typedef struct PmhWorldStats {
     uint32_t pairs, manifolds, points, colors, islands, impact, budget,
         unsupported_pairs,       /* round-shape pairs with no analytic path */
         cache_hits, phases;
-    uint32_t status;             /* sticky overflow bits: 1 pairs of a body, 2 points, 4 colours */
+    uint32_t status;             /* the last substep's status bits: 1 pair pool, 2 point pool,
+                                    4 colours (all overflows: PMH_ERROR_CAPACITY_EXCEEDED), 16 a
+                                    colouring that gave up (PMH_ERROR_INVALID_STATE) */
     uint32_t pair_capacity, point_capacity, points_allocated;
     uint32_t triangle_pairs;     /* mesh pairs on the triangle-pair path */
     uint32_t fallback_pairs;     /* of those, solved by BVH traversal (no candidate slot or too many) */
@@ -347,7 +352,24 @@ PmhResult pmhWorldApplyRigidInputs(PmhWorld world, uint32_t count, const PmhRigi
 PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t count,
                                       const PmhRigidBodyState *targets);
 
+/* pmCmdStepSimulation (src/api/resources.cpp:1230-1288): `steps` steps
+ * of `substeps` substeps each.  Returns at the first substep whose status
+ * fails (PMH_ERROR_CAPACITY_EXCEEDED for a full pair or point pool or more
+ * than 256 colours, PMH_ERROR_INVALID_STATE for a colouring that gave up);
+ * the states are then those before that substep.  Worlds of fewer than four
+ * bodies are refused with PMH_ERROR_FEATURE_NOT_PRESENT: the first GPU steps
+ * of a two- and a three-body world with no active pair did not return on an
+ * earlier build, and no cause has been shown.  PMH_ERROR_DEVICE_LOST when
+ * the device does not finish within PMH_SYNC_TIMEOUT_S seconds (default
+ * 120). */
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
+
+/* Polls of one colouring wait before it gives up (default 2^22): a wait
+ * that runs out ends the whole colouring (every other wait gives up too)
+ * and the step returns PMH_ERROR_INVALID_STATE.  No counterpart in the
+ * reference; 0 makes the first wait on an uncoloured neighbour give up, so
+ * a test can take that path without a stalled device. */
+PmhResult pmhWorldSetColorWaitLimit(PmhWorld world, uint32_t polls);
 
 /* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;
  * runtime_rigid.cu.inc:147-195: a set-state resets AVBD history and the

@@ -25,7 +25,10 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace cg = cooperative_groups;
 
@@ -241,7 +244,7 @@ __global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *
                                const PmhRigidBodyState *st, const uint2 *ab, const uint32_t *m_count,
                                const uint32_t *m_start, float4 *c_np, float4 *c_pt, float4 *c_aa, float4 *c_ab,
                                float4 *c_err, float4 *c_lam, float4 *c_pen, const uint32_t *c_flags,
-                               uint32_t *m_impact, float dt, uint32_t profile) {
+                               uint32_t *m_impact, uint32_t *c_entry, float dt, uint32_t profile) {
     const uint32_t M = *npairs;
     const bool realtime = profile == kRealtimeProfile;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
@@ -254,6 +257,10 @@ __global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *
         const uint32_t start = m_start[m], count = m_count[m];
         for (uint32_t k = 0; k < count; ++k) {
             const uint32_t c = start + k;
+            // Colour-ordered slots of the point's endpoints: written once
+            // the bodies are placed by colour; a body the colouring gave up
+            // on has none.
+            c_entry[2 * c] = c_entry[2 * c + 1] = kInvalid;
             const bool cached = (c_flags[c] & 1u) != 0u;
             float4 np = c_np[c], pt = c_pt[c];
             const V3 normal = v3(np), point = v3(pt);
@@ -516,9 +523,12 @@ __device__ __forceinline__ double iteration_bytes(uint32_t d, uint32_t m1, uint3
 // neighbours with a cursor, waiting at the first one not yet coloured;
 // the wavefront loops until all its lanes have published.  Every lower
 // ticket is held by a resident wavefront whose lanes progress on their own,
-// so the waits end; a bounded loop turns a logic error into a timeout
-// status instead of a hang.
-constexpr uint32_t kColorSpinLimit = 1u << 22;
+// so the waits end.  Every wait is bounded all the same: a wait that runs
+// past `limit` polls raises ctr->color_abort and the timeout status, every
+// other wait gives up as soon as it sees the flag, and a body that gave up
+// publishes kColorAborted, so no body waits on it.  The whole colouring then
+// ends within about one limit's worth of polls, and the step returns an
+// error before its solve.
 constexpr float kLaneColorNeighbours = 6.0f;   // below this many neighbours per body: per-lane tickets
 
 __device__ __forceinline__ bool color_wait(const uint32_t *color, uint32_t j, uint64_t &u0, uint64_t &u1,
@@ -539,7 +549,8 @@ __device__ __forceinline__ bool color_wait(const uint32_t *color, uint32_t j, ui
 __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
                                                const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
                                                const uint32_t *jstart, const uint32_t *jadj,
-                                               const PmhRigidConstraint *joints, StepCounters *ctr) {
+                                               const PmhRigidConstraint *joints, StepCounters *ctr,
+                                               uint32_t limit) {
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         uint32_t base = 0;
@@ -559,7 +570,7 @@ __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyPar
         }
         uint64_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;
         uint32_t top = 0;   // largest colour published by this lane, plus one
-        for (uint32_t spins = 0;; ++spins) {
+        for (uint32_t polls = 0;; ++polls) {
             if (pending) {
                 bool blocked = false;
                 for (; k < ke; ++k) {
@@ -590,8 +601,12 @@ __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyPar
                 }
             }
             if (__ballot(pending) == 0ull) break;
-            if (spins >= kColorSpinLimit) {
-                if (lane == 0) atomicOr(&ctr->status, kStatusColorTimeout);
+            if (polls >= limit && lane == 0) st_relaxed(&ctr->color_abort, 1u);
+            if (polls >= limit || ld_relaxed(&ctr->color_abort) != 0u) {
+                if (pending) {
+                    st_relaxed(&color[i], kColorAborted);
+                    atomicOr(&ctr->status, kStatusColorTimeout);
+                }
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -607,10 +622,22 @@ __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyPar
 // neighbours.  Tickets are issued in index order, so every lower-index
 // neighbour was taken earlier by a resident wavefront and eventually
 // publishes its colour: the wait chain always ends.
+__device__ __forceinline__ uint32_t color_wait_bounded(const uint32_t *color, uint32_t j, StepCounters *ctr,
+                                                       uint32_t limit) {
+    uint32_t c, polls = 0;
+    while ((c = ld_relaxed(&color[j])) == kInvalid) {
+        if (polls++ >= limit) st_relaxed(&ctr->color_abort, 1u);
+        if (ld_relaxed(&ctr->color_abort) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return c;
+}
+
 __global__ void __launch_bounds__(256) k_color_wave(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
                                                const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
                                                const uint32_t *jstart, const uint32_t *jadj,
-                                               const PmhRigidConstraint *joints, StepCounters *ctr) {
+                                               const PmhRigidConstraint *joints, StepCounters *ctr,
+                                               uint32_t limit) {
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         uint32_t t = 0;
@@ -620,31 +647,27 @@ __global__ void __launch_bounds__(256) k_color_wave(uint32_t n, const PmhBodyPar
         const uint32_t i = t;
         if (P[i].motion != kDynamic) continue;
         uint64_t used[kMaxColors / 64] = {0, 0, 0, 0};
+        bool gave_up = false;
         const uint32_t b = adj_start[i], e = adj_start[i + 1];
         for (uint32_t k = b + lane; k < e; k += 64) {
             const uint2 p = ab[adj[k]];
             const uint32_t j = p.x == i ? p.y : p.x;
             if (j >= i || P[j].motion != kDynamic) continue;
-            uint32_t c;
-            uint32_t spins = 0;
-            // Bounded wait: a lower-index neighbour always finishes, but a
-            // logic error must end the kernel with an error, not hang it.
-            while ((c = ld_relaxed(&color[j])) == kInvalid && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
-            if (c == kInvalid) { atomicOr(&ctr->status, kStatusColorTimeout); continue; }
+            const uint32_t c = color_wait_bounded(color, j, ctr, limit);
+            if (c == kInvalid) { gave_up = true; break; }
             if (c < kMaxColors) used[c >> 6] |= 1ull << (c & 63);
         }
-        if (joints) {
+        if (joints && !gave_up) {
             for (uint32_t k = jstart[i] + lane; k < jstart[i + 1]; k += 64) {
                 const PmhRigidConstraint &jt = joints[jadj[k]];
                 const uint32_t j = jt.dense_a == i ? jt.dense_b : jt.dense_a;
                 if (j >= i || P[j].motion != kDynamic) continue;
-                uint32_t c;
-                uint32_t spins = 0;
-                while ((c = ld_relaxed(&color[j])) == kInvalid && ++spins < (1u << 22)) __builtin_amdgcn_s_sleep(1);
-                if (c == kInvalid) { atomicOr(&ctr->status, kStatusColorTimeout); continue; }
+                const uint32_t c = color_wait_bounded(color, j, ctr, limit);
+                if (c == kInvalid) { gave_up = true; break; }
                 if (c < kMaxColors) used[c >> 6] |= 1ull << (c & 63);
             }
         }
+        const bool abandoned = __ballot(gave_up) != 0ull;
         uint32_t chosen = kMaxColors;
 #pragma unroll
         for (uint32_t w = 0; w < kMaxColors / 64; ++w) {
@@ -653,7 +676,10 @@ __global__ void __launch_bounds__(256) k_color_wave(uint32_t n, const PmhBodyPar
             for (int o = 1; o < 64; o <<= 1) m |= __shfl_xor(m, o, 64);
             if (chosen == kMaxColors && m != ~0ull) chosen = w * 64 + __builtin_ctzll(~m);
         }
-        if (lane == 0) {
+        if (lane == 0 && abandoned) {
+            st_relaxed(&color[i], kColorAborted);
+            atomicOr(&ctr->status, kStatusColorTimeout);
+        } else if (lane == 0) {
             if (chosen >= kMaxColors) { atomicOr(&ctr->status, kStatusColorOverflow); chosen = kMaxColors - 1; }
             st_relaxed(&color[i], chosen);
             atomicMax(&ctr->colors, chosen + 1);
@@ -667,7 +693,7 @@ __global__ void k_color_count(uint32_t n, const uint32_t *color, uint32_t *count
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t c = i < n ? color[i] : kInvalid;
-    bool pending = c != kInvalid;
+    bool pending = c < kMaxColors;
     while (__ballot(pending) != 0ull) {
         const uint64_t live = __ballot(pending);
         const uint32_t leader = __builtin_ctzll(live);
@@ -695,7 +721,7 @@ __global__ void k_color_place(uint32_t n, const uint32_t *color, const uint32_t 
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lane = threadIdx.x & 63;
     uint32_t c = i < n ? color[i] : kInvalid;
-    bool pending = c != kInvalid;
+    bool pending = c < kMaxColors;
     while (__ballot(pending) != 0ull) {
         const uint64_t live = __ballot(pending);
         const uint32_t leader = __builtin_ctzll(live);
@@ -753,15 +779,14 @@ __device__ __forceinline__ void store_rows(float4 *jac, float4 *erec, size_t eca
     float4 *dst = jac + 9 * (size_t)c;
 #pragma unroll
     for (int q = 0; q < 9; ++q) dst[q] = J[q];
-    if (dyn_a) {
-        const size_t e = c_entry[2 * c];
+    const uint32_t ea = dyn_a ? c_entry[2 * c] : kInvalid, eb = dyn_b ? c_entry[2 * c + 1] : kInvalid;
+    if (ea != kInvalid) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) erec[q * ecap + e] = J[q];
+        for (int q = 0; q < 9; ++q) erec[q * ecap + ea] = J[q];
     }
-    if (dyn_b) {
-        const size_t e = c_entry[2 * c + 1];
+    if (eb != kInvalid) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) erec[q * ecap + e] = J[q];
+        for (int q = 0; q < 9; ++q) erec[q * ecap + eb] = J[q];
     }
 }
 
@@ -2073,8 +2098,10 @@ __global__ void k_residual_reset(ResidualLog *log) {
 }
 
 __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32_t budget, uint32_t profile,
-                                uint32_t base_iterations) {
+                                uint32_t base_iterations, uint32_t reset_status) {
     if (threadIdx.x != 0) return;
+    if (reset_status) ctr->status = 0;
+    ctr->color_abort = 0;
     PmhContactSchedule v;
     memset(&v, 0, sizeof v);
     v.budget = budget;
@@ -2102,6 +2129,33 @@ __global__ void k_schedule_finish(PmhContactSchedule *s, const StepCounters *ctr
     s->candidates = ctr->pairs;
     s->contacts = ctr->manifolds;
     s->blocks = blocks;
+}
+
+PmhResult status_result(uint32_t status) {
+    if (status & (kStatusPairOverflow | kStatusPointOverflow | kStatusColorOverflow)) return PMH_ERROR_CAPACITY_EXCEEDED;
+    if (status & kStatusColorTimeout) return PMH_ERROR_INVALID_STATE;
+    return PMH_SUCCESS;
+}
+
+PmhResult sync_stream(hipStream_t s) {
+    static const double limit = [] {
+        const char *e = getenv("PMH_SYNC_TIMEOUT_S");
+        const double v = e ? atof(e) : 0.0;
+        return v > 0.0 ? v : 120.0;
+    }();
+    const auto start = std::chrono::steady_clock::now();
+    for (uint32_t polls = 0;; ++polls) {
+        const hipError_t e = hipStreamQuery(s);
+        if (e == hipSuccess) return PMH_SUCCESS;
+        if (e != hipErrorNotReady) return hip_result(e, "hipStreamQuery");
+        if (polls >= 64) {
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() > limit) {
+                (void)hip_result(hipErrorNotReady, "device did not finish the step within PMH_SYNC_TIMEOUT_S");
+                return PMH_ERROR_DEVICE_LOST;
+            }
+            std::this_thread::yield();
+        }
+    }
 }
 
 static int g_cus = 0;
@@ -2255,7 +2309,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.tol = sp.tolerance;
     a.c_flags = v.c_flags.p;
     const uint32_t jg = grid_for(std::max<uint32_t>(1, v.nj), 256, 0xffffffffu);
-    k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations);
+    k_schedule_init<<<1, 64, 0, s>>>(v.schedule.p, ctr, budget, sp.profile, sp.base_iterations,
+                                     sp.reset_status ? 1u : 0u);
     k_residual_reset<<<32, 256, 0, s>>>(v.residual.p);
     k_body_init<<<bg, 256, 0, s>>>(n, P, previous, states, dt, sp.gravity, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
                                    v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
@@ -2263,7 +2318,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                                    v.jdeg.p, v.jcursor.p);
     k_contact_init<<<pg, 256, 0, s>>>(npairs, P, previous, states, v.ab.p, v.m_count.p, v.m_start.p, v.c_np.p,
                                       v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_flags.p,
-                                      v.m_impact.p, dt, sp.profile);
+                                      v.m_impact.p, v.c_entry.p, dt, sp.profile);
     const uint32_t cold_cap = uint32_t(v.cold_list.n);
     const bool warm = (v.mlp && sp.warm_infer) || v.log;
     if (warm && cold_cap) {
@@ -2315,11 +2370,11 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     if (v.neighbour_hint < kLaneColorNeighbours) {
         const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, 4 * device_compute_units()));
         k_color_lane<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
-                                            v.nj ? v.joints : nullptr, ctr);
+                                            v.nj ? v.joints : nullptr, ctr, sp.color_wait_limit);
     } else {
         const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
         k_color_wave<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
-                                            v.nj ? v.joints : nullptr, ctr);
+                                            v.nj ? v.joints : nullptr, ctr, sp.color_wait_limit);
     }
     PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
     k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
@@ -2371,8 +2426,12 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         PMH_CHECK(hipMemcpyAsync(v.host_ctr, ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
         PMH_CHECK(hipMemcpyAsync(v.host_color_start, v.color_start.p, sizeof(uint32_t) * (kMaxColors + 1),
                                  hipMemcpyDeviceToHost, s));
-        PMH_CHECK(hipStreamSynchronize(s));
+        PmhResult sr = sync_stream(s);
+        if (sr != PMH_SUCCESS) return sr;
         const StepCounters host = *v.host_ctr;
+        // Overflowed pools or a colouring that gave up: no solve on a
+        // partial schedule; the caller keeps the pre-step states.
+        if (host.status & kStatusFatal) return status_result(host.status);
         v.neighbour_hint = 2.0f * float(uint64_t(host.manifolds) + v.nj) / float(std::max<uint32_t>(1, n));
         const uint32_t lanes = sp.lanes;
         if (!realtime && sp.requested_iterations == 0u) a.budget = std::max(budget, host.joint_budget);

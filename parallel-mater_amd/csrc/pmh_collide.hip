@@ -1893,6 +1893,8 @@ __global__ void k_clamp_speeds(uint32_t n, const PmhBodyParameters *P, PmhRigidB
 
 __global__ void k_step_begin(StepCounters *ctr) {
     if (threadIdx.x != 0) return;
+    ctr->status = 0;   // per substep: the host checks it before the solve and after the last substep
+    ctr->color_abort = 0;
     ctr->pairs = 0;
     ctr->points = 0;
     ctr->manifolds = 0;
@@ -2013,7 +2015,7 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
                                     v.c_flags.p, dt, ctr);
     mark(4);
     SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes,
-                   residual_tolerance, warm_infer && warm_enabled};
+                   residual_tolerance, warm_infer && warm_enabled, color_wait_limit};
     SolveTiming st;
     st.kernels = &kernel_timer;
     if (timing_enabled) {

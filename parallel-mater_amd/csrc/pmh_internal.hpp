@@ -104,6 +104,7 @@ struct StepCounters {
     uint32_t joint_budget;     // largest joint iteration count (legacy budget floor)
     uint32_t cold_points;      // cold contact points listed for the learned warm start
     uint32_t log_count;        // contact samples logged so far (learned warm start)
+    uint32_t color_abort;      // set when a colour wait ran out: every other wait gives up
 };
 
 // Running totals over every step since the last reset.
@@ -113,6 +114,18 @@ struct SolveTotals {
 };
 
 constexpr uint32_t kMaxColors = 256;
+// A body the colouring gave up on (a wait ran out) holds this value: not a
+// colour, so no solve stage sees it, and published, so nobody waits on it.
+constexpr uint32_t kColorAborted = kMaxColors;
+constexpr uint32_t kColorWaitLimit = 1u << 22;
+// Status bits that end a step with an error before its solve.
+constexpr uint32_t kStatusFatal = kStatusPairOverflow | kStatusPointOverflow | kStatusColorOverflow |
+                                  kStatusColorTimeout;
+PmhResult status_result(uint32_t status);
+// Waits for the stream with a deadline (PMH_SYNC_TIMEOUT_S, default 120 s):
+// a device that stops answering gives PMH_ERROR_DEVICE_LOST, not a caller
+// blocked for ever.
+PmhResult sync_stream(hipStream_t s);
 constexpr uint32_t kDefaultLanes = 32;   // lanes per body in the colour sweep
 // Joint types (RigidConstraintType, src/simulation/types.hpp:662).
 enum : uint32_t { kJointFixed = 0, kJointPoint = 1, kJointHinge = 2, kJointSlider = 3, kJointPiston = 4,
@@ -260,6 +273,8 @@ struct SolveParams {
     uint32_t lanes;                      // lanes of a wavefront per body in the colour sweep: 8, 16, 32, 64
     float tolerance = 0.0f;              // > 0: to-residual mode, the budget caps each phase
     bool warm_infer = false;             // predict cold points' normal duals
+    uint32_t color_wait_limit = kColorWaitLimit;   // polls of one colour wait before it gives up
+    bool reset_status = false;           // clear the status word first (the kernel seam; a world step clears it itself)
 };
 
 

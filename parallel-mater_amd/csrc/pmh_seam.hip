@@ -158,6 +158,7 @@ extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
     k_seam_history_in<<<bg, 256, 0, s>>>(body_count, bodies, v.prev_vel.p);
     SolveParams sp{timestep, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations,
                    profile_version, 0u, kDefaultLanes};
+    sp.reset_status = true;
     r = avbd_prepare_solve(s, v, parameters, previous, states, sp, nullptr);
     if (r != PMH_SUCCESS) return r;
     if (M)

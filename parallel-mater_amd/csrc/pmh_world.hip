@@ -703,7 +703,7 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     w->residual_tolerance = desc->residual_tolerance;
     w->kernel_timer.stride = desc->kernel_timing_stride;
     w->device = int(desc->device);
-    w->timing_enabled = true;
+    w->timing_enabled = (desc->flags & PMH_WORLD_PHASE_TIMING) != 0;
     *out = reinterpret_cast<PmhWorld>(w);
     return PMH_SUCCESS;
 }
@@ -879,14 +879,23 @@ uint32_t pmhWorldSceneCount(PmhWorld world) { return world ? uint32_t(W(world)->
 
 uint32_t pmhWorldBodyCount(PmhWorld world) { return world ? W(world)->body_count : 0; }
 
+// Worlds of fewer than kMinStepBodies bodies are refused: the first GPU
+// steps of a two- and a three-body world with no active pair did not return
+// on an earlier build of the step, and no cause has been shown.
+constexpr uint32_t kMinStepBodies = 4;
+
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     if (!world) return PMH_ERROR_INVALID_ARGUMENT;
     World *w = W(world);
     if (w->body_count == 0 || steps == 0) return PMH_SUCCESS;
+    if (w->body_count < kMinStepBodies) return PMH_ERROR_FEATURE_NOT_PRESENT;
     PmhResult r = w->ensure_device();
     if (r != PMH_SUCCESS) return r;
     // Substeps are enqueued back to back; each one's solve launch is
-    // bracketed by its own event pair, read once the batch is done.
+    // bracketed by its own event pair, read once the batch is done.  Every
+    // substep reads its status back before its solve (with the colour
+    // schedule) and returns at the first overflow or colouring failure; the
+    // states then stay those before the failing substep.
     const uint64_t total = uint64_t(steps) * w->substeps;
     const uint32_t batch = 256;
     for (uint64_t done = 0; done < total; done += batch) {
@@ -899,9 +908,14 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
         const bool fallback = w->warm_enabled && w->residual_tolerance > 0.0f;
         for (uint32_t k = 0; k < count; ++k) {
             r = fallback ? w->step_with_fallback(k) : w->step_once(k);
-            if (r != PMH_SUCCESS) return r;
+            if (r != PMH_SUCCESS) {
+                w->has_stepped = true;   // earlier substeps of the batch did run
+                const PmhResult s = pmh::sync_stream(w->stream);
+                return s != PMH_SUCCESS ? s : r;
+            }
         }
-        PMH_CHECK(hipStreamSynchronize(w->stream));
+        r = pmh::sync_stream(w->stream);
+        if (r != PMH_SUCCESS) return r;
         if (w->timing_enabled) {
             for (uint32_t k = 0; k < count; ++k) {
                 float ms = 0.0f;
@@ -918,9 +932,12 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     w->has_stepped = true;
     pmh::StepCounters c;
     PMH_CHECK(hipMemcpy(&c, w->solver.counters.p, sizeof c, hipMemcpyDeviceToHost));
-    if (c.status & (pmh::kStatusPairOverflow | pmh::kStatusPointOverflow | pmh::kStatusColorOverflow))
-        return PMH_ERROR_CAPACITY_EXCEEDED;
-    if (c.status & pmh::kStatusColorTimeout) return PMH_ERROR_INVALID_STATE;
+    return pmh::status_result(c.status);
+}
+
+PmhResult pmhWorldSetColorWaitLimit(PmhWorld world, uint32_t polls) {
+    if (!world) return PMH_ERROR_INVALID_ARGUMENT;
+    W(world)->color_wait_limit = polls;
     return PMH_SUCCESS;
 }
 
@@ -1203,6 +1220,7 @@ PmhResult pmhWorldGetPhaseTimes(PmhWorld world, PmhPhaseTimes *times) {
     World *w = W(world);
     std::memset(times, 0, sizeof *times);
     if (!w->has_stepped) return PMH_SUCCESS;
+    if (!w->timing_enabled) return PMH_ERROR_FEATURE_NOT_PRESENT;
     for (int k = 0; k < PMH_PHASE_COUNT - 1; ++k)
         PMH_CHECK(hipEventElapsedTime(&times->ms[k], w->events[k], w->events[k + 1]));
     // Split the prepare+solve interval at the solve kernel's own events.

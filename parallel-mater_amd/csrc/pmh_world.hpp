@@ -15,6 +15,7 @@ struct World {
     bool cooperative = false;
     uint32_t solve_lanes = kDefaultLanes;
     float residual_tolerance = 0.0f;
+    uint32_t color_wait_limit = kColorWaitLimit;
     int device = 0;
     hipStream_t stream = nullptr;
 

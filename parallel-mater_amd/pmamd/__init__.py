@@ -28,6 +28,7 @@ RESULT_NAMES = {0: "PM_SUCCESS", -1: "PM_ERROR_INVALID_ARGUMENT", -2: "PM_ERROR_
                 -5: "PM_ERROR_FEATURE_NOT_PRESENT", -7: "PM_ERROR_CAPACITY_EXCEEDED",
                 -8: "PM_ERROR_DEVICE_LOST"}
 COOPERATIVE_SOLVE = 1
+PHASE_TIMING = 2
 PHASES = ("integrate", "broadphase", "narrowphase", "contact_cache", "prepare", "solve", "finish")
 
 _P = ctypes.c_void_p
@@ -98,6 +99,7 @@ def lib():
         "pmhWorldSceneCount": (_U, [_P]),
         "pmhWorldSetSolveLanes": (i32, [_P, _U]),
         "pmhWorldStep": (i32, [_P, _U]),
+        "pmhWorldSetColorWaitLimit": (i32, [_P, _U]),
         "pmhWorldApplyRigidInputs": (i32, [_P, _U, _P]),
         "pmhWorldSetKinematicTargets": (i32, [_P, _U, _U, _P]),
         "pmhWorldAddJoints": (i32, [_P, _U, _P]),
@@ -168,10 +170,11 @@ class World:
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
                  pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
-                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0, substeps=1):
+                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0, substeps=1, timing=True):
+        flags = (COOPERATIVE_SOLVE if cooperative else 0) | (PHASE_TIMING if timing else 0)
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
-                         points_per_body, device, COOPERATIVE_SOLVE if cooperative else 0,
+                         points_per_body, device, flags,
                          solve_lanes, kernel_timing_stride, residual_tolerance, substeps)
         h = _P()
         _check(lib().pmhCreateWorld(ctypes.byref(desc), ctypes.byref(h)), "pmhCreateWorld")
@@ -326,6 +329,11 @@ class World:
 
     def step(self, steps=1):
         _check(lib().pmhWorldStep(self._h, steps), "pmhWorldStep")
+
+    def set_color_wait_limit(self, polls):
+        """Polls of one colouring wait before it gives up (default 2**22);
+        0 gives up at the first wait on an uncoloured neighbour."""
+        _check(lib().pmhWorldSetColorWaitLimit(self._h, polls), "pmhWorldSetColorWaitLimit")
 
     def residuals(self, maximum=False):
         """(per-iteration RMS constraint residuals [2, 64], NaN where no

@@ -425,16 +425,20 @@ def concave_drop(count=12, seed=5, speed=8.0):
     return Scene(shapes, p, st, "concave_drop")
 
 
-def crossing_edges(gap=5e-4, half=0.5):
+def crossing_edges(gap=5e-4, half=0.5, far=0):
     """Two boxes meeting edge on edge (the convex face path declines this
     pair): the static one turned 45 degrees about z, the dynamic one 45
-    degrees about x and `gap` above it."""
+    degrees about x and `gap` above it.  `far` more static boxes stand 50 m
+    away along x, out of reach of both (they only raise the body count)."""
     h = np.float32(half)
     top = h * np.float32(np.sqrt(2.0))
     shapes = [("mesh", *box_mesh(h, h, h))]
-    motion = np.array([L.MOTION_STATIC, L.MOTION_DYNAMIC], np.uint32)
-    p = params(2, motion=motion, mass=1000.0, inertia=box_inertia(1000.0, (h, h, h)), shape=np.array([0, 0]))
-    st = states(2, np.array([[0, 0, 0], [0, 2 * top + np.float32(gap), 0]], np.float32))
+    n = 2 + far
+    motion = np.full(n, L.MOTION_STATIC, np.uint32)
+    motion[1] = L.MOTION_DYNAMIC
+    p = params(n, motion=motion, mass=1000.0, inertia=box_inertia(1000.0, (h, h, h)), shape=np.zeros(n, np.int64))
+    pos = [[0, 0, 0], [0, 2 * top + np.float32(gap), 0]] + [[50.0 * (k + 1), 0, 0] for k in range(far)]
+    st = states(n, np.array(pos, np.float32))
     st["orientation"][0] = quat((0, 0, 1), np.pi / 4)
     st["orientation"][1] = quat((1, 0, 0), np.pi / 4)
     return Scene(shapes, p, st, "crossing_edges")
