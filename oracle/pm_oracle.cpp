@@ -2988,8 +2988,17 @@ int pmo_avbd_prepare_solve(const PmoParams *parameters, const PmoState *previous
     return prepare_solve(x, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations, profile_version);
 }
 
+// The realtime contract: a 1/60 s step within one float ulp and one
+// substep (rigid_realtime.hpp:16-26; pmCmdStepSimulation rejects anything
+// else, src/api/resources.cpp:1240-1241).
+static bool realtime_step_valid(float dt, uint32_t substeps) {
+    const float r = 1.0f / 60.0f;
+    return std::isfinite(dt) && dt >= std::nextafter(r, 0.0f) && dt <= std::nextafter(r, INFINITY) && substeps <= 1u;
+}
+
 void *pmo_world_create(const PmoWorldDesc *desc) {
     if (!desc || !(desc->dt > 0.0f) || (desc->lanes & (desc->lanes - 1u))) return nullptr;
+    if (desc->profile_version == 1 && !realtime_step_valid(desc->dt, desc->substeps)) return nullptr;
     World *w = new World();
     w->desc = *desc;
     return w;

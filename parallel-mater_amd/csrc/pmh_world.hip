@@ -669,8 +669,19 @@ static PmhResult fetch(std::vector<T> &out, const T *src, size_t count) {
 
 extern "C" {
 
+// The realtime contract (rigid_realtime.hpp:16-26, checked by
+// pmCmdStepSimulation at src/api/resources.cpp:1240-1241): a 1/60 s step,
+// within one float ulp either side, and one substep.
+static bool realtime_step_valid(float timestep, uint32_t substeps) {
+    const float dt = 1.0f / 60.0f;
+    return std::isfinite(timestep) && timestep >= std::nextafter(dt, 0.0f) &&
+           timestep <= std::nextafter(dt, INFINITY) && substeps <= 1u;
+}
+
 PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     if (!desc || !out || !(desc->timestep > 0.0f) || !std::isfinite(desc->timestep) || desc->profile_version > 1)
+        return PMH_ERROR_INVALID_ARGUMENT;
+    if (desc->profile_version == 1 && !realtime_step_valid(desc->timestep, desc->substeps))
         return PMH_ERROR_INVALID_ARGUMENT;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= int(desc->device)) return PMH_ERROR_FEATURE_NOT_PRESENT;

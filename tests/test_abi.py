@@ -70,3 +70,14 @@ def test_invalid_arguments_are_rejected():
     assert lib.pmhPrepareSolveGeometryConstraints(None, None, None, None, 0, None, None, None, None,
                                                   None, 0.0, None, 0, 0, 1) == -1
     assert lib.pmhWorldAddMesh(None, None, 0, None, 0, ctypes.byref(out)) == -1
+
+
+def test_realtime_worlds_take_the_realtime_step_only():
+    """Realtime v1 is a 1/60 s step (one float ulp either side) of one
+    substep (pmCmdStepSimulation, resources.cpp:1240-1241): anything else is
+    an invalid argument, refused before any device is touched."""
+    lib = pmamd.lib()
+    out = ctypes.c_void_p()
+    for dt, substeps in ((1.0 / 120.0, 1), (1.0 / 60.0, 4), (float(np.float32(1.0 / 60.0) * np.float32(1.0 + 3e-7)), 1)):
+        desc = pmamd.WorldDesc((ctypes.c_float * 3)(0.0, -9.81, 0.0), dt, 1, 10, 0, 0, 0, 0, 0, 0, 0, 0, 0.0, substeps)
+        assert lib.pmhCreateWorld(ctypes.byref(desc), ctypes.byref(out)) == -1

@@ -12,6 +12,7 @@ impulse 0.896216, broken and disabled), a three-body weld compound moving as
 one, released (its members then touch: 8 contact events with their
 positions, normals and impulses) and rebuilt (fresh joint state).
 """
+import numpy as np
 import pytest
 
 import _oracle as O
@@ -33,10 +34,26 @@ def test_oracle_matches_the_cuda_capture(name):
 
 
 @pytest.mark.parametrize("name, change", [("rigid-direct", dict(substeps=1)),
-                                          ("constraint-breaking", dict(profile=1)),
+                                          ("constraint-breaking", dict(profile=1, substeps=1)),
                                           ("compound-weld-lifecycle", dict(substeps=2))])
 def test_a_different_step_contract_fails_the_capture(name, change):
-    """One or two substeps per frame, or the realtime profile, each move the
-    oracle off the capture: the comparison discriminates."""
+    """One or two substeps per frame, or the realtime profile (whose contract
+    is one substep), each move the oracle off the capture: the comparison
+    discriminates."""
     result = C.run(name, oracle_world(**change))
     assert C.compare(name, result, C.EXACT_FRAMES.get(name)) != []
+
+
+@pytest.mark.parametrize("dt, substeps, ok", [(1.0 / 60.0, 1, True), (np.nextafter(np.float32(1.0 / 60.0), 1.0), 1, True),
+                                              (1.0 / 60.0, 4, False), (1.0 / 120.0, 1, False),
+                                              (np.float32(1.0 / 60.0) * np.float32(1.0 + 3e-7), 1, False)])
+def test_realtime_worlds_take_the_realtime_step_only(dt, substeps, ok):
+    """Realtime v1 is a 1/60 s step (within one float ulp) of one substep
+    (rigid_realtime.hpp:16-26, pmCmdStepSimulation at resources.cpp:1240-1241);
+    anything else is refused at creation.  The legacy profile takes any."""
+    if ok:
+        O.OracleWorld(dt=float(dt), profile=1, substeps=substeps)
+    else:
+        with pytest.raises(ValueError):
+            O.OracleWorld(dt=float(dt), profile=1, substeps=substeps)
+    O.OracleWorld(dt=float(dt), profile=0, substeps=substeps)
