@@ -364,6 +364,10 @@ PmhResult pmhWorldSetKinematicTargets(PmhWorld world, uint32_t first, uint32_t c
  * 120). */
 PmhResult pmhWorldStep(PmhWorld world, uint32_t steps);
 
+/* PmStepInfo.gravity of pmCmdStepSimulation (src/api/resources.cpp:1262):
+ * the gravity of the following steps (finite components). */
+PmhResult pmhWorldSetGravity(PmhWorld world, const float gravity[3]);
+
 /* Polls of one colouring wait before it gives up (default 2^22): a wait
  * that runs out ends the whole colouring (every other wait gives up too)
  * and the step returns PMH_ERROR_INVALID_STATE.  No counterpart in the

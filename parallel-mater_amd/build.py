@@ -53,7 +53,34 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for obj in objs:
         os.remove(obj)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "pmh_driver.cpp")
+DRIVER = os.path.join(OUT, "libpm_hip_driver.so")
+MANIFEST = os.path.join(OUT, "pm_hip_driver.json")
+
+
+def build_driver():
+    """The installable driver: host C++ over libpm_hip_rigid.so, exporting
+    only the four pm_driver* entry points, and its loader manifest
+    (config/driver-manifest.json.in's format; a relative library_path is
+    resolved against the manifest's directory)."""
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, "-std=c++17", "-O2", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall",
+           "-Wno-unused-command-line-argument", DRIVER_SRC, "-o", tmp, "-L" + OUT, "-lpm_hip_rigid",
+           "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("driver build failed")
+    os.replace(tmp, DRIVER)
+    with open(MANIFEST, "w") as f:
+        f.write('{\n  "file_format_version": "1.0.0",\n  "PMD": {\n'
+                '    "library_path": "libpm_hip_driver.so",\n    "api_version": "0.3.0",\n'
+                '    "library_arch": "64"\n  }\n}\n')
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -946,6 +946,13 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     return pmh::status_result(c.status);
 }
 
+PmhResult pmhWorldSetGravity(PmhWorld world, const float gravity[3]) {
+    if (!world || !gravity || !std::isfinite(gravity[0]) || !std::isfinite(gravity[1]) || !std::isfinite(gravity[2]))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    W(world)->gravity = {gravity[0], gravity[1], gravity[2]};
+    return PMH_SUCCESS;
+}
+
 PmhResult pmhWorldSetColorWaitLimit(PmhWorld world, uint32_t polls) {
     if (!world) return PMH_ERROR_INVALID_ARGUMENT;
     W(world)->color_wait_limit = polls;
