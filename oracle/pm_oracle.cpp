@@ -1617,6 +1617,10 @@ struct Ctx {
     float last_residual = 0.0f;
     uint32_t executed[2] = {0u, 0u};     // iterations run per phase
     std::vector<uint8_t> stick{};        // per point of the last dual update
+    // Mistakes planted by tests to show that a pin discriminates (never set
+    // otherwise): bit 0 skips the free-motion mean, bit 1 the crossing
+    // safeguard.
+    uint32_t planted = 0u;
 };
 
 inline bool dyn(const Ctx &x, uint32_t i) { return x.P[i].motion == kDynamic; }
@@ -2172,7 +2176,9 @@ void solve_body(Ctx &x, uint32_t i, uint32_t phase, std::vector<PointRef> &pts) 
     Vec6 update = zero6();
     if (!block_solve(blk, update)) return;
     bool crossing = false;
-    if (x.lanes == 0u) {
+    if (x.planted & 2u) {
+        // planted mistake: no crossing safeguard
+    } else if (x.lanes == 0u) {
         for (auto &r : pts) crossing = crossing_row(x, i, r, update, blk) || crossing;
     } else {
         std::vector<Block> parts(x.lanes, zero_block());
@@ -2521,7 +2527,7 @@ int prepare_solve(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t request
             }
         }
         x.executed[phase] = executed;
-        free_motion(x, phase);
+        if (!(x.planted & 1u)) free_motion(x, phase);
         apply_states(x, phase, realtime);
         if (phase == 0u) project_hinges(x);
         x.S->counts[phase == 0u ? 6u : 7u] = executed;
@@ -2896,6 +2902,7 @@ void world_step(World &w) {
           uint32_t(w.pairs.size()), w.bodies.data(), &w.schedule, dt, w.desc.lanes};
     x.J = w.joints.data();
     x.nj = uint32_t(w.joints.size());
+    x.planted = (w.desc.flags >> 1) & 3u;
     std::fill(w.residual.begin(), w.residual.end(), std::nanf(""));
     std::fill(w.residual_max.begin(), w.residual_max.end(), std::nanf(""));
     x.residual = w.residual.data();

@@ -203,7 +203,8 @@ typedef struct PmoWorldDesc {
     uint32_t requested_iterations;
     uint32_t body_capacity;        /* 0: the body count at first step */
     uint32_t lanes;                /* accumulation order, see above */
-    uint32_t flags;                /* bit 0: brute-force broadphase */
+    uint32_t flags;                /* bit 0: brute-force broadphase; bits 1-2: mistakes planted by tests
+                                      (1: no free-motion mean, 2: no crossing safeguard) */
     float residual_tolerance;      /* > 0: iterate each phase until the constraint residual
                                       is at most this (the budget caps); 0: fixed budget */
     uint32_t substeps;             /* substeps per step of dt / substeps (0 = 1), as

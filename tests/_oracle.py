@@ -126,10 +126,12 @@ class OracleWorld:
 
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0,
-                 body_capacity=0, lanes=0, brute_force=False, residual_tolerance=0.0, substeps=1):
+                 body_capacity=0, lanes=0, brute_force=False, residual_tolerance=0.0, substeps=1, planted=0):
+        """planted: mistakes planted by tests (1: no free-motion mean,
+        2: no crossing safeguard)."""
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, lanes,
-                         1 if brute_force else 0, residual_tolerance, substeps)
+                         (1 if brute_force else 0) | (planted << 1), residual_tolerance, substeps)
         self._h = lib().pmo_world_create(ctypes.byref(desc))
         if not self._h:
             raise ValueError("invalid world description")
