@@ -15,7 +15,8 @@ W = 5, so steps 540..600 are timed).
 value    = AVBD body-iterations per second of wall time over the K steps
            (whole steps: predictor, broadphase, narrowphase, contact cache,
            prepare, solve);
-roofline = the dominant kernel, the colour sweep k_stage_color (one launch
+roofline = the dominant kernel, the dataflow colour sweep k_sweep (one
+           launch per iteration; --per-color-sweep: k_stage_color, one launch
            per colour per iteration): its algorithmic bytes per launch over
            its average launch duration, both from the launches of every
            4th timed step, each bracketed by HIP events on the world's
@@ -48,17 +49,18 @@ KERNEL_TIMING_STRIDE = 4
 RUN_STEPS = 600   # the pile workload's run length
 
 
-def traffic_from_profiles(window):
+def traffic_from_profiles(window, kernel):
     """HBM bytes per colour-sweep launch measured by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration) over the
-    same timed window of the same command, when recorded; None otherwise."""
+    same timed window of the same command and kernel, when recorded; None
+    otherwise."""
     path = os.path.join(ROOT, "profiles", "color_sweep_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if list(rec.get("window", [])) != list(window):
+    if list(rec.get("window", [])) != list(window) or rec.get("kernel") != kernel:
         return None
     return rec.get("bytes_per_launch")
 
@@ -196,6 +198,8 @@ def main():
                     help="warmstart workload: also run the predicted-lambda mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
+    ap.add_argument("--per-color-sweep", action="store_true",
+                    help="one colour-sweep launch per colour per iteration instead of the dataflow sweep")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body in the colour sweep (8, 16, 32, 64; 0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -245,6 +249,7 @@ def main():
              "batch": dict(pairs_per_body=40, points_per_body=40)}.get(args.workload, {})
     world = scene.build(pmamd.World(profile=args.profile, base_iterations=args.iterations,
                                     cooperative=args.cooperative, solve_lanes=args.lanes,
+                                    per_color_sweep=args.per_color_sweep,
                                     kernel_timing_stride=KERNEL_TIMING_STRIDE, **pools))
     print(f"# bench: {scene.n} bodies, warmup {args.warmup}, timed {args.steps}", file=sys.stderr, flush=True)
     tuned = None
@@ -301,11 +306,15 @@ def main():
         per_launch_bytes = kt["algorithmic_bytes"] / kt["launches"]
         per_launch_s = kt["milliseconds"] / 1e3 / kt["launches"]
         lanes_used = args.lanes or (tuned[0] if tuned else 32)
-        kernel = (f"k_stage_color<{lanes_used}, {'true' if scene.joints is not None else 'false'}> "
-                  f"(colour sweep, one launch per colour per iteration)")
+        joints = 'true' if scene.joints is not None else 'false'
+        if args.per_color_sweep:
+            kernel = f"k_stage_color<{lanes_used}, {joints}> (colour sweep, one launch per colour per iteration)"
+        else:
+            kernel = f"k_sweep<{lanes_used}, {joints}> (dataflow colour sweep, one launch per iteration)"
     achieved = per_launch_bytes / per_launch_s / 1e9
     window = (settle, settle + args.steps)
-    traffic = traffic_from_profiles(window) if args.workload == "pile" and args.side == 32 else None
+    traffic = (traffic_from_profiles(window, kernel.split(" ")[0])
+               if args.workload == "pile" and args.side == 32 else None)
     name, cus = pmamd.device_info()
     line = {
         "metric": METRIC,

@@ -166,6 +166,9 @@ typedef struct PmhWorldDesc {
 #define PMH_WORLD_PHASE_TIMING 2u       /* HIP events around each phase and the solve of every step
                                            (pmhWorldGetPhaseTimes, pmhWorldSolveTiming, and the per-launch
                                            colour-sweep timing of kernel_timing_stride); off by default */
+#define PMH_WORLD_PER_COLOR_SWEEP 4u    /* one launch per colour per iteration instead of the dataflow
+                                           sweep (one launch per iteration, each body starting once its
+                                           lower-colour neighbours are done); same results */
 
 typedef struct PmhWorldStats {
     uint32_t pairs, manifolds, points, colors, islands, impact, budget,

@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <thread>
 
 namespace cg = cooperative_groups;
@@ -40,12 +41,39 @@ constexpr float kRealtimeDamping = 0.05f;
 constexpr float kSurfaceTolerance = 1.0e-5f;
 constexpr float kRestOffset = 1.0e-3f;
 constexpr uint32_t kSideFirst = 0x80000000u;
+// Colour-ordered entry, second word: the other body, and this flag when it
+// is dynamic and of a lower colour (the dataflow sweep waits for it).
+constexpr uint32_t kEntryLower = 0x80000000u;
+constexpr uint32_t kEntryBody = 0x7fffffffu;
 
 __device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Agent-coherent 64-bit words: relaxed agent-scope atomics, issued as
+// global_load/global_store with the sc1 bit.  The dataflow sweep hands
+// displacements between the wavefronts of one launch with them: inside a
+// kernel the L2 caches of the eight XCDs are not coherent with each other
+// for ordinary accesses.
+__device__ __forceinline__ uint64_t ld64_co(const float *p) {
+    return __hip_atomic_load(reinterpret_cast<const uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st64_co(float *p, float x, float y) {
+    const uint64_t v = uint64_t(__float_as_uint(x)) | (uint64_t(__float_as_uint(y)) << 32);
+    __hip_atomic_store(reinterpret_cast<uint64_t *>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float lo_f(uint64_t v) { return __uint_as_float(uint32_t(v)); }
+__device__ __forceinline__ float hi_f(uint64_t v) { return __uint_as_float(uint32_t(v >> 32)); }
+__device__ __forceinline__ float4 ld4_co(const float4 *p) {
+    const uint64_t a = ld64_co(&p->x), b = ld64_co(&p->z);
+    return make_float4(lo_f(a), hi_f(a), lo_f(b), hi_f(b));
+}
+__device__ __forceinline__ void st4_co(float4 *p, float4 v) {
+    st64_co(&p->x, v.x, v.y);
+    st64_co(&p->z, v.z, v.w);
 }
 
 // Sum over a converged wavefront (every lane active).
@@ -704,13 +732,22 @@ __global__ void k_color_count(uint32_t n, const uint32_t *color, uint32_t *count
     }
 }
 
-__global__ void k_color_scan(uint32_t *count, uint32_t *start, const StepCounters *ctr) {
+// Colour starts at multiples of `groups` (bodies per wavefront in the
+// sweep), the gaps filled with kInvalid: a wavefront never holds two
+// colours.
+__global__ void k_color_scan(uint32_t *count, uint32_t *start, const StepCounters *ctr, uint32_t *bodies,
+                             uint32_t groups) {
     if (threadIdx.x != 0) return;
     uint32_t s = 0;
     const uint32_t C = ctr->colors;
     for (uint32_t c = 0; c <= C && c <= kMaxColors; ++c) {
         start[c] = s;
-        if (c < C) { s += count[c]; count[c] = 0; }
+        if (c < C) {
+            const uint32_t m = count[c], padded = (m + groups - 1) / groups * groups;
+            for (uint32_t k = s + m; k < s + padded; ++k) bodies[k] = kInvalid;
+            s += padded;
+            count[c] = 0;
+        }
     }
 }
 
@@ -744,8 +781,8 @@ __global__ void k_bp_count(uint32_t n, const StepCounters *ctr, const uint32_t *
     const uint32_t D = start[ctr->colors];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    if (k >= D) { bp_count[k] = 0; return; }
-    const uint32_t i = bodies[k];
+    const uint32_t i = k < D ? bodies[k] : kInvalid;
+    if (i == kInvalid) { bp_count[k] = 0; return; }
     uint32_t s = 0;
     for (uint32_t e = adj_start[i]; e < adj_start[i + 1]; ++e) s += m_count[adj[e]];
     bp_count[k] = s;
@@ -754,20 +791,24 @@ __global__ void k_bp_count(uint32_t n, const StepCounters *ctr, const uint32_t *
 __global__ void k_bp_fill(const StepCounters *ctr, const uint32_t *start, const uint32_t *bodies,
                           const uint32_t *adj_start, const uint32_t *adj, const uint32_t *m_count,
                           const uint32_t *m_start, const uint2 *ab, const uint32_t *bp_start, uint2 *entries,
-                          uint32_t *c_entry) {
+                          uint32_t *c_entry, const uint32_t *color) {
     const uint32_t D = start[ctr->colors];
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= D) return;
     const uint32_t i = bodies[k];
+    if (i == kInvalid) return;
+    const uint32_t ci = color[i];
     uint32_t o = bp_start[k];
     for (uint32_t e = adj_start[i]; e < adj_start[i + 1]; ++e) {
         const uint32_t m = adj[e];
         const uint2 p = ab[m];
         const bool first = p.x == i;
         const uint32_t other = first ? p.y : p.x;
+        // Static bodies hold no colour (kInvalid): never lower.
+        const uint32_t tag = other | (color[other] < ci ? kEntryLower : 0u);
         for (uint32_t q = 0; q < m_count[m]; ++q) {
             c_entry[2 * (m_start[m] + q) + (first ? 0u : 1u)] = o;
-            entries[o++] = make_uint2((m_start[m] + q) | (first ? kSideFirst : 0u), other);
+            entries[o++] = make_uint2((m_start[m] + q) | (first ? kSideFirst : 0u), tag);
         }
     }
 }
@@ -868,11 +909,48 @@ struct SolveArgs {
     uint32_t nj;
     const uint32_t *jstart, *jadj;       // per-body incident joints, descending index
     const PmhRigidBodyState *prev;
+    // dataflow sweep: colours, published stamps, this sweep's stamp (0: not
+    // in a dataflow sweep) and the poll limit of its waits
+    const uint32_t *color;
+    uint32_t *done;
+    uint32_t flow_stamp, flow_limit;
 };
 
 __device__ __forceinline__ void load_disp(const float4 *disp, uint32_t i, float *d) {
     const float4 a = disp[2 * i], b = disp[2 * i + 1];
     d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = b.x; d[4] = b.y; d[5] = b.z;
+}
+// Another body's displacement inside a dataflow sweep (agent-coherent).
+__device__ __forceinline__ void load_disp_co(const float4 *disp, uint32_t i, float *d) {
+    const float *p = &disp[2 * i].x;
+    const uint64_t t = ld64_co(p), tz = ld64_co(p + 2), r = ld64_co(p + 4), rz = ld64_co(p + 6);
+    d[0] = lo_f(t); d[1] = hi_f(t); d[2] = lo_f(tz); d[3] = lo_f(r); d[4] = hi_f(r); d[5] = lo_f(rz);
+}
+__device__ __forceinline__ void load_disp_a(const SolveArgs &a, uint32_t i, float *d) {
+    if (a.flow_stamp) load_disp_co(a.disp, i, d);
+    else load_disp(a.disp, i, d);
+}
+
+// Dataflow sweep: waits until body j has published this sweep's
+// displacement.  Bounded: a wait that runs out raises the abort flag, which
+// ends every other wait of the step, and marks the status, so the step
+// fails instead of hanging.
+__device__ __forceinline__ void wait_published(const SolveArgs &a, uint32_t j) {
+    for (uint32_t polls = 0; int32_t(ld_relaxed(&a.done[j]) - a.flow_stamp) < 0; ++polls) {
+        if (polls >= a.flow_limit) {
+            st_relaxed(&a.ctr->color_abort, 1u);
+            atomicOr(&a.ctr->status, kStatusColorTimeout);
+        }
+        if (ld_relaxed(&a.ctr->color_abort) != 0u) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    asm volatile("" ::: "memory");   // the displacement reads stay after the wait
+}
+// Dataflow sweep: body i's displacement stores (agent-coherent) have
+// completed before the stamp that releases its waiters is stored.
+__device__ __forceinline__ void publish_done(const SolveArgs &a, uint32_t i) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    st_relaxed(&a.done[i], a.flow_stamp);
 }
 
 // err = e0 + dot6(JA, dA) + dot6(JB, dB) with JA = (-axis, angA),
@@ -898,167 +976,208 @@ __device__ __forceinline__ void group_reduce(Block &b) {
     }
 }
 
-// apply_pose_update, geometry_constraints.slang:386-408.
-__device__ __forceinline__ void apply_pose_update(float4 *disp, float4 *dqa, uint32_t i, const float *u) {
-    float4 t = disp[2 * i], r = disp[2 * i + 1];
+// apply_pose_update, geometry_constraints.slang:386-408: the translation
+// always, the rotation delta and its quaternion only for a turn (returns
+// whether it turned).
+__device__ __forceinline__ bool pose_update(float4 &t, float4 &r, float4 &dqv, const float *u) {
     t.x += u[0]; t.y += u[1]; t.z += u[2];
-    disp[2 * i] = t;
     const V3 rot = {u[3], u[4], u[5]};
-    if (rot.x == 0.0f && rot.y == 0.0f && rot.z == 0.0f) return;
-    Q rel = q4(dqa[i]);
+    if (rot.x == 0.0f && rot.y == 0.0f && rot.z == 0.0f) return false;
+    Q rel = q4(dqv);
     const Q der = qmul({rot.x, rot.y, rot.z, 0.0f}, rel);
     rel = qnormalize({rel.x + 0.5f * der.x, rel.y + 0.5f * der.y, rel.z + 0.5f * der.z, rel.w + 0.5f * der.w});
     if (rel.w < 0.0f) rel = {-rel.x, -rel.y, -rel.z, -rel.w};
-    dqa[i] = f4q(rel);
+    dqv = f4q(rel);
     r.x = 2.0f * rel.x; r.y = 2.0f * rel.y; r.z = 2.0f * rel.z;
-    disp[2 * i + 1] = r;
+    return true;
 }
 
-// One colour sweep for the bodies of `color` (solve_color,
-// geometry_constraints_solve.slang:331-491).  `gw`/`nw` index the
-// wavefront within the launch.
+// The bodies at colour-ordered positions k = base + group of one wavefront
+// (solve_color, geometry_constraints_solve.slang:331-491).  Each body is
+// owned by L lanes: every lane streams every L-th of the body's contact
+// points, the partial blocks meet in a butterfly, the LDLT solve and the
+// crossing safeguard run on the whole group.
 // kJoints: the world has joints.  Joint-free worlds run the false
 // instance, which carries none of the joint code (registers, stack).
-template <int L, bool kJoints>
-__device__ void solve_color(const SolveArgs &a, uint32_t begin, uint32_t end, uint32_t phase, uint32_t gw,
-                            uint32_t nw) {
+// kFlow: inside a dataflow sweep (one launch for every colour): a body
+// waits for the lower-colour neighbours it reads, reads other bodies'
+// displacements agent-coherently, and publishes its own with a stamp.
+template <int L, bool kJoints, bool kFlow>
+__device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, uint32_t end, uint32_t phase) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t grp = lane / L, sub_lane = lane % L;
-    constexpr uint32_t groups = 64 / L;
-    for (uint32_t base = begin + gw * groups; base < end; base += nw * groups) {
-        const uint32_t k = base + grp;
-        bool active = k < end;
-        uint32_t i = 0;
-        if (active) {
-            i = a.color_bodies[k];
-            if (phase != 0 && a.impact[i] == 0) active = false;
+    const uint32_t k = base + grp;
+    bool active = k < end;
+    uint32_t i = 0;
+    if (active) {
+        i = a.color_bodies[k];
+        if (i == kInvalid || (phase != 0 && a.impact[i] == 0)) active = false;
+    }
+    Block blk;
+    block_zero(blk);
+    float dI[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t eb = 0, ee = 0;
+    if (active) {
+        load_disp(a.disp, i, dI);
+        eb = a.bp_start[k];
+        ee = a.bp_start[k + 1];
+        if (sub_lane == 0) {
+            const float4 i0 = a.inertia[2 * i], i1 = a.inertia[2 * i + 1];
+            // Full packed block of the body: diag(m/dt^2) and the
+            // rotational lower triangle; the rest is zero.
+            float h[21];
+#pragma unroll
+            for (int q = 0; q < 21; ++q) h[q] = 0.0f;
+            h[0] = i0.x; h[2] = i0.x; h[5] = i0.x;
+            h[9] = i0.y; h[13] = i0.z; h[14] = i0.w; h[18] = i1.x; h[19] = i1.y; h[20] = i1.z;
+#pragma unroll
+            for (int q = 0; q < 21; ++q) blk.h[q] = h[q];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    blk.g[r] += h[r > c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r] * dI[c];
         }
-        Block blk;
-        block_zero(blk);
-        float dI[6] = {0, 0, 0, 0, 0, 0};
-        uint32_t eb = 0, ee = 0;
-        if (active) {
-            load_disp(a.disp, i, dI);
-            eb = a.bp_start[k];
-            ee = a.bp_start[k + 1];
-            if (sub_lane == 0) {
-                const float4 i0 = a.inertia[2 * i], i1 = a.inertia[2 * i + 1];
-                // Full packed block of the body: diag(m/dt^2) and the
-                // rotational lower triangle; the rest is zero.
-                float h[21];
+    }
+    for (uint32_t e = eb + sub_lane; e < ee; e += L) {
+        const uint2 ent = a.entries[e];
+        const uint32_t c = ent.x & 0x7fffffffu;
+        const bool first = (ent.x & kSideFirst) != 0u;
+        const uint32_t other = ent.y & kEntryBody;
+        const float4 lam = a.c_lam[c], pen = a.c_pen[c];
+        float4 Jx[9];
 #pragma unroll
-                for (int q = 0; q < 21; ++q) h[q] = 0.0f;
-                h[0] = i0.x; h[2] = i0.x; h[5] = i0.x;
-                h[9] = i0.y; h[13] = i0.z; h[14] = i0.w; h[18] = i1.x; h[19] = i1.y; h[20] = i1.z;
-#pragma unroll
-                for (int q = 0; q < 21; ++q) blk.h[q] = h[q];
-#pragma unroll
-                for (int r = 0; r < 6; ++r)
-#pragma unroll
-                    for (int c = 0; c < 6; ++c)
-                        blk.g[r] += h[r > c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r] * dI[c];
-            }
+        for (int q = 0; q < 9; ++q) Jx[q] = a.erec[q * a.ecap + e];
+        float dO[6];
+        if (kFlow) {
+            if ((ent.y & kEntryLower) != 0u && (phase == 0 || a.impact[other] != 0u)) wait_published(a, other);
+            load_disp_co(a.disp, other, dO);
+        } else {
+            load_disp(a.disp, other, dO);
         }
+        const float *dA = first ? dI : dO;
+        const float *dB = first ? dO : dI;
+        const float err0 = row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB);
+        const float err1 = row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB);
+        const float err2 = row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB);
+        const Force3 trial = {lam.x + pen.x * err0, lam.y + pen.y * err1, lam.z + pen.z * err2};
+        const float mu = Jx[6].w;
+        const Force3 f = project_contact(trial, mu);
+        const Force3 sc = stiffness_scales(trial, mu);
+        const float stiff[3] = {pen.x * sc.n, pen.y * sc.t0, pen.z * sc.t1};
+        const float force[3] = {f.n - Jx[3].w, f.t0 - Jx[4].w, f.t1 - Jx[5].w};
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const float4 axis = Jx[ax];
+            const float4 ang = first ? Jx[3 + ax] : Jx[6 + ax];
+            const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
+                                  first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
+            add_row(blk, row, stiff[ax], force[ax]);
+        }
+    }
+    // Joint rows follow the contact rows (solve.slang:401-430), each
+    // joint in one lane's partial block.
+    if (kJoints && active) body_joint_rows(a, i, phase, blk, sub_lane, L);
+    group_reduce<L>(blk);
+    float update[6];
+    const bool solved = active && block_solve(blk, update);
+    // Crossing safeguard (solve.slang:433-475).
+    Block cross_blk;
+    block_zero(cross_blk);
+    bool crossing = false;
+    if (solved) {
         for (uint32_t e = eb + sub_lane; e < ee; e += L) {
             const uint2 ent = a.entries[e];
             const uint32_t c = ent.x & 0x7fffffffu;
             const bool first = (ent.x & kSideFirst) != 0u;
+            const uint32_t other = ent.y & kEntryBody;
             float dO[6];
-            load_disp(a.disp, ent.y, dO);
+            if (kFlow) load_disp_co(a.disp, other, dO);
+            else load_disp(a.disp, other, dO);
             const float *dA = first ? dI : dO;
             const float *dB = first ? dO : dI;
-            const float4 lam = a.c_lam[c], pen = a.c_pen[c];
-            float4 Jx[9];
-#pragma unroll
-            for (int q = 0; q < 9; ++q) Jx[q] = a.erec[q * a.ecap + e];
-            const float err0 = row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB);
-            const float err1 = row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB);
-            const float err2 = row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB);
-            const Force3 trial = {lam.x + pen.x * err0, lam.y + pen.y * err1, lam.z + pen.z * err2};
-            const float mu = Jx[6].w;
-            const Force3 f = project_contact(trial, mu);
-            const Force3 sc = stiffness_scales(trial, mu);
-            const float stiff[3] = {pen.x * sc.n, pen.y * sc.t0, pen.z * sc.t1};
-            const float force[3] = {f.n - Jx[3].w, f.t0 - Jx[4].w, f.t1 - Jx[5].w};
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                const float4 axis = Jx[ax];
-                const float4 ang = first ? Jx[3 + ax] : Jx[6 + ax];
-                const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
-                                      first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
-                add_row(blk, row, stiff[ax], force[ax]);
-            }
-        }
-        // Joint rows follow the contact rows (solve.slang:401-430), each
-        // joint in one lane's partial block.
-        if (kJoints && active) body_joint_rows(a, i, phase, blk, sub_lane, L);
-        group_reduce<L>(blk);
-        float update[6];
-        const bool solved = active && block_solve(blk, update);
-        // Crossing safeguard (solve.slang:433-475).
-        Block cross_blk;
-        block_zero(cross_blk);
-        bool crossing = false;
-        if (solved) {
-            for (uint32_t e = eb + sub_lane; e < ee; e += L) {
-                const uint2 ent = a.entries[e];
-                const uint32_t c = ent.x & 0x7fffffffu;
-                const bool first = (ent.x & kSideFirst) != 0u;
-                float dO[6];
-                load_disp(a.disp, ent.y, dO);
-                const float *dA = first ? dI : dO;
-                const float *dB = first ? dO : dI;
-                const float4 axis = a.erec[e], angA = a.erec[3 * a.ecap + e], angB = a.erec[6 * a.ecap + e];
-                const float err = row_error(axis.w, axis, angA, angB, dA, dB);
-                const float lam0 = a.c_lam[c].x, k0 = a.c_pen[c].x;
-                const float4 ang = first ? angA : angB;
-                const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
-                                      first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
-                const float trial = lam0 + k0 * err;
-                if (trial <= 0.0f && trial + k0 * dot6(row, update) > k0 * 1.0e-6f) {
-                    add_row(cross_blk, row, k0, trial);
-                    crossing = true;
-                }
-            }
-        }
-        // Group-wide "any crossing" (uniform across the group's lanes).
-        uint64_t bal = __ballot(crossing);
-        const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << L) - 1ull)) << (grp * L);
-        const bool group_crossing = (bal & gmask) != 0ull;
-        if (bal != 0ull) {
-            group_reduce<L>(cross_blk);
-            if (group_crossing && solved) {
-#pragma unroll
-                for (int q = 0; q < 21; ++q) blk.h[q] += cross_blk.h[q];
-#pragma unroll
-                for (int q = 0; q < 6; ++q) blk.g[q] += cross_blk.g[q];
-                float safe[6];
-                if (block_solve(blk, safe))
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) update[q] = safe[q];
-            }
-        }
-        if (solved && sub_lane == 0) {
-            if (phase == 0) {
-                apply_pose_update(a.disp, a.dq, i, update);
-            } else {
-                float4 t = a.disp[2 * i], r = a.disp[2 * i + 1];
-                t.x += update[0]; t.y += update[1]; t.z += update[2];
-                r.x += update[3]; r.y += update[4]; r.z += update[5];
-                a.disp[2 * i] = t;
-                a.disp[2 * i + 1] = r;
+            const float4 axis = a.erec[e], angA = a.erec[3 * a.ecap + e], angB = a.erec[6 * a.ecap + e];
+            const float err = row_error(axis.w, axis, angA, angB, dA, dB);
+            const float lam0 = a.c_lam[c].x, k0 = a.c_pen[c].x;
+            const float4 ang = first ? angA : angB;
+            const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
+                                  first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
+            const float trial = lam0 + k0 * err;
+            if (trial <= 0.0f && trial + k0 * dot6(row, update) > k0 * 1.0e-6f) {
+                add_row(cross_blk, row, k0, trial);
+                crossing = true;
             }
         }
     }
+    // Group-wide "any crossing" (uniform across the group's lanes).
+    uint64_t bal = __ballot(crossing);
+    const uint64_t gmask = (L == 64 ? ~0ull : ((1ull << L) - 1ull)) << (grp * L);
+    const bool group_crossing = (bal & gmask) != 0ull;
+    if (bal != 0ull) {
+        group_reduce<L>(cross_blk);
+        if (group_crossing && solved) {
+#pragma unroll
+            for (int q = 0; q < 21; ++q) blk.h[q] += cross_blk.h[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) blk.g[q] += cross_blk.g[q];
+            float safe[6];
+            if (block_solve(blk, safe))
+#pragma unroll
+                for (int q = 0; q < 6; ++q) update[q] = safe[q];
+        }
+    }
+    if (active && sub_lane == 0) {
+        if (solved) {
+            float4 t = a.disp[2 * i], r = a.disp[2 * i + 1];
+            if (phase == 0) {
+                float4 q = a.dq[i];
+                const bool turned = pose_update(t, r, q, update);
+                if (kFlow) {
+                    st4_co(&a.disp[2 * i], t);
+                    if (turned) {
+                        st4_co(&a.dq[i], q);
+                        st4_co(&a.disp[2 * i + 1], r);
+                    }
+                } else {
+                    a.disp[2 * i] = t;
+                    if (turned) {
+                        a.dq[i] = q;
+                        a.disp[2 * i + 1] = r;
+                    }
+                }
+            } else {
+                t.x += update[0]; t.y += update[1]; t.z += update[2];
+                r.x += update[3]; r.y += update[4]; r.z += update[5];
+                if (kFlow) {
+                    st4_co(&a.disp[2 * i], t);
+                    st4_co(&a.disp[2 * i + 1], r);
+                } else {
+                    a.disp[2 * i] = t;
+                    a.disp[2 * i + 1] = r;
+                }
+            }
+        }
+        if (kFlow) publish_done(a, i);
+    }
+}
+
+// One colour: the positions [begin, end), `gw`/`nw` indexing the wavefront
+// within the launch.
+template <int L, bool kJoints>
+__device__ void solve_color(const SolveArgs &a, uint32_t begin, uint32_t end, uint32_t phase, uint32_t gw,
+                            uint32_t nw) {
+    constexpr uint32_t groups = 64 / L;
+    for (uint32_t base = begin + gw * groups; base < end; base += nw * groups)
+        solve_group<L, kJoints, false>(a, base, end, phase);
 }
 
 __device__ __forceinline__ PmhRigidBodyState avbd_pose(const SolveArgs &a, uint32_t i) {
     PmhRigidBodyState s;
     const V3 t = v3(a.tpos[i]);
-    const float4 d = a.disp[2 * i];
+    const float4 d = a.flow_stamp ? ld4_co(&a.disp[2 * i]) : a.disp[2 * i];
     s.position = pv(add(t, {d.x, d.y, d.z}));
     Q o = q4(a.tq[i]);
-    const Q dq = q4(a.dq[i]);
+    const Q dq = q4(a.flow_stamp ? ld4_co(&a.dq[i]) : a.dq[i]);
     if (dq.x != 0.0f || dq.y != 0.0f || dq.z != 0.0f) o = qnormalize(qmul(dq, o));
     s.orientation = pq(o);
     return s;
@@ -1145,8 +1264,8 @@ __device__ void visit_joint_row(const SolveArgs &a, PmhRigidConstraint &j, Block
     PmhAvbdRow row = j.rows[ri];
     const uint32_t ia = j.dense_a, ib = j.dense_b;
     float da[6], db[6];
-    load_disp(a.disp, ia, da);
-    load_disp(a.disp, ib, db);
+    load_disp_a(a, ia, da);
+    load_disp_a(a, ib, db);
     const float linearized = dot6(r.ja, da) + dot6(r.jb, db);
 #pragma unroll
     for (int k = 0; k < 6; ++k) { row.a[k] = r.ja[k]; row.b[k] = r.jb[k]; }
@@ -1355,13 +1474,17 @@ __device__ void body_joint_rows(const SolveArgs &a, uint32_t i, uint32_t phase, 
         if (j.state_enabled == 0u) continue;
         if (pos++ % L != sub_lane) continue;
         const bool first = i == j.dense_a;
+        if (a.flow_stamp) {
+            const uint32_t other = first ? j.dense_b : j.dense_a;
+            if (a.color[other] < a.color[i] && (phase == 0u || a.impact[other] != 0u)) wait_published(a, other);
+        }
         if (phase == 0u) {
             visit_joint(a, j, blk, first, kVisitAccumulate);
             continue;
         }
         float da[6], db[6];
-        load_disp(a.disp, j.dense_a, da);
-        load_disp(a.disp, j.dense_b, db);
+        load_disp_a(a, j.dense_a, da);
+        load_disp_a(a, j.dense_b, db);
         for (uint32_t r = 0; r < j.avbd_count; ++r) {
             const PmhAvbdRow row = j.rows[r];
             if (row.stiffness >= kMaxPenalty) jrow_accumulate(blk, row, first, da, db, a.dt);
@@ -1782,6 +1905,31 @@ __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t begin
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     solve_color<L, kJoints>(a, begin, end, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
+// Dataflow sweep: every colour in one launch per iteration.  Wavefronts
+// take tickets of one wavefront's worth of colour-ordered positions, in
+// order (a colour starts at a multiple of that, so a wavefront holds one
+// colour).  A body waits only for lower-colour neighbours: their positions
+// were ticketed earlier, to wavefronts that are running, so every wait ends
+// (and each is bounded besides).  Colour c + 1 starts wherever colour c is
+// done, not after all of colour c.
+template <int L, bool kJoints>
+__global__ void __launch_bounds__(256) k_sweep(SolveArgs a, uint32_t phase) {
+    uint32_t *tickets = a.ctr->sweep_ticket;
+    // The other counter is the next sweep's: it starts at zero.
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&tickets[(a.flow_stamp + 1u) & 1u], 0u);
+    if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
+    constexpr uint32_t groups = 64 / L;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t end = a.color_start[a.ctr->colors];
+    uint32_t *ticket = &tickets[a.flow_stamp & 1u];
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(ticket, 1u);
+        t = __shfl(t, 0, 64);
+        if (t >= (end + groups - 1) / groups) break;
+        solve_group<L, kJoints, true>(a, t * groups, end, phase);
+    }
+}
 template <bool kJoints>
 __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase, uint32_t it) {
     if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
@@ -2113,6 +2261,7 @@ __global__ void k_schedule_init(PmhContactSchedule *s, StepCounters *ctr, uint32
     ctr->islands = 0;
     ctr->impact = 0;
     ctr->color_ticket = 0;
+    ctr->sweep_ticket[0] = ctr->sweep_ticket[1] = 0;
     ctr->dyn_bodies = ctr->impact_bodies = 0;
     ctr->mean_roots = 0;
     ctr->joint_budget = 0;
@@ -2202,16 +2351,23 @@ PmhResult Solver::reserve_joints(uint32_t count) {
 
 PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     PmhResult r = PMH_SUCCESS;
+    // Colour-ordered positions: the bodies and at most 7 gap slots per
+    // colour (8 bodies per wavefront at 8 lanes each).
+    const uint32_t positions = bodies + kMaxColors * 7u;
 #define R(buf, cnt) if ((r = buf.reserve(cnt)) != PMH_SUCCESS) return r
     R(tpos, bodies); R(tq, bodies); R(tv, bodies); R(tw, bodies); R(disp, 2 * (size_t)bodies);
     R(dq, bodies); R(inertia, 2 * (size_t)bodies); R(color, bodies); R(parent, bodies);
-    R(impact, bodies); R(deg, bodies + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
-    R(color_bodies, bodies); R(bp_start, bodies + 1); R(anchored, bodies); R(mean, bodies);
+    R(impact, bodies); R(deg, positions + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
+    R(color_bodies, positions); R(bp_start, positions + 1); R(anchored, bodies); R(mean, bodies);
     R(members, bodies); R(mean_list, bodies);
     R(jdeg, bodies + 1); R(jstart, bodies + 1); R(jcursor, bodies); R(joint_head, bodies);
     if (prev_vel.n < bodies) {
         R(prev_vel, bodies);
         PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
+    }
+    if (done.n < bodies) {
+        R(done, bodies);
+        PMH_CHECK(hipMemset(done.p, 0, sizeof(uint32_t) * bodies));
     }
     R(code, pairs); R(ab, pairs); R(m_count, pairs + 1); R(m_start, pairs + 1); R(m_slot, pairs + 1);
     R(m_impact, pairs); R(m_face, pairs); R(m_init_rel, pairs); R(adj, 2 * (size_t)pairs);
@@ -2232,6 +2388,7 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     }
 #undef R
     n = bodies;
+    slots = positions;
     pair_cap = pairs;
     point_cap = points;
     return PMH_SUCCESS;
@@ -2241,7 +2398,7 @@ void Solver::release() {
     tpos.release(); tq.release(); tv.release(); tw.release(); disp.release(); dq.release();
     inertia.release(); prev_vel.release(); color.release(); parent.release(); impact.release();
     deg.release(); adj_start.release(); adj_cursor.release(); color_bodies.release();
-    bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release();
+    bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release(); done.release();
     jdeg.release(); jstart.release(); jcursor.release(); joint_head.release(); jadj.release(); code.release(); ab.release();
     m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
     m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
@@ -2258,6 +2415,20 @@ void Solver::release() {
 }
 
 static uint32_t g_coop_blocks = 0;
+
+// Resident blocks of a dataflow-sweep kernel over the device (at most 8 per
+// CU), found once per kernel.
+static uint32_t sweep_blocks(const void *kernel) {
+    static std::mutex lock;
+    static std::vector<std::pair<const void *, uint32_t>> known;
+    std::lock_guard<std::mutex> guard(lock);
+    for (const auto &k : known) if (k.first == kernel) return k.second;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const uint32_t blocks = uint32_t(std::min(per_cu, 8)) * uint32_t(device_compute_units());
+    known.emplace_back(kernel, blocks);
+    return blocks;
+}
 
 PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *P,
                              const PmhRigidBodyState *previous, PmhRigidBodyState *states,
@@ -2305,6 +2476,10 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     a.jstart = v.jstart.p;
     a.jadj = v.jadj.p;
     a.prev = previous;
+    a.color = v.color.p;
+    a.done = v.done.p;
+    a.flow_stamp = 0;
+    a.flow_limit = sp.sweep_wait_limit;
     a.log = v.residual.p;
     a.tol = sp.tolerance;
     a.c_flags = v.c_flags.p;
@@ -2378,18 +2553,19 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
     }
     PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
     k_color_count<<<bg, 256, 0, s>>>(n, v.color.p, v.color_count.p);
-    k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr);
+    k_color_scan<<<1, 64, 0, s>>>(v.color_count.p, v.color_start.p, ctr, v.color_bodies.p, 64u / sp.lanes);
     k_color_place<<<bg, 256, 0, s>>>(n, v.color.p, v.color_start.p, v.color_count.p, v.color_bodies.p);
     // colour-ordered point lists (deg array reused as the per-position count)
     // The degree array is free again once the adjacency is built (the
     // fill cursor keeps each body's degree): it holds the point counts.
-    k_bp_count<<<bg, 256, 0, s>>>(n, ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p,
+    const uint32_t sg = grid_for(v.slots, 256, 0xffffffffu);
+    k_bp_count<<<sg, 256, 0, s>>>(v.slots, ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p,
                                   v.m_count.p, v.deg.p);
-    r = exclusive_scan(s, v.deg.p, v.bp_start.p, n, nullptr, nullptr, v.scan_tmp);
+    r = exclusive_scan(s, v.deg.p, v.bp_start.p, v.slots, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
-    k_bp_fill<<<bg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
+    k_bp_fill<<<sg, 256, 0, s>>>(ctr, v.color_start.p, v.color_bodies.p, v.adj_start.p, v.adj.p, v.m_count.p,
                                  v.m_start.p, v.ab.p, v.bp_start.p, reinterpret_cast<uint2 *>(v.bp_entries.p),
-                                 v.c_entry.p);
+                                 v.c_entry.p, v.color.p);
     k_jacobians<<<cg_, 256, 0, s>>>(npts, P, v.c_pair.p, v.ab.p, v.tpos.p, v.tq.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
                                     v.c_ab.p, v.c_err.p, v.c_acc.p, v.jac.p, dt, 0, v.impact.p, v.erec.p, v.ecap,
                                     v.c_entry.p);
@@ -2471,6 +2647,29 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             if (start) return hipExtLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s, start, stop, 0);
             return hipLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s);
         };
+        // Dataflow sweep: one launch per iteration, as many resident
+        // wavefronts as fit (tickets hand out the positions).
+        const bool flow = sp.dataflow;
+        const uint32_t positions = v.host_color_start[colors];
+        auto launch_sweep = [&](SolveArgs &af, uint32_t phase, hipEvent_t start, hipEvent_t stop) -> hipError_t {
+            const void *kernel;
+            if (joints) {
+                kernel = lanes == 64   ? reinterpret_cast<const void *>(&k_sweep<64, true>)
+                         : lanes == 32 ? reinterpret_cast<const void *>(&k_sweep<32, true>)
+                         : lanes == 8  ? reinterpret_cast<const void *>(&k_sweep<8, true>)
+                                       : reinterpret_cast<const void *>(&k_sweep<16, true>);
+            } else {
+                kernel = lanes == 64   ? reinterpret_cast<const void *>(&k_sweep<64, false>)
+                         : lanes == 32 ? reinterpret_cast<const void *>(&k_sweep<32, false>)
+                         : lanes == 8  ? reinterpret_cast<const void *>(&k_sweep<8, false>)
+                                       : reinterpret_cast<const void *>(&k_sweep<16, false>);
+            }
+            const uint32_t waves = (positions + (64 / lanes) - 1) / (64 / lanes);
+            const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((waves + 3) / 4, sweep_blocks(kernel)));
+            void *args[] = {&af, &phase};
+            if (start) return hipExtLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s, start, stop, 0);
+            return hipLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s);
+        };
         uint32_t launches = 0;
         for (uint32_t phase = 0; phase < phases; ++phase) {
             if (phase != 0) {
@@ -2482,6 +2681,34 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                 launches += 4;
             }
             for (uint32_t it = 0; it < budget; ++it) {
+                if (flow) {
+                    SolveArgs af = a;
+                    if (++v.sweep_stamp == 0u) {   // stamps wrapped: no stale stamp may match
+                        PMH_CHECK(hipMemsetAsync(v.done.p, 0, sizeof(uint32_t) * n, s));
+                        v.sweep_stamp = 1u;
+                    }
+                    af.flow_stamp = v.sweep_stamp;
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    if (time_colors) {
+                        if (kt->events.size() < 2 * size_t(timed + 1)) {
+                            const size_t old = kt->events.size();
+                            kt->events.resize(2 * size_t(timed + 1) + 64, nullptr);
+                            for (size_t e = old; e < kt->events.size(); ++e) PMH_CHECK(hipEventCreate(&kt->events[e]));
+                        }
+                        e0 = kt->events[2 * timed];
+                        e1 = kt->events[2 * timed + 1];
+                        ++timed;
+                    }
+                    PMH_CHECK(launch_sweep(af, phase, e0, e1));
+                    if (joints) k_stage_duals<true><<<pts_blocks, 256, 0, s>>>(a, phase, it);
+                    else k_stage_duals<false><<<pts_blocks, 256, 0, s>>>(a, phase, it);
+                    launches += 2;
+                    if (sp.tolerance > 0.0f) {
+                        k_stage_check<<<1, 64, 0, s>>>(a, phase, it);
+                        launches += 1;
+                    }
+                    continue;
+                }
                 for (uint32_t c = 0; c < colors; ++c) {
                     const uint32_t bodies = v.host_color_start[c + 1] - v.host_color_start[c];
                     const uint32_t per_wave = 64 / lanes;

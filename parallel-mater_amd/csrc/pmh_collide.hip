@@ -2016,6 +2016,8 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     mark(4);
     SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes,
                    residual_tolerance, warm_infer && warm_enabled, color_wait_limit};
+    sp.sweep_wait_limit = color_wait_limit;
+    sp.dataflow = !per_color_sweep;
     SolveTiming st;
     st.kernels = &kernel_timer;
     if (timing_enabled) {

@@ -104,7 +104,8 @@ struct StepCounters {
     uint32_t joint_budget;     // largest joint iteration count (legacy budget floor)
     uint32_t cold_points;      // cold contact points listed for the learned warm start
     uint32_t log_count;        // contact samples logged so far (learned warm start)
-    uint32_t color_abort;      // set when a colour wait ran out: every other wait gives up
+    uint32_t color_abort;      // set when a bounded wait (colouring or sweep) ran out: every other wait gives up
+    uint32_t sweep_ticket[2];  // work tickets of the dataflow sweep, alternating per sweep
 };
 
 // Running totals over every step since the last reset.
@@ -255,6 +256,13 @@ struct Solver {
     // schedule
     DBuf<PmhContactSchedule> schedule;
     DBuf<uint32_t> color_count, color_start;
+    // Dataflow sweep: each colour's positions start at a multiple of the
+    // bodies per wavefront (64 / lanes), the gaps holding kInvalid, so one
+    // wavefront never holds two colours; `done[i]` is the stamp of the last
+    // sweep that published body i's displacement.
+    uint32_t slots = 0;                  // colour-ordered positions (bodies plus padding)
+    DBuf<uint32_t> done;
+    uint32_t sweep_stamp = 0;
     DBuf<StepCounters> counters;
     DBuf<SolveTotals> totals;
     DBuf<uint32_t> scan_tmp;
@@ -274,6 +282,8 @@ struct SolveParams {
     float tolerance = 0.0f;              // > 0: to-residual mode, the budget caps each phase
     bool warm_infer = false;             // predict cold points' normal duals
     uint32_t color_wait_limit = kColorWaitLimit;   // polls of one colour wait before it gives up
+    uint32_t sweep_wait_limit = kColorWaitLimit;   // polls of one dataflow-sweep wait before it gives up
+    bool dataflow = true;                // one dataflow sweep per iteration instead of one launch per colour
     bool reset_status = false;           // clear the status word first (the kernel seam; a world step clears it itself)
 };
 

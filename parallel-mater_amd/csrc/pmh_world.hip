@@ -701,6 +701,7 @@ PmhResult pmhCreateWorld(const PmhWorldDesc *desc, PmhWorld *out) {
     w->pairs_per_body = desc->pairs_per_body ? desc->pairs_per_body : 96;
     w->points_per_body = desc->points_per_body ? desc->points_per_body : 256;
     w->cooperative = (desc->flags & PMH_WORLD_COOPERATIVE_SOLVE) != 0;
+    w->per_color_sweep = (desc->flags & PMH_WORLD_PER_COLOR_SWEEP) != 0;
     if (desc->solve_lanes != 0 && desc->solve_lanes != 8 && desc->solve_lanes != 16 && desc->solve_lanes != 32 &&
         desc->solve_lanes != 64) {
         delete w;

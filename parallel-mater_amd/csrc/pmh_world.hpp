@@ -13,6 +13,7 @@ struct World {
     uint32_t profile = 1, base_iterations = 10, requested_iterations = 0;
     uint32_t body_capacity = 0, pairs_per_body = 96, points_per_body = 256;
     bool cooperative = false;
+    bool per_color_sweep = false;     // one launch per colour instead of the dataflow sweep
     uint32_t solve_lanes = kDefaultLanes;
     float residual_tolerance = 0.0f;
     uint32_t color_wait_limit = kColorWaitLimit;

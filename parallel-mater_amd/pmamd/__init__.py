@@ -29,6 +29,7 @@ RESULT_NAMES = {0: "PM_SUCCESS", -1: "PM_ERROR_INVALID_ARGUMENT", -2: "PM_ERROR_
                 -8: "PM_ERROR_DEVICE_LOST"}
 COOPERATIVE_SOLVE = 1
 PHASE_TIMING = 2
+PER_COLOR_SWEEP = 4
 PHASES = ("integrate", "broadphase", "narrowphase", "contact_cache", "prepare", "solve", "finish")
 
 _P = ctypes.c_void_p
@@ -170,8 +171,10 @@ class World:
     def __init__(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), profile=1,
                  base_iterations=10, requested_iterations=0, body_capacity=0,
                  pairs_per_body=0, points_per_body=0, device=0, cooperative=False,
-                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0, substeps=1, timing=True):
-        flags = (COOPERATIVE_SOLVE if cooperative else 0) | (PHASE_TIMING if timing else 0)
+                 solve_lanes=0, kernel_timing_stride=0, residual_tolerance=0.0, substeps=1, timing=True,
+                 per_color_sweep=False):
+        flags = ((COOPERATIVE_SOLVE if cooperative else 0) | (PHASE_TIMING if timing else 0) |
+                 (PER_COLOR_SWEEP if per_color_sweep else 0))
         desc = WorldDesc((_F * 3)(*gravity), dt, profile, base_iterations,
                          requested_iterations, body_capacity, pairs_per_body,
                          points_per_body, device, flags,
