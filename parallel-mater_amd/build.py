@@ -53,8 +53,38 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for obj in objs:
         os.remove(obj)
+    found = packed_fp32_ops(LIB)
+    if found:
+        raise RuntimeError(f"packed FP32 arithmetic in the device code: {found}")
     build_driver()
     return LIB
+
+
+OBJDUMP = os.environ.get("PMH_OBJDUMP", "/opt/rocm/llvm/bin/llvm-objdump")
+
+
+def packed_fp32_ops(lib):
+    """Counts v_pk_add_f32 / v_pk_mul_f32 / v_pk_fma_f32 in the gfx950 code
+    objects of a library built here (the compiler forms them from plain
+    float code; they are kept out of every kernel).  The code objects are
+    unbundled in a scratch directory."""
+    import re
+    import shutil
+    import tempfile
+    pattern = re.compile(r"\bv_pk_(?:add|mul|fma)_f32\b")
+    counts = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        copy = os.path.join(tmp, os.path.basename(lib))
+        shutil.copyfile(lib, copy)
+        subprocess.run([OBJDUMP, "--offloading", copy], cwd=tmp, check=True, capture_output=True)
+        for name in sorted(os.listdir(tmp)):
+            if not name.endswith(ARCH):
+                continue
+            text = subprocess.run([OBJDUMP, "-d", f"--mcpu={ARCH}", os.path.join(tmp, name)], check=True,
+                                  capture_output=True, text=True).stdout
+            for op in pattern.findall(text):
+                counts[op] = counts.get(op, 0) + 1
+    return counts
 
 
 DRIVER_SRC = os.path.join(HERE, "driver", "pmh_driver.cpp")
