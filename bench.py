@@ -65,6 +65,67 @@ def traffic_from_profiles(window, kernel):
     return rec.get("bytes_per_launch")
 
 
+def summarize_pmc(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json, out_path):
+    """Per-dispatch HBM traffic of the colour sweep from rocprofv3 --pmc runs
+    of this script (FETCH_SIZE, then WRITE_SIZE, each restricted to the sweep
+    kernel and k_bp_fill) over the bench line's timed window, each counter
+    scaled by what it reads for a device copy of known size (the
+    --copy-calibration runs).  k_bp_fill runs once per step and marks the
+    steps.  Writes the record that the bench line's `traffic` reads."""
+    import csv
+    import glob
+
+    def dispatches(root):
+        rows = {}
+        for path in glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    key = int(r["Dispatch_Id"])
+                    name = r["Kernel_Name"]
+                    rows.setdefault(key, [name, 0.0])[1] += float(r["Counter_Value"])
+        return [rows[k] for k in sorted(rows)]
+
+    with open(bench_json) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+    kernel = line["roofline"]["kernel"].split(" ")[0]
+    first, last = line["untimed_steps"], line["untimed_steps"] + line["steps"]
+
+    def window_mean(root):
+        step, values = -1, []
+        for name, value in dispatches(root):
+            if "k_bp_fill" in name:
+                step += 1
+            elif kernel.split("<")[0] in name and first <= step < last:
+                values.append(value)
+        return sum(values) / max(1, len(values)), len(values)
+
+    def copy_kib(root):
+        # the calibration's largest dispatch: one 256 MiB copy (the fill
+        # before it writes as much and reads nothing)
+        return max(v for _, v in dispatches(root))
+
+    fetch, count = window_mean(fetch_dir)
+    write, _ = window_mean(write_dir)
+    copy_bytes = float(256 << 20)
+    fetch_scale = copy_bytes / (copy_kib(cal_fetch_dir) * 1024.0)
+    write_scale = copy_bytes / (copy_kib(cal_write_dir) * 1024.0)
+    rec = {
+        "kernel": kernel,
+        "workload": line["config"]["workload"],
+        "window": [first, last],
+        "fetch_kib_per_dispatch": fetch,
+        "write_kib_per_dispatch": write,
+        "fetch_scale": fetch_scale,
+        "write_scale": write_scale,
+        "bytes_per_launch": (fetch * fetch_scale + write * write_scale) * 1024.0,
+        "algorithmic_bytes_per_launch": line["roofline"]["algorithmic_bytes_per_launch"],
+        "dispatches": count,
+    }
+    with open(out_path, "w") as f:
+        json.dump(rec, f, indent=1)
+    return rec
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -203,7 +264,16 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="lanes per body in the colour sweep (8, 16, 32, 64; 0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "CAL_FETCH_DIR", "CAL_WRITE_DIR",
+                                                         "BENCH_JSON"),
+                    help="no GPU: write profiles/color_sweep_traffic.json from rocprofv3 --pmc output directories "
+                         "(FETCH_SIZE and WRITE_SIZE of a bench run and of --copy-calibration 256) and that run's "
+                         "bench line")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        rec = summarize_pmc(*args.summarize_pmc, os.path.join(ROOT, "profiles", "color_sweep_traffic.json"))
+        print(json.dumps(rec))
+        return
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 4

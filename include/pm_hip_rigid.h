@@ -378,6 +378,15 @@ PmhResult pmhWorldSetGravity(PmhWorld world, const float gravity[3]);
  * a test can take that path without a stalled device. */
 PmhResult pmhWorldSetColorWaitLimit(PmhWorld world, uint32_t polls);
 
+/* Polls of one dataflow-sweep wait (a body waiting for a lower-colour
+ * neighbour) before it fails the step (default 2^22).  A wait that runs out
+ * ends every other wait of the step; the step keeps its states and
+ * pmhWorldStep returns PMH_ERROR_INVALID_STATE.  The ticket order of the
+ * sweep rules this out short of a device fault.  No counterpart in the
+ * reference; 0 makes the first unmet wait fail, so a test can take that
+ * path without a stalled device. */
+PmhResult pmhWorldSetSweepWaitLimit(PmhWorld world, uint32_t polls);
+
 /* pmCmdCopyRigidStates / pmCmdSetRigidStates (src/api/resources.cpp:1293;
  * runtime_rigid.cu.inc:147-195: a set-state resets AVBD history and the
  * contact cache). */

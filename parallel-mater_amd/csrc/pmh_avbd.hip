@@ -630,7 +630,8 @@ __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyPar
             }
             if (__ballot(pending) == 0ull) break;
             if (polls >= limit && lane == 0) st_relaxed(&ctr->color_abort, 1u);
-            if (polls >= limit || ld_relaxed(&ctr->color_abort) != 0u) {
+            // The abort flag every 256th poll only (one hot address).
+            if (polls >= limit || ((polls & 255u) == 255u && ld_relaxed(&ctr->color_abort) != 0u)) {
                 if (pending) {
                     st_relaxed(&color[i], kColorAborted);
                     atomicOr(&ctr->status, kStatusColorTimeout);
@@ -645,35 +646,45 @@ __global__ void __launch_bounds__(256) k_color_lane(uint32_t n, const PmhBodyPar
     }
 }
 
-// The same colouring with one wavefront per body ticket, its lanes sharing
-// the body's neighbour scan: the faster form when bodies have many
-// neighbours.  Tickets are issued in index order, so every lower-index
+// The same colouring with one wavefront per body, its lanes sharing the
+// body's neighbour scan: the faster form when bodies have many neighbours.
+// Blocks take tickets of 16 consecutive bodies in index order (one atomic
+// per block: same-address atomics serialise), so every lower-index
 // neighbour was taken earlier by a resident wavefront and eventually
 // publishes its colour: the wait chain always ends.
+constexpr uint32_t kColorWaveBlock = 16;   // wavefronts (bodies) per block ticket
 __device__ __forceinline__ uint32_t color_wait_bounded(const uint32_t *color, uint32_t j, StepCounters *ctr,
                                                        uint32_t limit) {
     uint32_t c, polls = 0;
     while ((c = ld_relaxed(&color[j])) == kInvalid) {
-        if (polls++ >= limit) st_relaxed(&ctr->color_abort, 1u);
-        if (ld_relaxed(&ctr->color_abort) != 0u) break;
+        if (++polls > limit) {
+            st_relaxed(&ctr->color_abort, 1u);
+            break;
+        }
+        // The abort flag every 256th poll only (one hot address).
+        if ((polls & 255u) == 0u && ld_relaxed(&ctr->color_abort) != 0u) break;
         __builtin_amdgcn_s_sleep(1);
     }
     return c;
 }
 
-__global__ void __launch_bounds__(256) k_color_wave(uint32_t n, const PmhBodyParameters *P, const uint2 *ab,
-                                               const uint32_t *adj_start, const uint32_t *adj, uint32_t *color,
-                                               const uint32_t *jstart, const uint32_t *jadj,
-                                               const PmhRigidConstraint *joints, StepCounters *ctr,
-                                               uint32_t limit) {
-    const uint32_t lane = threadIdx.x & 63;
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(&ctr->color_ticket, 1u);
-        t = __shfl(t, 0, 64);
-        if (t >= n) break;
-        const uint32_t i = t;
-        if (P[i].motion != kDynamic) continue;
+__global__ void __launch_bounds__(64 * kColorWaveBlock) k_color_wave(uint32_t n, const PmhBodyParameters *P,
+                                                                   const uint2 *ab, const uint32_t *adj_start,
+                                                                   const uint32_t *adj, uint32_t *color,
+                                                                   const uint32_t *jstart, const uint32_t *jadj,
+                                                                   const PmhRigidConstraint *joints,
+                                                                   StepCounters *ctr, uint32_t limit) {
+    __shared__ uint32_t s_ticket[2];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t round = 0;; ++round) {
+        // Double-buffered: the slot written in round r + 1 was last read
+        // before every wavefront passed round r's barrier.
+        if (threadIdx.x == 0) s_ticket[round & 1u] = atomicAdd(&ctr->color_ticket, 1u);
+        __syncthreads();
+        const uint32_t first = s_ticket[round & 1u] * kColorWaveBlock;
+        if (first >= n) break;               // the same for the whole block
+        const uint32_t i = first + wave;
+        if (i >= n || P[i].motion != kDynamic) continue;
         uint64_t used[kMaxColors / 64] = {0, 0, 0, 0};
         bool gave_up = false;
         const uint32_t b = adj_start[i], e = adj_start[i + 1];
@@ -935,14 +946,21 @@ __device__ __forceinline__ void load_disp_a(const SolveArgs &a, uint32_t i, floa
 // displacement.  Bounded: a wait that runs out raises the abort flag, which
 // ends every other wait of the step, and marks the status, so the step
 // fails instead of hanging.
+// A wait that runs out also marks the sweep failed (kept until the host
+// reports it): the step then keeps its states and fails.  The abort flag
+// is read every 256th poll only: thousands of waiting lanes reading one
+// address on every poll would queue on one memory channel, and the waits
+// would slow down the bodies they wait for.
 __device__ __forceinline__ void wait_published(const SolveArgs &a, uint32_t j) {
-    for (uint32_t polls = 0; int32_t(ld_relaxed(&a.done[j]) - a.flow_stamp) < 0; ++polls) {
-        if (polls >= a.flow_limit) {
+    for (uint32_t polls = 1; int32_t(ld_relaxed(&a.done[j]) - a.flow_stamp) < 0; ++polls) {
+        if (polls > a.flow_limit) {
             st_relaxed(&a.ctr->color_abort, 1u);
+            st_relaxed(&a.ctr->sweep_failed, 1u);
             atomicOr(&a.ctr->status, kStatusColorTimeout);
+            break;
         }
-        if (ld_relaxed(&a.ctr->color_abort) != 0u) break;
-        __builtin_amdgcn_s_sleep(1);
+        if ((polls & 255u) == 0u && ld_relaxed(&a.ctr->color_abort) != 0u) break;
+        __builtin_amdgcn_s_sleep(2);
     }
     asm volatile("" ::: "memory");   // the displacement reads stay after the wait
 }
@@ -1039,6 +1057,11 @@ __device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, u
                     blk.g[r] += h[r > c ? r * (r + 1) / 2 + c : c * (c + 1) / 2 + r] * dI[c];
         }
     }
+    // The crossing safeguard revisits the lane's rows: the first one's
+    // values stay in registers (a lane holds one row in most bodies).
+    float4 keep_axis = make_float4(0.0f, 0.0f, 0.0f, 0.0f), keep_angA = keep_axis, keep_angB = keep_axis;
+    float keep_lam = 0.0f, keep_pen = 0.0f, keep_dO[6] = {0, 0, 0, 0, 0, 0};
+    bool keep_first = false;
     for (uint32_t e = eb + sub_lane; e < ee; e += L) {
         const uint2 ent = a.entries[e];
         const uint32_t c = ent.x & 0x7fffffffu;
@@ -1054,6 +1077,16 @@ __device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, u
             load_disp_co(a.disp, other, dO);
         } else {
             load_disp(a.disp, other, dO);
+        }
+        if (e == eb + sub_lane) {
+            keep_axis = Jx[0];
+            keep_angA = Jx[3];
+            keep_angB = Jx[6];
+            keep_lam = lam.x;
+            keep_pen = pen.x;
+            keep_first = first;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) keep_dO[q] = dO[q];
         }
         const float *dA = first ? dI : dO;
         const float *dB = first ? dO : dI;
@@ -1087,18 +1120,34 @@ __device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, u
     bool crossing = false;
     if (solved) {
         for (uint32_t e = eb + sub_lane; e < ee; e += L) {
-            const uint2 ent = a.entries[e];
-            const uint32_t c = ent.x & 0x7fffffffu;
-            const bool first = (ent.x & kSideFirst) != 0u;
-            const uint32_t other = ent.y & kEntryBody;
-            float dO[6];
-            if (kFlow) load_disp_co(a.disp, other, dO);
-            else load_disp(a.disp, other, dO);
+            float4 axis, angA, angB;
+            float lam0, k0, dO[6];
+            bool first;
+            if (e == eb + sub_lane) {
+                axis = keep_axis;
+                first = keep_first;
+                angA = keep_angA;
+                angB = keep_angB;
+                lam0 = keep_lam;
+                k0 = keep_pen;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) dO[q] = keep_dO[q];
+            } else {
+                const uint2 ent = a.entries[e];
+                const uint32_t c = ent.x & 0x7fffffffu;
+                first = (ent.x & kSideFirst) != 0u;
+                const uint32_t other = ent.y & kEntryBody;
+                if (kFlow) load_disp_co(a.disp, other, dO);
+                else load_disp(a.disp, other, dO);
+                axis = a.erec[e];
+                angA = a.erec[3 * a.ecap + e];
+                angB = a.erec[6 * a.ecap + e];
+                lam0 = a.c_lam[c].x;
+                k0 = a.c_pen[c].x;
+            }
             const float *dA = first ? dI : dO;
             const float *dB = first ? dO : dI;
-            const float4 axis = a.erec[e], angA = a.erec[3 * a.ecap + e], angB = a.erec[6 * a.ecap + e];
             const float err = row_error(axis.w, axis, angA, angB, dA, dB);
-            const float lam0 = a.c_lam[c].x, k0 = a.c_pen[c].x;
             const float4 ang = first ? angA : angB;
             const float row[6] = {first ? -axis.x : axis.x, first ? -axis.y : axis.y,
                                   first ? -axis.z : axis.z, ang.x, ang.y, ang.z};
@@ -1905,29 +1954,38 @@ __global__ void __launch_bounds__(256) k_stage_color(SolveArgs a, uint32_t begin
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     solve_color<L, kJoints>(a, begin, end, phase, tid >> 6, (gridDim.x * blockDim.x) >> 6);
 }
-// Dataflow sweep: every colour in one launch per iteration.  Wavefronts
-// take tickets of one wavefront's worth of colour-ordered positions, in
-// order (a colour starts at a multiple of that, so a wavefront holds one
-// colour).  A body waits only for lower-colour neighbours: their positions
-// were ticketed earlier, to wavefronts that are running, so every wait ends
+// Dataflow sweep: every colour in one launch per iteration.  Blocks take
+// tickets of one block's worth of wavefront slots (each slot one
+// wavefront's worth of colour-ordered positions; a colour starts at a slot
+// boundary, so a wavefront holds one colour), in order, one atomic per
+// block.  A body waits only for lower-colour neighbours: their positions
+// were ticketed earlier, to blocks that are running, so every wait ends
 // (and each is bounded besides).  Colour c + 1 starts wherever colour c is
 // done, not after all of colour c.
+constexpr uint32_t kSweepWaves = 8;          // wavefronts per block without joints (512 threads)
+constexpr uint32_t kSweepJointWaves = 4;     // with joints (more registers per lane)
 template <int L, bool kJoints>
-__global__ void __launch_bounds__(256) k_sweep(SolveArgs a, uint32_t phase) {
+__global__ void __launch_bounds__(64 * (kJoints ? kSweepJointWaves : kSweepWaves))
+k_sweep(SolveArgs a, uint32_t phase) {
+    __shared__ uint32_t s_ticket[2];
     uint32_t *tickets = a.ctr->sweep_ticket;
     // The other counter is the next sweep's: it starts at zero.
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&tickets[(a.flow_stamp + 1u) & 1u], 0u);
     if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
     constexpr uint32_t groups = 64 / L;
-    const uint32_t lane = threadIdx.x & 63;
+    constexpr uint32_t waves = kJoints ? kSweepJointWaves : kSweepWaves;
+    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t end = a.color_start[a.ctr->colors];
+    const uint32_t wave_slots = (end + groups - 1) / groups;
     uint32_t *ticket = &tickets[a.flow_stamp & 1u];
-    for (;;) {
-        uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(ticket, 1u);
-        t = __shfl(t, 0, 64);
-        if (t >= (end + groups - 1) / groups) break;
-        solve_group<L, kJoints, true>(a, t * groups, end, phase);
+    for (uint32_t round = 0;; ++round) {
+        // Double-buffered: the slot written in round r + 1 was last read
+        // before every wavefront passed round r's barrier.
+        if (threadIdx.x == 0) s_ticket[round & 1u] = atomicAdd(ticket, 1u);
+        __syncthreads();
+        const uint32_t first = s_ticket[round & 1u] * waves;
+        if (first >= wave_slots) break;      // the same for the whole block
+        if (first + wave < wave_slots) solve_group<L, kJoints, true>(a, (first + wave) * groups, end, phase);
     }
 }
 template <bool kJoints>
@@ -2280,6 +2338,13 @@ __global__ void k_schedule_finish(PmhContactSchedule *s, const StepCounters *ctr
     s->blocks = blocks;
 }
 
+PmhResult report_sweep_failure(hipStream_t s, StepCounters *ctr) {
+    static const uint32_t zero = 0u;
+    PMH_CHECK(hipMemcpyAsync(&ctr->sweep_failed, &zero, sizeof zero, hipMemcpyHostToDevice, s));
+    PmhResult r = sync_stream(s);
+    return r != PMH_SUCCESS ? r : PMH_ERROR_INVALID_STATE;
+}
+
 PmhResult status_result(uint32_t status) {
     if (status & (kStatusPairOverflow | kStatusPointOverflow | kStatusColorOverflow)) return PMH_ERROR_CAPACITY_EXCEEDED;
     if (status & kStatusColorTimeout) return PMH_ERROR_INVALID_STATE;
@@ -2418,13 +2483,14 @@ static uint32_t g_coop_blocks = 0;
 
 // Resident blocks of a dataflow-sweep kernel over the device (at most 8 per
 // CU), found once per kernel.
-static uint32_t sweep_blocks(const void *kernel) {
+static uint32_t sweep_blocks(const void *kernel, uint32_t threads) {
     static std::mutex lock;
     static std::vector<std::pair<const void *, uint32_t>> known;
     std::lock_guard<std::mutex> guard(lock);
     for (const auto &k : known) if (k.first == kernel) return k.second;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, int(threads), 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
     const uint32_t blocks = uint32_t(std::min(per_cu, 8)) * uint32_t(device_compute_units());
     known.emplace_back(kernel, blocks);
     return blocks;
@@ -2547,8 +2613,9 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         k_color_lane<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
                                             v.nj ? v.joints : nullptr, ctr, sp.color_wait_limit);
     } else {
-        const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((n + 3) / 4, 4 * device_compute_units()));
-        k_color_wave<<<blocks, 256, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
+        const uint32_t blocks = std::max<uint32_t>(
+            1, std::min<uint32_t>((n + kColorWaveBlock - 1) / kColorWaveBlock, 2 * device_compute_units()));
+        k_color_wave<<<blocks, 64 * kColorWaveBlock, 0, s>>>(n, P, v.ab.p, v.adj_start.p, v.adj.p, v.color.p, v.jstart.p, v.jadj.p,
                                             v.nj ? v.joints : nullptr, ctr, sp.color_wait_limit);
     }
     PMH_CHECK(hipMemsetAsync(v.color_count.p, 0, sizeof(uint32_t) * (kMaxColors + 1), s));
@@ -2608,6 +2675,9 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         // Overflowed pools or a colouring that gave up: no solve on a
         // partial schedule; the caller keeps the pre-step states.
         if (host.status & kStatusFatal) return status_result(host.status);
+        // A dataflow sweep of an earlier step that ran out of its wait
+        // (that step kept its states): reported once.
+        if (host.sweep_failed) return report_sweep_failure(s, ctr);
         v.neighbour_hint = 2.0f * float(uint64_t(host.manifolds) + v.nj) / float(std::max<uint32_t>(1, n));
         const uint32_t lanes = sp.lanes;
         if (!realtime && sp.requested_iterations == 0u) a.budget = std::max(budget, host.joint_budget);
@@ -2664,11 +2734,14 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                          : lanes == 8  ? reinterpret_cast<const void *>(&k_sweep<8, false>)
                                        : reinterpret_cast<const void *>(&k_sweep<16, false>);
             }
-            const uint32_t waves = (positions + (64 / lanes) - 1) / (64 / lanes);
-            const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((waves + 3) / 4, sweep_blocks(kernel)));
+            const uint32_t block_waves = joints ? kSweepJointWaves : kSweepWaves;
+            const uint32_t slots = (positions + (64 / lanes) - 1) / (64 / lanes);
+            const uint32_t blocks = std::max<uint32_t>(
+                1, std::min<uint32_t>((slots + block_waves - 1) / block_waves, sweep_blocks(kernel, 64 * block_waves)));
             void *args[] = {&af, &phase};
-            if (start) return hipExtLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s, start, stop, 0);
-            return hipLaunchKernel(kernel, dim3(blocks), dim3(256), args, 0, s);
+            const dim3 block(64 * block_waves);
+            if (start) return hipExtLaunchKernel(kernel, dim3(blocks), block, args, 0, s, start, stop, 0);
+            return hipLaunchKernel(kernel, dim3(blocks), block, args, 0, s);
         };
         uint32_t launches = 0;
         for (uint32_t phase = 0; phase < phases; ++phase) {

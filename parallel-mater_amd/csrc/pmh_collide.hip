@@ -1884,9 +1884,17 @@ __global__ void k_cache_save(const uint32_t *npairs, const uint32_t *code, const
     if (blockIdx.x == 0 && threadIdx.x == 0) *n_saved = std::min(ctr->manifolds, cap);
 }
 
-__global__ void k_clamp_speeds(uint32_t n, const PmhBodyParameters *P, PmhRigidBodyState *s) {
+// A step whose dataflow sweep failed keeps its states: the pre-step
+// states replace the result.
+__global__ void k_clamp_speeds(uint32_t n, const PmhBodyParameters *P, PmhRigidBodyState *s,
+                               const PmhRigidBodyState *before, const StepCounters *ctr) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || P[i].motion != kDynamic) return;
+    if (i >= n) return;
+    if (ctr->color_abort != 0u) {
+        s[i] = before[i];
+        return;
+    }
+    if (P[i].motion != kDynamic) return;
     s[i].linear_velocity = pv(clamp_length(sv(s[i].linear_velocity), P[i].maximum_linear_speed));
     s[i].angular_velocity = pv(clamp_length(sv(s[i].angular_velocity), P[i].maximum_angular_speed));
 }
@@ -2016,7 +2024,7 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     mark(4);
     SolveParams sp{dt, gravity, base_iterations, requested_iterations, profile, cooperative ? 1u : 0u, solve_lanes,
                    residual_tolerance, warm_infer && warm_enabled, color_wait_limit};
-    sp.sweep_wait_limit = color_wait_limit;
+    sp.sweep_wait_limit = sweep_wait_limit;
     sp.dataflow = !per_color_sweep;
     SolveTiming st;
     st.kernels = &kernel_timer;
@@ -2028,7 +2036,7 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     solve_timing.launches += st.launches;
     if (r != PMH_SUCCESS) return r;
     mark(5);
-    k_clamp_speeds<<<bg, 256, 0, s>>>(n, params.p, pred);
+    k_clamp_speeds<<<bg, 256, 0, s>>>(n, params.p, pred, cur, ctr);
     k_cache_save<<<pg, 256, 0, s>>>(&ctr->pairs, v.code.p, v.ab.p, v.m_count.p, v.m_start.p, v.m_slot.p, pred,
                                     v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_lam.p, v.c_pen.p, v.c_acc.p,
                                     cache_code[wr].p, cache_count[wr].p, cache_data[wr].p, &cache_n.p[wr],

@@ -106,6 +106,7 @@ struct StepCounters {
     uint32_t log_count;        // contact samples logged so far (learned warm start)
     uint32_t color_abort;      // set when a bounded wait (colouring or sweep) ran out: every other wait gives up
     uint32_t sweep_ticket[2];  // work tickets of the dataflow sweep, alternating per sweep
+    uint32_t sweep_failed;     // a dataflow-sweep wait ran out; stays set until the host reports it
 };
 
 // Running totals over every step since the last reset.
@@ -123,6 +124,9 @@ constexpr uint32_t kColorWaitLimit = 1u << 22;
 constexpr uint32_t kStatusFatal = kStatusPairOverflow | kStatusPointOverflow | kStatusColorOverflow |
                                   kStatusColorTimeout;
 PmhResult status_result(uint32_t status);
+// Clears the sweep-failure mark on the device and returns
+// PMH_ERROR_INVALID_STATE (or the stream's own error).
+PmhResult report_sweep_failure(hipStream_t s, StepCounters *ctr);
 // Waits for the stream with a deadline (PMH_SYNC_TIMEOUT_S, default 120 s):
 // a device that stops answering gives PMH_ERROR_DEVICE_LOST, not a caller
 // blocked for ever.
@@ -282,7 +286,7 @@ struct SolveParams {
     float tolerance = 0.0f;              // > 0: to-residual mode, the budget caps each phase
     bool warm_infer = false;             // predict cold points' normal duals
     uint32_t color_wait_limit = kColorWaitLimit;   // polls of one colour wait before it gives up
-    uint32_t sweep_wait_limit = kColorWaitLimit;   // polls of one dataflow-sweep wait before it gives up
+    uint32_t sweep_wait_limit = kColorWaitLimit;   // polls of one dataflow-sweep wait before it fails the step
     bool dataflow = true;                // one dataflow sweep per iteration instead of one launch per colour
     bool reset_status = false;           // clear the status word first (the kernel seam; a world step clears it itself)
 };

@@ -944,6 +944,7 @@ PmhResult pmhWorldStep(PmhWorld world, uint32_t steps) {
     w->has_stepped = true;
     pmh::StepCounters c;
     PMH_CHECK(hipMemcpy(&c, w->solver.counters.p, sizeof c, hipMemcpyDeviceToHost));
+    if (c.sweep_failed) return pmh::report_sweep_failure(w->stream, w->solver.counters.p);
     return pmh::status_result(c.status);
 }
 
@@ -951,6 +952,12 @@ PmhResult pmhWorldSetGravity(PmhWorld world, const float gravity[3]) {
     if (!world || !gravity || !std::isfinite(gravity[0]) || !std::isfinite(gravity[1]) || !std::isfinite(gravity[2]))
         return PMH_ERROR_INVALID_ARGUMENT;
     W(world)->gravity = {gravity[0], gravity[1], gravity[2]};
+    return PMH_SUCCESS;
+}
+
+PmhResult pmhWorldSetSweepWaitLimit(PmhWorld world, uint32_t polls) {
+    if (!world) return PMH_ERROR_INVALID_ARGUMENT;
+    W(world)->sweep_wait_limit = polls;
     return PMH_SUCCESS;
 }
 

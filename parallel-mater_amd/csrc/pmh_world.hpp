@@ -17,6 +17,7 @@ struct World {
     uint32_t solve_lanes = kDefaultLanes;
     float residual_tolerance = 0.0f;
     uint32_t color_wait_limit = kColorWaitLimit;
+    uint32_t sweep_wait_limit = kColorWaitLimit;
     int device = 0;
     hipStream_t stream = nullptr;
 

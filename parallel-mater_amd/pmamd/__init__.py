@@ -101,6 +101,7 @@ def lib():
         "pmhWorldSetSolveLanes": (i32, [_P, _U]),
         "pmhWorldStep": (i32, [_P, _U]),
         "pmhWorldSetColorWaitLimit": (i32, [_P, _U]),
+        "pmhWorldSetSweepWaitLimit": (i32, [_P, _U]),
         "pmhWorldApplyRigidInputs": (i32, [_P, _U, _P]),
         "pmhWorldSetKinematicTargets": (i32, [_P, _U, _U, _P]),
         "pmhWorldAddJoints": (i32, [_P, _U, _P]),
@@ -338,6 +339,11 @@ class World:
         0 gives up at the first wait on an uncoloured neighbour."""
         _check(lib().pmhWorldSetColorWaitLimit(self._h, polls), "pmhWorldSetColorWaitLimit")
 
+    def set_sweep_wait_limit(self, polls):
+        """Polls of one dataflow-sweep wait before it fails the step
+        (default 2**22); 0 fails at the first unmet wait."""
+        _check(lib().pmhWorldSetSweepWaitLimit(self._h, polls), "pmhWorldSetSweepWaitLimit")
+
     def residuals(self, maximum=False):
         """(per-iteration RMS constraint residuals [2, 64], NaN where no
         update ran; iterations run per phase) of the last step; with
@@ -445,10 +451,11 @@ class DeviceArray:
 
 def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
                   dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), base=10, requested=0,
-                  profile=1):
-    """Kernel seam on the GPU with host arrays: copies in, runs
-    pmhPrepareSolveGeometryConstraints, copies back in place (states,
-    manifolds, bodies) and returns the schedule record."""
+                  profile=1, entry="pmhPrepareSolveGeometryConstraints"):
+    """Kernel seam on the GPU with host arrays: copies in, runs `entry`
+    (pmhPrepareSolveGeometryConstraints, or its variant that allocates
+    nothing), copies back in place (states, manifolds, bodies) and returns
+    the schedule record."""
     n = len(params)
     if bodies is None:
         bodies = np.zeros(n, dtype=AVBD_BODY)
@@ -461,9 +468,9 @@ def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
     d_b = DeviceArray(np.ascontiguousarray(bodies, dtype=AVBD_BODY))
     d_sched = DeviceArray(np.zeros(1, dtype=SCHEDULE))
     g = np.asarray(gravity, dtype=np.float32)
-    _check(lib().pmhPrepareSolveGeometryConstraints(
+    _check(getattr(lib(), entry)(
         None, d_p.ptr, d_prev.ptr, d_s.ptr, n, d_m.ptr, d_pairs.ptr, d_count.ptr, d_b.ptr,
-        d_sched.ptr, dt, _ptr(g), base, requested, profile), "pmhPrepareSolveGeometryConstraints")
+        d_sched.ptr, dt, _ptr(g), base, requested, profile), entry)
     states[...] = d_s.read()
     manifolds[...] = d_m.read()
     bodies[...] = d_b.read()

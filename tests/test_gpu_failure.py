@@ -6,6 +6,10 @@ still ends normally.
   wait on an uncoloured neighbour gives up) ends the colouring, the substep
   returns PM_ERROR_INVALID_STATE before its solve, and the states stay those
   before that step; with the default limit the world then steps on.
+- A dataflow-sweep wait that runs out (the sweep's wait limit set to 0, so
+  the first unmet wait on a lower-colour neighbour runs out) ends every wait
+  of the step; the step keeps its states and returns
+  PM_ERROR_INVALID_STATE, and the world then steps on.
 - Worlds of fewer than four bodies are refused before any device work.
 """
 import sys
@@ -50,6 +54,36 @@ def test_a_colouring_that_gives_up_fails_the_step_and_keeps_the_states():
     s = g.states()
     assert np.isfinite(s["position"]).all()
     # the pyramid still stands: every box within 5 cm of where it started
+    assert np.abs(s["position"][1:] - scene.states["position"][1:]).max() < 0.05
+    g.close()
+
+
+def test_a_sweep_wait_that_runs_out_fails_the_step_and_keeps_the_states():
+    say("pyramid of 55 boxes: 2 steps, steps with sweep wait limit 0 until one fails, 3 more steps")
+    scene = S.pyramid()
+    g = scene.build(pmamd.World(profile=0, base_iterations=10))
+    g.step(2)
+    g.set_sweep_wait_limit(0)
+    failed = None
+    for attempt in range(4):
+        before = g.states()
+        try:
+            g.step(1)
+        except pmamd.PmhError as e:
+            failed = e.code
+            break
+    # The boxes of the second colour rest on those of the first and are
+    # ticketed right after them: one of them meets an unpublished neighbour.
+    assert failed == INVALID_STATE
+    say(f"failed at attempt {attempt}")
+    after = g.states()
+    for f in ("position", "orientation", "linear_velocity", "angular_velocity"):
+        np.testing.assert_array_equal(after[f], before[f], err_msg=f)
+    g.set_sweep_wait_limit(1 << 22)
+    g.step(3)
+    assert g.stats()["status"] == 0
+    s = g.states()
+    assert np.isfinite(s["position"]).all()
     assert np.abs(s["position"][1:] - scene.states["position"][1:]).max() < 0.05
     g.close()
 
