@@ -487,6 +487,25 @@ PmhResult pmhPrepareSolveGeometryConstraints(
     const float gravity[3], uint32_t base_iterations,
     uint32_t requested_iterations, uint32_t profile_version);
 
+/* The same seam without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as pmhPrepareSolveGeometryConstraints.
+ * Every buffer of the solve is a slice of one static 24 MiB device array per
+ * GPU (held until the process ends): at most 4096 bodies and 4096 active
+ * pairs (8 contact points each), PMH_ERROR_CAPACITY_EXCEEDED beyond.  Calls
+ * on one GPU share that array; they are serialised inside the library, and
+ * work that a caller replays from a capture of one call must not overlap
+ * another call on the same GPU. */
+PmhResult pmhPrepareSolveGeometryConstraintsNoAlloc(
+    void *stream, const PmhBodyParameters *parameters,
+    const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds,
+    const uint32_t *active_pairs, const uint32_t *active_pair_count,
+    PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations,
+    uint32_t requested_iterations, uint32_t profile_version);
+
 /* Device memory helpers for callers without a HIP runtime of their own. */
 PmhResult pmhDeviceAlloc(uint64_t bytes, void **pointer);
 PmhResult pmhDeviceFree(void *pointer);

@@ -2414,26 +2414,22 @@ PmhResult Solver::reserve_joints(uint32_t count) {
     return PMH_SUCCESS;
 }
 
-PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
+// Every device buffer of the solver with its element count for `bodies`
+// bodies, `pairs` pairs and `points` contact points.
+template <class F>
+PmhResult Solver::each_buffer(uint32_t bodies, uint32_t pairs, uint32_t points, F &&f) {
     PmhResult r = PMH_SUCCESS;
     // Colour-ordered positions: the bodies and at most 7 gap slots per
     // colour (8 bodies per wavefront at 8 lanes each).
     const uint32_t positions = bodies + kMaxColors * 7u;
-#define R(buf, cnt) if ((r = buf.reserve(cnt)) != PMH_SUCCESS) return r
+#define R(buf, cnt) if ((r = f(buf, size_t(cnt))) != PMH_SUCCESS) return r
     R(tpos, bodies); R(tq, bodies); R(tv, bodies); R(tw, bodies); R(disp, 2 * (size_t)bodies);
     R(dq, bodies); R(inertia, 2 * (size_t)bodies); R(color, bodies); R(parent, bodies);
     R(impact, bodies); R(deg, positions + 1); R(adj_start, bodies + 1); R(adj_cursor, bodies);
     R(color_bodies, positions); R(bp_start, positions + 1); R(anchored, bodies); R(mean, bodies);
     R(members, bodies); R(mean_list, bodies);
     R(jdeg, bodies + 1); R(jstart, bodies + 1); R(jcursor, bodies); R(joint_head, bodies);
-    if (prev_vel.n < bodies) {
-        R(prev_vel, bodies);
-        PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
-    }
-    if (done.n < bodies) {
-        R(done, bodies);
-        PMH_CHECK(hipMemset(done.p, 0, sizeof(uint32_t) * bodies));
-    }
+    R(prev_vel, bodies); R(done, bodies);
     R(code, pairs); R(ab, pairs); R(m_count, pairs + 1); R(m_start, pairs + 1); R(m_slot, pairs + 1);
     R(m_impact, pairs); R(m_face, pairs); R(m_init_rel, pairs); R(adj, 2 * (size_t)pairs);
     R(tri_list, pairs);
@@ -2441,19 +2437,76 @@ PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
     R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
     R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
     R(erec, 18 * (size_t)points); R(c_entry, 2 * (size_t)points); R(residual, 1);
-    ecap = 2 * (size_t)points;
     R(schedule, 1); R(color_count, kMaxColors + 1); R(color_start, kMaxColors + 1);
-    if (!counters.p) {
-        R(counters, 1);
-        PMH_CHECK(hipMemset(counters.p, 0, sizeof(StepCounters)));
-    }
-    if (!totals.p) {
-        R(totals, 1);
-        PMH_CHECK(hipMemset(totals.p, 0, sizeof(SolveTotals)));
-    }
+    R(counters, 1); R(totals, 1);
 #undef R
+    return PMH_SUCCESS;
+}
+
+PmhResult Solver::reserve(uint32_t bodies, uint32_t pairs, uint32_t points) {
+    // Zeroed when first allocated: the velocity history, the published
+    // sweep stamps, the counters and the running totals.
+    const bool fresh_prev = prev_vel.n < bodies, fresh_done = done.n < bodies;
+    const bool fresh_ctr = !counters.p, fresh_totals = !totals.p;
+    PmhResult r = each_buffer(bodies, pairs, points, [](auto &buf, size_t count) { return buf.reserve(count); });
+    if (r != PMH_SUCCESS) return r;
+    if (fresh_prev) PMH_CHECK(hipMemset(prev_vel.p, 0, sizeof(float4) * bodies));
+    if (fresh_done) PMH_CHECK(hipMemset(done.p, 0, sizeof(uint32_t) * bodies));
+    if (fresh_ctr) PMH_CHECK(hipMemset(counters.p, 0, sizeof(StepCounters)));
+    if (fresh_totals) PMH_CHECK(hipMemset(totals.p, 0, sizeof(SolveTotals)));
+    ecap = 2 * (size_t)points;
     n = bodies;
-    slots = positions;
+    slots = bodies + kMaxColors * 7u;
+    pair_cap = pairs;
+    point_cap = points;
+    return PMH_SUCCESS;
+}
+
+// Scan partials of the largest exclusive scan over these counts.
+static uint32_t scan_blocks_for(uint32_t bodies, uint32_t pairs) {
+    const uint32_t count = std::max(bodies + kMaxColors * 7u, pairs);
+    return (count + kScanTile - 1) / kScanTile + 1;
+}
+
+size_t Solver::carve_bytes(uint32_t bodies, uint32_t pairs, uint32_t points) {
+    Solver probe;
+    size_t used = 0;
+    auto count = [&](auto &buf, size_t cnt) {
+        using T = std::remove_reference_t<decltype(*buf.p)>;
+        used += (std::max<size_t>(cnt, 1) * sizeof(T) + 255) & ~size_t(255);
+        return PMH_SUCCESS;
+    };
+    (void)probe.each_buffer(bodies, pairs, points, count);
+    (void)count(probe.scan_tmp, scan_blocks_for(bodies, pairs));
+    return used;
+}
+
+PmhResult Solver::carve(uint8_t *base, size_t bytes, uint32_t bodies, uint32_t pairs, uint32_t points,
+                        StepCounters *host_counters, uint32_t *host_starts) {
+    size_t used = 0;
+    auto take = [&](auto &buf, size_t cnt) {
+        using T = std::remove_reference_t<decltype(*buf.p)>;
+        const size_t need = (std::max<size_t>(cnt, 1) * sizeof(T) + 255) & ~size_t(255);
+        if (used + need > bytes) return PMH_ERROR_CAPACITY_EXCEEDED;
+        buf.borrow(reinterpret_cast<T *>(base + used), cnt);
+        used += need;
+        return PMH_SUCCESS;
+    };
+    PmhResult r = each_buffer(bodies, pairs, points, take);
+    if (r == PMH_SUCCESS) r = take(scan_tmp, scan_blocks_for(bodies, pairs));
+    if (r != PMH_SUCCESS) return r;
+    host_ctr = host_counters;
+    host_color_start = host_starts;
+    host_owned = false;
+    ecap = 2 * (size_t)points;
+    return use(bodies, pairs, points);
+}
+
+PmhResult Solver::use(uint32_t bodies, uint32_t pairs, uint32_t points) {
+    if (size_t(bodies) > tpos.n || size_t(pairs) > code.n || size_t(points) > c_np.n)
+        return PMH_ERROR_CAPACITY_EXCEEDED;
+    n = bodies;
+    slots = bodies + kMaxColors * 7u;
     pair_cap = pairs;
     point_cap = points;
     return PMH_SUCCESS;
@@ -2473,10 +2526,11 @@ void Solver::release() {
     cold_list.release(); cold_feat.release(); cold_scale.release();
     schedule.release(); color_count.release(); color_start.release(); counters.release(); totals.release();
     scan_tmp.release();
-    if (host_ctr) (void)hipHostFree(host_ctr);
-    if (host_color_start) (void)hipHostFree(host_color_start);
+    if (host_owned && host_ctr) (void)hipHostFree(host_ctr);
+    if (host_owned && host_color_start) (void)hipHostFree(host_color_start);
     host_ctr = nullptr;
     host_color_start = nullptr;
+    host_owned = true;
 }
 
 static uint32_t g_coop_blocks = 0;

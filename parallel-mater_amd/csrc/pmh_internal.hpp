@@ -141,8 +141,10 @@ template <class T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    bool borrowed = false;               // a slice of memory the buffer does not own (never freed or grown)
     PmhResult reserve(size_t count) {
         if (count <= n && p) return PMH_SUCCESS;
+        if (borrowed) return PMH_ERROR_CAPACITY_EXCEEDED;
         if (p) { (void)hipFree(p); p = nullptr; n = 0; }
         size_t bytes = (count ? count : 1) * sizeof(T);
         hipError_t e = hipMalloc(&p, bytes);
@@ -150,7 +152,18 @@ struct DBuf {
         n = count;
         return PMH_SUCCESS;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void borrow(T *slice, size_t count) {
+        release();
+        p = slice;
+        n = count;
+        borrowed = true;
+    }
+    void release() {
+        if (p && !borrowed) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        borrowed = false;
+    }
 };
 
 // Constraint residual after each dual update of the step, 64 slots per
@@ -272,8 +285,19 @@ struct Solver {
     DBuf<uint32_t> scan_tmp;
     StepCounters *host_ctr = nullptr;        // pinned read-back of the schedule
     uint32_t *host_color_start = nullptr;
+    bool host_owned = true;                  // the read-back records were allocated here
     PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
     PmhResult reserve_joints(uint32_t count);
+    // Every buffer as a slice of `bytes` bytes at `base` (device memory the
+    // caller owns), sized for at most the given counts: no allocation
+    // afterwards.  The host read-back records are the caller's too.
+    PmhResult carve(uint8_t *base, size_t bytes, uint32_t bodies, uint32_t pairs, uint32_t points,
+                    StepCounters *host_counters, uint32_t *host_starts);
+    // Bytes carve() takes for these counts.
+    static size_t carve_bytes(uint32_t bodies, uint32_t pairs, uint32_t points);
+    // Sizes of this call within the carved capacity.
+    PmhResult use(uint32_t bodies, uint32_t pairs, uint32_t points);
+    template <class F> PmhResult each_buffer(uint32_t bodies, uint32_t pairs, uint32_t points, F &&f);
     void release();
 };
 

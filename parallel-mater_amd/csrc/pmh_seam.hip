@@ -8,6 +8,7 @@
 #include "pmh_internal.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <mutex>
 
 namespace pmh {
@@ -124,33 +125,16 @@ __global__ void k_seam_bodies_out(uint32_t n, PmhAvbdBody *bodies, PmhContactMan
     bodies[i] = b;
 }
 
-static std::mutex g_seam_mutex;
-static Solver *g_seam_solver = nullptr;
-
-}  // namespace pmh
-
-extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
-    void *stream, const PmhBodyParameters *parameters, const PmhRigidBodyState *previous, PmhRigidBodyState *states,
-    uint32_t body_count, PmhContactManifold *manifolds, const uint32_t *active_pairs,
-    const uint32_t *active_pair_count, PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
-    const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
-    using namespace pmh;
-    if (!parameters || !previous || !states || !bodies || !schedule || !gravity || !(timestep > 0.0f) ||
-        body_count == 0 || profile_version > 1 || (!active_pair_count))
-        return PMH_ERROR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lock(g_seam_mutex);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    uint32_t M = 0;
-    PMH_CHECK(hipMemcpyAsync(&M, active_pair_count, sizeof M, hipMemcpyDeviceToHost, s));
-    PMH_CHECK(hipStreamSynchronize(s));
-    if (M && (!manifolds || !active_pairs)) return PMH_ERROR_INVALID_ARGUMENT;
-    if (!g_seam_solver) g_seam_solver = new Solver();
-    Solver &v = *g_seam_solver;
-    PmhResult r = v.reserve(body_count, std::max<uint32_t>(M, 1), std::max<uint32_t>(8 * M, 8));
-    if (r != PMH_SUCCESS) return r;
+// The seam over a solver whose buffers hold at least `body_count` bodies,
+// `M` pairs and 8 M points.
+static PmhResult seam_run(Solver &v, hipStream_t s, const PmhBodyParameters *parameters,
+                          const PmhRigidBodyState *previous, PmhRigidBodyState *states, uint32_t body_count,
+                          PmhContactManifold *manifolds, const uint32_t *active_pairs, uint32_t M,
+                          PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep, const float gravity[3],
+                          uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
     const uint32_t mg = std::max<uint32_t>(1, (M + 255) / 256), bg = (body_count + 255) / 256;
     if (M) k_seam_counts<<<mg, 256, 0, s>>>(M, manifolds, v.m_count.p);
-    r = exclusive_scan(s, v.m_count.p, v.m_start.p, M, nullptr, nullptr, v.scan_tmp);
+    PmhResult r = exclusive_scan(s, v.m_count.p, v.m_start.p, M, nullptr, nullptr, v.scan_tmp);
     if (r != PMH_SUCCESS) return r;
     k_seam_in<<<mg, 256, 0, s>>>(body_count, M, active_pairs, manifolds, v.m_start.p, v.code.p, v.ab.p, v.m_count.p,
                                  v.m_init_rel.p, v.m_face.p, v.c_np.p, v.c_pt.p, v.c_aa.p, v.c_ab.p, v.c_err.p,
@@ -171,4 +155,123 @@ extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
     PMH_CHECK(hipStreamSynchronize(s));
     PMH_CHECK(hipGetLastError());
     return PMH_SUCCESS;
+}
+
+static bool seam_args_valid(const PmhBodyParameters *parameters, const PmhRigidBodyState *previous,
+                            PmhRigidBodyState *states, uint32_t body_count, PmhAvbdBody *bodies,
+                            PmhContactSchedule *schedule, float timestep, const float gravity[3],
+                            uint32_t profile_version, const uint32_t *active_pair_count) {
+    return parameters && previous && states && bodies && schedule && gravity && timestep > 0.0f && body_count != 0 &&
+           profile_version <= 1 && active_pair_count;
+}
+
+static std::mutex g_seam_mutex;
+static Solver *g_seam_solver = nullptr;
+
+// ---- the seam without allocation ------------------------------------------
+// Every buffer of the solve is a slice of one file-scope device array per
+// GPU, sized for kStaticBodies bodies, kStaticPairs manifolds and 8 points
+// per manifold; the host read-back records are file-scope host arrays.
+constexpr uint32_t kStaticBodies = 4096, kStaticPairs = 4096, kStaticPoints = 8 * kStaticPairs;
+constexpr size_t kStaticScratchBytes = size_t(24) << 20;
+constexpr int kStaticDevices = 64;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+// Clears the carved counters and totals (a fill kernel, so the call records
+// no memset node where a caller captures the stream).
+__global__ void k_seam_clear(uint32_t *words, uint32_t count) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) words[k] = 0u;
+}
+
+struct StaticSeam {
+    uint8_t *base = nullptr;             // the device's address of `scratch`
+    bool ready = false;
+    Solver solver;
+    StepCounters host_counters;
+    uint32_t host_starts[kMaxColors + 1];
+};
+static StaticSeam g_static_seam[kStaticDevices];
+
+}  // namespace pmh
+
+extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
+    void *stream, const PmhBodyParameters *parameters, const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds, const uint32_t *active_pairs,
+    const uint32_t *active_pair_count, PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
+    using namespace pmh;
+    if (!seam_args_valid(parameters, previous, states, body_count, bodies, schedule, timestep, gravity,
+                         profile_version, active_pair_count))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_seam_mutex);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint32_t M = 0;
+    PMH_CHECK(hipMemcpyAsync(&M, active_pair_count, sizeof M, hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    if (M && (!manifolds || !active_pairs)) return PMH_ERROR_INVALID_ARGUMENT;
+    if (!g_seam_solver) g_seam_solver = new Solver();
+    Solver &v = *g_seam_solver;
+    PmhResult r = v.reserve(body_count, std::max<uint32_t>(M, 1), std::max<uint32_t>(8 * M, 8));
+    if (r != PMH_SUCCESS) return r;
+    return seam_run(v, s, parameters, previous, states, body_count, manifolds, active_pairs, M, bodies, schedule,
+                    timestep, gravity, base_iterations, requested_iterations, profile_version);
+}
+
+extern "C" PmhResult pmhPrepareSolveGeometryConstraintsNoAlloc(
+    void *stream, const PmhBodyParameters *parameters, const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds, const uint32_t *active_pairs,
+    const uint32_t *active_pair_count, PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
+    using namespace pmh;
+    if (!seam_args_valid(parameters, previous, states, body_count, bodies, schedule, timestep, gravity,
+                         profile_version, active_pair_count))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    if (body_count > kStaticBodies) {
+        std::fprintf(stderr, "pmhPrepareSolveGeometryConstraintsNoAlloc: %u bodies, at most %u\n", body_count,
+                     kStaticBodies);
+        return PMH_ERROR_CAPACITY_EXCEEDED;
+    }
+    std::lock_guard<std::mutex> lock(g_seam_mutex);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int dev = 0;
+    PMH_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kStaticDevices) return PMH_ERROR_INVALID_ARGUMENT;
+    StaticSeam &g = g_static_seam[dev];
+    if (!g.base) {
+        // The array's address on this device (host code never uses the
+        // symbol itself as a device address).
+        void *ptr = nullptr;
+        PMH_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return PMH_ERROR_INVALID_STATE;
+        g.base = static_cast<uint8_t *>(ptr);
+    }
+    if (!g.ready) {
+        if (Solver::carve_bytes(kStaticBodies, kStaticPairs, kStaticPoints) > kStaticScratchBytes)
+            return PMH_ERROR_CAPACITY_EXCEEDED;
+        PmhResult r = g.solver.carve(g.base, kStaticScratchBytes, kStaticBodies, kStaticPairs, kStaticPoints,
+                                     &g.host_counters, g.host_starts);
+        if (r != PMH_SUCCESS) return r;
+        // The published sweep stamps start below every stamp to come.
+        k_seam_clear<<<std::max<uint32_t>(1, (kStaticBodies + 255) / 256), 256, 0, s>>>(g.solver.done.p,
+                                                                                      kStaticBodies);
+        g.ready = true;
+    }
+    uint32_t M = 0;
+    PMH_CHECK(hipMemcpyAsync(&M, active_pair_count, sizeof M, hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    if (M && (!manifolds || !active_pairs)) return PMH_ERROR_INVALID_ARGUMENT;
+    if (M > kStaticPairs) {
+        std::fprintf(stderr, "pmhPrepareSolveGeometryConstraintsNoAlloc: %u active pairs, at most %u\n", M,
+                     kStaticPairs);
+        return PMH_ERROR_CAPACITY_EXCEEDED;
+    }
+    Solver &v = g.solver;
+    PmhResult r = v.use(body_count, std::max<uint32_t>(M, 1), std::max<uint32_t>(8 * M, 8));
+    if (r != PMH_SUCCESS) return r;
+    // The array keeps the last call's contents: the counters and totals
+    // start from zero.
+    k_seam_clear<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(v.counters.p), sizeof(StepCounters) / 4);
+    k_seam_clear<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(v.totals.p), sizeof(SolveTotals) / 4);
+    return seam_run(v, s, parameters, previous, states, body_count, manifolds, active_pairs, M, bodies, schedule,
+                    timestep, gravity, base_iterations, requested_iterations, profile_version);
 }

@@ -126,6 +126,8 @@ def lib():
         "pmhWorldGetProxies": (i32, [_P, _P]),
         "pmhPrepareSolveGeometryConstraints": (
             i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
+        "pmhPrepareSolveGeometryConstraintsNoAlloc": (
+            i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
         "pmhWorldSetWarmstart": (i32, [_P, _P, _U]),
         "pmhWorldSetContactLog": (i32, [_P, _U]),
         "pmhWorldGetContactLog": (i32, [_P, _P, _U, _P]),
