@@ -87,7 +87,7 @@ def summarize_pmc(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json
 
     with open(bench_json) as f:
         line = json.loads(f.read().strip().splitlines()[-1])
-    kernel = line["roofline"]["kernel"].split(" ")[0]
+    kernel = line["roofline"]["kernel"].split(" (")[0]
     first, last = line["untimed_steps"], line["untimed_steps"] + line["steps"]
 
     def window_mean(root):
@@ -383,7 +383,7 @@ def main():
             kernel = f"k_sweep<{lanes_used}, {joints}> (dataflow colour sweep, one launch per iteration)"
     achieved = per_launch_bytes / per_launch_s / 1e9
     window = (settle, settle + args.steps)
-    traffic = (traffic_from_profiles(window, kernel.split(" ")[0])
+    traffic = (traffic_from_profiles(window, kernel.split(" (")[0])
                if args.workload == "pile" and args.side == 32 else None)
     name, cus = pmamd.device_info()
     line = {
