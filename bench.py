@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--warm-lambda", action="store_true",
                     help="warmstart workload: also run the predicted-lambda mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warmstart", action="store_true",
+                    help="pile: skip the learned-vs-plain warm-start half of the metric")
     ap.add_argument("--cooperative", action="store_true", help="whole solve in one cooperative launch (grid barriers)")
     ap.add_argument("--per-color-sweep", action="store_true",
                     help="one colour-sweep launch per colour per iteration instead of the dataflow sweep")
@@ -477,6 +479,14 @@ def main():
                                                 args.iterations)
     else:
         line["cpu_baseline"] = None
+    if args.workload == "pile" and not args.no_warmstart:
+        # The metric's second half: sim-steps/s to an RMS residual of 1e-4,
+        # learned warm start against plain (the warmstart workload, compact).
+        print("# bench: learned warm start vs plain, to RMS 1e-4", file=sys.stderr, flush=True)
+        ws = warmstart_bench(args)
+        line["warmstart"] = {k: ws[k] for k in ("unit", "plain", "learned", "iteration_ratio", "steps_per_s_ratio")}
+        line["warmstart"]["workload"] = ws["config"]["workload"]
+        line["warmstart"]["training_samples"] = ws["training"]["samples"]
     print(json.dumps(line), flush=True)
 
 

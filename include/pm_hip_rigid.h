@@ -487,6 +487,25 @@ PmhResult pmhPrepareSolveGeometryConstraints(
     const float gravity[3], uint32_t base_iterations,
     uint32_t requested_iterations, uint32_t profile_version);
 
+/* The seam with the simulation's joints: prepare_geometry_constraints's
+ * `constraints` (the reference's RigidConstraintResource records,
+ * PmhRigidConstraint, `constraint_capacity` slots, dead slots not alive) and
+ * `ids` (the id of each dense body, which the joints' body ids resolve
+ * through; rigid_avbd_find_dense, geometry_constraints.slang:318-325), with
+ * the same meaning as in src/simulation/geometry_constraints_dispatch.hpp:
+ * 15-24.  The joints' state, applied impulse, dense ids, rows and list links
+ * are written back; their options are not touched.  No joints
+ * (capacity 0) is pmhPrepareSolveGeometryConstraints. */
+PmhResult pmhPrepareSolveGeometryConstraintsJoints(
+    void *stream, const PmhBodyParameters *parameters,
+    const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds,
+    const uint32_t *active_pairs, const uint32_t *active_pair_count,
+    PmhRigidConstraint *constraints, uint32_t constraint_capacity,
+    const PmhBodyId *ids, PmhAvbdBody *bodies, PmhContactSchedule *schedule,
+    float timestep, const float gravity[3], uint32_t base_iterations,
+    uint32_t requested_iterations, uint32_t profile_version);
+
 /* The same seam without allocating: no counterpart in the reference; for
  * callers that cannot allocate device or pinned memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).

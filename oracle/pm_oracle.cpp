@@ -597,9 +597,58 @@ void clear_manifold(PmoManifold &m) { std::memset(&m, 0, sizeof m); }   // geome
 // rigid_add_contact, geometry.slang:1129-1220, for unguided contacts (no
 // hinge joints exist on this path) and no swept-witness canonicalisation
 // (the convex face path passes false).
+// rigid_guided_pair / rigid_guided_contact, geometry.slang:373-382: an
+// axially guided member (a slider or piston held by a static body) against
+// a static body.
+inline bool guided_pair(const PmoHingeFrame &a, const PmoHingeFrame &b) {
+    return (a.axial != 0u && b.static_body != 0u) || (b.axial != 0u && a.static_body != 0u);
+}
+inline bool guided_contact(const PmoContact &c) { return guided_pair(c.body_hinge, c.collider_hinge); }
+
+// rigid_guide_direction, geometry.slang:1118-1127: the contact's response
+// along the guide's free directions (axial translation; rotation about the
+// axis for a piston), each scaled by the square root of its inverse mass.
+inline V3 guide_direction(const PmoContact &c) {
+    const PmoHingeFrame &f = c.body_hinge.axial != 0u ? c.body_hinge : c.collider_hinge;
+    return {dot(c.normal, f.axis) * std::sqrt(f.inverse_mass),
+            f.axial_rotation != 0u ? dot(f.axis, cross(sub(c.point, f.anchor), c.normal)) * std::sqrt(f.inverse_moment)
+                                   : 0.0f,
+            0.0f};
+}
+
+// rigid_add_contact, geometry.slang:1129-1220.  A guided contact is merged
+// by its guide direction: the first impact keeps its rows (later impacts
+// are dropped, earlier ones replace the swept rows), and a contact whose
+// direction matches a kept one replaces it when it is deeper per unit of
+// direction.
 void add_contact(PmoManifold &m, const PmoContact &c, float spacing) {
+    const bool guided = guided_contact(c);
+    const V3 cand_dir = guided ? guide_direction(c) : V3{0, 0, 0};
+    const float cand_len = guided ? len(cand_dir) : 1.0f;
+    if (guided && cand_len * cand_len <= kEpsilon) return;
+    if (guided && c.impact_fraction > 0.0f) {
+        float first_impact = 1.0f;
+        for (uint32_t r = 0; r < m.count; ++r)
+            if (m.contacts[r].impact_fraction > 0.0f) first_impact = fmin_(first_impact, m.contacts[r].impact_fraction);
+        if (c.impact_fraction > first_impact + 1.0e-4f) return;
+        if (c.impact_fraction < first_impact - 1.0e-4f) {
+            uint32_t kept = 0;
+            for (uint32_t r = 0; r < m.count; ++r)
+                if (m.contacts[r].impact_fraction == 0.0f) m.contacts[kept++] = m.contacts[r];
+            m.count = kept;
+        }
+    }
     float sp2 = spacing * spacing;
     for (uint32_t i = 0; i < m.count; ++i) {
+        if (guided) {
+            const V3 dir = guide_direction(m.contacts[i]);
+            const float size = len(dir);
+            if (dot(cand_dir, dir) > 0.9999f * cand_len * size) {
+                if (c.penetration / cand_len > m.contacts[i].penetration / size) m.contacts[i] = c;
+                return;
+            }
+            continue;
+        }
         if (len2(sub(m.contacts[i].point, c.point)) < sp2) {
             float dd = c.penetration - m.contacts[i].penetration;
             if (dd > 0.0f) m.contacts[i] = c;
@@ -1029,15 +1078,17 @@ V3 triangle_feature_normal(const V3 A[3], const V3 B[3], V3 pa, V3 pb, V3 fallba
     return normal;
 }
 
-// The per-pair inputs of the triangle path.  Hinge frames carry only the
-// static flag: no joints act on this path (rigid_hinge_contact_frame,
-// geometry.slang:404-491, with no incident constraint), and the body
-// reference point is the body position.
+// The per-pair inputs of the triangle path: shapes, previous and predicted
+// states, the hinge contact frames of both bodies at the predicted states
+// (rigid_hinge_contact_frame, geometry.slang:404-491) and the body's
+// reference point at the previous and predicted states (its hinge anchor
+// when it has a non-axial frame, else its position).
 struct TrianglePair {
     const Shape *am, *bm;
     PmoState ap, a, bp, b;          // previous and predicted states
     PmoHingeFrame ha, hb;
     float margin, dt;
+    V3 ref_prev, ref;
 };
 
 inline void world_triangle(const Shape &m, const PmoState &st, uint32_t t, V3 W[3]) {
@@ -1048,9 +1099,10 @@ inline void world_triangle(const Shape &m, const PmoState &st, uint32_t t, V3 W[
 void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count, uint32_t b_first,
                              uint32_t b_count, PmoManifold &m) {
     const Shape &am = *P.am, &bm = *P.bm;
+    if (guided_pair(P.ha, P.hb)) return;
     const float margin = P.margin;
     const float rest = fmin_(margin, 1.0e-3f);
-    const V3 body_ref = P.a.position;
+    const V3 body_ref = P.ref;
     const bool inside_a = P.ha.static_body != 0u && point_inside_convex(am, P.ap, P.bp.position);
     const bool inside_b = P.hb.static_body != 0u && point_inside_convex(bm, P.bp, P.ap.position);
     for (uint32_t ai = 0; ai < a_count; ++ai) {
@@ -1160,13 +1212,25 @@ void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a
     }
 }
 
-// rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154
-// (unguided: no hinge joints act on this path).
+// rigid_triangle_pair_face_contact, geometry.slang:1910-1919: the
+// separation lies along a face normal of either triangle.
+inline bool triangle_pair_face_contact(const V3 A[3], const V3 B[3], V3 separation) {
+    V3 n = normalized_or(separation, {0, 0, 0});
+    V3 fa = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0, 0, 0});
+    V3 fb = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0, 0, 0});
+    return std::fabs(dot(n, fa)) > 0.9999f || std::fabs(dot(n, fb)) > 0.9999f;
+}
+
+// rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154.  A guided
+// pair rests at 0.1 mm, takes edge contacts that both slide along and turn
+// about the guide axis at zero offset, and advances by the faces' closing
+// speed along the feature normal.
 void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first, uint32_t a_count, uint32_t b_first,
                                    uint32_t b_count, PmoManifold &m) {
     const Shape &am = *P.am, &bm = *P.bm;
     const float margin = P.margin;
-    const float rest = fmin_(margin, 1.0e-3f);
+    const bool guided = guided_pair(P.ha, P.hb);
+    const float rest = guided ? fmin_(margin, 1.0e-4f) : fmin_(margin, 1.0e-3f);
     const bool inside_a = P.ha.static_body != 0u && point_inside_convex(am, P.ap, P.bp.position);
     const bool inside_b = P.hb.static_body != 0u && point_inside_convex(bm, P.bp, P.ap.position);
     for (uint32_t ai = 0; ai < a_count; ++ai) {
@@ -1225,12 +1289,23 @@ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first, uint
                 closest_triangle_pair(A, B, pa, pb);
                 V3 delta = sub(pa, pb);
                 float distance = std::sqrt(fmax_(0.0f, len2(delta)));
-                const float offset = rest;
+                float offset = rest;
+                if (guided && !triangle_pair_face_contact(A, B, delta)) {
+                    const PmoHingeFrame &guide = P.ha.axial != 0u ? P.ha : P.hb;
+                    V3 direction = normalized_or(delta, {0, 0, 0});
+                    float axial = dot(direction, guide.axis);
+                    float angular = dot(guide.axis, cross(sub(pa, guide.anchor), direction));
+                    if (std::fabs(axial) > 1.0e-3f && std::fabs(angular) > 1.0e-3f) offset = 0.0f;
+                }
                 if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
                 if (distance <= offset + 1.0e-5f) {
                     V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
                     V3 center = add(P.ap.position, mul(sub(P.a.position, P.ap.position), time));
                     V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+                    if (P.ha.present != 0u) {
+                        V3 reference = add(P.ref_prev, mul(sub(P.ref, P.ref_prev), time));
+                        fallback = dot(cn, sub(reference, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+                    }
                     V3 oa = shell_outward(am, at, A), ob = shell_outward(bm, bt, B);
                     V3 coa = P.ha.static_body != 0u && inside_a ? V3{0, 0, 0} : oa;
                     V3 cob = P.hb.static_body != 0u && inside_b ? V3{0, 0, 0} : ob;
@@ -1258,6 +1333,21 @@ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_first, uint
                 sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
                 have_sep = true;
                 float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+                if (guided) {
+                    V3 plane = triangle_feature_normal(A, B, pa, pb, sep_normal);
+                    float min_a = 3.402823466e+38f, max_b = -3.402823466e+38f;
+                    float min_speed_a = 3.402823466e+38f, max_speed_b = -3.402823466e+38f;
+                    for (uint32_t v = 0; v < 3; ++v) {
+                        min_a = fmin_(min_a, dot(sub(A[v], pb), plane));
+                        max_b = fmax_(max_b, dot(sub(B[v], pb), plane));
+                        min_speed_a = fmin_(min_speed_a, dot(dA[v], plane));
+                        max_speed_b = fmax_(max_speed_b, dot(dB[v], plane));
+                    }
+                    float gap = min_a - max_b - offset;
+                    float closing = max_speed_b - min_speed_a;
+                    if (gap > 1.0e-5f && closing <= 0.0f) break;
+                    if (gap > 0.0f && closing > kEpsilon) advance = fmax_(advance, 0.9f * gap / closing);
+                }
                 time += fmax_(advance, 1.0e-5f);
                 if (time > 1.0f) break;
             }
@@ -1539,8 +1629,12 @@ inline float comp(V3 v, uint32_t a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z)
 inline float pair_stiffness(const PmoParams &a, const PmoParams &b, float dt) {  // :432-436
     return 1.0f / (fmax_(a.inverse_mass + b.inverse_mass, 1.0e-6f) * dt * dt);
 }
-inline float contact_skin(const PmoParams &a, const PmoParams &b) {  // :438-445 (unguided)
-    return fmin_(a.collision_margin + b.collision_margin, kRestOffset) + 2.0f * kSurfaceTolerance;
+constexpr float kGuidedRestOffset = 1.0e-4f;     // geometry_constraints.slang:31
+// rigid_avbd_contact_skin, :438-445: a guided contact rests at 0.1 mm.
+inline float contact_skin(const PmoContact &c, const PmoParams &a, const PmoParams &b) {
+    const float offset = guided_contact(c) ? fmin_(a.collision_margin + b.collision_margin, kGuidedRestOffset)
+                                           : fmin_(a.collision_margin + b.collision_margin, kRestOffset);
+    return offset + 2.0f * kSurfaceTolerance;
 }
 
 PmoState avbd_pose(const PmoAvbdBody &b) {                       // :374-384
@@ -1619,7 +1713,8 @@ struct Ctx {
     std::vector<uint8_t> stick{};        // per point of the last dual update
     // Mistakes planted by tests to show that a pin discriminates (never set
     // otherwise): bit 0 skips the free-motion mean, bit 1 the crossing
-    // safeguard.
+    // safeguard (bit 2, read by the narrowphase, drops the hinge contact
+    // frames).
     uint32_t planted = 0u;
 };
 
@@ -1632,7 +1727,7 @@ bool contact_impact(const Ctx &x, uint32_t active) {
     bool impact = false;
     for (uint32_t p = 0; p < m.count; ++p) {
         const PmoContact &c = m.contacts[p];
-        float skin = contact_skin(x.P[a], x.P[b]);
+        float skin = contact_skin(c, x.P[a], x.P[b]);
         bool fresh = (c.avbd_cached == 0u || c.avbd_dual[0].lambda <= 1.0e-6f ||
                       (c.impact_fraction > 0.0f && c.impact_fraction < 1.0f)) &&
                      c.initial_normal_speed < -0.1f && c.penetration >= -kSurfaceTolerance;
@@ -1949,7 +2044,7 @@ void prepare(Ctx &x, V3 gravity, uint32_t base_iterations, uint32_t requested, u
                 sub(rotate(x.prev[b].orientation, c.avbd_anchor_b), rotate(x.prev[a].orientation, c.avbd_anchor_a)));
             V3 terr = add(anchor_motion, mul(old_delta, 1.0f - kAlpha));
             c.initial_normal_speed = -closing / dt;
-            float skin = contact_skin(x.P[a], x.P[b]);
+            float skin = contact_skin(c, x.P[a], x.P[b]);
             float impact_disp = realtime && c.avbd_cached == 0u && c.initial_normal_speed < -0.1f
                 ? -(1.0f + fmin_(x.P[a].restitution, x.P[b].restitution)) * c.initial_normal_speed * dt
                 : 0.0f;
@@ -2717,6 +2812,88 @@ void find_pairs(World &w, const std::vector<PmoState> &pred) {
 
 uint32_t slot_capacity(const World &w);
 
+// rigid_fixed_hinge_inverse_moment, geometry.slang:384-402: the inverse
+// moment about a guide axis through its anchor (a piston's turn).
+float fixed_hinge_inverse_moment(const PmoParams &p, const PmoState &s, const PmoHingeFrame &f) {
+    if ((f.fixed == 0u && f.axial_rotation == 0u) || p.inverse_mass <= kEpsilon) return 0.0f;
+    V3 la = inv_rotate(s.orientation, f.axis);
+    float center = la.x * la.x / fmax_(p.inverse_inertia_local.x, kEpsilon) +
+                   la.y * la.y / fmax_(p.inverse_inertia_local.y, kEpsilon) +
+                   la.z * la.z / fmax_(p.inverse_inertia_local.z, kEpsilon);
+    V3 arm = sub(s.position, f.anchor);
+    V3 perpendicular = sub(arm, mul(f.axis, dot(arm, f.axis)));
+    float pivot = center + len2(perpendicular) / p.inverse_mass;
+    return pivot > kEpsilon ? 1.0f / pivot : 0.0f;
+}
+
+// rigid_hinge_contact_frame, geometry.slang:404-491: over the body's
+// enabled, unbroken joints in index order, the first hinge (or slider or
+// piston held by a static body with no breaking threshold) gives the
+// frame: its anchor and axis, from the static body when it holds the
+// joint; a slider or piston counts only as the body's single joint.  Fixed
+// and point members are flagged.  `reference` is the anchor of a
+// non-axial frame, else the body's position.  Body ids are {dense index, 1}.
+PmoHingeFrame hinge_frame(const World &w, uint32_t dense, const std::vector<PmoState> &states, const PmoState &state,
+                          V3 &reference) {
+    PmoHingeFrame r;
+    std::memset(&r, 0, sizeof r);
+    const uint32_t n = uint32_t(w.params.size());
+    const PmoParams &pd = w.params[dense];
+    r.static_body = pd.motion == kStatic ? 1u : 0u;
+    if (pd.inverse_mass <= 0.0f) {
+        reference = state.position;
+        return r;
+    }
+    if ((w.desc.flags >> 1) & 4u) {          // planted mistake: joints shape no contact
+        reference = state.position;
+        return r;
+    }
+    auto same = [&](const PmoBodyId &id, uint32_t d) { return id.index == d && id.generation == 1u; };
+    auto find = [&](const PmoBodyId &id) { return id.index < n && id.generation == 1u ? id.index : kInvalid; };
+    uint32_t incident = 0;
+    for (const PmoJoint &j : w.joints) {
+        const PmoJointOptions &o = j.options;
+        if (j.alive == 0u || o.enabled == 0u || j.state_broken != 0u) continue;
+        const bool is_a = same(o.body_a, dense), is_b = same(o.body_b, dense);
+        if (!is_a && !is_b) continue;
+        ++incident;
+        if (o.type == kFixed) r.fixed_member = 1u;
+        if (o.type == kPoint) r.point_member = 1u;
+        const bool axial = o.type == kPiston || o.type == kSlider;
+        const uint32_t other = find(is_a ? o.body_b : o.body_a);
+        const bool static_anchor = other != kInvalid && w.params[other].motion == kStatic;
+        if (r.present != 0u || (o.type != kHinge && !(axial && static_anchor && o.breaking_impulse_threshold <= 0.0f)))
+            continue;
+        const V3 local_axis = axial ? V3{1.0f, 0.0f, 0.0f} : V3{0.0f, 0.0f, 1.0f};
+        r.local_anchor = is_a ? o.local_anchor_a : o.local_anchor_b;
+        const Q local_orientation = is_a ? o.local_orientation_a : o.local_orientation_b;
+        r.anchor = add(state.position, rotate(state.orientation, r.local_anchor));
+        r.axis = normalized_or(rotate(qmul(state.orientation, local_orientation), local_axis), local_axis);
+        r.present = 1u;
+        r.fixed = static_anchor && !axial ? 1u : 0u;
+        r.axial = static_anchor && axial ? 1u : 0u;
+        r.axial_rotation = r.axial != 0u && o.type == kPiston ? 1u : 0u;
+        if (static_anchor) {
+            const V3 other_anchor = is_a ? o.local_anchor_b : o.local_anchor_a;
+            const Q other_orientation = is_a ? o.local_orientation_b : o.local_orientation_a;
+            const PmoState &os = states[other];
+            r.anchor = add(os.position, rotate(os.orientation, other_anchor));
+            r.axis = normalized_or(rotate(qmul(os.orientation, other_orientation), local_axis), r.axis);
+        }
+    }
+    if (r.axial != 0u && incident != 1u) {
+        r.axial = 0u;
+        r.axial_rotation = 0u;
+        r.present = 0u;
+    }
+    if (r.axial != 0u) {
+        r.inverse_mass = pd.inverse_mass;
+        r.inverse_moment = fixed_hinge_inverse_moment(pd, state, r);
+    }
+    reference = r.present != 0u && r.axial == 0u ? r.anchor : state.position;
+    return r;
+}
+
 // Per pair narrowphase, in the reference's dispatch order
 // (production_realtime_backend.cu.inc:7710-7776): leaf-pair candidates
 // (pm_generate_rigid_leaf_pairs, geometry.slang:2303-2492), the convex face
@@ -2742,11 +2919,14 @@ void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<
         const Shape &sa = w.shapes[w.params[a].mesh_index], &sb = w.shapes[w.params[b].mesh_index];
         float margin = w.params[a].collision_margin + w.params[b].collision_margin;
         float cr = correction_reach(w.params[a], sa) + correction_reach(w.params[b], sb);
-        PmoHingeFrame ha, hb;
-        std::memset(&ha, 0, sizeof ha);
-        std::memset(&hb, 0, sizeof hb);
-        ha.static_body = w.params[a].motion == kStatic ? 1u : 0u;
-        hb.static_body = w.params[b].motion == kStatic ? 1u : 0u;
+        // Frames at the predicted states (both bodies) and the body's at
+        // its previous state, for its reference point (geometry.slang:
+        // 2561-2578, 2771-2779, 2834-2846).
+        V3 ref_prev, ref, collider_ref;
+        (void)hinge_frame(w, a, prev, prev[a], ref_prev);
+        const PmoHingeFrame ha = hinge_frame(w, a, pred, pred[a], ref);
+        const PmoHingeFrame hb = hinge_frame(w, b, pred, pred[b], collider_ref);
+        const bool guided = guided_pair(ha, hb);
         PmoManifold m;
         clear_manifold(m);
         if (sa.kind != kMesh || sb.kind != kMesh) {
@@ -2755,9 +2935,13 @@ void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<
                 w.stats.unsupported_pairs++;
             }
         } else {
+            // The leaf-pair candidates use the motion test alone; the
+            // triangle ranges of a guided pair always take the swept pass
+            // (geometry.slang:2350, 2574-2579, 2167-2170).
             bool swept = requires_swept(prev[a], pred[a], sa, prev[b], pred[b], sb, margin);
+            const bool swept_ranges = guided || swept;
             bool face = false;
-            if (convex_face_shape(sa) && convex_face_shape(sb)) {
+            if (convex_face_shape(sa) && convex_face_shape(sb) && !guided) {
                 if (swept) face = swept_convex_face_manifold(prev[a], pred[a], sa, prev[b], pred[b], sb, margin, m);
                 if (!face) {
                     face = convex_face_manifold(pred[a], sa, pred[b], sb, margin, cr, m);
@@ -2766,19 +2950,19 @@ void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<
             }
             if (!face) {
                 clear_manifold(m);
-                TrianglePair P{&sa, &sb, prev[a], pred[a], prev[b], pred[b], ha, hb, margin, w.desc.dt};
+                TrianglePair P{&sa, &sb, prev[a], pred[a], prev[b], pred[b], ha, hb, margin, w.desc.dt, ref_prev, ref};
                 w.stats.triangle_pairs++;
                 if (active < slots && leaf_candidates(P, swept, per_slot, candidates)) {
                     const float spacing = fmax_(margin * 2.0f, 1.0e-4f);
                     PmoManifold local;
                     for (auto &c : candidates) {
                         clear_manifold(local);
-                        collide_leaf_ranges(P, swept, sa.nodes[c.first], sb.nodes[c.second], local);
+                        collide_leaf_ranges(P, swept_ranges, sa.nodes[c.first], sb.nodes[c.second], local);
                         for (uint32_t p = 0; p < local.count; ++p) add_contact(m, local.contacts[p], spacing);
                     }
                 } else {
                     w.stats.fallback_pairs++;
-                    collide_meshes(P, swept, m);
+                    collide_meshes(P, swept_ranges, m);
                 }
             }
         }
@@ -2790,7 +2974,12 @@ void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<
             (w.params[b].motion != kDynamic || sb.kind != kMesh || (sb.has_planes && sb.index_count() <= 96u));
         m.initial_relative_position = sub(pred[a].position, pred[b].position);
         for (uint32_t p = 0; p < m.count; ++p) {
-            m.contacts[p].persistent = persistent ? 1u : 0u;
+            // geometry.slang:2855-2862: guided contacts persist; contacts of
+            // a hinged or fixed member do not.
+            const PmoContact &c = m.contacts[p];
+            m.contacts[p].persistent = guided_contact(c) ||
+                (persistent && c.body_hinge.present == 0u && c.collider_hinge.present == 0u &&
+                 c.body_hinge.fixed_member == 0u && c.collider_hinge.fixed_member == 0u) ? 1u : 0u;
             m.contacts[p].initial_normal_speed = contact_normal_speed(pred[a], pred[b], m.contacts[p].point, m.contacts[p].normal);
             m.contacts[p].reported_normal_impulse = m.contacts[p].accumulated_normal_impulse;
             m.contacts[p].reported_friction_impulse = m.contacts[p].accumulated_friction_impulse;
@@ -2993,6 +3182,47 @@ int pmo_avbd_prepare_solve(const PmoParams *parameters, const PmoState *previous
     if (lanes & (lanes - 1u)) return -1;
     Ctx x{parameters, previous, states, count, manifolds, pairs, pair_count, bodies, schedule, dt, lanes};
     return prepare_solve(x, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations, profile_version);
+}
+
+// The seam with joints (prepare_geometry_constraints' constraints,
+// constraint_capacity and ids, geometry_constraints_dispatch.hpp:15-24):
+// the joints' body ids resolve through `ids` (rigid_avbd_find_dense,
+// geometry_constraints.slang:318-325); the solve runs on a copy whose ids
+// are {dense index, 1}, and the solver-owned fields go back to `joints`.
+int pmo_avbd_prepare_solve_joints(const PmoParams *parameters, const PmoState *previous, PmoState *states,
+                                  uint32_t count, PmoManifold *manifolds, const uint32_t *pairs,
+                                  uint32_t pair_count, PmoJoint *joints, uint32_t joint_count,
+                                  const PmoBodyId *ids, PmoAvbdBody *bodies, PmoSchedule *schedule, float dt,
+                                  const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations,
+                                  uint32_t profile_version, uint32_t lanes) {
+    if (lanes & (lanes - 1u)) return -1;
+    std::vector<PmoJoint> copy(joints, joints + joint_count);
+    for (PmoJoint &j : copy)
+        for (PmoBodyId *id : {&j.options.body_a, &j.options.body_b}) {
+            uint32_t found = kInvalid;
+            for (uint32_t i = 0; i < count; ++i)
+                if (ids[i].index == id->index && ids[i].generation == id->generation) { found = i; break; }
+            *id = found == kInvalid ? PmoBodyId{kInvalid, 0u} : PmoBodyId{found, 1u};
+        }
+    Ctx x{parameters, previous, states, count, manifolds, pairs, pair_count, bodies, schedule, dt, lanes};
+    x.J = copy.data();
+    x.nj = joint_count;
+    const int r = prepare_solve(x, {gravity[0], gravity[1], gravity[2]}, base_iterations, requested_iterations,
+                                profile_version);
+    for (uint32_t k = 0; k < joint_count; ++k) {
+        PmoJoint &d = joints[k];
+        const PmoJoint &s = copy[k];
+        d.state_enabled = s.state_enabled;
+        d.state_broken = s.state_broken;
+        d.applied_impulse = s.applied_impulse;
+        d.dense_a = s.dense_a;
+        d.dense_b = s.dense_b;
+        std::memcpy(d.rows, s.rows, sizeof d.rows);
+        d.avbd_count = s.avbd_count;
+        d.avbd_next_a = s.avbd_next_a;
+        d.avbd_next_b = s.avbd_next_b;
+    }
+    return r;
 }
 
 // The realtime contract: a 1/60 s step within one float ulp and one

@@ -193,6 +193,14 @@ int pmo_avbd_prepare_solve(const PmoParams *parameters,
                            uint32_t base_iterations,
                            uint32_t requested_iterations,
                            uint32_t profile_version, uint32_t lanes);
+/* The seam with joints: the joints' body ids resolve through ids[dense];
+ * their solver-owned fields are written back. */
+int pmo_avbd_prepare_solve_joints(const PmoParams *parameters, const PmoState *previous, PmoState *states,
+                                  uint32_t count, PmoManifold *manifolds, const uint32_t *pairs,
+                                  uint32_t pair_count, PmoJoint *joints, uint32_t joint_count,
+                                  const PmoBodyId *ids, PmoAvbdBody *bodies, PmoSchedule *schedule, float dt,
+                                  const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations,
+                                  uint32_t profile_version, uint32_t lanes);
 
 /* ---- world ------------------------------------------------------------ */
 typedef struct PmoWorldDesc {
@@ -203,8 +211,9 @@ typedef struct PmoWorldDesc {
     uint32_t requested_iterations;
     uint32_t body_capacity;        /* 0: the body count at first step */
     uint32_t lanes;                /* accumulation order, see above */
-    uint32_t flags;                /* bit 0: brute-force broadphase; bits 1-2: mistakes planted by tests
-                                      (1: no free-motion mean, 2: no crossing safeguard) */
+    uint32_t flags;                /* bit 0: brute-force broadphase; bits 1-3: mistakes planted by tests
+                                      (1: no free-motion mean, 2: no crossing safeguard, 4: no hinge
+                                      contact frames) */
     float residual_tolerance;      /* > 0: iterate each phase until the constraint residual
                                       is at most this (the budget caps); 0: fixed budget */
     uint32_t substeps;             /* substeps per step of dt / substeps (0 = 1), as

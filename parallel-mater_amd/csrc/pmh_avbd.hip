@@ -104,8 +104,13 @@ __device__ __forceinline__ PmhQuat pq(Q q) { return {q.x, q.y, q.z, q.w}; }
 __device__ __forceinline__ float pair_stiffness(const PmhBodyParameters &a, const PmhBodyParameters &b, float dt) {
     return 1.0f / (smax(a.inverse_mass + b.inverse_mass, 1.0e-6f) * dt * dt);
 }
-__device__ __forceinline__ float contact_skin(const PmhBodyParameters &a, const PmhBodyParameters &b) {
-    return smin(a.collision_margin + b.collision_margin, kRestOffset) + 2.0f * kSurfaceTolerance;
+// rigid_avbd_contact_skin (geometry_constraints.slang:438-445): a guided
+// contact (point flag 8, set by the narrowphase) rests at 0.1 mm.
+constexpr float kGuidedRestOffset = 1.0e-4f;
+constexpr uint32_t kPointGuided = 8u;
+__device__ __forceinline__ float contact_skin(const PmhBodyParameters &a, const PmhBodyParameters &b, bool guided) {
+    return smin(a.collision_margin + b.collision_margin, guided ? kGuidedRestOffset : kRestOffset) +
+           2.0f * kSurfaceTolerance;
 }
 
 // ---------------------------------------------------------------------
@@ -280,7 +285,6 @@ __global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *
         const PmhBodyParameters pa = P[p.x], pb = P[p.y];
         const float penalty = pair_stiffness(pa, pb, dt);
         const PmhRigidBodyState A0 = prev[p.x], B0 = prev[p.y], A = st[p.x], B = st[p.y];
-        const float skin = contact_skin(pa, pb);
         bool impact = false;
         const uint32_t start = m_start[m], count = m_count[m];
         for (uint32_t k = 0; k < count; ++k) {
@@ -290,6 +294,7 @@ __global__ void k_contact_init(const uint32_t *npairs, const PmhBodyParameters *
             // on has none.
             c_entry[2 * c] = c_entry[2 * c + 1] = kInvalid;
             const bool cached = (c_flags[c] & 1u) != 0u;
+            const float skin = contact_skin(pa, pb, (c_flags[c] & kPointGuided) != 0u);
             float4 np = c_np[c], pt = c_pt[c];
             const V3 normal = v3(np), point = v3(pt);
             V3 aa = v3(c_aa[c]), bb = v3(c_ab[c]);
@@ -2141,8 +2146,9 @@ __global__ void k_joint_init(SolveArgs a) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= a.nj) return;
     PmhRigidConstraint &j = a.joints[k];
+    // Links stay as they were for a joint without rows this step (the
+    // reference rewrites them only for listed joints).
     j.avbd_count = 0u;
-    j.avbd_next_a = j.avbd_next_b = kInvalid;
     if (j.alive == 0u) return;
     j.state_enabled = j.options.enabled != 0u && j.state_broken == 0u ? 1u : 0u;
     if (j.state_enabled == 0u) return;
@@ -2343,6 +2349,51 @@ PmhResult report_sweep_failure(hipStream_t s, StepCounters *ctr) {
     PMH_CHECK(hipMemcpyAsync(&ctr->sweep_failed, &zero, sizeof zero, hipMemcpyHostToDevice, s));
     PmhResult r = sync_stream(s);
     return r != PMH_SUCCESS ? r : PMH_ERROR_INVALID_STATE;
+}
+
+void build_joint_lists(const PmhRigidConstraint *joints, uint32_t nj, uint32_t n, JointLists &out) {
+    auto dense = [&](PmhBodyId id) { return id.index < n && id.generation == 1u ? id.index : kInvalid; };
+    out.start.assign(n + 1, 0u);
+    for (uint32_t k = 0; k < nj; ++k) {
+        const PmhRigidConstraint &j = joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
+        out.start[a]++;
+        out.start[b]++;
+    }
+    uint32_t total = 0;
+    for (uint32_t i = 0; i <= n; ++i) { const uint32_t c = out.start[i]; out.start[i] = total; total += c; }
+    out.list.assign(std::max<uint32_t>(1, total), 0u);
+    std::vector<uint32_t> cursor(out.start.begin(), out.start.end());
+    for (uint32_t k = nj; k-- > 0;) {
+        const PmhRigidConstraint &j = joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
+        out.list[cursor[a]++] = k;
+        out.list[cursor[b]++] = k;
+    }
+    out.heavy.clear();
+    for (uint32_t i = 0; i < n; ++i)
+        if (out.start[i + 1] - out.start[i] > 64u) out.heavy.push_back(i);
+    std::vector<uint32_t> body_level(n, 0u), hinge_level, hinge_ids;
+    uint32_t levels = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        const PmhRigidConstraint &j = joints[k];
+        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
+        if (j.alive == 0u || j.options.type != kJointHinge || a == kInvalid || b == kInvalid) continue;
+        const uint32_t level = std::max(body_level[a], body_level[b]) + 1u;
+        body_level[a] = body_level[b] = level;
+        hinge_ids.push_back(k);
+        hinge_level.push_back(level);
+        levels = std::max(levels, level);
+    }
+    std::vector<uint32_t> level_start(levels + 2u, 0u);
+    out.hinges.assign(hinge_ids.size(), 0u);
+    for (uint32_t level : hinge_level) level_start[level + 1u]++;
+    for (uint32_t l = 1; l <= levels + 1u; ++l) level_start[l] += level_start[l - 1];
+    std::vector<uint32_t> cur(level_start.begin(), level_start.end());
+    for (size_t h = 0; h < hinge_ids.size(); ++h) out.hinges[cur[hinge_level[h]]++] = hinge_ids[h];
+    out.level_start.assign(level_start.begin() + 1, level_start.end());
 }
 
 PmhResult status_result(uint32_t status) {

@@ -476,6 +476,185 @@ __device__ void add_contact(Manifold &m, const Contact &c, float spacing) {
     if (c.pen > m.c[sh].pen) m.c[sh] = c;
 }
 
+// ---- hinge contact frames (joints that shape contacts) ------------------
+using HingeFrame = PmhHingeContactFrame;
+
+__device__ __forceinline__ bool guided_pair(const HingeFrame &a, const HingeFrame &b) {   // geometry.slang:373-377
+    return (a.axial != 0u && b.static_body != 0u) || (b.axial != 0u && a.static_body != 0u);
+}
+
+// rigid_guide_direction, geometry.slang:1118-1127 (`g` is the pair's axial frame).
+__device__ __forceinline__ V3 guide_direction(const Contact &c, const HingeFrame &g) {
+    return {dot(c.normal, sv(g.axis)) * sqrtf(g.inverse_mass),
+            g.axial_rotation != 0u ? dot(sv(g.axis), cross(sub(c.point, sv(g.anchor)), c.normal)) * sqrtf(g.inverse_moment)
+                                   : 0.0f,
+            0.0f};
+}
+
+// rigid_add_contact's guided branch, geometry.slang:1129-1168: the first
+// impact keeps its rows (later impacts are dropped, earlier ones replace
+// the swept rows), and a contact whose guide direction matches a kept one
+// replaces it when it is deeper per unit of direction.  All contacts of a
+// pair share its frames, so `g` serves for both.
+__device__ void add_contact_guided(Manifold &m, const Contact &c, const HingeFrame &g) {
+    const V3 cand_dir = guide_direction(c, g);
+    const float cand_len = len(cand_dir);
+    if (cand_len * cand_len <= kEpsilon) return;
+    if (c.impact_fraction > 0.0f) {
+        float first_impact = 1.0f;
+        for (uint32_t r = 0; r < m.count; ++r)
+            if (m.c[r].impact_fraction > 0.0f) first_impact = smin(first_impact, m.c[r].impact_fraction);
+        if (c.impact_fraction > first_impact + 1.0e-4f) return;
+        if (c.impact_fraction < first_impact - 1.0e-4f) {
+            uint32_t kept = 0;
+            for (uint32_t r = 0; r < m.count; ++r)
+                if (m.c[r].impact_fraction == 0.0f) m.c[kept++] = m.c[r];
+            m.count = kept;
+        }
+    }
+    for (uint32_t i = 0; i < m.count; ++i) {
+        const V3 dir = guide_direction(m.c[i], g);
+        const float size = len(dir);
+        if (dot(cand_dir, dir) > 0.9999f * cand_len * size) {
+            if (c.pen / cand_len > m.c[i].pen / size) m.c[i] = c;
+            return;
+        }
+    }
+    if (m.count < 8u) { m.c[m.count++] = c; return; }
+    uint32_t sh = 0;
+    for (uint32_t i = 1; i < m.count; ++i) if (m.c[i].pen < m.c[sh].pen) sh = i;
+    if (c.pen > m.c[sh].pen) m.c[sh] = c;
+}
+
+// rigid_fixed_hinge_inverse_moment, geometry.slang:384-402.
+__device__ float fixed_hinge_inverse_moment(const PmhBodyParameters &p, const PmhRigidBodyState &s,
+                                            const HingeFrame &f) {
+    if ((f.fixed == 0u && f.axial_rotation == 0u) || p.inverse_mass <= kEpsilon) return 0.0f;
+    const V3 la = inv_rotate(sq(s.orientation), sv(f.axis));
+    const float center = la.x * la.x / smax(p.inverse_inertia_local.x, kEpsilon) +
+                         la.y * la.y / smax(p.inverse_inertia_local.y, kEpsilon) +
+                         la.z * la.z / smax(p.inverse_inertia_local.z, kEpsilon);
+    const V3 arm = sub(sv(s.position), sv(f.anchor));
+    const V3 perpendicular = sub(arm, mul(sv(f.axis), dot(arm, sv(f.axis))));
+    const float pivot = center + len2(perpendicular) / p.inverse_mass;
+    return pivot > kEpsilon ? 1.0f / pivot : 0.0f;
+}
+
+// The world's joints as the narrowphase sees them: the records and each
+// body's candidate joints (alive, both ids valid) in descending index.
+struct JointView {
+    const PmhRigidConstraint *joints = nullptr;
+    const uint32_t *list_start = nullptr, *list = nullptr;
+};
+
+// rigid_hinge_contact_frame, geometry.slang:404-491: over the body's
+// enabled, unbroken joints in ascending index, the first hinge (or slider
+// or piston held by a static body with no breaking threshold) gives the
+// frame, anchored on the static body when that holds it; a slider or
+// piston counts only as the body's single joint; fixed and point members
+// are flagged.  `reference` is the anchor of a non-axial frame, else the
+// body's position.  Body ids are {dense index, 1}.
+__device__ HingeFrame hinge_frame(const JointView &jv, const PmhBodyParameters *P, uint32_t n,
+                                  const PmhRigidBodyState *states, uint32_t dense, const PmhRigidBodyState &state,
+                                  V3 &reference) {
+    HingeFrame r;
+    memset(&r, 0, sizeof r);
+    const PmhBodyParameters pd = P[dense];
+    r.static_body = pd.motion == kStatic ? 1u : 0u;
+    if (pd.inverse_mass <= 0.0f || !jv.joints) {
+        reference = sv(state.position);
+        return r;
+    }
+    uint32_t incident = 0;
+    for (uint32_t e = jv.list_start[dense + 1]; e > jv.list_start[dense]; --e) {
+        const PmhRigidConstraint &j = jv.joints[jv.list[e - 1]];
+        const PmhRigidConstraintOptions &o = j.options;
+        if (j.alive == 0u || o.enabled == 0u || j.state_broken != 0u) continue;
+        const bool is_a = o.body_a.index == dense && o.body_a.generation == 1u;
+        const bool is_b = o.body_b.index == dense && o.body_b.generation == 1u;
+        if (!is_a && !is_b) continue;
+        ++incident;
+        if (o.type == kJointFixed) r.fixed_member = 1u;
+        if (o.type == kJointPoint) r.point_member = 1u;
+        const bool axial = o.type == kJointPiston || o.type == kJointSlider;
+        const PmhBodyId oid = is_a ? o.body_b : o.body_a;
+        const uint32_t other = oid.index < n && oid.generation == 1u ? oid.index : kInvalid;
+        const bool static_anchor = other != kInvalid && P[other].motion == kStatic;
+        if (r.present != 0u ||
+            (o.type != kJointHinge && !(axial && static_anchor && o.breaking_impulse_threshold <= 0.0f)))
+            continue;
+        const V3 local_axis = axial ? V3{1.0f, 0.0f, 0.0f} : V3{0.0f, 0.0f, 1.0f};
+        r.local_anchor = is_a ? o.local_anchor_a : o.local_anchor_b;
+        const Q lo = sq(is_a ? o.local_orientation_a : o.local_orientation_b);
+        const Q so = sq(state.orientation);
+        V3 anchor = add(sv(state.position), rotate(so, sv(r.local_anchor)));
+        V3 axis = normalized_or(rotate(qmul(so, lo), local_axis), local_axis);
+        r.present = 1u;
+        r.fixed = static_anchor && !axial ? 1u : 0u;
+        r.axial = static_anchor && axial ? 1u : 0u;
+        r.axial_rotation = r.axial != 0u && o.type == kJointPiston ? 1u : 0u;
+        if (static_anchor) {
+            const PmhRigidBodyState os = states[other];
+            const Q oo = sq(os.orientation);
+            anchor = add(sv(os.position), rotate(oo, sv(is_a ? o.local_anchor_b : o.local_anchor_a)));
+            axis = normalized_or(rotate(qmul(oo, sq(is_a ? o.local_orientation_b : o.local_orientation_a)), local_axis),
+                                 axis);
+        }
+        r.anchor = pv(anchor);
+        r.axis = pv(axis);
+    }
+    if (r.axial != 0u && incident != 1u) {
+        r.axial = 0u;
+        r.axial_rotation = 0u;
+        r.present = 0u;
+    }
+    if (r.axial != 0u) {
+        r.inverse_mass = pd.inverse_mass;
+        r.inverse_moment = fixed_hinge_inverse_moment(pd, state, r);
+    }
+    reference = r.present != 0u && r.axial == 0u ? sv(r.anchor) : sv(state.position);
+    return r;
+}
+
+// Per body: its frame at the predicted state and its reference point at the
+// previous and the predicted state (.xyz of ref[2i], ref[2i + 1]).  Worlds
+// without joints skip this: every frame then carries only the static flag.
+__global__ void k_hinge_frames(uint32_t n, const PmhBodyParameters *P, const PmhRigidBodyState *prev,
+                               const PmhRigidBodyState *pred, JointView jv, HingeFrame *frames, float4 *ref) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3 rp, rc;
+    (void)hinge_frame(jv, P, n, prev, i, prev[i], rp);
+    frames[i] = hinge_frame(jv, P, n, pred, i, pred[i], rc);
+    ref[2 * i] = f4(rp, 0.0f);
+    ref[2 * i + 1] = f4(rc, 0.0f);
+}
+
+// The frames of one pair, and the body's reference points.
+struct PairFrames {
+    HingeFrame ha, hb;
+    V3 ref_prev, ref;
+};
+__device__ __forceinline__ PairFrames pair_frames(const HingeFrame *frames, const float4 *ref, const PmhBodyParameters &pa,
+                                                  const PmhBodyParameters &pb, uint2 p, const PmhRigidBodyState &A0,
+                                                  const PmhRigidBodyState &A) {
+    PairFrames f;
+    if (frames) {
+        f.ha = frames[p.x];
+        f.hb = frames[p.y];
+        f.ref_prev = v3(ref[2 * p.x]);
+        f.ref = v3(ref[2 * p.x + 1]);
+    } else {
+        memset(&f.ha, 0, sizeof f.ha);
+        memset(&f.hb, 0, sizeof f.hb);
+        f.ha.static_body = pa.motion == kStatic ? 1u : 0u;
+        f.hb.static_body = pb.motion == kStatic ? 1u : 0u;
+        f.ref_prev = sv(A0.position);
+        f.ref = sv(A.position);
+    }
+    return f;
+}
+
 // rigid_reduce_face_patch, geometry.slang:1228-1270.
 __device__ void reduce_face_patch(Manifold &m, float tol) {
     const float t2 = tol * tol;
@@ -980,13 +1159,13 @@ __device__ __noinline__ V3 triangle_feature_normal(const V3 *A, const V3 *B, V3 
     return normal;
 }
 
-// Inputs of one pair on the triangle path.  Without joints the hinge
-// frames carry only the static flag and the body reference is the body
-// position (rigid_hinge_contact_frame, geometry.slang:404-491).
+// Inputs of one pair on the triangle path: shapes, previous and predicted
+// states, the pair's hinge contact frames and the body's reference points
+// (rigid_hinge_contact_frame, geometry.slang:404-491).
 struct TrianglePair {
     ShapeView am, bm;
     PmhRigidBodyState ap, a, bp, b;   // previous and predicted states
-    uint32_t a_static, b_static;
+    PairFrames f;
     float margin, dt;
 };
 
@@ -1001,7 +1180,7 @@ __device__ __noinline__ bool triangle_pair_contact(const TrianglePair &P, uint32
                                                    bool inside_b, Contact &c) {
     const float margin = P.margin;
     const float rest = smin(margin, 1.0e-3f);
-    const V3 body_ref = sv(P.a.position);
+    const V3 body_ref = P.f.ref;
     V3 A[3], B[3];
     world_triangle(P.am, P.a, at, A);
     world_triangle(P.bm, P.b, bt, B);
@@ -1012,8 +1191,8 @@ __device__ __noinline__ bool triangle_pair_contact(const TrianglePair &P, uint32
         if (!boxes_touch(amin, amax, bmin, bmax)) return false;
     }
     const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
-    const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
-    const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
+    const V3 coa = P.f.ha.static_body != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
+    const V3 cob = P.f.hb.static_body != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
     if (dot(oa, ob) > 0.9999f) return false;
     V3 pa, pb;
     closest_triangle_pair(A, B, pa, pb);
@@ -1091,7 +1270,7 @@ __device__ __noinline__ bool triangle_pair_contact(const TrianglePair &P, uint32
         }
     }
     if (distance <= 1.0e-5f) {
-        if (!recovered && on_face) {   // no fixed joints act on this path
+        if (!recovered && (P.f.ha.fixed_member != 0u || P.f.hb.fixed_member != 0u || on_face)) {
             const float depth = smax(0.0f, -smin(dot(sub(A[0], pb), normal),
                                                  smin(dot(sub(A[1], pb), normal), dot(sub(A[2], pb), normal))));
             c.pen = smin(margin, depth + rest) + 1.0e-5f;
@@ -1104,13 +1283,24 @@ __device__ __noinline__ bool triangle_pair_contact(const TrianglePair &P, uint32
     return true;
 }
 
+// rigid_triangle_pair_face_contact, geometry.slang:1910-1919.
+__device__ __forceinline__ bool triangle_pair_face_contact(const V3 *A, const V3 *B, V3 separation) {
+    const V3 n = normalized_or(separation, {0.0f, 0.0f, 0.0f});
+    const V3 fa = normalized_or(cross(sub(A[1], A[0]), sub(A[2], A[0])), {0.0f, 0.0f, 0.0f});
+    const V3 fb = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 0.0f, 0.0f});
+    return fabsf(dot(n, fa)) > 0.9999f || fabsf(dot(n, fb)) > 0.9999f;
+}
+
 // The swept pass for one triangle pair (conservative advancement, the body
-// of rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154, for
-// unguided pairs): at most one contact.
+// of rigid_collide_triangle_ranges_swept, geometry.slang:1921-2154): at
+// most one contact.  A guided pair rests at 0.1 mm, takes edge contacts that
+// both slide along and turn about the guide axis at zero offset, and
+// advances by the faces' closing speed along the feature normal.
 __device__ __noinline__ bool triangle_pair_swept_contact(const TrianglePair &P, uint32_t at, uint32_t bt,
                                                          bool inside_a, bool inside_b, Contact &c) {
     const float margin = P.margin;
-    const float rest = smin(margin, 1.0e-3f);
+    const bool guided = guided_pair(P.f.ha, P.f.hb);
+    const float rest = guided ? smin(margin, 1.0e-4f) : smin(margin, 1.0e-3f);
     V3 pA[3], cA[3], pB[3], cB[3];
     world_triangle(P.am, P.ap, at, pA);
     world_triangle(P.am, P.a, at, cA);
@@ -1160,15 +1350,26 @@ __device__ __noinline__ bool triangle_pair_swept_contact(const TrianglePair &P, 
         closest_triangle_pair(A, B, pa, pb);
         const V3 delta = sub(pa, pb);
         const float distance = sqrtf(smax(0.0f, len2(delta)));
-        const float offset = rest;
+        float offset = rest;
+        if (guided && !triangle_pair_face_contact(A, B, delta)) {
+            const HingeFrame &guide = P.f.ha.axial != 0u ? P.f.ha : P.f.hb;
+            const V3 direction = normalized_or(delta, {0.0f, 0.0f, 0.0f});
+            const float axial = dot(direction, sv(guide.axis));
+            const float angular = dot(sv(guide.axis), cross(sub(pa, sv(guide.anchor)), direction));
+            if (fabsf(axial) > 1.0e-3f && fabsf(angular) > 1.0e-3f) offset = 0.0f;
+        }
         if (it == 0u) starts_near = distance <= offset + 5.0e-5f;
         if (distance <= offset + 1.0e-5f) {
             const V3 cn = normalized_or(cross(sub(B[1], B[0]), sub(B[2], B[0])), {0.0f, 1.0f, 0.0f});
             const V3 center = add(sv(P.ap.position), mul(sub(sv(P.a.position), sv(P.ap.position)), time));
             V3 fallback = dot(cn, sub(center, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+            if (P.f.ha.present != 0u) {
+                const V3 reference = add(P.f.ref_prev, mul(sub(P.f.ref, P.f.ref_prev), time));
+                fallback = dot(cn, sub(reference, pb)) >= 0.0f ? cn : mul(cn, -1.0f);
+            }
             const V3 oa = shell_outward(P.am, at, A), ob = shell_outward(P.bm, bt, B);
-            const V3 coa = P.a_static != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
-            const V3 cob = P.b_static != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
+            const V3 coa = P.f.ha.static_body != 0u && inside_a ? V3{0.0f, 0.0f, 0.0f} : oa;
+            const V3 cob = P.f.hb.static_body != 0u && inside_b ? V3{0.0f, 0.0f, 0.0f} : ob;
             if (dot(oa, ob) > 0.9999f) return false;
             if (len2(cob) > 0.5f) fallback = cob;
             else if (len2(coa) > 0.5f) fallback = mul(coa, -1.0f);
@@ -1186,7 +1387,22 @@ __device__ __noinline__ bool triangle_pair_swept_contact(const TrianglePair &P, 
         }
         sep_normal = triangle_feature_normal(A, B, pa, pb, normalized_or(delta, {0.0f, 1.0f, 0.0f}));
         have_sep = true;
-        const float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+        float advance = (distance - offset) / (bound + kEpsilon) * 0.9f;
+        if (guided) {
+            const V3 plane = triangle_feature_normal(A, B, pa, pb, sep_normal);
+            float min_a = 3.402823466e+38f, max_b = -3.402823466e+38f;
+            float min_speed_a = 3.402823466e+38f, max_speed_b = -3.402823466e+38f;
+            for (uint32_t q = 0; q < 3; ++q) {
+                min_a = smin(min_a, dot(sub(A[q], pb), plane));
+                max_b = smax(max_b, dot(sub(B[q], pb), plane));
+                min_speed_a = smin(min_speed_a, dot(dA[q], plane));
+                max_speed_b = smax(max_speed_b, dot(dB[q], plane));
+            }
+            const float gap = min_a - max_b - offset;
+            const float closing = max_speed_b - min_speed_a;
+            if (gap > 1.0e-5f && closing <= 0.0f) return false;
+            if (gap > 0.0f && closing > kEpsilon) advance = smax(advance, 0.9f * gap / closing);
+        }
         time += smax(advance, 1.0e-5f);
         if (time > 1.0f) return false;
     }
@@ -1194,13 +1410,20 @@ __device__ __noinline__ bool triangle_pair_swept_contact(const TrianglePair &P, 
 }
 
 __device__ __forceinline__ void inside_flags(const TrianglePair &P, bool &inside_a, bool &inside_b) {
-    inside_a = P.a_static != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
-    inside_b = P.b_static != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+    inside_a = P.f.ha.static_body != 0u && point_inside_convex(P.am, P.ap, sv(P.bp.position));
+    inside_b = P.f.hb.static_body != 0u && point_inside_convex(P.bm, P.bp, sv(P.ap.position));
+}
+
+// The merge rule of the pair: guided pairs merge by guide direction.
+__device__ __forceinline__ void merge_contact(const TrianglePair &P, Manifold &m, const Contact &c, float spacing) {
+    if (guided_pair(P.f.ha, P.f.hb)) add_contact_guided(m, c, P.f.ha.axial != 0u ? P.f.ha : P.f.hb);
+    else add_contact(m, c, spacing);
 }
 
 // rigid_collide_triangle_ranges: triangles in order, body-major.
 __device__ void collide_triangle_ranges(const TrianglePair &P, uint32_t a_first, uint32_t a_count,
                                         uint32_t b_first, uint32_t b_count, Manifold &m) {
+    if (guided_pair(P.f.ha, P.f.hb)) return;   // geometry.slang:1371
     bool inside_a, inside_b;
     inside_flags(P, inside_a, inside_b);
     const float spacing = smax(2.0f * P.margin, 1.0e-4f);
@@ -1228,7 +1451,7 @@ __device__ void collide_triangle_ranges_swept(const TrianglePair &P, uint32_t a_
             const uint32_t bt = b_first + bi;
             if (3u * bt + 2u >= P.bm.s.idx_count) break;
             Contact c;
-            if (triangle_pair_swept_contact(P, at, bt, inside_a, inside_b, c)) add_contact(m, c, spacing);
+            if (triangle_pair_swept_contact(P, at, bt, inside_a, inside_b, c)) merge_contact(P, m, c, spacing);
         }
     }
 }
@@ -1377,8 +1600,16 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t cou
 // pm_finalize_rigid_contact_manifolds (geometry.slang:2805-2873): the
 // initial relative position, persistence and the initial normal speed, and
 // the point range.  Where the range lies does not affect any result.
+// Guided contacts persist; contacts of a hinged or fixed member do not
+// (:2855-2862).  Point flags: 2 persistent, 8 guided (the solver's rest
+// offset).
+constexpr uint32_t kPointGuided = 8u;
 __device__ void emit_manifold(const NarrowOut &o, uint32_t m, Manifold &man, const PmhRigidBodyState &A,
-                              const PmhRigidBodyState &B, bool persistent) {
+                              const PmhRigidBodyState &B, bool persistent_shapes, const HingeFrame &ha,
+                              const HingeFrame &hb) {
+    const bool guided = guided_pair(ha, hb);
+    const bool persistent = guided || (persistent_shapes && ha.present == 0u && hb.present == 0u &&
+                                       ha.fixed_member == 0u && hb.fixed_member == 0u);
     uint32_t start = wave_reserve(&o.ctr->point_alloc, man.count);
     if (man.count) {
         if (start + man.count > o.point_cap) {
@@ -1404,7 +1635,7 @@ __device__ void emit_manifold(const NarrowOut &o, uint32_t m, Manifold &man, con
         o.c_pen[i] = z;
         o.c_acc[i] = z;
         o.c_rep[i] = z;
-        o.c_flags[i] = persistent ? 2u : 0u;
+        o.c_flags[i] = (persistent ? 2u : 0u) | (guided ? kPointGuided : 0u);
         o.c_pair[i] = m;
     }
 }
@@ -1421,7 +1652,8 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
                                                      const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
                                                      const float4 *planes, const PmhRigidBodyState *prev,
                                                      const PmhRigidBodyState *pred, const uint2 *ab, NarrowOut o,
-                                                     uint32_t *tri_list) {
+                                                     uint32_t *tri_list, const HingeFrame *frames,
+                                                     const float4 *frame_ref) {
     const uint32_t M = *npairs;
     uint32_t unsupported = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
@@ -1432,6 +1664,7 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
         const PmhRigidBodyState A0 = prev[p.x], A = pred[p.x], B0 = prev[p.y], B = pred[p.y];
         const float margin = pa.collision_margin + pb.collision_margin;
         const float cr = correction_reach(pa, sa.s) + correction_reach(pb, sb.s);
+        const PairFrames fr = pair_frames(frames, frame_ref, pa, pb, p, A0, A);
         Manifold man;
         man.count = 0;
         man.face_patch = 0;
@@ -1443,7 +1676,8 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
             }
         } else {
             bool face = false;
-            if (sa.s.convex_face && sb.s.convex_face) {
+            // A guided pair leaves the face path (geometry.slang:2779).
+            if (sa.s.convex_face && sb.s.convex_face && !guided_pair(fr.ha, fr.hb)) {
                 const bool swept = requires_swept(A0, A, sa.s, B0, B, sb.s, margin);
                 if (swept) face = swept_convex_face_manifold(A0, A, sa, B0, B, sb, margin, man);
                 if (!face) {
@@ -1456,7 +1690,7 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
                 continue;
             }
         }
-        emit_manifold(o, m, man, A, B, persistent_pair(sa.s, sb.s, pb));
+        emit_manifold(o, m, man, A, B, persistent_pair(sa.s, sb.s, pb), fr.ha, fr.hb);
     }
     if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
 }
@@ -1580,13 +1814,15 @@ __device__ uint32_t traversal_list(const TrianglePair &P, bool swept, uint2 *lis
 }
 
 // Serial evaluation for the rare pairs whose leaf list does not fit in
-// LDS: the same candidate order and merge rule on one lane.
-__device__ void triangle_pair_serial(const TrianglePair &tp, bool swept, bool leaf_mode, Manifold &man,
-                                     Manifold &local) {
+// LDS: the same candidate order and merge rule on one lane.  Leaf
+// candidates come from the motion test alone (`swept`); the ranges of a
+// guided pair always take the swept pass (`swept_eval`).
+__device__ void triangle_pair_serial(const TrianglePair &tp, bool swept, bool swept_eval, bool leaf_mode,
+                                     Manifold &man, Manifold &local) {
     man.count = 0;
     man.face_patch = 0;
     if (!leaf_mode) {
-        collide_meshes(tp, swept, man);
+        collide_meshes(tp, swept_eval, man);
         return;
     }
     const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
@@ -1600,8 +1836,8 @@ __device__ void triangle_pair_serial(const TrianglePair &tp, bool swept, bool le
             if (!leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nodes)) continue;
             local.count = 0;
             local.face_patch = 0;
-            collide_leaf_ranges(tp, swept, tp.am.node(nodes.x), tp.bm.node(nodes.y), local);
-            for (uint32_t k = 0; k < local.count; ++k) add_contact(man, local.c[k], spacing);
+            collide_leaf_ranges(tp, swept_eval, tp.am.node(nodes.x), tp.bm.node(nodes.y), local);
+            for (uint32_t k = 0; k < local.count; ++k) merge_contact(tp, man, local.c[k], spacing);
         }
 }
 
@@ -1622,7 +1858,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                                                         const PmhRigidBodyState *prev,
                                                         const PmhRigidBodyState *pred, const uint2 *ab,
                                                         NarrowOut o, uint32_t slots, uint32_t limit, float dt,
-                                                        SceneRules scenes) {
+                                                        SceneRules scenes, const HingeFrame *frames,
+                                                        const float4 *frame_ref) {
     __shared__ TriWaveShared shared_waves[kTriWaves];
     const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     TriWaveShared &S = shared_waves[wv];
@@ -1639,11 +1876,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             shared_pair = TrianglePair{ShapeView{shapes[pa.mesh_index], verts, idx, planes, nodes, leaves, shell},
                                        ShapeView{shapes[pb.mesh_index], verts, idx, planes, nodes, leaves, shell},
                                        prev[p.x], pred[p.x], prev[p.y], pred[p.y],
-                                       pa.motion == kStatic ? 1u : 0u, pb.motion == kStatic ? 1u : 0u,
+                                       pair_frames(frames, frame_ref, pa, pb, p, prev[p.x], pred[p.x]),
                                        pa.collision_margin + pb.collision_margin, dt};
         wave_sync();
         const TrianglePair &tp = shared_pair;
+        // Leaf candidates from the motion test alone; the ranges of a guided
+        // pair always take the swept pass (geometry.slang:2350, 2574-2579,
+        // 2167-2170), and its discrete pass finds nothing (:1371).
+        const bool guided = guided_pair(tp.f.ha, tp.f.hb);
         const bool swept = requires_swept(tp.ap, tp.a, tp.am.s, tp.bp, tp.b, tp.bm.s, tp.margin);
+        const bool swept_eval = guided || swept;
         const BoundsFrame fa = bounds_frame(tp.a), fb = bounds_frame(tp.b);
         const BoundsFrame pfa = swept ? bounds_frame(tp.ap) : BoundsFrame{};
         const BoundsFrame pfb = swept ? bounds_frame(tp.bp) : BoundsFrame{};
@@ -1692,7 +1934,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         } else {
             if (lane == 0) {
                 ++fallback;
-                S.count = traversal_list(tp, swept, S.list);
+                S.count = traversal_list(tp, swept_eval, S.list);
             }
             wave_sync();
             nlist = S.count;
@@ -1700,7 +1942,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         wave_sync();
         // 2. Contacts.
         if (nlist == kListOverflow) {
-            if (lane == 0) triangle_pair_serial(tp, swept, leaf_mode, S.man, S.local);
+            if (lane == 0) triangle_pair_serial(tp, swept, swept_eval, leaf_mode, S.man, S.local);
         } else {
             if (lane == 0) {
                 S.man.count = 0;
@@ -1718,8 +1960,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                     if (ai < na.count && bi < nb.count) {
                         const uint32_t at = na.first + ai, bt = nb.first + bi;
                         if (3u * at + 2u < tp.am.s.idx_count && 3u * bt + 2u < tp.bm.s.idx_count) {
-                            hd = triangle_pair_contact(tp, at, bt, inside_a, inside_b, cd);
-                            if (swept) hs = triangle_pair_swept_contact(tp, at, bt, inside_a, inside_b, cs);
+                            hd = !guided && triangle_pair_contact(tp, at, bt, inside_a, inside_b, cd);
+                            if (swept_eval) hs = triangle_pair_swept_contact(tp, at, bt, inside_a, inside_b, cs);
                         }
                     }
                 }
@@ -1743,15 +1985,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                         for (uint32_t k = 16u * g; k < 16u * g + 16u; ++k) {
                             if (!((md >> k) & 1ull)) continue;
                             const Contact c{v3(S.disc[k][0]), v3(S.disc[k][1]), S.disc[k][0].w, S.disc[k][1].w};
-                            add_contact(dst, c, spacing);
+                            merge_contact(tp, dst, c, spacing);
                         }
                         for (uint32_t k = 16u * g; k < 16u * g + 16u; ++k) {
                             if (!((ms >> k) & 1ull)) continue;
                             const Contact c{v3(S.swept[k][0]), v3(S.swept[k][1]), S.swept[k][0].w, S.swept[k][1].w};
-                            add_contact(dst, c, spacing);
+                            merge_contact(tp, dst, c, spacing);
                         }
                         if (leaf_mode)
-                            for (uint32_t k = 0; k < S.local.count; ++k) add_contact(S.man, S.local.c[k], spacing);
+                            for (uint32_t k = 0; k < S.local.count; ++k) merge_contact(tp, S.man, S.local.c[k], spacing);
                     }
                 }
                 wave_sync();
@@ -1760,7 +2002,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         // 3. Emit.
         if (lane == 0) {
             Manifold man = S.man;
-            emit_manifold(o, m, man, tp.a, tp.b, persistent_pair(tp.am.s, tp.bm.s, pb));
+            emit_manifold(o, m, man, tp.a, tp.b, persistent_pair(tp.am.s, tp.bm.s, pb), tp.f.ha, tp.f.hb);
         }
         wave_sync();
     }
@@ -2004,12 +2246,26 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
     const NarrowOut out{v.m_count.p, v.m_start.p, v.m_face.p, v.m_init_rel.p, v.c_np.p, v.c_pt.p, v.c_aa.p,
                         v.c_ab.p, v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p, v.c_flags.p,
                         v.c_pair.p, v.point_cap, ctr};
+    // Hinge contact frames of every body (worlds with joints only).
+    HingeFrame *frames = nullptr;
+    float4 *frame_ref = nullptr;
+    if (v.nj) {
+        if ((r = hinge_frames.reserve(n)) != PMH_SUCCESS || (r = hinge_ref.reserve(2 * size_t(n))) != PMH_SUCCESS)
+            return r;
+        frames = hinge_frames.p;
+        frame_ref = hinge_ref.p;
+        JointView jv;
+        jv.joints = v.joints;
+        jv.list_start = v.jlist_start;
+        jv.list = v.jlist;
+        k_hinge_frames<<<bg, 256, 0, s>>>(n, params.p, prev, pred, jv, frames, frame_ref);
+    }
     k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
         &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p, out,
-        v.tri_list.p);
+        v.tri_list.p, frames, frame_ref);
     k_triangle_pairs<<<2048, 256, 0, s>>>(
         v.tri_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, nodes_dev.p, leaves_dev.p,
-        shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt, rules);
+        shell_dev.p, prev, pred, v.ab.p, out, leaf_slots, leaf_limit, dt, rules, frames, frame_ref);
     k_guard_points<<<1, 64, 0, s>>>(ctr, v.point_cap);
     mark(3);
     // Response slots and the contact cache.

@@ -301,6 +301,18 @@ struct Solver {
     void release();
 };
 
+// Host-built joint schedule: each body's candidate joints (alive, both ids
+// valid) in descending joint index (the reference pushes a joint to the
+// front of its bodies' lists in index order), the bodies with more than 64
+// (one wavefront each), and the hinge anchor projection schedule: alive
+// hinges by level, a hinge's level one above the highest already on either
+// of its bodies (in joint order); level l spans level_start[l - 1] ..
+// level_start[l].  Body ids are {dense index, generation 1}.
+struct JointLists {
+    std::vector<uint32_t> start, list, heavy, hinges, level_start;
+};
+void build_joint_lists(const PmhRigidConstraint *joints, uint32_t nj, uint32_t n, JointLists &out);
+
 struct SolveParams {
     float dt;
     V3 gravity;

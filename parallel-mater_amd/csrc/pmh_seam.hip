@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <mutex>
+#include <unordered_map>
+#include <vector>
 
 namespace pmh {
 
@@ -50,7 +52,12 @@ __global__ void k_seam_in(uint32_t n, uint32_t M, const uint32_t *pairs, const P
                                t.accumulated_friction_impulse.y, t.accumulated_friction_impulse.z);
         c_rep[c] = make_float4(t.reported_normal_impulse, t.reported_friction_impulse.x,
                                t.reported_friction_impulse.y, t.reported_friction_impulse.z);
-        c_flags[c] = (t.avbd_cached ? 1u : 0u) | (t.persistent ? 2u : 0u);
+        // Point flags: 1 cached, 2 persistent, 8 guided (an axially guided
+        // member against a static body, rigid_avbd_guided_contact,
+        // geometry_constraints.slang:327-333).
+        const bool guided = (t.body_hinge.axial != 0u && t.collider_hinge.static_body != 0u) ||
+                            (t.collider_hinge.axial != 0u && t.body_hinge.static_body != 0u);
+        c_flags[c] = (t.avbd_cached ? 1u : 0u) | (t.persistent ? 2u : 0u) | (guided ? 8u : 0u);
         c_pair[c] = m;
     }
 }
@@ -92,7 +99,7 @@ __global__ void k_seam_bodies_out(uint32_t n, PmhAvbdBody *bodies, PmhContactMan
                                   const float4 *tpos, const float4 *tq, const float4 *tv, const float4 *tw,
                                   const float4 *disp, const float4 *dq, const float4 *inertia, const float4 *prev_vel,
                                   const uint32_t *color, const uint32_t *parent, const uint32_t *impact,
-                                  const uint32_t *adj_start, const uint32_t *adj) {
+                                  const uint32_t *adj_start, const uint32_t *adj, const uint32_t *joint_head) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     PmhAvbdBody b;
@@ -108,7 +115,7 @@ __global__ void k_seam_bodies_out(uint32_t n, PmhAvbdBody *bodies, PmhContactMan
     b.inertia_h[0] = b.inertia_h[2] = b.inertia_h[5] = i0.x;
     b.inertia_h[9] = i0.y; b.inertia_h[13] = i0.z; b.inertia_h[14] = i0.w;
     b.inertia_h[18] = i1.x; b.inertia_h[19] = i1.y; b.inertia_h[20] = i1.z;
-    b.joint_head = kInvalid;
+    b.joint_head = joint_head ? joint_head[i] : kInvalid;
     b.color = color[i];
     b.parent = parent[i];
     b.impact = impact[i];
@@ -150,7 +157,7 @@ static PmhResult seam_run(Solver &v, hipStream_t s, const PmhBodyParameters *par
                                       v.c_err.p, v.c_lam.p, v.c_pen.p, v.c_acc.p, v.c_rep.p);
     k_seam_bodies_out<<<bg, 256, 0, s>>>(body_count, bodies, manifolds, v.ab.p, v.tpos.p, v.tq.p, v.tv.p, v.tw.p,
                                          v.disp.p, v.dq.p, v.inertia.p, v.prev_vel.p, v.color.p, v.parent.p,
-                                         v.impact.p, v.adj_start.p, v.adj.p);
+                                         v.impact.p, v.adj_start.p, v.adj.p, v.nj ? v.joint_head.p : nullptr);
     PMH_CHECK(hipMemcpyAsync(schedule, v.schedule.p, sizeof(PmhContactSchedule), hipMemcpyDeviceToDevice, s));
     PMH_CHECK(hipStreamSynchronize(s));
     PMH_CHECK(hipGetLastError());
@@ -165,8 +172,103 @@ static bool seam_args_valid(const PmhBodyParameters *parameters, const PmhRigidB
            profile_version <= 1 && active_pair_count;
 }
 
+// ---- joints at the seam -----------------------------------------------------
+// The caller's RigidConstraintResource records name their bodies by id;
+// `ids` maps dense index -> id (rigid_avbd_find_dense,
+// geometry_constraints.slang:318-325).  The solve runs on a copy whose body
+// ids are rewritten to {dense index, 1}; afterwards the solver-owned fields
+// (state, applied impulse, dense ids, rows, list links) go back to the
+// caller's records, whose options stay as they were.
+struct SeamJoints {
+    DBuf<PmhRigidConstraint> copy;
+    DBuf<uint32_t> list_start, list, heavy, hinges, levels;
+    std::vector<PmhRigidConstraint> host;
+    std::vector<PmhBodyId> host_ids;
+};
+
+__global__ void k_seam_joints_out(uint32_t nj, PmhRigidConstraint *dst, const PmhRigidConstraint *src) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nj) return;
+    PmhRigidConstraint &d = dst[k];
+    const PmhRigidConstraint &s = src[k];
+    d.state_enabled = s.state_enabled;
+    d.state_broken = s.state_broken;
+    d.applied_impulse = s.applied_impulse;
+    d.dense_a = s.dense_a;
+    d.dense_b = s.dense_b;
+    for (uint32_t r = 0; r < 18u; ++r) d.rows[r] = s.rows[r];
+    d.avbd_count = s.avbd_count;
+    d.avbd_next_a = s.avbd_next_a;
+    d.avbd_next_b = s.avbd_next_b;
+}
+
+// Uploads the translated copy and its schedule and points the solver at them.
+static PmhResult seam_joints_in(Solver &v, SeamJoints &sj, hipStream_t s, const PmhRigidConstraint *constraints,
+                                uint32_t capacity, const PmhBodyId *ids, uint32_t n) {
+    sj.host.resize(capacity);
+    sj.host_ids.resize(n);
+    PMH_CHECK(hipMemcpyAsync(sj.host.data(), constraints, sizeof(PmhRigidConstraint) * capacity,
+                             hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipMemcpyAsync(sj.host_ids.data(), ids, sizeof(PmhBodyId) * n, hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    std::unordered_map<uint64_t, uint32_t> dense;
+    dense.reserve(2 * size_t(n));
+    for (uint32_t i = 0; i < n; ++i)
+        dense.emplace((uint64_t(sj.host_ids[i].index) << 32) | sj.host_ids[i].generation, i);
+    auto resolve = [&](PmhBodyId &id) {
+        const auto it = dense.find((uint64_t(id.index) << 32) | id.generation);
+        id = it == dense.end() ? PmhBodyId{kInvalid, 0u} : PmhBodyId{it->second, 1u};
+    };
+    for (PmhRigidConstraint &j : sj.host) {
+        resolve(j.options.body_a);
+        resolve(j.options.body_b);
+    }
+    JointLists jl;
+    build_joint_lists(sj.host.data(), capacity, n, jl);
+    PmhResult r;
+    if ((r = sj.copy.reserve(std::max<uint32_t>(1, capacity))) || (r = sj.list_start.reserve(n + 1)) ||
+        (r = sj.list.reserve(jl.list.size())) || (r = sj.heavy.reserve(std::max<size_t>(1, jl.heavy.size()))) ||
+        (r = sj.hinges.reserve(std::max<size_t>(1, jl.hinges.size()))) ||
+        (r = sj.levels.reserve(std::max<size_t>(1, jl.level_start.size()))) ||
+        (r = v.jadj.reserve(std::max<size_t>(1, jl.list.size()))))
+        return r;
+    PMH_CHECK(hipMemcpyAsync(sj.copy.p, sj.host.data(), sizeof(PmhRigidConstraint) * capacity, hipMemcpyHostToDevice, s));
+    PMH_CHECK(hipMemcpyAsync(sj.list_start.p, jl.start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    PMH_CHECK(hipMemcpyAsync(sj.list.p, jl.list.data(), sizeof(uint32_t) * jl.list.size(), hipMemcpyHostToDevice, s));
+    if (!jl.heavy.empty())
+        PMH_CHECK(hipMemcpyAsync(sj.heavy.p, jl.heavy.data(), sizeof(uint32_t) * jl.heavy.size(), hipMemcpyHostToDevice, s));
+    if (!jl.hinges.empty())
+        PMH_CHECK(hipMemcpyAsync(sj.hinges.p, jl.hinges.data(), sizeof(uint32_t) * jl.hinges.size(), hipMemcpyHostToDevice, s));
+    if (!jl.level_start.empty())
+        PMH_CHECK(hipMemcpyAsync(sj.levels.p, jl.level_start.data(), sizeof(uint32_t) * jl.level_start.size(),
+                                 hipMemcpyHostToDevice, s));
+    PMH_CHECK(hipStreamSynchronize(s));   // the host vectors stay valid until the copies are done
+    v.joints = sj.copy.p;
+    v.nj = capacity;
+    v.jlist_start = sj.list_start.p;
+    v.jlist = sj.list.p;
+    v.jheavy = sj.heavy.p;
+    v.nheavy = uint32_t(jl.heavy.size());
+    v.hinge_list = sj.hinges.p;
+    v.hinge_level_start = jl.level_start;
+    v.hinge_level_dev = sj.levels.p;
+    v.hinges = !jl.hinges.empty();
+    return PMH_SUCCESS;
+}
+
+static void seam_joints_clear(Solver &v) {
+    v.joints = nullptr;
+    v.nj = 0;
+    v.jlist_start = v.jlist = v.jheavy = nullptr;
+    v.nheavy = 0;
+    v.hinge_list = v.hinge_level_dev = nullptr;
+    v.hinge_level_start.clear();
+    v.hinges = false;
+}
+
 static std::mutex g_seam_mutex;
 static Solver *g_seam_solver = nullptr;
+static SeamJoints *g_seam_joints = nullptr;
 
 // ---- the seam without allocation ------------------------------------------
 // Every buffer of the solve is a slice of one file-scope device array per
@@ -215,6 +317,47 @@ extern "C" PmhResult pmhPrepareSolveGeometryConstraints(
     if (r != PMH_SUCCESS) return r;
     return seam_run(v, s, parameters, previous, states, body_count, manifolds, active_pairs, M, bodies, schedule,
                     timestep, gravity, base_iterations, requested_iterations, profile_version);
+}
+
+extern "C" PmhResult pmhPrepareSolveGeometryConstraintsJoints(
+    void *stream, const PmhBodyParameters *parameters, const PmhRigidBodyState *previous, PmhRigidBodyState *states,
+    uint32_t body_count, PmhContactManifold *manifolds, const uint32_t *active_pairs,
+    const uint32_t *active_pair_count, PmhRigidConstraint *constraints, uint32_t constraint_capacity,
+    const PmhBodyId *ids, PmhAvbdBody *bodies, PmhContactSchedule *schedule, float timestep,
+    const float gravity[3], uint32_t base_iterations, uint32_t requested_iterations, uint32_t profile_version) {
+    using namespace pmh;
+    if (!seam_args_valid(parameters, previous, states, body_count, bodies, schedule, timestep, gravity,
+                         profile_version, active_pair_count) ||
+        (constraint_capacity && (!constraints || !ids)))
+        return PMH_ERROR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(g_seam_mutex);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint32_t M = 0;
+    PMH_CHECK(hipMemcpyAsync(&M, active_pair_count, sizeof M, hipMemcpyDeviceToHost, s));
+    PMH_CHECK(hipStreamSynchronize(s));
+    if (M && (!manifolds || !active_pairs)) return PMH_ERROR_INVALID_ARGUMENT;
+    if (!g_seam_solver) g_seam_solver = new Solver();
+    if (!g_seam_joints) g_seam_joints = new SeamJoints();
+    Solver &v = *g_seam_solver;
+    PmhResult r = v.reserve(body_count, std::max<uint32_t>(M, 1), std::max<uint32_t>(8 * M, 8));
+    if (r != PMH_SUCCESS) return r;
+    if (constraint_capacity) {
+        r = seam_joints_in(v, *g_seam_joints, s, constraints, constraint_capacity, ids, body_count);
+        if (r != PMH_SUCCESS) {
+            seam_joints_clear(v);
+            return r;
+        }
+    }
+    r = seam_run(v, s, parameters, previous, states, body_count, manifolds, active_pairs, M, bodies, schedule,
+                 timestep, gravity, base_iterations, requested_iterations, profile_version);
+    if (r == PMH_SUCCESS && constraint_capacity) {
+        k_seam_joints_out<<<(constraint_capacity + 255) / 256, 256, 0, s>>>(constraint_capacity, constraints,
+                                                                             g_seam_joints->copy.p);
+        const hipError_t e = hipGetLastError();
+        r = e != hipSuccess ? hip_result(e, "k_seam_joints_out") : sync_stream(s);
+    }
+    seam_joints_clear(v);
+    return r;
 }
 
 extern "C" PmhResult pmhPrepareSolveGeometryConstraintsNoAlloc(

@@ -472,81 +472,37 @@ PmhResult World::upload_joints() {
         list[cursor[a]++] = make_uint2(b, k);
         list[cursor[b]++] = make_uint2(a, k);
     }
-    // Candidate joint lists: each body's alive joints with valid ids, in
-    // descending joint index (the reference pushes joints to the front of
-    // a body's list in index order).
-    std::vector<uint32_t> lstart(n + 1, 0u), lcursor;
-    for (const PmhRigidConstraint &j : host_joints) {
-        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
-        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
-        lstart[a]++;
-        lstart[b]++;
-    }
-    uint32_t ltotal = 0;
-    for (uint32_t i = 0; i <= n; ++i) { const uint32_t c = lstart[i]; lstart[i] = ltotal; ltotal += c; }
-    std::vector<uint32_t> llist(std::max<uint32_t>(1, ltotal));
-    lcursor.assign(lstart.begin(), lstart.end());
-    for (uint32_t k = nj; k-- > 0;) {
-        const PmhRigidConstraint &j = host_joints[k];
-        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
-        if (j.alive == 0u || a == kInvalid || b == kInvalid) continue;
-        llist[lcursor[a]++] = k;
-        llist[lcursor[b]++] = k;
-    }
-    std::vector<uint32_t> heavy;
-    for (uint32_t i = 0; i < n; ++i)
-        if (lstart[i + 1] - lstart[i] > 64u) heavy.push_back(i);
-    if ((r = jlist_start.reserve(n + 1)) || (r = jlist.reserve(llist.size())) ||
-        (r = jheavy.reserve(std::max<size_t>(1, heavy.size()))))
+    // Candidate joint lists and the hinge projection schedule.
+    JointLists jl;
+    build_joint_lists(host_joints.data(), nj, n, jl);
+    if ((r = jlist_start.reserve(n + 1)) || (r = jlist.reserve(jl.list.size())) ||
+        (r = jheavy.reserve(std::max<size_t>(1, jl.heavy.size()))))
         return r;
-    PMH_CHECK(hipMemcpy(jlist_start.p, lstart.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
-    PMH_CHECK(hipMemcpy(jlist.p, llist.data(), sizeof(uint32_t) * llist.size(), hipMemcpyHostToDevice));
-    if (!heavy.empty())
-        PMH_CHECK(hipMemcpy(jheavy.p, heavy.data(), sizeof(uint32_t) * heavy.size(), hipMemcpyHostToDevice));
+    PMH_CHECK(hipMemcpy(jlist_start.p, jl.start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
+    PMH_CHECK(hipMemcpy(jlist.p, jl.list.data(), sizeof(uint32_t) * jl.list.size(), hipMemcpyHostToDevice));
+    if (!jl.heavy.empty())
+        PMH_CHECK(hipMemcpy(jheavy.p, jl.heavy.data(), sizeof(uint32_t) * jl.heavy.size(), hipMemcpyHostToDevice));
     solver.jlist_start = jlist_start.p;
     solver.jlist = jlist.p;
     solver.jheavy = jheavy.p;
-    solver.nheavy = uint32_t(heavy.size());
-    if ((r = solver.jadj.reserve(std::max<size_t>(1, ltotal)))) return r;
+    solver.nheavy = uint32_t(jl.heavy.size());
+    if ((r = solver.jadj.reserve(std::max<size_t>(1, jl.list.size())))) return r;
     excl_count = total;
     if ((r = excl_list.reserve(list.size()))) return r;
     PMH_CHECK(hipMemcpy(excl_start.p, start.data(), sizeof(uint32_t) * (n + 1), hipMemcpyHostToDevice));
     PMH_CHECK(hipMemcpy(excl_list.p, list.data(), sizeof(uint2) * list.size(), hipMemcpyHostToDevice));
-    // Hinge projection levels, in joint index order: a hinge's level is one
-    // above the highest level already on either of its bodies.
-    std::vector<uint32_t> body_level(n, 0u), hinge_level;
-    std::vector<uint32_t> hinge_ids;
-    uint32_t levels = 0;
-    for (uint32_t k = 0; k < nj; ++k) {
-        const PmhRigidConstraint &j = host_joints[k];
-        const uint32_t a = dense(j.options.body_a), b = dense(j.options.body_b);
-        if (j.alive == 0u || j.options.type != kJointHinge || a == kInvalid || b == kInvalid) continue;
-        const uint32_t level = std::max(body_level[a], body_level[b]) + 1u;
-        body_level[a] = body_level[b] = level;
-        hinge_ids.push_back(k);
-        hinge_level.push_back(level);
-        levels = std::max(levels, level);
-    }
-    std::vector<uint32_t> level_start(levels + 2u, 0u), ordered(hinge_ids.size());
-    for (uint32_t level : hinge_level) level_start[level + 1u]++;
-    for (uint32_t l = 1; l <= levels + 1u; ++l) level_start[l] += level_start[l - 1];
-    {
-        std::vector<uint32_t> cur(level_start.begin(), level_start.end());
-        for (size_t h = 0; h < hinge_ids.size(); ++h) ordered[cur[hinge_level[h]]++] = hinge_ids[h];
-    }
-    if ((r = hinge_list.reserve(std::max<size_t>(1, ordered.size())))) return r;
-    if (!ordered.empty())
-        PMH_CHECK(hipMemcpy(hinge_list.p, ordered.data(), sizeof(uint32_t) * ordered.size(), hipMemcpyHostToDevice));
+    if ((r = hinge_list.reserve(std::max<size_t>(1, jl.hinges.size())))) return r;
+    if (!jl.hinges.empty())
+        PMH_CHECK(hipMemcpy(hinge_list.p, jl.hinges.data(), sizeof(uint32_t) * jl.hinges.size(), hipMemcpyHostToDevice));
     solver.hinge_list = hinge_list.p;
-    // level_start[l] .. level_start[l + 1] holds level l (levels start at 1).
-    solver.hinge_level_start.assign(level_start.begin() + 1, level_start.end());
+    solver.hinge_level_start = jl.level_start;
     if ((r = hinge_levels.reserve(solver.hinge_level_start.size()))) return r;
     PMH_CHECK(hipMemcpy(hinge_levels.p, solver.hinge_level_start.data(),
                         sizeof(uint32_t) * solver.hinge_level_start.size(), hipMemcpyHostToDevice));
     solver.hinge_level_dev = hinge_levels.p;
     solver.joints = joints_dev.p;
     solver.nj = nj;
-    solver.hinges = hinges && !ordered.empty();
+    solver.hinges = hinges && !jl.hinges.empty();
     joints_dirty = false;
     joints_on_device = nj != 0;
     return PMH_SUCCESS;
@@ -630,7 +586,7 @@ void World::release() {
     nodes_dev.release(); leaves_dev.release(); shell_dev.release();
     inputs_dev.release(); targets_dev.release(); target_flags_dev.release();
     joints_dev.release(); weld.release(); excl_start.release(); excl_list.release();
-    jlist_start.release(); jlist.release(); jheavy.release(); hinge_list.release(); hinge_levels.release();
+    jlist_start.release(); jlist.release(); jheavy.release(); hinge_frames.release(); hinge_ref.release(); hinge_list.release(); hinge_levels.release();
     scene_of.release(); scene_count.release(); scene_pair_start.release(); scene_first_dev.release();
     scene_large_count.release(); scene_large_start.release(); scene_large.release();
     mlp_dev.release(); log_dev.release(); snap_vel.release(); snap_inputs.release(); snap_flags.release();
@@ -1366,6 +1322,10 @@ PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *out) {
         (r = fetch(err, v.c_err.p, P)) || (r = fetch(lam, v.c_lam.p, P)) || (r = fetch(pen, v.c_pen.p, P)) ||
         (r = fetch(acc, v.c_acc.p, P)) || (r = fetch(rep, v.c_rep.p, P)) || (r = fetch(flags, v.c_flags.p, P)))
         return r;
+    // The bodies' hinge contact frames of the step (worlds with joints);
+    // without joints a frame carries only the static flag.
+    std::vector<PmhHingeContactFrame> frames;
+    if (v.nj && w->has_stepped && w->hinge_frames.p && (r = fetch(frames, w->hinge_frames.p, w->body_count))) return r;
     for (uint32_t m = 0; m < M; ++m) {
         PmhContactManifold &o = out[m];
         std::memset(&o, 0, sizeof o);
@@ -1393,9 +1353,13 @@ PmhResult pmhWorldGetManifolds(PmhWorld world, PmhContactManifold *out) {
             t.reported_friction_impulse = {rep[i].y, rep[i].z, rep[i].w};
             t.avbd_cached = flags[i] & 1u;
             t.persistent = (flags[i] >> 1) & 1u;
-            // Hinge frames without joints carry only the static flag.
-            t.body_hinge.static_body = w->host_params[pair_ab[m].x].motion == 0u ? 1u : 0u;
-            t.collider_hinge.static_body = w->host_params[pair_ab[m].y].motion == 0u ? 1u : 0u;
+            if (!frames.empty()) {
+                t.body_hinge = frames[pair_ab[m].x];
+                t.collider_hinge = frames[pair_ab[m].y];
+            } else {
+                t.body_hinge.static_body = w->host_params[pair_ab[m].x].motion == 0u ? 1u : 0u;
+                t.collider_hinge.static_body = w->host_params[pair_ab[m].y].motion == 0u ? 1u : 0u;
+            }
         }
     }
     return PMH_SUCCESS;

@@ -61,6 +61,10 @@ struct World {
     // collisions) and, per body, its disable_collisions partners with the
     // joint that excludes them.
     DBuf<uint32_t> jlist_start, jlist, jheavy;   // candidate joint lists (descending index)
+    // Per body, at the last step's predicted states: its hinge contact frame,
+    // and its reference point at the previous and the predicted state.
+    DBuf<PmhHingeContactFrame> hinge_frames;
+    DBuf<float4> hinge_ref;
     DBuf<uint32_t> hinge_list, hinge_levels;     // hinge projection schedule and its level offsets
     DBuf<uint32_t> weld, excl_start;
     DBuf<uint2> excl_list;

@@ -126,6 +126,8 @@ def lib():
         "pmhWorldGetProxies": (i32, [_P, _P]),
         "pmhPrepareSolveGeometryConstraints": (
             i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
+        "pmhPrepareSolveGeometryConstraintsJoints": (
+            i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _U, _P, _P, _P, _F, _P, _U, _U, _U]),
         "pmhPrepareSolveGeometryConstraintsNoAlloc": (
             i32, [_P, _P, _P, _P, _U, _P, _P, _P, _P, _P, _F, _P, _U, _U, _U]),
         "pmhWorldSetWarmstart": (i32, [_P, _P, _U]),
@@ -453,7 +455,7 @@ class DeviceArray:
 
 def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
                   dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), base=10, requested=0,
-                  profile=1, entry="pmhPrepareSolveGeometryConstraints"):
+                  profile=1, entry="pmhPrepareSolveGeometryConstraints", joints=None, ids=None):
     """Kernel seam on the GPU with host arrays: copies in, runs `entry`
     (pmhPrepareSolveGeometryConstraints, or its variant that allocates
     nothing), copies back in place (states, manifolds, bodies) and returns
@@ -470,9 +472,20 @@ def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
     d_b = DeviceArray(np.ascontiguousarray(bodies, dtype=AVBD_BODY))
     d_sched = DeviceArray(np.zeros(1, dtype=SCHEDULE))
     g = np.asarray(gravity, dtype=np.float32)
-    _check(getattr(lib(), entry)(
-        None, d_p.ptr, d_prev.ptr, d_s.ptr, n, d_m.ptr, d_pairs.ptr, d_count.ptr, d_b.ptr,
-        d_sched.ptr, dt, _ptr(g), base, requested, profile), entry)
+    if joints is not None:
+        # pmhPrepareSolveGeometryConstraintsJoints: the joints (JOINT records)
+        # and the id of each dense body; the joints' solver fields come back.
+        d_j = DeviceArray(np.ascontiguousarray(joints, dtype=JOINT))
+        d_ids = DeviceArray(np.ascontiguousarray(ids, dtype=np.uint32).reshape(n, 2))
+        _check(lib().pmhPrepareSolveGeometryConstraintsJoints(
+            None, d_p.ptr, d_prev.ptr, d_s.ptr, n, d_m.ptr, d_pairs.ptr, d_count.ptr, d_j.ptr, len(joints),
+            d_ids.ptr, d_b.ptr, d_sched.ptr, dt, _ptr(g), base, requested, profile),
+            "pmhPrepareSolveGeometryConstraintsJoints")
+        joints[...] = d_j.read()
+    else:
+        _check(getattr(lib(), entry)(
+            None, d_p.ptr, d_prev.ptr, d_s.ptr, n, d_m.ptr, d_pairs.ptr, d_count.ptr, d_b.ptr,
+            d_sched.ptr, dt, _ptr(g), base, requested, profile), entry)
     states[...] = d_s.read()
     manifolds[...] = d_m.read()
     bodies[...] = d_b.read()

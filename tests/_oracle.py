@@ -89,6 +89,8 @@ def lib():
         _lib.pmo_world_get_mesh_planes.argtypes = [_P, _U, _P]
         _lib.pmo_avbd_prepare_solve.argtypes = [
             _P, _P, _P, _U, _P, _P, _U, _P, _P, _F, _P, _U, _U, _U, _U]
+        _lib.pmo_avbd_prepare_solve_joints.argtypes = [
+            _P, _P, _P, _U, _P, _P, _U, _P, _U, _P, _P, _P, _F, _P, _U, _U, _U, _U]
         for name in ("pmo_row_error", "pmo_row_trial", "pmo_row_force",
                      "pmo_force"):
             getattr(_lib, name).restype = _F
@@ -330,15 +332,26 @@ def triangle_feature_normal(a, b, on_a, on_b, fallback):
 
 def prepare_solve(params, previous, states, manifolds, pairs, bodies=None,
                   dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), base=10,
-                  requested=0, profile=1, lanes=0):
+                  requested=0, profile=1, lanes=0, joints=None, ids=None):
     """Kernel-seam restatement: prepare_avbd_kernel + solve_avbd_kernel on
     caller-built manifolds. States, manifolds and bodies are updated in
-    place; the schedule record is returned."""
+    place; the schedule record is returned.  With `joints` (JOINT records)
+    and `ids` (the id of each dense body), the joints take part and their
+    solver-owned fields are updated in place."""
     n = len(params)
     if bodies is None:
         bodies = np.zeros(n, dtype=L.AVBD_BODY)
     schedule = np.zeros(1, dtype=L.SCHEDULE)
     g = f32(gravity)
+    if joints is not None:
+        ids = np.ascontiguousarray(ids, dtype=np.uint32).reshape(n, 2)
+        r = lib().pmo_avbd_prepare_solve_joints(
+            _ptr(params), _ptr(previous), _ptr(states), n, _ptr(manifolds),
+            _ptr(pairs), len(pairs), _ptr(joints), len(joints), _ptr(ids), _ptr(bodies), _ptr(schedule), dt,
+            _ptr(g), base, requested, profile, lanes)
+        if r != 0:
+            raise RuntimeError("prepare_solve failed")
+        return schedule[0]
     r = lib().pmo_avbd_prepare_solve(
         _ptr(params), _ptr(previous), _ptr(states), n, _ptr(manifolds),
         _ptr(pairs), len(pairs), _ptr(bodies), _ptr(schedule), dt, _ptr(g),

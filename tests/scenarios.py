@@ -548,6 +548,66 @@ def joint_rig():
     return Scene(shapes, p, st, name="joint_rig", joints=np.concatenate(joints))
 
 
+def hinge_contacts(planted=False):
+    """Joints that shape contacts (rigid_hinge_contact_frame,
+    geometry.slang:404-491), over a static floor:
+    - a slider held by a static post, sent along its axis into a static
+      wall at 3 m/s (an axially guided member against a static body: the
+      triangle path's guided swept pass, 0.1 mm rest offset);
+    - a piston held the same way, spinning about its axis, into a second
+      wall (the guide also turns);
+    - a box hinged to a static post along its lower left edge, lying on
+      the floor (a hinged member: its contacts do not persist);
+    - two welded boxes on the lower step of a concave L prism (fixed
+      members on the triangle path)."""
+    prism_v, prism_i = l_prism()
+    shapes = [("mesh",) + box_mesh(0.1, 0.1, 0.1), ("mesh",) + box_mesh(0.25, 0.25, 0.25),
+              ("mesh",) + box_mesh(8.0, 0.5, 8.0), ("mesh",) + box_mesh(0.1, 0.5, 0.5),
+              ("mesh", prism_v, prism_i), ("mesh",) + box_mesh(0.2, 0.2, 0.2)]
+    motion, shape, mass, inertia, pos = [], [], [], [], []
+    halves = {0: (0.1, 0.1, 0.1), 1: (0.25, 0.25, 0.25), 2: (8.0, 0.5, 8.0), 3: (0.1, 0.5, 0.5),
+              4: (1.0, 1.0, 1.0), 5: (0.2, 0.2, 0.2)}
+
+    def body(m, sh, ms, p):
+        motion.append(m)
+        shape.append(sh)
+        mass.append(ms)
+        inertia.append(box_inertia(ms, halves[sh]))
+        pos.append(p)
+        return len(pos) - 1
+
+    S_, D_ = L.MOTION_STATIC, L.MOTION_DYNAMIC
+    body(S_, 2, 1.0, (0, -0.5, 0))                     # floor
+    joints = []
+    # slider and piston on static posts, each running into a static wall
+    post_s = body(S_, 0, 1.0, (0.0, 3.0, 0.0))
+    slider = body(D_, 1, 2.0, (0.0, 1.0, 0.0))
+    body(S_, 3, 1.0, (1.0, 1.0, 0.0))
+    joints.append(L.joint_options(1, L.JOINT_SLIDER, post_s, slider, (0.0, -2.0, 0.0), (0, 0, 0)))
+    post_p = body(S_, 0, 1.0, (0.0, 3.0, 2.0))
+    piston = body(D_, 1, 2.0, (0.0, 1.0, 2.0))
+    body(S_, 3, 1.0, (1.0, 1.0, 2.0))
+    joints.append(L.joint_options(1, L.JOINT_PISTON, post_p, piston, (0.0, -2.0, 0.0), (0, 0, 0)))
+    # a box hinged along its lower left edge (hinge axis z) to a static post
+    post_h = body(S_, 0, 1.0, (-2.5, 0.1, 0.0))
+    lid = body(D_, 1, 1.0, (-2.0, 0.2505, 0.0))
+    joints.append(L.joint_options(1, L.JOINT_HINGE, post_h, lid, (0.25, -0.1, 0.0), (-0.25, -0.25, 0.0)))
+    # two welded boxes on the prism's lower step (its top at y = 1 over x in [1, 2])
+    prism = body(S_, 4, 1.0, (3.0, 0.0, -3.0))
+    wa = body(D_, 5, 1.0, (4.5, 1.2005, -3.4))
+    wb = body(D_, 5, 1.0, (4.5, 1.2005, -2.6))
+    joints.append(L.joint_options(1, L.JOINT_FIXED, wa, wb, (0.0, 0.0, 0.4), (0.0, 0.0, -0.4)))
+    n = len(pos)
+    p = params(n, motion=np.array(motion, np.uint32), mass=np.array(mass, np.float32),
+               inertia=np.array(inertia, np.float32), shape=np.array(shape), margin=REF_MARGIN,
+               damping=REF_DAMPING, max_linear=REF_MAX_SPEED, max_angular=REF_MAX_SPEED)
+    st = states(n, np.array(pos, np.float32))
+    st["linear_velocity"][slider] = (3.0, 0.0, 0.0)
+    st["linear_velocity"][piston] = (3.0, 0.0, 0.0)
+    st["angular_velocity"][piston] = (2.0, 0.0, 0.0)
+    return Scene(shapes, p, st, name="hinge_contacts", joints=np.concatenate(joints))
+
+
 def capsule_inertia(mass, radius, half_length):
     """Solid capsule along local y (cylinder of length 2h plus two
     hemispheres), principal moments (x, y, z)."""
@@ -745,3 +805,29 @@ def batch(count=4096, seed=7, min_bodies=8, max_bodies=512, palette=64, region=4
         st = states(n, np.concatenate([np.array([[ox, -0.5, oz]], np.float32), pos]))
         scenes.append((p, st))
     return Batch(shapes, scenes, f"batch{count}")
+
+
+def joint_seam_inputs(steps=5, id_offset=1000, id_generation=3):
+    """Kernel-seam inputs with joints from the joint rig stepped by the
+    oracle: parameters, the states of step `steps` as the previous states
+    and of step `steps + 1` as the predicted ones, that step's manifolds,
+    pair codes, AvbdBody history and joint records.  Dense body i gets the id
+    (i + id_offset, id_generation), and the joints name their bodies by those
+    ids, so the seam has to resolve them."""
+    import _oracle as O
+    scene = joint_rig()
+    w = scene.build(O.OracleWorld(profile=0, base_iterations=10, lanes=32))
+    w.step(steps)
+    previous = w.states()
+    w.step(1)
+    joints = w.joints()
+    n = scene.n
+    ids = np.zeros((n, 2), np.uint32)
+    ids[:, 0] = np.arange(n) + id_offset
+    ids[:, 1] = id_generation
+    for side in ("body_a", "body_b"):
+        dense = joints["options"][side][:, 0].copy()
+        joints["options"][side][:, 0] = dense + id_offset
+        joints["options"][side][:, 1] = id_generation
+    return dict(params=scene.params.copy(), previous=previous, states=w.states(), manifolds=w.manifolds(),
+                pairs=w.pairs(), bodies=w.bodies(), joints=joints, ids=ids)

@@ -68,6 +68,30 @@ def test_seam_realtime_fixture():
     np.testing.assert_allclose(g_m["contacts"]["avbd_dual"], o_m["contacts"]["avbd_dual"], rtol=1e-5, atol=1e-5)
 
 
+def test_seam_with_joints_matches_the_oracle():
+    say("seam with joints: joint rig inputs, bodies named by shifted ids")
+    inp = S.joint_seam_inputs()
+    runs = []
+    for side in ("gpu", "oracle"):
+        st, man, bod, jo = inp["states"].copy(), inp["manifolds"].copy(), inp["bodies"].copy(), inp["joints"].copy()
+        args = (inp["params"], inp["previous"], st, man, inp["pairs"])
+        kw = dict(bodies=bod, base=10, profile=0, joints=jo, ids=inp["ids"])
+        sched = (pmamd.prepare_solve(*args, **kw) if side == "gpu" else O.prepare_solve(*args, lanes=LANES, **kw))
+        runs.append((st, jo, bod, sched))
+    (gs, gj, gb, gc), (os_, oj, ob, oc) = runs
+    say(f"bit-exact state fraction {exact_fraction(gs.view(np.float32), os_.view(np.float32)):.4f}")
+    assert_states_close(gs, os_)
+    for f in ("avbd_count", "avbd_next_a", "avbd_next_b", "dense_a", "dense_b", "enabled", "broken"):
+        np.testing.assert_array_equal(gj[f], oj[f], err_msg=f)
+    valid = np.arange(18)[None, :] < oj["avbd_count"][:, None]
+    for f in ("lambda", "penalty", "error", "a", "b"):
+        np.testing.assert_allclose(gj["rows"][f][valid], oj["rows"][f][valid], rtol=1e-4, atol=1e-4, err_msg=f)
+    assert gj["options"].tobytes() == inp["joints"]["options"].tobytes()
+    for f in ("joint_head", "color", "parent", "impact"):
+        np.testing.assert_array_equal(gb[f], ob[f], err_msg=f)
+    assert int(gc["budget"]) == int(oc["budget"])
+
+
 def test_seam_stack_window():
     say("seam: two-body stack window, 720 ticks")
 
@@ -113,8 +137,30 @@ def assert_manifolds_equal(gm, om):
               "avbd_anchor_b"):
         np.testing.assert_array_equal(gm["contacts"][f][valid], om["contacts"][f][valid], err_msg=f)
     for h in ("body_hinge", "collider_hinge"):
-        np.testing.assert_array_equal(gm["contacts"][h]["static_body"][valid],
-                                      om["contacts"][h]["static_body"][valid], err_msg=h)
+        for f in gm["contacts"][h].dtype.names:
+            np.testing.assert_array_equal(gm["contacts"][h][f][valid], om["contacts"][h][f][valid],
+                                          err_msg=f"{h}.{f}")
+    np.testing.assert_array_equal(gm["contacts"]["persistent"][valid], om["contacts"]["persistent"][valid],
+                                  err_msg="persistent")
+
+
+def test_joints_that_shape_contacts_follow_the_oracle():
+    say("hinge contact frames: guided slider and piston into walls, hinged lid, welded pair on a prism; 40 steps")
+    scene = S.hinge_contacts()
+    g = gpu_world(scene, profile=0, base_iterations=10)
+    o = oracle_world(scene, profile=0, base_iterations=10)
+    guided_steps = 0
+    for step in range(40):
+        g.step(1)
+        o.step(1)
+        np.testing.assert_array_equal(g.pairs(), o.pairs())
+        gm, om = g.manifolds(), o.manifolds()
+        assert_manifolds_equal(gm, om)
+        valid = np.arange(8)[None, :] < gm["count"][:, None]
+        guided_steps += int((gm["contacts"]["body_hinge"]["axial"][valid] != 0).any())
+        assert_states_close(g.states(), o.states())
+    assert guided_steps > 0                   # the guided pairs were in contact
+    assert g.stats()["triangle_pairs"] == o.stats()["triangle_pairs"]
 
 
 def test_crossing_edges_take_the_triangle_path():

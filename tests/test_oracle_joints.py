@@ -207,3 +207,32 @@ def test_capsule_chains_hold_together_and_rest_on_the_ground(iterations, max_gap
     assert s["position"][2:, 1].min() > 0.05 - 0.006
     assert w.joints()["broken"].sum() == 0
     assert w.stats()["manifolds"] > 0
+
+
+def test_the_seam_resolves_joint_bodies_through_ids():
+    """The seam's joints name their bodies by id (rigid_avbd_find_dense,
+    geometry_constraints.slang:318-325): the same joints under shifted ids
+    solve exactly as under {dense index, 1}; a joint whose body id matches
+    no body gets no rows."""
+    import _oracle as O
+    base = S.joint_seam_inputs(id_offset=0, id_generation=1)
+    moved = S.joint_seam_inputs(id_offset=1000, id_generation=3)
+    out = []
+    for inp in (base, moved):
+        st, man, bod, jo = inp["states"].copy(), inp["manifolds"].copy(), inp["bodies"].copy(), inp["joints"].copy()
+        sched = O.prepare_solve(inp["params"], inp["previous"], st, man, inp["pairs"], bodies=bod, base=10,
+                                profile=0, lanes=32, joints=jo, ids=inp["ids"])
+        out.append((st, jo, bod, sched))
+    (s0, j0, b0, c0), (s1, j1, b1, c1) = out
+    assert s0.tobytes() == s1.tobytes()
+    np.testing.assert_array_equal(j0["rows"], j1["rows"])
+    np.testing.assert_array_equal(j0["avbd_count"], j1["avbd_count"])
+    np.testing.assert_array_equal(b0["joint_head"], b1["joint_head"])
+    assert (j0["avbd_count"][j0["alive"] != 0] > 0).any()
+    assert c0.tobytes() == c1.tobytes()
+    lost = moved["joints"].copy()
+    lost["options"]["body_b"][0] = (999999, 3)          # names no body
+    st = moved["states"].copy()
+    O.prepare_solve(moved["params"], moved["previous"], st, moved["manifolds"].copy(), moved["pairs"],
+                    bodies=moved["bodies"].copy(), base=10, profile=0, lanes=32, joints=lost, ids=moved["ids"])
+    assert lost["avbd_count"][0] == 0 and lost["dense_b"][0] == 0xffffffff

@@ -15,7 +15,14 @@ planted in the oracle (never set outside these tests):
 - the swept face manifold (geometry.slang:1751-1853): a box translating
   through the ground in one step gets its contacts at the time of impact
   (gap - rest) / travel, at its four bottom corners on the mid-gap plane,
-  with the depth the remaining travel adds.
+  with the depth the remaining travel adds;
+- hinge contact frames (rigid_hinge_contact_frame, geometry.slang:404-491):
+  a slider held by a static post comes to rest against a static wall at
+  the guided rest offset (0.1 mm), not the 1 mm of an unguided contact; a
+  hinged lid's frame holds the post's anchor and axis and its contacts do
+  not persist; a piston's frame has the closed-form inverse moment about
+  its axis; welded boxes are fixed members.  Without frames (planted) each
+  of these fails.
 """
 import numpy as np
 import pytest
@@ -132,3 +139,50 @@ def test_the_swept_face_manifold_finds_the_time_of_impact():
     np.testing.assert_allclose(c["normal"], np.tile([0.0, 1.0, 0.0], (4, 1)), atol=1e-7)
     np.testing.assert_allclose(np.sort(np.abs(c["point"][:, [0, 2]]).ravel()), 0.5, atol=1e-6)
     np.testing.assert_allclose(c["point"][:, 1], rest / 2, atol=1e-6)
+
+
+def _hinge_world(planted, steps):
+    scene = S.hinge_contacts()
+    w = scene.build(O.OracleWorld(profile=0, base_iterations=10, planted=planted))
+    frames = []
+    for _ in range(steps):
+        w.step(1)
+        frames.append((w.pairs().copy(), w.manifolds().copy()))
+    return scene, w, frames
+
+
+def _contacts(scene, pairs, manifolds, a, b):
+    n = scene.n
+    for k, code in enumerate(pairs):
+        if code // n == a and code % n == b:
+            m = manifolds[k]
+            return m["contacts"][:m["count"]]
+    return None
+
+
+def test_hinge_contact_frames():
+    for planted in (0, 4):
+        scene, w, frames = _hinge_world(planted, 40)
+        s = w.states()
+        # slider (body 2) against its wall (body 3, face at x = 0.9)
+        gap = 0.9 - (float(s["position"][2][0]) + 0.25)
+        lid = _contacts(scene, *frames[-1], 8, 0)
+        weld = _contacts(scene, *frames[-1], 10, 9)
+        piston = [c for pr, ms in frames for c in [_contacts(scene, pr, ms, 5, 6)] if c is not None and len(c)]
+        if planted == 0:
+            assert 0.0 < gap < 1.5e-4, gap
+            h = lid["body_hinge"]
+            assert (h["present"] == 1).all() and (h["fixed"] == 1).all() and (lid["persistent"] == 0).all()
+            np.testing.assert_allclose(h["anchor"], np.tile([-2.25, 0.0, 0.0], (len(lid), 1)), atol=1e-6)
+            np.testing.assert_allclose(h["axis"], np.tile([0.0, 0.0, 1.0], (len(lid), 1)), atol=1e-6)
+            assert (weld["body_hinge"]["fixed_member"] == 1).all() and (weld["persistent"] == 0).all()
+            pf = piston[0]["body_hinge"]
+            assert (pf["axial"] == 1).all() and (pf["axial_rotation"] == 1).all()
+            np.testing.assert_allclose(pf["inverse_mass"], 0.5)
+            np.testing.assert_allclose(pf["inverse_moment"], 12.0, rtol=1e-3)   # 1 / I_xx of the 0.5 m cube, 2 kg
+            assert (piston[0]["persistent"] == 1).all()                          # guided contacts persist
+        else:
+            assert gap > 5e-4, gap
+            assert (lid["body_hinge"]["present"] == 0).all() and (lid["persistent"] == 1).all()
+            assert (weld["body_hinge"]["fixed_member"] == 0).all()
+            assert all((c["body_hinge"]["axial"] == 0).all() for c in piston)
