@@ -2904,6 +2904,14 @@ PmoHingeFrame hinge_frame(const World &w, uint32_t dense, const std::vector<PmoS
 // traversal for pairs without a candidate slot or with too many
 // candidates.  Round shapes (no reference counterpart) take the analytic
 // path instead.
+// The timestep of one pass of the pipeline: the frame's dt over its
+// substeps.  Every substep runs the whole pipeline (integration,
+// narrowphase, contact cache, solve) at this timestep.
+inline float substep_dt(const World &w) {
+    const uint32_t substeps = w.desc.substeps ? w.desc.substeps : 1u;
+    return substeps == 1u ? w.desc.dt : w.desc.dt / float(substeps);
+}
+
 void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<PmoState> &pred) {
     const uint32_t n = uint32_t(w.params.size());
     const uint32_t slots = slot_capacity(w);
@@ -2950,7 +2958,7 @@ void narrowphase(World &w, const std::vector<PmoState> &prev, const std::vector<
             }
             if (!face) {
                 clear_manifold(m);
-                TrianglePair P{&sa, &sb, prev[a], pred[a], prev[b], pred[b], ha, hb, margin, w.desc.dt, ref_prev, ref};
+                TrianglePair P{&sa, &sb, prev[a], pred[a], prev[b], pred[b], ha, hb, margin, substep_dt(w), ref_prev, ref};
                 w.stats.triangle_pairs++;
                 if (active < slots && leaf_candidates(P, swept, per_slot, candidates)) {
                     const float spacing = fmax_(margin * 2.0f, 1.0e-4f);
@@ -3023,8 +3031,8 @@ void load_cache(World &w, const std::vector<PmoState> &pred) {
             for (int ax = 0; ax < 3; ++ax) c.avbd_dual[ax] = saved.dual[match][ax];
             V3 t0 = normalized_or(cross(c.normal, std::fabs(c.normal.x) < 0.57735f ? V3{1, 0, 0} : V3{0, 1, 0}), {0, 0, 1});
             V3 t1 = cross(c.normal, t0);
-            c.avbd_dual[1].lambda = dot(saved.friction_impulse[match], t0) / w.desc.dt;
-            c.avbd_dual[2].lambda = dot(saved.friction_impulse[match], t1) / w.desc.dt;
+            c.avbd_dual[1].lambda = dot(saved.friction_impulse[match], t0) / substep_dt(w);
+            c.avbd_dual[2].lambda = dot(saved.friction_impulse[match], t1) / substep_dt(w);
             c.avbd_anchor_a = saved.anchor_a[match];
             c.avbd_anchor_b = saved.anchor_b[match];
             c.avbd_cached = 1u;
@@ -3065,7 +3073,7 @@ void world_step(World &w) {
     const uint32_t n = uint32_t(w.params.size());
     const V3 g = {w.desc.gravity[0], w.desc.gravity[1], w.desc.gravity[2]};
     const uint32_t substeps = w.desc.substeps ? w.desc.substeps : 1u;
-    const float dt = substeps == 1u ? w.desc.dt : w.desc.dt / float(substeps);
+    const float dt = substep_dt(w);
     if (w.cache_bodies != n) { w.cache.clear(); w.cache_bodies = n; }
     std::vector<PmoState> prev = w.states, pred(n);
     w.accumulators.resize(n, std::array<V3, 4>{});

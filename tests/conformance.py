@@ -34,6 +34,7 @@ import pathlib
 
 import numpy as np
 
+import glb_scenes
 import scenarios as S
 from pmamd import layouts as L
 
@@ -63,6 +64,8 @@ def _unit_inertia(world, shape):
 def build(name, world):
     """Adds the case's meshes, bodies and joints to `world`; returns the body
     names, constraint names and the joint options (for add_constraints)."""
+    if name in glb_scenes.CASES:
+        return glb_scenes.Scene(name).build(world)
     if name == "rigid-direct":                                  # direct_scene, :243-275
         for half in ((0.25, 0.25, 0.25), (3.0, 0.05, 3.0), (0.2, 0.2, 0.2)):
             world.add_mesh(*S.box_mesh(*half))
@@ -140,8 +143,9 @@ def run(name, world):
                 inp["body_index"] = bodies.index(command["target"])
                 inp["linear_impulse"] = value
                 world.apply_inputs(inp)
-            elif op == "set_kinematic_target":
-                k = bodies.index(command["target"])
+            elif op in ("set_kinematic_target", "set_first_kinematic_target"):
+                k = (bodies.index(command["target"]) if op == "set_kinematic_target" else
+                     [b["motion"] for b in glb_scenes.Scene(name).bodies].index(L.MOTION_KINEMATIC))
                 target = initial[k:k + 1].copy()
                 target["position"] = value
                 world.set_kinematic_targets(k, target)
@@ -150,6 +154,14 @@ def run(name, world):
                 state = initial[k:k + 1].copy()
                 state["position"] = value
                 world.set_rigid_states(k, state)
+            elif op == "set_constraints_enabled":
+                # write_constraint with the options' enabled flag, which the
+                # command scene passes on as the constraint's active flag
+                # (examples/gallery/scene_builder.cpp:131): a disabled
+                # constraint leaves the solve, its record stays present
+                joints = joints.copy()
+                joints["enabled"] = 1 if value[0] != 0 else 0
+                world.write_joints(0, joints, joints["enabled"].astype(np.uint32))
             elif op in ("remove_constraints", "add_constraints"):
                 active = op == "add_constraints"
                 world.write_joints(0, joints, np.full(len(joints), 1 if active else 0, dtype=np.uint32))
@@ -214,9 +226,10 @@ def _match_contacts(case, expected, actual):
     return [i for i in range(len(expected)) if not augment(i, set())]
 
 
-def compare(name, result, frames=None):
-    """Failures (strings) of `result` against the capture at the listed
-    checkpoint frames (all by default)."""
+def compare_records(name, result, frames=None):
+    """Failures of `result` against the capture at the listed checkpoint
+    frames (all by default), as (frame, record id, message); contact
+    failures carry the id "contacts"."""
     fixture = load(name)
     case = fixture["case"]
     failures = []
@@ -233,27 +246,74 @@ def compare(name, result, frames=None):
                     absolute, relative = _limit(case, r["tolerance_class"], leaf)
                     for axis, (e, a) in enumerate(zip(r[leaf], actual[leaf])):
                         if not _close(e, a, absolute, relative):
-                            failures.append(f"frame {frame} {rid}.{leaf}[{axis}]: expected {e:.7g}, got {a:.7g}")
+                            failures.append((frame, rid, f"{leaf}[{axis}]: expected {e:.7g}, got {a:.7g}"))
                 limit, _ = _limit(case, r["tolerance_class"], "orientation")
                 angle = quaternion_angle(r["orientation"], actual["orientation"])
                 if angle > limit:
-                    failures.append(f"frame {frame} {rid}.orientation: angle {angle:.3g} > {limit}")
+                    failures.append((frame, rid, f"orientation: angle {angle:.3g} > {limit}"))
             else:
                 actual = got["constraints"][rid]
                 for key in ("present", "broken", "enabled"):
                     if bool(r[key]) != actual[key]:
-                        failures.append(f"frame {frame} {rid}.{key}: expected {r[key]}, got {actual[key]}")
+                        failures.append((frame, rid, f"{key}: expected {r[key]}, got {actual[key]}"))
                 absolute, relative = _limit(case, r["tolerance_class"], "applied_impulse")
                 if not _close(r["applied_impulse"], actual["applied_impulse"], absolute, relative):
-                    failures.append(f"frame {frame} {rid}.applied_impulse: expected {r['applied_impulse']:.7g}, "
-                                    f"got {actual['applied_impulse']:.7g}")
+                    failures.append((frame, rid, f"applied_impulse: expected {r['applied_impulse']:.7g}, "
+                                                 f"got {actual['applied_impulse']:.7g}"))
         if cp["rigid_contact_count"] != len(got["contacts"]):
-            failures.append(f"frame {frame} rigid_contact_count: expected {cp['rigid_contact_count']}, "
-                            f"got {len(got['contacts'])}")
+            failures.append((frame, "contacts", f"rigid_contact_count: expected {cp['rigid_contact_count']}, "
+                                                f"got {len(got['contacts'])}"))
         else:
             for i in _match_contacts(case, cp["rigid_contacts"], got["contacts"]):
-                failures.append(f"frame {frame} contact {cp['rigid_contacts'][i]} has no match")
+                failures.append((frame, "contacts", f"{cp['rigid_contacts'][i]} has no match"))
     return failures
+
+
+def compare(name, result, frames=None):
+    """Failures (strings) of `result` against the capture at the listed
+    checkpoint frames (all by default)."""
+    return [f"frame {frame} {rid} {msg}" for frame, rid, msg in compare_records(name, result, frames)]
+
+
+def record_ids(name):
+    """The capture's record ids (bodies and constraints), in order."""
+    return [r["id"] for r in load(name)["checkpoints"][0]["resources"]]
+
+
+# The integrated (GLB) cases: the records each case's capture pins on this
+# step ("contacts" stands for the checkpoint's rigid contacts), each with
+# the checkpoints at which it is compared (None: all).  Every other record
+# departs from the capture through contacts whose outcome turns on near ties
+# (which face of two touching meshes is the reference face, which facet of a
+# faceted sphere comes to rest on a block or the ground) or through chaotic
+# motion (gear teeth, a thrown monkey head); those are not compared.
+GLB_FOLLOWS = {
+    "constraint-piston": {"rigid/Cylinder/0": None, "rigid/Circle/1": None, "rigid/Circle.001/2": None,
+                          "constraint/Constraint/0": None, "contacts": None},
+    "constraint-slider": {"rigid/Ground/0": None, "rigid/SliderRail/1": None, "rigid/SliderBlock/2": None,
+                          "constraint/SliderJoint/0": None, "rigid/SliderSphere/3": {30}, "contacts": None},
+    "constraint-generic": {"rigid/Ground/0": None, "rigid/GenericBlockA/1": {30}},
+    "constraint-generic-spring": {"rigid/Ground/0": None, "rigid/GenericBlockA/1": None,
+                                  "constraint/GenericJoint/0": None},
+    "constraint-motor": {"rigid/Ground/0": None, "rigid/MotorChassis/1": None, "rigid/MotorWheelLeftFront/2": None,
+                         "rigid/MotorWheelLeftRear/3": None, "rigid/MotorWheelRightFront/4": None,
+                         "rigid/MotorWheelRightRear/5": None, "constraint/MotorJointRightFront/2": None,
+                         "constraint/MotorJointRightRear/3": None},
+    "constraint-hinge": {"rigid/Ground.001/3": None, "rigid/HingeFrame/5": None, "constraint/Constraint1/0": None,
+                         "constraint/Constraint2/1": None, "constraint/Constraint3/2": None,
+                         "constraint/SliderReturnSpring/4": None},
+    "constraint-fixed": {"rigid/Ground/0": None, "constraint/FixedJoint/0": None},
+    "constraint-point": {"rigid/Ground/0": None, "rigid/PointPost/1": None},
+    "passive-active": {"rigid/Plane/0": None, "rigid/Cube/1": None, "rigid/Suzanne.001/4": {24}},
+}
+
+
+def glb_failures(name, result):
+    """Failures of the records GLB_FOLLOWS pins at their checkpoints, and of
+    every record at frame 0 (the loaded scene), as strings."""
+    follows = GLB_FOLLOWS[name]
+    return [f"frame {frame} {rid} {msg}" for frame, rid, msg in compare_records(name, result)
+            if frame == 0 or (rid in follows and (follows[rid] is None or frame in follows[rid]))]
 
 
 # rigid-direct's frame 40 follows the kinematic box's 120 m/s sweep through

@@ -13,7 +13,9 @@ import json
 import pathlib
 import sys
 
-CASES = ("rigid-direct", "constraint-breaking", "compound-weld-lifecycle")
+CASES = ("rigid-direct", "constraint-breaking", "compound-weld-lifecycle",
+         "constraint-point", "constraint-fixed", "constraint-hinge", "constraint-slider", "constraint-piston",
+         "constraint-generic", "constraint-generic-spring", "constraint-motor", "passive-active")
 KEEP_CASE = ("id", "checkpoints", "commands", "resources", "tolerances", "world")
 KEEP_RESOURCE = ("id", "type", "tolerance_class", "present", "position", "orientation",
                  "linear_velocity", "angular_velocity", "applied_impulse", "broken", "enabled")

@@ -6,10 +6,12 @@ per-body state replacement (set_rigid_body_state), joint removal and
 re-adding (pmCmdWriteRigidConstraints) and contact events
 (pmCmdCopyContactEvents), on scenes of 16 or more bodies.
 
-The conformance cases themselves (two and three bodies) run against the
+The analytic conformance cases (two and three bodies) run against the
 reference's CUDA captures on the oracle only (test_oracle_conformance.py):
 the first GPU steps of those worlds, which start with no active pair, did
-not return, and no cause has been shown.
+not return, and no cause has been shown.  The integrated (GLB) cases of
+four or more bodies run here: against the captures for the records the
+captures pin on this step, and against the oracle for every record.
 """
 import sys
 
@@ -18,12 +20,15 @@ import pytest
 
 import _oracle as O
 import conformance as C
+import glb_scenes as G
 import pmamd
 import scenarios as S
 from pmamd import layouts as L
 
 pytestmark = pytest.mark.gpu
 LANES = 32
+GLB_RTOL = 1e-4
+GLB_ATOL = {"position": 1e-4, "orientation": 1e-4, "linear_velocity": 1e-3, "angular_velocity": 1e-3}
 
 
 def say(msg):
@@ -180,3 +185,34 @@ def test_new_entry_points_reject_bad_input():
     with pytest.raises(pmamd.PmhError):
         pmamd.World(substeps=65)
     g.close()
+
+
+GLB_GPU = [name for name in G.CASES if len(G.Scene(name).bodies) >= 4]
+
+
+@pytest.mark.parametrize("name", GLB_GPU)
+def test_glb_cases_follow_the_capture_and_the_oracle(name):
+    """Each integrated case on the GPU: the records the capture pins
+    (C.GLB_FOLLOWS, every record at frame 0) against the capture, and every
+    body and joint record at every checkpoint against the oracle run in the
+    GPU's summation order."""
+    say(f"{name}: GPU run")
+    kw = dict(dt=1.0 / 60.0, profile=0, base_iterations=C.ITERATIONS, substeps=C.SUBSTEPS)
+    g = pmamd.World(solve_lanes=LANES, **kw)
+    got = C.run(name, g)
+    g.close()
+    assert C.glb_failures(name, got) == []
+    say(f"{name}: oracle run")
+    want = C.run(name, O.OracleWorld(lanes=LANES, **kw))
+    for frame in sorted(want):
+        for rid, o in want[frame]["rigid"].items():
+            a = got[frame]["rigid"][rid]
+            for leaf in ("position", "orientation", "linear_velocity", "angular_velocity"):
+                np.testing.assert_allclose(a[leaf], o[leaf], rtol=GLB_RTOL, atol=GLB_ATOL[leaf],
+                                           err_msg=f"frame {frame} {rid}.{leaf}")
+        for rid, o in want[frame]["constraints"].items():
+            a = got[frame]["constraints"][rid]
+            assert (a["present"], a["enabled"], a["broken"]) == (o["present"], o["enabled"], o["broken"]), rid
+            np.testing.assert_allclose(a["applied_impulse"], o["applied_impulse"], rtol=GLB_RTOL, atol=1e-4,
+                                       err_msg=f"frame {frame} {rid}")
+        assert len(got[frame]["contacts"]) == len(want[frame]["contacts"]), f"frame {frame} contacts"

@@ -2,7 +2,9 @@
 """The oracle against the reference's CUDA conformance captures of its three
 analytic rigid cases (conformance/v1/golden/cuda/{rigid-direct,
 constraint-breaking,compound-weld-lifecycle}.json, kept as fixtures under
-tests/golden/conformance).
+tests/golden/conformance) and of its nine integrated rigid cases, whose
+scenes are GLB files (examples/assets/Constraint*.glb, PassiveActive.glb,
+kept as fixtures under tests/golden/glb).
 
 The captures pin, within the cases' own tolerance classes: free flight under
 4 substeps (direct class, 1e-5), an applied impulse, a kinematic target
@@ -17,6 +19,7 @@ import pytest
 
 import _oracle as O
 import conformance as C
+import glb_scenes as G
 
 
 def oracle_world(**kw):
@@ -33,7 +36,23 @@ def test_oracle_matches_the_cuda_capture(name):
         assert C.impact_band(result) == []
 
 
-@pytest.mark.parametrize("name, change", [("rigid-direct", dict(substeps=1)),
+@pytest.mark.parametrize("name", G.CASES)
+def test_oracle_follows_the_cuda_capture_of_the_glb_cases(name):
+    """The integrated cases, loaded from their GLB scenes as the reference's
+    loader does: every record at frame 0 (centred shapes, joint frames), and
+    the records of C.GLB_FOLLOWS at the later checkpoints: the piston case
+    whole; the slider case whole up to frame 30 (its sphere thrown at the
+    rail's end, the rebound and the contacts) and its block and joint to
+    frame 60; the spring-driven generic joint's block and impulse; the motor
+    car's chassis and four wheels; the gear train's three hinges and its
+    return spring; the thrown case's second monkey head to frame 24."""
+    result = C.run(name, oracle_world())
+    assert C.glb_failures(name, result) == []
+
+
+@pytest.mark.parametrize("name, change", [("constraint-piston", dict(substeps=1)),
+                                          ("constraint-slider", dict(substeps=2)),
+                                          ("rigid-direct", dict(substeps=1)),
                                           ("constraint-breaking", dict(profile=1, substeps=1)),
                                           ("compound-weld-lifecycle", dict(substeps=2))])
 def test_a_different_step_contract_fails_the_capture(name, change):
@@ -41,7 +60,10 @@ def test_a_different_step_contract_fails_the_capture(name, change):
     is one substep), each move the oracle off the capture: the comparison
     discriminates."""
     result = C.run(name, oracle_world(**change))
-    assert C.compare(name, result, C.EXACT_FRAMES.get(name)) != []
+    if name in G.CASES:
+        assert C.glb_failures(name, result) != []
+    else:
+        assert C.compare(name, result, C.EXACT_FRAMES.get(name)) != []
 
 
 @pytest.mark.parametrize("dt, substeps, ok", [(1.0 / 60.0, 1, True), (np.nextafter(np.float32(1.0 / 60.0), 1.0), 1, True),
