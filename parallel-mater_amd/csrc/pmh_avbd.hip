@@ -393,7 +393,7 @@ __global__ void k_adj_sort(uint32_t n, const uint32_t *adj_start, uint32_t *adj)
     }
 }
 
-__device__ uint32_t uf_find(uint32_t *parent, uint32_t v) {
+__device__ uint32_t uf_find(const uint32_t *parent, uint32_t v) {
     uint32_t p = ld_relaxed(&parent[v]);
     while (p != v) { v = p; p = ld_relaxed(&parent[v]); }
     return v;
@@ -2483,7 +2483,7 @@ PmhResult Solver::each_buffer(uint32_t bodies, uint32_t pairs, uint32_t points, 
     R(prev_vel, bodies); R(done, bodies);
     R(code, pairs); R(ab, pairs); R(m_count, pairs + 1); R(m_start, pairs + 1); R(m_slot, pairs + 1);
     R(m_impact, pairs); R(m_face, pairs); R(m_init_rel, pairs); R(adj, 2 * (size_t)pairs);
-    R(tri_list, pairs);
+    R(tri_list, pairs); R(face_list, pairs);
     R(c_np, points); R(c_pt, points); R(c_aa, points); R(c_ab, points); R(c_err, points);
     R(c_lam, points); R(c_pen, points); R(c_acc, points); R(c_rep, points); R(c_flags, points);
     R(c_pair, points); R(jac, 9 * (size_t)points); R(bp_entries, 4 * (size_t)points);
@@ -2570,7 +2570,7 @@ void Solver::release() {
     bp_start.release(); anchored.release(); mean.release(); members.release(); mean_list.release(); done.release();
     jdeg.release(); jstart.release(); jcursor.release(); joint_head.release(); jadj.release(); code.release(); ab.release();
     m_count.release(); m_start.release(); m_slot.release(); m_impact.release(); m_face.release();
-    m_init_rel.release(); adj.release(); tri_list.release(); c_np.release(); c_pt.release(); c_aa.release();
+    m_init_rel.release(); adj.release(); tri_list.release(); face_list.release(); c_np.release(); c_pt.release(); c_aa.release();
     c_ab.release(); c_err.release(); c_lam.release(); c_pen.release(); c_acc.release();
     c_rep.release(); c_flags.release(); c_pair.release(); jac.release(); bp_entries.release();
     erec.release(); c_entry.release(); residual.release();

@@ -1648,15 +1648,53 @@ __device__ __forceinline__ bool persistent_pair(const ShapeDev &a, const ShapeDe
 // One thread per pair: the convex face path (and its swept variant) for
 // convex meshes, closed forms for round shapes.  Mesh pairs the face path
 // declines are queued for the triangle-pair kernel.
+// The narrowphase runs in two kernels so that no wavefront mixes the cheap
+// analytic pairs with the convex face path's separating-axis search:
+// k_narrowphase solves every pair with an analytic shape and queues the
+// pairs of two meshes; k_face_pairs runs the face path over that queue and
+// passes what it declines on to the triangle-pair path.  Each pair's output
+// sits at its own index, so the queues' order changes nothing.
 __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, const PmhBodyParameters *P,
-                                                     const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
-                                                     const float4 *planes, const PmhRigidBodyState *prev,
+                                                     const ShapeDev *shapes, const PmhRigidBodyState *prev,
                                                      const PmhRigidBodyState *pred, const uint2 *ab, NarrowOut o,
-                                                     uint32_t *tri_list, const HingeFrame *frames,
+                                                     uint32_t *face_list, const HingeFrame *frames,
                                                      const float4 *frame_ref) {
     const uint32_t M = *npairs;
     uint32_t unsupported = 0;
     for (uint32_t m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const uint2 p = ab[m];
+        const PmhBodyParameters pa = P[p.x], pb = P[p.y];
+        const ShapeDev sa = shapes[pa.mesh_index], sb = shapes[pb.mesh_index];
+        if (sa.kind == kShapeMesh && sb.kind == kShapeMesh) {
+            face_list[wave_reserve(&o.ctr->face_pairs, 1u)] = m;
+            continue;
+        }
+        const PmhRigidBodyState A0 = prev[p.x], A = pred[p.x], B = pred[p.y];
+        const float margin = pa.collision_margin + pb.collision_margin;
+        const float cr = correction_reach(pa, sa) + correction_reach(pb, sb);
+        const PairFrames fr = pair_frames(frames, frame_ref, pa, pb, p, A0, A);
+        Manifold man;
+        man.count = 0;
+        man.face_patch = 0;
+        if (!analytic_manifold(A, sa, B, sb, margin, margin + cr, man)) {
+            man.count = 0;
+            man.face_patch = 0;
+            ++unsupported;
+        }
+        emit_manifold(o, m, man, A, B, persistent_pair(sa, sb, pb), fr.ha, fr.hb);
+    }
+    if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
+}
+
+__global__ void __launch_bounds__(128) k_face_pairs(const uint32_t *face_list, const PmhBodyParameters *P,
+                                                    const ShapeDev *shapes, const V3 *verts, const uint32_t *idx,
+                                                    const float4 *planes, const PmhRigidBodyState *prev,
+                                                    const PmhRigidBodyState *pred, const uint2 *ab, NarrowOut o,
+                                                    uint32_t *tri_list, const HingeFrame *frames,
+                                                    const float4 *frame_ref) {
+    const uint32_t F = o.ctr->face_pairs;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < F; f += gridDim.x * blockDim.x) {
+        const uint32_t m = face_list[f];
         const uint2 p = ab[m];
         const PmhBodyParameters pa = P[p.x], pb = P[p.y];
         ShapeView sa{shapes[pa.mesh_index], verts, idx, planes};
@@ -1668,31 +1706,22 @@ __global__ void __launch_bounds__(128) k_narrowphase(const uint32_t *npairs, con
         Manifold man;
         man.count = 0;
         man.face_patch = 0;
-        if (sa.s.kind != kShapeMesh || sb.s.kind != kShapeMesh) {
-            if (!analytic_manifold(A, sa.s, B, sb.s, margin, margin + cr, man)) {
-                man.count = 0;
-                man.face_patch = 0;
-                ++unsupported;
-            }
-        } else {
-            bool face = false;
-            // A guided pair leaves the face path (geometry.slang:2779).
-            if (sa.s.convex_face && sb.s.convex_face && !guided_pair(fr.ha, fr.hb)) {
-                const bool swept = requires_swept(A0, A, sa.s, B0, B, sb.s, margin);
-                if (swept) face = swept_convex_face_manifold(A0, A, sa, B0, B, sb, margin, man);
-                if (!face) {
-                    face = convex_face_manifold(A, sa, B, sb, margin, cr, man);
-                    if (swept && man.count == 0u) face = false;
-                }
-            }
+        bool face = false;
+        // A guided pair leaves the face path (geometry.slang:2779).
+        if (sa.s.convex_face && sb.s.convex_face && !guided_pair(fr.ha, fr.hb)) {
+            const bool swept = requires_swept(A0, A, sa.s, B0, B, sb.s, margin);
+            if (swept) face = swept_convex_face_manifold(A0, A, sa, B0, B, sb, margin, man);
             if (!face) {
-                tri_list[wave_reserve(&o.ctr->tri_pairs, 1u)] = m;
-                continue;
+                face = convex_face_manifold(A, sa, B, sb, margin, cr, man);
+                if (swept && man.count == 0u) face = false;
             }
+        }
+        if (!face) {
+            tri_list[wave_reserve(&o.ctr->tri_pairs, 1u)] = m;
+            continue;
         }
         emit_manifold(o, m, man, A, B, persistent_pair(sa.s, sb.s, pb), fr.ha, fr.hb);
     }
-    if (unsupported) atomicAdd(&o.ctr->unsupported, unsupported);
 }
 
 // Scenes of a batch: each body's scene, each scene's body count and the
@@ -2157,6 +2186,7 @@ __global__ void k_step_begin(StepCounters *ctr) {
     ctr->color_ticket = 0;
     ctr->point_alloc = 0;
     ctr->tri_pairs = 0;
+    ctr->face_pairs = 0;
     ctr->fallback_pairs = 0;
     ctr->cell_size_bits = 0.0f;
 }
@@ -2260,8 +2290,10 @@ PmhResult World::step_once(uint32_t k_step, bool warm_infer) {
         jv.list = v.jlist;
         k_hinge_frames<<<bg, 256, 0, s>>>(n, params.p, prev, pred, jv, frames, frame_ref);
     }
-    k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
-        &ctr->pairs, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p, out,
+    k_narrowphase<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(&ctr->pairs, params.p, shapes_dev.p, prev, pred,
+                                                                    v.ab.p, out, v.face_list.p, frames, frame_ref);
+    k_face_pairs<<<blocks_for(v.pair_cap, 128, 4096), 128, 0, s>>>(
+        v.face_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, prev, pred, v.ab.p, out,
         v.tri_list.p, frames, frame_ref);
     k_triangle_pairs<<<2048, 256, 0, s>>>(
         v.tri_list.p, params.p, shapes_dev.p, verts_dev.p, idx_dev.p, planes_dev.p, nodes_dev.p, leaves_dev.p,

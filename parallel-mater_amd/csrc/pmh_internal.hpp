@@ -107,6 +107,7 @@ struct StepCounters {
     uint32_t color_abort;      // set when a bounded wait (colouring or sweep) ran out: every other wait gives up
     uint32_t sweep_ticket[2];  // work tickets of the dataflow sweep, alternating per sweep
     uint32_t sweep_failed;     // a dataflow-sweep wait ran out; stays set until the host reports it
+    uint32_t face_pairs;       // mesh pairs queued for the convex face path
 };
 
 // Running totals over every step since the last reset.
@@ -265,6 +266,7 @@ struct Solver {
     DBuf<float4> m_init_rel;
     DBuf<uint32_t> adj;                  // 2 per pair
     DBuf<uint32_t> tri_list;             // pairs for the triangle-pair path
+    DBuf<uint32_t> face_list;            // mesh pairs for the convex face path
     // points
     DBuf<float4> c_np, c_pt, c_aa, c_ab, c_err, c_lam, c_pen, c_acc, c_rep;
     DBuf<uint32_t> c_flags, c_pair;

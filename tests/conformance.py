@@ -300,7 +300,7 @@ GLB_FOLLOWS = {
                          "rigid/MotorWheelRightRear/5": None, "constraint/MotorJointRightFront/2": None,
                          "constraint/MotorJointRightRear/3": None},
     "constraint-hinge": {"rigid/Ground.001/3": None, "rigid/HingeFrame/5": None, "constraint/Constraint1/0": None,
-                         "constraint/Constraint2/1": None, "constraint/Constraint3/2": None,
+                         "constraint/Constraint2/1": {45}, "constraint/Constraint3/2": None,
                          "constraint/SliderReturnSpring/4": None},
     "constraint-fixed": {"rigid/Ground/0": None, "constraint/FixedJoint/0": None},
     "constraint-point": {"rigid/Ground/0": None, "rigid/PointPost/1": None},
