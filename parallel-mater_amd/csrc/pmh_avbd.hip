@@ -2423,6 +2423,27 @@ PmhResult sync_stream(hipStream_t s) {
     }
 }
 
+PmhResult sync_event(hipEvent_t ev) {
+    static const double limit = [] {
+        const char *e = getenv("PMH_SYNC_TIMEOUT_S");
+        const double v = e ? atof(e) : 0.0;
+        return v > 0.0 ? v : 120.0;
+    }();
+    const auto start = std::chrono::steady_clock::now();
+    for (uint32_t polls = 0;; ++polls) {
+        const hipError_t e = hipEventQuery(ev);
+        if (e == hipSuccess) return PMH_SUCCESS;
+        if (e != hipErrorNotReady) return hip_result(e, "hipEventQuery");
+        if (polls >= 64) {
+            if (std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count() > limit) {
+                (void)hip_result(hipErrorNotReady, "device did not finish the step within PMH_SYNC_TIMEOUT_S");
+                return PMH_ERROR_DEVICE_LOST;
+            }
+            std::this_thread::yield();
+        }
+    }
+}
+
 static int g_cus = 0;
 int device_compute_units() {
     if (!g_cus) {
@@ -2579,6 +2600,12 @@ void Solver::release() {
     scan_tmp.release();
     if (host_owned && host_ctr) (void)hipHostFree(host_ctr);
     if (host_owned && host_color_start) (void)hipHostFree(host_color_start);
+    if (host_exec) (void)hipHostFree(host_exec);
+    for (hipEvent_t &e : exec_events) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    host_exec = nullptr;
     host_ctr = nullptr;
     host_color_start = nullptr;
     host_owned = true;
@@ -2771,6 +2798,15 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             PMH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&v.host_ctr), sizeof(StepCounters)));
             PMH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&v.host_color_start), sizeof(uint32_t) * (kMaxColors + 1)));
         }
+        // Iterating to a residual, the host stops enqueueing iterations
+        // once a phase has converged (every iteration after that exits at
+        // once on the device, so the results are those of enqueueing the
+        // whole budget).  A solver without allocation enqueues the budget.
+        const bool chunked = sp.tolerance > 0.0f && v.host_owned;
+        if (chunked && !v.host_exec) {
+            PMH_CHECK(hipHostMalloc(reinterpret_cast<void **>(&v.host_exec), 2 * sizeof(uint32_t)));
+            for (hipEvent_t &e : v.exec_events) PMH_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
         PMH_CHECK(hipMemcpyAsync(v.host_ctr, ctr, sizeof(StepCounters), hipMemcpyDeviceToHost, s));
         PMH_CHECK(hipMemcpyAsync(v.host_color_start, v.color_start.p, sizeof(uint32_t) * (kMaxColors + 1),
                                  hipMemcpyDeviceToHost, s));
@@ -2849,7 +2885,27 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             return hipLaunchKernel(kernel, dim3(blocks), block, args, 0, s);
         };
         uint32_t launches = 0;
+        // Every kCheckChunk iterations the phase's executed count is copied
+        // to pinned memory; the host then waits for the previous chunk's
+        // copy (the device works on this chunk meanwhile) and stops once
+        // the phase has converged: at most two chunks run empty.
+        constexpr uint32_t kCheckChunk = 4;
+        auto check_chunk = [&](uint32_t phase, uint32_t it, uint32_t &chunk, bool &done) -> PmhResult {
+            done = false;
+            if (!chunked || (it + 1) % kCheckChunk != 0 || it + 1 >= budget) return PMH_SUCCESS;
+            const uint32_t slot = chunk & 1u;
+            PMH_CHECK(hipMemcpyAsync(&v.host_exec[slot], &v.residual.p->executed[phase], sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s));
+            PMH_CHECK(hipEventRecord(v.exec_events[slot], s));
+            if (chunk++ > 0) {
+                PmhResult r = sync_event(v.exec_events[slot ^ 1u]);
+                if (r != PMH_SUCCESS) return r;
+                done = v.host_exec[slot ^ 1u] != 0u;
+            }
+            return PMH_SUCCESS;
+        };
         for (uint32_t phase = 0; phase < phases; ++phase) {
+            uint32_t chunk = 0;
             if (phase != 0) {
                 k_stage_impact_gaps<<<pts_blocks, 256, 0, s>>>(a);
                 k_stage_impact_bodies<<<bg, 256, 0, s>>>(a);
@@ -2885,6 +2941,10 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                         k_stage_check<<<1, 64, 0, s>>>(a, phase, it);
                         launches += 1;
                     }
+                    bool converged = false;
+                    PmhResult cr = check_chunk(phase, it, chunk, converged);
+                    if (cr != PMH_SUCCESS) return cr;
+                    if (converged) break;
                     continue;
                 }
                 for (uint32_t c = 0; c < colors; ++c) {
@@ -2911,6 +2971,10 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                     k_stage_check<<<1, 64, 0, s>>>(a, phase, it);
                     launches += 1;
                 }
+                bool converged = false;
+                PmhResult cr = check_chunk(phase, it, chunk, converged);
+                if (cr != PMH_SUCCESS) return cr;
+                if (converged) break;
             }
             if (sp.tolerance > 0.0f) {
                 k_stage_publish<<<pts_blocks, 256, 0, s>>>(a, phase);

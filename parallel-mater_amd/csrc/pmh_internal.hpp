@@ -132,6 +132,8 @@ PmhResult report_sweep_failure(hipStream_t s, StepCounters *ctr);
 // a device that stops answering gives PMH_ERROR_DEVICE_LOST, not a caller
 // blocked for ever.
 PmhResult sync_stream(hipStream_t s);
+// Waits for an event with the same deadline.
+PmhResult sync_event(hipEvent_t e);
 constexpr uint32_t kDefaultLanes = 32;   // lanes per body in the colour sweep
 // Joint types (RigidConstraintType, src/simulation/types.hpp:662).
 enum : uint32_t { kJointFixed = 0, kJointPoint = 1, kJointHinge = 2, kJointSlider = 3, kJointPiston = 4,
@@ -288,6 +290,11 @@ struct Solver {
     StepCounters *host_ctr = nullptr;        // pinned read-back of the schedule
     uint32_t *host_color_start = nullptr;
     bool host_owned = true;                  // the read-back records were allocated here
+    // Iterating to a residual: pinned copies of a phase's executed-iteration
+    // count, taken after every few iterations, and their events (allocating
+    // solvers only).
+    uint32_t *host_exec = nullptr;
+    hipEvent_t exec_events[2] = {nullptr, nullptr};
     PmhResult reserve(uint32_t bodies, uint32_t pairs, uint32_t points);
     PmhResult reserve_joints(uint32_t count);
     // Every buffer as a slice of `bytes` bytes at `base` (device memory the
