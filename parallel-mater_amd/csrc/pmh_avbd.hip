@@ -1015,6 +1015,40 @@ __device__ __forceinline__ bool pose_update(float4 &t, float4 &r, float4 &dqv, c
     return true;
 }
 
+// A lane's share of one update's residual: maximum, fixed-point sum of
+// squares and count of its points or joints.
+struct ResidualAcc {
+    float max = 0.0f;
+    unsigned long long sum = 0;
+    uint32_t count = 0;
+    __device__ void add(float r) { max = smax(max, r); sum += residual_square(r); ++count; }
+};
+
+// Combines the lanes of a converged wavefront into the log's slot with one
+// atomic per field (non-negative floats order like their bit patterns).
+__device__ __forceinline__ void residual_commit(ResidualLog *log, uint32_t slot, const ResidualAcc &acc) {
+    uint32_t bits = __float_as_uint(acc.max), count = acc.count;
+    unsigned long long sum = acc.sum;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        bits = max(bits, uint32_t(__shfl_xor(bits, d, 64)));
+        count += uint32_t(__shfl_xor(count, d, 64));
+        sum += (unsigned long long)__shfl_xor((long long)sum, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && count) {
+        const uint32_t bank = blockIdx.x & (kResidualBanks - 1);
+        atomicMax(&log->bits[slot][bank], bits);
+        atomicAdd(&log->sum[slot][bank], sum);
+        atomicAdd(&log->count[slot][bank], count);
+    }
+}
+
+// One contact point's dual update (advance_duals, solve.slang:492-566)
+// from both ends' displacements and the point's rows; defined with
+// advance_duals below.
+__device__ void dual_point(const SolveArgs &a, uint32_t c, uint32_t phase, const float *dA, const float *dB,
+                           const float4 *Jx, bool last, ResidualAcc &residual);
+
 // The bodies at colour-ordered positions k = base + group of one wavefront
 // (solve_color, geometry_constraints_solve.slang:331-491).  Each body is
 // owned by L lanes: every lane streams every L-th of the body's contact
@@ -1026,7 +1060,8 @@ __device__ __forceinline__ bool pose_update(float4 &t, float4 &r, float4 &dqv, c
 // waits for the lower-colour neighbours it reads, reads other bodies'
 // displacements agent-coherently, and publishes its own with a stamp.
 template <int L, bool kJoints, bool kFlow>
-__device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, uint32_t end, uint32_t phase) {
+__device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, uint32_t end, uint32_t phase,
+                                            uint32_t it = 0u) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t grp = lane / L, sub_lane = lane % L;
     const uint32_t k = base + grp;
@@ -1180,6 +1215,11 @@ __device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, u
                 for (int q = 0; q < 6; ++q) update[q] = safe[q];
         }
     }
+    // The body's displacement after this update (the group's first lane
+    // writes it).
+    float nd[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nd[q] = dI[q];
     if (active && sub_lane == 0) {
         if (solved) {
             float4 t = a.disp[2 * i], r = a.disp[2 * i + 1];
@@ -1210,8 +1250,38 @@ __device__ __forceinline__ void solve_group(const SolveArgs &a, uint32_t base, u
                     a.disp[2 * i + 1] = r;
                 }
             }
+            nd[0] = t.x; nd[1] = t.y; nd[2] = t.z; nd[3] = r.x; nd[4] = r.y; nd[5] = r.z;
         }
         if (kFlow) publish_done(a, i);
+    }
+    if (kFlow) {
+        // This update's duals (advance_duals) of the points whose other end
+        // is solved before this body in the sweep (a lower colour) or not in
+        // it (not dynamic, or not an impact body in the velocity phase):
+        // both ends' displacements are final here, and the points' duals are
+        // read again only by the next update's launch.  Every point has one
+        // such end, so each is updated once, as the separate dual pass did.
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nd[q] = __shfl(nd[q], int(grp * L), 64);
+        const bool last = a.tol <= 0.0f && it + 1u == a.budget;
+        ResidualAcc residual;
+        if (active) {
+            for (uint32_t e = eb + sub_lane; e < ee; e += L) {
+                const uint2 ent = a.entries[e];
+                const uint32_t other = ent.y & kEntryBody;
+                const bool other_solved = a.P[other].motion == kDynamic && (phase == 0 || a.impact[other] != 0u);
+                if (other_solved && (ent.y & kEntryLower) == 0u) continue;
+                const uint32_t c = ent.x & 0x7fffffffu;
+                const bool first = (ent.x & kSideFirst) != 0u;
+                float dO[6];
+                load_disp_co(a.disp, other, dO);
+                float4 Jx[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Jx[q] = a.erec[q * a.ecap + e];
+                dual_point(a, c, phase, first ? nd : dO, first ? dO : nd, Jx, last, residual);
+            }
+        }
+        if (it < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + it, residual);
     }
 }
 
@@ -1632,32 +1702,30 @@ __device__ __forceinline__ float contact_residual(const float *err, float fn, bo
     return r;
 }
 
-// A lane's share of one update's residual: maximum, fixed-point sum of
-// squares and count of its points or joints.
-struct ResidualAcc {
-    float max = 0.0f;
-    unsigned long long sum = 0;
-    uint32_t count = 0;
-    __device__ void add(float r) { max = smax(max, r); sum += residual_square(r); ++count; }
-};
-
-// Combines the lanes of a converged wavefront into the log's slot with one
-// atomic per field (non-negative floats order like their bit patterns).
-__device__ __forceinline__ void residual_commit(ResidualLog *log, uint32_t slot, const ResidualAcc &acc) {
-    uint32_t bits = __float_as_uint(acc.max), count = acc.count;
-    unsigned long long sum = acc.sum;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        bits = max(bits, uint32_t(__shfl_xor(bits, d, 64)));
-        count += uint32_t(__shfl_xor(count, d, 64));
-        sum += (unsigned long long)__shfl_xor((long long)sum, d, 64);
+__device__ void dual_point(const SolveArgs &a, uint32_t c, uint32_t phase, const float *dA, const float *dB,
+                           const float4 *Jx, bool last, ResidualAcc &residual) {
+    float4 lam = a.c_lam[c], pen = a.c_pen[c];
+    const float err[3] = {row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB),
+                          row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB),
+                          row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB)};
+    const float mu = Jx[6].w, beta = Jx[7].w;
+    const Force3 trial = {lam.x + pen.x * err[0], lam.y + pen.y * err[1], lam.z + pen.z * err[2]};
+    const Force3 f = project_contact(trial, mu);
+    const float tx = lam.y + pen.y * err[1];
+    const float ty = lam.z + pen.z * err[2];
+    const float fb = mu * f.n;
+    const bool stick = tx * tx + ty * ty <= fb * fb;
+    lam = make_float4(f.n, f.t0, f.t1, 0.0f);
+    if (f.n > 0.0f) pen.x = smin(kMaxPenalty, pen.x + beta * fabsf(err[0]));
+    if (stick) {
+        pen.y = smin(kMaxPenalty, pen.y + beta * fabsf(err[1]));
+        pen.z = smin(kMaxPenalty, pen.z + beta * fabsf(err[2]));
     }
-    if ((threadIdx.x & 63) == 0 && count) {
-        const uint32_t bank = blockIdx.x & (kResidualBanks - 1);
-        atomicMax(&log->bits[slot][bank], bits);
-        atomicAdd(&log->sum[slot][bank], sum);
-        atomicAdd(&log->count[slot][bank], count);
-    }
+    a.c_lam[c] = lam;
+    a.c_pen[c] = pen;
+    residual.add(contact_residual(err, f.n, stick));
+    if (last) publish_point(a, c, phase, a.ab[a.c_pair[c]], f.n, f.t0, f.t1, stick, v3(Jx[1]), v3(Jx[2]));
+    else if (a.tol > 0.0f) a.c_flags[c] = (a.c_flags[c] & ~kFlagSlipping) | (stick ? 0u : kFlagSlipping);
 }
 
 // advance_duals for contacts (solve.slang:492-566): one thread per point.
@@ -1675,28 +1743,7 @@ __device__ void advance_duals(const SolveArgs &a, uint32_t phase, uint32_t itera
         float4 Jx[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) Jx[q] = J[q];
-        float4 lam = a.c_lam[c], pen = a.c_pen[c];
-        const float err[3] = {row_error(Jx[0].w, Jx[0], Jx[3], Jx[6], dA, dB),
-                              row_error(Jx[1].w, Jx[1], Jx[4], Jx[7], dA, dB),
-                              row_error(Jx[2].w, Jx[2], Jx[5], Jx[8], dA, dB)};
-        const float mu = Jx[6].w, beta = Jx[7].w;
-        const Force3 trial = {lam.x + pen.x * err[0], lam.y + pen.y * err[1], lam.z + pen.z * err[2]};
-        const Force3 f = project_contact(trial, mu);
-        const float tx = lam.y + pen.y * err[1];
-        const float ty = lam.z + pen.z * err[2];
-        const float fb = mu * f.n;
-        const bool stick = tx * tx + ty * ty <= fb * fb;
-        lam = make_float4(f.n, f.t0, f.t1, 0.0f);
-        if (f.n > 0.0f) pen.x = smin(kMaxPenalty, pen.x + beta * fabsf(err[0]));
-        if (stick) {
-            pen.y = smin(kMaxPenalty, pen.y + beta * fabsf(err[1]));
-            pen.z = smin(kMaxPenalty, pen.z + beta * fabsf(err[2]));
-        }
-        a.c_lam[c] = lam;
-        a.c_pen[c] = pen;
-        residual.add(contact_residual(err, f.n, stick));
-        if (last) publish_point(a, c, phase, p, f.n, f.t0, f.t1, stick, v3(Jx[1]), v3(Jx[2]));
-        else if (a.tol > 0.0f) a.c_flags[c] = (a.c_flags[c] & ~kFlagSlipping) | (stick ? 0u : kFlagSlipping);
+        dual_point(a, c, phase, dA, dB, Jx, last, residual);
     }
     if (iteration < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + iteration, residual);
 }
@@ -1971,7 +2018,7 @@ constexpr uint32_t kSweepWaves = 8;          // wavefronts per block without joi
 constexpr uint32_t kSweepJointWaves = 4;     // with joints (more registers per lane)
 template <int L, bool kJoints>
 __global__ void __launch_bounds__(64 * (kJoints ? kSweepJointWaves : kSweepWaves))
-k_sweep(SolveArgs a, uint32_t phase) {
+k_sweep(SolveArgs a, uint32_t phase, uint32_t it) {
     __shared__ uint32_t s_ticket[2];
     uint32_t *tickets = a.ctr->sweep_ticket;
     // The other counter is the next sweep's: it starts at zero.
@@ -1990,7 +2037,7 @@ k_sweep(SolveArgs a, uint32_t phase) {
         __syncthreads();
         const uint32_t first = s_ticket[round & 1u] * waves;
         if (first >= wave_slots) break;      // the same for the whole block
-        if (first + wave < wave_slots) solve_group<L, kJoints, true>(a, (first + wave) * groups, end, phase);
+        if (first + wave < wave_slots) solve_group<L, kJoints, true>(a, (first + wave) * groups, end, phase, it);
     }
 }
 template <bool kJoints>
@@ -2008,6 +2055,21 @@ __global__ void __launch_bounds__(256) k_stage_duals(SolveArgs a, uint32_t phase
         }
         if (it < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + it, r);
     }
+}
+
+// The joint half of k_stage_duals, after a dataflow sweep (which updates
+// the contact points' duals itself).
+__global__ void __launch_bounds__(256) k_stage_joint_duals(SolveArgs a, uint32_t phase, uint32_t it) {
+    if (a.tol > 0.0f && a.log->executed[phase] != 0u) return;   // converged earlier
+    ResidualAcc r;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < a.nj; k += gridDim.x * blockDim.x) {
+        const PmhRigidConstraint &j = a.joints[k];
+        if (j.avbd_count == 0u || j.state_enabled == 0u) continue;
+        if (phase != 0u && a.impact[j.dense_a] == 0u && a.impact[j.dense_b] == 0u) continue;
+        advance_joint(a, k, phase);
+        r.add(joint_residual(a, j));
+    }
+    if (it < kResidualSlots) residual_commit(a.log, phase * kResidualSlots + it, r);
 }
 
 // To-residual mode: the phase stops at the first update whose residual is
@@ -2834,6 +2896,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         if (timing && timing->start) PMH_CHECK(hipEventRecord(timing->start, s));
         const uint32_t colors = std::min(host.colors, kMaxColors);
         const uint32_t pts_blocks = std::max<uint32_t>(1, (std::max(host.points, v.nj) + 255) / 256);
+        const uint32_t joint_blocks = std::max<uint32_t>(1, (v.nj + 255) / 256);
         const uint32_t phases = realtime ? 1u : (host.impact ? 2u : 1u);
         const bool joints = v.nj != 0;
         // A timed launch goes through hipExtLaunchKernel, which writes the
@@ -2862,7 +2925,8 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         // wavefronts as fit (tickets hand out the positions).
         const bool flow = sp.dataflow;
         const uint32_t positions = v.host_color_start[colors];
-        auto launch_sweep = [&](SolveArgs &af, uint32_t phase, hipEvent_t start, hipEvent_t stop) -> hipError_t {
+        auto launch_sweep = [&](SolveArgs &af, uint32_t phase, uint32_t it, hipEvent_t start,
+                                hipEvent_t stop) -> hipError_t {
             const void *kernel;
             if (joints) {
                 kernel = lanes == 64   ? reinterpret_cast<const void *>(&k_sweep<64, true>)
@@ -2879,7 +2943,7 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
             const uint32_t slots = (positions + (64 / lanes) - 1) / (64 / lanes);
             const uint32_t blocks = std::max<uint32_t>(
                 1, std::min<uint32_t>((slots + block_waves - 1) / block_waves, sweep_blocks(kernel, 64 * block_waves)));
-            void *args[] = {&af, &phase};
+            void *args[] = {&af, &phase, &it};
             const dim3 block(64 * block_waves);
             if (start) return hipExtLaunchKernel(kernel, dim3(blocks), block, args, 0, s, start, stop, 0);
             return hipLaunchKernel(kernel, dim3(blocks), block, args, 0, s);
@@ -2933,10 +2997,12 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
                         e1 = kt->events[2 * timed + 1];
                         ++timed;
                     }
-                    PMH_CHECK(launch_sweep(af, phase, e0, e1));
-                    if (joints) k_stage_duals<true><<<pts_blocks, 256, 0, s>>>(a, phase, it);
-                    else k_stage_duals<false><<<pts_blocks, 256, 0, s>>>(a, phase, it);
-                    launches += 2;
+                    PMH_CHECK(launch_sweep(af, phase, it, e0, e1));
+                    launches += 1;
+                    if (joints) {
+                        k_stage_joint_duals<<<joint_blocks, 256, 0, s>>>(a, phase, it);
+                        launches += 1;
+                    }
                     if (sp.tolerance > 0.0f) {
                         k_stage_check<<<1, 64, 0, s>>>(a, phase, it);
                         launches += 1;
@@ -3005,8 +3071,12 @@ PmhResult avbd_prepare_solve(hipStream_t s, Solver &v, const PmhBodyParameters *
         if (time_colors) {
             // Algorithmic bytes of the colour sweeps: per iteration 136 B per
             // body, 84 B per contact point and dynamic endpoint, 64 B per
-            // manifold and dynamic endpoint (pair record + other body).
-            auto sweep = [](double d, double m1, double m2, double p1, double p2) {
+            // manifold and dynamic endpoint (pair record + other body); the
+            // dataflow sweep adds the dual update's (iteration_bytes).
+            auto sweep = [flow](double d, double m1, double m2, double p1, double p2) {
+                if (flow)   // the dataflow sweep also updates the duals: the whole iteration
+                    return 136.0 * d + (84.0 * 2 + 24.0) * p1 + (84.0 * 3 + 24.0) * p2 + (64.0 + 104.0) * m1 +
+                           (128.0 + 104.0) * m2;
                 return 136.0 * d + 84.0 * (p1 + 2.0 * p2) + 64.0 * (m1 + 2.0 * m2);
             };
             double bytes = budget * sweep(host.dyn_bodies, host.m1, host.m2, host.p1, host.p2);
