@@ -355,21 +355,25 @@ __global__ void __launch_bounds__(256) k_bp_query(uint32_t n, const PmhBodyParam
     uint32_t *out = code + base;
     uint32_t at = incl - count;
     walk([&](uint32_t b) { out[at++] = b; });
-    // The segment's writes of every lane are visible to lane 0 of the group.
+    // The segment's writes of every lane are visible to the whole group.
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
-    if (q != 0) return;
-    for (uint32_t k = 1; k < total; ++k) {
+    // Rank sort of the segment over the group's lanes: element k goes to
+    // the count of elements below it, ties in segment order (the order a
+    // stable insertion sort leaves).  The pairs are written first, then the
+    // codes from them, since the codes' slots hold the unsorted segment.
+    for (uint32_t k = q; k < total; k += kQueryLanes) {
         const uint32_t v = out[k];
-        uint32_t j = k;
-        while (j > 0 && out[j - 1] > v) { out[j] = out[j - 1]; --j; }
-        out[j] = v;
+        uint32_t rank = 0;
+        for (uint32_t i = 0; i < total; ++i) {
+            const uint32_t w = out[i];
+            rank += (w < v || (w == v && i < k)) ? 1u : 0u;
+        }
+        ab[base + rank] = make_uint2(a, v);
     }
-    for (uint32_t k = 0; k < total; ++k) {
-        const uint32_t b = out[k];
-        ab[base + k] = make_uint2(a, b);
-        out[k] = a * n + b;
-    }
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = q; k < total; k += kQueryLanes) out[k] = a * n + ab[base + k].y;
 }
 
 __global__ void k_set_pairs(StepCounters *ctr, const uint32_t *offset, uint32_t n, uint32_t pair_cap) {
@@ -2119,7 +2123,10 @@ __global__ void k_cache_load(const uint32_t *npairs, const uint32_t *code, const
             ++hits;
         }
     }
-    if (hits) atomicAdd(&ctr->cache_hits, hits);
+    // One atomic per wavefront (one per thread queued ~10^5 atomics on one
+    // address).
+    hits = wave_sum(hits);
+    if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&ctr->cache_hits, hits);
 }
 
 // pm_save_rigid_contact_cache (geometry.slang:3011-3070): the pair of
