@@ -747,7 +747,9 @@ __device__ bool convex_face_manifold(const PmhRigidBodyState &bs, const ShapeVie
     for (uint32_t t = 0; t < im.s.idx_count / 3; ++t) {
         const float4 face = im.plane(t);
         if (dot(v3(face), v3(incident)) < 0.99999f || fabsf(face.w - incident.w) > 1.0e-5f) continue;
-        V3 poly[36], clipped[36];
+        // The polygon is clipped back and forth between two buffers.
+        V3 buf[2][36];
+        V3 *poly = buf[0], *clipped = buf[1];
         uint32_t count = 3;
         for (uint32_t c = 0; c < 3; ++c)
             poly[c] = inv_rotate(rq, sub(transform_point(is, im.vert(im.index(t * 3 + c))), sv(rs.position)));
@@ -768,7 +770,9 @@ __device__ bool convex_face_manifold(const PmhRigidBodyState &bs, const ShapeVie
                 pd = d;
             }
             count = cc;
-            for (uint32_t k = 0; k < count; ++k) poly[k] = clipped[k];
+            V3 *const t_ = poly;
+            poly = clipped;
+            clipped = t_;
         }
         for (uint32_t k = 0; k < count; ++k) {
             const float d = dot(rn, poly[k]) - reference.w;
