@@ -461,6 +461,9 @@ __global__ void k_uf_hook(const uint32_t *npairs, const PmhBodyParameters *P, co
         if (m_count[m] == 0) continue;
         const uint2 p = ab[m];
         if (P[p.x].motion != kDynamic || P[p.y].motion != kDynamic) continue;
+        // After the sampled hooks and a compression most pairs already
+        // share a parent: nothing to join, and no walk to the root.
+        if (ld_relaxed(&parent[p.x]) == ld_relaxed(&parent[p.y])) continue;
         uf_union(parent, p.x, p.y);
     }
 }
@@ -698,7 +701,12 @@ __global__ void __launch_bounds__(64 * kColorWaveBlock) k_color_wave(uint32_t n,
                                                                    const PmhRigidConstraint *joints,
                                                                    StepCounters *ctr, uint32_t limit) {
     __shared__ uint32_t s_ticket[2];
+    __shared__ uint32_t s_top[kColorWaveBlock];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // The colour count (largest colour + 1) is gathered per wavefront and
+    // added with one atomic per block at the end: one atomic per body on
+    // that one address queued behind each other.
+    uint32_t top = 0;
     for (uint32_t round = 0;; ++round) {
         // Double-buffered: the slot written in round r + 1 was last read
         // before every wavefront passed round r's barrier.
@@ -744,8 +752,15 @@ __global__ void __launch_bounds__(64 * kColorWaveBlock) k_color_wave(uint32_t n,
         } else if (lane == 0) {
             if (chosen >= kMaxColors) { atomicOr(&ctr->status, kStatusColorOverflow); chosen = kMaxColors - 1; }
             st_relaxed(&color[i], chosen);
-            atomicMax(&ctr->colors, chosen + 1);
+            top = max(top, chosen + 1);
         }
+    }
+    if (lane == 0) s_top[wave] = top;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < kColorWaveBlock; ++w) m = max(m, s_top[w]);
+        if (m) atomicMax(&ctr->colors, m);
     }
 }
 
