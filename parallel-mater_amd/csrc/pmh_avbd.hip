@@ -2339,6 +2339,7 @@ __global__ void k_joint_hook(SolveArgs a, uint32_t *parent) {
     const PmhRigidConstraint &j = a.joints[k];
     if (j.avbd_count == 0u) return;
     if (a.P[j.dense_a].motion != kDynamic || a.P[j.dense_b].motion != kDynamic) return;
+    if (ld_relaxed(&parent[j.dense_a]) == ld_relaxed(&parent[j.dense_b])) return;   // already joined
     uf_union(parent, j.dense_a, j.dense_b);
 }
 
