@@ -1941,14 +1941,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         } else {
             leaf_mode = m < slots;
         }
-        uint32_t total = 0;
+        // Each lane counts the cells f = lane (mod 128) and f = lane + 64
+        // (mod 128) apart, so the same counts give the total and the list
+        // offsets.
+        uint32_t total = 0, c0 = 0, c1 = 0;
         if (leaf_mode) {
-            uint32_t mine = 0;
-            for (uint64_t f = lane; f < cells; f += 64u) {
-                uint2 nd;
-                mine += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
-            }
-            total = wave_sum(mine);
+            uint2 nd;
+            for (uint64_t f = lane; f < cells; f += 128u) c0 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
+            for (uint64_t f = lane + 64u; f < cells; f += 128u) c1 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
+            total = wave_sum(c0 + c1);
             leaf_mode = total <= limit;
         }
         uint32_t nlist = 0;
@@ -1956,10 +1957,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if (total > kLeafListCap) {
                 nlist = kListOverflow;
             } else {
-                uint32_t c0 = 0, c1 = 0;
                 uint2 nd;
-                for (uint64_t f = lane; f < cells; f += 128u) c0 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
-                for (uint64_t f = lane + 64u; f < cells; f += 128u) c1 += leaf_cell_overlaps(tp, swept, fa, fb, pfa, pfb, f, nd) ? 1u : 0u;
                 uint32_t at0 = wave_exclusive_scan(c0, lane);
                 uint32_t at1 = wave_sum(c0) + wave_exclusive_scan(c1, lane);
                 for (uint64_t f = lane; f < cells; f += 128u)
