@@ -91,11 +91,15 @@ def summarize_pmc(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json
     first, last = line["untimed_steps"], line["untimed_steps"] + line["steps"]
 
     def window_mean(root):
+        # without k_bp_fill in the run there are no step marks: every
+        # dispatch of the kernel counts (warm-up steps included)
+        rows = dispatches(root)
+        marked = any("k_bp_fill" in name for name, _ in rows)
         step, values = -1, []
-        for name, value in dispatches(root):
+        for name, value in rows:
             if "k_bp_fill" in name:
                 step += 1
-            elif kernel.split("<")[0] in name and first <= step < last:
+            elif kernel in name and (not marked or first <= step < last):
                 values.append(value)
         return sum(values) / max(1, len(values)), len(values)
 
@@ -120,6 +124,7 @@ def summarize_pmc(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, bench_json
         "bytes_per_launch": (fetch * fetch_scale + write * write_scale) * 1024.0,
         "algorithmic_bytes_per_launch": line["roofline"]["algorithmic_bytes_per_launch"],
         "dispatches": count,
+        "steps_marked": any("k_bp_fill" in name for name, _ in dispatches(fetch_dir)),
     }
     with open(out_path, "w") as f:
         json.dump(rec, f, indent=1)
@@ -271,9 +276,11 @@ def main():
                     help="no GPU: write profiles/color_sweep_traffic.json from rocprofv3 --pmc output directories "
                          "(FETCH_SIZE and WRITE_SIZE of a bench run and of --copy-calibration 256) and that run's "
                          "bench line")
+    ap.add_argument("--pmc-out", default=os.path.join(ROOT, "profiles", "color_sweep_traffic.json"),
+                    help="where --summarize-pmc writes its record (the bench line's traffic reads the default)")
     args = ap.parse_args()
     if args.summarize_pmc:
-        rec = summarize_pmc(*args.summarize_pmc, os.path.join(ROOT, "profiles", "color_sweep_traffic.json"))
+        rec = summarize_pmc(*args.summarize_pmc, args.pmc_out)
         print(json.dumps(rec))
         return
     if args.copy_calibration:
